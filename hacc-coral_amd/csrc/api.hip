@@ -1,0 +1,604 @@
+// api.hip -- the C ABI of libhaccsr.so (include/hacc_sr.h).
+//
+// The entry points stage the caller's ten particle arrays (host or device
+// memory), run the device tree build, list build and leaf kernel on the
+// context's stream, and write the permuted arrays back in place exactly as
+// RCBForceTree's constructor does (RCBForceTree.cxx:336-450).
+#include "hsr_internal.h"
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+namespace {
+
+thread_local std::string g_err;
+
+// BGQStep16.c:40-45,167 and ForceLaw.cxx:109-114 (POLY_ORDER 5)
+const float kP5[6] = {0.269327f, -0.0750978f, 0.0114808f,
+                      -0.00109313f, 0.0000605491f, -0.00000147177f};
+// ForceLaw.cxx:98-104 (POLY_ORDER 6)
+const float kP6[7] = {0.271431f, -0.0783394f, 0.0133122f, -0.00159485f,
+                      0.000132336f, -0.00000663394f, 0.000000147305f};
+// FGrid::m_rmax, ForceLaw.cxx:32
+const float kRmax = 3.116326355f;
+
+// The device law for a ForceLaw choice.  ForceLawSR zeroes r2 > rmax^2
+// (ForceLaw.cxx:189) and nbody1 zeroes r2 >= fsrrmax^2
+// (RCBForceTree.cxx:613); both become one strict cutoff rc2.
+hsr_status make_law(const hsr_params &p, ForceLawDev &L) {
+  memset(&L, 0, sizeof(L));
+  float fsr2 = p.fsrrmax * p.fsrrmax;
+  L.eps2 = p.rsm * p.rsm;
+  switch (p.law) {
+  case HSR_LAW_STEP16:
+    for (int i = 0; i < 6; ++i) L.c[i] = kP5[i];
+    L.order = 5;
+    L.rc2 = fsr2;
+    L.guard = L.eps2 > 0.f ? 0 : 1;
+    break;
+  case HSR_LAW_SR_POLY5:
+  case HSR_LAW_SR_POLY6: {
+    if (p.law == HSR_LAW_SR_POLY5) {
+      for (int i = 0; i < 6; ++i) L.c[i] = kP5[i];
+      L.order = 5;
+    } else {
+      for (int i = 0; i < 7; ++i) L.c[i] = kP6[i];
+      L.order = 6;
+    }
+    float r2max = kRmax * kRmax;
+    // r2 <= r2max  <=>  r2 < nextafter(r2max, +inf)
+    float incl = std::nextafter(r2max, INFINITY);
+    L.rc2 = std::min(fsr2, incl);
+    // r2 == 0 pairs have dx = 0 and contribute nothing when eps2 > 0
+    L.guard = L.eps2 > 0.f ? 0 : 1;
+    break;
+  }
+  default:
+    g_err = "unsupported force law (device path implements STEP16, SR_POLY5, SR_POLY6)";
+    return HSR_ERR_UNSUPPORTED;
+  }
+  return HSR_OK;
+}
+
+__global__ void k_gather_f32(int64_t n, const int32_t *perm, const float *in,
+                             float *out) {
+  int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < n) out[k] = in[perm[k]];
+}
+__global__ void k_gather_i64(int64_t n, const int32_t *perm, const int64_t *in,
+                             int64_t *out) {
+  int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < n) out[k] = in[perm[k]];
+}
+__global__ void k_gather_u16(int64_t n, const int32_t *perm,
+                             const uint16_t *in, uint16_t *out) {
+  int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < n) out[k] = in[perm[k]];
+}
+
+inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+hsr_status check_ctx(hsr_ctx *c) {
+  if (!c) {
+    g_err = "null context";
+    return HSR_ERR_ARG;
+  }
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) {
+    g_err = std::string("hipSetDevice: ") + hipGetErrorString(e);
+    return HSR_ERR_HIP;
+  }
+  return HSR_OK;
+}
+
+// Permute the ten arrays (device memory) in place with the build's final
+// permutation.
+hsr_status permute_in_place(hsr_ctx *c, int64_t n, float *f[8], int64_t *id,
+                            uint16_t *mask) {
+  hipStream_t st = c->stream;
+  const int32_t *P = dptr<int32_t>(c->tree.pfin);
+  HSR_HIP(dbuf_reserve(c->s.x1, sizeof(int64_t) * std::max<int64_t>(n, 1)));
+  float *tmp = dptr<float>(c->s.x1);
+  for (int k = 0; k < 8; ++k) {
+    if (!f[k]) continue;
+    k_gather_f32<<<nblk(n, 256), 256, 0, st>>>(n, P, f[k], tmp);
+    HSR_HIP(hipMemcpyAsync(f[k], tmp, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+  }
+  if (id) {
+    int64_t *t64 = dptr<int64_t>(c->s.x1);
+    k_gather_i64<<<nblk(n, 256), 256, 0, st>>>(n, P, id, t64);
+    HSR_HIP(hipMemcpyAsync(id, t64, sizeof(int64_t) * n, hipMemcpyDeviceToDevice, st));
+  }
+  if (mask) {
+    uint16_t *t16 = dptr<uint16_t>(c->s.x1);
+    k_gather_u16<<<nblk(n, 256), 256, 0, st>>>(n, P, mask, t16);
+    HSR_HIP(hipMemcpyAsync(mask, t16, sizeof(uint16_t) * n, hipMemcpyDeviceToDevice, st));
+  }
+  HSR_HIP(hipGetLastError());
+  return HSR_OK;
+}
+
+hsr_status build_common(hsr_ctx *c, int64_t nboxes, const int64_t *box_off,
+                        float *x, float *y, float *z, float *vx, float *vy,
+                        float *vz, float *mass, float *phi, int64_t *id,
+                        uint16_t *mask, const float *min_loc,
+                        const float *max_loc, int ptr_flags) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (nboxes < 1 || !box_off || !x || !y || !z || !mass || !min_loc || !max_loc) {
+    g_err = "hsr_build: null argument";
+    return HSR_ERR_ARG;
+  }
+  int64_t n = box_off[nboxes];
+  for (int64_t b = 0; b < nboxes; ++b)
+    if (box_off[b + 1] < box_off[b]) {
+      g_err = "hsr_build_many: offsets must be non-decreasing";
+      return HSR_ERR_ARG;
+    }
+  if (box_off[0] != 0 || n < 0 || n >= (int64_t)INT32_MAX) {
+    g_err = "hsr_build: particle count must be in [0, 2^31)";
+    return HSR_ERR_ARG;
+  }
+  hipStream_t st = c->stream;
+  c->built = c->listed = false;
+  float *f[8] = {x, y, z, vx, vy, vz, mass, phi};
+  float *df[8];
+  int64_t *did = id;
+  uint16_t *dmask = mask;
+  const size_t nn = (size_t)std::max<int64_t>(n, 1);
+  if (ptr_flags & HSR_DEVICE_PTRS) {
+    for (int k = 0; k < 8; ++k) df[k] = f[k];
+  } else {
+    for (int k = 0; k < 8; ++k) {
+      df[k] = nullptr;
+      if (!f[k]) continue;
+      HSR_HIP(dbuf_reserve(c->s.host_in[k], sizeof(float) * nn));
+      df[k] = dptr<float>(c->s.host_in[k]);
+      if (n) HSR_HIP(hipMemcpyAsync(df[k], f[k], sizeof(float) * n, hipMemcpyHostToDevice, st));
+    }
+    did = nullptr;
+    dmask = nullptr;
+    if (id) {
+      HSR_HIP(dbuf_reserve(c->s.host_in[8], sizeof(int64_t) * nn));
+      did = dptr<int64_t>(c->s.host_in[8]);
+      if (n) HSR_HIP(hipMemcpyAsync(did, id, sizeof(int64_t) * n, hipMemcpyHostToDevice, st));
+    }
+    if (mask) {
+      HSR_HIP(dbuf_reserve(c->s.host_in[9], sizeof(uint16_t) * nn));
+      dmask = dptr<uint16_t>(c->s.host_in[9]);
+      if (n) HSR_HIP(hipMemcpyAsync(dmask, mask, sizeof(uint16_t) * n, hipMemcpyHostToDevice, st));
+    }
+  }
+  HSR_HIP(hipEventRecord(c->ev[0], st));
+  s = tree_build_device(c, nboxes, box_off, min_loc, max_loc, df[0], df[1], df[2], df[6]);
+  if (s) return s;
+  HSR_HIP(hipEventRecord(c->ev[1], st));
+  if (n) {
+    s = permute_in_place(c, n, df, did, dmask);
+    if (s) return s;
+  }
+  if (!(ptr_flags & HSR_DEVICE_PTRS) && n) {
+    for (int k = 0; k < 8; ++k)
+      if (f[k]) HSR_HIP(hipMemcpyAsync(f[k], df[k], sizeof(float) * n, hipMemcpyDeviceToHost, st));
+    if (id) HSR_HIP(hipMemcpyAsync(id, did, sizeof(int64_t) * n, hipMemcpyDeviceToHost, st));
+    if (mask) HSR_HIP(hipMemcpyAsync(mask, dmask, sizeof(uint16_t) * n, hipMemcpyDeviceToHost, st));
+  }
+  HSR_HIP(hipStreamSynchronize(st));
+  float ms = 0.f;
+  HSR_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  c->build_ms = ms;
+  c->user_v[0] = vx;
+  c->user_v[1] = vy;
+  c->user_v[2] = vz;
+  c->user_mass = mass;
+  c->user_flags = ptr_flags;
+  // device-side velocity arrays for hsr_kick
+  if (!(ptr_flags & HSR_DEVICE_PTRS)) {
+    c->user_dv[0] = df[3];
+    c->user_dv[1] = df[4];
+    c->user_dv[2] = df[5];
+  } else {
+    c->user_dv[0] = vx;
+    c->user_dv[1] = vy;
+    c->user_dv[2] = vz;
+  }
+  c->built = true;
+  return HSR_OK;
+}
+
+}  // namespace
+
+void hsr_set_error(const std::string &msg) { g_err = msg; }
+
+hipError_t dbuf_reserve(DBuf &b, size_t bytes) {
+  if (bytes <= b.bytes && b.p) return hipSuccess;
+  if (b.p) {
+    hipError_t e = hipFree(b.p);
+    if (e != hipSuccess) return e;
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  size_t want = std::max<size_t>(bytes, 256);
+  hipError_t e = hipMalloc(&b.p, want);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return e;
+  }
+  b.bytes = want;
+  return hipSuccess;
+}
+
+void dbuf_free(DBuf &b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+extern "C" {
+
+const char *hsr_last_error(void) { return g_err.c_str(); }
+
+const char *hsr_version(void) { return "hacc-sr-mi355x 0.1 (gfx950)"; }
+
+hsr_status hsr_create(hsr_ctx **out, const hsr_params *p) {
+  if (!out || !p) {
+    g_err = "hsr_create: null argument";
+    return HSR_ERR_ARG;
+  }
+  *out = nullptr;
+  if (p->tdpts != 1) {
+    g_err = "hsr_create: only TDPTS = 1 (RCBMonopoleForceTree) is implemented on the device";
+    return HSR_ERR_UNSUPPORTED;
+  }
+  if (p->nd < 1) {
+    g_err = "hsr_create: nd must be >= 1";
+    return HSR_ERR_ARG;
+  }
+  if (!(p->fsrrmax > 0.f) || !(p->rsm >= 0.f) || !(p->ppc > 0.f)) {
+    g_err = "hsr_create: fsrrmax > 0, rsm >= 0, ppc > 0 required";
+    return HSR_ERR_ARG;
+  }
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) {
+    g_err = "hsr_create: no HIP device available (there is no CPU path)";
+    return HSR_ERR_HIP;
+  }
+  if (p->device < 0 || p->device >= ndev) {
+    g_err = "hsr_create: device ordinal out of range";
+    return HSR_ERR_ARG;
+  }
+  hsr_ctx *c = new hsr_ctx();
+  c->p = *p;
+  c->device = p->device;
+  hsr_status s = make_law(*p, c->law);
+  if (s) {
+    delete c;
+    return s;
+  }
+  memset(&c->tuning, 0, sizeof(c->tuning));
+  if (hipSetDevice(c->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    g_err = "hsr_create: stream creation failed";
+    delete c;
+    return HSR_ERR_HIP;
+  }
+  for (int k = 0; k < 8; ++k) (void)hipEventCreate(&c->ev[k]);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, c->device) == hipSuccess)
+    c->num_cus = prop.multiProcessorCount;
+  *out = c;
+  return HSR_OK;
+}
+
+hsr_status hsr_destroy(hsr_ctx *c) {
+  if (!c) return HSR_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  DBuf *bufs[] = {&c->tree.bnodes, &c->tree.canon_nodes, &c->tree.pfin, &c->tree.pj,
+                  &c->tree.xs, &c->tree.ys, &c->tree.zs, &c->tree.ms,
+                  &c->lists.leaves, &c->lists.seg_ptr, &c->lists.seg, &c->lists.ent_ptr,
+                  &c->lists.jidx, &c->lists.items, &c->lists.stack,
+                  &c->s.a, &c->s.b, &c->s.c, &c->s.d, &c->s.e, &c->s.f, &c->s.g, &c->s.h,
+                  &c->s.cub, &c->s.x1, &c->s.y1, &c->s.z1, &c->s.m1, &c->s.p1,
+                  &c->s.x0, &c->s.y0, &c->s.z0, &c->s.m0, &c->s.p0, &c->s.blocks,
+                  &c->s.pblocks, &c->s.counters, &c->dev_v, &c->force};
+  for (DBuf *b : bufs) dbuf_free(*b);
+  for (int k = 0; k < 10; ++k) dbuf_free(c->s.host_in[k]);
+  for (int k = 0; k < 8; ++k) (void)hipEventDestroy(c->ev[k]);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return HSR_OK;
+}
+
+hsr_status hsr_get_stream(hsr_ctx *c, void **stream) {
+  if (!c || !stream) {
+    g_err = "hsr_get_stream: null argument";
+    return HSR_ERR_ARG;
+  }
+  *stream = (void *)c->stream;
+  return HSR_OK;
+}
+
+hsr_status hsr_build(hsr_ctx *c, int64_t count, float *x, float *y, float *z,
+                     float *vx, float *vy, float *vz, float *mass, float *phi,
+                     int64_t *id, uint16_t *mask, const float min_loc[3],
+                     const float max_loc[3], int ptr_flags) {
+  if (count < 0) {
+    g_err = "hsr_build: negative count";
+    return HSR_ERR_ARG;
+  }
+  int64_t off[2] = {0, count};
+  return build_common(c, 1, off, x, y, z, vx, vy, vz, mass, phi, id, mask,
+                      min_loc, max_loc, ptr_flags);
+}
+
+hsr_status hsr_build_many(hsr_ctx *c, int64_t nboxes, const int64_t *offsets,
+                          float *x, float *y, float *z, float *vx, float *vy,
+                          float *vz, float *mass, float *phi, int64_t *id,
+                          uint16_t *mask, const float *min_loc,
+                          const float *max_loc, int ptr_flags) {
+  return build_common(c, nboxes, offsets, x, y, z, vx, vy, vz, mass, phi, id,
+                      mask, min_loc, max_loc, ptr_flags);
+}
+
+hsr_status hsr_lists(hsr_ctx *c, const float min_force[3],
+                     const float max_force[3]) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (!c->built) {
+    g_err = "hsr_lists: no tree (call hsr_build first)";
+    return HSR_ERR_STATE;
+  }
+  if (c->tree.nboxes != 1) {
+    g_err = "hsr_lists: context holds several boxes; use hsr_lists_many";
+    return HSR_ERR_STATE;
+  }
+  if (!min_force || !max_force) {
+    g_err = "hsr_lists: null force box";
+    return HSR_ERR_ARG;
+  }
+  HSR_HIP(hipEventRecord(c->ev[2], c->stream));
+  s = lists_build_device(c, min_force, max_force);
+  if (s) return s;
+  HSR_HIP(hipEventRecord(c->ev[3], c->stream));
+  HSR_HIP(hipEventSynchronize(c->ev[3]));
+  float ms = 0.f;
+  HSR_HIP(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+  c->lists_ms = ms;
+  c->listed = true;
+  return HSR_OK;
+}
+
+hsr_status hsr_lists_many(hsr_ctx *c, const float *min_force,
+                          const float *max_force) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (!c->built) {
+    g_err = "hsr_lists_many: no tree";
+    return HSR_ERR_STATE;
+  }
+  if (!min_force || !max_force) {
+    g_err = "hsr_lists_many: null force boxes";
+    return HSR_ERR_ARG;
+  }
+  HSR_HIP(hipEventRecord(c->ev[2], c->stream));
+  s = lists_build_device(c, min_force, max_force);
+  if (s) return s;
+  HSR_HIP(hipEventRecord(c->ev[3], c->stream));
+  HSR_HIP(hipEventSynchronize(c->ev[3]));
+  float ms = 0.f;
+  HSR_HIP(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+  c->lists_ms = ms;
+  c->listed = true;
+  return HSR_OK;
+}
+
+hsr_status hsr_forces(hsr_ctx *c, float *ax, float *ay, float *az,
+                      int ptr_flags) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (!c->listed) {
+    g_err = "hsr_forces: no lists (call hsr_lists first)";
+    return HSR_ERR_STATE;
+  }
+  if (!ax || !ay || !az) {
+    g_err = "hsr_forces: null output";
+    return HSR_ERR_ARG;
+  }
+  int64_t n = c->tree.n;
+  float *fx = ax, *fy = ay, *fz = az;
+  if (!(ptr_flags & HSR_DEVICE_PTRS)) {
+    HSR_HIP(dbuf_reserve(c->force, sizeof(float) * 3 * std::max<int64_t>(n, 1)));
+    fx = dptr<float>(c->force);
+    fy = fx + n;
+    fz = fy + n;
+  }
+  HSR_HIP(hipEventRecord(c->ev[4], c->stream));
+  s = force_run(c, fx, fy, fz);
+  if (s) return s;
+  HSR_HIP(hipEventRecord(c->ev[5], c->stream));
+  if (!(ptr_flags & HSR_DEVICE_PTRS) && n) {
+    HSR_HIP(hipMemcpyAsync(ax, fx, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    HSR_HIP(hipMemcpyAsync(ay, fy, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    HSR_HIP(hipMemcpyAsync(az, fz, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+  }
+  HSR_HIP(hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  HSR_HIP(hipEventElapsedTime(&ms, c->ev[4], c->ev[5]));
+  c->force_ms = ms;
+  return HSR_OK;
+}
+
+hsr_status hsr_kick(hsr_ctx *c, float fcoeff) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (!c->listed) {
+    g_err = "hsr_kick: no lists (call hsr_lists first)";
+    return HSR_ERR_STATE;
+  }
+  if (!c->user_dv[0] || !c->user_dv[1] || !c->user_dv[2]) {
+    g_err = "hsr_kick: the build was given no velocity arrays";
+    return HSR_ERR_STATE;
+  }
+  int64_t n = c->tree.n;
+  HSR_HIP(hipEventRecord(c->ev[4], c->stream));
+  s = kick_run(c, fcoeff, c->user_dv[0], c->user_dv[1], c->user_dv[2], dptr<float>(c->tree.ms));
+  if (s) return s;
+  HSR_HIP(hipEventRecord(c->ev[5], c->stream));
+  if (!(c->user_flags & HSR_DEVICE_PTRS) && n) {
+    for (int k = 0; k < 3; ++k)
+      HSR_HIP(hipMemcpyAsync(c->user_v[k], c->user_dv[k], sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+  }
+  HSR_HIP(hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  HSR_HIP(hipEventElapsedTime(&ms, c->ev[4], c->ev[5]));
+  c->force_ms = ms;
+  return HSR_OK;
+}
+
+hsr_status hsr_adjoint(hsr_ctx *c, const float *lx, const float *ly,
+                       const float *lz, float *gx, float *gy, float *gz,
+                       int ptr_flags) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (!c->listed) {
+    g_err = "hsr_adjoint: no lists";
+    return HSR_ERR_STATE;
+  }
+  if (!(ptr_flags & HSR_DEVICE_PTRS)) {
+    g_err = "hsr_adjoint: device pointers required";
+    return HSR_ERR_ARG;
+  }
+  return adjoint_run(c, lx, ly, lz, gx, gy, gz);
+}
+
+hsr_status hsr_get_stats(hsr_ctx *c, hsr_stats *st) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (!st) {
+    g_err = "hsr_get_stats: null output";
+    return HSR_ERR_ARG;
+  }
+  memset(st, 0, sizeof(*st));
+  st->count = c->tree.n;
+  st->nodes = c->tree.nnodes;
+  st->max_depth = c->tree.max_depth;
+  if (c->built && c->tree.nnodes > 0) {
+    std::vector<hsr_node> nodes(c->tree.nnodes);
+    HSR_HIP(hipMemcpy(nodes.data(), c->tree.canon_nodes.p, sizeof(hsr_node) * c->tree.nnodes,
+                      hipMemcpyDeviceToHost));
+    // printStats (RCBForceTree.cxx:460-511) counts from node 1
+    for (size_t t = 1; t < nodes.size(); ++t) {
+      if (nodes[t].cl == 0 && nodes[t].cr == 0) {
+        if (nodes[t].count > 0) {
+          st->leaves++;
+          st->max_ppn = std::max<int64_t>(st->max_ppn, nodes[t].count);
+        } else {
+          st->empty_leaves++;
+        }
+      }
+    }
+  }
+  st->listed_leaves = c->lists.nleaves;
+  st->segments = c->lists.nsegs;
+  st->entries = c->lists.nentries;
+  st->pairs = c->lists.pairs;
+  st->build_ms = c->build_ms;
+  st->lists_ms = c->lists_ms;
+  st->force_ms = c->force_ms;
+  return HSR_OK;
+}
+
+hsr_status hsr_get_nodes(hsr_ctx *c, hsr_node *out, int64_t cap) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (!c->built) {
+    g_err = "hsr_get_nodes: no tree";
+    return HSR_ERR_STATE;
+  }
+  if (!out || cap < c->tree.nnodes) {
+    g_err = "hsr_get_nodes: output too small";
+    return HSR_ERR_ARG;
+  }
+  if (c->tree.nnodes)
+    HSR_HIP(hipMemcpy(out, c->tree.canon_nodes.p, sizeof(hsr_node) * c->tree.nnodes,
+                      hipMemcpyDeviceToHost));
+  return HSR_OK;
+}
+
+hsr_status hsr_get_perm(hsr_ctx *c, int64_t *perm, int64_t cap) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (!c->built) {
+    g_err = "hsr_get_perm: no tree";
+    return HSR_ERR_STATE;
+  }
+  int64_t n = c->tree.n;
+  if (!perm || cap < n) {
+    g_err = "hsr_get_perm: output too small";
+    return HSR_ERR_ARG;
+  }
+  std::vector<int32_t> p(n);
+  if (n) HSR_HIP(hipMemcpy(p.data(), c->tree.pfin.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+  for (int64_t k = 0; k < n; ++k) perm[k] = p[k];
+  return HSR_OK;
+}
+
+hsr_status hsr_get_lists(hsr_ctx *c, int64_t *leaves, int64_t *seg_ptr,
+                         int64_t *seg_a, int64_t *seg_b, int64_t *entries,
+                         int64_t cap_leaves, int64_t cap_segs) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (!c->listed) {
+    g_err = "hsr_get_lists: no lists";
+    return HSR_ERR_STATE;
+  }
+  ListState &L = c->lists;
+  if (cap_leaves < L.nleaves || cap_segs < L.nsegs) {
+    g_err = "hsr_get_lists: output too small";
+    return HSR_ERR_ARG;
+  }
+  int64_t nl = L.nleaves, ns = L.nsegs;
+  std::vector<int32_t> lv(nl);
+  std::vector<int64_t> ep(nl + 1);
+  if (nl) {
+    HSR_HIP(hipMemcpy(lv.data(), L.leaves.p, sizeof(int32_t) * nl, hipMemcpyDeviceToHost));
+    HSR_HIP(hipMemcpy(seg_ptr, L.seg_ptr.p, sizeof(int64_t) * (nl + 1), hipMemcpyDeviceToHost));
+    HSR_HIP(hipMemcpy(ep.data(), L.ent_ptr.p, sizeof(int64_t) * (nl + 1), hipMemcpyDeviceToHost));
+  } else if (seg_ptr) {
+    seg_ptr[0] = 0;
+  }
+  if (ns) {
+    const int64_t *sa = dptr<int64_t>(L.seg);
+    HSR_HIP(hipMemcpy(seg_a, sa, sizeof(int64_t) * ns, hipMemcpyDeviceToHost));
+    HSR_HIP(hipMemcpy(seg_b, sa + ns + 1, sizeof(int64_t) * ns, hipMemcpyDeviceToHost));
+  }
+  for (int64_t k = 0; k < nl; ++k) {
+    leaves[k] = lv[k];
+    if (entries) entries[k] = ep[k + 1] - ep[k];
+  }
+  return HSR_OK;
+}
+
+hsr_status hsr_set_tuning(hsr_ctx *c, const hsr_tuning *t) {
+  if (!c || !t) {
+    g_err = "hsr_set_tuning: null argument";
+    return HSR_ERR_ARG;
+  }
+  c->tuning = *t;
+  return HSR_OK;
+}
+
+hsr_status hsr_get_tuning(hsr_ctx *c, hsr_tuning *t) {
+  if (!c || !t) {
+    g_err = "hsr_get_tuning: null argument";
+    return HSR_ERR_ARG;
+  }
+  *t = c->tuning;
+  return HSR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,170 @@
+// hsr_internal.h -- device data layout and kernel launchers shared by the
+// translation units of libhaccsr.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/hacc_sr.h"
+
+#define HSR_WAVE 64
+
+// Node of the breadth-first build table.  Nodes of one level are contiguous;
+// children of a level are appended as the next level.  The canonical
+// (reference) numbering is derived after the build.
+struct BNode {
+  int64_t count, offset;
+  int32_t parent, cl, cr;  // BFS ids, -1 if none
+  int32_t kind;            // NODE_LEAF / NODE_SPLIT / NODE_DEGEN
+  int32_t split_d;
+  int32_t depth;
+  int64_t is;              // partition result (left count)
+  float xmin[3], xmax[3], xc[3];
+  float tdr, ppm;
+  int64_t canon;           // canonical id (allocation order of the unthreaded build)
+  int64_t sa;              // allocating nodes in the subtree (incl. itself)
+  int64_t a;               // allocating nodes before this one in preorder
+  int64_t nleaf;           // visited leaves in the subtree
+  int64_t leaf_rank;       // rank in calcInternodeForces' right-first order
+  int64_t blk0;            // first block of this node in a level block table
+  int64_t nblk;
+};
+
+enum { NODE_LEAF = 0, NODE_SPLIT = 1, NODE_DEGEN = 2 };
+
+// Exact centre-of-mass accumulators of one block of a big node.
+struct CmBlock {
+  int64_t node;            // BFS id
+  int64_t start, len;      // absolute element range
+  float approx[4];         // float sums of x, y, z, m addends
+  float ein[4];            // approximate exclusive prefix within the node
+  uint64_t d0[4], d1[4];   // fixed-grid maps
+  int32_t E[4];            // grid exponent each map was built for
+  int32_t bad[4];
+  float bmin[3], bmax[3];
+};
+
+struct PartBlock {
+  int64_t node;            // BFS id
+  int64_t start, len;      // absolute element range
+  int64_t cnt;             // keys < pivot in the block
+  int64_t base;            // exclusive prefix within the node
+};
+
+// Growable device buffer.
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+hipError_t dbuf_reserve(DBuf &b, size_t bytes);
+void dbuf_free(DBuf &b);
+
+template <class T>
+T *dptr(DBuf &b) {
+  return reinterpret_cast<T *>(b.p);
+}
+
+// Sorted j-particle record (x, y, z, m) for the leaf kernel; pseudo-particles
+// of node k follow the particles at index count + k.
+struct __align__(16) PJ {
+  float x, y, z, m;
+};
+
+struct ForceLawDev {
+  float c[7];     // polynomial P(r2) coefficients a0..a6
+  int order;      // 5 or 6 (0 = no polynomial)
+  float eps2;     // rsm^2
+  float rc2;      // strict cutoff: pairs with r2 < rc2 contribute
+  int guard;      // 1: r2 > 0 guard (needed when eps2 == 0)
+  int newton;     // 1: f = r^-3 evaluated without softening
+};
+
+// Tree-build state kept in the context between calls.
+struct TreeState {
+  int64_t n = 0;
+  int64_t nnodes_bfs = 0;
+  int64_t nnodes = 0;         // canonical count
+  int64_t max_depth = 0;
+  std::vector<int64_t> level_start;  // BFS id ranges per level
+  DBuf bnodes;                // BNode[nnodes_bfs]
+  DBuf canon_nodes;           // hsr_node[nnodes]
+  DBuf pfin;                  // int32 final permutation (sorted -> input)
+  DBuf pj;                    // PJ[n + nnodes]
+  DBuf xs, ys, zs, ms;        // sorted float arrays
+  int64_t nboxes = 1;
+  std::vector<int64_t> box_root;  // BFS id of each box root
+};
+
+struct ListState {
+  int64_t nleaves = 0;        // listed leaves
+  int64_t nsegs = 0;
+  int64_t nentries = 0;
+  int64_t pairs = 0;
+  DBuf leaves;                // int32 BFS ids of listed leaves (reference order)
+  DBuf seg_ptr;               // int64 [nleaves+1]
+  DBuf seg;                   // int2-like (int64 a, int64 b) pairs
+  DBuf ent_ptr;               // int64 [nleaves+1] list-entry CSR
+  DBuf jidx;                  // int32 [nentries] index into PJ
+  DBuf items;                 // work items for the force kernel
+  int64_t nitems = 0;
+  DBuf stack;                 // DFS stacks
+};
+
+struct Scratch {
+  DBuf a, b, c, d, e, f, g, h, cub;
+  DBuf x1, y1, z1, m1, p1, x0, y0, z0, m0, p0;
+  DBuf blocks, pblocks;
+  DBuf counters;
+  DBuf host_in[10];
+};
+
+struct hsr_ctx {
+  hsr_params p;
+  ForceLawDev law;
+  hsr_tuning tuning;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[8];
+  int device = 0;
+  int num_cus = 0;
+  TreeState tree;
+  ListState lists;
+  Scratch s;
+  // caller arrays of the last build (for hsr_kick)
+  float *user_v[3] = {nullptr, nullptr, nullptr};
+  float *user_mass = nullptr;
+  float *user_dv[3] = {nullptr, nullptr, nullptr};  // device velocity arrays
+  int user_flags = 0;
+  DBuf dev_v;  // device copy of velocities for host-pointer builds
+  DBuf force;  // float3 SoA force [3][n]
+  bool built = false, listed = false;
+  double build_ms = 0, lists_ms = 0, force_ms = 0;
+};
+
+void hsr_set_error(const std::string &msg);
+
+#define HSR_HIP(call)                                                     \
+  do {                                                                    \
+    hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess) {                                               \
+      hsr_set_error(std::string("HIP error: ") + hipGetErrorString(e_) +  \
+                    " at " + __FILE__ + ":" + std::to_string(__LINE__));  \
+      return HSR_ERR_HIP;                                                 \
+    }                                                                     \
+  } while (0)
+
+// launchers (defined in the .hip files)
+hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
+                             const int64_t *box_off, const float *min_loc,
+                             const float *max_loc, const float *x,
+                             const float *y, const float *z, const float *m);
+hsr_status lists_build_device(hsr_ctx *c, const float *min_force,
+                              const float *max_force);
+hsr_status force_run(hsr_ctx *c, float *ax, float *ay, float *az);
+hsr_status kick_run(hsr_ctx *c, float fcoeff, float *vx, float *vy, float *vz,
+                    const float *mass_sorted);
+hsr_status adjoint_run(hsr_ctx *c, const float *lx, const float *ly,
+                       const float *lz, float *gx, float *gy, float *gz);

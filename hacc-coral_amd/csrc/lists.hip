@@ -1,0 +1,387 @@
+// lists.hip -- bit-exact per-leaf interaction lists on the GPU.
+//
+// Restates RCBForceTree<1>::calcInternodeForces / calcInternodeForce
+// (RCBForceTree.cxx:924-1197, useRealOA = false): for every leaf inside the
+// force box, a right-child-first LIFO walk from the root that
+//   * expands the leaf's ancestors to their children (except the leaf),
+//   * accepts a node whose angular size passes the tan(oa) test and drops
+//     it if its centre is farther than fsrrmax; an accepted node with
+//     count <= TDPTS contributes its particles, else its pseudo-particle,
+//   * takes a not-accepted leaf's particles,
+//   * pushes the children of a not-accepted internal node whose L-infinity
+//     gap to the target leaf is <= fsrrmax,
+// and finally appends the leaf's own particles.  One thread walks one leaf
+// (two passes: count, then fill), with its LIFO stack in HBM laid out
+// level-major so the threads of a wave touch neighbouring words.
+//
+// Compiled with -ffp-contract=off so dist2, l2, dtt2 and the corner
+// diagonals round exactly as the reference's x86 float code does.
+#include "hsr_internal.h"
+#include "wave.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <vector>
+
+namespace {
+
+struct ListParams {
+  float fsrrmax, fsrrmax2, tan_oa;
+  int64_t stack_depth;
+  int64_t nthreads;
+  int64_t *overflow;
+};
+
+// Leaf of box b in the right-first order gets slot leaf_base[b] + leaf_rank.
+__global__ void k_leaf_slots(const BNode *nodes, int64_t nb,
+                             const int32_t *box_of, const int64_t *leaf_base,
+                             const float *fmin, const float *fmax,
+                             int32_t *slot_canon, const int64_t *cbase) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nb) return;
+  const BNode &n = nodes[t];
+  if (n.kind == NODE_SPLIT) return;
+  int b = box_of[t];
+  const float *lo = fmin + 3 * b, *hi = fmax + 3 * b;
+  // RCBForceTree.cxx:1166-1170 (strict open interval on either face)
+  bool inside = true;
+  for (int i = 0; i < 3; ++i) {
+    inside &= (n.xmax[i] < hi[i] && n.xmax[i] > lo[i]) ||
+              (n.xmin[i] < hi[i] && n.xmin[i] > lo[i]);
+  }
+  slot_canon[leaf_base[b] + n.leaf_rank] =
+      inside ? (int32_t)(cbase[b] + n.canon) : -1;
+}
+
+__global__ void k_flag_valid(const int32_t *slot, int64_t n, int32_t *flag) {
+  int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < n) flag[k] = slot[k] >= 0 ? 1 : 0;
+}
+
+__global__ void k_compact(const int32_t *slot, const int32_t *flag,
+                          const int32_t *pos, int64_t n, int32_t *out,
+                          int32_t *out_box, const int32_t *slot_box) {
+  int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < n && flag[k]) {
+    out[pos[k]] = slot[k];
+    out_box[pos[k]] = slot_box[k];
+  }
+}
+
+__global__ void k_slot_box(int64_t nboxes, const int64_t *leaf_base,
+                           int32_t *slot_box) {
+  int64_t b = blockIdx.x;
+  if (b >= nboxes) return;
+  for (int64_t k = leaf_base[b] + threadIdx.x; k < leaf_base[b + 1];
+       k += blockDim.x)
+    slot_box[k] = (int32_t)b;
+}
+
+template <bool FILL>
+__global__ void __launch_bounds__(64)
+k_walk(const hsr_node *__restrict__ t, const int32_t *__restrict__ leaves,
+       const int32_t *__restrict__ leaf_box, const int64_t *__restrict__ cbase,
+       int64_t nleaves, ListParams lp, int32_t *stack, int64_t *nseg_out,
+       int64_t *nent_out, const int64_t *seg_ptr, int64_t *seg_a,
+       int64_t *seg_b, int64_t *seg_e) {
+  int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= nleaves) return;
+  const int64_t tl = leaves[k];
+  const int64_t root = cbase[leaf_box[k]];
+  const float fsrrmax = lp.fsrrmax, fsrrmax2 = lp.fsrrmax2;
+  const float tanOA = lp.tan_oa;
+  const hsr_node &L = t[tl];
+  const int64_t toff = L.offset, tend = L.offset + L.count;
+  const float lxc0 = L.xc[0], lxc1 = L.xc[1], lxc2 = L.xc[2];
+  float lmin[3] = {L.xmin[0], L.xmin[1], L.xmin[2]};
+  float lmax[3] = {L.xmax[0], L.xmax[1], L.xmax[2]};
+  int32_t *stk = stack + k;
+  const int64_t sstride = lp.nthreads;
+  int64_t sp = 0;
+  int64_t nseg = 0, nent = 0;
+  int64_t so = FILL ? seg_ptr[k] : 0;
+  bool dead = false;
+  stk[0] = (int32_t)root;
+  sp = 1;
+  // the LIFO never holds more than (tree depth + 2) nodes; the guard only
+  // protects the stack buffer
+#define PUSH(v)                                             \
+  do {                                                      \
+    if (sp >= lp.stack_depth) {                             \
+      atomicAdd((unsigned long long *)lp.overflow, 1ull);   \
+      dead = true;                                          \
+    } else {                                                \
+      stk[sp * sstride] = (int32_t)(v);                     \
+      ++sp;                                                 \
+    }                                                       \
+  } while (0)
+#define EMIT(A, B, LEN)                   \
+  do {                                    \
+    if (FILL) {                           \
+      seg_a[so + nseg] = (A);             \
+      seg_b[so + nseg] = (B);             \
+      seg_e[so + nseg] = nent;            \
+    }                                     \
+    ++nseg;                               \
+    nent += (LEN);                        \
+  } while (0)
+  while (sp > 0 && !dead) {
+    --sp;
+    const int64_t tln = stk[sp * sstride];
+    const hsr_node &N = t[tln];
+    if (tln < tl) {
+      // ancestors of tl are exactly the nodes whose particle range contains
+      // tl's (node ranges nest or are disjoint)
+      if (N.offset <= toff && tend <= N.offset + N.count) {
+        const int64_t cr = N.cr, cl = N.cl;
+        if (cl != tl && cl > 0 && t[cl].count > 0) PUSH(cl);
+        if (cr != tl && cr > 0 && t[cr].count > 0) PUSH(cr);
+        continue;
+      }
+    }
+    float dx = N.xc[0] - lxc0;
+    float dy = N.xc[1] - lxc1;
+    float dz = N.xc[2] - lxc2;
+    float dist2 = dx * dx + dy * dy + dz * dz;
+    float sx = N.xmax[0] - N.xmin[0];
+    float sy = N.xmax[1] - N.xmin[1];
+    float sz = N.xmax[2] - N.xmin[2];
+    float sx2 = sx * sx, sy2 = sy * sy, sz2 = sz * sz;
+    float syz = (sz2 < sy2) ? sz2 : sy2;
+    float l2 = (syz < sx2) ? syz : sx2;
+    float dtt2 = dist2 * tanOA * tanOA;
+    bool looksBig;
+    if (l2 > dtt2) {
+      looksBig = true;
+    } else {
+      looksBig = false;
+      for (int i = 0; i < 2 && !looksBig; ++i)
+        for (int j = 0; j < 2; ++j) {
+          float x1 = (i == 0 ? N.xmin[0] : N.xmax[0]) - lxc0;
+          float y1 = (j == 0 ? N.xmin[1] : N.xmax[1]) - lxc1;
+          float z1 = N.xmin[2] - lxc2;
+          float x2 = (i == 0 ? N.xmax[0] : N.xmin[0]) - lxc0;
+          float y2 = (j == 0 ? N.xmax[1] : N.xmin[1]) - lxc1;
+          float z2 = N.xmax[2] - lxc2;
+          float ddx = x1 - x2, ddy = y1 - y2, ddz = z1 - z2;
+          float dh2 = ddx * ddx + ddy * ddy + ddz * ddz;
+          if (dh2 > dtt2) {
+            looksBig = true;
+            break;
+          }
+        }
+    }
+    if (!looksBig) {
+      if (dist2 > fsrrmax2) continue;
+      if (N.count <= 1) {
+        EMIT(N.offset, N.count, N.count);
+        continue;
+      }
+      EMIT(tln, -1, 1);
+      continue;
+    } else if (N.cr == 0 && N.cl == 0) {
+      EMIT(N.offset, N.count, N.count);
+      continue;
+    }
+    const int64_t ch[2] = {N.cl, N.cr};
+    for (int q = 0; q < 2; ++q) {
+      const int64_t c = ch[q];
+      if (c > 0 && t[c].count > 0) {
+        const hsr_node &Cn = t[c];
+        bool close = true;
+        for (int i = 0; i < 3; ++i) {
+          float dist = 0;
+          if (lmax[i] < Cn.xmin[i]) dist = Cn.xmin[i] - lmax[i];
+          else if (lmin[i] > Cn.xmax[i]) dist = lmin[i] - Cn.xmax[i];
+          if (dist > fsrrmax) {
+            close = false;
+            break;
+          }
+        }
+        if (close) PUSH(c);
+      }
+    }
+  }
+  EMIT(L.offset, L.count, L.count);
+#undef PUSH
+#undef EMIT
+  if (!FILL) {
+    nseg_out[k] = nseg;
+    nent_out[k] = nent;
+  }
+}
+
+// Expand segments into j indices (one wave per listed leaf): particle
+// ranges by all lanes, pseudo-particles as index count + node.
+__global__ void __launch_bounds__(64)
+k_expand(int64_t nleaves, int64_t n, const int64_t *seg_ptr,
+         const int64_t *seg_a, const int64_t *seg_b, const int64_t *ent_ptr,
+         int32_t *jidx) {
+  int64_t k = blockIdx.x;
+  if (k >= nleaves) return;
+  int l = lane_id();
+  int64_t e = ent_ptr[k];
+  for (int64_t s = seg_ptr[k]; s < seg_ptr[k + 1]; ++s) {
+    int64_t a = seg_a[s], b = seg_b[s];
+    if (b < 0) {
+      if (l == 0) jidx[e] = (int32_t)(n + a);
+      e += 1;
+    } else {
+      for (int64_t i = l; i < b; i += 64) jidx[e + i] = (int32_t)(a + i);
+      e += b;
+    }
+  }
+}
+
+__global__ void k_pairs(int64_t nleaves, const hsr_node *t,
+                        const int32_t *leaves, const int64_t *ent_ptr,
+                        int64_t *pairs_out) {
+  int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t v = 0;
+  if (k < nleaves) v = t[leaves[k]].count * (ent_ptr[k + 1] - ent_ptr[k]);
+  v = wave_sum_i64(v);
+  if (lane_id() == 0 && v)
+    atomicAdd((unsigned long long *)pairs_out, (unsigned long long)v);
+}
+
+__global__ void k_zero_i64(int64_t *p, int64_t n) {
+  int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < n) p[k] = 0;
+}
+
+inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+template <class T>
+hsr_status excl_scan(hsr_ctx *c, const T *in, T *out, int64_t n) {
+  size_t tmp = 0;
+  HSR_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, c->stream));
+  HSR_HIP(dbuf_reserve(c->s.cub, tmp + 256));
+  HSR_HIP(hipcub::DeviceScan::ExclusiveSum(c->s.cub.p, tmp, in, out, (int)n, c->stream));
+  return HSR_OK;
+}
+
+}  // namespace
+
+// BFS node table -> listed leaves, per-leaf lists (segments + j indices).
+hsr_status lists_build_device(hsr_ctx *c, const float *min_force_h,
+                              const float *max_force_h) {
+  hipStream_t st = c->stream;
+  TreeState &T = c->tree;
+  ListState &LS = c->lists;
+  Scratch &S = c->s;
+  const int64_t nb = T.nnodes_bfs, nboxes = T.nboxes, n = T.n;
+  BNode *nodes = dptr<BNode>(T.bnodes);
+
+  // per-box leaf counts: root.nleaf (read back the roots)
+  std::vector<BNode> roots(nboxes);
+  HSR_HIP(hipMemcpyAsync(roots.data(), nodes, sizeof(BNode) * nboxes, hipMemcpyDeviceToHost, st));
+  HSR_HIP(hipStreamSynchronize(st));
+  std::vector<int64_t> leaf_base(nboxes + 1, 0);
+  for (int64_t b = 0; b < nboxes; ++b) leaf_base[b + 1] = leaf_base[b] + roots[b].nleaf;
+  const int64_t nslots = leaf_base[nboxes];
+
+  // device scratch: leaf_base, cbase, force boxes, slot arrays
+  size_t need = sizeof(int64_t) * 2 * (nboxes + 1) + sizeof(float) * 6 * nboxes +
+                sizeof(int32_t) * 4 * (nslots + 1) + 256;
+  HSR_HIP(dbuf_reserve(S.g, need));
+  int64_t *d_leaf_base = dptr<int64_t>(S.g);
+  int64_t *d_cbase = d_leaf_base + nboxes + 1;
+  float *d_fmin = reinterpret_cast<float *>(d_cbase + nboxes + 1);
+  float *d_fmax = d_fmin + 3 * nboxes;
+  int32_t *slot = reinterpret_cast<int32_t *>(d_fmax + 3 * nboxes);
+  int32_t *flag = slot + nslots + 1;
+  int32_t *pos = flag + nslots + 1;
+  int32_t *slot_box = pos + nslots + 1;
+  HSR_HIP(hipMemcpyAsync(d_leaf_base, leaf_base.data(), sizeof(int64_t) * (nboxes + 1), hipMemcpyHostToDevice, st));
+  HSR_HIP(hipMemcpyAsync(d_cbase, T.box_root.data(), sizeof(int64_t) * (nboxes + 1), hipMemcpyHostToDevice, st));
+  HSR_HIP(hipMemcpyAsync(d_fmin, min_force_h, sizeof(float) * 3 * nboxes, hipMemcpyHostToDevice, st));
+  HSR_HIP(hipMemcpyAsync(d_fmax, max_force_h, sizeof(float) * 3 * nboxes, hipMemcpyHostToDevice, st));
+  const int32_t *box_of = dptr<int32_t>(S.f);  // from the build
+
+  if (nslots > 0) {
+    k_leaf_slots<<<nblk(nb, 256), 256, 0, st>>>(nodes, nb, box_of, d_leaf_base, d_fmin, d_fmax, slot, d_cbase);
+    k_slot_box<<<(unsigned)nboxes, 256, 0, st>>>(nboxes, d_leaf_base, slot_box);
+    k_flag_valid<<<nblk(nslots, 256), 256, 0, st>>>(slot, nslots, flag);
+    if (excl_scan(c, flag, pos, nslots) != HSR_OK) return HSR_ERR_HIP;
+  }
+  int32_t last_pos = 0, last_flag = 0;
+  if (nslots > 0) {
+    HSR_HIP(hipMemcpyAsync(&last_pos, pos + nslots - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HSR_HIP(hipMemcpyAsync(&last_flag, flag + nslots - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HSR_HIP(hipStreamSynchronize(st));
+  }
+  const int64_t nl = nslots > 0 ? (int64_t)last_pos + last_flag : 0;
+  LS.nleaves = nl;
+  HSR_HIP(dbuf_reserve(LS.leaves, sizeof(int32_t) * (2 * nl + 2)));
+  int32_t *leaves = dptr<int32_t>(LS.leaves);
+  int32_t *leaf_box = leaves + nl + 1;
+  if (nl > 0)
+    k_compact<<<nblk(nslots, 256), 256, 0, st>>>(slot, flag, pos, nslots, leaves, leaf_box, slot_box);
+
+  // ---- pass 1: counts
+  HSR_HIP(dbuf_reserve(LS.seg_ptr, sizeof(int64_t) * (nl + 1)));
+  HSR_HIP(dbuf_reserve(LS.ent_ptr, sizeof(int64_t) * (nl + 1)));
+  HSR_HIP(dbuf_reserve(S.h, sizeof(int64_t) * 2 * (nl + 1) + 64));
+  int64_t *nseg = dptr<int64_t>(S.h);
+  int64_t *nent = nseg + nl + 1;
+  const int64_t depth = T.max_depth + 8;
+  HSR_HIP(dbuf_reserve(LS.stack, sizeof(int32_t) * (size_t)std::max<int64_t>(1, nl) * depth));
+  ListParams lp;
+  lp.fsrrmax = c->p.fsrrmax;
+  lp.fsrrmax2 = c->p.fsrrmax * c->p.fsrrmax;
+  lp.tan_oa = c->p.tan_oa;
+  lp.stack_depth = depth;
+  lp.nthreads = std::max<int64_t>(1, nl);
+  HSR_HIP(dbuf_reserve(S.counters, 256));
+  lp.overflow = dptr<int64_t>(S.counters) + 20;
+  k_zero_i64<<<1, 64, 0, st>>>(lp.overflow, 1);
+  const hsr_node *cn = dptr<hsr_node>(T.canon_nodes);
+  int32_t *stk = dptr<int32_t>(LS.stack);
+  if (nl > 0) {
+    k_walk<false><<<nblk(nl, 64), 64, 0, st>>>(cn, leaves, leaf_box, d_cbase, nl, lp, stk, nseg, nent,
+                                                nullptr, nullptr, nullptr, nullptr);
+    k_zero_i64<<<1, 64, 0, st>>>(nseg + nl, 1);
+    k_zero_i64<<<1, 64, 0, st>>>(nent + nl, 1);
+    if (excl_scan(c, nseg, dptr<int64_t>(LS.seg_ptr), nl + 1) != HSR_OK) return HSR_ERR_HIP;
+    if (excl_scan(c, nent, dptr<int64_t>(LS.ent_ptr), nl + 1) != HSR_OK) return HSR_ERR_HIP;
+  }
+  int64_t tot[2] = {0, 0};
+  if (nl > 0) {
+    HSR_HIP(hipMemcpyAsync(&tot[0], dptr<int64_t>(LS.seg_ptr) + nl, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HSR_HIP(hipMemcpyAsync(&tot[1], dptr<int64_t>(LS.ent_ptr) + nl, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HSR_HIP(hipStreamSynchronize(st));
+  }
+  LS.nsegs = tot[0];
+  LS.nentries = tot[1];
+  if (LS.nentries >= (int64_t)INT32_MAX) {
+    hsr_set_error("interaction lists exceed 2^31 entries");
+    return HSR_ERR_UNSUPPORTED;
+  }
+  // ---- pass 2: fill
+  HSR_HIP(dbuf_reserve(LS.seg, sizeof(int64_t) * 3 * (LS.nsegs + 1)));
+  int64_t *seg_a = dptr<int64_t>(LS.seg);
+  int64_t *seg_b = seg_a + LS.nsegs + 1;
+  int64_t *seg_e = seg_b + LS.nsegs + 1;
+  HSR_HIP(dbuf_reserve(LS.jidx, sizeof(int32_t) * (LS.nentries + 1)));
+  if (nl > 0) {
+    k_walk<true><<<nblk(nl, 64), 64, 0, st>>>(cn, leaves, leaf_box, d_cbase, nl, lp, stk, nullptr, nullptr,
+                                               dptr<int64_t>(LS.seg_ptr), seg_a, seg_b, seg_e);
+    k_expand<<<(unsigned)nl, 64, 0, st>>>(nl, n, dptr<int64_t>(LS.seg_ptr), seg_a, seg_b,
+                                          dptr<int64_t>(LS.ent_ptr), dptr<int32_t>(LS.jidx));
+  }
+  // pairs = sum count * list length
+  int64_t *d_pairs = dptr<int64_t>(S.counters) + 16;
+  k_zero_i64<<<1, 64, 0, st>>>(d_pairs, 1);
+  if (nl > 0) k_pairs<<<nblk(nl, 64), 64, 0, st>>>(nl, cn, leaves, dptr<int64_t>(LS.ent_ptr), d_pairs);
+  int64_t ovf = 0;
+  HSR_HIP(hipMemcpyAsync(&LS.pairs, d_pairs, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HSR_HIP(hipMemcpyAsync(&ovf, lp.overflow, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HSR_HIP(hipStreamSynchronize(st));
+  HSR_HIP(hipGetLastError());
+  if (ovf) {
+    hsr_set_error("interaction-list walk exceeded its stack bound");
+    return HSR_ERR_STATE;
+  }
+  return HSR_OK;
+}

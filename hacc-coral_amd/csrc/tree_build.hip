@@ -1,0 +1,921 @@
+// tree_build.hip -- level-synchronous, bit-exact RCB tree build for gfx950.
+//
+// Restates RCBForceTree<1>::createRCBForceTree and its helpers
+// (RCBForceTree.cxx:519-569, 623-918; BGQCM.c:181-212) as breadth-first
+// passes over SoA float32 arrays in HBM:
+//
+//   per level:  cm (tight bbox + exact double centre of mass)  -> classify
+//               (leaf / split dim / pivot)  -> partition of every splitting
+//               node (flag count, in-node scan, rank, swap-chain resolve,
+//               scatter to the other buffer)  -> children of the next level
+//   after:      pseudo-particles bottom-up, canonical (unthreaded allocation
+//               order) numbering top-down, one payload gather.
+//
+// Compiled with -ffp-contract=off: every float operation that shapes the
+// tree (pptdr, tdr, split-length compares) rounds exactly as the reference's
+// x86 build does.
+#include "hsr_internal.h"
+#include "softdouble.h"
+#include "wave.h"
+
+#include <algorithm>
+#include <cstdio>
+
+namespace {
+
+constexpr int CM_BLK = 4096;        // elements per block of a big node
+constexpr int64_t CM_SMALL = 16384; // nodes up to this size: one wave each
+constexpr int PB = 1024;            // partition block (256 threads x 4)
+constexpr int MAXHOP = 256;         // swap-chain hops before pointer jumping
+
+struct Counters {
+  int64_t cm_blocks;      // big-node cm blocks allocated this level
+  int64_t part_blocks;    // partition blocks allocated this level
+  int64_t nsplit;         // non-degenerate splits this level
+  int64_t active_next;    // particles in the next level's nodes
+  int64_t unresolved;     // swap chains longer than MAXHOP
+  int64_t pad[3];
+};
+
+// ------------------------------------------------------------ helpers
+__device__ __forceinline__ float pptdr_dev(const float *xmin, const float *xmax,
+                                           const float *xc) {
+  float a = xmax[0] - xc[0], b = xmax[1] - xc[1], c = xmax[2] - xc[2];
+  float d = xc[0] - xmin[0], e = xc[1] - xmin[1], f = xc[2] - xmin[2];
+  // std::min nesting of RCBForceTree.cxx:521-522; std::min(p,q) = q<p ? q : p
+  float r = (f < e) ? f : e;
+  r = (r < d) ? r : d;
+  r = (r < c) ? r : c;
+  r = (r < b) ? r : b;
+  r = (r < a) ? r : a;
+  return r;
+}
+
+__device__ __forceinline__ const float *coord(int d, const float *X,
+                                              const float *Y, const float *Z) {
+  return d == 0 ? X : (d == 1 ? Y : Z);
+}
+
+// ------------------------------------------------------------ cm (small)
+// One wave per node with count <= CM_SMALL (BGQCM.c:181-212).
+__global__ void __launch_bounds__(64)
+k_cm_small(BNode *nodes, int64_t lb, int64_t le, const float *X,
+           const float *Y, const float *Z, const float *M) {
+  int64_t t = lb + blockIdx.x;
+  if (t >= le) return;
+  int64_t cnt = nodes[t].count;
+  if (cnt > CM_SMALL || cnt == 0) return;
+  int64_t off = nodes[t].offset;
+  int l = lane_id();
+  sd_t S[4] = {sd_zero(), sd_zero(), sd_zero(), sd_zero()};
+  float mn[3] = {INFINITY, INFINITY, INFINITY};
+  float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t c0 = 0; c0 < cnt; c0 += 64) {
+    int64_t k = c0 + l;
+    int nvalid = (int)min<int64_t>(64, cnt - c0);
+    float v[4] = {0.f, 0.f, 0.f, 0.f};
+    if (k < cnt) {
+      float w = M[off + k];
+      float px = X[off + k], py = Y[off + k], pz = Z[off + k];
+      mn[0] = fminf(mn[0], px); mx[0] = fmaxf(mx[0], px);
+      mn[1] = fminf(mn[1], py); mx[1] = fmaxf(mx[1], py);
+      mn[2] = fminf(mn[2], pz); mx[2] = fmaxf(mx[2], pz);
+      v[0] = w * px;
+      v[1] = w * py;
+      v[2] = w * pz;
+      v[3] = w;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sd_chunk(S[q], v[q], nvalid);
+  }
+  for (int d = 0; d < 3; ++d) {
+    mn[d] = wave_min_f32(mn[d]);
+    mx[d] = wave_max_f32(mx[d]);
+  }
+  if (l == 0) {
+    for (int d = 0; d < 3; ++d) {
+      nodes[t].xmin[d] = mn[d];
+      nodes[t].xmax[d] = mx[d];
+      nodes[t].xc[d] = sd_div_to_f32(S[d], S[3]);
+    }
+  }
+}
+
+// ------------------------------------------------------------ cm (big)
+// Block table for nodes with count > CM_SMALL.
+__global__ void __launch_bounds__(64)
+k_cm_plan(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk,
+          Counters *ctr) {
+  int64_t t = lb + blockIdx.x;
+  if (t >= le) return;
+  int64_t cnt = nodes[t].count;
+  if (cnt <= CM_SMALL) {
+    if (lane_id() == 0) nodes[t].nblk = 0;
+    return;
+  }
+  int64_t nb = (cnt + CM_BLK - 1) / CM_BLK;
+  int64_t b0 = 0;
+  if (lane_id() == 0) {
+    b0 = (int64_t)atomicAdd((unsigned long long *)&ctr->cm_blocks,
+                            (unsigned long long)nb);
+    nodes[t].blk0 = b0;
+    nodes[t].nblk = nb;
+  }
+  b0 = (int64_t)__shfl((long long)b0, 0);
+  int64_t off = nodes[t].offset;
+  for (int64_t b = lane_id(); b < nb; b += 64) {
+    CmBlock &B = blk[b0 + b];
+    B.node = t;
+    B.start = off + b * CM_BLK;
+    B.len = min<int64_t>(CM_BLK, cnt - b * CM_BLK);
+  }
+}
+
+// Approximate sums and bbox per block (one wave per block).
+__global__ void __launch_bounds__(64)
+k_cm_blk_stats(CmBlock *blk, const Counters *ctr, const float *X,
+               const float *Y, const float *Z, const float *M) {
+  int64_t b = blockIdx.x;
+  if (b >= ctr->cm_blocks) return;
+  CmBlock &B = blk[b];
+  int l = lane_id();
+  float s[4] = {0.f, 0.f, 0.f, 0.f};
+  float mn[3] = {INFINITY, INFINITY, INFINITY};
+  float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t k = l; k < B.len; k += 64) {
+    int64_t e = B.start + k;
+    float w = M[e], px = X[e], py = Y[e], pz = Z[e];
+    mn[0] = fminf(mn[0], px); mx[0] = fmaxf(mx[0], px);
+    mn[1] = fminf(mn[1], py); mx[1] = fmaxf(mx[1], py);
+    mn[2] = fminf(mn[2], pz); mx[2] = fmaxf(mx[2], pz);
+    s[0] += w * px;
+    s[1] += w * py;
+    s[2] += w * pz;
+    s[3] += w;
+  }
+  for (int q = 0; q < 4; ++q) s[q] = wave_sum_f32(s[q]);
+  for (int d = 0; d < 3; ++d) {
+    mn[d] = wave_min_f32(mn[d]);
+    mx[d] = wave_max_f32(mx[d]);
+  }
+  if (l == 0) {
+    for (int q = 0; q < 4; ++q) B.approx[q] = s[q];
+    for (int d = 0; d < 3; ++d) {
+      B.bmin[d] = mn[d];
+      B.bmax[d] = mx[d];
+    }
+  }
+}
+
+// Per big node: exclusive prefix of the approximate sums (binade guesses)
+// and the node bbox.
+__global__ void k_cm_blk_prefix(BNode *nodes, int64_t lb, int64_t le,
+                                CmBlock *blk) {
+  int64_t t = lb + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= le) return;
+  if (nodes[t].count <= CM_SMALL) return;
+  int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
+  float run[4] = {0.f, 0.f, 0.f, 0.f};
+  float mn[3] = {INFINITY, INFINITY, INFINITY};
+  float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t b = b0; b < b0 + nb; ++b) {
+    for (int q = 0; q < 4; ++q) {
+      blk[b].ein[q] = run[q];
+      run[q] += blk[b].approx[q];
+    }
+    for (int d = 0; d < 3; ++d) {
+      mn[d] = fminf(mn[d], blk[b].bmin[d]);
+      mx[d] = fmaxf(mx[d], blk[b].bmax[d]);
+    }
+  }
+  for (int d = 0; d < 3; ++d) {
+    nodes[t].xmin[d] = mn[d];
+    nodes[t].xmax[d] = mx[d];
+  }
+}
+
+__device__ __forceinline__ int32_t grid_exp_guess(float ein) {
+  uint32_t u = sd_f2u(ein);
+  int32_t ef = (int32_t)((u >> 23) & 255u);
+  if (ein <= 0.f || ef == 0 || ef == 255) return INT32_MIN;
+  return (ef - 127) - 52;
+}
+
+// Fixed-grid maps of each block for its guessed binade.
+__global__ void __launch_bounds__(64)
+k_cm_blk_maps(CmBlock *blk, const Counters *ctr, const float *X,
+              const float *Y, const float *Z, const float *M) {
+  int64_t b = blockIdx.x;
+  if (b >= ctr->cm_blocks) return;
+  CmBlock &B = blk[b];
+  int l = lane_id();
+  int32_t E[4];
+  sd_map acc[4];
+  for (int q = 0; q < 4; ++q) {
+    E[q] = grid_exp_guess(B.ein[q]);
+    acc[q] = sd_map_identity();
+    if (E[q] == INT32_MIN) acc[q].bad = 1;
+  }
+  for (int64_t c0 = 0; c0 < B.len; c0 += 64) {
+    int64_t k = c0 + l;
+    float v[4] = {0.f, 0.f, 0.f, 0.f};
+    if (k < B.len) {
+      int64_t e = B.start + k;
+      float w = M[e];
+      v[0] = w * X[e];
+      v[1] = w * Y[e];
+      v[2] = w * Z[e];
+      v[3] = w;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (acc[q].bad) continue;  // uniform
+      sd_map m = wave_compose_maps(sd_map_of(v[q], E[q]));
+      acc[q] = sd_map_compose(acc[q], m);
+    }
+  }
+  if (l == 0) {
+    for (int q = 0; q < 4; ++q) {
+      B.d0[q] = acc[q].d0;
+      B.d1[q] = acc[q].d1;
+      B.bad[q] = acc[q].bad;
+      B.E[q] = E[q];
+    }
+  }
+}
+
+// Chain the block maps of each big node in order; blocks whose map does not
+// apply (binade guess wrong, binade crossed) are re-accumulated chunk by
+// chunk from the particles.
+__global__ void __launch_bounds__(64)
+k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, const CmBlock *blk,
+               const float *X, const float *Y, const float *Z,
+               const float *M) {
+  int64_t t = lb + blockIdx.x;
+  if (t >= le) return;
+  if (nodes[t].count <= CM_SMALL) return;
+  int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
+  int l = lane_id();
+  sd_t S[4] = {sd_zero(), sd_zero(), sd_zero(), sd_zero()};
+  for (int64_t b = b0; b < b0 + nb; ++b) {
+    const CmBlock &B = blk[b];
+    for (int q = 0; q < 4; ++q) {
+      sd_map g;
+      g.d0 = B.d0[q];
+      g.d1 = B.d1[q];
+      g.bad = B.bad[q];
+      if (sd_map_apply(&S[q], g, B.E[q])) continue;  // uniform decision
+      for (int64_t c0 = 0; c0 < B.len; c0 += 64) {
+        int64_t k = c0 + l;
+        int nvalid = (int)min<int64_t>(64, B.len - c0);
+        float v = 0.f;
+        if (k < B.len) {
+          int64_t e = B.start + k;
+          float w = M[e];
+          v = q == 3 ? w : w * (q == 0 ? X[e] : (q == 1 ? Y[e] : Z[e]));
+        }
+        sd_chunk(S[q], v, nvalid);
+      }
+    }
+  }
+  if (l == 0) {
+    for (int d = 0; d < 3; ++d) nodes[t].xc[d] = sd_div_to_f32(S[d], S[3]);
+  }
+}
+
+// ------------------------------------------------------------ classify
+// Leaves (count <= nd) get tdr and, for count > 1, ppm = sequential float
+// sum of member masses (pp with TDPTS = 1: mass*(1+0+0)); other nodes get
+// the split dimension (longest edge, ties x > y > z as coded at
+// RCBForceTree.cxx:843-852).
+__global__ void k_classify(BNode *nodes, int64_t lb, int64_t le, int64_t nd,
+                           float ppc, const float *M) {
+  int64_t t = lb + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= le) return;
+  BNode &n = nodes[t];
+  n.cl = n.cr = -1;
+  if (n.count <= nd) {
+    n.kind = NODE_LEAF;
+    if (n.count == 0) {
+      n.xc[0] = n.xc[1] = n.xc[2] = __int_as_float(0x7fc00000);
+    }
+    n.tdr = ppc * pptdr_dev(n.xmin, n.xmax, n.xc);
+    float ppm = 0.f;
+    if (n.count > 1) {
+      for (int64_t i = 0; i < n.count; ++i) ppm += M[n.offset + i] * 1.0f;
+    }
+    n.ppm = ppm;
+    return;
+  }
+  n.kind = NODE_SPLIT;
+  float xl0 = n.xmax[0] - n.xmin[0];
+  float xl1 = n.xmax[1] - n.xmin[1];
+  float xl2 = n.xmax[2] - n.xmin[2];
+  int d;
+  if (xl0 > xl1 && xl0 > xl2) d = 0;
+  else if (xl1 > xl2) d = 1;
+  else d = 2;
+  n.split_d = d;
+}
+
+// Copy the final permutation of nodes that stop here (leaves at classify
+// time read buffer `P`; degenerate splits read the partition's output).
+__global__ void __launch_bounds__(64)
+k_finalize_leaves(const BNode *nodes, int64_t lb, int64_t le, int kind,
+                  const int32_t *P, int32_t *Pfin) {
+  int64_t t = lb + blockIdx.x;
+  if (t >= le) return;
+  if (nodes[t].kind != kind) return;
+  int64_t off = nodes[t].offset, cnt = nodes[t].count;
+  for (int64_t k = lane_id(); k < cnt; k += 64) Pfin[off + k] = P[off + k];
+}
+
+// ------------------------------------------------------------ partition
+__global__ void __launch_bounds__(64)
+k_part_plan(BNode *nodes, int64_t lb, int64_t le, PartBlock *pb,
+            Counters *ctr) {
+  int64_t t = lb + blockIdx.x;
+  if (t >= le) return;
+  if (nodes[t].kind != NODE_SPLIT) return;
+  int64_t cnt = nodes[t].count;
+  int64_t nb = (cnt + PB - 1) / PB;
+  int64_t b0 = 0;
+  if (lane_id() == 0) {
+    b0 = (int64_t)atomicAdd((unsigned long long *)&ctr->part_blocks,
+                            (unsigned long long)nb);
+    nodes[t].blk0 = b0;
+    nodes[t].nblk = nb;
+  }
+  b0 = (int64_t)__shfl((long long)b0, 0);
+  int64_t off = nodes[t].offset;
+  for (int64_t b = lane_id(); b < nb; b += 64) {
+    PartBlock &B = pb[b0 + b];
+    B.node = t;
+    B.start = off + b * PB;
+    B.len = min<int64_t>(PB, cnt - b * PB);
+  }
+}
+
+// Keys < pivot per block (RCBForceTree.cxx:638-645), pivot = xc[d].
+__global__ void __launch_bounds__(256)
+k_part_count(const BNode *nodes, PartBlock *pb, const Counters *ctr,
+             const float *X, const float *Y, const float *Z) {
+  int64_t b = blockIdx.x;
+  if (b >= ctr->part_blocks) return;
+  const PartBlock &B = pb[b];
+  const BNode &n = nodes[B.node];
+  const float *key = coord(n.split_d, X, Y, Z);
+  float pv = n.xc[n.split_d];
+  int c = 0;
+  for (int k = threadIdx.x; k < B.len; k += 256) c += key[B.start + k] < pv;
+  __shared__ int red[4];
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if (lane_id() == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) pb[b].cnt = red[0] + red[1] + red[2] + red[3];
+}
+
+// In-order exclusive scan of block counts per node; degenerate splits
+// (is == 0 or is == count, RCBForceTree.cxx:727-729) are marked.
+__global__ void __launch_bounds__(64)
+k_part_scan(BNode *nodes, int64_t lb, int64_t le, PartBlock *pb) {
+  int64_t t = lb + blockIdx.x;
+  if (t >= le) return;
+  if (nodes[t].kind != NODE_SPLIT) return;
+  int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
+  int64_t run = 0;
+  int l = lane_id();
+  for (int64_t c0 = 0; c0 < nb; c0 += 64) {
+    int64_t b = c0 + l;
+    int64_t v = b < nb ? pb[b0 + b].cnt : 0;
+    // inclusive scan
+    int64_t s = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      int64_t u = (int64_t)__shfl_up((long long)s, o);
+      if (l >= o) s += u;
+    }
+    if (b < nb) pb[b0 + b].base = run + s - v;
+    run += (int64_t)__shfl((long long)s, 63);
+  }
+  if (l == 0) {
+    nodes[t].is = run;
+    if (run == 0 || run == nodes[t].count) nodes[t].kind = NODE_DEGEN;
+  }
+}
+
+// Ranks: a key < pivot at local position p moves to its rank r and is
+// recorded as idx[r] = p; others at p >= is stay; others below is are
+// resolved by the swap chain (dest = -1 here).
+__global__ void __launch_bounds__(256)
+k_part_rank(const BNode *nodes, const PartBlock *pb, const Counters *ctr,
+            const float *X, const float *Y, const float *Z, int32_t *idx,
+            int32_t *dest) {
+  int64_t b = blockIdx.x;
+  if (b >= ctr->part_blocks) return;
+  const PartBlock &B = pb[b];
+  const BNode &n = nodes[B.node];
+  const float *key = coord(n.split_d, X, Y, Z);
+  float pv = n.xc[n.split_d];
+  int64_t off = n.offset, is = n.is;
+  __shared__ int wsum[4];
+  int64_t run = B.base;
+  int w = threadIdx.x >> 6;
+  for (int r = 0; r < PB / 256; ++r) {
+    int k = r * 256 + threadIdx.x;
+    bool valid = k < B.len;
+    int64_t e = B.start + k;
+    bool f = valid && key[e] < pv;
+    uint64_t bal = __ballot(f);
+    int pre = __popcll(bal & lanemask_lt());
+    if (lane_id() == 0) wsum[w] = __popcll(bal);
+    __syncthreads();
+    int wbase = 0, tot = 0;
+    for (int i = 0; i < 4; ++i) {
+      if (i < w) wbase += wsum[i];
+      tot += wsum[i];
+    }
+    if (valid) {
+      int64_t p = e - off;
+      if (f) {
+        int64_t rank = run + wbase + pre;
+        idx[off + rank] = (int32_t)p;
+        dest[e] = (int32_t)rank;
+      } else {
+        dest[e] = p >= is ? (int32_t)p : -1;
+      }
+    }
+    run += tot;
+    __syncthreads();
+  }
+}
+
+// Swap chains (RCBForceTree.cxx:647-669): an element with key >= pivot at
+// p < is ends at the first idx-iterate p -> idx[p] -> ... that is >= is.
+__global__ void __launch_bounds__(256)
+k_part_chain(const BNode *nodes, const PartBlock *pb, const Counters *ctrc,
+             const int32_t *idx, int32_t *dest, Counters *ctr) {
+  int64_t b = blockIdx.x;
+  if (b >= ctrc->part_blocks) return;
+  const PartBlock &B = pb[b];
+  const BNode &n = nodes[B.node];
+  int64_t off = n.offset, is = n.is;
+  for (int k = threadIdx.x; k < B.len; k += 256) {
+    int64_t e = B.start + k;
+    if (dest[e] != -1) continue;
+    int64_t j = e - off;
+    int hop = 0;
+    while (j < is && hop < MAXHOP) {
+      j = idx[off + j];
+      ++hop;
+    }
+    if (j >= is) {
+      dest[e] = (int32_t)j;
+    } else {
+      dest[e] = -2 - (int32_t)j;  // unresolved; continue from j
+      atomicAdd((unsigned long long *)&ctr->unresolved, 1ull);
+    }
+  }
+}
+
+// Pointer-jumping fallback for long chains: jmp[off+j] = idx-iterate of j
+// advanced until >= is; log2 rounds resolve every chain.
+__global__ void __launch_bounds__(256)
+k_part_jump_init(const BNode *nodes, const PartBlock *pb, const Counters *ctr,
+                 const int32_t *idx, int32_t *jmp) {
+  int64_t b = blockIdx.x;
+  if (b >= ctr->part_blocks) return;
+  const PartBlock &B = pb[b];
+  const BNode &n = nodes[B.node];
+  for (int k = threadIdx.x; k < B.len; k += 256) {
+    int64_t p = B.start + k - n.offset;
+    if (p < n.is) jmp[n.offset + p] = idx[n.offset + p];
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_part_jump_round(const BNode *nodes, const PartBlock *pb,
+                  const Counters *ctr, int32_t *jmp) {
+  int64_t b = blockIdx.x;
+  if (b >= ctr->part_blocks) return;
+  const PartBlock &B = pb[b];
+  const BNode &n = nodes[B.node];
+  for (int k = threadIdx.x; k < B.len; k += 256) {
+    int64_t p = B.start + k - n.offset;
+    if (p < n.is) {
+      int32_t j = jmp[n.offset + p];
+      if (j < n.is) jmp[n.offset + p] = jmp[n.offset + j];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_part_jump_final(const BNode *nodes, const PartBlock *pb,
+                  const Counters *ctr, const int32_t *jmp, int32_t *dest) {
+  int64_t b = blockIdx.x;
+  if (b >= ctr->part_blocks) return;
+  const PartBlock &B = pb[b];
+  const BNode &n = nodes[B.node];
+  for (int k = threadIdx.x; k < B.len; k += 256) {
+    int64_t e = B.start + k;
+    int32_t d = dest[e];
+    if (d <= -2) dest[e] = jmp[n.offset + (-2 - d)];
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_part_scatter(const BNode *nodes, const PartBlock *pb, const Counters *ctr,
+               const int32_t *dest, const float *X, const float *Y,
+               const float *Z, const float *M, const int32_t *P, float *X2,
+               float *Y2, float *Z2, float *M2, int32_t *P2) {
+  int64_t b = blockIdx.x;
+  if (b >= ctr->part_blocks) return;
+  const PartBlock &B = pb[b];
+  int64_t off = nodes[B.node].offset;
+  for (int k = threadIdx.x; k < B.len; k += 256) {
+    int64_t e = B.start + k;
+    int64_t o = off + dest[e];
+    X2[o] = X[e];
+    Y2[o] = Y[e];
+    Z2[o] = Z[e];
+    M2[o] = M[e];
+    P2[o] = P[e];
+  }
+}
+
+// Children of non-degenerate splits form the next level
+// (RCBForceTree.cxx:731-770: left = [offset, offset+is), right = rest).
+__global__ void k_make_children(BNode *nodes, int64_t lb, int64_t le,
+                                int64_t next_base, Counters *ctr) {
+  int64_t t = lb + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= le) return;
+  BNode &n = nodes[t];
+  if (n.kind != NODE_SPLIT) return;
+  int64_t k = (int64_t)atomicAdd((unsigned long long *)&ctr->nsplit, 1ull);
+  atomicAdd((unsigned long long *)&ctr->active_next,
+            (unsigned long long)n.count);
+  int64_t cl = next_base + 2 * k, cr = cl + 1;
+  n.cl = (int32_t)cl;
+  n.cr = (int32_t)cr;
+  BNode &L = nodes[cl];
+  BNode &R = nodes[cr];
+  L.count = n.is;
+  L.offset = n.offset;
+  R.count = n.count - n.is;
+  R.offset = n.offset + n.is;
+  L.parent = R.parent = (int32_t)t;
+  L.depth = R.depth = n.depth + 1;
+  L.cl = L.cr = R.cl = R.cr = -1;
+  L.nblk = R.nblk = 0;
+}
+
+// ------------------------------------------------------------ finishing
+// Bottom-up: parent pseudo-particles from the children, left then right
+// (RCBForceTree.cxx:856-889 with TDPTS = 1), allocating-node and leaf counts.
+__global__ void k_bottom_up(BNode *nodes, int64_t lb, int64_t le, float ppc,
+                            const int32_t *Pfin, const float *Min) {
+  int64_t t = lb + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= le) return;
+  BNode &n = nodes[t];
+  if (n.kind == NODE_LEAF) {
+    n.sa = 0;
+    n.nleaf = 1;
+    return;
+  }
+  n.tdr = ppc * pptdr_dev(n.xmin, n.xmax, n.xc);
+  if (n.kind == NODE_DEGEN) {
+    // both reserved children stay empty: no pseudo-particle mass
+    n.ppm = 0.f;
+    n.sa = 1;
+    n.nleaf = 1;
+    return;
+  }
+  const BNode &L = nodes[n.cl];
+  const BNode &R = nodes[n.cr];
+  float ppm = 0.f;
+  const BNode *ch[2] = {&L, &R};
+  for (int k = 0; k < 2; ++k) {
+    const BNode &c = *ch[k];
+    if (c.count > 0) {
+      float cm = (c.count <= 1) ? Min[Pfin[c.offset]] : c.ppm;
+      ppm += cm * 1.0f;
+    }
+  }
+  n.ppm = ppm;
+  n.sa = 1 + L.sa + R.sa;
+  n.nleaf = L.nleaf + R.nleaf;
+}
+
+// Top-down: canonical ids (children get 1 + 2A and 2 + 2A, A = allocating
+// nodes before the parent in preorder) and the right-first leaf rank.
+__global__ void k_top_down(BNode *nodes, int64_t lb, int64_t le) {
+  int64_t t = lb + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= le) return;
+  BNode &n = nodes[t];
+  if (n.kind != NODE_SPLIT) return;
+  BNode &L = nodes[n.cl];
+  BNode &R = nodes[n.cr];
+  L.a = n.a + 1;
+  R.a = n.a + 1 + L.sa;
+  L.canon = 1 + 2 * n.a;
+  R.canon = 2 + 2 * n.a;
+  R.leaf_rank = n.leaf_rank;
+  L.leaf_rank = n.leaf_rank + R.nleaf;
+}
+
+__global__ void k_write_canon(const BNode *nodes, int64_t nb,
+                              const int64_t *box_cbase, const int32_t *box_of,
+                              hsr_node *out) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nb) return;
+  const BNode &n = nodes[t];
+  int64_t base = box_cbase[box_of[t]];
+  hsr_node o;
+  o.count = n.count;
+  o.offset = n.offset;
+  o.cl = (n.kind == NODE_SPLIT) ? base + nodes[n.cl].canon : 0;
+  o.cr = (n.kind == NODE_SPLIT) ? base + nodes[n.cr].canon : 0;
+  o.ppm[0] = n.ppm;
+  o.tdr = n.tdr;
+  for (int d = 0; d < 3; ++d) {
+    o.xmin[d] = n.xmin[d];
+    o.xmax[d] = n.xmax[d];
+    o.xc[d] = n.xc[d];
+  }
+  out[base + n.canon] = o;
+  if (n.kind == NODE_DEGEN) {
+    // the two reserved child slots (RCBForceTree.cxx:825-835): zeroed, with
+    // the parent's bbox copied in
+    hsr_node e;
+    e.count = 0;
+    e.offset = 0;
+    e.cl = e.cr = 0;
+    e.ppm[0] = 0.f;
+    e.tdr = 0.f;
+    for (int d = 0; d < 3; ++d) {
+      e.xmin[d] = n.xmin[d];
+      e.xmax[d] = n.xmax[d];
+      e.xc[d] = 0.f;
+    }
+    out[base + 1 + 2 * n.a] = e;
+    out[base + 2 + 2 * n.a] = e;
+  }
+}
+
+__global__ void k_box_of(const BNode *nodes, int64_t nb, int32_t *box_of,
+                         const int32_t *root_box) {
+  // root_box[t] = box of root t or -1; walk parents
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nb) return;
+  int64_t u = t;
+  while (nodes[u].parent >= 0) u = nodes[u].parent;
+  box_of[t] = root_box[u];
+}
+
+__global__ void k_gather_sorted(int64_t n, const int32_t *Pfin,
+                                const float *x, const float *y, const float *z,
+                                const float *m, float *xs, float *ys,
+                                float *zs, float *ms, PJ *pj) {
+  int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int32_t s = Pfin[k];
+  float a = x[s], b = y[s], c = z[s], d = m[s];
+  xs[k] = a;
+  ys[k] = b;
+  zs[k] = c;
+  ms[k] = d;
+  pj[k] = PJ{a, b, c, d};
+}
+
+// Pseudo-particle records (pppts with the TDPTS = 1 design point (0,0,0):
+// position tdr*0 + xc, mass ppm), RCBForceTree.cxx:525-534, 1057-1060.
+__global__ void k_pseudo_pj(int64_t n, int64_t nc, const hsr_node *cn,
+                            PJ *pj) {
+  int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= nc) return;
+  const hsr_node &o = cn[k];
+  PJ r;
+  r.x = o.tdr * 0.0f + o.xc[0];
+  r.y = o.tdr * 0.0f + o.xc[1];
+  r.z = o.tdr * 0.0f + o.xc[2];
+  r.m = o.ppm[0];
+  pj[n + k] = r;
+}
+
+__global__ void k_init_roots(BNode *nodes, int64_t nboxes,
+                             const int64_t *box_off, const float *min_loc,
+                             const float *max_loc, int32_t *P, int64_t n) {
+  int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (b < nboxes) {
+    BNode &r = nodes[b];
+    r.count = box_off[b + 1] - box_off[b];
+    r.offset = box_off[b];
+    r.parent = -1;
+    r.cl = r.cr = -1;
+    r.depth = 0;
+    r.kind = NODE_LEAF;
+    r.nblk = 0;
+    r.canon = 0;
+    r.a = 0;
+    r.leaf_rank = 0;
+    r.ppm = 0.f;
+    r.tdr = 0.f;
+    for (int d = 0; d < 3; ++d) {
+      r.xmin[d] = min_loc[3 * b + d];
+      r.xmax[d] = max_loc[3 * b + d];
+      r.xc[d] = 0.f;
+    }
+  }
+}
+
+__global__ void k_zero_counters(Counters *ctr) {
+  int64_t *p = reinterpret_cast<int64_t *>(ctr);
+  if (threadIdx.x < sizeof(Counters) / sizeof(int64_t)) p[threadIdx.x] = 0;
+}
+
+__global__ void k_iota(int64_t n, int32_t *P) {
+  int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < n) P[k] = (int32_t)k;
+}
+
+inline unsigned nblk_for(int64_t n, int t) {
+  return (unsigned)((n + t - 1) / t);
+}
+
+}  // namespace
+
+hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
+                             const int64_t *box_off_h, const float *min_loc_h,
+                             const float *max_loc_h, const float *x,
+                             const float *y, const float *z, const float *m) {
+  hipStream_t st = c->stream;
+  TreeState &T = c->tree;
+  Scratch &S = c->s;
+  int64_t n = box_off_h[nboxes];
+  T.n = n;
+  T.nboxes = nboxes;
+  const int64_t nd = c->p.nd;
+
+  // node table capacity: grows between levels when needed
+  int64_t cap = std::max<int64_t>(
+      64 + 2 * nboxes, (((int64_t)((double)n / (double)nd)) << std::max<int64_t>(c->p.ds, 1)) + 1 + nboxes);
+  HSR_HIP(dbuf_reserve(T.bnodes, sizeof(BNode) * cap));
+  size_t nn = (size_t)std::max<int64_t>(n, 1);
+  for (DBuf *b : {&S.x0, &S.y0, &S.z0, &S.m0, &S.x1, &S.y1, &S.z1, &S.m1})
+    HSR_HIP(dbuf_reserve(*b, sizeof(float) * nn));
+  HSR_HIP(dbuf_reserve(S.p0, sizeof(int32_t) * nn));
+  HSR_HIP(dbuf_reserve(S.p1, sizeof(int32_t) * nn));
+  HSR_HIP(dbuf_reserve(T.pfin, sizeof(int32_t) * nn));
+  HSR_HIP(dbuf_reserve(S.a, sizeof(int32_t) * nn));  // idx
+  HSR_HIP(dbuf_reserve(S.b, sizeof(int32_t) * nn));  // dest
+  HSR_HIP(dbuf_reserve(S.c, sizeof(int32_t) * nn));  // jmp
+  HSR_HIP(dbuf_reserve(S.counters, sizeof(Counters) + 64));
+  HSR_HIP(dbuf_reserve(S.d, sizeof(int64_t) * (nboxes + 1) + sizeof(float) * 6 * nboxes + 64));
+
+  float *X[2] = {dptr<float>(S.x0), dptr<float>(S.x1)};
+  float *Y[2] = {dptr<float>(S.y0), dptr<float>(S.y1)};
+  float *Z[2] = {dptr<float>(S.z0), dptr<float>(S.z1)};
+  float *Mm[2] = {dptr<float>(S.m0), dptr<float>(S.m1)};
+  int32_t *P[2] = {dptr<int32_t>(S.p0), dptr<int32_t>(S.p1)};
+  int32_t *Pfin = dptr<int32_t>(T.pfin);
+  int32_t *idx = dptr<int32_t>(S.a);
+  int32_t *dest = dptr<int32_t>(S.b);
+  int32_t *jmp = dptr<int32_t>(S.c);
+  Counters *ctr = dptr<Counters>(S.counters);
+
+  if (n > 0) {
+    HSR_HIP(hipMemcpyAsync(X[0], x, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+    HSR_HIP(hipMemcpyAsync(Y[0], y, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+    HSR_HIP(hipMemcpyAsync(Z[0], z, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+    HSR_HIP(hipMemcpyAsync(Mm[0], m, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+    k_iota<<<nblk_for(n, 256), 256, 0, st>>>(n, P[0]);
+  }
+  // box table
+  int64_t *d_box_off = dptr<int64_t>(S.d);
+  float *d_min = reinterpret_cast<float *>(d_box_off + nboxes + 1);
+  float *d_max = d_min + 3 * nboxes;
+  HSR_HIP(hipMemcpyAsync(d_box_off, box_off_h, sizeof(int64_t) * (nboxes + 1), hipMemcpyHostToDevice, st));
+  HSR_HIP(hipMemcpyAsync(d_min, min_loc_h, sizeof(float) * 3 * nboxes, hipMemcpyHostToDevice, st));
+  HSR_HIP(hipMemcpyAsync(d_max, max_loc_h, sizeof(float) * 3 * nboxes, hipMemcpyHostToDevice, st));
+  BNode *nodes = dptr<BNode>(T.bnodes);
+  k_init_roots<<<nblk_for(nboxes, 64), 64, 0, st>>>(nodes, nboxes, d_box_off, d_min, d_max, P[0], n);
+
+  T.level_start.assign(1, 0);
+  T.level_start.push_back(nboxes);
+  int cur = 0;
+  int64_t active = n;
+  Counters hc;
+  int depth = 0;
+  for (;;) {
+    int64_t lb = T.level_start[depth], le = T.level_start[depth + 1];
+    int64_t nl = le - lb;
+    if (nl == 0) break;
+    // capacity for the children of this level (at most 2 per node)
+    if ((le + 2 * nl) > cap) {
+      int64_t ncap = std::max<int64_t>(cap * 2, le + 2 * nl + 64);
+      DBuf nb;
+      HSR_HIP(dbuf_reserve(nb, sizeof(BNode) * ncap));
+      HSR_HIP(hipMemcpyAsync(nb.p, T.bnodes.p, sizeof(BNode) * le, hipMemcpyDeviceToDevice, st));
+      HSR_HIP(hipStreamSynchronize(st));
+      dbuf_free(T.bnodes);
+      T.bnodes = nb;
+      cap = ncap;
+      nodes = dptr<BNode>(T.bnodes);
+    }
+    k_zero_counters<<<1, 64, 0, st>>>(ctr);
+    // ---- centre of mass
+    int64_t cm_cap = active / CM_BLK + active / CM_SMALL + 2;
+    HSR_HIP(dbuf_reserve(S.blocks, sizeof(CmBlock) * cm_cap));
+    CmBlock *cmb = dptr<CmBlock>(S.blocks);
+    k_cm_small<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, X[cur], Y[cur], Z[cur], Mm[cur]);
+    k_cm_plan<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, cmb, ctr);
+    unsigned cmg = (unsigned)std::max<int64_t>(1, active / CM_BLK + active / CM_SMALL + 1);
+    k_cm_blk_stats<<<cmg, 64, 0, st>>>(cmb, ctr, X[cur], Y[cur], Z[cur], Mm[cur]);
+    k_cm_blk_prefix<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, cmb);
+    k_cm_blk_maps<<<cmg, 64, 0, st>>>(cmb, ctr, X[cur], Y[cur], Z[cur], Mm[cur]);
+    k_cm_blk_chain<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, cmb, X[cur], Y[cur], Z[cur], Mm[cur]);
+    // ---- classify, finalize plain leaves
+    k_classify<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, nd, c->p.ppc, Mm[cur]);
+    k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_LEAF, P[cur], Pfin);
+    // ---- partition of splitting nodes
+    int64_t pcap = active / PB + nl + 2;
+    HSR_HIP(dbuf_reserve(S.pblocks, sizeof(PartBlock) * pcap));
+    PartBlock *pb = dptr<PartBlock>(S.pblocks);
+    k_part_plan<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, pb, ctr);
+    unsigned pg = (unsigned)std::max<int64_t>(1, pcap);
+    k_part_count<<<pg, 256, 0, st>>>(nodes, pb, ctr, X[cur], Y[cur], Z[cur]);
+    k_part_scan<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, pb);
+    k_part_rank<<<pg, 256, 0, st>>>(nodes, pb, ctr, X[cur], Y[cur], Z[cur], idx, dest);
+    k_part_chain<<<pg, 256, 0, st>>>(nodes, pb, ctr, idx, dest, ctr);
+    HSR_HIP(hipMemcpyAsync(&hc, ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
+    HSR_HIP(hipStreamSynchronize(st));
+    if (hc.unresolved > 0) {
+      k_part_jump_init<<<pg, 256, 0, st>>>(nodes, pb, ctr, idx, jmp);
+      for (int r = 0; r < 32; ++r)
+        k_part_jump_round<<<pg, 256, 0, st>>>(nodes, pb, ctr, jmp);
+      k_part_jump_final<<<pg, 256, 0, st>>>(nodes, pb, ctr, jmp, dest);
+    }
+    k_part_scatter<<<pg, 256, 0, st>>>(nodes, pb, ctr, dest, X[cur], Y[cur], Z[cur], Mm[cur], P[cur],
+                                        X[cur ^ 1], Y[cur ^ 1], Z[cur ^ 1], Mm[cur ^ 1], P[cur ^ 1]);
+    k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_DEGEN, P[cur ^ 1], Pfin);
+    k_make_children<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, le, ctr);
+    HSR_HIP(hipMemcpyAsync(&hc, ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
+    HSR_HIP(hipStreamSynchronize(st));
+    HSR_HIP(hipGetLastError());
+    T.level_start.push_back(le + 2 * hc.nsplit);
+    active = hc.active_next;
+    cur ^= 1;
+    ++depth;
+  }
+  int nlev = (int)T.level_start.size() - 1;  // last level is empty
+  T.nnodes_bfs = T.level_start.back();
+  T.max_depth = std::max(0, nlev - 2);  // deepest non-empty level index
+  int64_t nb = T.nnodes_bfs;
+
+  // ---- bottom-up, top-down
+  const float *Min = m;
+  for (int l = nlev - 1; l >= 0; --l) {
+    int64_t lb = T.level_start[l], le = T.level_start[l + 1];
+    if (le > lb)
+      k_bottom_up<<<nblk_for(le - lb, 64), 64, 0, st>>>(nodes, lb, le, c->p.ppc, Pfin, Min);
+  }
+  for (int l = 0; l < nlev; ++l) {
+    int64_t lb = T.level_start[l], le = T.level_start[l + 1];
+    if (le > lb) k_top_down<<<nblk_for(le - lb, 64), 64, 0, st>>>(nodes, lb, le);
+  }
+  // per-box canonical bases: box b holds 1 + 2 * sa(root_b) nodes
+  std::vector<BNode> roots(nboxes);
+  HSR_HIP(hipMemcpyAsync(roots.data(), nodes, sizeof(BNode) * nboxes, hipMemcpyDeviceToHost, st));
+  HSR_HIP(hipStreamSynchronize(st));
+  std::vector<int64_t> cbase(nboxes + 1, 0);
+  for (int64_t b = 0; b < nboxes; ++b) cbase[b + 1] = cbase[b] + 1 + 2 * roots[b].sa;
+  T.nnodes = cbase[nboxes];
+  T.box_root.assign(cbase.begin(), cbase.end());
+  HSR_HIP(dbuf_reserve(S.e, sizeof(int64_t) * (nboxes + 1) + sizeof(int32_t) * (nb + nboxes) + 64));
+  int64_t *d_cbase = dptr<int64_t>(S.e);
+  int32_t *root_box = reinterpret_cast<int32_t *>(d_cbase + nboxes + 1);
+  HSR_HIP(hipMemcpyAsync(d_cbase, cbase.data(), sizeof(int64_t) * (nboxes + 1), hipMemcpyHostToDevice, st));
+  {
+    std::vector<int32_t> rb(nboxes);
+    for (int64_t b = 0; b < nboxes; ++b) rb[b] = (int32_t)b;
+    HSR_HIP(hipMemcpyAsync(root_box, rb.data(), sizeof(int32_t) * nboxes, hipMemcpyHostToDevice, st));
+    HSR_HIP(hipStreamSynchronize(st));
+  }
+  HSR_HIP(dbuf_reserve(S.f, sizeof(int32_t) * (nb + 1)));
+  int32_t *box_of = dptr<int32_t>(S.f);
+  k_box_of<<<nblk_for(nb, 256), 256, 0, st>>>(nodes, nb, box_of, root_box);
+  HSR_HIP(dbuf_reserve(T.canon_nodes, sizeof(hsr_node) * std::max<int64_t>(T.nnodes, 1)));
+  hsr_node *cn = dptr<hsr_node>(T.canon_nodes);
+  k_write_canon<<<nblk_for(nb, 256), 256, 0, st>>>(nodes, nb, d_cbase, box_of, cn);
+
+  // ---- sorted arrays and j-particle records
+  for (DBuf *bb : {&T.xs, &T.ys, &T.zs, &T.ms})
+    HSR_HIP(dbuf_reserve(*bb, sizeof(float) * nn));
+  HSR_HIP(dbuf_reserve(T.pj, sizeof(PJ) * (n + T.nnodes + 1)));
+  if (n > 0)
+    k_gather_sorted<<<nblk_for(n, 256), 256, 0, st>>>(n, Pfin, x, y, z, m, dptr<float>(T.xs), dptr<float>(T.ys),
+                                                     dptr<float>(T.zs), dptr<float>(T.ms), dptr<PJ>(T.pj));
+  if (T.nnodes > 0)
+    k_pseudo_pj<<<nblk_for(T.nnodes, 256), 256, 0, st>>>(n, T.nnodes, cn, dptr<PJ>(T.pj));
+  HSR_HIP(hipGetLastError());
+  return HSR_OK;
+}

@@ -1,0 +1,98 @@
+// wave.h -- wave64 reduction helpers for gfx950.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "softdouble.h"
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  int l = lane_id();
+  return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+__device__ __forceinline__ int wave_min_i32(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+
+__device__ __forceinline__ int wave_max_i32(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+  for (int o = 32; o > 0; o >>= 1)
+    v += (uint64_t)__shfl_xor((unsigned long long)v, o);
+  return v;
+}
+
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
+  for (int o = 32; o > 0; o >>= 1) v += (int64_t)__shfl_xor((long long)v, o);
+  return v;
+}
+
+__device__ __forceinline__ float wave_sum_f32(float v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+__device__ __forceinline__ float wave_min_f32(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+  return v;
+}
+
+__device__ __forceinline__ float wave_max_f32(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  return v;
+}
+
+// In-order composition of the lanes' maps (lane 0's element first); the
+// result is returned in every lane.
+__device__ __forceinline__ sd_map wave_compose_maps(sd_map m) {
+  int l = lane_id();
+  for (int o = 1; o < 64; o <<= 1) {
+    sd_map h;
+    h.d0 = (uint64_t)__shfl_down((unsigned long long)m.d0, o);
+    h.d1 = (uint64_t)__shfl_down((unsigned long long)m.d1, o);
+    h.bad = __shfl_down(m.bad, o);
+    if ((l & (2 * o - 1)) == 0) m = sd_map_compose(m, h);
+  }
+  m.d0 = (uint64_t)__shfl((unsigned long long)m.d0, 0);
+  m.d1 = (uint64_t)__shfl((unsigned long long)m.d1, 0);
+  m.bad = __shfl(m.bad, 0);
+  return m;
+}
+
+// Accumulate one 64-addend chunk (lane k holds addend k; lanes >= nvalid
+// hold 0) into the wave-uniform exact double S, reproducing k sequential
+// correctly rounded double additions.
+__device__ __forceinline__ void sd_chunk(sd_t &S, float v, int nvalid) {
+  if (S.m != 0 && S.s == 0) {
+    sd_map mp = wave_compose_maps(sd_map_of(v, S.e));
+    sd_t S2 = S;
+    if (sd_map_apply(&S2, mp, S.e)) {
+      S = S2;
+      return;
+    }
+  }
+  {
+    int t = wave_min_i32(min(sd_lsb_f32(v), sd_lsb(S)));
+    uint64_t u = sd_units_f32(v, t);
+    int bad = __any(u == ~0ull);
+    uint64_t su = sd_units(S, t);
+    if (!bad && su != ~0ull) {
+      uint64_t tot = wave_sum_u64(u) + su;
+      if (tot < SD_M53) {
+        S = sd_from_units(tot, t);
+        return;
+      }
+    }
+  }
+  for (int k = 0; k < nvalid; ++k) {
+    float vk = __shfl(v, k);
+    S = sd_add_f32(S, vk);
+  }
+}

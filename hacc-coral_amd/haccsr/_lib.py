@@ -1,0 +1,266 @@
+"""ctypes binding of libhaccsr.so (include/hacc_sr.h).
+
+The library is built in-tree (hacc-coral_amd/libhaccsr.so) by
+``__graft_entry__.build()`` / ``make -C hacc-coral_amd``.  There is no CPU
+fallback: every entry point runs on a HIP device and raises when the library
+or the device is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "libhaccsr.so")
+
+HSR_HOST_PTRS, HSR_DEVICE_PTRS = 0, 1
+LAW_STEP16, LAW_SR_POLY5, LAW_SR_POLY6, LAW_NEWTON = 0, 1, 2, 3
+
+EXPORTS = [
+    "hsr_last_error", "hsr_version", "hsr_create", "hsr_destroy",
+    "hsr_get_stream", "hsr_build", "hsr_lists", "hsr_forces", "hsr_kick",
+    "hsr_adjoint", "hsr_get_stats", "hsr_get_nodes", "hsr_get_perm",
+    "hsr_get_lists", "hsr_build_many", "hsr_lists_many", "hsr_set_tuning",
+    "hsr_get_tuning",
+]
+
+
+class HsrError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [("device", C.c_int), ("tdpts", C.c_int), ("law", C.c_int),
+                ("reserved0", C.c_int), ("nd", C.c_int64), ("ds", C.c_int64),
+                ("tmin", C.c_int64), ("fsrrmax", C.c_float), ("rsm", C.c_float),
+                ("oa", C.c_float), ("tan_oa", C.c_float), ("ppc", C.c_float),
+                ("reserved1", C.c_float * 3)]
+
+
+class Node(C.Structure):
+    _fields_ = [("count", C.c_int64), ("offset", C.c_int64),
+                ("cl", C.c_int64), ("cr", C.c_int64),
+                ("ppm", C.c_float * 1), ("tdr", C.c_float),
+                ("xmin", C.c_float * 3), ("xmax", C.c_float * 3),
+                ("xc", C.c_float * 3)]
+
+
+NODE_DTYPE = np.dtype([("count", "<i8"), ("offset", "<i8"), ("cl", "<i8"),
+                       ("cr", "<i8"), ("ppm", "<f4", (1,)), ("tdr", "<f4"),
+                       ("xmin", "<f4", (3,)), ("xmax", "<f4", (3,)),
+                       ("xc", "<f4", (3,))], align=True)
+assert NODE_DTYPE.itemsize == C.sizeof(Node)
+
+
+class Stats(C.Structure):
+    _fields_ = [("count", C.c_int64), ("nodes", C.c_int64), ("leaves", C.c_int64),
+                ("empty_leaves", C.c_int64), ("max_ppn", C.c_int64),
+                ("listed_leaves", C.c_int64), ("segments", C.c_int64),
+                ("entries", C.c_int64), ("pairs", C.c_int64),
+                ("max_depth", C.c_int64), ("build_ms", C.c_double),
+                ("lists_ms", C.c_double), ("force_ms", C.c_double)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("i_per_lane", C.c_int), ("tile", C.c_int), ("chunk", C.c_int),
+                ("waves_per_cu", C.c_int), ("use_graph", C.c_int),
+                ("reserved", C.c_int * 3)]
+
+
+_lib = None
+
+
+def load(path: str | None = None):
+    """Load libhaccsr.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise HsrError(f"{p} is missing: build it with __graft_entry__.build() "
+                       "or `make -C hacc-coral_amd`")
+    L = C.CDLL(p)
+    vp, i64, f32p = C.c_void_p, C.c_int64, C.c_void_p
+    L.hsr_last_error.restype = C.c_char_p
+    L.hsr_version.restype = C.c_char_p
+    L.hsr_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(Params)]
+    L.hsr_destroy.argtypes = [vp]
+    L.hsr_get_stream.argtypes = [vp, C.POINTER(C.c_void_p)]
+    L.hsr_build.argtypes = [vp, i64] + [vp] * 10 + [vp, vp, C.c_int]
+    L.hsr_build_many.argtypes = [vp, i64, vp] + [vp] * 10 + [vp, vp, C.c_int]
+    L.hsr_lists.argtypes = [vp, vp, vp]
+    L.hsr_lists_many.argtypes = [vp, vp, vp]
+    L.hsr_forces.argtypes = [vp, vp, vp, vp, C.c_int]
+    L.hsr_kick.argtypes = [vp, C.c_float]
+    L.hsr_adjoint.argtypes = [vp] + [vp] * 6 + [C.c_int]
+    L.hsr_get_stats.argtypes = [vp, C.POINTER(Stats)]
+    L.hsr_get_nodes.argtypes = [vp, vp, i64]
+    L.hsr_get_perm.argtypes = [vp, vp, i64]
+    L.hsr_get_lists.argtypes = [vp, vp, vp, vp, vp, vp, i64, i64]
+    L.hsr_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    L.hsr_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    for name in EXPORTS:
+        if name not in ("hsr_last_error", "hsr_version"):
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(status: int, what: str):
+    if status != 0:
+        msg = load().hsr_last_error().decode(errors="replace")
+        raise HsrError(f"{what} failed (status {status}): {msg}")
+
+
+def tanf(x: float) -> float:
+    """tanf(oa) as the host libm evaluates it (RCBForceTree.cxx:381)."""
+    libm = C.CDLL("libm.so.6")
+    libm.tanf.argtypes = [C.c_float]
+    libm.tanf.restype = C.c_float
+    return float(libm.tanf(C.c_float(x)))
+
+
+def _ptr(a):
+    """Address of a numpy array or torch tensor (None -> NULL)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"]
+        return a.ctypes.data
+    return int(a.data_ptr())
+
+
+class Context:
+    """One device context (hsr_create ... hsr_destroy)."""
+
+    def __init__(self, nd: int = 128, fsrrmax: float = 3.12, rsm: float = 0.1,
+                 oa: float = 0.1, ppc: float = 0.9, law: int = LAW_STEP16,
+                 device: int = 0, ds: int = 2, tmin: int = 128, tdpts: int = 1):
+        self.L = load()
+        p = Params()
+        p.device = device
+        p.tdpts = tdpts
+        p.law = law
+        p.nd = int(nd)
+        p.ds = int(ds)
+        p.tmin = int(tmin)
+        p.fsrrmax = float(np.float32(fsrrmax))
+        p.rsm = float(np.float32(rsm))
+        p.oa = float(np.float32(oa))
+        p.tan_oa = tanf(float(np.float32(oa)))
+        p.ppc = float(np.float32(ppc))
+        self.params = p
+        h = C.c_void_p()
+        check(self.L.hsr_create(C.byref(h), C.byref(p)), "hsr_create")
+        self.h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.hsr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- phases
+    def build(self, x, y, z, mass, vx=None, vy=None, vz=None, phi=None,
+              ids=None, mask=None, min_loc=(0, 0, 0), max_loc=(1, 1, 1),
+              device_ptrs=False):
+        n = int(x.shape[0])
+        lo = np.ascontiguousarray(min_loc, np.float32)
+        hi = np.ascontiguousarray(max_loc, np.float32)
+        fl = HSR_DEVICE_PTRS if device_ptrs else HSR_HOST_PTRS
+        check(self.L.hsr_build(self.h, n, _ptr(x), _ptr(y), _ptr(z), _ptr(vx),
+                               _ptr(vy), _ptr(vz), _ptr(mass), _ptr(phi),
+                               _ptr(ids), _ptr(mask), lo.ctypes.data,
+                               hi.ctypes.data, fl), "hsr_build")
+        self.n = n
+
+    def build_many(self, offsets, x, y, z, mass, min_loc, max_loc, vx=None,
+                   vy=None, vz=None, phi=None, ids=None, mask=None,
+                   device_ptrs=False):
+        off = np.ascontiguousarray(offsets, np.int64)
+        lo = np.ascontiguousarray(min_loc, np.float32).reshape(-1)
+        hi = np.ascontiguousarray(max_loc, np.float32).reshape(-1)
+        fl = HSR_DEVICE_PTRS if device_ptrs else HSR_HOST_PTRS
+        check(self.L.hsr_build_many(self.h, off.shape[0] - 1, off.ctypes.data,
+                                    _ptr(x), _ptr(y), _ptr(z), _ptr(vx), _ptr(vy),
+                                    _ptr(vz), _ptr(mass), _ptr(phi), _ptr(ids),
+                                    _ptr(mask), lo.ctypes.data, hi.ctypes.data, fl),
+              "hsr_build_many")
+        self.n = int(off[-1])
+
+    def lists(self, min_force, max_force):
+        lo = np.ascontiguousarray(min_force, np.float32)
+        hi = np.ascontiguousarray(max_force, np.float32)
+        check(self.L.hsr_lists(self.h, lo.ctypes.data, hi.ctypes.data), "hsr_lists")
+
+    def lists_many(self, min_force, max_force):
+        lo = np.ascontiguousarray(min_force, np.float32).reshape(-1)
+        hi = np.ascontiguousarray(max_force, np.float32).reshape(-1)
+        check(self.L.hsr_lists_many(self.h, lo.ctypes.data, hi.ctypes.data),
+              "hsr_lists_many")
+
+    def forces(self, ax=None, ay=None, az=None, device_ptrs=False):
+        if ax is None:
+            ax = np.zeros(self.n, np.float32)
+            ay = np.zeros(self.n, np.float32)
+            az = np.zeros(self.n, np.float32)
+        fl = HSR_DEVICE_PTRS if device_ptrs else HSR_HOST_PTRS
+        check(self.L.hsr_forces(self.h, _ptr(ax), _ptr(ay), _ptr(az), fl), "hsr_forces")
+        return ax, ay, az
+
+    def kick(self, fcoeff: float):
+        check(self.L.hsr_kick(self.h, C.c_float(fcoeff)), "hsr_kick")
+
+    def adjoint(self, lx, ly, lz, gx, gy, gz):
+        check(self.L.hsr_adjoint(self.h, _ptr(lx), _ptr(ly), _ptr(lz), _ptr(gx),
+                                 _ptr(gy), _ptr(gz), HSR_DEVICE_PTRS), "hsr_adjoint")
+
+    # -- introspection
+    def stats(self) -> Stats:
+        s = Stats()
+        check(self.L.hsr_get_stats(self.h, C.byref(s)), "hsr_get_stats")
+        return s
+
+    def nodes(self) -> np.ndarray:
+        st = self.stats()
+        out = np.zeros(max(st.nodes, 1), NODE_DTYPE)
+        check(self.L.hsr_get_nodes(self.h, out.ctypes.data, out.shape[0]), "hsr_get_nodes")
+        return out[: st.nodes]
+
+    def perm(self) -> np.ndarray:
+        out = np.zeros(max(self.n, 1), np.int64)
+        check(self.L.hsr_get_perm(self.h, out.ctypes.data, out.shape[0]), "hsr_get_perm")
+        return out[: self.n]
+
+    def get_lists(self):
+        st = self.stats()
+        nl, ns = st.listed_leaves, st.segments
+        leaves = np.zeros(max(nl, 1), np.int64)
+        seg_ptr = np.zeros(nl + 1, np.int64)
+        sa = np.zeros(max(ns, 1), np.int64)
+        sb = np.zeros(max(ns, 1), np.int64)
+        ent = np.zeros(max(nl, 1), np.int64)
+        check(self.L.hsr_get_lists(self.h, leaves.ctypes.data, seg_ptr.ctypes.data,
+                                   sa.ctypes.data, sb.ctypes.data, ent.ctypes.data,
+                                   max(nl, 1), max(ns, 1)), "hsr_get_lists")
+        return leaves[:nl], seg_ptr, sa[:ns], sb[:ns], ent[:nl]
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        check(self.L.hsr_get_stream(self.h, C.byref(s)), "hsr_get_stream")
+        return s.value or 0
+
+    def set_tuning(self, **kw):
+        t = Tuning()
+        check(self.L.hsr_get_tuning(self.h, C.byref(t)), "hsr_get_tuning")
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        check(self.L.hsr_set_tuning(self.h, C.byref(t)), "hsr_set_tuning")

@@ -1,0 +1,181 @@
+/*
+ * hacc_sr.h -- C ABI of the MI355X short-range force library (libhaccsr.so).
+ *
+ * One context per host thread and device.  Every entry point returns an
+ * hsr_status (0 = success); hsr_last_error() gives the message of the last
+ * failure on the calling thread.  There is no CPU fallback: a context can
+ * only be created on a HIP device.
+ *
+ * Reference surface replaced (BenWibking/hacc-coral, src/halo_finder):
+ *   RCBForceTree<TDPTS>::RCBForceTree(...)   RCBForceTree.h:96-124,
+ *                                             RCBForceTree.cxx:336-450
+ *       = hsr_create + hsr_build + hsr_lists + hsr_kick (+ hsr_destroy)
+ *   ForceLaw / ForceLawSR / FGridEvalPoly    ForceLaw.h:95-127
+ *       = hsr_params.law (+ coefficients)
+ *   extern "C" cm(...)                        RCBForceTree.cxx:514-516
+ *       = device centre of mass inside hsr_build
+ *   extern "C" Step16_int(...)                RCBForceTree.cxx:572
+ *       = device leaf kernel inside hsr_forces / hsr_kick
+ */
+#ifndef HACC_SR_H
+#define HACC_SR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int hsr_status;
+enum {
+  HSR_OK = 0,
+  HSR_ERR_ARG = 1,        /* invalid argument */
+  HSR_ERR_HIP = 2,        /* HIP runtime error */
+  HSR_ERR_UNSUPPORTED = 3,/* configuration the device path does not implement */
+  HSR_ERR_STATE = 4,      /* call out of order (e.g. kick before build) */
+  HSR_ERR_NOMEM = 5,      /* device allocation failed */
+  HSR_ERR_NUMERIC = 6     /* an input the exact build cannot represent */
+};
+
+/* force laws (ForceLaw.h:95-127, BGQStep16.c:167-187) */
+enum {
+  HSR_LAW_STEP16 = 0,    /* (r^2+rsm^2)^-3/2 - P5(r^2), r^2 < fsrrmax^2, r^2 > 0 */
+  HSR_LAW_SR_POLY5 = 1,  /* ForceLawSR(FGridEvalPoly), POLY_ORDER 5 */
+  HSR_LAW_SR_POLY6 = 2,  /* ForceLawSR(FGridEvalPoly), POLY_ORDER 6 (stock) */
+  HSR_LAW_NEWTON = 3     /* ForceLawNewton, 1/r^3 */
+};
+
+/* pointer-location flags */
+enum {
+  HSR_HOST_PTRS = 0,     /* particle arrays are host memory */
+  HSR_DEVICE_PTRS = 1    /* particle arrays are device memory on the context's device */
+};
+
+typedef struct {
+  int device;            /* HIP device ordinal */
+  int tdpts;             /* pseudo-particles per node: 1 (RCBMonopoleForceTree) */
+  int law;               /* HSR_LAW_* */
+  int reserved0;
+  int64_t nd;            /* leaf size ("nDirect", ctor arg nd) */
+  int64_t ds;            /* depth safety (ctor arg ds): initial node reservation */
+  int64_t tmin;          /* ctor arg tmin (task threshold; no effect on results) */
+  float fsrrmax;         /* ctor arg fsm */
+  float rsm;             /* ctor arg r (Plummer softening) */
+  float oa;              /* ctor arg oa (opening angle) */
+  float tan_oa;          /* tanf(oa) as the host libm evaluates it (RCBForceTree.cxx:381) */
+  float ppc;             /* ctor arg ppc (pseudo-particle contraction, 0.9) */
+  float reserved1[3];
+} hsr_params;
+
+/* Tree node in the reference's layout and canonical numbering
+ * (RCBForceTree.h:131-147 with TDPTS = 1; ids as the unthreaded build
+ * allocates them, RCBForceTree.cxx:803-825). */
+typedef struct {
+  int64_t count, offset, cl, cr;
+  float ppm[1];
+  float tdr;
+  float xmin[3], xmax[3], xc[3];
+} hsr_node;
+
+typedef struct hsr_ctx hsr_ctx;
+
+const char *hsr_last_error(void);
+const char *hsr_version(void);
+
+hsr_status hsr_create(hsr_ctx **ctx, const hsr_params *p);
+hsr_status hsr_destroy(hsr_ctx *ctx);
+
+/* The HIP stream the context launches on (hipStream_t as void*). */
+hsr_status hsr_get_stream(hsr_ctx *ctx, void **stream);
+
+/* Tree build (RCBForceTree.cxx:893-918 with cm, partition, pseudo-particles).
+ * Permutes all ten arrays in place exactly as the reference does and
+ * records the tree.  min_loc/max_loc: the tree box (root bbox seed). */
+hsr_status hsr_build(hsr_ctx *ctx, int64_t count, float *x, float *y,
+                     float *z, float *vx, float *vy, float *vz, float *mass,
+                     float *phi, int64_t *id, uint16_t *mask,
+                     const float min_loc[3], const float max_loc[3],
+                     int ptr_flags);
+
+/* Interaction lists of every leaf inside the force box
+ * (RCBForceTree.cxx:924-1197). */
+hsr_status hsr_lists(hsr_ctx *ctx, const float min_force[3],
+                     const float max_force[3]);
+
+/* Raw short-range force F_i = sum_j m_j f(r_ij^2) (x_j - x_i) for particles
+ * of the listed leaves, in tree (sorted) order; zero for other particles. */
+hsr_status hsr_forces(hsr_ctx *ctx, float *ax, float *ay, float *az,
+                      int ptr_flags);
+
+/* The kick of the reference constructor: v += fcoeff * m_i * F_i on the
+ * particles of the listed leaves, applied to the velocity arrays given to
+ * hsr_build (nbody1, RCBForceTree.cxx:599-619). */
+hsr_status hsr_kick(hsr_ctx *ctx, float fcoeff);
+
+/* Vector-Jacobian product of the force with respect to positions:
+ * g = d(sum_i lambda_i . F_i)/dx for the current tree and lists, including
+ * the pseudo-particle centre-of-mass dependence.  lambda and g are in tree
+ * order; g covers every particle (zero where no listed pair touches it). */
+hsr_status hsr_adjoint(hsr_ctx *ctx, const float *lx, const float *ly,
+                       const float *lz, float *gx, float *gy, float *gz,
+                       int ptr_flags);
+
+/* ---------------------------------------------------------- introspection */
+typedef struct {
+  int64_t count;          /* particles */
+  int64_t nodes;          /* tree.size() */
+  int64_t leaves;         /* leaves with count > 0 */
+  int64_t empty_leaves;   /* orphan/empty leaves (printStats) */
+  int64_t max_ppn;        /* largest leaf */
+  int64_t listed_leaves;  /* leaves inside the force box */
+  int64_t segments;       /* list segments over listed leaves */
+  int64_t entries;        /* list entries over listed leaves */
+  int64_t pairs;          /* sum over listed leaves of count * list length */
+  int64_t max_depth;
+  double build_ms, lists_ms, force_ms; /* device time of the last calls */
+} hsr_stats;
+
+hsr_status hsr_get_stats(hsr_ctx *ctx, hsr_stats *st);
+/* Tree in canonical numbering (host buffer of stats.nodes entries). */
+hsr_status hsr_get_nodes(hsr_ctx *ctx, hsr_node *out, int64_t cap);
+/* perm[k] = input index of the particle now at position k. */
+hsr_status hsr_get_perm(hsr_ctx *ctx, int64_t *perm, int64_t cap);
+/* Lists in the reference's order: listed leaves in the order
+ * calcInternodeForces visits them; per leaf, segments (a, b): particle
+ * range (offset a, count b >= 0) or pseudo-particle of node a (b = -1). */
+hsr_status hsr_get_lists(hsr_ctx *ctx, int64_t *leaves, int64_t *seg_ptr,
+                         int64_t *seg_a, int64_t *seg_b, int64_t *entries,
+                         int64_t cap_leaves, int64_t cap_segs);
+
+/* ------------------------------------------------------------- ensemble */
+/* Independent boxes in one context: box b owns particles
+ * [offsets[b], offsets[b+1]) of the arrays; each is built as its own tree,
+ * and all boxes' lists are processed by one persistent force launch.
+ * Boxes' tree/force boxes are given per box (3 floats each). */
+hsr_status hsr_build_many(hsr_ctx *ctx, int64_t nboxes, const int64_t *offsets,
+                          float *x, float *y, float *z, float *vx, float *vy,
+                          float *vz, float *mass, float *phi, int64_t *id,
+                          uint16_t *mask, const float *min_loc,
+                          const float *max_loc, int ptr_flags);
+hsr_status hsr_lists_many(hsr_ctx *ctx, const float *min_force,
+                          const float *max_force);
+
+/* ------------------------------------------------------------- tuning */
+typedef struct {
+  int i_per_lane;     /* i-particles held per lane (0 = automatic) */
+  int tile;           /* j entries staged per LDS tile (0 = automatic) */
+  int chunk;          /* maximum list entries per work item (0 = automatic) */
+  int waves_per_cu;   /* persistent waves per CU (0 = automatic) */
+  int use_graph;      /* capture the force phase in a hipGraph and replay it */
+  int reserved[3];
+} hsr_tuning;
+
+hsr_status hsr_set_tuning(hsr_ctx *ctx, const hsr_tuning *t);
+hsr_status hsr_get_tuning(hsr_ctx *ctx, hsr_tuning *t);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HACC_SR_H */

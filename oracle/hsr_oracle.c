@@ -1,0 +1,763 @@
+/*
+ * hsr_oracle.c -- CPU restatement of HACC's short-range RCB force path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker; never linked by the product).
+ * Build: oracle/Makefile, gcc -O2 -ffp-contract=off -fopenmp.  The
+ * reference's x86 build has no -march flag (src/env/bashrc.linux-generic.gcc
+ * :4-16), so it never contracts a*b+c into an FMA; -ffp-contract=off keeps
+ * that operation sequence here.
+ */
+#include "hsr_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ cm */
+/* BGQCM.c:181-212 (generic branch): tight bbox via fminf/fmaxf seeded with
+ * element 0; centre of mass as sequential double sums of float products,
+ * divided in double and rounded to float. */
+void hsro_cm(int64_t count, const float *xx, const float *yy, const float *zz,
+             const float *mass, float *xmin, float *xmax, float *xc) {
+  double x = 0, y = 0, z = 0, m = 0;
+  for (int64_t i = 0; i < count; ++i) {
+    if (i == 0) {
+      xmin[0] = xmax[0] = xx[0];
+      xmin[1] = xmax[1] = yy[0];
+      xmin[2] = xmax[2] = zz[0];
+    } else {
+      xmin[0] = fminf(xmin[0], xx[i]);
+      xmax[0] = fmaxf(xmax[0], xx[i]);
+      xmin[1] = fminf(xmin[1], yy[i]);
+      xmax[1] = fmaxf(xmax[1], yy[i]);
+      xmin[2] = fminf(xmin[2], zz[i]);
+      xmax[2] = fmaxf(xmax[2], zz[i]);
+    }
+    float w = mass[i];
+    x += (double)(w * xx[i]);
+    y += (double)(w * yy[i]);
+    z += (double)(w * zz[i]);
+    m += (double)w;
+  }
+  xc[0] = (float)(x / m);
+  xc[1] = (float)(y / m);
+  xc[2] = (float)(z / m);
+}
+
+/* ----------------------------------------------------------- partition */
+/* RCBForceTree.cxx:623-672.  idx collects ascending positions with
+ * key < pv; then for j < is the elements at j and idx[j] are swapped in
+ * every array.  Heap scratch instead of the reference's stack VLA. */
+int64_t hsro_partition(int64_t n, float *xx, float *yy, float *zz, float *vx,
+                       float *vy, float *vz, float *mass, float *phi,
+                       int64_t *id, uint16_t *mask, float pv) {
+  int64_t *idx = (int64_t *)malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+  int64_t is = 0;
+  for (int64_t i = 0; i < n; ++i)
+    if (xx[i] < pv) idx[is++] = i;
+  for (int64_t j = 0; j < is; ++j) {
+    int64_t i = idx[j];
+    float t;
+    int64_t t64;
+    uint16_t t16;
+    t = mass[i]; mass[i] = mass[j]; mass[j] = t;
+    t = phi[i]; phi[i] = phi[j]; phi[j] = t;
+    t16 = mask[i]; mask[i] = mask[j]; mask[j] = t16;
+    t64 = id[i]; id[i] = id[j]; id[j] = t64;
+    t = xx[i]; xx[i] = xx[j]; xx[j] = t;
+    t = yy[i]; yy[i] = yy[j]; yy[j] = t;
+    t = zz[i]; zz[i] = zz[j]; zz[j] = t;
+    t = vx[i]; vx[i] = vx[j]; vx[j] = t;
+    t = vy[i]; vy[i] = vy[j]; vy[j] = t;
+    t = vz[i]; vz[i] = vz[j]; vz[j] = t;
+  }
+  free(idx);
+  return is;
+}
+
+/* ------------------------------------------------------- pseudo-particles */
+/* RCBForceTree.cxx:519-523 */
+static float pptdr(const float *xmin, const float *xmax, const float *xc) {
+  float a = xmax[0] - xc[0], b = xmax[1] - xc[1], c = xmax[2] - xc[2];
+  float d = xc[0] - xmin[0], e = xc[1] - xmin[1], f = xc[2] - xmin[2];
+  /* std::min(a, std::min(b, std::min(c, std::min(d, std::min(e, f))))) ;
+   * std::min(p, q) returns (q < p) ? q : p. */
+  float r = (f < e) ? f : e;
+  r = (r < d) ? r : d;
+  r = (r < c) ? r : c;
+  r = (r < b) ? r : b;
+  r = (r < a) ? r : a;
+  return r;
+}
+
+/* RCBForceTree.cxx:525-534 with the TDPTS = 1 design (0,0,0). */
+static void pppts1(float tdr, const float *xc, float *px, float *py, float *pz) {
+  *px = tdr * 0.0f + xc[0];
+  *py = tdr * 0.0f + xc[1];
+  *pz = tdr * 0.0f + xc[2];
+}
+
+/* RCBForceTree.cxx:536-569 for TDPTS = 1: K = 1, odr0 = 1; the single
+ * design point sits at the centre (rj2 == 0), so odr1 = odr2 = 0 and each
+ * member adds mass*(1+0+0). */
+static void pp1(int64_t count, const float *xx, const float *yy,
+                const float *zz, const float *mass, const float *xc,
+                const float *ppx, const float *ppy, const float *ppz,
+                float *ppm, float tdr) {
+  const float K = 1.0f;
+  const float odr0 = 1 / K;
+  for (int64_t i = 0; i < count; ++i) {
+    float xi = xx[i] - xc[0], yi = yy[i] - xc[1], zi = zz[i] - xc[2];
+    float ri = sqrtf(xi * xi + yi * yi + zi * zi);
+    float xj = ppx[0] - xc[0], yj = ppy[0] - xc[1], zj = ppz[0] - xc[2];
+    float rj2 = xj * xj + yj * yj + zj * zj;
+    float odr1 = 0, odr2 = 0;
+    if (rj2 != 0) {
+      float rj = sqrtf(rj2);
+      float aij = (xi * xj + yi * yj + zi * zj) / (ri * rj);
+      odr1 = (3 / K) * (ri / tdr) * aij;
+      odr2 = (float)((double)((5 / K) * (ri / tdr) * (ri / tdr)) * 0.5 *
+                     (double)(3 * aij * aij - 1));
+    }
+    ppm[0] += mass[i] * (odr0 + odr1 + odr2);
+  }
+}
+
+/* ---------------------------------------------------------------- build */
+typedef struct {
+  const hsro_params *p;
+  float *a[8]; /* x y z vx vy vz mass phi */
+  int64_t *id;
+  uint16_t *mask;
+  hsro_node *t;
+  int64_t size, cap;
+  int overflow;
+} builder;
+
+static void build_node(builder *b, int64_t tl);
+
+/* RCBForceTree.cxx:690-775 (unthreaded): rotate coordinate pointers by d,
+ * partition at xc[d], and recurse left then right. */
+static void build_subtree(builder *b, int d, int64_t tl, int64_t tlcl,
+                          int64_t tlcr) {
+  float *x1, *x2, *x3;
+  float *xx = b->a[0], *yy = b->a[1], *zz = b->a[2];
+  if (d == 0) { x1 = xx; x2 = yy; x3 = zz; }
+  else if (d == 1) { x1 = yy; x2 = zz; x3 = xx; }
+  else { x1 = zz; x2 = xx; x3 = yy; }
+  hsro_node *t = b->t;
+  float split = t[tl].xc[d];
+  int64_t off = t[tl].offset;
+  int64_t is = hsro_partition(t[tl].count, x1 + off, x2 + off, x3 + off,
+                              b->a[3] + off, b->a[4] + off, b->a[5] + off,
+                              b->a[6] + off, b->a[7] + off, b->id + off,
+                              b->mask + off, split);
+  if (is == 0 || is == t[tl].count) return;
+  t[tlcl].count = is;
+  t[tlcr].count = t[tl].count - t[tlcl].count;
+  if (t[tlcl].count > 0) {
+    t[tl].cl = tlcl;
+    t[tlcl].offset = t[tl].offset;
+    t[tlcl].xmax[d] = split;
+    build_node(b, tlcl);
+  }
+  t = b->t;
+  if (t[tlcr].count > 0) {
+    t[tl].cr = tlcr;
+    t[tlcr].offset = t[tl].offset + t[tlcl].count;
+    t[tlcr].xmin[d] = split;
+    build_node(b, tlcr);
+  }
+}
+
+/* RCBForceTree.cxx:779-890 */
+static void build_node(builder *b, int64_t tl) {
+  hsro_node *t = b->t;
+  int64_t cnt = t[tl].count, off = t[tl].offset;
+  float **a = b->a;
+  hsro_cm(cnt, a[0] + off, a[1] + off, a[2] + off, a[6] + off, t[tl].xmin,
+          t[tl].xmax, t[tl].xc);
+  if (cnt <= b->p->nd) {
+    t[tl].tdr = b->p->ppc * pptdr(t[tl].xmin, t[tl].xmax, t[tl].xc);
+    t[tl].ppm[0] = 0;
+    if (cnt > 1) {
+      float px, py, pz;
+      pppts1(t[tl].tdr, t[tl].xc, &px, &py, &pz);
+      pp1(cnt, a[0] + off, a[1] + off, a[2] + off, a[6] + off, t[tl].xc,
+          &px, &py, &pz, t[tl].ppm, t[tl].tdr);
+    }
+    return;
+  }
+  int64_t tlcl = b->size, tlcr = tlcl + 1;
+  if (tlcr + 1 > b->cap) {
+    b->overflow = 1;
+    return;
+  }
+  b->size = tlcr + 1;
+  memset(&t[tlcl], 0, sizeof(hsro_node) * 2);
+  for (int i = 0; i < 3; ++i) {
+    t[tlcl].xmin[i] = t[tl].xmin[i];
+    t[tlcr].xmin[i] = t[tl].xmin[i];
+    t[tlcl].xmax[i] = t[tl].xmax[i];
+    t[tlcr].xmax[i] = t[tl].xmax[i];
+  }
+  float xlen[3];
+  for (int i = 0; i < 3; ++i) xlen[i] = t[tl].xmax[i] - t[tl].xmin[i];
+  int d;
+  if (xlen[0] > xlen[1] && xlen[0] > xlen[2]) d = 0;
+  else if (xlen[1] > xlen[2]) d = 1;
+  else d = 2;
+  build_subtree(b, d, tl, tlcl, tlcr);
+  if (b->overflow) return;
+
+  float ppx, ppy, ppz;
+  t[tl].tdr = b->p->ppc * pptdr(t[tl].xmin, t[tl].xmax, t[tl].xc);
+  pppts1(t[tl].tdr, t[tl].xc, &ppx, &ppy, &ppz);
+  t[tl].ppm[0] = 0;
+  const int64_t ch[2] = {tlcl, tlcr};
+  for (int k = 0; k < 2; ++k) {
+    int64_t c = ch[k];
+    if (t[c].count > 0) {
+      if (t[c].count <= 1) {
+        int64_t offc = t[c].offset;
+        pp1(t[c].count, a[0] + offc, a[1] + offc, a[2] + offc, a[6] + offc,
+            t[tl].xc, &ppx, &ppy, &ppz, t[tl].ppm, t[tl].tdr);
+      } else {
+        float cx, cy, cz;
+        pppts1(t[c].tdr, t[c].xc, &cx, &cy, &cz);
+        pp1(1, &cx, &cy, &cz, t[c].ppm, t[tl].xc, &ppx, &ppy, &ppz,
+            t[tl].ppm, t[tl].tdr);
+      }
+    }
+  }
+}
+
+int64_t hsro_build(const hsro_params *p, int64_t n, float *x, float *y,
+                   float *z, float *vx, float *vy, float *vz, float *mass,
+                   float *phi, int64_t *id, uint16_t *mask, hsro_node *nodes,
+                   int64_t cap) {
+  if (cap < 1) return -1;
+  builder b;
+  b.p = p;
+  b.a[0] = x; b.a[1] = y; b.a[2] = z; b.a[3] = vx; b.a[4] = vy;
+  b.a[5] = vz; b.a[6] = mass; b.a[7] = phi;
+  b.id = id; b.mask = mask; b.t = nodes; b.cap = cap; b.size = 1;
+  b.overflow = 0;
+  memset(&nodes[0], 0, sizeof(hsro_node));
+  nodes[0].count = n;
+  nodes[0].offset = 0;
+  for (int i = 0; i < 3; ++i) {
+    nodes[0].xmin[i] = p->min_loc[i];
+    nodes[0].xmax[i] = p->max_loc[i];
+  }
+  build_node(&b, 0);
+  return b.overflow ? -1 : b.size;
+}
+
+/* ------------------------------------------------------------ leaf order */
+/* RCBForceTree.cxx:1151-1197.  out receives, for every leaf inside the
+ * force box, its node id; the leaves come in the task-spawn order. */
+int64_t hsro_leaf_order(const hsro_params *p, const hsro_node *t,
+                        int64_t nnodes, int64_t *out, int64_t cap) {
+  int64_t *q = (int64_t *)malloc(sizeof(int64_t) * (nnodes + 2));
+  int64_t *parents = (int64_t *)malloc(sizeof(int64_t) * (nnodes + 2));
+  int64_t nq = 0, np = 0, nout = 0;
+  q[nq++] = 0;
+  while (nq > 0) {
+    int64_t tl = q[nq - 1];
+    if (t[tl].cr == 0 && t[tl].cl == 0) {
+      --nq;
+      int inside = 1;
+      for (int i = 0; i < 3; ++i) {
+        inside &= (t[tl].xmax[i] < p->max_force[i] &&
+                   t[tl].xmax[i] > p->min_force[i]) ||
+                  (t[tl].xmin[i] < p->max_force[i] &&
+                   t[tl].xmin[i] > p->min_force[i]);
+      }
+      if (inside) {
+        if (nout >= cap) { nout = -1; break; }
+        out[nout++] = tl;
+      }
+    } else if (np > 0 && parents[np - 1] == tl) {
+      --np;
+      --nq;
+    } else {
+      if (t[tl].cl > 0) q[nq++] = t[tl].cl;
+      if (t[tl].cr > 0) q[nq++] = t[tl].cr;
+      parents[np++] = tl;
+    }
+  }
+  free(q);
+  free(parents);
+  return nout;
+}
+
+/* ------------------------------------------------------------------ lists */
+/* Ancestors of tl: walk from the root along the child whose particle range
+ * contains tl's range (equivalent to the DFS `parents` stack at the time
+ * tl is visited in calcInternodeForces). */
+static int64_t ancestors(const hsro_node *t, int64_t tl, int64_t *par) {
+  int64_t n = 0, cur = 0;
+  int64_t off = t[tl].offset;
+  while (cur != tl) {
+    par[n++] = cur;
+    int64_t cl = t[cur].cl, cr = t[cur].cr;
+    if (cl == tl || cr == tl) break;
+    if (cl > 0 && off >= t[cl].offset && off < t[cl].offset + t[cl].count)
+      cur = cl;
+    else
+      cur = cr;
+  }
+  return n;
+}
+
+static int in_sorted(const int64_t *a, int64_t n, int64_t v) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) / 2;
+    if (a[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo < n && a[lo] == v;
+}
+
+#define EMIT(A, B, L)                                \
+  do {                                               \
+    if (nseg >= cap) { ovf = 1; goto done; }         \
+    seg_a[nseg] = (A);                               \
+    seg_b[nseg] = (B);                               \
+    ++nseg;                                          \
+    nent += (L);                                     \
+  } while (0)
+
+/* RCBForceTree.cxx:924-1140 (useRealOA = false). */
+int64_t hsro_list(const hsro_params *p, const hsro_node *t, int64_t nnodes,
+                  int64_t tl, int64_t *seg_a, int64_t *seg_b, int64_t cap,
+                  int64_t *n_entries) {
+  const float fsrrmax = p->fsrrmax;
+  const float fsrrmax2 = fsrrmax * fsrrmax;
+  const float tanOA = p->tan_oa;
+  int64_t *par = (int64_t *)malloc(sizeof(int64_t) * (nnodes + 1));
+  int64_t npar = ancestors(t, tl, par);
+  int64_t *q = (int64_t *)malloc(sizeof(int64_t) * (nnodes + 2));
+  int64_t nq = 0, nseg = 0, nent = 0;
+  int ovf = 0;
+  q[nq++] = 0;
+  while (nq > 0) {
+    int64_t tln = q[--nq];
+    if (tln < tl) {
+      if (in_sorted(par, npar, tln)) {
+        int64_t tlncr = t[tln].cr, tlncl = t[tln].cl;
+        if (tlncl != tl && tlncl > 0 && t[tlncl].count > 0) q[nq++] = tlncl;
+        if (tlncr != tl && tlncr > 0 && t[tlncr].count > 0) q[nq++] = tlncr;
+        continue;
+      }
+    }
+    float dx = t[tln].xc[0] - t[tl].xc[0];
+    float dy = t[tln].xc[1] - t[tl].xc[1];
+    float dz = t[tln].xc[2] - t[tl].xc[2];
+    float dist2 = dx * dx + dy * dy + dz * dz;
+    float sx = t[tln].xmax[0] - t[tln].xmin[0];
+    float sy = t[tln].xmax[1] - t[tln].xmin[1];
+    float sz = t[tln].xmax[2] - t[tln].xmin[2];
+    float syz = (sz * sz < sy * sy) ? sz * sz : sy * sy;
+    float l2 = (syz < sx * sx) ? syz : sx * sx;
+    float dtt2 = dist2 * tanOA * tanOA;
+    int looksBig;
+    if (l2 > dtt2) {
+      looksBig = 1;
+    } else {
+      looksBig = 0;
+      for (int i = 0; i < 2 && !looksBig; ++i)
+        for (int j = 0; j < 2; ++j) {
+          float x1 = (i == 0 ? t[tln].xmin : t[tln].xmax)[0] - t[tl].xc[0];
+          float y1 = (j == 0 ? t[tln].xmin : t[tln].xmax)[1] - t[tl].xc[1];
+          float z1 = t[tln].xmin[2] - t[tl].xc[2];
+          float x2 = (i == 0 ? t[tln].xmax : t[tln].xmin)[0] - t[tl].xc[0];
+          float y2 = (j == 0 ? t[tln].xmax : t[tln].xmin)[1] - t[tl].xc[1];
+          float z2 = t[tln].xmax[2] - t[tl].xc[2];
+          float ddx = x1 - x2, ddy = y1 - y2, ddz = z1 - z2;
+          float dh2 = ddx * ddx + ddy * ddy + ddz * ddz;
+          if (dh2 > dtt2) {
+            looksBig = 1;
+            break;
+          }
+        }
+    }
+    if (!looksBig) {
+      if (dist2 > fsrrmax2) continue;
+      if (t[tln].count <= 1) {
+        EMIT(t[tln].offset, t[tln].count, t[tln].count);
+        continue;
+      }
+      EMIT(tln, -1, 1);
+      continue;
+    } else if (t[tln].cr == 0 && t[tln].cl == 0) {
+      EMIT(t[tln].offset, t[tln].count, t[tln].count);
+      continue;
+    }
+    int64_t ch[2] = {t[tln].cl, t[tln].cr};
+    for (int k = 0; k < 2; ++k) {
+      int64_t c = ch[k];
+      if (c > 0 && t[c].count > 0) {
+        int close = 1;
+        for (int i = 0; i < 3; ++i) {
+          float dist = 0;
+          if (t[tl].xmax[i] < t[c].xmin[i])
+            dist = t[c].xmin[i] - t[tl].xmax[i];
+          else if (t[tl].xmin[i] > t[c].xmax[i])
+            dist = t[tl].xmin[i] - t[c].xmax[i];
+          if (dist > fsrrmax) {
+            close = 0;
+            break;
+          }
+        }
+        if (close) q[nq++] = c;
+      }
+    }
+  }
+  EMIT(t[tl].offset, t[tl].count, t[tl].count);
+done:
+  free(par);
+  free(q);
+  if (n_entries) *n_entries = nent;
+  return ovf ? -1 : nseg;
+}
+
+int64_t hsro_lists_all(const hsro_params *p, const hsro_node *t,
+                       int64_t nnodes, const int64_t *leaves, int64_t nleaves,
+                       int64_t *seg_ptr, int64_t *seg_a, int64_t *seg_b,
+                       int64_t cap, int64_t *entries_per_leaf) {
+  /* pass 1: counts (parallel), pass 2: fill */
+  int64_t *nsegs = (int64_t *)malloc(sizeof(int64_t) * (nleaves + 1));
+  int bad = 0;
+#pragma omp parallel
+  {
+    int64_t lcap = 1 << 16;
+    int64_t *la = (int64_t *)malloc(sizeof(int64_t) * lcap);
+    int64_t *lb = (int64_t *)malloc(sizeof(int64_t) * lcap);
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t k = 0; k < nleaves; ++k) {
+      int64_t ne = 0;
+      int64_t ns = hsro_list(p, t, nnodes, leaves[k], la, lb, lcap, &ne);
+      while (ns < 0) {
+        lcap *= 2;
+        la = (int64_t *)realloc(la, sizeof(int64_t) * lcap);
+        lb = (int64_t *)realloc(lb, sizeof(int64_t) * lcap);
+        ns = hsro_list(p, t, nnodes, leaves[k], la, lb, lcap, &ne);
+      }
+      nsegs[k] = ns;
+      if (entries_per_leaf) entries_per_leaf[k] = ne;
+    }
+    free(la);
+    free(lb);
+  }
+  seg_ptr[0] = 0;
+  for (int64_t k = 0; k < nleaves; ++k) seg_ptr[k + 1] = seg_ptr[k] + nsegs[k];
+  free(nsegs);
+  if (seg_ptr[nleaves] > cap) return -1;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t k = 0; k < nleaves; ++k) {
+    int64_t ne;
+    int64_t ns = hsro_list(p, t, nnodes, leaves[k], seg_a + seg_ptr[k],
+                           seg_b + seg_ptr[k], seg_ptr[k + 1] - seg_ptr[k],
+                           &ne);
+    if (ns != seg_ptr[k + 1] - seg_ptr[k]) bad = 1;
+  }
+  return bad ? -1 : seg_ptr[nleaves];
+}
+
+/* ------------------------------------------------------------ force laws */
+/* BGQStep16.c:40-45,167 (also ForceLaw.cxx:109-114, POLY_ORDER 5). */
+static const float P5[6] = {0.269327f, -0.0750978f, 0.0114808f,
+                            -0.00109313f, 0.0000605491f, -0.00000147177f};
+/* ForceLaw.cxx:98-104, POLY_ORDER 6 (stock). */
+static const float P6[7] = {0.271431f, -0.0783394f, 0.0133122f,
+                            -0.00159485f, 0.000132336f, -0.00000663394f,
+                            0.000000147305f};
+/* FGrid::m_rmax (ForceLaw.cxx:32) */
+static const float FGRID_RMAX = 3.116326355f;
+
+/* FGridEvalPoly::eval (ForceLaw.cxx:137-141) */
+static float poly_eval(const float *a, int order, float r2) {
+  float ret;
+  if (order == 6)
+    ret = a[0] + r2 * (a[1] + r2 * (a[2] + r2 * (a[3] + r2 * (a[4] +
+          r2 * (a[5] + r2 * a[6])))));
+  else
+    ret = a[0] + r2 * (a[1] + r2 * (a[2] + r2 * (a[3] + r2 * (a[4] +
+          r2 * (a[5] + r2 * 0.0f)))));
+  float r2max = FGRID_RMAX * FGRID_RMAX;
+  return ret * (float)(r2 >= 0.0f) * (float)(r2 <= r2max);
+}
+
+/* ForceLawSR::f_over_r (ForceLaw.cxx:187-192) */
+static float sr_f_over_r(int order, float r2, float rsm2) {
+  const float *a = order == 6 ? P6 : P5;
+  float ret = powf(r2 + rsm2, -1.5f) - poly_eval(a, order, r2);
+  float r2max = FGRID_RMAX * FGRID_RMAX;
+  ret *= (float)(r2 >= 0.0f) * (float)(r2 <= r2max);
+  return ret;
+}
+
+/* BGQStep16.c:172-182 scalar tail: f = pow(r2+rsm2, -1.5) - P5(r2) with the
+ * power in double and the polynomial in float; masked by r2 < fsrrmax2 and
+ * guarded by r2 > 0 (mass factor m_j applied by the caller). */
+static float step16_f(float r2, float rsm2, float fsrrmax2) {
+  float f = (float)(pow((double)(r2 + rsm2), -1.5) -
+                    (double)(P5[0] + r2 * (P5[1] + r2 * (P5[2] + r2 * (P5[3] +
+                             r2 * (P5[4] + r2 * P5[5]))))));
+  float msk = (r2 < fsrrmax2) ? 1.0f : 0.0f;
+  return (r2 > 0.0f) ? msk * f : 0.0f;
+}
+
+float hsro_law_f32(int law, float r2, float rsm, float fsrrmax) {
+  float rsm2 = rsm * rsm;
+  switch (law) {
+  case HSRO_LAW_STEP16: return step16_f(r2, rsm2, fsrrmax * fsrrmax);
+  case HSRO_LAW_SR_POLY5: return sr_f_over_r(5, r2, rsm2);
+  case HSRO_LAW_SR_POLY6: return sr_f_over_r(6, r2, rsm2);
+  case HSRO_LAW_NEWTON: return (float)(1.0 / (double)r2 / sqrt((double)r2));
+  }
+  return 0.0f;
+}
+
+/* ------------------------------------------------------------- kernels */
+/* Gather the list of one leaf into SoA (the reference's nx/ny/nz/nm). */
+static int64_t gather_list(const hsro_node *t, const float *x, const float *y,
+                           const float *z, const float *m, const int64_t *sa,
+                           const int64_t *sb, int64_t ns, float **bufs,
+                           int64_t *bcap) {
+  int64_t len = 0;
+  for (int64_t s = 0; s < ns; ++s) len += sb[s] >= 0 ? sb[s] : 1;
+  if (len > *bcap) {
+    *bcap = len * 2;
+    for (int k = 0; k < 4; ++k)
+      bufs[k] = (float *)realloc(bufs[k], sizeof(float) * (*bcap));
+  }
+  int64_t o = 0;
+  for (int64_t s = 0; s < ns; ++s) {
+    if (sb[s] >= 0) {
+      for (int64_t i = 0; i < sb[s]; ++i) {
+        bufs[0][o] = x[sa[s] + i];
+        bufs[1][o] = y[sa[s] + i];
+        bufs[2][o] = z[sa[s] + i];
+        bufs[3][o] = m[sa[s] + i];
+        ++o;
+      }
+    } else {
+      const hsro_node *n = &t[sa[s]];
+      pppts1(n->tdr, n->xc, &bufs[0][o], &bufs[1][o], &bufs[2][o]);
+      bufs[3][o] = n->ppm[0];
+      ++o;
+    }
+  }
+  return len;
+}
+
+void hsro_force_f32(int law, const hsro_params *p, const hsro_node *t,
+                    const float *x, const float *y, const float *z,
+                    const float *m, const int64_t *leaves, int64_t nleaves,
+                    const int64_t *seg_ptr, const int64_t *seg_a,
+                    const int64_t *seg_b, float *ax, float *ay, float *az) {
+  const float fsrrmax2 = p->fsrrmax * p->fsrrmax;
+  const float rsm2 = p->rsm * p->rsm;
+  const float fcoeff = 1.0f;
+#pragma omp parallel
+  {
+    float *bufs[4] = {0, 0, 0, 0};
+    int64_t bcap = 0;
+#pragma omp for schedule(dynamic, 4)
+    for (int64_t k = 0; k < nleaves; ++k) {
+      const hsro_node *n = &t[leaves[k]];
+      int64_t len = gather_list(t, x, y, z, m, seg_a + seg_ptr[k],
+                                seg_b + seg_ptr[k], seg_ptr[k + 1] - seg_ptr[k],
+                                bufs, &bcap);
+      for (int64_t i = n->offset; i < n->offset + n->count; ++i) {
+        float vx = 0, vy = 0, vz = 0;
+        for (int64_t j = 0; j < len; ++j) {
+          float dx = bufs[0][j] - x[i];
+          float dy = bufs[1][j] - y[i];
+          float dz = bufs[2][j] - z[i];
+          float dist2 = dx * dx + dy * dy + dz * dz;
+          float fl;
+          if (law == HSRO_LAW_STEP16) {
+            /* the Step16 law as a ForceLaw: the r2 > 0 guard of
+             * BGQStep16.c:182 replaces updateq's role for self pairs */
+            fl = step16_f(dist2, rsm2, fsrrmax2);
+          } else {
+            fl = hsro_law_f32(law, dist2, p->rsm, p->fsrrmax);
+          }
+          float f_over_r = m[i] * bufs[3][j] * fl;
+          float updateq = 1.0f;
+          updateq *= (float)(dist2 < fsrrmax2);
+          vx += updateq * fcoeff * f_over_r * dx;
+          vy += updateq * fcoeff * f_over_r * dy;
+          vz += updateq * fcoeff * f_over_r * dz;
+        }
+        ax[i] = vx;
+        ay[i] = vy;
+        az[i] = vz;
+      }
+    }
+    for (int k = 0; k < 4; ++k) free(bufs[k]);
+  }
+}
+
+void hsro_force_step16(const hsro_params *p, const hsro_node *t,
+                       const float *x, const float *y, const float *z,
+                       const float *m, const int64_t *leaves, int64_t nleaves,
+                       const int64_t *seg_ptr, const int64_t *seg_a,
+                       const int64_t *seg_b, float *ax, float *ay, float *az) {
+  const float fsrrmax2 = p->fsrrmax * p->fsrrmax;
+  const float rsm2 = p->rsm * p->rsm;
+#pragma omp parallel
+  {
+    float *bufs[4] = {0, 0, 0, 0};
+    int64_t bcap = 0;
+#pragma omp for schedule(dynamic, 4)
+    for (int64_t k = 0; k < nleaves; ++k) {
+      const hsro_node *n = &t[leaves[k]];
+      int64_t len = gather_list(t, x, y, z, m, seg_a + seg_ptr[k],
+                                seg_b + seg_ptr[k], seg_ptr[k + 1] - seg_ptr[k],
+                                bufs, &bcap);
+      for (int64_t i = n->offset; i < n->offset + n->count; ++i) {
+        float sx = 0, sy = 0, sz = 0;
+        for (int64_t j = 0; j < len; ++j) {
+          float dxc = bufs[0][j] - x[i];
+          float dyc = bufs[1][j] - y[i];
+          float dzc = bufs[2][j] - z[i];
+          float r2 = dxc * dxc + dyc * dyc + dzc * dzc;
+          float mm = (r2 < fsrrmax2) ? bufs[3][j] : 0.0f;
+          float f = (float)(pow((double)(r2 + rsm2), -1.5) -
+                            (double)(P5[0] + r2 * (P5[1] + r2 * (P5[2] + r2 *
+                            (P5[3] + r2 * (P5[4] + r2 * P5[5]))))));
+          f = (r2 > 0.0f) ? mm * f : 0.0f;
+          sx = sx + f * dxc;
+          sy = sy + f * dyc;
+          sz = sz + f * dzc;
+        }
+        ax[i] = sx;
+        ay[i] = sy;
+        az[i] = sz;
+      }
+    }
+    for (int k = 0; k < 4; ++k) free(bufs[k]);
+  }
+}
+
+static double law_f64(double r2, double rsm2, double rc2) {
+  if (!(r2 > 0.0) || !(r2 < rc2)) return 0.0;
+  double s = r2 + rsm2;
+  double poly = (double)P5[0] + r2 * ((double)P5[1] + r2 * ((double)P5[2] +
+                r2 * ((double)P5[3] + r2 * ((double)P5[4] + r2 * (double)P5[5]))));
+  return 1.0 / (s * sqrt(s)) - poly;
+}
+
+void hsro_force_f64(const hsro_params *p, const hsro_node *t,
+                    const float *x, const float *y, const float *z,
+                    const float *m, const int64_t *leaves, int64_t nleaves,
+                    const int64_t *seg_ptr, const int64_t *seg_a,
+                    const int64_t *seg_b, double *ax, double *ay, double *az,
+                    double *leaf_mom) {
+  const double rc2 = (double)(p->fsrrmax * p->fsrrmax);
+  const double rsm2 = (double)p->rsm * (double)p->rsm;
+#pragma omp parallel
+  {
+    float *bufs[4] = {0, 0, 0, 0};
+    int64_t bcap = 0;
+#pragma omp for schedule(dynamic, 4)
+    for (int64_t k = 0; k < nleaves; ++k) {
+      const hsro_node *n = &t[leaves[k]];
+      int64_t len = gather_list(t, x, y, z, m, seg_a + seg_ptr[k],
+                                seg_b + seg_ptr[k], seg_ptr[k + 1] - seg_ptr[k],
+                                bufs, &bcap);
+      double mx = 0, my = 0, mz = 0;
+      for (int64_t i = n->offset; i < n->offset + n->count; ++i) {
+        double sx = 0, sy = 0, sz = 0;
+        for (int64_t j = 0; j < len; ++j) {
+          double dx = (double)bufs[0][j] - (double)x[i];
+          double dy = (double)bufs[1][j] - (double)y[i];
+          double dz = (double)bufs[2][j] - (double)z[i];
+          double r2 = dx * dx + dy * dy + dz * dz;
+          double f = (double)bufs[3][j] * law_f64(r2, rsm2, rc2);
+          sx += f * dx;
+          sy += f * dy;
+          sz += f * dz;
+        }
+        ax[i] = sx;
+        ay[i] = sy;
+        az[i] = sz;
+        mx += (double)m[i] * sx;
+        my += (double)m[i] * sy;
+        mz += (double)m[i] * sz;
+      }
+      if (leaf_mom) {
+        leaf_mom[3 * k + 0] = mx;
+        leaf_mom[3 * k + 1] = my;
+        leaf_mom[3 * k + 2] = mz;
+      }
+    }
+    for (int k = 0; k < 4; ++k) free(bufs[k]);
+  }
+}
+
+void hsro_direct_f64(int64_t n, const float *x, const float *y, const float *z,
+                     const float *m, const int64_t *targets, int64_t ntargets,
+                     float rsm, float fsrrmax, double *ax, double *ay,
+                     double *az) {
+  const double rc2 = (double)(fsrrmax * fsrrmax);
+  const double rsm2 = (double)rsm * (double)rsm;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t k = 0; k < ntargets; ++k) {
+    int64_t i = targets[k];
+    double sx = 0, sy = 0, sz = 0;
+    for (int64_t j = 0; j < n; ++j) {
+      double dx = (double)x[j] - (double)x[i];
+      double dy = (double)y[j] - (double)y[i];
+      double dz = (double)z[j] - (double)z[i];
+      double r2 = dx * dx + dy * dy + dz * dz;
+      double f = (double)m[j] * law_f64(r2, rsm2, rc2);
+      sx += f * dx;
+      sy += f * dy;
+      sz += f * dz;
+    }
+    ax[k] = sx;
+    ay[k] = sy;
+    az[k] = sz;
+  }
+}
+
+/* ------------------------------------------------------------ cell sort */
+/* Particles.cxx:402-488: stable counting sort by the unit cell
+ * (floorf(x), floorf(y), floorf(z)); index (xx*ng1 + yy)*ng2 + zz; cells
+ * outside [0, ng) go to a tail bucket Ng. */
+int64_t hsro_cell_sort_perm(int64_t n, const float *x, const float *y,
+                            const float *z, const int32_t ng[3],
+                            int64_t *perm) {
+  int64_t Ng = (int64_t)ng[0] * ng[1] * ng[2];
+  int64_t *cnt = (int64_t *)calloc((size_t)(Ng + 1), sizeof(int64_t));
+  int64_t *ind = (int64_t *)malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+  for (int64_t i = 0; i < n; ++i) {
+    int xx = (int)floorf(x[i]), yy = (int)floorf(y[i]), zz = (int)floorf(z[i]);
+    int64_t c;
+    if (xx >= 0 && xx < ng[0] && yy >= 0 && yy < ng[1] && zz >= 0 &&
+        zz < ng[2])
+      c = ((int64_t)xx * ng[1] + yy) * ng[2] + zz;
+    else
+      c = Ng;
+    ind[i] = c;
+    cnt[c]++;
+  }
+  int64_t inside = n - cnt[Ng];
+  int64_t cum = n;
+  for (int64_t i = Ng; i >= 0; --i) {
+    int64_t tmp = cnt[i];
+    cnt[i] = cum - tmp;
+    cum -= tmp;
+  }
+  for (int64_t i = 0; i < n; ++i) perm[i] = cnt[ind[i]]++;
+  free(cnt);
+  free(ind);
+  return inside;
+}

@@ -1,0 +1,155 @@
+/*
+ * hsr_oracle.h -- CPU restatement of HACC's short-range RCB force path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product library never links it.
+ *
+ * Every function restates the reference algorithm (BenWibking/hacc-coral,
+ * src/halo_finder) in plain C with the same float/double operation order:
+ *   hsro_cm              BGQCM.c:181-212 (generic branch)
+ *   hsro_partition       RCBForceTree.cxx:623-672
+ *   hsro_build           RCBForceTree.cxx:690-918 (unthreaded build order)
+ *   hsro_leaf_order      RCBForceTree.cxx:1151-1197
+ *   hsro_list            RCBForceTree.cxx:924-1140
+ *   hsro_force_*         RCBForceTree.cxx:575-620 / BGQStep16.c:167-187
+ *
+ * Parity pins (see tests/test_oracle_*.py):
+ *   cm            bit-exact against the reference's own BGQCM.c compiled
+ *                 into oracle/_ref, on every node range of built trees.
+ *   force laws    bit-exact against the reference's ForceLaw.cxx (_ref).
+ *   partition     stable-filter + swap-chain invariants, brute-force replay.
+ *   tree          tiling/tight-bbox/split-rule/leaf-size invariants.
+ *   lists         mass-accounting invariant and oa->0 direct-sum limit.
+ *   kernels       FP64 brute force over the same lists.
+ */
+#ifndef HSR_ORACLE_H
+#define HSR_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HSRO_MAX_TDPTS 1
+
+/* Node layout follows RCBForceTree<TDPTS>::TreeNode (RCBForceTree.h:131-147)
+ * with TDPTS = 1 (RCBMonopoleForceTree). */
+typedef struct {
+  int64_t count, offset, cl, cr;
+  float ppm[HSRO_MAX_TDPTS];
+  float tdr;
+  float xmin[3], xmax[3], xc[3];
+} hsro_node;
+
+typedef struct {
+  int64_t nd;          /* leaf size: nodes with count <= nd are leaves */
+  float fsrrmax;       /* short-range cutoff */
+  float rsm;           /* Plummer softening */
+  float tan_oa;        /* tanf(opening angle), evaluated on the host */
+  float ppc;           /* pseudo-particle contraction factor (0.9) */
+  float min_loc[3], max_loc[3];     /* tree box (root bbox seed) */
+  float min_force[3], max_force[3]; /* force box */
+} hsro_params;
+
+/* force-law kinds */
+enum {
+  HSRO_LAW_STEP16 = 0,   /* BGQStep16.c:167-187: P5 law, r2>0 guard, pow() in double */
+  HSRO_LAW_SR_POLY5 = 1, /* ForceLawSR + FGridEvalPoly, POLY_ORDER 5 (ForceLaw.cxx:107-116,137-141,187-192) */
+  HSRO_LAW_SR_POLY6 = 2, /* ForceLawSR + FGridEvalPoly, POLY_ORDER 6 (stock build, ForceLaw.cxx:96-105) */
+  HSRO_LAW_NEWTON = 3    /* ForceLawNewton (ForceLaw.h:105-110) */
+};
+
+/* BGQCM.c:181-212, generic branch. */
+void hsro_cm(int64_t count, const float *xx, const float *yy, const float *zz,
+             const float *mass, float *xmin, float *xmax, float *xc);
+
+/* RCBForceTree.cxx:623-672; returns is (number of keys < pv). The first
+ * three arrays are the key-rotated coordinate arrays. */
+int64_t hsro_partition(int64_t n, float *xx, float *yy, float *zz, float *vx,
+                       float *vy, float *vz, float *mass, float *phi,
+                       int64_t *id, uint16_t *mask, float pv);
+
+/* Full unthreaded build (RCBForceTree.cxx:893-918).  Permutes all ten arrays
+ * in place.  nodes must hold cap entries; returns the node count or -1 when
+ * cap is too small. */
+int64_t hsro_build(const hsro_params *p, int64_t n, float *x, float *y,
+                   float *z, float *vx, float *vy, float *vz, float *mass,
+                   float *phi, int64_t *id, uint16_t *mask, hsro_node *nodes,
+                   int64_t cap);
+
+/* Leaves inside the force box in the order calcInternodeForces visits them
+ * (RCBForceTree.cxx:1151-1197).  Writes node ids and the ancestor chain
+ * needed by hsro_list into out; returns the count (or -1 if cap too small). */
+int64_t hsro_leaf_order(const hsro_params *p, const hsro_node *nodes,
+                        int64_t nnodes, int64_t *out, int64_t cap);
+
+/* Interaction list of target leaf tl (RCBForceTree.cxx:924-1140), as
+ * segments: seg_a[k] = particle offset and seg_b[k] = count for a particle
+ * range, or seg_a[k] = node id and seg_b[k] = -1 for a pseudo-particle.
+ * The target's own particles are the last segment.  Returns the segment
+ * count (or -1 when cap is too small); *n_entries receives the list length. */
+int64_t hsro_list(const hsro_params *p, const hsro_node *nodes,
+                  int64_t nnodes, int64_t tl, int64_t *seg_a, int64_t *seg_b,
+                  int64_t cap, int64_t *n_entries);
+
+/* All lists for the leaves in `leaves` (CSR over segments). seg_ptr has
+ * nleaves+1 entries.  Returns total segments or -1. Uses OpenMP. */
+int64_t hsro_lists_all(const hsro_params *p, const hsro_node *nodes,
+                       int64_t nnodes, const int64_t *leaves, int64_t nleaves,
+                       int64_t *seg_ptr, int64_t *seg_a, int64_t *seg_b,
+                       int64_t cap, int64_t *entries_per_leaf);
+
+/* Float force of the generic kernel nbody1 (RCBForceTree.cxx:599-619) with
+ * fcoeff = 1 and v = 0, i.e. the per-pair accumulation of the reference's
+ * float CPU path, for every listed leaf.  ax/ay/az are indexed by sorted
+ * particle position; entries of non-target particles are untouched. */
+void hsro_force_f32(int law, const hsro_params *p, const hsro_node *nodes,
+                    const float *x, const float *y, const float *z,
+                    const float *m, const int64_t *leaves, int64_t nleaves,
+                    const int64_t *seg_ptr, const int64_t *seg_a,
+                    const int64_t *seg_b, float *ax, float *ay, float *az);
+
+/* Step16 structure (BGQStep16.c:167-187): per-i accumulator a, then
+ * v += fcoeff * a.  Same lists. */
+void hsro_force_step16(const hsro_params *p, const hsro_node *nodes,
+                       const float *x, const float *y, const float *z,
+                       const float *m, const int64_t *leaves, int64_t nleaves,
+                       const int64_t *seg_ptr, const int64_t *seg_a,
+                       const int64_t *seg_b, float *ax, float *ay, float *az);
+
+/* FP64 evaluation of the north-star law over the same lists (inputs are the
+ * float positions promoted to double).  Also returns, per listed leaf, the
+ * FP64 momentum sum sum_i m_i F_i (3 doubles per leaf) if leaf_mom != 0. */
+void hsro_force_f64(const hsro_params *p, const hsro_node *nodes,
+                    const float *x, const float *y, const float *z,
+                    const float *m, const int64_t *leaves, int64_t nleaves,
+                    const int64_t *seg_ptr, const int64_t *seg_a,
+                    const int64_t *seg_b, double *ax, double *ay, double *az,
+                    double *leaf_mom);
+
+/* Scalar law f(r2) as used by the kernels above, for pinning against the
+ * reference's ForceLaw.cxx (returns f_over_r without the mass factors and,
+ * for STEP16, with the r2 < fsrrmax^2 mask and r2 > 0 guard applied). */
+float hsro_law_f32(int law, float r2, float rsm, float fsrrmax);
+
+/* Brute-force FP64 short-range force on particle i from all particles j with
+ * 0 < r2 < fsrrmax^2 (the oa -> 0 limit of the tree). */
+void hsro_direct_f64(int64_t n, const float *x, const float *y, const float *z,
+                     const float *m, const int64_t *targets, int64_t ntargets,
+                     float rsm, float fsrrmax, double *ax, double *ay,
+                     double *az);
+
+/* Stable counting sort by unit cell (Particles.cxx:402-488): returns the
+ * permutation perm[i] = destination of particle i and the number of
+ * particles inside the ng^3 grid (the rest go to the tail). */
+int64_t hsro_cell_sort_perm(int64_t n, const float *x, const float *y,
+                            const float *z, const int32_t ng[3],
+                            int64_t *perm);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

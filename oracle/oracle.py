@@ -1,0 +1,321 @@
+"""ctypes front end of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg may import this.
+
+Each wrapper follows the C function of the same name in hsr_oracle.c, which
+in turn cites the reference file:line it restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import math
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libhsrref.so")
+
+LAW_STEP16, LAW_SR_POLY5, LAW_SR_POLY6, LAW_NEWTON = 0, 1, 2, 3
+
+
+class Node(C.Structure):
+    _fields_ = [("count", C.c_int64), ("offset", C.c_int64),
+                ("cl", C.c_int64), ("cr", C.c_int64),
+                ("ppm", C.c_float * 1), ("tdr", C.c_float),
+                ("xmin", C.c_float * 3), ("xmax", C.c_float * 3),
+                ("xc", C.c_float * 3)]
+
+
+NODE_DTYPE = np.dtype([("count", "<i8"), ("offset", "<i8"), ("cl", "<i8"),
+                       ("cr", "<i8"), ("ppm", "<f4", (1,)), ("tdr", "<f4"),
+                       ("xmin", "<f4", (3,)), ("xmax", "<f4", (3,)),
+                       ("xc", "<f4", (3,))], align=True)
+assert NODE_DTYPE.itemsize == C.sizeof(Node)
+
+
+class Params(C.Structure):
+    _fields_ = [("nd", C.c_int64), ("fsrrmax", C.c_float), ("rsm", C.c_float),
+                ("tan_oa", C.c_float), ("ppc", C.c_float),
+                ("min_loc", C.c_float * 3), ("max_loc", C.c_float * 3),
+                ("min_force", C.c_float * 3), ("max_force", C.c_float * 3)]
+
+
+_lib = None
+_ref = None
+
+
+def build():
+    """Compile liboracle.so (and _ref/libhsrref.so when the reference tree
+    is present)."""
+    subprocess.check_call(["make", "-s", "-C", HERE, "all"])
+    if os.path.isdir("/root/reference/src/halo_finder"):
+        subprocess.check_call(["make", "-s", "-C", HERE, "ref"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = np.ctypeslib.ndpointer
+        f32 = P(np.float32, flags="C")
+        f64 = P(np.float64, flags="C")
+        i64 = P(np.int64, flags="C")
+        u16 = P(np.uint16, flags="C")
+        nodes = P(NODE_DTYPE, flags="C")
+        L.hsro_cm.argtypes = [C.c_int64, f32, f32, f32, f32, f32, f32, f32]
+        L.hsro_partition.argtypes = [C.c_int64] + [f32] * 8 + [i64, u16, C.c_float]
+        L.hsro_partition.restype = C.c_int64
+        L.hsro_build.argtypes = [C.POINTER(Params), C.c_int64] + [f32] * 8 + [i64, u16, nodes, C.c_int64]
+        L.hsro_build.restype = C.c_int64
+        L.hsro_leaf_order.argtypes = [C.POINTER(Params), nodes, C.c_int64, i64, C.c_int64]
+        L.hsro_leaf_order.restype = C.c_int64
+        L.hsro_list.argtypes = [C.POINTER(Params), nodes, C.c_int64, C.c_int64, i64, i64, C.c_int64, C.POINTER(C.c_int64)]
+        L.hsro_list.restype = C.c_int64
+        L.hsro_lists_all.argtypes = [C.POINTER(Params), nodes, C.c_int64, i64, C.c_int64, i64, i64, i64, C.c_int64, i64]
+        L.hsro_lists_all.restype = C.c_int64
+        L.hsro_force_f32.argtypes = [C.c_int, C.POINTER(Params), nodes, f32, f32, f32, f32, i64, C.c_int64, i64, i64, i64, f32, f32, f32]
+        L.hsro_force_step16.argtypes = [C.POINTER(Params), nodes, f32, f32, f32, f32, i64, C.c_int64, i64, i64, i64, f32, f32, f32]
+        L.hsro_force_f64.argtypes = [C.POINTER(Params), nodes, f32, f32, f32, f32, i64, C.c_int64, i64, i64, i64, f64, f64, f64, C.c_void_p]
+        L.hsro_law_f32.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float]
+        L.hsro_law_f32.restype = C.c_float
+        L.hsro_direct_f64.argtypes = [C.c_int64, f32, f32, f32, f32, i64, C.c_int64, C.c_float, C.c_float, f64, f64, f64]
+        L.hsro_cell_sort_perm.argtypes = [C.c_int64, f32, f32, f32, P(np.int32, flags="C"), i64]
+        L.hsro_cell_sort_perm.restype = C.c_int64
+        _lib = L
+    return _lib
+
+
+def ref():
+    """The reference's own ForceLaw.cxx + BGQCM.c (oracle/_ref), or None
+    when it was not built (it is only buildable where the reference tree
+    is mounted)."""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_PATH):
+            return None
+        R = C.CDLL(REF_PATH)
+        P = np.ctypeslib.ndpointer
+        f32 = P(np.float32, flags="C")
+        R.hsrref_cm.argtypes = [C.c_int64] + [f32] * 7
+        for name in ("hsrref_sr_poly_f_over_r", "hsrref_sr_fit_f_over_r"):
+            getattr(R, name).argtypes = [C.c_float, C.c_float]
+            getattr(R, name).restype = C.c_float
+        for name in ("hsrref_fgor_r2", "hsrref_poly_eval", "hsrref_newton_f_over_r"):
+            getattr(R, name).argtypes = [C.c_float]
+            getattr(R, name).restype = C.c_float
+        R.hsrref_fgrid_rmax.restype = C.c_float
+        _ref = R
+    return _ref
+
+
+@dataclasses.dataclass
+class TreeParams:
+    """Constructor arguments of RCBForceTree (RCBForceTree.h:96-124) that
+    shape the tree, the lists and the kernel."""
+    min_loc: np.ndarray
+    max_loc: np.ndarray
+    min_force: np.ndarray
+    max_force: np.ndarray
+    fsrrmax: float = 3.12
+    rsm: float = 0.1
+    oa: float = 0.1
+    nd: int = 128
+    ppc: float = 0.9
+
+    def tan_oa(self) -> float:
+        # tanf(oa) in float, as RCBForceTree.cxx:381 evaluates it
+        return float(np.float32(math.tan(float(np.float32(self.oa)))))
+
+    def ctypes(self) -> Params:
+        p = Params()
+        p.nd = int(self.nd)
+        p.fsrrmax = float(np.float32(self.fsrrmax))
+        p.rsm = float(np.float32(self.rsm))
+        p.tan_oa = tanf(self.oa)
+        p.ppc = float(np.float32(self.ppc))
+        for k in range(3):
+            p.min_loc[k] = float(self.min_loc[k])
+            p.max_loc[k] = float(self.max_loc[k])
+            p.min_force[k] = float(self.min_force[k])
+            p.max_force[k] = float(self.max_force[k])
+        return p
+
+
+def tanf(x: float) -> float:
+    """tanf as glibc evaluates it (the reference calls tanf(oa) once,
+    RCBForceTree.cxx:381)."""
+    libm = C.CDLL("libm.so.6")
+    libm.tanf.argtypes = [C.c_float]
+    libm.tanf.restype = C.c_float
+    return float(libm.tanf(C.c_float(x)))
+
+
+@dataclasses.dataclass
+class Tree:
+    nodes: np.ndarray      # NODE_DTYPE, canonical (unthreaded allocation) ids
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    vx: np.ndarray
+    vy: np.ndarray
+    vz: np.ndarray
+    mass: np.ndarray
+    phi: np.ndarray
+    id: np.ndarray
+    mask: np.ndarray
+
+
+def build_tree(tp: TreeParams, x, y, z, mass=None, vx=None, vy=None, vz=None,
+               phi=None, ids=None, mask=None) -> Tree:
+    """RCBForceTree constructor's build phase on copies of the inputs."""
+    n = x.shape[0]
+    f = lambda a, d=0.0: (np.full(n, d, np.float32) if a is None
+                          else np.ascontiguousarray(a, np.float32).copy())
+    X, Y, Z = f(x), f(y), f(z)
+    M = f(mass, 1.0)
+    VX, VY, VZ, PHI = f(vx), f(vy), f(vz), f(phi)
+    ID = np.arange(n, dtype=np.int64) if ids is None else np.ascontiguousarray(ids, np.int64).copy()
+    MASK = np.zeros(n, np.uint16) if mask is None else np.ascontiguousarray(mask, np.uint16).copy()
+    cap = max(16, 4 * (n // max(1, tp.nd // 2) + 2) + 64)
+    while True:
+        nodes = np.zeros(cap, NODE_DTYPE)
+        p = tp.ctypes()
+        # rebuild from pristine inputs each attempt
+        X2, Y2, Z2, M2 = X.copy(), Y.copy(), Z.copy(), M.copy()
+        VX2, VY2, VZ2, PHI2, ID2, MASK2 = VX.copy(), VY.copy(), VZ.copy(), PHI.copy(), ID.copy(), MASK.copy()
+        nn = lib().hsro_build(C.byref(p), n, X2, Y2, Z2, VX2, VY2, VZ2, M2,
+                              PHI2, ID2, MASK2, nodes, cap)
+        if nn >= 0:
+            return Tree(nodes[:nn].copy(), X2, Y2, Z2, VX2, VY2, VZ2, M2,
+                        PHI2, ID2, MASK2)
+        cap *= 2
+
+
+def leaf_order(tp: TreeParams, tree: Tree) -> np.ndarray:
+    nn = tree.nodes.shape[0]
+    out = np.zeros(nn + 1, np.int64)
+    p = tp.ctypes()
+    k = lib().hsro_leaf_order(C.byref(p), tree.nodes, nn, out, out.shape[0])
+    assert k >= 0
+    return out[:k].copy()
+
+
+@dataclasses.dataclass
+class Lists:
+    leaves: np.ndarray     # target leaf node ids, in leaf order
+    seg_ptr: np.ndarray    # CSR over segments (len nleaves+1)
+    seg_a: np.ndarray      # particle offset, or node id for a pseudo-particle
+    seg_b: np.ndarray      # particle count, or -1 for a pseudo-particle
+    entries: np.ndarray    # list length per leaf
+
+    def pairs(self, tree: Tree) -> int:
+        return int(np.sum(tree.nodes["count"][self.leaves] * self.entries))
+
+
+def build_lists(tp: TreeParams, tree: Tree, leaves=None) -> Lists:
+    if leaves is None:
+        leaves = leaf_order(tp, tree)
+    leaves = np.ascontiguousarray(leaves, np.int64)
+    nl = leaves.shape[0]
+    seg_ptr = np.zeros(nl + 1, np.int64)
+    ent = np.zeros(max(nl, 1), np.int64)
+    p = tp.ctypes()
+    cap = max(64, nl * 64)
+    while True:
+        sa = np.zeros(cap, np.int64)
+        sb = np.zeros(cap, np.int64)
+        tot = lib().hsro_lists_all(C.byref(p), tree.nodes, tree.nodes.shape[0],
+                                   leaves, nl, seg_ptr, sa, sb, cap, ent)
+        if tot >= 0:
+            return Lists(leaves, seg_ptr.copy(), sa[:tot].copy(), sb[:tot].copy(), ent[:nl].copy())
+        cap = int(seg_ptr[nl]) + 16
+
+
+def forces_f32(tp: TreeParams, tree: Tree, lists: Lists, law: int = LAW_STEP16):
+    n = tree.x.shape[0]
+    ax = np.zeros(n, np.float32); ay = np.zeros(n, np.float32); az = np.zeros(n, np.float32)
+    p = tp.ctypes()
+    lib().hsro_force_f32(law, C.byref(p), tree.nodes, tree.x, tree.y, tree.z,
+                         tree.mass, lists.leaves, lists.leaves.shape[0],
+                         lists.seg_ptr, lists.seg_a, lists.seg_b, ax, ay, az)
+    return ax, ay, az
+
+
+def forces_step16(tp: TreeParams, tree: Tree, lists: Lists):
+    n = tree.x.shape[0]
+    ax = np.zeros(n, np.float32); ay = np.zeros(n, np.float32); az = np.zeros(n, np.float32)
+    p = tp.ctypes()
+    lib().hsro_force_step16(C.byref(p), tree.nodes, tree.x, tree.y, tree.z,
+                            tree.mass, lists.leaves, lists.leaves.shape[0],
+                            lists.seg_ptr, lists.seg_a, lists.seg_b, ax, ay, az)
+    return ax, ay, az
+
+
+def forces_f64(tp: TreeParams, tree: Tree, lists: Lists, with_momentum=False):
+    n = tree.x.shape[0]
+    ax = np.zeros(n); ay = np.zeros(n); az = np.zeros(n)
+    mom = np.zeros((lists.leaves.shape[0], 3)) if with_momentum else None
+    p = tp.ctypes()
+    lib().hsro_force_f64(C.byref(p), tree.nodes, tree.x, tree.y, tree.z,
+                         tree.mass, lists.leaves, lists.leaves.shape[0],
+                         lists.seg_ptr, lists.seg_a, lists.seg_b, ax, ay, az,
+                         mom.ctypes.data if with_momentum else None)
+    return (ax, ay, az, mom) if with_momentum else (ax, ay, az)
+
+
+def direct_f64(x, y, z, m, targets, rsm=0.1, fsrrmax=3.12):
+    targets = np.ascontiguousarray(targets, np.int64)
+    k = targets.shape[0]
+    ax = np.zeros(k); ay = np.zeros(k); az = np.zeros(k)
+    lib().hsro_direct_f64(x.shape[0], np.ascontiguousarray(x, np.float32),
+                          np.ascontiguousarray(y, np.float32),
+                          np.ascontiguousarray(z, np.float32),
+                          np.ascontiguousarray(m, np.float32), targets, k,
+                          float(np.float32(rsm)), float(np.float32(fsrrmax)),
+                          ax, ay, az)
+    return ax, ay, az
+
+
+def law_f32(law: int, r2, rsm=0.1, fsrrmax=3.12):
+    L = lib()
+    return np.array([L.hsro_law_f32(law, float(v), float(np.float32(rsm)),
+                                    float(np.float32(fsrrmax)))
+                     for v in np.asarray(r2, np.float32)], np.float32)
+
+
+def cm(x, y, z, m):
+    xmin = np.zeros(3, np.float32); xmax = np.zeros(3, np.float32); xc = np.zeros(3, np.float32)
+    lib().hsro_cm(x.shape[0], np.ascontiguousarray(x, np.float32),
+                  np.ascontiguousarray(y, np.float32),
+                  np.ascontiguousarray(z, np.float32),
+                  np.ascontiguousarray(m, np.float32), xmin, xmax, xc)
+    return xmin, xmax, xc
+
+
+def partition(key, pv, *others):
+    """hsro_partition on copies; returns (is, key', others'...)."""
+    n = key.shape[0]
+    arrs = [np.ascontiguousarray(key, np.float32).copy()]
+    fl = [np.ascontiguousarray(o, np.float32).copy() for o in others[:7]]
+    while len(fl) < 7:
+        fl.append(np.zeros(n, np.float32))
+    ids = np.arange(n, dtype=np.int64)
+    mask = np.zeros(n, np.uint16)
+    is_ = lib().hsro_partition(n, arrs[0], *fl, ids, mask, float(np.float32(pv)))
+    return is_, arrs[0], ids
+
+
+def cell_sort_perm(x, y, z, ng):
+    perm = np.zeros(x.shape[0], np.int64)
+    inside = lib().hsro_cell_sort_perm(x.shape[0], np.ascontiguousarray(x, np.float32),
+                                       np.ascontiguousarray(y, np.float32),
+                                       np.ascontiguousarray(z, np.float32),
+                                       np.asarray(ng, np.int32), perm)
+    return perm, inside
