@@ -1,0 +1,206 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X short-range force path (one GPU).
+
+One step = one pass of the hot path over the resident 256^3 clustered load:
+restore the unsorted particle arrays (device copy), RCB tree build (sorting
+all ten arrays into leaf order), interaction lists of the leaves inside the
+force box, and the leaf kernel applying the short-range kick.  The metric is
+list pairs per second (pairs = sum over listed leaves of n_leaf * L_leaf, the
+reference's own unit of work: RCBForceTree.cxx:1143, 604-605).
+
+Prints one JSON line (see the repository's benchmark contract), with the
+leaf kernel's roofline against the FP32 vector peak and the CPU restatement
+of the reference kernel timed on the host as the baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "hacc-coral_amd"))
+
+FLOPS_PER_PAIR = 30          # d 3, r2 5, +eps2 1, rsqrt 1, cube 2, P5 10, sub 1, *m 1, acc 6
+PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 vector peak (packed); non-packed 78.6
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="B", help="B (256^3 clustered), A (64^3), D (128^3)")
+    ap.add_argument("--n-side", type=int, default=None, help="override the load's n^(1/3)")
+    ap.add_argument("--nd", type=int, default=0, help="leaf size (0 = autotuned)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run one plain device-to-device copy of BYTES (the "
+                         "reference kernel for scaling FETCH_SIZE/WRITE_SIZE counters)")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def workload_name(cfg, load, nd):
+    names = {"A": "64^3 Plummer mix (70% uniform, 30% in 20 clumps)",
+             "B": "256^3 clustered (GRF P(k)~k^-2, Zel'dovich rms 2, 500 NFW halos)",
+             "D": "128^3 clustered (GRF P(k)~k^-2, Zel'dovich rms 2, 62 NFW halos)"}
+    return (f"{names.get(cfg, cfg)}, box {load.box:g}, +{load.n - load.n_alive} ghosts "
+            f"(shell {load.overload:g}), eps 0.1, fsrrmax 3.12, oa 0.1, nd {nd}")
+
+
+def cpu_baseline(load, nd, seconds):
+    """The oracle (CPU restatement of the reference's float kernel, OpenMP
+    over leaves) on a sample of the same tree's leaves."""
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    lo, hi = load.tree_box
+    flo, fhi = load.force_box
+    tp = O.TreeParams(lo, hi, flo, fhi, nd=nd)
+    t0 = time.perf_counter()
+    tree = O.build_tree(tp, load.x, load.y, load.z)
+    t_tree = time.perf_counter() - t0
+    leaves = O.leaf_order(tp, tree)
+    rng = np.random.default_rng(7)
+    # grow a random leaf sample until its kernel time reaches the budget
+    order = rng.permutation(leaves.shape[0])
+    k = 64
+    done_pairs, done_t = 0, 0.0
+    used = 0
+    while used < order.shape[0] and done_t < seconds:
+        take = order[used:used + k]
+        used += take.shape[0]
+        sub = np.sort(leaves[take])
+        L = O.build_lists(tp, tree, sub)
+        t0 = time.perf_counter()
+        O.forces_f32(tp, tree, L)
+        done_t += time.perf_counter() - t0
+        done_pairs += L.pairs(tree)
+        k = min(k * 2, 4096)
+    rate = done_pairs / done_t
+    return {"value": rate, "unit": "pairs/s", "cores": threads, "kind": "port",
+            "sample": (f"oracle float kernel (nbody1 restatement, powf law) on {used} random "
+                       f"listed leaves of the same tree ({done_pairs:.3e} pairs in {done_t:.1f} s, "
+                       f"{threads} OpenMP threads); oracle tree build of the whole load took "
+                       f"{t_tree:.1f} s single-threaded")}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: the short-range path is "
+                          "per-rank with no communication inside it; see DESIGN.md for "
+                          "how it shards", "n_gpus": a.gpus}), flush=True)
+        sys.exit(2)
+    import torch
+    from haccsr import Context, autotune, ics
+
+    if a.calibrate_copy:
+        src = torch.ones(a.calibrate_copy // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copied {a.calibrate_copy} bytes")
+        return
+
+    t0 = time.perf_counter()
+    load = ics.config_load(a.config, n_side=a.n_side)
+    log(f"load {a.config}: {load.n} particles ({load.n_alive} alive) in {time.perf_counter() - t0:.1f} s")
+    n = load.n
+    dev = torch.device("cuda:0")
+    lo, hi = load.tree_box
+    flo, fhi = load.force_box
+
+    def T(arr):
+        return torch.from_numpy(np.ascontiguousarray(arr)).to(dev)
+    src = {"x": T(load.x), "y": T(load.y), "z": T(load.z),
+           "m": T(np.ones(n, np.float32)), "vx": T(np.zeros(n, np.float32)),
+           "vy": T(np.zeros(n, np.float32)), "vz": T(np.zeros(n, np.float32)),
+           "phi": T(np.zeros(n, np.float32)), "id": T(np.arange(n, dtype=np.int64)),
+           "mask": torch.zeros(n, dtype=torch.int16, device=dev)}
+    work = {k: v.clone() for k, v in src.items()}
+    fcoeff = 1.0
+    ctxs = {}
+
+    def step(nd):
+        ctx = ctxs.get(nd)
+        if ctx is None:
+            ctx = ctxs[nd] = Context(nd=nd)
+        for k in src:
+            work[k].copy_(src[k])
+        torch.cuda.synchronize()
+        ctx.build(work["x"], work["y"], work["z"], work["m"], work["vx"], work["vy"],
+                  work["vz"], work["phi"], work["id"], work["mask"],
+                  min_loc=lo, max_loc=hi, device_ptrs=True)
+        ctx.lists(flo, fhi)
+        ctx.kick(fcoeff)
+        return ctx.stats()
+
+    nd = a.nd
+    if nd <= 0:
+        nd, _ = autotune.tune_leaf_size(step, log=log)
+        log(f"autotuned leaf size nd={nd}")
+    for k in list(ctxs):
+        if k != nd:
+            ctxs.pop(k).close()
+    for _ in range(a.warmup):
+        step(nd)
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        st = step(nd)
+        stats.append((st.pairs, st.build_ms, st.lists_ms, st.force_ms, st.kernel_ms))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / a.steps
+    pairs = stats[-1][0]
+    kern_ms = float(np.mean([s[4] for s in stats]))
+    achieved = FLOPS_PER_PAIR * pairs / (kern_ms * 1e-3) / 1e12
+    st = ctxs[nd].stats()
+    log(f"pairs/step {pairs:.4e}  ms/step {ms_per_step:.2f}  build {np.mean([s[1] for s in stats]):.2f}"
+        f"  lists {np.mean([s[2] for s in stats]):.2f}  kick {np.mean([s[3] for s in stats]):.2f}"
+        f"  leaf kernel {kern_ms:.2f} ms  nodes {st.nodes} listed leaves {st.listed_leaves}"
+        f"  max depth {st.max_depth}")
+    out = {
+        "metric": "pair interactions/s & % FP32 VALU peak at 256^3; adjoint/forward time",
+        "value": pairs / (ms_per_step * 1e-3),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded clustered generator; unit masses)",
+        "config": {"workload": workload_name(a.config, load, nd), "particles": n,
+                   "pairs_per_step": pairs, "leaf_size": nd,
+                   "breakdown_ms": {"build": float(np.mean([s[1] for s in stats])),
+                                    "lists": float(np.mean([s[2] for s in stats])),
+                                    "kick": float(np.mean([s[3] for s in stats])),
+                                    "leaf_kernel": kern_ms}},
+        "roofline": {"bound": "mfma", "pipe": "FP32 VALU (no MFMA: the pair force is not a contraction)",
+                     "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "k_force (leaf kernel)", "flops_per_pair": FLOPS_PER_PAIR},
+    }
+    if not a.no_cpu:
+        log("cpu baseline leg ...")
+        out["cpu_baseline"] = cpu_baseline(load, nd, a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -429,6 +429,11 @@ hsr_status hsr_forces(hsr_ctx *c, float *ax, float *ay, float *az,
   float ms = 0.f;
   HSR_HIP(hipEventElapsedTime(&ms, c->ev[4], c->ev[5]));
   c->force_ms = ms;
+  c->kernel_ms = 0;
+  if (c->kernel_timed) {
+    HSR_HIP(hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
+    c->kernel_ms = ms;
+  }
   return HSR_OK;
 }
 
@@ -456,6 +461,11 @@ hsr_status hsr_kick(hsr_ctx *c, float fcoeff) {
   float ms = 0.f;
   HSR_HIP(hipEventElapsedTime(&ms, c->ev[4], c->ev[5]));
   c->force_ms = ms;
+  c->kernel_ms = 0;
+  if (c->kernel_timed) {
+    HSR_HIP(hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
+    c->kernel_ms = ms;
+  }
   return HSR_OK;
 }
 
@@ -509,6 +519,7 @@ hsr_status hsr_get_stats(hsr_ctx *c, hsr_stats *st) {
   st->build_ms = c->build_ms;
   st->lists_ms = c->lists_ms;
   st->force_ms = c->force_ms;
+  st->kernel_ms = c->kernel_ms;
   return HSR_OK;
 }
 

@@ -176,11 +176,14 @@ void launch_force(hsr_ctx *c, float *fx, float *fy, float *fz) {
 hsr_status force_run(hsr_ctx *c, float *fx, float *fy, float *fz) {
   hipStream_t st = c->stream;
   int64_t n = c->tree.n;
+  c->kernel_timed = false;
   if (n > 0)
     k_fill3<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, fx, fy, fz);
   if (c->lists.nleaves == 0) return HSR_OK;
   const bool guard = c->law.guard != 0;
   const int order = c->law.order;
+  // events bracket the leaf-kernel launch alone (the roofline's duration)
+  HSR_HIP(hipEventRecord(c->ev[6], st));
   if (order == 6) {
     if (guard) launch_force<2, 6, true>(c, fx, fy, fz);
     else launch_force<2, 6, false>(c, fx, fy, fz);
@@ -189,6 +192,8 @@ hsr_status force_run(hsr_ctx *c, float *fx, float *fy, float *fz) {
     else launch_force<2, 5, false>(c, fx, fy, fz);
   }
   HSR_HIP(hipGetLastError());
+  HSR_HIP(hipEventRecord(c->ev[7], st));
+  c->kernel_timed = true;
   return HSR_OK;
 }
 

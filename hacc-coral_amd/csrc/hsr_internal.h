@@ -141,7 +141,8 @@ struct hsr_ctx {
   DBuf dev_v;  // device copy of velocities for host-pointer builds
   DBuf force;  // float3 SoA force [3][n]
   bool built = false, listed = false;
-  double build_ms = 0, lists_ms = 0, force_ms = 0;
+  double build_ms = 0, lists_ms = 0, force_ms = 0, kernel_ms = 0;
+  bool kernel_timed = false;
 };
 
 void hsr_set_error(const std::string &msg);

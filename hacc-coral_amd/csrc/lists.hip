@@ -354,10 +354,6 @@ hsr_status lists_build_device(hsr_ctx *c, const float *min_force_h,
   }
   LS.nsegs = tot[0];
   LS.nentries = tot[1];
-  if (LS.nentries >= (int64_t)INT32_MAX) {
-    hsr_set_error("interaction lists exceed 2^31 entries");
-    return HSR_ERR_UNSUPPORTED;
-  }
   // ---- pass 2: fill
   HSR_HIP(dbuf_reserve(LS.seg, sizeof(int64_t) * 3 * (LS.nsegs + 1)));
   int64_t *seg_a = dptr<int64_t>(LS.seg);

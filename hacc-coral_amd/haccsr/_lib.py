@@ -60,7 +60,8 @@ class Stats(C.Structure):
                 ("listed_leaves", C.c_int64), ("segments", C.c_int64),
                 ("entries", C.c_int64), ("pairs", C.c_int64),
                 ("max_depth", C.c_int64), ("build_ms", C.c_double),
-                ("lists_ms", C.c_double), ("force_ms", C.c_double)]
+                ("lists_ms", C.c_double), ("force_ms", C.c_double),
+                ("kernel_ms", C.c_double)]
 
 
 class Tuning(C.Structure):

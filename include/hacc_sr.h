@@ -134,6 +134,7 @@ typedef struct {
   int64_t pairs;          /* sum over listed leaves of count * list length */
   int64_t max_depth;
   double build_ms, lists_ms, force_ms; /* device time of the last calls */
+  double kernel_ms;       /* the leaf-kernel launch alone (last force/kick) */
 } hsr_stats;
 
 hsr_status hsr_get_stats(hsr_ctx *ctx, hsr_stats *st);

@@ -1,0 +1,301 @@
+"""CPU tests pinning the oracle (oracle/hsr_oracle.c).
+
+The reference's RCBForceTree.cxx cannot be compiled here (it needs MPI
+headers and a generated CosmoDefinition.h), so the oracle is pinned by:
+  * the reference's own BGQCM.c and ForceLaw.cxx compiled into oracle/_ref
+    (live, where the reference tree is mounted) and the golden vectors they
+    wrote into tests/golden (everywhere);
+  * independent formulations of the partition (swap-chain destinations);
+  * structural invariants of the tree;
+  * the oa -> 0 limit of the lists, which must reproduce a brute-force
+    direct sum, and disjoint coverage of every list.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def f32bits(a):
+    return np.asarray(a, np.float32).view(np.uint32)
+
+
+# ------------------------------------------------------------------ cm
+def test_cm_golden_vectors():
+    d = np.load(os.path.join(GOLDEN, "ref_cm.npz"))
+    pos, mass, offs, out = d["pos"], d["mass"], d["offsets"], d["out"]
+    for k in range(offs.shape[0] - 1):
+        a, b = offs[k], offs[k + 1]
+        p = pos[a:b]
+        xmin, xmax, xc = O.cm(p[:, 0], p[:, 1], p[:, 2], mass[a:b])
+        got = np.concatenate([xmin, xmax, xc])
+        assert np.array_equal(f32bits(got), f32bits(out[k])), f"case {k}"
+
+
+def test_cm_live_reference_random():
+    R = O.ref()
+    if R is None:
+        pytest.skip("oracle/_ref not built (reference tree not mounted)")
+    rng = np.random.default_rng(5)
+    for trial in range(200):
+        n = int(rng.integers(1, 3000))
+        scale = np.exp2(rng.integers(-30, 12, size=(n, 3)))
+        p = (rng.random((n, 3)) * scale).astype(np.float32)
+        w = np.ones(n, np.float32) if trial % 2 else (rng.random(n) + 0.5).astype(np.float32)
+        x, y, z = (np.ascontiguousarray(p[:, k]) for k in range(3))
+        ref = [np.zeros(3, np.float32) for _ in range(3)]
+        R.hsrref_cm(n, x, y, z, w, *ref)
+        got = O.cm(x, y, z, w)
+        for g, r in zip(got, ref):
+            assert np.array_equal(f32bits(g), f32bits(r))
+
+
+# ------------------------------------------------------------ force laws
+def test_sr_poly6_law_matches_reference_bitwise():
+    d = np.load(os.path.join(GOLDEN, "ref_forcelaw.npz"))
+    got = O.law_f32(O.LAW_SR_POLY6, d["r2"], rsm=float(d["rsm"]))
+    assert np.array_equal(f32bits(got), f32bits(d["sr_poly6"]))
+    assert float(d["rmax"]) == np.float32(3.116326355)
+
+
+def test_p5_coefficients_fit_reference_grid_force():
+    """The 5th-order polynomial of BGQStep16.c approximates the reference's
+    fitted grid force fgor (FGridEvalFit, compiled from ForceLaw.cxx) over
+    [0, rmax]; flipping or dropping any one coefficient breaks the fit."""
+    d = np.load(os.path.join(GOLDEN, "ref_forcelaw.npz"))
+    r2 = d["r2"].astype(np.float64)
+    sel = r2 <= float(d["rmax"]) ** 2
+    rsm = float(d["rsm"])
+    # recover P5 from the oracle's Step16 law: f = (r2+rsm2)^-1.5 - P5(r2)
+    x = r2[sel][r2[sel] > 0]
+    f = O.law_f32(O.LAW_STEP16, x, rsm=rsm, fsrrmax=3.12).astype(np.float64)
+    p5 = (x + np.float32(rsm) ** 2) ** -1.5 - f
+    fg = d["fgor"].astype(np.float64)[sel][r2[sel] > 0]
+    err = np.max(np.abs(p5 - fg))
+    assert err < 0.03, err
+    c = np.polynomial.polynomial.polyfit(x, p5, 5)
+    for k in range(6):
+        for mod in (-1.0, 0.0):
+            cc = c.copy()
+            cc[k] *= mod
+            e = np.max(np.abs(np.polynomial.polynomial.polyval(x, cc) - fg))
+            assert e > 0.1, (k, mod, e)
+
+
+def test_step16_law_known_answers():
+    rsm, rc = 0.1, 3.12
+    a0 = np.float32(0.269327)
+    f = O.law_f32(O.LAW_STEP16, [0.0, 1e-12, rc * rc, 10.0, 9.0], rsm=rsm, fsrrmax=rc)
+    assert f[0] == 0.0                       # r2 > 0 guard (BGQStep16.c:182)
+    assert abs(f[1] - (np.float32(rsm) ** -3 - a0)) < 1e-3 * np.float32(rsm) ** -3
+    assert f[2] == 0.0 and f[3] == 0.0       # r2 < fsrrmax^2 strictly
+    s = 9.0 + 0.01
+    p = sum(c * 9.0 ** i for i, c in enumerate([0.269327, -0.0750978, 0.0114808,
+                                                -0.00109313, 0.0000605491, -0.00000147177]))
+    assert abs(f[4] - (s ** -1.5 - p)) < 1e-6
+
+
+# --------------------------------------------------------------- partition
+def test_partition_invariants():
+    rng = np.random.default_rng(9)
+    for trial in range(300):
+        n = int(rng.integers(1, 400))
+        key = rng.integers(0, 8, n).astype(np.float32)
+        pv = float(rng.integers(0, 9))
+        is_, out, ids = O.partition(key, pv)
+        left = np.nonzero(key < pv)[0]
+        assert is_ == left.shape[0]
+        # left side: the keys < pv in their original order
+        assert np.array_equal(ids[:is_], left)
+        # right side: elements originally at q >= is stay put unless they
+        # are left elements; a right element from q < is ends at the first
+        # idx-iterate q -> idx[q] -> ... that is >= is
+        dest = np.empty(n, np.int64)
+        dest[ids] = np.arange(n)
+        isleft = key < pv
+        for q in range(n):
+            if isleft[q]:
+                continue
+            if q >= is_:
+                assert dest[q] == q
+            else:
+                j = q
+                while j < is_:
+                    j = left[j]
+                assert dest[q] == j
+        assert np.array_equal(np.sort(ids), np.arange(n))
+
+
+# -------------------------------------------------------------------- tree
+def uniform(n, box, seed):
+    rng = np.random.default_rng(seed)
+    p = (rng.random((n, 3)) * box).astype(np.float32)
+    return p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy()
+
+
+def small_params(box, nd, oa=0.1, margin=1.0):
+    return O.TreeParams(np.zeros(3, np.float32), np.full(3, box, np.float32),
+                        np.full(3, margin, np.float32), np.full(3, box - margin, np.float32),
+                        nd=nd, oa=oa)
+
+
+@pytest.mark.parametrize("n,nd,seed", [(5000, 16, 1), (20000, 64, 2), (3000, 1, 3)])
+def test_tree_invariants(n, nd, seed):
+    x, y, z = uniform(n, 10.0, seed)
+    tp = small_params(10.0, nd)
+    tree = O.build_tree(tp, x, y, z)
+    t = tree.nodes
+    R = O.ref()
+    assert np.array_equal(np.sort(tree.id), np.arange(n))
+    # allocation order: children of node k sit at 1 + 2A and 2 + 2A where A
+    # counts the allocating nodes before k in preorder; here: children come
+    # in adjacent pairs after their parent
+    for k in range(t.shape[0]):
+        cl, cr = t["cl"][k], t["cr"][k]
+        if cl or cr:
+            assert cl > k and cr == cl + 1
+            assert t["count"][cl] + t["count"][cr] == t["count"][k]
+            assert t["offset"][cl] == t["offset"][k]
+            assert t["offset"][cr] == t["offset"][k] + t["count"][cl]
+            # split on the longest edge, ties x > y > z (RCBForceTree.cxx:843-852)
+            xl = (t["xmax"][k] - t["xmin"][k]).astype(np.float32)
+            d = 0 if (xl[0] > xl[1] and xl[0] > xl[2]) else (1 if xl[1] > xl[2] else 2)
+            coord = [tree.x, tree.y, tree.z][d]
+            a, m_, b = t["offset"][k], t["offset"][cr], t["offset"][k] + t["count"][k]
+            assert np.all(coord[a:m_] < t["xc"][k][d])
+            assert np.all(coord[m_:b] >= t["xc"][k][d])
+        elif t["count"][k] > 0:
+            assert t["count"][k] <= nd or np.all(tree.x[t["offset"][k]:t["offset"][k] + t["count"][k]] == tree.x[t["offset"][k]])
+            # leaf order is final at cm time, so the reference's cm over the
+            # leaf's range reproduces the node bit for bit
+            a, b = t["offset"][k], t["offset"][k] + t["count"][k]
+            if R is not None:
+                ref = [np.zeros(3, np.float32) for _ in range(3)]
+                R.hsrref_cm(b - a, tree.x[a:b].copy(), tree.y[a:b].copy(), tree.z[a:b].copy(),
+                            tree.mass[a:b].copy(), *ref)
+                assert np.array_equal(f32bits(ref[0]), f32bits(t["xmin"][k]))
+                assert np.array_equal(f32bits(ref[1]), f32bits(t["xmax"][k]))
+                assert np.array_equal(f32bits(ref[2]), f32bits(t["xc"][k]))
+            # tight bbox
+            assert np.float32(tree.x[a:b].min()) == t["xmin"][k][0]
+            assert np.float32(tree.z[a:b].max()) == t["xmax"][k][2]
+            # TDPTS = 1 pseudo-particle mass = member mass (when count > 1)
+            if t["count"][k] > 1:
+                assert t["ppm"][k][0] == np.float32(t["count"][k])
+
+
+def test_degenerate_nodes_stay_leaves():
+    """Duplicate positions cannot be split: the node stays a leaf with more
+    than nd particles and its two reserved child slots stay empty
+    (RCBForceTree.cxx:727-729, 825-835)."""
+    x = np.full(50, 2.5, np.float32)
+    y = np.full(50, 3.5, np.float32)
+    z = np.full(50, 4.5, np.float32)
+    x2, y2, z2 = uniform(200, 10.0, 4)
+    tp = small_params(10.0, 8)
+    tree = O.build_tree(tp, np.concatenate([x, x2]), np.concatenate([y, y2]), np.concatenate([z, z2]))
+    t = tree.nodes
+    big = np.nonzero((t["count"] > 8) & (t["cl"] == 0) & (t["cr"] == 0))[0]
+    assert big.size >= 1
+    assert np.any(t["count"] == 0)  # orphan slots
+    k = big[0]
+    assert t["ppm"][k][0] == 0.0
+
+
+# ------------------------------------------------------------------- lists
+def coverage(tree, lists, k):
+    """Particles represented by list k: direct entries and pseudo nodes."""
+    n = tree.x.shape[0]
+    cov = np.zeros(n, np.int64)
+    a, b = lists.seg_ptr[k], lists.seg_ptr[k + 1]
+    segs = list(zip(lists.seg_a[a:b], lists.seg_b[a:b]))
+    for sa, sb in segs[:-1]:  # the last segment is the leaf itself
+        if sb >= 0:
+            cov[sa:sa + sb] += 1
+        else:
+            nd = tree.nodes[sa]
+            cov[nd["offset"]:nd["offset"] + nd["count"]] += 1
+    return cov
+
+
+def test_lists_disjoint_coverage():
+    x, y, z = uniform(20000, 16.0, 5)
+    tp = small_params(16.0, 32)
+    tree = O.build_tree(tp, x, y, z)
+    lists = O.build_lists(tp, tree)
+    for k in range(0, lists.leaves.shape[0], max(1, lists.leaves.shape[0] // 40)):
+        cov = coverage(tree, lists, k)
+        assert cov.max() <= 1
+        nd = tree.nodes[lists.leaves[k]]
+        assert np.all(cov[nd["offset"]:nd["offset"] + nd["count"]] == 0)
+        # list length = direct entries + pseudo-particles + own particles
+        a, b = lists.seg_ptr[k], lists.seg_ptr[k + 1]
+        sb = lists.seg_b[a:b]
+        assert lists.entries[k] == np.sum(np.where(sb >= 0, sb, 1))
+
+
+def test_lists_oa0_reproduce_direct_sum():
+    """With tan(oa) = 0 no node is ever accepted, so each list holds every
+    particle within the L-infinity reach of the leaf: the tree force must
+    equal the brute-force direct sum (FP64, different summation order)."""
+    x, y, z = uniform(6000, 12.0, 6)
+    m = np.ones_like(x)
+    tp = small_params(12.0, 32, oa=0.0)
+    tree = O.build_tree(tp, x, y, z)
+    lists = O.build_lists(tp, tree)
+    assert np.all(lists.seg_b >= 0)  # no pseudo-particles
+    ax, ay, az = O.forces_f64(tp, tree, lists)
+    targets = np.concatenate([np.arange(tree.nodes[l]["offset"], tree.nodes[l]["offset"] + tree.nodes[l]["count"])
+                              for l in lists.leaves])
+    dx, dy, dz = O.direct_f64(tree.x, tree.y, tree.z, tree.mass, targets)
+    G = np.stack([ax[targets], ay[targets], az[targets]], 1)
+    D = np.stack([dx, dy, dz], 1)
+    assert np.max(np.abs(G - D)) <= 1e-9 * np.max(np.abs(D))
+
+
+def test_tree_force_accuracy_vs_direct():
+    """oa = 0.1 pseudo-particle approximation stays close to the direct sum."""
+    x, y, z = uniform(8000, 12.0, 7)
+    tp = small_params(12.0, 32, oa=0.1)
+    tree = O.build_tree(tp, x, y, z)
+    lists = O.build_lists(tp, tree)
+    ax, ay, az = O.forces_f64(tp, tree, lists)
+    targets = np.concatenate([np.arange(tree.nodes[l]["offset"], tree.nodes[l]["offset"] + tree.nodes[l]["count"])
+                              for l in lists.leaves])
+    dx, dy, dz = O.direct_f64(tree.x, tree.y, tree.z, tree.mass, targets)
+    G = np.stack([ax[targets], ay[targets], az[targets]], 1)
+    D = np.stack([dx, dy, dz], 1)
+    rel = np.sqrt(np.sum((G - D) ** 2) / np.sum(D ** 2))
+    assert rel < 2e-2, rel
+
+
+def test_float_kernel_vs_fp64():
+    x, y, z = uniform(8000, 12.0, 8)
+    tp = small_params(12.0, 32)
+    tree = O.build_tree(tp, x, y, z)
+    lists = O.build_lists(tp, tree)
+    f32 = np.stack(O.forces_f32(tp, tree, lists), 1)
+    s16 = np.stack(O.forces_step16(tp, tree, lists), 1)
+    f64 = np.stack(O.forces_f64(tp, tree, lists), 1)
+    sel = np.any(f64 != 0, axis=1)
+    rel = lambda a, b: np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2))
+    assert rel(f32[sel], f64[sel]) < 1e-5
+    assert rel(s16[sel], f64[sel]) < 1e-5
+
+
+def test_cell_sort_is_stable_counting_sort():
+    rng = np.random.default_rng(10)
+    x = (rng.random(5000) * 9.0 - 0.5).astype(np.float32)
+    y = (rng.random(5000) * 8.0).astype(np.float32)
+    z = (rng.random(5000) * 8.0).astype(np.float32)
+    perm, inside = O.cell_sort_perm(x, y, z, [8, 8, 8])
+    ix, iy, iz = np.floor(x).astype(int), np.floor(y).astype(int), np.floor(z).astype(int)
+    ok = (ix >= 0) & (ix < 8)
+    key = np.where(ok, (ix * 8 + iy) * 8 + iz, 512)
+    order = np.argsort(key, kind="stable")
+    assert np.array_equal(perm[order], np.arange(5000))
+    assert inside == ok.sum()
