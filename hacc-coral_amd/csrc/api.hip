@@ -300,10 +300,11 @@ hsr_status hsr_destroy(hsr_ctx *c) {
   DBuf *bufs[] = {&c->tree.bnodes, &c->tree.canon_nodes, &c->tree.pfin, &c->tree.pj,
                   &c->tree.xs, &c->tree.ys, &c->tree.zs, &c->tree.ms,
                   &c->lists.leaves, &c->lists.seg_ptr, &c->lists.seg, &c->lists.ent_ptr,
-                  &c->lists.jidx, &c->lists.items, &c->lists.stack,
+                  &c->lists.jidx, &c->lists.items, &c->lists.stack, &c->lists.item_keys,
+                  &c->lists.plan, &c->lists.part, &c->s.qctr,
                   &c->s.a, &c->s.b, &c->s.c, &c->s.d, &c->s.e, &c->s.f, &c->s.g, &c->s.h,
                   &c->s.cub, &c->s.x1, &c->s.y1, &c->s.z1, &c->s.m1, &c->s.p1,
-                  &c->s.x0, &c->s.y0, &c->s.z0, &c->s.m0, &c->s.p0, &c->s.blocks,
+                  &c->s.x0, &c->s.y0, &c->s.z0, &c->s.m0, &c->s.p0, &c->s.blocks, &c->s.rights,
                   &c->s.pblocks, &c->s.counters, &c->dev_v, &c->force};
   for (DBuf *b : bufs) dbuf_free(*b);
   for (int k = 0; k < 10; ++k) dbuf_free(c->s.host_in[k]);
@@ -363,6 +364,8 @@ hsr_status hsr_lists(hsr_ctx *c, const float min_force[3],
   HSR_HIP(hipEventRecord(c->ev[2], c->stream));
   s = lists_build_device(c, min_force, max_force);
   if (s) return s;
+  s = force_plan(c);
+  if (s) return s;
   HSR_HIP(hipEventRecord(c->ev[3], c->stream));
   HSR_HIP(hipEventSynchronize(c->ev[3]));
   float ms = 0.f;
@@ -386,6 +389,8 @@ hsr_status hsr_lists_many(hsr_ctx *c, const float *min_force,
   }
   HSR_HIP(hipEventRecord(c->ev[2], c->stream));
   s = lists_build_device(c, min_force, max_force);
+  if (s) return s;
+  s = force_plan(c);
   if (s) return s;
   HSR_HIP(hipEventRecord(c->ev[3], c->stream));
   HSR_HIP(hipEventSynchronize(c->ev[3]));

@@ -6,141 +6,275 @@
 //     f(r2) = m_j * [ (r2 + rsm^2)^(-3/2) - P(r2) ] * [r2 < rc2] (* [r2 > 0])
 //     F_i  += f * (x_j - x_i)
 // summed over every entry j of leaf i's interaction list (nbody1,
-// RCBForceTree.cxx:599-619).
+// RCBForceTree.cxx:599-619).  The polynomial is evaluated in s = r2 + rsm^2
+// (coefficients re-expanded on the host), so s serves the power, the
+// polynomial and the cutoff test.
 //
-// Mapping: one wave per work item (a listed leaf, or a chunk of a long
-// list).  The leaf's i-particles sit in registers, R per lane, in G lane
-// groups; each group walks its own stride of the j-tile staged in LDS from
-// the list's j indices (coalesced 16-byte records).  Group partials are
-// reduced in a fixed order, so results are bitwise reproducible.
+// Work: one persistent launch walks a queue of work items -- a listed leaf,
+// or a chunk of a long list -- sorted by decreasing pair count.  A wave
+// takes an item: the leaf's i-particles sit in registers, R per lane, in G
+// lane groups; the j-entries are staged in LDS per tile, each group owning a
+// contiguous, padded slice (stride = 1 mod 8 entries, so the groups' 16-byte
+// reads fall in distinct banks).  Chunk partials of long lists are summed in
+// chunk order afterwards, so results are bitwise reproducible.
 #include "hsr_internal.h"
 #include "wave.h"
 
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
+#include <vector>
 
 namespace {
 
-constexpr int TILE = 512;
-
-template <int ORDER>
-__device__ __forceinline__ float poly(const ForceLawDev &L, float r2) {
-  float p;
-  if (ORDER == 6) {
-    p = fmaf(L.c[6], r2, L.c[5]);
-    p = fmaf(p, r2, L.c[4]);
-  } else {
-    p = fmaf(L.c[5], r2, L.c[4]);
-  }
-  p = fmaf(p, r2, L.c[3]);
-  p = fmaf(p, r2, L.c[2]);
-  p = fmaf(p, r2, L.c[1]);
-  p = fmaf(p, r2, L.c[0]);
-  return p;
-}
+constexpr int R = 2;          // i-particles per lane
+constexpr int TILE = 256;     // j entries staged per tile
+constexpr int GMAX = 16;      // at most 16 lane groups (>= 4 lanes each)
+// staged entries incl. per-group padding: max over G <= 16 of
+// G * stride(G) is 330 (G = 10) for TILE = 256
+constexpr int LDS_ENT = 336;
+constexpr int UNROLL = 4;
 
 struct Item {
-  int32_t leaf;       // listed-leaf index
-  int32_t part;       // chunk index within the leaf's list
-  int64_t e0, e1;     // list entry range
+  int32_t leaf;     // listed-leaf index
+  int32_t chunk;    // chunk index within the leaf's list
+  int64_t e0, e1;   // list entry range
+  int64_t pbase;    // partial-force offset (-1: write F directly)
 };
 
-// Work item -> force partials.  R i-particles per lane; lane groups of
-// `lpg` lanes split the j-range by stride G.
-template <int R, int ORDER, bool GUARD>
+struct LawArgs {
+  float c[7];      // polynomial in s (degree 5 or 6)
+  float rcs;       // s cutoff: pairs with s < rcs contribute
+  float eps2;
+  int guard;
+};
+
+template <int ORDER>
+__device__ __forceinline__ float poly_tail(const LawArgs &L, float s) {
+  // q(s) = c1 + c2 s + ... ; f = y^3 - c0 - s q(s)
+  float q;
+  if (ORDER == 6) {
+    q = fmaf(L.c[6], s, L.c[5]);
+    q = fmaf(q, s, L.c[4]);
+  } else {
+    q = fmaf(L.c[5], s, L.c[4]);
+  }
+  q = fmaf(q, s, L.c[3]);
+  q = fmaf(q, s, L.c[2]);
+  q = fmaf(q, s, L.c[1]);
+  return q;
+}
+
+template <int ORDER, bool GUARD>
 __global__ void __launch_bounds__(64)
 k_force(const hsr_node *__restrict__ cn, const int32_t *__restrict__ leaves,
-        const int64_t *__restrict__ ent_ptr, const int32_t *__restrict__ jidx,
-        const PJ *__restrict__ pj, const float *__restrict__ xs,
-        const float *__restrict__ ys, const float *__restrict__ zs,
-        int64_t nleaves, ForceLawDev L, float *__restrict__ fx,
-        float *__restrict__ fy, float *__restrict__ fz) {
-  __shared__ float4 tile[TILE];
-  __shared__ float red[3][64 * R];
-  const int64_t k = blockIdx.x;
-  if (k >= nleaves) return;
+        const Item *__restrict__ items, const int64_t *__restrict__ nitems_p,
+        const int32_t *__restrict__ jidx, const PJ *__restrict__ pj,
+        LawArgs L, unsigned long long *__restrict__ qctr,
+        float *__restrict__ fx, float *__restrict__ fy,
+        float *__restrict__ fz, float *__restrict__ part) {
+  // the tile doubles as the reduction buffer once the j-walk is done
+  __shared__ float4 tile[LDS_ENT];
+  static_assert(3 * 64 * R <= 4 * LDS_ENT, "reduction buffer fits the tile");
+  float (*red)[64 * R] = reinterpret_cast<float (*)[64 * R]>(&tile[0]);
   const int l = lane_id();
-  const hsr_node &nd = cn[leaves[k]];
-  const int64_t off = nd.offset, cnt = nd.count;
-  const int64_t e0 = ent_ptr[k], e1 = ent_ptr[k + 1];
-  const float eps2 = L.eps2, rc2 = L.rc2;
-
-  for (int64_t ib = 0; ib < cnt; ib += 64 * R) {
-    const int ni = (int)min<int64_t>(64 * R, cnt - ib);
-    // lane groups: lpg lanes each holding R i's cover ni particles
-    const int lpg = (ni + R - 1) / R;
-    const int G = max(1, 64 / lpg);
-    const int g = l / lpg, tq = l - g * lpg;
-    const bool active = g < G;
-    float xi[R], yi[R], zi[R], ax[R], ay[R], az[R];
+  const int64_t nitems = *nitems_p;
+  const float c0 = L.c[0], rcs = L.rcs;
+  for (;;) {
+    int64_t it = 0;
+    if (l == 0) it = (int64_t)atomicAdd(qctr, 1ull);
+    it = (int64_t)readlane_u64((uint64_t)it, 0);
+    if (it >= nitems) break;
+    const Item I = items[it];
+    const hsr_node &nd = cn[leaves[I.leaf]];
+    const int64_t off = nd.offset, cnt = nd.count;
+    // padding entries: far enough to be masked, near enough to keep every
+    // intermediate finite
+    const float4 pad = make_float4(nd.xc[0] + 1000.f, nd.xc[1], nd.xc[2], 0.f);
+    for (int64_t ib = 0; ib < cnt; ib += 64 * R) {
+      const int ni = (int)min<int64_t>(64 * R, cnt - ib);
+      const int lpg = max((ni + R - 1) / R, 64 / GMAX);
+      const int G = max(1, 64 / lpg);
+      const int g = l / lpg, tq = l - g * lpg;
+      const bool active = g < G;
+      float xi[R], yi[R], zi[R], ax[R], ay[R], az[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      int ii = tq + r * lpg;
-      bool v = active && ii < ni;
-      xi[r] = v ? xs[off + ib + ii] : 1e30f;
-      yi[r] = v ? ys[off + ib + ii] : 1e30f;
-      zi[r] = v ? zs[off + ib + ii] : 1e30f;
-      ax[r] = ay[r] = az[r] = 0.f;
-    }
-    for (int64_t t0 = e0; t0 < e1; t0 += TILE) {
-      const int nt = (int)min<int64_t>(TILE, e1 - t0);
-      __syncthreads();
-      for (int q = l; q < nt; q += 64) {
-        PJ p = pj[jidx[t0 + q]];
-        tile[q] = make_float4(p.x, p.y, p.z, p.m);
+      for (int r = 0; r < R; ++r) {
+        const int ii = tq + r * lpg;
+        const bool v = active && ii < ni;
+        const PJ p = pj[off + ib + (v ? ii : 0)];
+        xi[r] = v ? p.x : 1e30f;
+        yi[r] = v ? p.y : 1e30f;
+        zi[r] = v ? p.z : 1e30f;
+        ax[r] = ay[r] = az[r] = 0.f;
       }
+      for (int64_t t0 = I.e0; t0 < I.e1; t0 += TILE) {
+        const int nt = (int)min<int64_t>(TILE, I.e1 - t0);
+        // per-group slice: tg entries (multiple of UNROLL), stride = 1 mod 8
+        int tg = (nt + G - 1) / G;
+        tg = (tg + UNROLL - 1) / UNROLL * UNROLL;
+        const int stride = tg + ((9 - (tg & 7)) & 7);
+        __syncthreads();
+        for (int q = l; q < G * tg; q += 64) {
+          const int gq = q / tg, kq = q - gq * tg;
+          float4 v = pad;
+          if (q < nt) {
+            const PJ p = pj[jidx[t0 + q]];
+            v = make_float4(p.x, p.y, p.z, p.m);
+          }
+          tile[gq * stride + kq] = v;
+        }
+        __syncthreads();
+        if (active) {
+          const float4 *tp = tile + g * stride;
+          for (int k = 0; k < tg; k += UNROLL) {
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+              const float4 p = tp[k + u];
+#pragma unroll
+              for (int r = 0; r < R; ++r) {
+                const float dx = p.x - xi[r];
+                const float dy = p.y - yi[r];
+                const float dz = p.z - zi[r];
+                float s = fmaf(dx, dx, L.eps2);
+                s = fmaf(dy, dy, s);
+                s = fmaf(dz, dz, s);
+                const float y = __builtin_amdgcn_rsqf(s);
+                const float y2 = y * y;
+                const float t3 = fmaf(y2, y, -c0);
+                const float q = poly_tail<ORDER>(L, s);
+                float f = fmaf(-s, q, t3);
+                float mm = (s < rcs) ? p.w : 0.f;
+                if (GUARD) mm = (s > L.eps2) ? mm : 0.f;
+                f *= mm;
+                ax[r] = fmaf(f, dx, ax[r]);
+                ay[r] = fmaf(f, dy, ay[r]);
+                az[r] = fmaf(f, dz, az[r]);
+              }
+            }
+          }
+        }
+      }
+      // fixed-order reduction over the G groups
       __syncthreads();
-      if (active) {
-        for (int q = g; q < nt; q += G) {
-          const float4 p = tile[q];
+      for (int q = l; q < 64 * R; q += 64) red[0][q] = red[1][q] = red[2][q] = 0.f;
+      __syncthreads();
+      for (int gg = 0; gg < G; ++gg) {
+        if (active && g == gg) {
 #pragma unroll
           for (int r = 0; r < R; ++r) {
-            float dx = p.x - xi[r];
-            float dy = p.y - yi[r];
-            float dz = p.z - zi[r];
-            float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            float s = r2 + eps2;
-            float y = __builtin_amdgcn_rsqf(s);
-            float y2 = y * y;
-            float f = fmaf(y2, y, -poly<ORDER>(L, r2));
-            float mm = (r2 < rc2) ? p.w : 0.f;
-            if (GUARD) mm = (r2 > 0.f) ? mm : 0.f;
-            f *= mm;
-            ax[r] = fmaf(f, dx, ax[r]);
-            ay[r] = fmaf(f, dy, ay[r]);
-            az[r] = fmaf(f, dz, az[r]);
+            const int ii = tq + r * lpg;
+            if (ii < ni) {
+              red[0][ii] += ax[r];
+              red[1][ii] += ay[r];
+              red[2][ii] += az[r];
+            }
           }
         }
+        __syncthreads();
       }
-    }
-    // fixed-order reduction over the G groups
-    __syncthreads();
-    for (int q = l; q < 64 * R; q += 64) red[0][q] = red[1][q] = red[2][q] = 0.f;
-    __syncthreads();
-    for (int gg = 0; gg < G; ++gg) {
-      if (active && g == gg) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          int ii = tq + r * lpg;
-          if (ii < ni) {
-            red[0][ii] += ax[r];
-            red[1][ii] += ay[r];
-            red[2][ii] += az[r];
-          }
+      if (I.pbase < 0) {
+        for (int ii = l; ii < ni; ii += 64) {
+          fx[off + ib + ii] = red[0][ii];
+          fy[off + ib + ii] = red[1][ii];
+          fz[off + ib + ii] = red[2][ii];
+        }
+      } else {
+        float *pp = part + I.pbase;
+        for (int ii = l; ii < ni; ii += 64) {
+          pp[3 * (ib + ii) + 0] = red[0][ii];
+          pp[3 * (ib + ii) + 1] = red[1][ii];
+          pp[3 * (ib + ii) + 2] = red[2][ii];
         }
       }
-      __syncthreads();
-    }
-    for (int ii = l; ii < ni; ii += 64) {
-      fx[off + ib + ii] = red[0][ii];
-      fy[off + ib + ii] = red[1][ii];
-      fz[off + ib + ii] = red[2][ii];
     }
   }
+}
+
+// Sum the chunk partials of long lists in chunk order.
+__global__ void __launch_bounds__(64)
+k_reduce_chunks(const hsr_node *cn, const int32_t *leaves,
+                const int64_t *multi, const int64_t *nmulti_p,
+                const int64_t *leaf_pbase, const int32_t *leaf_nchunk,
+                const float *part, float *fx, float *fy, float *fz) {
+  const int64_t k = blockIdx.x;
+  if (k >= *nmulti_p) return;
+  const int64_t lf = multi[k];
+  const hsr_node &nd = cn[leaves[lf]];
+  const int64_t cnt = nd.count, off = nd.offset;
+  const int nc = leaf_nchunk[lf];
+  const float *p0 = part + leaf_pbase[lf];
+  for (int64_t i = threadIdx.x; i < cnt; i += 64) {
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    for (int c = 0; c < nc; ++c) {
+      const float *p = p0 + (int64_t)c * 3 * cnt + 3 * i;
+      sx += p[0];
+      sy += p[1];
+      sz += p[2];
+    }
+    fx[off + i] = sx;
+    fy[off + i] = sy;
+    fz[off + i] = sz;
+  }
+}
+
+// Work items: chunk counts per leaf.
+__global__ void k_item_count(int64_t nleaves, const int64_t *ent_ptr,
+                             int64_t chunk, int32_t *nchunk, int64_t *nitem,
+                             int64_t *multi_flag, const hsr_node *cn,
+                             const int32_t *leaves, int64_t *psize) {
+  int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= nleaves) return;
+  int64_t L = ent_ptr[k + 1] - ent_ptr[k];
+  int64_t c = std::max<int64_t>(1, (L + chunk - 1) / chunk);
+  nchunk[k] = (int32_t)c;
+  nitem[k] = c;
+  multi_flag[k] = c > 1 ? 1 : 0;
+  psize[k] = c > 1 ? c * 3 * cn[leaves[k]].count : 0;
+}
+
+__global__ void k_item_fill(int64_t nleaves, const int64_t *ent_ptr,
+                            int64_t chunk, const int32_t *nchunk,
+                            const int64_t *item_base, const int64_t *pbase,
+                            const hsr_node *cn, const int32_t *leaves,
+                            Item *items, uint64_t *keys, int32_t *vals,
+                            int64_t *multi, const int64_t *multi_pos,
+                            const int64_t *multi_flag) {
+  int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= nleaves) return;
+  const int64_t e0 = ent_ptr[k], e1 = ent_ptr[k + 1];
+  const int nc = nchunk[k];
+  const int64_t cnt = cn[leaves[k]].count;
+  for (int c = 0; c < nc; ++c) {
+    int64_t j = item_base[k] + c;
+    Item I;
+    I.leaf = (int32_t)k;
+    I.chunk = c;
+    I.e0 = e0 + (int64_t)c * chunk;
+    I.e1 = std::min<int64_t>(e1, I.e0 + chunk);
+    I.pbase = nc > 1 ? pbase[k] + (int64_t)c * 3 * cnt : -1;
+    items[j] = I;
+    // work ~ i-passes * entries
+    uint64_t work = (uint64_t)((cnt + 64 * R - 1) / (64 * R)) * (uint64_t)(I.e1 - I.e0);
+    keys[j] = work;
+    vals[j] = (int32_t)j;
+  }
+  if (multi_flag[k]) multi[multi_pos[k]] = k;
+}
+
+__global__ void k_item_permute(int64_t n, const Item *in, const int32_t *order,
+                               Item *out) {
+  int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < n) out[k] = in[order[k]];
 }
 
 __global__ void k_fill3(int64_t n, float *a, float *b, float *c) {
   int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (k < n) a[k] = b[k] = c[k] = 0.f;
+}
+
+__global__ void k_zero_u64(unsigned long long *p) {
+  if (threadIdx.x == 0) *p = 0ull;
 }
 
 __global__ void k_kick(int64_t nleaves, const hsr_node *cn,
@@ -160,40 +294,153 @@ __global__ void k_kick(int64_t nleaves, const hsr_node *cn,
   }
 }
 
-template <int R, int ORDER, bool GUARD>
-void launch_force(hsr_ctx *c, float *fx, float *fy, float *fz) {
-  ListState &LS = c->lists;
-  TreeState &T = c->tree;
-  k_force<R, ORDER, GUARD><<<(unsigned)LS.nleaves, 64, 0, c->stream>>>(
-      dptr<hsr_node>(T.canon_nodes), dptr<int32_t>(LS.leaves),
-      dptr<int64_t>(LS.ent_ptr), dptr<int32_t>(LS.jidx), dptr<PJ>(T.pj),
-      dptr<float>(T.xs), dptr<float>(T.ys), dptr<float>(T.zs), LS.nleaves,
-      c->law, fx, fy, fz);
+inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+template <class T>
+hsr_status excl_scan(hsr_ctx *c, const T *in, T *out, int64_t n) {
+  size_t tmp = 0;
+  HSR_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, c->stream));
+  HSR_HIP(dbuf_reserve(c->s.cub, tmp + 256));
+  HSR_HIP(hipcub::DeviceScan::ExclusiveSum(c->s.cub.p, tmp, in, out, (int)n, c->stream));
+  return HSR_OK;
+}
+
+// P(r2) = sum a_k r2^k re-expanded in s = r2 + eps2 (host, double).
+LawArgs law_args(const ForceLawDev &D) {
+  LawArgs A;
+  double a[7] = {0, 0, 0, 0, 0, 0, 0}, b[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int k = 0; k <= 6; ++k) a[k] = D.c[k];
+  const double e = D.eps2;
+  // P(s - e) = sum_k a_k (s - e)^k
+  for (int k = 0; k <= 6; ++k) {
+    double binom = 1.0;
+    for (int i = 0; i <= k; ++i) {
+      // coefficient of s^i in (s - e)^k: C(k,i) (-e)^(k-i)
+      double cki = binom;
+      double term = cki * std::pow(-e, k - i);
+      b[i] += a[k] * term;
+      binom = binom * (double)(k - i) / (double)(i + 1);
+    }
+  }
+  for (int k = 0; k <= 6; ++k) A.c[k] = (float)b[k];
+  // r2 < rc2  <=>  s < rc2 + eps2 up to rounding at the boundary, where the
+  // short-range force is ~1e-5 of its r = 1 value
+  A.rcs = (float)((double)D.rc2 + e);
+  A.eps2 = D.eps2;
+  A.guard = D.guard;
+  return A;
 }
 
 }  // namespace
 
+// Work items of the current lists (rebuilt after every hsr_lists call).
+hsr_status force_plan(hsr_ctx *c) {
+  hipStream_t st = c->stream;
+  ListState &LS = c->lists;
+  const int64_t nl = LS.nleaves;
+  const int64_t chunk = c->tuning.chunk > 0 ? c->tuning.chunk : 16384;
+  LS.chunk = chunk;
+  if (nl == 0) {
+    LS.nitems = 0;
+    return HSR_OK;
+  }
+  // scratch layout (int64 unless noted)
+  HSR_HIP(dbuf_reserve(LS.plan, sizeof(int64_t) * 8 * (nl + 1) + sizeof(int32_t) * (nl + 1) + 256));
+  int64_t *nitem = dptr<int64_t>(LS.plan);
+  int64_t *item_base = nitem + (nl + 1);
+  int64_t *mflag = item_base + (nl + 1);
+  int64_t *mpos = mflag + (nl + 1);
+  int64_t *psize = mpos + (nl + 1);
+  int64_t *pbase = psize + (nl + 1);
+  int64_t *multi = pbase + (nl + 1);
+  int64_t *counts = multi + (nl + 1);  // [0] nitems, [1] nmulti, [2] partial floats
+  int32_t *nchunk = reinterpret_cast<int32_t *>(counts + (nl + 1));
+  const hsr_node *cn = dptr<hsr_node>(c->tree.canon_nodes);
+  const int32_t *leaves = dptr<int32_t>(LS.leaves);
+  const int64_t *ent_ptr = dptr<int64_t>(LS.ent_ptr);
+  k_item_count<<<nblk(nl, 256), 256, 0, st>>>(nl, ent_ptr, chunk, nchunk, nitem, mflag, cn, leaves, psize);
+  hsr_status s;
+  if ((s = excl_scan(c, nitem, item_base, nl)) != HSR_OK) return s;
+  if ((s = excl_scan(c, mflag, mpos, nl)) != HSR_OK) return s;
+  if ((s = excl_scan(c, psize, pbase, nl)) != HSR_OK) return s;
+  int64_t tail[6];
+  HSR_HIP(hipMemcpyAsync(&tail[0], item_base + nl - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HSR_HIP(hipMemcpyAsync(&tail[1], nitem + nl - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HSR_HIP(hipMemcpyAsync(&tail[2], mpos + nl - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HSR_HIP(hipMemcpyAsync(&tail[3], mflag + nl - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HSR_HIP(hipMemcpyAsync(&tail[4], pbase + nl - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HSR_HIP(hipMemcpyAsync(&tail[5], psize + nl - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HSR_HIP(hipStreamSynchronize(st));
+  const int64_t nitems = tail[0] + tail[1];
+  const int64_t nmulti = tail[2] + tail[3];
+  const int64_t npart = tail[4] + tail[5];
+  LS.nitems = nitems;
+  LS.nmulti = nmulti;
+  HSR_HIP(dbuf_reserve(LS.items, sizeof(Item) * 2 * (nitems + 1)));
+  HSR_HIP(dbuf_reserve(LS.item_keys, (sizeof(uint64_t) + sizeof(int32_t)) * 2 * (nitems + 1) + 256));
+  HSR_HIP(dbuf_reserve(LS.part, sizeof(float) * std::max<int64_t>(npart, 1)));
+  Item *items_tmp = dptr<Item>(LS.items) + (nitems + 1);
+  Item *items = dptr<Item>(LS.items);
+  uint64_t *keys = dptr<uint64_t>(LS.item_keys);
+  uint64_t *keys2 = keys + (nitems + 1);
+  int32_t *vals = reinterpret_cast<int32_t *>(keys2 + (nitems + 1));
+  int32_t *vals2 = vals + (nitems + 1);
+  k_item_fill<<<nblk(nl, 256), 256, 0, st>>>(nl, ent_ptr, chunk, nchunk, item_base, pbase, cn, leaves,
+                                             items_tmp, keys, vals, multi, mpos, mflag);
+  // heaviest items first
+  size_t tmp = 0;
+  HSR_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp, keys, keys2, vals, vals2, (int)nitems, 0, 64, st));
+  HSR_HIP(dbuf_reserve(c->s.cub, tmp + 256));
+  HSR_HIP(hipcub::DeviceRadixSort::SortPairsDescending(c->s.cub.p, tmp, keys, keys2, vals, vals2, (int)nitems, 0, 64, st));
+  k_item_permute<<<nblk(nitems, 256), 256, 0, st>>>(nitems, items_tmp, vals2, items);
+  // device-resident counts for the persistent launch
+  int64_t hc[2] = {nitems, nmulti};
+  HSR_HIP(hipMemcpyAsync(counts, hc, sizeof(hc), hipMemcpyHostToDevice, st));
+  HSR_HIP(hipStreamSynchronize(st));
+  LS.multi = multi;
+  LS.counts = counts;
+  LS.leaf_pbase = pbase;
+  LS.leaf_nchunk = nchunk;
+  HSR_HIP(hipGetLastError());
+  return HSR_OK;
+}
+
 hsr_status force_run(hsr_ctx *c, float *fx, float *fy, float *fz) {
   hipStream_t st = c->stream;
-  int64_t n = c->tree.n;
+  const int64_t n = c->tree.n;
+  ListState &LS = c->lists;
   c->kernel_timed = false;
   if (n > 0)
     k_fill3<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, fx, fy, fz);
-  if (c->lists.nleaves == 0) return HSR_OK;
-  const bool guard = c->law.guard != 0;
-  const int order = c->law.order;
+  if (LS.nleaves == 0 || LS.nitems == 0) return HSR_OK;
+  HSR_HIP(dbuf_reserve(c->s.qctr, 64));
+  unsigned long long *qctr = dptr<unsigned long long>(c->s.qctr);
+  k_zero_u64<<<1, 64, 0, st>>>(qctr);
+  const LawArgs A = law_args(c->law);
+  const int wpc = c->tuning.waves_per_cu > 0 ? c->tuning.waves_per_cu : 32;
+  const int64_t grid = std::min<int64_t>(LS.nitems, (int64_t)std::max(1, c->num_cus) * wpc);
+  const hsr_node *cn = dptr<hsr_node>(c->tree.canon_nodes);
+  const int32_t *leaves = dptr<int32_t>(LS.leaves);
+  const Item *items = dptr<Item>(LS.items);
+  float *part = dptr<float>(LS.part);
   // events bracket the leaf-kernel launch alone (the roofline's duration)
   HSR_HIP(hipEventRecord(c->ev[6], st));
-  if (order == 6) {
-    if (guard) launch_force<2, 6, true>(c, fx, fy, fz);
-    else launch_force<2, 6, false>(c, fx, fy, fz);
+#define LAUNCH(ORD, GRD)                                                          \
+  k_force<ORD, GRD><<<(unsigned)grid, 64, 0, st>>>(cn, leaves, items, LS.counts, \
+      dptr<int32_t>(LS.jidx), dptr<PJ>(c->tree.pj), A, qctr, fx, fy, fz, part)
+  if (c->law.order == 6) {
+    if (A.guard) LAUNCH(6, true); else LAUNCH(6, false);
   } else {
-    if (guard) launch_force<2, 5, true>(c, fx, fy, fz);
-    else launch_force<2, 5, false>(c, fx, fy, fz);
+    if (A.guard) LAUNCH(5, true); else LAUNCH(5, false);
   }
+#undef LAUNCH
   HSR_HIP(hipGetLastError());
   HSR_HIP(hipEventRecord(c->ev[7], st));
   c->kernel_timed = true;
+  if (LS.nmulti > 0)
+    k_reduce_chunks<<<(unsigned)LS.nmulti, 64, 0, st>>>(cn, leaves, LS.multi, LS.counts + 1, LS.leaf_pbase,
+                                                       LS.leaf_nchunk, part, fx, fy, fz);
+  HSR_HIP(hipGetLastError());
   return HSR_OK;
 }
 

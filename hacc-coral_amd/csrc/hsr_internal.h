@@ -109,16 +109,21 @@ struct ListState {
   DBuf seg;                   // int2-like (int64 a, int64 b) pairs
   DBuf ent_ptr;               // int64 [nleaves+1] list-entry CSR
   DBuf jidx;                  // int32 [nentries] index into PJ
-  DBuf items;                 // work items for the force kernel
-  int64_t nitems = 0;
+  DBuf items;                 // work items for the force kernel (sorted)
+  DBuf item_keys;             // sort scratch
+  DBuf plan;                  // per-leaf chunk plan
+  DBuf part;                  // chunk partial forces of long lists
+  int64_t nitems = 0, nmulti = 0, chunk = 0;
+  int64_t *multi = nullptr, *counts = nullptr, *leaf_pbase = nullptr;
+  int32_t *leaf_nchunk = nullptr;
   DBuf stack;                 // DFS stacks
 };
 
 struct Scratch {
   DBuf a, b, c, d, e, f, g, h, cub;
   DBuf x1, y1, z1, m1, p1, x0, y0, z0, m0, p0;
-  DBuf blocks, pblocks;
-  DBuf counters;
+  DBuf blocks, pblocks, rights;
+  DBuf counters, qctr;
   DBuf host_in[10];
 };
 
@@ -164,6 +169,7 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
                              const float *y, const float *z, const float *m);
 hsr_status lists_build_device(hsr_ctx *c, const float *min_force,
                               const float *max_force);
+hsr_status force_plan(hsr_ctx *c);
 hsr_status force_run(hsr_ctx *c, float *ax, float *ay, float *az);
 hsr_status kick_run(hsr_ctx *c, float fcoeff, float *vx, float *vy, float *vz,
                     const float *mass_sorted);

@@ -57,7 +57,8 @@ __device__ __forceinline__ const float *coord(int d, const float *X,
 }
 
 // ------------------------------------------------------------ cm (small)
-// One wave per node with count <= CM_SMALL (BGQCM.c:181-212).
+// One wave per node with count <= CM_SMALL (BGQCM.c:181-212).  The next
+// chunk's loads are issued before the current chunk is accumulated.
 __global__ void __launch_bounds__(64)
 k_cm_small(BNode *nodes, int64_t lb, int64_t le, const float *X,
            const float *Y, const float *Z, const float *M) {
@@ -70,13 +71,28 @@ k_cm_small(BNode *nodes, int64_t lb, int64_t le, const float *X,
   sd_t S[4] = {sd_zero(), sd_zero(), sd_zero(), sd_zero()};
   float mn[3] = {INFINITY, INFINITY, INFINITY};
   float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  float nw = 0.f, nx = 0.f, ny = 0.f, nz = 0.f;
+  bool nvld = l < cnt;
+  if (nvld) {
+    nw = M[off + l];
+    nx = X[off + l];
+    ny = Y[off + l];
+    nz = Z[off + l];
+  }
   for (int64_t c0 = 0; c0 < cnt; c0 += 64) {
-    int64_t k = c0 + l;
-    int nvalid = (int)min<int64_t>(64, cnt - c0);
+    const int nvalid = (int)min<int64_t>(64, cnt - c0);
+    const bool vld = nvld;
+    const float w = nw, px = nx, py = ny, pz = nz;
+    const int64_t k1 = c0 + 64 + l;
+    nvld = k1 < cnt;
+    if (nvld) {
+      nw = M[off + k1];
+      nx = X[off + k1];
+      ny = Y[off + k1];
+      nz = Z[off + k1];
+    }
     float v[4] = {0.f, 0.f, 0.f, 0.f};
-    if (k < cnt) {
-      float w = M[off + k];
-      float px = X[off + k], py = Y[off + k], pz = Z[off + k];
+    if (vld) {
       mn[0] = fminf(mn[0], px); mx[0] = fmaxf(mx[0], px);
       mn[1] = fminf(mn[1], py); mx[1] = fmaxf(mx[1], py);
       mn[2] = fminf(mn[2], pz); mx[2] = fmaxf(mx[2], pz);
@@ -201,80 +217,95 @@ __device__ __forceinline__ int32_t grid_exp_guess(float ein) {
   return (ef - 127) - 52;
 }
 
-// Fixed-grid maps of each block for its guessed binade.
+__device__ __forceinline__ float addend(int q, const float *X, const float *Y,
+                                        const float *Z, const float *M,
+                                        int64_t e) {
+  float w = M[e];
+  return q == 3 ? w : w * (q == 0 ? X[e] : (q == 1 ? Y[e] : Z[e]));
+}
+
+// Fixed-grid maps of each block for its guessed binade: the block is staged
+// in LDS, each lane composes 64 consecutive addends in order, then the 64
+// lane maps are composed in lane order.
 __global__ void __launch_bounds__(64)
 k_cm_blk_maps(CmBlock *blk, const Counters *ctr, const float *X,
               const float *Y, const float *Z, const float *M) {
+  __shared__ float buf[64 * 65];
   int64_t b = blockIdx.x;
   if (b >= ctr->cm_blocks) return;
   CmBlock &B = blk[b];
-  int l = lane_id();
-  int32_t E[4];
-  sd_map acc[4];
+  const int l = lane_id();
+  const int64_t start = B.start, len = B.len;
   for (int q = 0; q < 4; ++q) {
-    E[q] = grid_exp_guess(B.ein[q]);
-    acc[q] = sd_map_identity();
-    if (E[q] == INT32_MIN) acc[q].bad = 1;
-  }
-  for (int64_t c0 = 0; c0 < B.len; c0 += 64) {
-    int64_t k = c0 + l;
-    float v[4] = {0.f, 0.f, 0.f, 0.f};
-    if (k < B.len) {
-      int64_t e = B.start + k;
-      float w = M[e];
-      v[0] = w * X[e];
-      v[1] = w * Y[e];
-      v[2] = w * Z[e];
-      v[3] = w;
+    const int32_t E = grid_exp_guess(B.ein[q]);
+    sd_map acc = sd_map_identity();
+    if (E == INT32_MIN) {
+      acc.bad = 1;
+    } else {
+      __syncthreads();
+      for (int k = l; k < CM_BLK; k += 64) {
+        float v = k < len ? addend(q, X, Y, Z, M, start + k) : 0.f;
+        buf[(k >> 6) * 65 + (k & 63)] = v;
+      }
+      __syncthreads();
+      for (int k = 0; k < 64; ++k)
+        acc = sd_map_compose(acc, sd_map_of(buf[l * 65 + k], E));
+      acc = wave_compose_maps(acc);
     }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (acc[q].bad) continue;  // uniform
-      sd_map m = wave_compose_maps(sd_map_of(v[q], E[q]));
-      acc[q] = sd_map_compose(acc[q], m);
-    }
-  }
-  if (l == 0) {
-    for (int q = 0; q < 4; ++q) {
-      B.d0[q] = acc[q].d0;
-      B.d1[q] = acc[q].d1;
-      B.bad[q] = acc[q].bad;
-      B.E[q] = E[q];
+    if (l == 0) {
+      B.d0[q] = acc.d0;
+      B.d1[q] = acc.d1;
+      B.bad[q] = acc.bad;
+      B.E[q] = E;
     }
   }
 }
 
-// Chain the block maps of each big node in order; blocks whose map does not
-// apply (binade guess wrong, binade crossed) are re-accumulated chunk by
-// chunk from the particles.
+// Chain the block maps of each big node in order (64 blocks' maps are read
+// at once, one per lane, then applied serially); a block whose map does not
+// apply (binade guess wrong or binade crossed) is staged in LDS and
+// re-accumulated chunk by chunk.
 __global__ void __launch_bounds__(64)
 k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, const CmBlock *blk,
                const float *X, const float *Y, const float *Z,
                const float *M) {
+  __shared__ float buf[CM_BLK];
   int64_t t = lb + blockIdx.x;
   if (t >= le) return;
   if (nodes[t].count <= CM_SMALL) return;
-  int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
-  int l = lane_id();
+  const int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
+  const int l = lane_id();
   sd_t S[4] = {sd_zero(), sd_zero(), sd_zero(), sd_zero()};
-  for (int64_t b = b0; b < b0 + nb; ++b) {
-    const CmBlock &B = blk[b];
-    for (int q = 0; q < 4; ++q) {
-      sd_map g;
-      g.d0 = B.d0[q];
-      g.d1 = B.d1[q];
-      g.bad = B.bad[q];
-      if (sd_map_apply(&S[q], g, B.E[q])) continue;  // uniform decision
-      for (int64_t c0 = 0; c0 < B.len; c0 += 64) {
-        int64_t k = c0 + l;
-        int nvalid = (int)min<int64_t>(64, B.len - c0);
-        float v = 0.f;
-        if (k < B.len) {
-          int64_t e = B.start + k;
-          float w = M[e];
-          v = q == 3 ? w : w * (q == 0 ? X[e] : (q == 1 ? Y[e] : Z[e]));
+  for (int64_t bb = 0; bb < nb; bb += 64) {
+    uint64_t md0[4] = {0, 0, 0, 0}, md1[4] = {0, 0, 0, 0};
+    int mbad[4] = {1, 1, 1, 1}, mE[4] = {0, 0, 0, 0};
+    if (bb + l < nb) {
+      const CmBlock &B = blk[b0 + bb + l];
+      for (int q = 0; q < 4; ++q) {
+        md0[q] = B.d0[q];
+        md1[q] = B.d1[q];
+        mbad[q] = B.bad[q];
+        mE[q] = B.E[q];
+      }
+    }
+    const int cnt = (int)min<int64_t>(64, nb - bb);
+    for (int i = 0; i < cnt; ++i) {
+      for (int q = 0; q < 4; ++q) {
+        sd_map g;
+        g.d0 = readlane_u64(md0[q], i);
+        g.d1 = readlane_u64(md1[q], i);
+        g.bad = readlane_i32(mbad[q], i);
+        if (sd_map_apply(&S[q], g, readlane_i32(mE[q], i))) continue;  // uniform
+        const CmBlock &B = blk[b0 + bb + i];
+        const int64_t start = B.start, len = B.len;
+        __syncthreads();
+        for (int k = l; k < len; k += 64) buf[k] = addend(q, X, Y, Z, M, start + k);
+        __syncthreads();
+        for (int64_t c0 = 0; c0 < len; c0 += 64) {
+          int nvalid = (int)min<int64_t>(64, len - c0);
+          float v = (c0 + l < len) ? buf[c0 + l] : 0.f;
+          sd_chunk(S[q], v, nvalid);
         }
-        sd_chunk(S[q], v, nvalid);
       }
     }
   }
@@ -409,7 +440,7 @@ k_part_scan(BNode *nodes, int64_t lb, int64_t le, PartBlock *pb) {
 __global__ void __launch_bounds__(256)
 k_part_rank(const BNode *nodes, const PartBlock *pb, const Counters *ctr,
             const float *X, const float *Y, const float *Z, int32_t *idx,
-            int32_t *dest) {
+            int32_t *rights, int32_t *dest) {
   int64_t b = blockIdx.x;
   if (b >= ctr->part_blocks) return;
   const PartBlock &B = pb[b];
@@ -436,11 +467,12 @@ k_part_rank(const BNode *nodes, const PartBlock *pb, const Counters *ctr,
     }
     if (valid) {
       int64_t p = e - off;
+      int64_t lefts_before = run + wbase + pre;
       if (f) {
-        int64_t rank = run + wbase + pre;
-        idx[off + rank] = (int32_t)p;
-        dest[e] = (int32_t)rank;
+        idx[off + lefts_before] = (int32_t)p;
+        dest[e] = (int32_t)lefts_before;
       } else {
+        rights[off + (p - lefts_before)] = (int32_t)p;
         dest[e] = p >= is ? (int32_t)p : -1;
       }
     }
@@ -451,26 +483,42 @@ k_part_rank(const BNode *nodes, const PartBlock *pb, const Counters *ctr,
 
 // Swap chains (RCBForceTree.cxx:647-669): an element with key >= pivot at
 // p < is ends at the first idx-iterate p -> idx[p] -> ... that is >= is.
+// Between the k-th and (k+1)-th key >= pivot (positions rights[k-1] and
+// rights[k]) every position is a left one, so idx[j] = j + k there: a chain
+// that entered such a stretch with step k continues f, f+k, f+2k, ... up to
+// rights[k]; the walk therefore jumps whole stretches arithmetically.
 __global__ void __launch_bounds__(256)
 k_part_chain(const BNode *nodes, const PartBlock *pb, const Counters *ctrc,
-             const int32_t *idx, int32_t *dest, Counters *ctr) {
+             const int32_t *idx, const int32_t *rights, int32_t *dest,
+             Counters *ctr) {
   int64_t b = blockIdx.x;
   if (b >= ctrc->part_blocks) return;
   const PartBlock &B = pb[b];
   const BNode &n = nodes[B.node];
-  int64_t off = n.offset, is = n.is;
+  const int64_t off = n.offset, is = n.is, cnt = n.count, nr = cnt - is;
   for (int k = threadIdx.x; k < B.len; k += 256) {
     int64_t e = B.start + k;
     if (dest[e] != -1) continue;
     int64_t j = e - off;
-    int hop = 0;
-    while (j < is && hop < MAXHOP) {
-      j = idx[off + j];
-      ++hop;
+    bool done = false;
+    for (int it = 0; it < MAXHOP; ++it) {
+      const int64_t f = idx[off + j];  // f > j
+      if (f >= is) {
+        dest[e] = (int32_t)f;
+        done = true;
+        break;
+      }
+      const int64_t step = f - j;  // keys >= pivot below f
+      const int64_t lim = step < nr ? rights[off + step] : cnt;
+      const int64_t v = f + ((is - f + step - 1) / step) * step;
+      if (v < lim) {
+        dest[e] = (int32_t)v;
+        done = true;
+        break;
+      }
+      j = f + ((lim - 1 - f) / step) * step;
     }
-    if (j >= is) {
-      dest[e] = (int32_t)j;
-    } else {
+    if (!done) {
       dest[e] = -2 - (int32_t)j;  // unresolved; continue from j
       atomicAdd((unsigned long long *)&ctr->unresolved, 1ull);
     }
@@ -768,6 +816,7 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
   HSR_HIP(dbuf_reserve(S.a, sizeof(int32_t) * nn));  // idx
   HSR_HIP(dbuf_reserve(S.b, sizeof(int32_t) * nn));  // dest
   HSR_HIP(dbuf_reserve(S.c, sizeof(int32_t) * nn));  // jmp
+  HSR_HIP(dbuf_reserve(S.rights, sizeof(int32_t) * nn));
   HSR_HIP(dbuf_reserve(S.counters, sizeof(Counters) + 64));
   HSR_HIP(dbuf_reserve(S.d, sizeof(int64_t) * (nboxes + 1) + sizeof(float) * 6 * nboxes + 64));
 
@@ -780,6 +829,7 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
   int32_t *idx = dptr<int32_t>(S.a);
   int32_t *dest = dptr<int32_t>(S.b);
   int32_t *jmp = dptr<int32_t>(S.c);
+  int32_t *rights = dptr<int32_t>(S.rights);
   Counters *ctr = dptr<Counters>(S.counters);
 
   if (n > 0) {
@@ -844,8 +894,11 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     unsigned pg = (unsigned)std::max<int64_t>(1, pcap);
     k_part_count<<<pg, 256, 0, st>>>(nodes, pb, ctr, X[cur], Y[cur], Z[cur]);
     k_part_scan<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, pb);
-    k_part_rank<<<pg, 256, 0, st>>>(nodes, pb, ctr, X[cur], Y[cur], Z[cur], idx, dest);
-    k_part_chain<<<pg, 256, 0, st>>>(nodes, pb, ctr, idx, dest, ctr);
+    // children depend only on the partition counts: create them now so a
+    // single host synchronisation per level reads every counter
+    k_make_children<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, le, ctr);
+    k_part_rank<<<pg, 256, 0, st>>>(nodes, pb, ctr, X[cur], Y[cur], Z[cur], idx, rights, dest);
+    k_part_chain<<<pg, 256, 0, st>>>(nodes, pb, ctr, idx, rights, dest, ctr);
     HSR_HIP(hipMemcpyAsync(&hc, ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
     HSR_HIP(hipStreamSynchronize(st));
     if (hc.unresolved > 0) {
@@ -857,9 +910,6 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     k_part_scatter<<<pg, 256, 0, st>>>(nodes, pb, ctr, dest, X[cur], Y[cur], Z[cur], Mm[cur], P[cur],
                                         X[cur ^ 1], Y[cur ^ 1], Z[cur ^ 1], Mm[cur ^ 1], P[cur ^ 1]);
     k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_DEGEN, P[cur ^ 1], Pfin);
-    k_make_children<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, le, ctr);
-    HSR_HIP(hipMemcpyAsync(&hc, ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
-    HSR_HIP(hipStreamSynchronize(st));
     HSR_HIP(hipGetLastError());
     T.level_start.push_back(le + 2 * hc.nsplit);
     active = hc.active_next;

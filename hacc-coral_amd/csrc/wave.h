@@ -68,16 +68,10 @@ __device__ __forceinline__ sd_map wave_compose_maps(sd_map m) {
 
 // Accumulate one 64-addend chunk (lane k holds addend k; lanes >= nvalid
 // hold 0) into the wave-uniform exact double S, reproducing k sequential
-// correctly rounded double additions.
+// correctly rounded double additions.  The exact-sum test is cheapest and
+// covers most chunks; the fixed-grid map covers chunks that round inside one
+// binade; the serial path covers the rest (binade crossings, signs).
 __device__ __forceinline__ void sd_chunk(sd_t &S, float v, int nvalid) {
-  if (S.m != 0 && S.s == 0) {
-    sd_map mp = wave_compose_maps(sd_map_of(v, S.e));
-    sd_t S2 = S;
-    if (sd_map_apply(&S2, mp, S.e)) {
-      S = S2;
-      return;
-    }
-  }
   {
     int t = wave_min_i32(min(sd_lsb_f32(v), sd_lsb(S)));
     uint64_t u = sd_units_f32(v, t);
@@ -91,8 +85,26 @@ __device__ __forceinline__ void sd_chunk(sd_t &S, float v, int nvalid) {
       }
     }
   }
+  if (S.m != 0 && S.s == 0) {
+    sd_map mp = wave_compose_maps(sd_map_of(v, S.e));
+    sd_t S2 = S;
+    if (sd_map_apply(&S2, mp, S.e)) {
+      S = S2;
+      return;
+    }
+  }
   for (int k = 0; k < nvalid; ++k) {
     float vk = __shfl(v, k);
     S = sd_add_f32(S, vk);
   }
+}
+
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int lane) {
+  uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+  uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ int readlane_i32(int v, int lane) {
+  return __builtin_amdgcn_readlane(v, lane);
 }

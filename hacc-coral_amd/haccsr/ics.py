@@ -129,7 +129,7 @@ def clustered(n_side: int, box: float, n_halos: int, seed: int,
     disp = []
     for kk in (kx, ky, kz):
         psi_k = (1j * kk / k2).astype(np.complex64) * dk
-        disp.append(np.fft.irfftn(psi_k, s=(n_side,) * 3).astype(np.float32))
+        disp.append(np.fft.irfftn(psi_k, s=(n_side,) * 3, axes=(0, 1, 2)).astype(np.float32))
         del psi_k
     del dk
     rms = math.sqrt(sum(float(np.mean(d.astype(np.float64) ** 2)) for d in disp))

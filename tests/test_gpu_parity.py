@@ -73,11 +73,18 @@ def listed_mask(tree, lists):
 
 
 def check_forces(tp, tree, lists, ctx, tol=1e-5):
+    """hsr_forces returns the Step16 acceleration a_i = sum_j m_j f d
+    (BGQStep16.c:167-187); the reference's generic nbody1 accumulates
+    m_i * a_i per pair (RCBForceTree.cxx:610).  Both oracle restatements are
+    checked: the Step16 one directly, nbody1 after dividing by m_i."""
     ax, ay, az = ctx.forces()
-    rx, ry, rz = O.forces_f32(tp, tree, lists)
+    rx, ry, rz = O.forces_step16(tp, tree, lists)
     sel = listed_mask(tree, lists)
     G = np.stack([ax, ay, az], 1)[sel]
     R = np.stack([rx, ry, rz], 1)[sel]
+    nx, ny, nz = O.forces_f32(tp, tree, lists)
+    Nb = (np.stack([nx, ny, nz], 1) / tree.mass[:, None])[sel]
+    assert rel_rms(G, Nb) <= tol, f"vs nbody1/m_i: {rel_rms(G, Nb):.3e}"
     e = rel_rms(G, R)
     assert np.all(np.stack([ax, ay, az], 1)[~sel] == 0)
     assert e <= tol, f"relative RMS {e:.3e} > {tol}"
@@ -116,3 +123,75 @@ def test_config_a_parity():
     st = ctx.stats()
     say(f"  nodes {st.nodes} leaves {st.listed_leaves} pairs {st.pairs} rel rms {e:.3e} "
         f"build {st.build_ms:.2f} ms lists {st.lists_ms:.2f} ms force {st.force_ms:.2f} ms")
+
+
+def test_clustered_nd192_multipass():
+    say("case clustered 48^3 (config-B generator), nd=192")
+    load = ics.config_load("B", n_side=48)
+    lo, hi = load.tree_box
+    flo, fhi = load.force_box
+    m = np.ones(load.n, np.float32)
+    tp, tree, lists, ctx, arrs = run_both(load.x, load.y, load.z, m, lo, hi, flo, fhi, 192)
+    check_tree(tree, ctx, arrs)
+    check_lists(lists, ctx)
+    e = check_forces(tp, tree, lists, ctx)
+    say(f"  rel rms {e:.3e} depth {ctx.stats().max_depth}")
+
+
+def test_chunked_lists_and_determinism():
+    say("case chunked lists (chunk=256) and bitwise determinism")
+    x, y, z, m = uniform(30000, 14.0, 3)
+    lo, hi = [0, 0, 0], [14.0] * 3
+    flo, fhi = [1.0] * 3, [13.0] * 3
+    tp, tree, lists, ctx, arrs = run_both(x, y, z, m, lo, hi, flo, fhi, 96)
+    e1 = check_forces(tp, tree, lists, ctx)
+    a1 = np.stack(ctx.forces(), 1)
+    a2 = np.stack(ctx.forces(), 1)
+    assert np.array_equal(a1.view(np.uint32), a2.view(np.uint32)), "not bitwise reproducible"
+    ctx.set_tuning(chunk=256)
+    ctx.lists(flo, fhi)
+    e2 = check_forces(tp, tree, lists, ctx)
+    b1 = np.stack(ctx.forces(), 1)
+    assert np.array_equal(b1.view(np.uint32), np.stack(ctx.forces(), 1).view(np.uint32))
+    say(f"  rel rms unchunked {e1:.3e} chunked {e2:.3e}")
+
+
+def test_duplicates_and_degenerate_nodes():
+    say("case duplicate positions (unsplittable nodes)")
+    x, y, z, m = uniform(4000, 10.0, 4)
+    dup = np.full(300, 5.25, np.float32)
+    x = np.concatenate([x, dup]); y = np.concatenate([y, dup + 0.5]); z = np.concatenate([z, dup - 0.5])
+    m = np.ones(x.shape[0], np.float32)
+    tp, tree, lists, ctx, arrs = run_both(x, y, z, m, [0, 0, 0], [10.0] * 3, [1.0] * 3, [9.0] * 3, 32)
+    check_tree(tree, ctx, arrs)
+    check_lists(lists, ctx)
+    check_forces(tp, tree, lists, ctx)
+    st = ctx.stats()
+    assert st.empty_leaves > 0 and st.max_ppn >= 300
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 17])
+def test_tiny_inputs(n):
+    say(f"case tiny n={n}")
+    x, y, z, m = uniform(n, 4.0, 5)
+    tp, tree, lists, ctx, arrs = run_both(x, y, z, m, [0, 0, 0], [4.0] * 3, [0.5] * 3, [3.5] * 3, 8)
+    if n:
+        check_tree(tree, ctx, arrs)
+        check_lists(lists, ctx)
+        ax, ay, az = ctx.forces()
+        rx, ry, rz = O.forces_f32(tp, tree, lists)
+        assert np.allclose(ax, rx, rtol=1e-5, atol=1e-6)
+
+
+def test_mixed_magnitudes_and_masses():
+    say("case mixed coordinate magnitudes and non-unit masses (exact cm paths)")
+    rng = np.random.default_rng(12)
+    n = 20000
+    p = (rng.random((n, 3)) * 30.0).astype(np.float32)
+    p[: n // 10] *= np.float32(1e-4)          # coordinates near 0 force rounding
+    m = (0.5 + rng.random(n)).astype(np.float32)
+    tp, tree, lists, ctx, arrs = run_both(p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), m,
+                                          [0, 0, 0], [30.0] * 3, [1.0] * 3, [29.0] * 3, 48)
+    check_tree(tree, ctx, arrs)
+    check_lists(lists, ctx)
+    check_forces(tp, tree, lists, ctx)
