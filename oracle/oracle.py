@@ -107,7 +107,8 @@ def ref():
         for name in ("hsrref_sr_poly_f_over_r", "hsrref_sr_fit_f_over_r"):
             getattr(R, name).argtypes = [C.c_float, C.c_float]
             getattr(R, name).restype = C.c_float
-        for name in ("hsrref_fgor_r2", "hsrref_poly_eval", "hsrref_newton_f_over_r"):
+        for name in ("hsrref_fgor_r2", "hsrref_poly_eval", "hsrref_newton_f_over_r",
+                     "hsrref_interp_eval"):
             getattr(R, name).argtypes = [C.c_float]
             getattr(R, name).restype = C.c_float
         R.hsrref_fgrid_rmax.restype = C.c_float

@@ -53,6 +53,13 @@ float hsrref_poly_eval(float r2) {
   return ev.eval(r2);
 }
 
+// FGridEvalInterp(FGrid, 1024)::eval(r2) (ForceLaw.cxx:145-171).
+float hsrref_interp_eval(float r2) {
+  static FGrid fg;
+  static FGridEvalInterp ev(&fg, 1024);
+  return ev.eval(r2);
+}
+
 float hsrref_newton_f_over_r(float r2) {
   ForceLawNewton fl;
   return fl.f_over_r(r2);

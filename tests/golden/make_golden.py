@@ -6,8 +6,9 @@ ForceLaw.cxx and BGQCM.c where they lie).  Output:
                     (BGQCM.c:181-212) on ranges chosen to make the double
                     sums round (mixed magnitudes, ties, long runs)
   ref_forcelaw.npz  ForceLawSR(FGridEvalPoly) and ForceLawSR(FGridEvalFit)
-                    f_over_r, FGridEvalFit::eval and FGridEvalPoly::eval on
-                    an r^2 grid, plus FGrid::rmax (ForceLaw.cxx)
+                    f_over_r, FGridEvalFit::eval, FGridEvalPoly::eval and
+                    FGridEvalInterp(1024)::eval on an r^2 grid, plus
+                    FGrid::rmax (ForceLaw.cxx)
 """
 import ctypes as C
 import os
@@ -72,6 +73,7 @@ def main():
                         sr_poly6=f(R.hsrref_sr_poly_f_over_r, float(rsm)),
                         sr_fit=f(R.hsrref_sr_fit_f_over_r, float(rsm)),
                         fgor=f(R.hsrref_fgor_r2), poly6=f(R.hsrref_poly_eval),
+                        interp1024=f(R.hsrref_interp_eval),
                         rmax=np.float32(R.hsrref_fgrid_rmax()))
     print("wrote", os.listdir(HERE))
 

@@ -195,3 +195,58 @@ def test_mixed_magnitudes_and_masses():
     check_tree(tree, ctx, arrs)
     check_lists(lists, ctx)
     check_forces(tp, tree, lists, ctx)
+
+
+def test_ensemble_boxes_one_launch():
+    """Independent boxes in one context (hsr_build_many / hsr_lists_many):
+    every box's tree, lists and forces equal the oracle run on that box
+    alone; one persistent launch covers all boxes' lists."""
+    say("case ensemble of 6 boxes")
+    rng = np.random.default_rng(21)
+    boxes = []
+    for b in range(6):
+        ns = int(rng.integers(12, 22))
+        load = ics.add_ghosts(*ics.plummer_mix(ns ** 3, float(ns), 1100 + b), float(ns), 3.2)
+        load, _ = ics.cell_sort(load)
+        boxes.append(load)
+    offs = np.cumsum([0] + [b.n for b in boxes]).astype(np.int64)
+    X = np.concatenate([b.x for b in boxes]); Y = np.concatenate([b.y for b in boxes])
+    Z = np.concatenate([b.z for b in boxes]); M = np.ones(X.shape[0], np.float32)
+    ids = np.arange(X.shape[0], dtype=np.int64)
+    lo = np.stack([b.tree_box[0] for b in boxes]); hi = np.stack([b.tree_box[1] for b in boxes])
+    flo = np.stack([b.force_box[0] for b in boxes]); fhi = np.stack([b.force_box[1] for b in boxes])
+    ctx = Context(nd=64)
+    ctx.build_many(offs, X, Y, Z, M, lo, hi, ids=ids)
+    ctx.lists_many(flo, fhi)
+    ax, ay, az = ctx.forces()
+    nodes = ctx.nodes()
+    leaves, seg_ptr, sa, sb, ent = ctx.get_lists()
+    base = 0
+    lk = 0
+    for b, load in enumerate(boxes):
+        tp = O.TreeParams(lo[b], hi[b], flo[b], fhi[b], nd=64)
+        tree = O.build_tree(tp, load.x, load.y, load.z)
+        lists = O.build_lists(tp, tree)
+        a, e = offs[b], offs[b + 1]
+        assert np.array_equal(ids[a:e] - a, tree.id), f"box {b} permutation"
+        nn = tree.nodes.shape[0]
+        mine = nodes[base:base + nn]
+        for f in ("count", "cl", "cr", "ppm", "tdr", "xmin", "xmax", "xc"):
+            got, ref = mine[f].copy(), tree.nodes[f]
+            if f in ("cl", "cr"):
+                got = np.where(got > 0, got - base, 0)
+            if got.dtype.kind == "f":
+                got, ref = got.view(np.uint32), ref.view(np.uint32)
+            assert np.array_equal(got, ref), f"box {b} node field {f}"
+        assert np.array_equal(mine["offset"] - np.where(mine["count"] > 0, a, 0), tree.nodes["offset"])
+        nl = lists.leaves.shape[0]
+        assert np.array_equal(leaves[lk:lk + nl] - base, lists.leaves), f"box {b} leaves"
+        assert np.array_equal(ent[lk:lk + nl], lists.entries), f"box {b} list lengths"
+        rx, ry, rz = O.forces_step16(tp, tree, lists)
+        sel = listed_mask(tree, lists)
+        G = np.stack([ax[a:e], ay[a:e], az[a:e]], 1)[sel]
+        R = np.stack([rx, ry, rz], 1)[sel]
+        assert rel_rms(G, R) <= 1e-5, f"box {b} forces {rel_rms(G, R):.3e}"
+        base += nn
+        lk += nl
+    say(f"  {len(boxes)} boxes, {ctx.stats().listed_leaves} listed leaves, pairs {ctx.stats().pairs}")

@@ -299,3 +299,23 @@ def test_cell_sort_is_stable_counting_sort():
     order = np.argsort(key, kind="stable")
     assert np.array_equal(perm[order], np.arange(5000))
     assert inside == ok.sum()
+
+
+# ------------------------------------------------- drop-in host force laws
+def test_dropin_forcelaw_host_matches_reference_bitwise():
+    """hacc-coral_amd/host/ForceLaw.cxx (the drop-in's host ForceLaw
+    hierarchy) reproduces the reference's ForceLaw.cxx values bit for bit."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "hacc-coral_amd", "bin", "forcelaw_values")
+    if not os.path.exists(exe):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(root, "hacc-coral_amd"),
+                               os.path.join(root, "hacc-coral_amd", "bin", "forcelaw_values")])
+    d = np.load(os.path.join(GOLDEN, "ref_forcelaw.npz"))
+    inp = "\n".join(repr(float(v)) for v in d["r2"]) + "\n"
+    out = subprocess.run([exe], input=inp, capture_output=True, text=True, check=True).stdout.split("\n")
+    rows = [r.split() for r in out if r and not r.startswith("rmax")]
+    got = np.array([[int(h, 16) for h in r] for r in rows], np.uint32)
+    assert got.shape[0] == d["r2"].shape[0]
+    for k, name in enumerate(["sr_poly6", "sr_fit", "fgor", "poly6", "interp1024"]):
+        assert np.array_equal(got[:, k], f32bits(d[name])), name
