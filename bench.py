@@ -136,6 +136,7 @@ def main():
         ctx = ctxs.get(nd)
         if ctx is None:
             ctx = ctxs[nd] = Context(nd=nd)
+            ctx.set_tuning(use_graph=1)
         for k in src:
             work[k].copy_(src[k])
         torch.cuda.synchronize()

@@ -309,6 +309,8 @@ hsr_status hsr_destroy(hsr_ctx *c) {
   for (DBuf *b : bufs) dbuf_free(*b);
   for (int k = 0; k < 10; ++k) dbuf_free(c->s.host_in[k]);
   for (int k = 0; k < 8; ++k) (void)hipEventDestroy(c->ev[k]);
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return HSR_OK;
@@ -366,6 +368,7 @@ hsr_status hsr_lists(hsr_ctx *c, const float min_force[3],
   if (s) return s;
   s = force_plan(c);
   if (s) return s;
+  c->lists_gen++;
   HSR_HIP(hipEventRecord(c->ev[3], c->stream));
   HSR_HIP(hipEventSynchronize(c->ev[3]));
   float ms = 0.f;
@@ -392,6 +395,7 @@ hsr_status hsr_lists_many(hsr_ctx *c, const float *min_force,
   if (s) return s;
   s = force_plan(c);
   if (s) return s;
+  c->lists_gen++;
   HSR_HIP(hipEventRecord(c->ev[3], c->stream));
   HSR_HIP(hipEventSynchronize(c->ev[3]));
   float ms = 0.f;

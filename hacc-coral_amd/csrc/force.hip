@@ -397,6 +397,7 @@ hsr_status force_plan(hsr_ctx *c) {
   int64_t hc[2] = {nitems, nmulti};
   HSR_HIP(hipMemcpyAsync(counts, hc, sizeof(hc), hipMemcpyHostToDevice, st));
   HSR_HIP(hipStreamSynchronize(st));
+  HSR_HIP(dbuf_reserve(c->s.qctr, 64));
   LS.multi = multi;
   LS.counts = counts;
   LS.leaf_pbase = pbase;
@@ -405,15 +406,60 @@ hsr_status force_plan(hsr_ctx *c) {
   return HSR_OK;
 }
 
+static hsr_status force_enqueue(hsr_ctx *c, float *fx, float *fy, float *fz,
+                                bool timed);
+
+// The force phase: zero F, reset the work queue, persistent leaf kernel,
+// chunk reduction.  With tuning.use_graph the four launches are captured
+// once per list build (and output buffers) and replayed as one graph.
 hsr_status force_run(hsr_ctx *c, float *fx, float *fy, float *fz) {
+  hipStream_t st = c->stream;
+  if (!c->tuning.use_graph) return force_enqueue(c, fx, fy, fz, true);
+  const bool stale = c->graph_exec == nullptr || c->graph_gen != c->lists_gen ||
+                     c->graph_out[0] != fx || c->graph_out[1] != fy ||
+                     c->graph_out[2] != fz;
+  if (stale) {
+    if (c->graph_exec) {
+      HSR_HIP(hipGraphExecDestroy(c->graph_exec));
+      c->graph_exec = nullptr;
+    }
+    if (c->graph) {
+      HSR_HIP(hipGraphDestroy(c->graph));
+      c->graph = nullptr;
+    }
+    HSR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hsr_status s = force_enqueue(c, fx, fy, fz, false);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (s != HSR_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return s;
+    }
+    HSR_HIP(e);
+    c->graph = g;
+    HSR_HIP(hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+    c->graph_gen = c->lists_gen;
+    c->graph_out[0] = fx;
+    c->graph_out[1] = fy;
+    c->graph_out[2] = fz;
+  }
+  HSR_HIP(hipEventRecord(c->ev[6], st));
+  HSR_HIP(hipGraphLaunch(c->graph_exec, st));
+  HSR_HIP(hipEventRecord(c->ev[7], st));
+  c->kernel_timed = true;
+  c->graph_launches++;
+  return HSR_OK;
+}
+
+static hsr_status force_enqueue(hsr_ctx *c, float *fx, float *fy, float *fz,
+                                bool timed) {
   hipStream_t st = c->stream;
   const int64_t n = c->tree.n;
   ListState &LS = c->lists;
-  c->kernel_timed = false;
+  if (timed) c->kernel_timed = false;
   if (n > 0)
     k_fill3<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, fx, fy, fz);
   if (LS.nleaves == 0 || LS.nitems == 0) return HSR_OK;
-  HSR_HIP(dbuf_reserve(c->s.qctr, 64));
   unsigned long long *qctr = dptr<unsigned long long>(c->s.qctr);
   k_zero_u64<<<1, 64, 0, st>>>(qctr);
   const LawArgs A = law_args(c->law);
@@ -424,7 +470,7 @@ hsr_status force_run(hsr_ctx *c, float *fx, float *fy, float *fz) {
   const Item *items = dptr<Item>(LS.items);
   float *part = dptr<float>(LS.part);
   // events bracket the leaf-kernel launch alone (the roofline's duration)
-  HSR_HIP(hipEventRecord(c->ev[6], st));
+  if (timed) HSR_HIP(hipEventRecord(c->ev[6], st));
 #define LAUNCH(ORD, GRD)                                                          \
   k_force<ORD, GRD><<<(unsigned)grid, 64, 0, st>>>(cn, leaves, items, LS.counts, \
       dptr<int32_t>(LS.jidx), dptr<PJ>(c->tree.pj), A, qctr, fx, fy, fz, part)
@@ -435,8 +481,10 @@ hsr_status force_run(hsr_ctx *c, float *fx, float *fy, float *fz) {
   }
 #undef LAUNCH
   HSR_HIP(hipGetLastError());
-  HSR_HIP(hipEventRecord(c->ev[7], st));
-  c->kernel_timed = true;
+  if (timed) {
+    HSR_HIP(hipEventRecord(c->ev[7], st));
+    c->kernel_timed = true;
+  }
   if (LS.nmulti > 0)
     k_reduce_chunks<<<(unsigned)LS.nmulti, 64, 0, st>>>(cn, leaves, LS.multi, LS.counts + 1, LS.leaf_pbase,
                                                        LS.leaf_nchunk, part, fx, fy, fz);

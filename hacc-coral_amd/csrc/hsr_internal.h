@@ -148,6 +148,13 @@ struct hsr_ctx {
   bool built = false, listed = false;
   double build_ms = 0, lists_ms = 0, force_ms = 0, kernel_ms = 0;
   bool kernel_timed = false;
+  // force phase captured as a hipGraph (tuning.use_graph)
+  int64_t lists_gen = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int64_t graph_gen = -1;
+  const float *graph_out[3] = {nullptr, nullptr, nullptr};
+  int64_t graph_launches = 0;
 };
 
 void hsr_set_error(const std::string &msg);
