@@ -302,6 +302,8 @@ hsr_status hsr_destroy(hsr_ctx *c) {
                   &c->lists.leaves, &c->lists.seg_ptr, &c->lists.seg, &c->lists.ent_ptr,
                   &c->lists.jidx, &c->lists.items, &c->lists.stack, &c->lists.item_keys,
                   &c->lists.plan, &c->lists.part, &c->s.qctr,
+                  &c->adj.fw, &c->adj.fw_items, &c->adj.tr, &c->adj.tr_items, &c->adj.seg,
+                  &c->adj.tj, &c->adj.par, &c->adj.counts, &c->adj.qctr, &c->adj.part, &c->adj.H,
                   &c->s.a, &c->s.b, &c->s.c, &c->s.d, &c->s.e, &c->s.f, &c->s.g, &c->s.h,
                   &c->s.cub, &c->s.x1, &c->s.y1, &c->s.z1, &c->s.m1, &c->s.p1,
                   &c->s.x0, &c->s.y0, &c->s.z0, &c->s.m0, &c->s.p0, &c->s.blocks, &c->s.rights,
@@ -487,11 +489,35 @@ hsr_status hsr_adjoint(hsr_ctx *c, const float *lx, const float *ly,
     g_err = "hsr_adjoint: no lists";
     return HSR_ERR_STATE;
   }
-  if (!(ptr_flags & HSR_DEVICE_PTRS)) {
-    g_err = "hsr_adjoint: device pointers required";
+  if (!lx || !ly || !lz || !gx || !gy || !gz) {
+    g_err = "hsr_adjoint: null argument";
     return HSR_ERR_ARG;
   }
-  return adjoint_run(c, lx, ly, lz, gx, gy, gz);
+  const int64_t n = c->tree.n;
+  const float *dl[3] = {lx, ly, lz};
+  float *dg[3] = {gx, gy, gz};
+  if (!(ptr_flags & HSR_DEVICE_PTRS)) {
+    HSR_HIP(dbuf_reserve(c->s.host_in[0], sizeof(float) * 6 * std::max<int64_t>(n, 1)));
+    float *b = dptr<float>(c->s.host_in[0]);
+    for (int k = 0; k < 3; ++k) {
+      if (n) HSR_HIP(hipMemcpyAsync(b + k * n, dl[k], sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+      dl[k] = b + k * n;
+      dg[k] = b + (3 + k) * n;
+    }
+  }
+  s = adjoint_run(c, dl[0], dl[1], dl[2], dg[0], dg[1], dg[2]);
+  if (s) return s;
+  if (!(ptr_flags & HSR_DEVICE_PTRS) && n) {
+    HSR_HIP(hipMemcpyAsync(gx, dg[0], sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    HSR_HIP(hipMemcpyAsync(gy, dg[1], sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    HSR_HIP(hipMemcpyAsync(gz, dg[2], sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+  }
+  HSR_HIP(hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  HSR_HIP(hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
+  c->kernel_ms = ms;
+  c->adjoint_ms = ms;
+  return HSR_OK;
 }
 
 hsr_status hsr_get_stats(hsr_ctx *c, hsr_stats *st) {
@@ -529,6 +555,7 @@ hsr_status hsr_get_stats(hsr_ctx *c, hsr_stats *st) {
   st->lists_ms = c->lists_ms;
   st->force_ms = c->force_ms;
   st->kernel_ms = c->kernel_ms;
+  st->adjoint_ms = c->adjoint_ms;
   return HSR_OK;
 }
 

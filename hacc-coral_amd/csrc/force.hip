@@ -510,10 +510,3 @@ hsr_status kick_run(hsr_ctx *c, float fcoeff, float *vx, float *vy, float *vz,
   HSR_HIP(hipGetLastError());
   return HSR_OK;
 }
-
-hsr_status adjoint_run(hsr_ctx *c, const float *lx, const float *ly,
-                       const float *lz, float *gx, float *gy, float *gz) {
-  (void)c; (void)lx; (void)ly; (void)lz; (void)gx; (void)gy; (void)gz;
-  hsr_set_error("hsr_adjoint: not available in this build");
-  return HSR_ERR_UNSUPPORTED;
-}

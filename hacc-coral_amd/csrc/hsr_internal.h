@@ -119,6 +119,17 @@ struct ListState {
   DBuf stack;                 // DFS stacks
 };
 
+// Adjoint plan: item queues of the target and source passes, the
+// transposed (source-major) lists and the pseudo-particle segments.
+struct AdjState {
+  int64_t gen = -1;
+  DBuf fw, fw_items, tr, tr_items, seg, tj, par, counts, qctr, part, H;
+  int64_t n_fw_items = 0, n_fw_leaves = 0, n_tr_items = 0, n_tr_leaves = 0;
+  int64_t nsp = 0, npseudo = 0, fw_part = 0, tr_part = 0;
+  int64_t *fw_first = nullptr, *tr_first = nullptr;
+  int32_t *key = nullptr, *val = nullptr, *seg_t = nullptr;
+};
+
 struct Scratch {
   DBuf a, b, c, d, e, f, g, h, cub;
   DBuf x1, y1, z1, m1, p1, x0, y0, z0, m0, p0;
@@ -137,6 +148,7 @@ struct hsr_ctx {
   int num_cus = 0;
   TreeState tree;
   ListState lists;
+  AdjState adj;
   Scratch s;
   // caller arrays of the last build (for hsr_kick)
   float *user_v[3] = {nullptr, nullptr, nullptr};
@@ -146,7 +158,7 @@ struct hsr_ctx {
   DBuf dev_v;  // device copy of velocities for host-pointer builds
   DBuf force;  // float3 SoA force [3][n]
   bool built = false, listed = false;
-  double build_ms = 0, lists_ms = 0, force_ms = 0, kernel_ms = 0;
+  double build_ms = 0, lists_ms = 0, force_ms = 0, kernel_ms = 0, adjoint_ms = 0;
   bool kernel_timed = false;
   // force phase captured as a hipGraph (tuning.use_graph)
   int64_t lists_gen = 0;

@@ -61,7 +61,7 @@ class Stats(C.Structure):
                 ("entries", C.c_int64), ("pairs", C.c_int64),
                 ("max_depth", C.c_int64), ("build_ms", C.c_double),
                 ("lists_ms", C.c_double), ("force_ms", C.c_double),
-                ("kernel_ms", C.c_double)]
+                ("kernel_ms", C.c_double), ("adjoint_ms", C.c_double)]
 
 
 class Tuning(C.Structure):
@@ -220,9 +220,16 @@ class Context:
     def kick(self, fcoeff: float):
         check(self.L.hsr_kick(self.h, C.c_float(fcoeff)), "hsr_kick")
 
-    def adjoint(self, lx, ly, lz, gx, gy, gz):
+    def adjoint(self, lx, ly, lz, gx=None, gy=None, gz=None, device_ptrs=False):
+        """g = d(sum lambda . F)/dx (tree order).  Host arrays by default."""
+        if gx is None:
+            gx = np.zeros(self.n, np.float32)
+            gy = np.zeros(self.n, np.float32)
+            gz = np.zeros(self.n, np.float32)
+        fl = HSR_DEVICE_PTRS if device_ptrs else HSR_HOST_PTRS
         check(self.L.hsr_adjoint(self.h, _ptr(lx), _ptr(ly), _ptr(lz), _ptr(gx),
-                                 _ptr(gy), _ptr(gz), HSR_DEVICE_PTRS), "hsr_adjoint")
+                                 _ptr(gy), _ptr(gz), fl), "hsr_adjoint")
+        return gx, gy, gz
 
     # -- introspection
     def stats(self) -> Stats:

@@ -115,8 +115,11 @@ hsr_status hsr_kick(hsr_ctx *ctx, float fcoeff);
 
 /* Vector-Jacobian product of the force with respect to positions:
  * g = d(sum_i lambda_i . F_i)/dx for the current tree and lists, including
- * the pseudo-particle centre-of-mass dependence.  lambda and g are in tree
- * order; g covers every particle (zero where no listed pair touches it). */
+ * the pseudo-particle centre-of-mass dependence (F as returned by
+ * hsr_forces).  lambda and g are in tree order; g covers every particle
+ * (zero where no listed pair touches it).  No counterpart in the reference:
+ * the derivative of its pair law (BGQStep16.c:167-187) with the tree
+ * topology and lists held fixed. */
 hsr_status hsr_adjoint(hsr_ctx *ctx, const float *lx, const float *ly,
                        const float *lz, float *gx, float *gy, float *gz,
                        int ptr_flags);
@@ -135,6 +138,7 @@ typedef struct {
   int64_t max_depth;
   double build_ms, lists_ms, force_ms; /* device time of the last calls */
   double kernel_ms;       /* the leaf-kernel launch alone (last force/kick) */
+  double adjoint_ms;      /* device time of the last hsr_adjoint */
 } hsr_stats;
 
 hsr_status hsr_get_stats(hsr_ctx *ctx, hsr_stats *st);

@@ -761,3 +761,138 @@ int64_t hsro_cell_sort_perm(int64_t n, const float *x, const float *y,
   free(ind);
   return inside;
 }
+
+/* -------------------------------------------------------------- adjoint */
+static void law_f64_g(double r2, double rsm2, double rc2, double *g,
+                      double *gp) {
+  if (!(r2 > 0.0) || !(r2 < rc2)) {
+    *g = 0.0;
+    *gp = 0.0;
+    return;
+  }
+  double s = r2 + rsm2;
+  double c[6];
+  for (int k = 0; k < 6; ++k) c[k] = (double)P5[k];
+  double poly = c[0] + r2 * (c[1] + r2 * (c[2] + r2 * (c[3] + r2 * (c[4] + r2 * c[5]))));
+  double dpoly = c[1] + r2 * (2 * c[2] + r2 * (3 * c[3] + r2 * (4 * c[4] + r2 * 5 * c[5])));
+  double rs = 1.0 / sqrt(s);
+  *g = rs * rs * rs - poly;
+  *gp = -1.5 * rs * rs * rs * rs * rs - dpoly;
+}
+
+/* FP64 centre of mass of a node's members */
+static void node_com(const hsro_node *n, const double *x, const double *y,
+                     const double *z, const double *m, double *c, double *M) {
+  double sx = 0, sy = 0, sz = 0, sm = 0;
+  for (int64_t k = n->offset; k < n->offset + n->count; ++k) {
+    sx += m[k] * x[k];
+    sy += m[k] * y[k];
+    sz += m[k] * z[k];
+    sm += m[k];
+  }
+  c[0] = sx / sm;
+  c[1] = sy / sm;
+  c[2] = sz / sm;
+  *M = sm;
+}
+
+double hsro_objective_f64(const hsro_params *p, const hsro_node *t,
+                          const double *x, const double *y, const double *z,
+                          const double *m, const int64_t *leaves,
+                          int64_t nleaves, const int64_t *seg_ptr,
+                          const int64_t *seg_a, const int64_t *seg_b,
+                          const double *lx, const double *ly,
+                          const double *lz) {
+  const double rc2 = (double)(p->fsrrmax * p->fsrrmax);
+  const double rsm2 = (double)p->rsm * (double)p->rsm;
+  double J = 0.0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : J)
+  for (int64_t k = 0; k < nleaves; ++k) {
+    const hsro_node *L = &t[leaves[k]];
+    for (int64_t s = seg_ptr[k]; s < seg_ptr[k + 1]; ++s) {
+      int64_t j0, j1;
+      double pc[3], pm = 0;
+      int pseudo = seg_b[s] < 0;
+      if (pseudo) {
+        double M;
+        node_com(&t[seg_a[s]], x, y, z, m, pc, &M);
+        pm = (double)t[seg_a[s]].ppm[0];
+        j0 = 0;
+        j1 = 1;
+      } else {
+        j0 = seg_a[s];
+        j1 = seg_a[s] + seg_b[s];
+      }
+      for (int64_t j = j0; j < j1; ++j) {
+        double xj = pseudo ? pc[0] : x[j], yj = pseudo ? pc[1] : y[j],
+               zj = pseudo ? pc[2] : z[j], mj = pseudo ? pm : m[j];
+        for (int64_t i = L->offset; i < L->offset + L->count; ++i) {
+          double dx = xj - x[i], dy = yj - y[i], dz = zj - z[i];
+          double g, gp;
+          law_f64_g(dx * dx + dy * dy + dz * dz, rsm2, rc2, &g, &gp);
+          J += mj * g * (lx[i] * dx + ly[i] * dy + lz[i] * dz);
+        }
+      }
+    }
+  }
+  return J;
+}
+
+void hsro_adjoint_f64(const hsro_params *p, const hsro_node *t,
+                      const double *x, const double *y, const double *z,
+                      const double *m, const int64_t *leaves, int64_t nleaves,
+                      const int64_t *seg_ptr, const int64_t *seg_a,
+                      const int64_t *seg_b, const double *lx, const double *ly,
+                      const double *lz, double *gx, double *gy, double *gz) {
+  const double rc2 = (double)(p->fsrrmax * p->fsrrmax);
+  const double rsm2 = (double)p->rsm * (double)p->rsm;
+  /* serial on purpose: source-side accumulations cross leaves */
+  for (int64_t k = 0; k < nleaves; ++k) {
+    const hsro_node *L = &t[leaves[k]];
+    for (int64_t s = seg_ptr[k]; s < seg_ptr[k + 1]; ++s) {
+      int pseudo = seg_b[s] < 0;
+      int64_t j0 = pseudo ? 0 : seg_a[s];
+      int64_t j1 = pseudo ? 1 : seg_a[s] + seg_b[s];
+      double pc[3], M = 1, pm = 0;
+      double G[3] = {0, 0, 0};
+      if (pseudo) {
+        node_com(&t[seg_a[s]], x, y, z, m, pc, &M);
+        pm = (double)t[seg_a[s]].ppm[0];
+      }
+      for (int64_t j = j0; j < j1; ++j) {
+        double xj = pseudo ? pc[0] : x[j], yj = pseudo ? pc[1] : y[j],
+               zj = pseudo ? pc[2] : z[j], mj = pseudo ? pm : m[j];
+        for (int64_t i = L->offset; i < L->offset + L->count; ++i) {
+          double dx = xj - x[i], dy = yj - y[i], dz = zj - z[i];
+          double g, gp;
+          law_f64_g(dx * dx + dy * dy + dz * dz, rsm2, rc2, &g, &gp);
+          if (g == 0.0 && gp == 0.0) continue;
+          double dl = lx[i] * dx + ly[i] * dy + lz[i] * dz;
+          double cx = mj * (g * lx[i] + 2 * gp * dl * dx);
+          double cy = mj * (g * ly[i] + 2 * gp * dl * dy);
+          double cz = mj * (g * lz[i] + 2 * gp * dl * dz);
+          gx[i] -= cx;
+          gy[i] -= cy;
+          gz[i] -= cz;
+          if (pseudo) {
+            G[0] += cx;
+            G[1] += cy;
+            G[2] += cz;
+          } else {
+            gx[j] += cx;
+            gy[j] += cy;
+            gz[j] += cz;
+          }
+        }
+      }
+      if (pseudo) {
+        const hsro_node *N = &t[seg_a[s]];
+        for (int64_t q = N->offset; q < N->offset + N->count; ++q) {
+          gx[q] += m[q] / M * G[0];
+          gy[q] += m[q] / M * G[1];
+          gz[q] += m[q] / M * G[2];
+        }
+      }
+    }
+  }
+}

@@ -141,6 +141,26 @@ void hsro_direct_f64(int64_t n, const float *x, const float *y, const float *z,
                      float rsm, float fsrrmax, double *ax, double *ay,
                      double *az);
 
+/* Adjoint of the north-star law over the same lists, in FP64 (there is no
+ * reference adjoint; the mathematics is pinned by finite differences of
+ * hsro_objective_f64).  J = sum_i lam_i . F_i with F as hsro_force_f64
+ * defines it and pseudo-particles at the FP64 centre of mass of their
+ * node's members; returns dJ/dx (n x 3, by sorted position). */
+void hsro_adjoint_f64(const hsro_params *p, const hsro_node *nodes,
+                      const double *x, const double *y, const double *z,
+                      const double *m, const int64_t *leaves, int64_t nleaves,
+                      const int64_t *seg_ptr, const int64_t *seg_a,
+                      const int64_t *seg_b, const double *lx, const double *ly,
+                      const double *lz, double *gx, double *gy, double *gz);
+
+double hsro_objective_f64(const hsro_params *p, const hsro_node *nodes,
+                          const double *x, const double *y, const double *z,
+                          const double *m, const int64_t *leaves,
+                          int64_t nleaves, const int64_t *seg_ptr,
+                          const int64_t *seg_a, const int64_t *seg_b,
+                          const double *lx, const double *ly,
+                          const double *lz);
+
 /* Stable counting sort by unit cell (Particles.cxx:402-488): returns the
  * permutation perm[i] = destination of particle i and the number of
  * particles inside the ng^3 grid (the rest go to the tail). */

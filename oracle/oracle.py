@@ -86,6 +86,9 @@ def lib():
         L.hsro_law_f32.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float]
         L.hsro_law_f32.restype = C.c_float
         L.hsro_direct_f64.argtypes = [C.c_int64, f32, f32, f32, f32, i64, C.c_int64, C.c_float, C.c_float, f64, f64, f64]
+        L.hsro_adjoint_f64.argtypes = [C.POINTER(Params), nodes, f64, f64, f64, f64, i64, C.c_int64, i64, i64, i64, f64, f64, f64, f64, f64, f64]
+        L.hsro_objective_f64.argtypes = [C.POINTER(Params), nodes, f64, f64, f64, f64, i64, C.c_int64, i64, i64, i64, f64, f64, f64]
+        L.hsro_objective_f64.restype = C.c_double
         L.hsro_cell_sort_perm.argtypes = [C.c_int64, f32, f32, f32, P(np.int32, flags="C"), i64]
         L.hsro_cell_sort_perm.restype = C.c_int64
         _lib = L
@@ -320,3 +323,32 @@ def cell_sort_perm(x, y, z, ng):
                                        np.ascontiguousarray(z, np.float32),
                                        np.asarray(ng, np.int32), perm)
     return perm, inside
+
+
+def _d(a):
+    return np.ascontiguousarray(a, np.float64)
+
+
+def adjoint_f64(tp: TreeParams, tree: Tree, lists: Lists, lam, pos=None):
+    """FP64 dJ/dx for J = sum lam . F (lam: n x 3, sorted order)."""
+    n = tree.x.shape[0]
+    x, y, z = (tree.x, tree.y, tree.z) if pos is None else pos
+    g = [np.zeros(n) for _ in range(3)]
+    p = tp.ctypes()
+    lib().hsro_adjoint_f64(C.byref(p), tree.nodes, _d(x), _d(y), _d(z), _d(tree.mass),
+                           lists.leaves, lists.leaves.shape[0], lists.seg_ptr,
+                           lists.seg_a, lists.seg_b, _d(lam[:, 0]), _d(lam[:, 1]),
+                           _d(lam[:, 2]), *g)
+    return np.stack(g, 1)
+
+
+def objective_f64(tp: TreeParams, tree: Tree, lists: Lists, lam, pos):
+    """J = sum_i lam_i . F_i in FP64 at positions pos = (x, y, z) (sorted
+    order), topology and lists frozen, pseudo-particles at the FP64 centre
+    of mass of their members."""
+    p = tp.ctypes()
+    return lib().hsro_objective_f64(C.byref(p), tree.nodes, _d(pos[0]), _d(pos[1]),
+                                    _d(pos[2]), _d(tree.mass), lists.leaves,
+                                    lists.leaves.shape[0], lists.seg_ptr, lists.seg_a,
+                                    lists.seg_b, _d(lam[:, 0]), _d(lam[:, 1]),
+                                    _d(lam[:, 2]))

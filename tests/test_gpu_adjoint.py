@@ -1,0 +1,94 @@
+"""GPU adjoint (hsr_adjoint) against the FP64 oracle adjoint on the same
+tree and lists; the oracle adjoint itself is pinned by finite differences
+(tests/test_oracle.py)."""
+import numpy as np
+import pytest
+
+from haccsr import Context, ics
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def setup(x, y, z, box, nd, oa, margin=1.0, seed=3):
+    m = np.ones_like(x)
+    lo, hi = np.zeros(3, np.float32), np.full(3, box, np.float32)
+    flo, fhi = np.full(3, margin, np.float32), np.full(3, box - margin, np.float32)
+    tp = O.TreeParams(lo, hi, flo, fhi, nd=nd, oa=oa)
+    tree = O.build_tree(tp, x, y, z, m)
+    lists = O.build_lists(tp, tree)
+    ctx = Context(nd=nd, oa=oa)
+    X, Y, Z, M = x.copy(), y.copy(), z.copy(), m.copy()
+    ctx.build(X, Y, Z, M, min_loc=lo, max_loc=hi)
+    ctx.lists(flo, fhi)
+    rng = np.random.default_rng(seed)
+    lam = rng.normal(size=(x.shape[0], 3))
+    sel = np.zeros(x.shape[0], bool)
+    for l in lists.leaves:
+        nd_ = tree.nodes[l]
+        sel[nd_["offset"]:nd_["offset"] + nd_["count"]] = True
+    lam[~sel] = 0.0
+    return tp, tree, lists, ctx, lam
+
+
+def rel_rms(a, b):
+    return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
+
+
+@pytest.mark.parametrize("n,box,nd,oa,chunk", [(6000, 10.0, 16, 0.7, 0), (20000, 14.0, 64, 0.1, 0),
+                                               (20000, 14.0, 64, 0.5, 300)])
+def test_adjoint_matches_oracle(n, box, nd, oa, chunk):
+    say(f"case adjoint n={n} nd={nd} oa={oa} chunk={chunk}")
+    rng = np.random.default_rng(n)
+    p = (rng.random((n, 3)) * box).astype(np.float32)
+    tp, tree, lists, ctx, lam = setup(p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), box, nd, oa)
+    if chunk:
+        ctx.set_tuning(chunk=chunk)
+        ctx.lists(tp.min_force, tp.max_force)
+    ref = O.adjoint_f64(tp, tree, lists, lam)
+    L32 = lam.astype(np.float32)
+    gx, gy, gz = ctx.adjoint(L32[:, 0].copy(), L32[:, 1].copy(), L32[:, 2].copy())
+    got = np.stack([gx, gy, gz], 1)
+    e = rel_rms(got, ref)
+    say(f"  rel rms vs FP64 {e:.3e}  pseudo segments {(lists.seg_b < 0).sum()}")
+    assert e < 1e-4
+    # bitwise reproducible
+    g2 = np.stack(ctx.adjoint(L32[:, 0].copy(), L32[:, 1].copy(), L32[:, 2].copy()), 1)
+    assert np.array_equal(got.view(np.uint32), g2.view(np.uint32))
+
+
+def test_adjoint_config_a():
+    say("case adjoint config A (64^3, nd=128)")
+    load = ics.config_load("A")
+    lo, hi = load.tree_box
+    flo, fhi = load.force_box
+    m = np.ones(load.n, np.float32)
+    tp = O.TreeParams(lo, hi, flo, fhi, nd=128)
+    tree = O.build_tree(tp, load.x, load.y, load.z, m)
+    lists = O.build_lists(tp, tree)
+    ctx = Context(nd=128)
+    X, Y, Z, M = load.x.copy(), load.y.copy(), load.z.copy(), m.copy()
+    ctx.build(X, Y, Z, M, min_loc=lo, max_loc=hi)
+    ctx.lists(flo, fhi)
+    rng = np.random.default_rng(1)
+    lam = rng.normal(size=(load.n, 3))
+    sel = np.zeros(load.n, bool)
+    for l in lists.leaves:
+        nd_ = tree.nodes[l]
+        sel[nd_["offset"]:nd_["offset"] + nd_["count"]] = True
+    lam[~sel] = 0.0
+    ref = O.adjoint_f64(tp, tree, lists, lam)
+    L32 = lam.astype(np.float32)
+    got = np.stack(ctx.adjoint(L32[:, 0].copy(), L32[:, 1].copy(), L32[:, 2].copy()), 1)
+    e = rel_rms(got, ref)
+    ctx.forces()
+    fwd = ctx.stats().kernel_ms
+    ctx.adjoint(L32[:, 0].copy(), L32[:, 1].copy(), L32[:, 2].copy())
+    adj = ctx.stats().adjoint_ms
+    say(f"  rel rms vs FP64 {e:.3e}; forward kernel {fwd:.2f} ms, adjoint {adj:.2f} ms "
+        f"(ratio {adj / fwd:.2f})")
+    assert e < 1e-4

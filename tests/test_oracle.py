@@ -319,3 +319,40 @@ def test_dropin_forcelaw_host_matches_reference_bitwise():
     assert got.shape[0] == d["r2"].shape[0]
     for k, name in enumerate(["sr_poly6", "sr_fit", "fgor", "poly6", "interp1024"]):
         assert np.array_equal(got[:, k], f32bits(d[name])), name
+
+
+# ------------------------------------------------------------------ adjoint
+def test_adjoint_f64_matches_finite_differences():
+    """The FP64 adjoint (including the pseudo-particle centre-of-mass term)
+    equals central finite differences of the FP64 objective, h = 1e-3,
+    on random coordinates (topology frozen)."""
+    # a wide opening angle makes accepted pseudo-particle nodes common
+    x, y, z = uniform(4000, 10.0, 13)
+    tp = small_params(10.0, 8, oa=0.7)
+    tree = O.build_tree(tp, x, y, z)
+    lists = O.build_lists(tp, tree)
+    assert np.sum(lists.seg_b < 0) > 100, "case must contain pseudo-particles"
+    rng = np.random.default_rng(3)
+    lam = rng.normal(size=(tree.x.shape[0], 3))
+    sel = np.zeros(tree.x.shape[0], bool)
+    for l in lists.leaves:
+        nd = tree.nodes[l]
+        sel[nd["offset"]:nd["offset"] + nd["count"]] = True
+    lam[~sel] = 0.0
+    grad = O.adjoint_f64(tp, tree, lists, lam)
+    pos = [tree.x.astype(np.float64), tree.y.astype(np.float64), tree.z.astype(np.float64)]
+    h = 1e-3
+    # coordinates: random, plus members of accepted pseudo-particle nodes
+    pseudo_nodes = np.unique(lists.seg_a[lists.seg_b < 0])
+    members = [tree.nodes[n_]["offset"] for n_ in pseudo_nodes[:16]]
+    idx = list(rng.choice(tree.x.shape[0], 48, replace=False)) + members
+    errs, scale = [], np.max(np.abs(grad))
+    for k in idx:
+        d = int(rng.integers(0, 3))
+        pp = [a.copy() for a in pos]
+        pm = [a.copy() for a in pos]
+        pp[d][k] += h
+        pm[d][k] -= h
+        fd = (O.objective_f64(tp, tree, lists, lam, pp) - O.objective_f64(tp, tree, lists, lam, pm)) / (2 * h)
+        errs.append(abs(fd - grad[k, d]) / max(abs(grad[k, d]), 1e-3 * scale))
+    assert np.median(errs) < 1e-4 and np.max(errs) < 5e-2, (np.median(errs), np.max(errs))
