@@ -1,7 +1,7 @@
 """ctypes binding of libhaccsr.so (include/hacc_sr.h).
 
 The library is built in-tree (hacc-coral_amd/libhaccsr.so) by
-``__graft_entry__.build()`` / ``make -C hacc-coral_amd``.  There is no CPU
+``make -C hacc-coral_amd``.  There is no CPU
 fallback: every entry point runs on a HIP device and raises when the library
 or the device is missing.
 """
@@ -17,6 +17,10 @@ LIB_PATH = os.path.join(PKG_ROOT, "libhaccsr.so")
 
 HSR_HOST_PTRS, HSR_DEVICE_PTRS = 0, 1
 LAW_STEP16, LAW_SR_POLY5, LAW_SR_POLY6, LAW_NEWTON = 0, 1, 2, 3
+
+# hsr_status codes (include/hacc_sr.h)
+HSR_OK, HSR_ERR_ARG, HSR_ERR_HIP, HSR_ERR_UNSUPPORTED = 0, 1, 2, 3
+HSR_ERR_STATE, HSR_ERR_NOMEM, HSR_ERR_NUMERIC = 4, 5, 6
 
 EXPORTS = [
     "hsr_last_error", "hsr_version", "hsr_create", "hsr_destroy",
@@ -80,8 +84,7 @@ def load(path: str | None = None):
         return _lib
     p = path or LIB_PATH
     if not os.path.exists(p):
-        raise HsrError(f"{p} is missing: build it with __graft_entry__.build() "
-                       "or `make -C hacc-coral_amd`")
+        raise HsrError(f"{p} is missing: build it with `make -C hacc-coral_amd`")
     L = C.CDLL(p)
     vp, i64, f32p = C.c_void_p, C.c_int64, C.c_void_p
     L.hsr_last_error.restype = C.c_char_p

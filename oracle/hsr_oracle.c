@@ -896,3 +896,38 @@ void hsro_adjoint_f64(const hsro_params *p, const hsro_node *t,
     }
   }
 }
+
+void hsro_force_f64d(const hsro_params *p, const hsro_node *t,
+                     const double *x, const double *y, const double *z,
+                     const double *m, const int64_t *leaves, int64_t nleaves,
+                     const int64_t *seg_ptr, const int64_t *seg_a,
+                     const int64_t *seg_b, double *ax, double *ay, double *az) {
+  const double rc2 = (double)(p->fsrrmax * p->fsrrmax);
+  const double rsm2 = (double)p->rsm * (double)p->rsm;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int64_t k = 0; k < nleaves; ++k) {
+    const hsro_node *L = &t[leaves[k]];
+    for (int64_t i = L->offset; i < L->offset + L->count; ++i) ax[i] = ay[i] = az[i] = 0.0;
+    for (int64_t s = seg_ptr[k]; s < seg_ptr[k + 1]; ++s) {
+      int pseudo = seg_b[s] < 0;
+      double pc[3], M, pm = 0;
+      if (pseudo) {
+        node_com(&t[seg_a[s]], x, y, z, m, pc, &M);
+        pm = (double)t[seg_a[s]].ppm[0];
+      }
+      int64_t j0 = pseudo ? 0 : seg_a[s], j1 = pseudo ? 1 : seg_a[s] + seg_b[s];
+      for (int64_t j = j0; j < j1; ++j) {
+        double xj = pseudo ? pc[0] : x[j], yj = pseudo ? pc[1] : y[j],
+               zj = pseudo ? pc[2] : z[j], mj = pseudo ? pm : m[j];
+        for (int64_t i = L->offset; i < L->offset + L->count; ++i) {
+          double dx = xj - x[i], dy = yj - y[i], dz = zj - z[i];
+          double g, gp;
+          law_f64_g(dx * dx + dy * dy + dz * dz, rsm2, rc2, &g, &gp);
+          ax[i] += mj * g * dx;
+          ay[i] += mj * g * dy;
+          az[i] += mj * g * dz;
+        }
+      }
+    }
+  }
+}

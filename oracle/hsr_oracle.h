@@ -2,7 +2,7 @@
  * hsr_oracle.h -- CPU restatement of HACC's short-range RCB force path.
  *
  * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
- * implementation: only tests/, __graft_entry__.smoke() and bench.py's
+ * implementation: only tests/, the smoke entry point and bench.py's
  * cpu_baseline leg may load it.  The product library never links it.
  *
  * Every function restates the reference algorithm (BenWibking/hacc-coral,
@@ -119,7 +119,7 @@ void hsro_force_step16(const hsro_params *p, const hsro_node *nodes,
                        const int64_t *seg_ptr, const int64_t *seg_a,
                        const int64_t *seg_b, float *ax, float *ay, float *az);
 
-/* FP64 evaluation of the north-star law over the same lists (inputs are the
+/* FP64 evaluation of the Step16 5th-order law over the same lists (inputs are the
  * float positions promoted to double).  Also returns, per listed leaf, the
  * FP64 momentum sum sum_i m_i F_i (3 doubles per leaf) if leaf_mom != 0. */
 void hsro_force_f64(const hsro_params *p, const hsro_node *nodes,
@@ -141,7 +141,7 @@ void hsro_direct_f64(int64_t n, const float *x, const float *y, const float *z,
                      float rsm, float fsrrmax, double *ax, double *ay,
                      double *az);
 
-/* Adjoint of the north-star law over the same lists, in FP64 (there is no
+/* Adjoint of the Step16 5th-order law over the same lists, in FP64 (there is no
  * reference adjoint; the mathematics is pinned by finite differences of
  * hsro_objective_f64).  J = sum_i lam_i . F_i with F as hsro_force_f64
  * defines it and pseudo-particles at the FP64 centre of mass of their
@@ -152,6 +152,15 @@ void hsro_adjoint_f64(const hsro_params *p, const hsro_node *nodes,
                       const int64_t *seg_ptr, const int64_t *seg_a,
                       const int64_t *seg_b, const double *lx, const double *ly,
                       const double *lz, double *gx, double *gy, double *gz);
+
+/* FP64 force at double positions (sorted order) with the tree topology and
+ * lists frozen; pseudo-particles at the FP64 centre of mass of their
+ * members, mass ppm.  Writes a (n x 3) for listed particles. */
+void hsro_force_f64d(const hsro_params *p, const hsro_node *nodes,
+                     const double *x, const double *y, const double *z,
+                     const double *m, const int64_t *leaves, int64_t nleaves,
+                     const int64_t *seg_ptr, const int64_t *seg_a,
+                     const int64_t *seg_b, double *ax, double *ay, double *az);
 
 double hsro_objective_f64(const hsro_params *p, const hsro_node *nodes,
                           const double *x, const double *y, const double *z,

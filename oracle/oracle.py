@@ -1,7 +1,7 @@
 """ctypes front end of the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Only tests/,
-__graft_entry__.smoke() and bench.py's cpu_baseline leg may import this.
+the smoke entry point and bench.py's cpu_baseline leg may import this.
 
 Each wrapper follows the C function of the same name in hsr_oracle.c, which
 in turn cites the reference file:line it restates.
@@ -89,6 +89,7 @@ def lib():
         L.hsro_adjoint_f64.argtypes = [C.POINTER(Params), nodes, f64, f64, f64, f64, i64, C.c_int64, i64, i64, i64, f64, f64, f64, f64, f64, f64]
         L.hsro_objective_f64.argtypes = [C.POINTER(Params), nodes, f64, f64, f64, f64, i64, C.c_int64, i64, i64, i64, f64, f64, f64]
         L.hsro_objective_f64.restype = C.c_double
+        L.hsro_force_f64d.argtypes = [C.POINTER(Params), nodes, f64, f64, f64, f64, i64, C.c_int64, i64, i64, i64, f64, f64, f64]
         L.hsro_cell_sort_perm.argtypes = [C.c_int64, f32, f32, f32, P(np.int32, flags="C"), i64]
         L.hsro_cell_sort_perm.restype = C.c_int64
         _lib = L
@@ -352,3 +353,27 @@ def objective_f64(tp: TreeParams, tree: Tree, lists: Lists, lam, pos):
                                     lists.leaves.shape[0], lists.seg_ptr, lists.seg_a,
                                     lists.seg_b, _d(lam[:, 0]), _d(lam[:, 1]),
                                     _d(lam[:, 2]))
+
+
+def force_f64_at(tp: TreeParams, nodes, lists: Lists, pos, mass):
+    """FP64 forces at double positions pos = (x, y, z) (sorted order) with
+    frozen topology; pseudo-particles at the FP64 centre of mass."""
+    n = pos[0].shape[0]
+    a = [np.zeros(n) for _ in range(3)]
+    p = tp.ctypes()
+    lib().hsro_force_f64d(C.byref(p), np.ascontiguousarray(nodes), _d(pos[0]), _d(pos[1]),
+                          _d(pos[2]), _d(mass), lists.leaves, lists.leaves.shape[0],
+                          lists.seg_ptr, lists.seg_a, lists.seg_b, *a)
+    return np.stack(a, 1)
+
+
+def adjoint_f64_at(tp: TreeParams, nodes, lists: Lists, pos, mass, lam):
+    """FP64 dJ/dx (J = sum lam . F) at double positions, frozen topology."""
+    n = pos[0].shape[0]
+    g = [np.zeros(n) for _ in range(3)]
+    p = tp.ctypes()
+    lib().hsro_adjoint_f64(C.byref(p), np.ascontiguousarray(nodes), _d(pos[0]), _d(pos[1]),
+                           _d(pos[2]), _d(mass), lists.leaves, lists.leaves.shape[0],
+                           lists.seg_ptr, lists.seg_a, lists.seg_b, _d(lam[:, 0]),
+                           _d(lam[:, 1]), _d(lam[:, 2]), *g)
+    return np.stack(g, 1)

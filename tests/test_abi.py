@@ -1,0 +1,96 @@
+"""CPU checks of the C ABI (include/hacc_sr.h): the built library loads and
+exports every declared entry point, the ctypes mirrors of the public structs
+have the C layout, and argument validation runs before any device work.
+No compute call is made (there is no GPU here)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from haccsr import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "hacc_sr.h")
+LIB = os.path.join(ROOT, "hacc-coral_amd", "libhaccsr.so")
+LIBCXX = os.path.join(ROOT, "hacc-coral_amd", "libhaccsr_cxx.so")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(hsr_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_the_entry_points():
+    names = declared_functions()
+    for must in ("hsr_create", "hsr_destroy", "hsr_build", "hsr_lists", "hsr_forces",
+                 "hsr_kick", "hsr_adjoint", "hsr_build_many", "hsr_lists_many"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(LIB), "build with make -C hacc-coral_amd"
+    L = C.CDLL(LIB)
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_cxx_dropin_library_loads():
+    assert os.path.exists(LIBCXX)
+    C.CDLL(LIBCXX)
+
+
+def test_struct_layouts_match_c(tmp_path):
+    src = tmp_path / "layout.c"
+    src.write_text(r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "hacc_sr.h"
+int main(void) {
+  printf("%zu %zu %zu %zu\n", sizeof(hsr_params), sizeof(hsr_node),
+         sizeof(hsr_stats), sizeof(hsr_tuning));
+  printf("%zu %zu %zu %zu\n", offsetof(hsr_params, nd), offsetof(hsr_params, fsrrmax),
+         offsetof(hsr_params, ppc), offsetof(hsr_node, xc));
+  printf("%zu %zu\n", offsetof(hsr_stats, build_ms), offsetof(hsr_tuning, use_graph));
+  return 0;
+}
+""")
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)])
+    out = subprocess.check_output([str(exe)]).decode().split()
+    got = [int(v) for v in out]
+    want = [C.sizeof(_lib.Params), C.sizeof(_lib.Node), C.sizeof(_lib.Stats),
+            C.sizeof(_lib.Tuning), _lib.Params.nd.offset, _lib.Params.fsrrmax.offset,
+            _lib.Params.ppc.offset, _lib.Node.xc.offset, _lib.Stats.build_ms.offset,
+            _lib.Tuning.use_graph.offset]
+    assert got == want
+    assert _lib.NODE_DTYPE.itemsize == C.sizeof(_lib.Node)
+
+
+def test_create_validates_arguments_before_device_work():
+    L = _lib.load()
+    h = C.c_void_p()
+    assert L.hsr_create(None, None) == _lib.HSR_ERR_ARG
+    p = _lib.Params()
+    p.tdpts, p.nd, p.fsrrmax, p.rsm, p.ppc = 12, 128, 3.12, 0.1, 0.9
+    assert L.hsr_create(C.byref(h), C.byref(p)) == _lib.HSR_ERR_UNSUPPORTED
+    assert b"TDPTS" in L.hsr_last_error()
+    p.tdpts, p.nd = 1, 0
+    assert L.hsr_create(C.byref(h), C.byref(p)) == _lib.HSR_ERR_ARG
+    p.nd, p.fsrrmax = 128, 0.0
+    assert L.hsr_create(C.byref(h), C.byref(p)) == _lib.HSR_ERR_ARG
+    assert L.hsr_version().startswith(b"hacc-sr")
+
+
+def test_no_device_is_an_error_not_a_fallback():
+    """Without a HIP device the product path raises; there is no CPU path."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a device is present")
+    except ImportError:
+        pass
+    with pytest.raises(_lib.HsrError, match="no HIP device"):
+        _lib.Context()

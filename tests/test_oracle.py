@@ -356,3 +356,50 @@ def test_adjoint_f64_matches_finite_differences():
         fd = (O.objective_f64(tp, tree, lists, lam, pp) - O.objective_f64(tp, tree, lists, lam, pm)) / (2 * h)
         errs.append(abs(fd - grad[k, d]) / max(abs(grad[k, d]), 1e-3 * scale))
     assert np.median(errs) < 1e-4 and np.max(errs) < 5e-2, (np.median(errs), np.max(errs))
+
+
+def test_force_f64_at_matches_tree_force():
+    """force_f64_at (double positions, frozen topology) at the tree's own
+    float positions equals hsro_force_f64 for direct entries; pseudo-
+    particles then sit at the FP64 centre of mass instead of the float xc,
+    so the two differ only at that rounding level."""
+    x, y, z = uniform(3000, 10.0, 21)
+    tp = small_params(10.0, 16, oa=0.5)
+    tree = O.build_tree(tp, x, y, z)
+    lists = O.build_lists(tp, tree)
+    ax, ay, az = O.forces_f64(tp, tree, lists)
+    pos = (tree.x.astype(np.float64), tree.y.astype(np.float64), tree.z.astype(np.float64))
+    a = O.force_f64_at(tp, tree.nodes, lists, pos, tree.mass.astype(np.float64))
+    ref = np.stack([ax, ay, az], 1)
+    assert np.sqrt(np.sum((a - ref) ** 2) / np.sum(ref ** 2)) < 1e-5
+
+
+def test_kick_drift_reverse_pass_matches_finite_differences():
+    """The FP64 reverse pass over 5 kick-drift sub-steps with periodic
+    images (tests/kdref.py, the reference gradient of the GPU autograd
+    test) equals central finite differences of the FP64 trajectory, h = 1e-3,
+    with every sub-step's topology frozen."""
+    import kdref as K
+    from haccsr import ics
+    box, o, dt = 10.0, 3.2, 0.01
+    kick = dt / (4 * np.pi)
+    x, y, z = ics.clustered(10, box, 1, 31)
+    load = ics.add_ghosts(x, y, z, box, o)
+    base = np.stack([x, y, z], 1).astype(np.float64)
+    idx = load.alive_index
+    shift = np.stack([load.x, load.y, load.z], 1).astype(np.float64) - base[idx]
+    rng = np.random.default_rng(9)
+    v0, wx, wv = (rng.normal(size=base.shape) for _ in range(3))
+    tp = O.TreeParams(np.full(3, -1, np.float32), np.full(3, box + 2 * o + 1, np.float32),
+                      np.full(3, o, np.float32), np.full(3, box + o, np.float32), nd=32, oa=0.5)
+    topos = K.oracle_topos(tp, base, v0, idx, shift, dt, kick, 5)
+    assert sum(int(np.sum(t.lists.seg_b < 0)) for t in topos) > 100
+    gx, gv = K.gradient(tp, topos, base, v0, idx, shift, dt, kick, wx, wv)
+    coords = list(zip(rng.integers(0, base.shape[0], 24).tolist(), rng.integers(0, 3, 24).tolist()))
+    fd = K.fd_gradient(tp, topos, base, v0, idx, shift, dt, kick, wx, wv, coords)
+    w = np.array([wx[i, a] for i, a in coords])
+    ref = np.array([gx[i, a] for i, a in coords]) - w
+    err = np.sqrt(np.sum((fd - w - ref) ** 2) / np.sum(ref ** 2))
+    assert err < 1e-3, err
+    # a dropped image contribution or a wrong sub-step order would fail this
+    assert np.sqrt(np.mean(ref ** 2)) > 1e-2
