@@ -28,6 +28,15 @@ sys.path.insert(0, os.path.join(ROOT, "hacc-coral_amd"))
 
 FLOPS_PER_PAIR = 30          # d 3, r2 5, +eps2 1, rsqrt 1, cube 2, P5 10, sub 1, *m 1, acc 6
 PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 vector peak (packed); non-packed 78.6
+PEAK_HBM_GBPS = 8000.0       # MI355X HBM3E
+# HBM bytes per k_force launch measured with rocprofv3 (FETCH_SIZE x 2.0 + WRITE_SIZE
+# x 1.0, factors from a 256 MiB device copy) on the 256^3 clustered load at leaf size
+# 256: profiles/k_force_256c_traffic.md.  Reported only for that workload.
+MEASURED_TRAFFIC = {("B", 256, 256): 65423846084}
+# Algorithmic bytes of the tree build: 48 B per particle per level it is split at
+# (cm reads x, y, z, m; partition reads and writes key + index) plus one 84 B
+# gather of the ten payload arrays.
+BUILD_BYTES_PER_LEVEL, BUILD_BYTES_GATHER = 48, 84
 
 
 def parse():
@@ -40,6 +49,8 @@ def parse():
     ap.add_argument("--nd", type=int, default=0, help="leaf size (0 = autotuned)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--adjoint-reps", type=int, default=3,
+                    help="timed hsr_adjoint calls on the last step's tree (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run one plain device-to-device copy of BYTES (the "
                          "reference kernel for scaling FETCH_SIZE/WRITE_SIZE counters)")
@@ -92,6 +103,44 @@ def cpu_baseline(load, nd, seconds):
                        f"listed leaves of the same tree ({done_pairs:.3e} pairs in {done_t:.1f} s, "
                        f"{threads} OpenMP threads); oracle tree build of the whole load took "
                        f"{t_tree:.1f} s single-threaded")}
+
+
+def build_roofline(ctx, n, build_ms):
+    """Tree build against the HBM roofline with its algorithmic bytes."""
+    nodes = ctx.nodes()
+    internal = (nodes["cl"] != 0) | (nodes["cr"] != 0)
+    split_particles = int(nodes["count"][internal].sum())
+    nbytes = BUILD_BYTES_PER_LEVEL * split_particles + BUILD_BYTES_GATHER * n
+    gbps = nbytes / (build_ms * 1e-3) / 1e9
+    return {"tree_build": {"algorithmic_bytes": nbytes, "ms": build_ms, "GB/s": gbps,
+                           "frac": gbps / PEAK_HBM_GBPS,
+                           "bytes_rule": "48 B x particles per split level + 84 B x N gather"}}
+
+
+def adjoint_leg(ctx, n, dev, reps, fw_stats):
+    """hsr_adjoint (d(sum lambda.F)/dx, frozen topology) on the resident tree
+    and lists of the last step, with a seeded Gaussian lambda.  The first call
+    also builds the transposed lists (once per list build); the ratio compares
+    the adjoint passes with the forward force phase on the same lists."""
+    import torch
+    g = torch.Generator().manual_seed(11)
+    lam = [torch.randn(n, generator=g).to(dev) for _ in range(3)]
+    out = [torch.empty(n, dtype=torch.float32, device=dev) for _ in range(3)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.adjoint(*lam, *out, device_ptrs=True)
+    first_ms = 1e3 * (time.perf_counter() - t0)
+    ms = []
+    for _ in range(reps):
+        ctx.adjoint(*lam, *out, device_ptrs=True)
+        ms.append(ctx.stats().adjoint_ms)
+    adj_ms = float(np.mean(ms))
+    fw_ms = float(np.mean([s[3] for s in fw_stats]))
+    fw_kernel_ms = float(np.mean([s[4] for s in fw_stats]))
+    return {"adjoint_ms": adj_ms, "forward_ms": fw_ms, "forward_kernel_ms": fw_kernel_ms,
+            "ratio": adj_ms / fw_ms, "first_call_ms_with_transpose": first_ms,
+            "note": "adjoint_ms: target, source and pseudo passes (HIP events); forward_ms: "
+                    "force phase of the same lists (fill, leaf kernel, chunk reduction)"}
 
 
 def main():
@@ -194,9 +243,14 @@ def main():
                                     "leaf_kernel": kern_ms}},
         "roofline": {"bound": "mfma", "pipe": "FP32 VALU (no MFMA: the pair force is not a contraction)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": MEASURED_TRAFFIC.get((a.config, a.n_side or 256, nd)),
                      "kernel": "k_force (leaf kernel)", "flops_per_pair": FLOPS_PER_PAIR},
+        "hbm_passes": build_roofline(ctxs[nd], n, float(np.mean([s[1] for s in stats]))),
     }
+    if a.adjoint_reps > 0:
+        out["adjoint"] = adjoint_leg(ctxs[nd], n, dev, a.adjoint_reps, stats)
+        log(f"adjoint {out['adjoint']}")
     if not a.no_cpu:
         log("cpu baseline leg ...")
         out["cpu_baseline"] = cpu_baseline(load, nd, a.cpu_seconds)

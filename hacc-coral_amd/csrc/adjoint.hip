@@ -346,13 +346,6 @@ __global__ void k_tr_counts(int64_t ns, int64_t nn, const int32_t *key,
   start[p] = (p == 0 || key[p - 1] != k) ? 1 : 0;
 }
 
-__global__ void k_tr_runs(int64_t ns, const int32_t *start, const int32_t *rpos,
-                          int64_t *run_first) {
-  int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (p >= ns) return;
-  if (start[p]) run_first[rpos[p]] = p;
-}
-
 // One wave per sorted particle segment: entries = target leaf particles.
 __global__ void __launch_bounds__(64)
 k_tr_expand(int64_t nsp, const int32_t *val, const int32_t *seg_t,
@@ -370,13 +363,16 @@ k_tr_expand(int64_t nsp, const int32_t *val, const int32_t *seg_t,
 __global__ void k_tr_items(int64_t nruns, int64_t nsp, const int64_t *run_first,
                            const int32_t *key, const int64_t *eoff,
                            const int64_t *cnt, const hsr_node *cn,
-                           int64_t chunk, int64_t *nchunks) {
+                           int64_t chunk, int64_t *nchunks, int64_t *psize) {
   int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (r >= nruns) return;
   const int64_t p0 = run_first[r];
   const int64_t p1 = (r + 1 < nruns) ? run_first[r + 1] : nsp;
   const int64_t e0 = eoff[p0], e1 = eoff[p1 - 1] + cnt[p1 - 1];
-  nchunks[r] = std::max<int64_t>(1, (e1 - e0 + chunk - 1) / chunk);
+  const int64_t nc = std::max<int64_t>(1, (e1 - e0 + chunk - 1) / chunk);
+  nchunks[r] = nc;
+  // chunk partials of a chunked run: 3 floats per source-leaf particle per chunk
+  psize[r] = nc > 1 ? nc * 3 * cn[key[p0]].count : 0;
 }
 
 __global__ void k_tr_items_fill(int64_t nruns, int64_t nsp,
@@ -594,6 +590,8 @@ hsr_status adjoint_plan(hsr_ctx *c) {
   TreeState &T = c->tree;
   const int64_t nl = LS.nleaves, ns = LS.nsegs, nn = T.nnodes, n = T.n;
   const int64_t chunk = LS.chunk > 0 ? LS.chunk : 16384;
+  // host sources of asynchronous copies stay alive until the final sync
+  int64_t tail_items[4] = {0, 0, 0, 0};
   const hsr_node *cn = dptr<hsr_node>(T.canon_nodes);
   const int32_t *leaves = dptr<int32_t>(LS.leaves);
   const int64_t *seg_ptr = dptr<int64_t>(LS.seg_ptr);
@@ -661,42 +659,26 @@ hsr_status adjoint_plan(hsr_ctx *c) {
   int64_t *tnc = dptr<int64_t>(A.tr), *tib = tnc + (nruns + 2), *tps = tib + (nruns + 2);
   int64_t *tpb = tps + (nruns + 2), *tfirst = tpb + (nruns + 2);
   if (nruns) {
-    k_tr_items<<<nblk(nruns, 256), 256, 0, st>>>(nruns, nsp, run_first, key2, eoff, tcnt, cn, chunk, tnc);
-    std::vector<int64_t> hnc(nruns), hleaf(nruns);
-    HSR_HIP(hipMemcpyAsync(hnc.data(), tnc, sizeof(int64_t) * nruns, hipMemcpyDeviceToHost, st));
-    HSR_HIP(hipStreamSynchronize(st));
-    // partial sizes need the source leaf counts: 3 * count per chunk when chunked
-    std::vector<hsr_node> lnode(1);
-    std::vector<int64_t> hib(nruns), hpb(nruns);
-    int64_t acc_i = 0, acc_p = 0;
-    for (int64_t r = 0; r < nruns; ++r) {
-      hib[r] = acc_i;
-      acc_i += hnc[r];
-      hpb[r] = acc_p;
-      if (hnc[r] > 1) {
-        HSR_HIP(hipMemcpy(lnode.data(), cn + hkey[hruns[r]], sizeof(hsr_node), hipMemcpyDeviceToHost));
-        acc_p += hnc[r] * 3 * lnode[0].count;
-      }
-    }
-    HSR_HIP(hipMemcpyAsync(tib, hib.data(), sizeof(int64_t) * nruns, hipMemcpyHostToDevice, st));
-    HSR_HIP(hipMemcpyAsync(tpb, hpb.data(), sizeof(int64_t) * nruns, hipMemcpyHostToDevice, st));
+    k_tr_items<<<nblk(nruns, 256), 256, 0, st>>>(nruns, nsp, run_first, key2, eoff, tcnt, cn, chunk, tnc, tps);
+    if ((s = excl_scan(c, tnc, tib, nruns)) != HSR_OK) return s;
+    if ((s = excl_scan(c, tps, tpb, nruns)) != HSR_OK) return s;
+    const int64_t acc_i = tail_sum(c, tib, tnc, nruns);
+    const int64_t acc_p = tail_sum(c, tpb, tps, nruns);
     A.n_tr_items = acc_i;
     A.tr_part = acc_p;
     HSR_HIP(dbuf_reserve(A.tr_items, sizeof(AItem) * (acc_i + 1)));
     k_tr_items_fill<<<nblk(nruns, 256), 256, 0, st>>>(nruns, nsp, run_first, key2, eoff, tcnt, cn, chunk, tnc,
                                                       tib, tpb, dptr<AItem>(A.tr_items), tfirst);
-    int64_t last = acc_i;
-    HSR_HIP(hipMemcpyAsync(tfirst + nruns, &last, sizeof(int64_t), hipMemcpyHostToDevice, st));
+    tail_items[0] = acc_i;
+    HSR_HIP(hipMemcpyAsync(tfirst + nruns, &tail_items[0], sizeof(int64_t), hipMemcpyHostToDevice, st));
   } else {
     A.n_tr_items = 0;
     A.tr_part = 0;
   }
   A.tr_first = tfirst;
   A.n_tr_leaves = nruns;
-  {
-    int64_t last = A.n_fw_items;
-    HSR_HIP(hipMemcpyAsync(ffirst + nl, &last, sizeof(int64_t), hipMemcpyHostToDevice, st));
-  }
+  tail_items[1] = A.n_fw_items;
+  HSR_HIP(hipMemcpyAsync(ffirst + nl, &tail_items[1], sizeof(int64_t), hipMemcpyHostToDevice, st));
   // canonical parents (for the pseudo push-down)
   HSR_HIP(dbuf_reserve(A.par, sizeof(int32_t) * (nn + 1) + sizeof(int64_t) * (T.nboxes + 1)));
   int32_t *par = dptr<int32_t>(A.par);
@@ -705,8 +687,9 @@ hsr_status adjoint_plan(hsr_ctx *c) {
   k_cparent<<<nblk(nn, 256), 256, 0, st>>>(nn, cn, par);
   k_roots<<<nblk(T.nboxes, 64), 64, 0, st>>>(T.nboxes, cb, par);
   HSR_HIP(dbuf_reserve(A.counts, sizeof(int64_t) * 4));
-  int64_t hc[2] = {A.n_fw_items, A.n_tr_items};
-  HSR_HIP(hipMemcpyAsync(dptr<int64_t>(A.counts), hc, sizeof(hc), hipMemcpyHostToDevice, st));
+  tail_items[2] = A.n_fw_items;
+  tail_items[3] = A.n_tr_items;
+  HSR_HIP(hipMemcpyAsync(dptr<int64_t>(A.counts), &tail_items[2], 2 * sizeof(int64_t), hipMemcpyHostToDevice, st));
   HSR_HIP(hipStreamSynchronize(st));
   HSR_HIP(hipGetLastError());
   A.gen = c->lists_gen;
