@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--adjoint-reps", type=int, default=3,
                     help="timed hsr_adjoint calls on the last step's tree (0 = skip)")
+    ap.add_argument("--adjoint-mode", type=int, default=0,
+                    help="hsr_tuning.adjoint_mode: 0 automatic, 1 fused, 2 two-pass")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run one plain device-to-device copy of BYTES (the "
                          "reference kernel for scaling FETCH_SIZE/WRITE_SIZE counters)")
@@ -249,6 +251,7 @@ def main():
         "hbm_passes": build_roofline(ctxs[nd], n, float(np.mean([s[1] for s in stats]))),
     }
     if a.adjoint_reps > 0:
+        ctxs[nd].set_tuning(adjoint_mode=a.adjoint_mode)
         out["adjoint"] = adjoint_leg(ctxs[nd], n, dev, a.adjoint_reps, stats)
         log(f"adjoint {out['adjoint']}")
     if not a.no_cpu:

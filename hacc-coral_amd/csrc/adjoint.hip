@@ -13,13 +13,20 @@
 // each member k as (m_k / M_n) G_n.  The tree topology and lists are frozen
 // (the cutoff and opening decisions are a measure-zero set).
 //
-// Three deterministic passes, no float atomics:
-//   target pass   forward-shaped persistent kernel over the list items;
-//   source pass   per source leaf, over the transposed lists (all target
-//                 particles whose lists hold the leaf), built by a stable
-//                 sort of the list segments by source;
-//   pseudo pass   per (target leaf, pseudo segment) sums, reduced per node
-//                 in segment order, then pushed down to the members.
+// Two schedules, both deterministic (no float atomics):
+//   fused (default)  one persistent kernel over the forward list items
+//                 evaluates each pair once: the target side accumulates in
+//                 registers, the source side W_e = sum_i V_ie of every list
+//                 entry e is summed over the lanes holding i (DPP) and stored
+//                 per entry; a second kernel sums W over the entries that
+//                 reference each source particle in the order of a stable sort
+//                 of the list segments by source, and pseudo-particle entries
+//                 are summed per node and pushed down to the members;
+//   two-pass      target pass as above without W; source pass per source leaf
+//                 over the transposed lists (all target particles whose lists
+//                 hold the leaf); pseudo pass per (target leaf, pseudo
+//                 segment).  Used when a leaf holds more than 256 particles or
+//                 W does not fit in device memory.
 #include "hsr_internal.h"
 #include "wave.h"
 
@@ -307,6 +314,273 @@ k_adj_reduce(const hsr_node *cn, const AItem *items, const int64_t *first,
   }
 }
 
+// ------------------------------------------------------------- fused pass
+constexpr int FR = 2;                       // targets per lane
+constexpr int FMAXP = 2;                    // passes over a leaf's targets
+constexpr int FMAX_LEAF = 64 * FR * FMAXP;  // largest leaf of the fused schedule
+constexpr int FTILE = 256;                  // entries per LDS tile
+constexpr int FUNROLL = 2;
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_add(float v) {
+  const int t = __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROW_MASK, 0xf, true);
+  return v + __builtin_bit_cast(float, t);
+}
+
+// Sum over aligned groups of lpg lanes (lpg a power of two, 4..64); the sum
+// lands in the group's last lane.  Needs all 64 lanes active.
+//   row_shr:1,2,4,8 sum within 16-lane rows; row_bcast:15 adds row 0 (2) into
+//   row 1 (3); row_bcast:31 adds lane 31 into rows 2-3.
+__device__ __forceinline__ float group_sum_last(float v, int lpg) {
+  v = dpp_add<0x111, 0xf>(v);
+  v = dpp_add<0x112, 0xf>(v);
+  if (lpg >= 8) v = dpp_add<0x114, 0xf>(v);
+  if (lpg >= 16) v = dpp_add<0x118, 0xf>(v);
+  if (lpg >= 32) v = dpp_add<0x142, 0xa>(v);
+  if (lpg >= 64) v = dpp_add<0x143, 0xc>(v);
+  return v;
+}
+
+// One pair evaluation for both sides.  Lanes hold targets i (x_i, lambda_i)
+// in groups of lpg lanes (FR per lane); each group takes a slice of the
+// staged entries j (x_j, m_j).  Per pair, with V = g lam_i + 2g'(d.lam_i) d:
+//   target   A_i += m_j g,  B_i += m_j 2g'(d.lam_i) d;  dJ/dx_i = -(A_i lam_i + B_i)
+//   source   W_j += V (summed over the group's lanes, then over passes)
+template <int ORDER>
+__global__ void __launch_bounds__(64)
+k_adj_fused(const hsr_node *__restrict__ cn, const AItem *__restrict__ items,
+            const int64_t *__restrict__ nitems_p, const int32_t *__restrict__ jidx,
+            const PJ *__restrict__ pj, const float *__restrict__ lx,
+            const float *__restrict__ ly, const float *__restrict__ lz, AdjLaw L,
+            unsigned long long *__restrict__ qctr, float *__restrict__ gx,
+            float *__restrict__ gy, float *__restrict__ gz, float *__restrict__ part,
+            float *__restrict__ wx, float *__restrict__ wy, float *__restrict__ wz) {
+  __shared__ float4 tile[LDS_ENT];
+  __shared__ float jacc[3][FTILE];
+  float *red = reinterpret_cast<float *>(&tile[0]);  // [4][64 * FR] after the tiles
+  const int l = lane_id();
+  const int64_t nitems = *nitems_p;
+  for (;;) {
+    int64_t it = 0;
+    if (l == 0) it = (int64_t)atomicAdd(qctr, 1ull);
+    it = (int64_t)readlane_u64((uint64_t)it, 0);
+    if (it >= nitems) break;
+    const AItem I = items[it];
+    const hsr_node &nd = cn[I.leaf];
+    const int64_t off = nd.offset;
+    const int cnt = (int)nd.count;
+    if (cnt == 0) {  // an empty leaf contributes nothing to its entries
+      for (int64_t e = I.e0 + l; e < I.e1; e += 64) wx[e] = wy[e] = wz[e] = 0.f;
+      continue;
+    }
+    const int np = (cnt + 64 * FR - 1) / (64 * FR);
+    const float4 padJ = make_float4(nd.xc[0] + 1000.f, nd.xc[1], nd.xc[2], 0.f);
+    const float padI = nd.xc[0] + 3000.f;
+    int lpg[FMAXP];
+    float xi[FMAXP][FR], yi[FMAXP][FR], zi[FMAXP][FR];
+    float la[FMAXP][FR], lb[FMAXP][FR], lc[FMAXP][FR];
+    float A[FMAXP][FR], Bx[FMAXP][FR], By[FMAXP][FR], Bz[FMAXP][FR];
+#pragma unroll
+    for (int p = 0; p < FMAXP; ++p) {
+      const int ni = min(64 * FR, cnt - p * 64 * FR);
+      const int need = max((ni + FR - 1) / FR, 4);
+      int lp = 4;
+      while (lp < need) lp <<= 1;
+      lpg[p] = lp;
+      const int tq = l & (lp - 1);
+#pragma unroll
+      for (int r = 0; r < FR; ++r) {
+        const int ii = tq + r * lp;
+        const bool v = ii < ni;
+        const int64_t q = off + p * 64 * FR + (v ? ii : 0);
+        const PJ P = pj[v ? q : off];
+        xi[p][r] = v ? P.x : padI;
+        yi[p][r] = v ? P.y : nd.xc[1];
+        zi[p][r] = v ? P.z : nd.xc[2];
+        la[p][r] = v ? lx[q] : 0.f;
+        lb[p][r] = v ? ly[q] : 0.f;
+        lc[p][r] = v ? lz[q] : 0.f;
+        A[p][r] = Bx[p][r] = By[p][r] = Bz[p][r] = 0.f;
+      }
+    }
+    for (int64_t t0 = I.e0; t0 < I.e1; t0 += FTILE) {
+      const int nt = (int)min<int64_t>(FTILE, I.e1 - t0);
+#pragma unroll
+      for (int p = 0; p < FMAXP; ++p) {
+        if (p >= np) break;
+        const int lp = lpg[p], G = 64 / lp;
+        const int g = l / lp, tq = l & (lp - 1);
+        int tg = (nt + G - 1) / G;
+        tg = (tg + FUNROLL - 1) / FUNROLL * FUNROLL;
+        const int stride = tg + ((9 - (tg & 7)) & 7);
+        __syncthreads();
+        for (int q = l; q < G * tg; q += 64) {
+          const int gq = q / tg, kq = q - gq * tg;
+          float4 a = padJ;
+          if (q < nt) {
+            const PJ P = pj[jidx[t0 + q]];
+            a = make_float4(P.x, P.y, P.z, P.m);
+          }
+          tile[gq * stride + kq] = a;
+        }
+        __syncthreads();
+        const float4 *pa = tile + g * stride;
+        for (int k = 0; k < tg; k += FUNROLL) {
+#pragma unroll
+          for (int u = 0; u < FUNROLL; ++u) {
+            const float4 a = pa[k + u];
+            float vx = 0.f, vy = 0.f, vz = 0.f;
+#pragma unroll
+            for (int r = 0; r < FR; ++r) {
+              const float dx = a.x - xi[p][r];
+              const float dy = a.y - yi[p][r];
+              const float dz = a.z - zi[p][r];
+              float s = fmaf(dx, dx, L.eps2);
+              s = fmaf(dy, dy, s);
+              s = fmaf(dz, dz, s);
+              float gg, g2p;
+              pair_g<ORDER>(L, s, gg, g2p);
+              const bool in = s < L.rcs;
+              float dl = dx * la[p][r];
+              dl = fmaf(dy, lb[p][r], dl);
+              dl = fmaf(dz, lc[p][r], dl);
+              const float gm = in ? gg : 0.f;
+              const float cc = in ? g2p * dl : 0.f;
+              vx = fmaf(gm, la[p][r], vx);
+              vy = fmaf(gm, lb[p][r], vy);
+              vz = fmaf(gm, lc[p][r], vz);
+              vx = fmaf(cc, dx, vx);
+              vy = fmaf(cc, dy, vy);
+              vz = fmaf(cc, dz, vz);
+              A[p][r] = fmaf(a.w, gm, A[p][r]);
+              const float mc = a.w * cc;
+              Bx[p][r] = fmaf(mc, dx, Bx[p][r]);
+              By[p][r] = fmaf(mc, dy, By[p][r]);
+              Bz[p][r] = fmaf(mc, dz, Bz[p][r]);
+            }
+            vx = group_sum_last(vx, lp);
+            vy = group_sum_last(vy, lp);
+            vz = group_sum_last(vz, lp);
+            const int e = g * tg + k + u;
+            if (tq == lp - 1 && e < nt) {
+              if (p == 0) {
+                jacc[0][e] = vx;
+                jacc[1][e] = vy;
+                jacc[2][e] = vz;
+              } else {
+                jacc[0][e] += vx;
+                jacc[1][e] += vy;
+                jacc[2][e] += vz;
+              }
+            }
+          }
+        }
+      }
+      __syncthreads();
+      for (int q = l; q < nt; q += 64) {
+        wx[t0 + q] = jacc[0][q];
+        wy[t0 + q] = jacc[1][q];
+        wz[t0 + q] = jacc[2][q];
+      }
+    }
+    // target side: sum the groups in order, then -(A lam + B)
+#pragma unroll
+    for (int p = 0; p < FMAXP; ++p) {
+      if (p >= np) break;
+      const int lp = lpg[p], G = 64 / lp;
+      const int g = l / lp, tq = l & (lp - 1);
+      const int ni = min(64 * FR, cnt - p * 64 * FR);
+      __syncthreads();
+      for (int q = l; q < 4 * 64 * FR; q += 64) red[q] = 0.f;
+      __syncthreads();
+      for (int gg = 0; gg < G; ++gg) {
+        if (g == gg) {
+#pragma unroll
+          for (int r = 0; r < FR; ++r) {
+            const int ii = tq + r * lp;
+            if (ii < ni) {
+              red[ii] += A[p][r];
+              red[64 * FR + ii] += Bx[p][r];
+              red[2 * 64 * FR + ii] += By[p][r];
+              red[3 * 64 * FR + ii] += Bz[p][r];
+            }
+          }
+        }
+        __syncthreads();
+      }
+      for (int ii = l; ii < ni; ii += 64) {
+        const int64_t q = off + p * 64 * FR + ii;
+        const float a = red[ii];
+        const float vx = -fmaf(a, lx[q], red[64 * FR + ii]);
+        const float vy = -fmaf(a, ly[q], red[2 * 64 * FR + ii]);
+        const float vz = -fmaf(a, lz[q], red[3 * 64 * FR + ii]);
+        if (I.pbase < 0) {
+          gx[q] = vx;
+          gy[q] = vy;
+          gz[q] = vz;
+        } else {
+          float *pp = part + I.pbase + 3 * (p * 64 * FR + ii);
+          pp[0] = vx;
+          pp[1] = vy;
+          pp[2] = vz;
+        }
+      }
+    }
+  }
+}
+
+// Fused schedule, source side: one wave per source leaf (a run of sorted
+// particle segments); particle k of the leaf receives m_k times the sum, over
+// the run's segments in sorted order, of W at the segment's entry for k.
+__global__ void __launch_bounds__(64)
+k_src_reduce(int64_t nruns, int64_t nsp, const int64_t *run_first, const int32_t *key,
+             const int32_t *val, const int32_t *seg_t, const int64_t *ent_ptr,
+             const int64_t *seg_e, const hsr_node *cn, const PJ *pj, const float *wx,
+             const float *wy, const float *wz, float *gx, float *gy, float *gz) {
+  const int64_t r = blockIdx.x;
+  if (r >= nruns) return;
+  const int64_t p0 = run_first[r];
+  const int64_t p1 = (r + 1 < nruns) ? run_first[r + 1] : nsp;
+  const hsr_node &S = cn[key[p0]];
+  for (int64_t k = threadIdx.x; k < S.count; k += 64) {
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    for (int64_t p = p0; p < p1; ++p) {
+      const int32_t sg = val[p];
+      const int64_t e = ent_ptr[seg_t[sg]] + seg_e[sg] + k;
+      sx += wx[e];
+      sy += wy[e];
+      sz += wz[e];
+    }
+    const int64_t q = S.offset + k;
+    const float m = pj[q].m;
+    gx[q] += m * sx;
+    gy[q] += m * sy;
+    gz[q] += m * sz;
+  }
+}
+
+// Fused schedule, pseudo side: W of every sorted pseudo segment's entry.
+__global__ void k_pseudo_from_w(int64_t np, int64_t p0, const int32_t *val,
+                                const int32_t *seg_t, const int64_t *ent_ptr,
+                                const int64_t *seg_e, const float *wx,
+                                const float *wy, const float *wz, float *gseg) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  const int32_t sg = val[p0 + q];
+  const int64_t e = ent_ptr[seg_t[sg]] + seg_e[sg];
+  gseg[3 * q + 0] = wx[e];
+  gseg[3 * q + 1] = wy[e];
+  gseg[3 * q + 2] = wz[e];
+}
+
+// Largest listed leaf.
+__global__ void k_max_leaf(int64_t nl, const int32_t *leaves, const hsr_node *cn,
+                           unsigned long long *out) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= nl) return;
+  atomicMax(out, (unsigned long long)cn[leaves[k]].count);
+}
+
 // ---------------------------------------------------------- transposition
 __global__ void k_seg_target(int64_t nl, const int64_t *seg_ptr,
                              int32_t *seg_t) {
@@ -323,13 +597,16 @@ __global__ void k_leaf_at(int64_t nn, const hsr_node *cn, int32_t *leaf_at) {
   if (o.cl == 0 && o.cr == 0 && o.count > 0) leaf_at[o.offset] = (int32_t)c;
 }
 
-// Sort keys: particle segment -> source leaf id; pseudo segment -> nn + node.
+// Sort keys: particle segment -> source leaf id; pseudo segment -> nn + node;
+// an empty segment (the self segment of an empty listed leaf) -> 2 nn + 1,
+// after everything else (it has no source particle).
 __global__ void k_seg_keys(int64_t ns, int64_t nn, const int64_t *sa,
                            const int64_t *sb, const int32_t *leaf_at,
                            int32_t *key, int32_t *val) {
   int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (s >= ns) return;
-  key[s] = sb[s] >= 0 ? leaf_at[sa[s]] : (int32_t)(nn + sa[s]);
+  const int64_t b = sb[s];
+  key[s] = b > 0 ? leaf_at[sa[s]] : (b < 0 ? (int32_t)(nn + sa[s]) : (int32_t)(2 * nn + 1));
   val[s] = (int32_t)s;
 }
 
@@ -490,10 +767,12 @@ k_pseudo_grad(int64_t np, int64_t p0, int64_t nn, int64_t n,
   }
 }
 
-// Per node: G_n / M_n summed over its pseudo segments in order.
+// Per node: the pseudo segment sums in order, divided by M_n when they carry
+// the pseudo-particle's mass (two-pass schedule) and as they are when they are
+// W sums (fused schedule).
 __global__ void k_pseudo_node(int64_t np, int64_t p0, int64_t nn,
                               const int32_t *key, const float *gseg,
-                              const hsr_node *cn, float *H) {
+                              const hsr_node *cn, int divide, float *H) {
   int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (q >= np) return;
   const int64_t p = p0 + q;
@@ -505,10 +784,10 @@ __global__ void k_pseudo_node(int64_t np, int64_t p0, int64_t nn,
     sy += gseg[3 * r + 1];
     sz += gseg[3 * r + 2];
   }
-  const float M = cn[node].ppm[0];
-  H[3 * node + 0] = sx / M;
-  H[3 * node + 1] = sy / M;
-  H[3 * node + 2] = sz / M;
+  const float M = divide ? cn[node].ppm[0] : 1.f;
+  H[3 * node + 0] = divide ? sx / M : sx;
+  H[3 * node + 1] = divide ? sy / M : sy;
+  H[3 * node + 2] = divide ? sz / M : sz;
 }
 
 __global__ void k_fill(int64_t n, float *a) {
@@ -636,29 +915,57 @@ hsr_status adjoint_plan(hsr_ctx *c) {
   A.key = key2;
   A.val = val2;
   A.seg_t = seg_t;
-  k_tr_counts<<<nblk(ns, 256), 256, 0, st>>>(ns, nn, key2, val2, seg_t, cn, leaves, tcnt, start);
-  if ((s = excl_scan(c, tcnt, eoff, ns)) != HSR_OK) return s;
+  // ---- schedule: fused needs every listed leaf within FMAX_LEAF and room for W
+  {
+    HSR_HIP(dbuf_reserve(A.counts, sizeof(int64_t) * 4));
+    unsigned long long *mx = reinterpret_cast<unsigned long long *>(dptr<int64_t>(A.counts) + 2);
+    k_zero_u64a<<<1, 64, 0, st>>>(mx);
+    if (nl) k_max_leaf<<<nblk(nl, 256), 256, 0, st>>>(nl, leaves, cn, mx);
+    HSR_HIP(hipMemcpyAsync(&tail_items[0], mx, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HSR_HIP(hipStreamSynchronize(st));
+    const int64_t max_leaf = tail_items[0];
+    const size_t wbytes = sizeof(float) * 3 * (size_t)(LS.nentries + 1);
+    size_t free_b = 0, total_b = 0;
+    HSR_HIP(hipMemGetInfo(&free_b, &total_b));
+    const bool fits = wbytes <= A.W.bytes || (double)wbytes <= 0.7 * (double)free_b;
+    const int mode = c->tuning.adjoint_mode;
+    if (mode == 1 && max_leaf > FMAX_LEAF) {
+      hsr_set_error("hsr_adjoint: the fused schedule takes leaves of at most 256 particles");
+      return HSR_ERR_UNSUPPORTED;
+    }
+    A.fused = mode == 1 || (mode == 0 && max_leaf <= FMAX_LEAF && fits);
+    if (A.fused) HSR_HIP(dbuf_reserve(A.W, wbytes));
+  }
+  if (!A.fused) {
+    k_tr_counts<<<nblk(ns, 256), 256, 0, st>>>(ns, nn, key2, val2, seg_t, cn, leaves, tcnt, start);
+    if ((s = excl_scan(c, tcnt, eoff, ns)) != HSR_OK) return s;
+  }
   // number of particle segments = first position whose key >= nn
   std::vector<int32_t> hkey(ns);
   HSR_HIP(hipMemcpyAsync(hkey.data(), key2, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, st));
   HSR_HIP(hipStreamSynchronize(st));
   const int64_t nsp = std::lower_bound(hkey.begin(), hkey.end(), (int32_t)nn) - hkey.begin();
+  const int64_t npe = std::lower_bound(hkey.begin(), hkey.end(), (int32_t)(2 * nn)) - hkey.begin();
   A.nsp = nsp;
-  A.npseudo = ns - nsp;
-  const int64_t ntj = nsp ? tail_sum(c, eoff, tcnt, nsp) : 0;
+  A.npseudo = npe - nsp;
+  const int64_t ntj = (nsp && !A.fused) ? tail_sum(c, eoff, tcnt, nsp) : 0;
   // runs (source leaves) among the particle segments
   std::vector<int64_t> hruns;
   for (int64_t p = 0; p < nsp; ++p)
     if (p == 0 || hkey[p - 1] != hkey[p]) hruns.push_back(p);
   const int64_t nruns = (int64_t)hruns.size();
   HSR_HIP(hipMemcpyAsync(run_first, hruns.data(), sizeof(int64_t) * nruns, hipMemcpyHostToDevice, st));
-  HSR_HIP(dbuf_reserve(A.tj, sizeof(int32_t) * (ntj + 1)));
-  if (nsp) k_tr_expand<<<(unsigned)nsp, 64, 0, st>>>(nsp, val2, seg_t, cn, leaves, eoff, dptr<int32_t>(A.tj));
-  // source-pass items
+  A.nruns = nruns;
+  A.run_first = run_first;
+  if (!A.fused) {
+    HSR_HIP(dbuf_reserve(A.tj, sizeof(int32_t) * (ntj + 1)));
+    if (nsp) k_tr_expand<<<(unsigned)nsp, 64, 0, st>>>(nsp, val2, seg_t, cn, leaves, eoff, dptr<int32_t>(A.tj));
+  }
+  // source-pass items (two-pass schedule)
   HSR_HIP(dbuf_reserve(A.tr, sizeof(int64_t) * 5 * (nruns + 2)));
   int64_t *tnc = dptr<int64_t>(A.tr), *tib = tnc + (nruns + 2), *tps = tib + (nruns + 2);
   int64_t *tpb = tps + (nruns + 2), *tfirst = tpb + (nruns + 2);
-  if (nruns) {
+  if (nruns && !A.fused) {
     k_tr_items<<<nblk(nruns, 256), 256, 0, st>>>(nruns, nsp, run_first, key2, eoff, tcnt, cn, chunk, tnc, tps);
     if ((s = excl_scan(c, tnc, tib, nruns)) != HSR_OK) return s;
     if ((s = excl_scan(c, tps, tpb, nruns)) != HSR_OK) return s;
@@ -686,13 +993,13 @@ hsr_status adjoint_plan(hsr_ctx *c) {
   HSR_HIP(hipMemcpyAsync(cb, T.box_root.data(), sizeof(int64_t) * (T.nboxes + 1), hipMemcpyHostToDevice, st));
   k_cparent<<<nblk(nn, 256), 256, 0, st>>>(nn, cn, par);
   k_roots<<<nblk(T.nboxes, 64), 64, 0, st>>>(T.nboxes, cb, par);
-  HSR_HIP(dbuf_reserve(A.counts, sizeof(int64_t) * 4));
   tail_items[2] = A.n_fw_items;
   tail_items[3] = A.n_tr_items;
   HSR_HIP(hipMemcpyAsync(dptr<int64_t>(A.counts), &tail_items[2], 2 * sizeof(int64_t), hipMemcpyHostToDevice, st));
   HSR_HIP(hipStreamSynchronize(st));
   HSR_HIP(hipGetLastError());
   A.gen = c->lists_gen;
+  A.planned_mode = c->tuning.adjoint_mode;
   return HSR_OK;
 }
 
@@ -700,7 +1007,7 @@ hsr_status adjoint_run(hsr_ctx *c, const float *lx, const float *ly,
                        const float *lz, float *gx, float *gy, float *gz) {
   hipStream_t st = c->stream;
   AdjState &A = c->adj;
-  if (A.gen != c->lists_gen) {
+  if (A.gen != c->lists_gen || A.planned_mode != c->tuning.adjoint_mode) {
     hsr_status s = adjoint_plan(c);
     if (s != HSR_OK) return s;
   }
@@ -734,14 +1041,35 @@ hsr_status adjoint_run(hsr_ctx *c, const float *lx, const float *ly,
 #define LA(MODE, ORD, ITEMS, CNT)                                                          \
   k_adj<MODE, ORD><<<(unsigned)grid, 64, 0, st>>>(cn, ITEMS, CNT, MODE == 0 ?               \
       dptr<int32_t>(c->lists.jidx) : dptr<int32_t>(A.tj), pj, lx, ly, lz, L, q, gx, gy, gz, part)
-    if (L.order == 6) LA(0, 6, dptr<AItem>(A.fw_items), counts);
-    else LA(0, 5, dptr<AItem>(A.fw_items), counts);
+    if (A.fused) {
+      float *W = dptr<float>(A.W);
+      const int64_t ne = c->lists.nentries + 1;
+      if (L.order == 6)
+        k_adj_fused<6><<<(unsigned)grid, 64, 0, st>>>(cn, dptr<AItem>(A.fw_items), counts,
+            dptr<int32_t>(c->lists.jidx), pj, lx, ly, lz, L, q, gx, gy, gz, part, W, W + ne, W + 2 * ne);
+      else
+        k_adj_fused<5><<<(unsigned)grid, 64, 0, st>>>(cn, dptr<AItem>(A.fw_items), counts,
+            dptr<int32_t>(c->lists.jidx), pj, lx, ly, lz, L, q, gx, gy, gz, part, W, W + ne, W + 2 * ne);
+    } else if (L.order == 6) {
+      LA(0, 6, dptr<AItem>(A.fw_items), counts);
+    } else {
+      LA(0, 5, dptr<AItem>(A.fw_items), counts);
+    }
     if (A.fw_part)
       k_adj_reduce<<<(unsigned)A.n_fw_leaves, 64, 0, st>>>(cn, dptr<AItem>(A.fw_items), A.fw_first,
                                                             A.n_fw_leaves, 0, part, gx, gy, gz);
   }
-  // source pass
-  if (A.n_tr_items) {
+  const int64_t ns = c->lists.nsegs;
+  const int64_t *seg_e = dptr<int64_t>(c->lists.seg) + 2 * (ns + 1);
+  const int64_t *ent_ptr = dptr<int64_t>(c->lists.ent_ptr);
+  const float *W = A.fused ? dptr<float>(A.W) : nullptr;
+  const int64_t ne = c->lists.nentries + 1;
+  // source side of the fused schedule
+  if (A.fused && A.nruns)
+    k_src_reduce<<<(unsigned)A.nruns, 64, 0, st>>>(A.nruns, A.nsp, A.run_first, A.key, A.val, A.seg_t, ent_ptr,
+                                                  seg_e, cn, pj, W, W + ne, W + 2 * ne, gx, gy, gz);
+  // source pass of the two-pass schedule
+  if (!A.fused && A.n_tr_items) {
     k_zero_u64a<<<1, 64, 0, st>>>(q);
     const int64_t grid = std::min<int64_t>(A.n_tr_items, (int64_t)std::max(1, c->num_cus) * wpc);
     if (L.order == 6) LA(1, 6, dptr<AItem>(A.tr_items), counts + 1);
@@ -753,14 +1081,19 @@ hsr_status adjoint_run(hsr_ctx *c, const float *lx, const float *ly,
   }
   // pseudo pass
   if (nn) k_fill<<<nblk(3 * nn, 256), 256, 0, st>>>(3 * nn, H);
-  if (A.npseudo) {
+  if (A.npseudo && A.fused) {
+    k_pseudo_from_w<<<nblk(A.npseudo, 256), 256, 0, st>>>(A.npseudo, A.nsp, A.val, A.seg_t, ent_ptr, seg_e,
+                                                          W, W + ne, W + 2 * ne, gseg);
+    k_pseudo_node<<<nblk(A.npseudo, 256), 256, 0, st>>>(A.npseudo, A.nsp, nn, A.key, gseg, cn, 0, H);
+    k_pseudo_push<<<(unsigned)nn, 64, 0, st>>>(nn, cn, dptr<int32_t>(A.par), H, pj, gx, gy, gz);
+  } else if (A.npseudo) {
     if (L.order == 6)
       k_pseudo_grad<6><<<(unsigned)A.npseudo, 64, 0, st>>>(A.npseudo, A.nsp, nn, n, A.key, A.val, A.seg_t,
                                                            dptr<int32_t>(c->lists.leaves), cn, pj, lx, ly, lz, L, gseg);
     else
       k_pseudo_grad<5><<<(unsigned)A.npseudo, 64, 0, st>>>(A.npseudo, A.nsp, nn, n, A.key, A.val, A.seg_t,
                                                            dptr<int32_t>(c->lists.leaves), cn, pj, lx, ly, lz, L, gseg);
-    k_pseudo_node<<<nblk(A.npseudo, 256), 256, 0, st>>>(A.npseudo, A.nsp, nn, A.key, gseg, cn, H);
+    k_pseudo_node<<<nblk(A.npseudo, 256), 256, 0, st>>>(A.npseudo, A.nsp, nn, A.key, gseg, cn, 1, H);
     k_pseudo_push<<<(unsigned)nn, 64, 0, st>>>(nn, cn, dptr<int32_t>(A.par), H, pj, gx, gy, gz);
   }
   HSR_HIP(hipEventRecord(c->ev[7], st));

@@ -123,9 +123,13 @@ struct ListState {
 // transposed (source-major) lists and the pseudo-particle segments.
 struct AdjState {
   int64_t gen = -1;
-  DBuf fw, fw_items, tr, tr_items, seg, tj, par, counts, qctr, part, H;
+  DBuf fw, fw_items, tr, tr_items, seg, tj, par, counts, qctr, part, H, W;
   int64_t n_fw_items = 0, n_fw_leaves = 0, n_tr_items = 0, n_tr_leaves = 0;
   int64_t nsp = 0, npseudo = 0, fw_part = 0, tr_part = 0;
+  bool fused = false;        // one pair evaluation feeds both sides (W per entry)
+  int planned_mode = -1;     // tuning.adjoint_mode the plan was made for
+  int64_t nruns = 0;         // source leaves among the sorted particle segments
+  int64_t *run_first = nullptr;
   int64_t *fw_first = nullptr, *tr_first = nullptr;
   int32_t *key = nullptr, *val = nullptr, *seg_t = nullptr;
 };

@@ -71,7 +71,7 @@ class Stats(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("i_per_lane", C.c_int), ("tile", C.c_int), ("chunk", C.c_int),
                 ("waves_per_cu", C.c_int), ("use_graph", C.c_int),
-                ("reserved", C.c_int * 3)]
+                ("adjoint_mode", C.c_int), ("reserved", C.c_int * 2)]
 
 
 _lib = None

@@ -173,7 +173,11 @@ typedef struct {
   int chunk;          /* maximum list entries per work item (0 = automatic) */
   int waves_per_cu;   /* persistent waves per CU (0 = automatic) */
   int use_graph;      /* capture the force phase in a hipGraph and replay it */
-  int reserved[3];
+  int adjoint_mode;   /* hsr_adjoint passes: 0 automatic, 1 fused (one pair
+                         evaluation feeds target and source; leaves of at most
+                         256 particles; 12 B of device memory per list entry),
+                         2 separate target and source passes */
+  int reserved[2];
 } hsr_tuning;
 
 hsr_status hsr_set_tuning(hsr_ctx *ctx, const hsr_tuning *t);

@@ -763,9 +763,13 @@ int64_t hsro_cell_sort_perm(int64_t n, const float *x, const float *y,
 }
 
 /* -------------------------------------------------------------- adjoint */
+/* g(r2) and g'(r2) of the Step16 law.  With rsm > 0 the pair term
+ * m g(r2 + rsm2) d is smooth through d = 0, so a coincident pair (r2 == 0,
+ * whose force term the r2 > 0 guard of BGQStep16.c:180 zeroes) still has
+ * the derivative m g(rsm2) I; the guard only removes the pair when rsm = 0. */
 static void law_f64_g(double r2, double rsm2, double rc2, double *g,
                       double *gp) {
-  if (!(r2 > 0.0) || !(r2 < rc2)) {
+  if (!(r2 < rc2) || (!(rsm2 > 0.0) && !(r2 > 0.0))) {
     *g = 0.0;
     *gp = 0.0;
     return;

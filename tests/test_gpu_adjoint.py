@@ -39,13 +39,17 @@ def rel_rms(a, b):
     return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
 
 
+# adjoint_mode 1 = fused schedule (one pair evaluation for both sides),
+# 2 = separate target and source passes
+@pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("n,box,nd,oa,chunk", [(6000, 10.0, 16, 0.7, 0), (20000, 14.0, 64, 0.1, 0),
-                                               (20000, 14.0, 64, 0.5, 300)])
-def test_adjoint_matches_oracle(n, box, nd, oa, chunk):
-    say(f"case adjoint n={n} nd={nd} oa={oa} chunk={chunk}")
+                                               (20000, 14.0, 64, 0.5, 300), (30000, 12.0, 256, 0.5, 0)])
+def test_adjoint_matches_oracle(n, box, nd, oa, chunk, mode):
+    say(f"case adjoint n={n} nd={nd} oa={oa} chunk={chunk} mode={mode}")
     rng = np.random.default_rng(n)
     p = (rng.random((n, 3)) * box).astype(np.float32)
     tp, tree, lists, ctx, lam = setup(p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), box, nd, oa)
+    ctx.set_tuning(adjoint_mode=mode)
     if chunk:
         ctx.set_tuning(chunk=chunk)
         ctx.lists(tp.min_force, tp.max_force)
@@ -61,8 +65,9 @@ def test_adjoint_matches_oracle(n, box, nd, oa, chunk):
     assert np.array_equal(got.view(np.uint32), g2.view(np.uint32))
 
 
-def test_adjoint_config_a():
-    say("case adjoint config A (64^3, nd=128)")
+@pytest.mark.parametrize("mode", [1, 2])
+def test_adjoint_config_a(mode):
+    say(f"case adjoint config A (64^3, nd=128) mode={mode}")
     load = ics.config_load("A")
     lo, hi = load.tree_box
     flo, fhi = load.force_box
@@ -74,6 +79,7 @@ def test_adjoint_config_a():
     X, Y, Z, M = load.x.copy(), load.y.copy(), load.z.copy(), m.copy()
     ctx.build(X, Y, Z, M, min_loc=lo, max_loc=hi)
     ctx.lists(flo, fhi)
+    ctx.set_tuning(adjoint_mode=mode)
     rng = np.random.default_rng(1)
     lam = rng.normal(size=(load.n, 3))
     sel = np.zeros(load.n, bool)
@@ -91,4 +97,49 @@ def test_adjoint_config_a():
     adj = ctx.stats().adjoint_ms
     say(f"  rel rms vs FP64 {e:.3e}; forward kernel {fwd:.2f} ms, adjoint {adj:.2f} ms "
         f"(ratio {adj / fwd:.2f})")
+    assert e < 1e-4
+
+
+def test_adjoint_schedules_agree_and_large_leaves_fall_back():
+    """Fused and two-pass schedules agree to rounding; a leaf above 256
+    particles (nd 300) makes the automatic choice take the two-pass schedule
+    and the fused one refuse."""
+    say("case adjoint schedules nd=300")
+    rng = np.random.default_rng(77)
+    n, box = 20000, 12.0
+    p = (rng.random((n, 3)) * box).astype(np.float32)
+    tp, tree, lists, ctx, lam = setup(p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), box, 300, 0.5)
+    assert tree.nodes["count"][lists.leaves].max() > 256
+    L32 = lam.astype(np.float32)
+    args = (L32[:, 0].copy(), L32[:, 1].copy(), L32[:, 2].copy())
+    ref = O.adjoint_f64(tp, tree, lists, lam)
+    got = np.stack(ctx.adjoint(*args), 1)          # automatic -> two-pass
+    assert rel_rms(got, ref) < 1e-4
+    ctx.set_tuning(adjoint_mode=1)
+    with pytest.raises(Exception, match="256"):
+        ctx.adjoint(*args)
+    ctx.set_tuning(adjoint_mode=2)
+    g2 = np.stack(ctx.adjoint(*args), 1)
+    assert np.array_equal(got, g2)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_adjoint_degenerate_tree(mode):
+    """Duplicate positions make an unsplittable leaf of 200 particles (two
+    passes over its targets in the fused schedule) next to ordinary leaves."""
+    say(f"case adjoint degenerate duplicates mode={mode}")
+    rng = np.random.default_rng(4)
+    p = (rng.random((4000, 3)) * 10.0).astype(np.float32)
+    dup = np.full(200, 5.25, np.float32)
+    x = np.concatenate([p[:, 0], dup])
+    y = np.concatenate([p[:, 1], dup + 0.5])
+    z = np.concatenate([p[:, 2], dup - 0.5])
+    tp, tree, lists, ctx, lam = setup(x, y, z, 10.0, 32, 0.5)
+    assert tree.nodes["count"][lists.leaves].max() == 200
+    ctx.set_tuning(adjoint_mode=mode)
+    ref = O.adjoint_f64(tp, tree, lists, lam)
+    L32 = lam.astype(np.float32)
+    got = np.stack(ctx.adjoint(L32[:, 0].copy(), L32[:, 1].copy(), L32[:, 2].copy()), 1)
+    e = rel_rms(got, ref)
+    say(f"  rel rms vs FP64 {e:.3e}")
     assert e < 1e-4

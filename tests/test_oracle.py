@@ -403,3 +403,35 @@ def test_kick_drift_reverse_pass_matches_finite_differences():
     assert err < 1e-3, err
     # a dropped image contribution or a wrong sub-step order would fail this
     assert np.sqrt(np.mean(ref ** 2)) > 1e-2
+
+
+def test_adjoint_f64_coincident_pairs_match_finite_differences():
+    """Exactly coincident particles: the r2 > 0 guard zeroes their pair force
+    (BGQStep16.c:180) but with rsm > 0 the pair term is smooth through d = 0,
+    so the adjoint keeps the derivative m g(rsm^2); finite differences of the
+    FP64 objective at members of a 200-fold duplicate pin it (a dropped
+    coincident-pair term is off by ~1e5 there)."""
+    rng = np.random.default_rng(4)
+    p = (rng.random((4000, 3)) * 10.0).astype(np.float32)
+    dup = np.full(200, 5.25, np.float32)
+    x = np.concatenate([p[:, 0], dup])
+    y = np.concatenate([p[:, 1], dup + 0.5])
+    z = np.concatenate([p[:, 2], dup - 0.5])
+    tp = small_params(10.0, 32, oa=0.5)
+    tree = O.build_tree(tp, x, y, z)
+    lists = O.build_lists(tp, tree)
+    lam = np.random.default_rng(3).normal(size=(tree.x.shape[0], 3))
+    grad = O.adjoint_f64(tp, tree, lists, lam)
+    pos = [tree.x.astype(np.float64), tree.y.astype(np.float64), tree.z.astype(np.float64)]
+    members = np.nonzero((tree.x == np.float32(5.25)) & (tree.y == np.float32(5.75)))[0]
+    assert members.size == 200
+    h = 1e-4
+    for k in members[:6]:
+        for d in range(3):
+            pp = [a.copy() for a in pos]
+            pm = [a.copy() for a in pos]
+            pp[d][k] += h
+            pm[d][k] -= h
+            fd = (O.objective_f64(tp, tree, lists, lam, pp)
+                  - O.objective_f64(tp, tree, lists, lam, pm)) / (2 * h)
+            assert abs(fd - grad[k, d]) <= 1e-4 * max(abs(fd), 1.0), (k, d, fd, grad[k, d])
