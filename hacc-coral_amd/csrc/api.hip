@@ -304,7 +304,7 @@ hsr_status hsr_destroy(hsr_ctx *c) {
                   &c->lists.plan, &c->lists.part, &c->s.qctr,
                   &c->adj.fw, &c->adj.fw_items, &c->adj.tr, &c->adj.tr_items, &c->adj.seg,
                   &c->adj.tj, &c->adj.par, &c->adj.counts, &c->adj.qctr, &c->adj.part, &c->adj.H,
-                  &c->adj.W,
+                  &c->adj.W, &c->adj.msum,
                   &c->s.a, &c->s.b, &c->s.c, &c->s.d, &c->s.e, &c->s.f, &c->s.g, &c->s.h,
                   &c->s.cub, &c->s.x1, &c->s.y1, &c->s.z1, &c->s.m1, &c->s.p1,
                   &c->s.x0, &c->s.y0, &c->s.z0, &c->s.m0, &c->s.p0, &c->s.blocks, &c->s.rights,
@@ -518,6 +518,76 @@ hsr_status hsr_adjoint(hsr_ctx *c, const float *lx, const float *ly,
   HSR_HIP(hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
   c->kernel_ms = ms;
   c->adjoint_ms = ms;
+  return HSR_OK;
+}
+
+hsr_status hsr_forces_f64(hsr_ctx *c, double *ax, double *ay, double *az,
+                          double *leaf_mom, int ptr_flags) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (!c->listed) {
+    g_err = "hsr_forces_f64: no lists (call hsr_lists first)";
+    return HSR_ERR_STATE;
+  }
+  if (!ax || !ay || !az) {
+    g_err = "hsr_forces_f64: null output";
+    return HSR_ERR_ARG;
+  }
+  const int64_t n = c->tree.n, nl = c->lists.nleaves;
+  double *d[4] = {ax, ay, az, leaf_mom};
+  DBuf &buf = c->s.host_in[1];
+  if (!(ptr_flags & HSR_DEVICE_PTRS)) {
+    HSR_HIP(dbuf_reserve(buf, sizeof(double) * (3 * (size_t)std::max<int64_t>(n, 1) + 3 * (size_t)(nl + 1))));
+    double *b = dptr<double>(buf);
+    d[0] = b;
+    d[1] = b + n;
+    d[2] = b + 2 * n;
+    d[3] = leaf_mom ? b + 3 * n : nullptr;
+  }
+  s = forces_f64_run(c, d[0], d[1], d[2], d[3]);
+  if (s) return s;
+  if (!(ptr_flags & HSR_DEVICE_PTRS)) {
+    if (n)
+      for (int k = 0; k < 3; ++k)
+        HSR_HIP(hipMemcpyAsync(k == 0 ? ax : (k == 1 ? ay : az), d[k], sizeof(double) * n,
+                               hipMemcpyDeviceToHost, c->stream));
+    if (leaf_mom && nl)
+      HSR_HIP(hipMemcpyAsync(leaf_mom, d[3], sizeof(double) * 3 * nl, hipMemcpyDeviceToHost, c->stream));
+  }
+  HSR_HIP(hipStreamSynchronize(c->stream));
+  return HSR_OK;
+}
+
+hsr_status hsr_leaf_momentum(hsr_ctx *c, const float *ax, const float *ay,
+                             const float *az, double *leaf_mom, int ptr_flags) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (!c->listed) {
+    g_err = "hsr_leaf_momentum: no lists (call hsr_lists first)";
+    return HSR_ERR_STATE;
+  }
+  if (!ax || !ay || !az || !leaf_mom) {
+    g_err = "hsr_leaf_momentum: null argument";
+    return HSR_ERR_ARG;
+  }
+  const int64_t n = c->tree.n, nl = c->lists.nleaves;
+  const float *a[3] = {ax, ay, az};
+  double *mom = leaf_mom;
+  DBuf &buf = c->s.host_in[2];
+  if (!(ptr_flags & HSR_DEVICE_PTRS)) {
+    HSR_HIP(dbuf_reserve(buf, sizeof(float) * 3 * (size_t)std::max<int64_t>(n, 1) + sizeof(double) * 3 * (size_t)(nl + 2)));
+    float *b = dptr<float>(buf);
+    for (int k = 0; k < 3; ++k) {
+      if (n) HSR_HIP(hipMemcpyAsync(b + k * n, a[k], sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+      a[k] = b + k * n;
+    }
+    mom = reinterpret_cast<double *>(b + 3 * std::max<int64_t>(n, 1) + (n % 2 ? 1 : 0));
+  }
+  s = leaf_momentum_run(c, a[0], a[1], a[2], mom);
+  if (s) return s;
+  if (!(ptr_flags & HSR_DEVICE_PTRS) && nl)
+    HSR_HIP(hipMemcpyAsync(leaf_mom, mom, sizeof(double) * 3 * nl, hipMemcpyDeviceToHost, c->stream));
+  HSR_HIP(hipStreamSynchronize(c->stream));
   return HSR_OK;
 }
 

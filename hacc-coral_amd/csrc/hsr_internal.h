@@ -104,7 +104,7 @@ struct ListState {
   int64_t nsegs = 0;
   int64_t nentries = 0;
   int64_t pairs = 0;
-  DBuf leaves;                // int32 BFS ids of listed leaves (reference order)
+  DBuf leaves;                // int32 canonical ids of listed leaves (reference order)
   DBuf seg_ptr;               // int64 [nleaves+1]
   DBuf seg;                   // int2-like (int64 a, int64 b) pairs
   DBuf ent_ptr;               // int64 [nleaves+1] list-entry CSR
@@ -123,7 +123,8 @@ struct ListState {
 // transposed (source-major) lists and the pseudo-particle segments.
 struct AdjState {
   int64_t gen = -1;
-  DBuf fw, fw_items, tr, tr_items, seg, tj, par, counts, qctr, part, H, W;
+  DBuf fw, fw_items, tr, tr_items, seg, tj, par, counts, qctr, part, H, W, msum;
+  const double *mpre = nullptr;  // FP64 prefix of the sorted masses [n+1]
   int64_t n_fw_items = 0, n_fw_leaves = 0, n_tr_items = 0, n_tr_leaves = 0;
   int64_t nsp = 0, npseudo = 0, fw_part = 0, tr_part = 0;
   bool fused = false;        // one pair evaluation feeds both sides (W per entry)
@@ -198,3 +199,6 @@ hsr_status kick_run(hsr_ctx *c, float fcoeff, float *vx, float *vy, float *vz,
                     const float *mass_sorted);
 hsr_status adjoint_run(hsr_ctx *c, const float *lx, const float *ly,
                        const float *lz, float *gx, float *gy, float *gz);
+hsr_status forces_f64_run(hsr_ctx *c, double *ax, double *ay, double *az, double *mom);
+hsr_status leaf_momentum_run(hsr_ctx *c, const float *ax, const float *ay, const float *az,
+                             double *mom);

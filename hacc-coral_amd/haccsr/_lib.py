@@ -27,7 +27,7 @@ EXPORTS = [
     "hsr_get_stream", "hsr_build", "hsr_lists", "hsr_forces", "hsr_kick",
     "hsr_adjoint", "hsr_get_stats", "hsr_get_nodes", "hsr_get_perm",
     "hsr_get_lists", "hsr_build_many", "hsr_lists_many", "hsr_set_tuning",
-    "hsr_get_tuning",
+    "hsr_get_tuning", "hsr_forces_f64", "hsr_leaf_momentum",
 ]
 
 
@@ -99,6 +99,8 @@ def load(path: str | None = None):
     L.hsr_forces.argtypes = [vp, vp, vp, vp, C.c_int]
     L.hsr_kick.argtypes = [vp, C.c_float]
     L.hsr_adjoint.argtypes = [vp] + [vp] * 6 + [C.c_int]
+    L.hsr_forces_f64.argtypes = [vp] + [vp] * 4 + [C.c_int]
+    L.hsr_leaf_momentum.argtypes = [vp] + [vp] * 4 + [C.c_int]
     L.hsr_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.hsr_get_nodes.argtypes = [vp, vp, i64]
     L.hsr_get_perm.argtypes = [vp, vp, i64]
@@ -233,6 +235,28 @@ class Context:
         check(self.L.hsr_adjoint(self.h, _ptr(lx), _ptr(ly), _ptr(lz), _ptr(gx),
                                  _ptr(gy), _ptr(gz), fl), "hsr_adjoint")
         return gx, gy, gz
+
+    def forces_f64(self, ax=None, ay=None, az=None, leaf_mom=None, device_ptrs=False,
+                   with_momentum=False):
+        """FP64 evaluation of the same lists (tree order).  Host arrays by
+        default; with_momentum also returns sum_i m_i a_i per listed leaf."""
+        if ax is None:
+            ax, ay, az = (np.zeros(self.n) for _ in range(3))
+        if with_momentum and leaf_mom is None:
+            leaf_mom = np.zeros((max(self.stats().listed_leaves, 1), 3))
+        fl = HSR_DEVICE_PTRS if device_ptrs else HSR_HOST_PTRS
+        check(self.L.hsr_forces_f64(self.h, _ptr(ax), _ptr(ay), _ptr(az), _ptr(leaf_mom), fl),
+              "hsr_forces_f64")
+        return (ax, ay, az, leaf_mom) if with_momentum else (ax, ay, az)
+
+    def leaf_momentum(self, ax, ay, az, leaf_mom=None, device_ptrs=False):
+        """sum_i m_i a_i per listed leaf (FP64) of float accelerations."""
+        if leaf_mom is None:
+            leaf_mom = np.zeros((max(self.stats().listed_leaves, 1), 3))
+        fl = HSR_DEVICE_PTRS if device_ptrs else HSR_HOST_PTRS
+        check(self.L.hsr_leaf_momentum(self.h, _ptr(ax), _ptr(ay), _ptr(az), _ptr(leaf_mom), fl),
+              "hsr_leaf_momentum")
+        return leaf_mom
 
     # -- introspection
     def stats(self) -> Stats:

@@ -7,10 +7,10 @@ d(sum lambda . F)/dx with the topology frozen).  Only the positions of each
 call are kept (a checkpoint); the backward pass rebuilds the tree and lists
 from them, so memory stays at one tree per call in flight.
 
-``kick_drift`` runs sub-steps v += c F(x); x += dt v, where c is the kick
+``kick_drift`` runs sub-steps v += c F(x); x += dt v (or the drift-kick-drift
+form of Particles::subCycle, Particles.cxx:1176-1201), where c is the kick
 coefficient map2 passes to the tree (Particles.cxx:1231-1234,
-c = gpscal^3/(4 pi) * fscal * tau * stepFraction) and Particles::subCycle
-(Particles.cxx:1176-1201) is the sub-step loop.  Periodic images are
+c = gpscal^3/(4 pi) * fscal * tau * stepFraction).  Periodic images are
 re-derived from the box's own particles at every sub-step, so gradients
 that reach an image flow back to the particle it images.
 """
@@ -148,14 +148,28 @@ def images_from_layout(alive_x, alive_y, alive_z, box: float, overload: float,
 
 
 def kick_drift(x: torch.Tensor, v: torch.Tensor, solver: ForceSolver, nsub: int,
-               dt: float, kick: float, images: Images | None = None):
-    """nsub sub-steps v += kick * F(x); x += dt * v, differentiable through
-    ShortRangeForce.  x, v: the box's own particles (n_alive x 3)."""
-    for _ in range(nsub):
+               dt: float, kick: float, images: Images | None = None,
+               scheme: str = "kd"):
+    """nsub differentiable sub-steps through ShortRangeForce.  x, v: the
+    box's own particles (n_alive x 3).
+
+    scheme "kd":  v += kick * F(x); x += dt * v.
+    scheme "dkd": x += dt/2 * v; v += kick * F(x); x += dt/2 * v, the shape of
+                  Particles::subCycle (half stream, kick, half stream)."""
+    if scheme not in ("kd", "dkd"):
+        raise ValueError("scheme must be 'kd' or 'dkd'")
+
+    def force(x):
         pos = x if images is None else x[images.index] + images.shift
         F = ShortRangeForce.apply(pos, solver)
-        if images is not None:
-            F = F[: images.n_alive]
-        v = v + kick * F
-        x = x + dt * v
+        return F if images is None else F[: images.n_alive]
+
+    for _ in range(nsub):
+        if scheme == "kd":
+            v = v + kick * force(x)
+            x = x + dt * v
+        else:
+            x = x + (0.5 * dt) * v
+            v = v + kick * force(x)
+            x = x + (0.5 * dt) * v
     return x, v

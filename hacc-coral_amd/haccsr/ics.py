@@ -105,12 +105,14 @@ def _nfw_radii(rng, n, c, r200):
 
 def clustered(n_side: int, box: float, n_halos: int, seed: int,
               rms_disp: float = 2.0, halo_min: float = 1e3,
-              halo_max: float = 1e5):
+              halo_max: float = 1e5, log=None):
     """Gaussian random field with P(k) ~ k^-2 (k from 2 pi / L to Nyquist),
     first-order Zel'dovich displacement scaled to RMS ``rms_disp``, plus
     ``n_halos`` NFW halos (c ~ U[5,15], N_h ~ logU[halo_min, halo_max],
     R200 = (3 N_h / (4 pi 200))^(1/3) at mean density 1).  Halo particles
-    replace field particles so the total stays n_side^3."""
+    replace field particles so the total stays n_side^3.  ``log``, if given,
+    receives a line after each stage."""
+    say = log or (lambda msg: None)
     rng = np.random.default_rng(seed)
     n = n_side ** 3
     kf = 2.0 * np.pi / box
@@ -126,11 +128,13 @@ def clustered(n_side: int, box: float, n_halos: int, seed: int,
     noise = rng.normal(size=(n_side, n_side, n_side)).astype(np.float32)
     dk = np.fft.rfftn(noise) * amp.astype(np.float32)
     del noise
+    say(f"clustered {n_side}^3: density field done")
     disp = []
     for kk in (kx, ky, kz):
         psi_k = (1j * kk / k2).astype(np.complex64) * dk
         disp.append(np.fft.irfftn(psi_k, s=(n_side,) * 3, axes=(0, 1, 2)).astype(np.float32))
         del psi_k
+        say(f"clustered {n_side}^3: displacement {len(disp)}/3 done")
     del dk
     rms = math.sqrt(sum(float(np.mean(d.astype(np.float64) ** 2)) for d in disp))
     scale = rms_disp / rms if rms > 0 else 0.0
@@ -163,6 +167,8 @@ def clustered(n_side: int, box: float, n_halos: int, seed: int,
         d /= np.linalg.norm(d, axis=1, keepdims=True)
         pos[repl[o:o + nh]] = (centre + r[:, None] * d).astype(np.float32)
         o += nh
+        if h % 250 == 249:
+            say(f"clustered {n_side}^3: {h + 1}/{n_halos} halos placed")
     pos = np.mod(pos.astype(np.float64), box).astype(np.float32)
     pos[pos >= np.float32(box)] = np.float32(0.0)
     return pos[:, 0].copy(), pos[:, 1].copy(), pos[:, 2].copy()
@@ -227,7 +233,7 @@ def cell_sort(load: Load) -> Load:
 
 
 def config_load(name: str, seed: int | None = None, sort: bool = True,
-                overload: float = 3.2, n_side: int | None = None):
+                overload: float = 3.2, n_side: int | None = None, log=None):
     """Named loads.  'A': 64^3 Plummer mix in a 64 box (seed 1001);
     'B': 256^3 clustered + 500 halos (seed 1002); 'D': 128^3 clustered,
     halos scaled by volume (seed 1004); 'S<n>': small clustered n^3 test
@@ -240,17 +246,17 @@ def config_load(name: str, seed: int | None = None, sort: bool = True,
     elif name == "B":
         s = 1002 if seed is None else seed
         ns = 256 if n_side is None else n_side
-        x, y, z = clustered(ns, float(ns), max(1, round(500 * (ns / 256) ** 3)), s)
+        x, y, z = clustered(ns, float(ns), max(1, round(500 * (ns / 256) ** 3)), s, log=log)
         box = float(ns)
     elif name == "C":
         s = 1003 if seed is None else seed
         ns = 512 if n_side is None else n_side
-        x, y, z = clustered(ns, float(ns), max(1, round(2000 * (ns / 512) ** 3)), s)
+        x, y, z = clustered(ns, float(ns), max(1, round(2000 * (ns / 512) ** 3)), s, log=log)
         box = float(ns)
     elif name == "D":
         s = 1004 if seed is None else seed
         ns = 128 if n_side is None else n_side
-        x, y, z = clustered(ns, float(ns), max(1, round(500 * (ns / 256) ** 3)), s)
+        x, y, z = clustered(ns, float(ns), max(1, round(500 * (ns / 256) ** 3)), s, log=log)
         box = float(ns)
     else:
         raise ValueError(f"unknown load {name!r}")

@@ -166,6 +166,25 @@ hsr_status hsr_build_many(hsr_ctx *ctx, int64_t nboxes, const int64_t *offsets,
 hsr_status hsr_lists_many(hsr_ctx *ctx, const float *min_force,
                           const float *max_force);
 
+/* ------------------------------------------------------ FP64 checks */
+/* No counterpart in the reference: FP64 evaluations for the precision and
+ * stability studies, on the device, over the current tree and lists.
+ *
+ * hsr_forces_f64: a_i = sum_j m_j f(r2) d in double (float positions and
+ * masses promoted; pseudo-particles at their float centre with mass ppm;
+ * f = (r2 + rsm^2)^-3/2 - P(r2) with the law's float coefficients promoted,
+ * r2 > 0 and r2 < cutoff as in BGQStep16.c:167-187).  Tree order, like
+ * hsr_forces; particles outside the listed leaves get 0.  If leaf_mom is not
+ * NULL it receives sum_i m_i a_i of every listed leaf (3 doubles per leaf, in
+ * hsr_get_lists leaf order).
+ *
+ * hsr_leaf_momentum: sum_i m_i a_i per listed leaf, accumulated in double, of
+ * float accelerations a in tree order (as hsr_forces returns them). */
+hsr_status hsr_forces_f64(hsr_ctx *ctx, double *ax, double *ay, double *az,
+                          double *leaf_mom, int ptr_flags);
+hsr_status hsr_leaf_momentum(hsr_ctx *ctx, const float *ax, const float *ay,
+                             const float *az, double *leaf_mom, int ptr_flags);
+
 /* ------------------------------------------------------------- tuning */
 typedef struct {
   int i_per_lane;     /* i-particles held per lane (0 = automatic) */

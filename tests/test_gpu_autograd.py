@@ -56,7 +56,7 @@ def make_case(n_side, seed, nd, oa):
     return dict(tp=tp, base=base, idx=idx, shift=shift, v0=v0, wx=wx, wv=wv)
 
 
-def run_gpu(c):
+def run_gpu(c, scheme="kd"):
     tp = c["tp"]
     solver = ForceSolver(tp.min_loc, tp.max_loc, tp.min_force, tp.max_force,
                          nd=tp.nd, oa=tp.oa, fsrrmax=tp.fsrrmax, rsm=tp.rsm, record=True)
@@ -64,7 +64,7 @@ def run_gpu(c):
     img = Images(torch.from_numpy(c["idx"]).cuda(), torch.from_numpy(c["shift"]).cuda(), n)
     x0 = torch.from_numpy(c["base"]).cuda().requires_grad_(True)
     v0 = torch.from_numpy(c["v0"]).cuda().requires_grad_(True)
-    xN, vN = kick_drift(x0, v0, solver, NSUB, DT, KICK, img)
+    xN, vN = kick_drift(x0, v0, solver, NSUB, DT, KICK, img, scheme=scheme)
     wx = torch.from_numpy(c["wx"].astype(np.float32)).cuda()
     wv = torch.from_numpy(c["wv"].astype(np.float32)).cuda()
     J = (xN * wx).sum() + (vN * wv).sum()
@@ -153,3 +153,25 @@ def test_single_force_vjp_with_resident_tree():
     say(f"  rel rms vs FP64 adjoint {e:.3e}")
     assert e < 1e-4
     solver.ctx.close()
+
+
+def test_drift_kick_drift_gradient():
+    """The drift-kick-drift form of Particles::subCycle through the same
+    autograd path: GPU gradient vs the FP64 reverse pass and 64 FD
+    coordinates."""
+    say("case drift-kick-drift n_side=12")
+    c = make_case(12, 45, 32, 0.5)
+    g = run_gpu(c, scheme="dkd")
+    tp, base, idx, shift = c["tp"], c["base"], c["idx"], c["shift"]
+    args = (tp, g["topos"], base, c["v0"], idx, shift.astype(np.float64), DT, KICK)
+    gx64, gv64 = K.gradient(*args, c["wx"], c["wv"], scheme="dkd")
+    fx64, fv64 = gx64 - c["wx"], gv64 - (c["wv"] + NSUB * DT * c["wx"])
+    fx, fv = g["gx"] - c["wx"], g["gv"] - (c["wv"] + NSUB * DT * c["wx"])
+    e_gx, e_gv = rel_rms(fx, fx64), rel_rms(fv, fv64)
+    rng = np.random.default_rng(46)
+    coords = list(zip(rng.integers(0, base.shape[0], 64).tolist(), rng.integers(0, 3, 64).tolist()))
+    fd = K.fd_gradient(*args, c["wx"], c["wv"], coords, h=1e-3, scheme="dkd")
+    w = np.array([c["wx"][i, a] for i, a in coords])
+    e_fd = rel_rms(np.array([fx[i, a] for i, a in coords]), fd - w)
+    say(f"  gradient rel rms dx0 {e_gx:.3e} dv0 {e_gv:.3e}; GPU vs FD {e_fd:.3e}")
+    assert e_gx < 1e-3 and e_gv < 1e-3 and e_fd < 2e-3

@@ -374,11 +374,12 @@ def test_force_f64_at_matches_tree_force():
     assert np.sqrt(np.sum((a - ref) ** 2) / np.sum(ref ** 2)) < 1e-5
 
 
-def test_kick_drift_reverse_pass_matches_finite_differences():
-    """The FP64 reverse pass over 5 kick-drift sub-steps with periodic
-    images (tests/kdref.py, the reference gradient of the GPU autograd
-    test) equals central finite differences of the FP64 trajectory, h = 1e-3,
-    with every sub-step's topology frozen."""
+@pytest.mark.parametrize("scheme", ["kd", "dkd"])
+def test_kick_drift_reverse_pass_matches_finite_differences(scheme):
+    """The FP64 reverse pass over 5 kick-drift (or drift-kick-drift) sub-steps
+    with periodic images (tests/kdref.py, the reference gradient of the GPU
+    autograd test) equals central finite differences of the FP64 trajectory,
+    h = 1e-3, with every sub-step's topology frozen."""
     import kdref as K
     from haccsr import ics
     box, o, dt = 10.0, 3.2, 0.01
@@ -392,11 +393,11 @@ def test_kick_drift_reverse_pass_matches_finite_differences():
     v0, wx, wv = (rng.normal(size=base.shape) for _ in range(3))
     tp = O.TreeParams(np.full(3, -1, np.float32), np.full(3, box + 2 * o + 1, np.float32),
                       np.full(3, o, np.float32), np.full(3, box + o, np.float32), nd=32, oa=0.5)
-    topos = K.oracle_topos(tp, base, v0, idx, shift, dt, kick, 5)
+    topos = K.oracle_topos(tp, base, v0, idx, shift, dt, kick, 5, scheme)
     assert sum(int(np.sum(t.lists.seg_b < 0)) for t in topos) > 100
-    gx, gv = K.gradient(tp, topos, base, v0, idx, shift, dt, kick, wx, wv)
+    gx, gv = K.gradient(tp, topos, base, v0, idx, shift, dt, kick, wx, wv, scheme)
     coords = list(zip(rng.integers(0, base.shape[0], 24).tolist(), rng.integers(0, 3, 24).tolist()))
-    fd = K.fd_gradient(tp, topos, base, v0, idx, shift, dt, kick, wx, wv, coords)
+    fd = K.fd_gradient(tp, topos, base, v0, idx, shift, dt, kick, wx, wv, coords, scheme=scheme)
     w = np.array([wx[i, a] for i, a in coords])
     ref = np.array([gx[i, a] for i, a in coords]) - w
     err = np.sqrt(np.sum((fd - w - ref) ** 2) / np.sum(ref ** 2))
