@@ -25,8 +25,7 @@
 //   two-pass      target pass as above without W; source pass per source leaf
 //                 over the transposed lists (all target particles whose lists
 //                 hold the leaf); pseudo pass per (target leaf, pseudo
-//                 segment).  Used when a leaf holds more than 256 particles or
-//                 W does not fit in device memory.
+//                 segment).  Used when W does not fit in device memory.
 #include "hsr_internal.h"
 #include "wave.h"
 
@@ -56,7 +55,16 @@ struct AdjLaw {
   float dp[6];  // 2 * k * c_k: 2 P'(s) coefficients (dp[k-1] for s^(k-1))
   float rcs, eps2;
   int order;
+  int cull;     // drop entries farther than the cutoff from the target leaf's box
+  float cull2;  // squared box distance beyond which every pair is masked
 };
+
+__device__ __forceinline__ float box_d2(const hsr_node &nd, float x, float y, float z) {
+  const float dx = fmaxf(0.f, fmaxf(nd.xmin[0] - x, x - nd.xmax[0]));
+  const float dy = fmaxf(0.f, fmaxf(nd.xmin[1] - y, y - nd.xmax[1]));
+  const float dz = fmaxf(0.f, fmaxf(nd.xmin[2] - z, z - nd.xmax[2]));
+  return dx * dx + dy * dy + dz * dz;
+}
 
 AdjLaw adj_law(const ForceLawDev &D) {
   AdjLaw A;
@@ -75,6 +83,8 @@ AdjLaw adj_law(const ForceLawDev &D) {
   A.rcs = (float)((double)D.rc2 + e);
   A.eps2 = D.eps2;
   A.order = D.order;
+  A.cull = 1;
+  A.cull2 = (float)((double)D.rc2 * (1.0 + 1e-4));
   return A;
 }
 
@@ -316,10 +326,11 @@ k_adj_reduce(const hsr_node *cn, const AItem *items, const int64_t *first,
 
 // ------------------------------------------------------------- fused pass
 constexpr int FR = 2;                       // targets per lane
-constexpr int FMAXP = 2;                    // passes over a leaf's targets
-constexpr int FMAX_LEAF = 64 * FR * FMAXP;  // largest leaf of the fused schedule
 constexpr int FTILE = 256;                  // entries per LDS tile
-constexpr int FUNROLL = 2;
+#ifndef HSR_FUSED_UNROLL
+#define HSR_FUSED_UNROLL 2
+#endif
+constexpr int FUNROLL = HSR_FUSED_UNROLL;  // entries per step of a slice
 
 template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ float dpp_add(float v) {
@@ -327,37 +338,103 @@ __device__ __forceinline__ float dpp_add(float v) {
   return v + __builtin_bit_cast(float, t);
 }
 
-// Sum over aligned groups of lpg lanes (lpg a power of two, 4..64); the sum
-// lands in the group's last lane.  Needs all 64 lanes active.
+// Sum over aligned groups of LP lanes (a power of two, 4..64); the sum lands
+// in the group's last lane.  Needs all 64 lanes active.
 //   row_shr:1,2,4,8 sum within 16-lane rows; row_bcast:15 adds row 0 (2) into
 //   row 1 (3); row_bcast:31 adds lane 31 into rows 2-3.
-__device__ __forceinline__ float group_sum_last(float v, int lpg) {
+template <int LP>
+__device__ __forceinline__ float group_sum_last(float v) {
   v = dpp_add<0x111, 0xf>(v);
   v = dpp_add<0x112, 0xf>(v);
-  if (lpg >= 8) v = dpp_add<0x114, 0xf>(v);
-  if (lpg >= 16) v = dpp_add<0x118, 0xf>(v);
-  if (lpg >= 32) v = dpp_add<0x142, 0xa>(v);
-  if (lpg >= 64) v = dpp_add<0x143, 0xc>(v);
+  if (LP >= 8) v = dpp_add<0x114, 0xf>(v);
+  if (LP >= 16) v = dpp_add<0x118, 0xf>(v);
+  if (LP >= 32) v = dpp_add<0x142, 0xa>(v);
+  if (LP >= 64) v = dpp_add<0x143, 0xc>(v);
   return v;
+}
+
+// A group's slice of the staged tile: every pair of its FR targets per lane
+// with each entry, the target sums B in registers, and the entry's source sum
+// W (over the group's lpg = LP lanes) written back into its tile slot.
+template <int ORDER, int LP>
+__device__ __forceinline__ void fused_slice(const AdjLaw &L, float4 *pa, int tg, bool last,
+                                            const float (&xi)[FR], const float (&yi)[FR],
+                                            const float (&zi)[FR], const float (&la)[FR],
+                                            const float (&lb)[FR], const float (&lc)[FR],
+                                            float (&am)[FR], float (&bx)[FR], float (&by)[FR],
+                                            float (&bz)[FR]) {
+  for (int k = 0; k < tg; k += FUNROLL) {
+    float4 a[FUNROLL];
+#pragma unroll
+    for (int u = 0; u < FUNROLL; ++u) a[u] = pa[k + u];
+    float vx[FUNROLL], vy[FUNROLL], vz[FUNROLL];
+#pragma unroll
+    for (int u = 0; u < FUNROLL; ++u) {
+      vx[u] = vy[u] = vz[u] = 0.f;
+#pragma unroll
+      for (int r = 0; r < FR; ++r) {
+        const float dx = a[u].x - xi[r];
+        const float dy = a[u].y - yi[r];
+        const float dz = a[u].z - zi[r];
+        float s = fmaf(dx, dx, L.eps2);
+        s = fmaf(dy, dy, s);
+        s = fmaf(dz, dz, s);
+        float gg, g2p;
+        pair_g<ORDER>(L, s, gg, g2p);
+        const bool in = s < L.rcs;
+        float dl = dx * la[r];
+        dl = fmaf(dy, lb[r], dl);
+        dl = fmaf(dz, lc[r], dl);
+        const float gm = in ? gg : 0.f;
+        const float cc = in ? g2p * dl : 0.f;
+        vx[u] = fmaf(gm, la[r], vx[u]);
+        vy[u] = fmaf(gm, lb[r], vy[u]);
+        vz[u] = fmaf(gm, lc[r], vz[u]);
+        vx[u] = fmaf(cc, dx, vx[u]);
+        vy[u] = fmaf(cc, dy, vy[u]);
+        vz[u] = fmaf(cc, dz, vz[u]);
+        // target: m_j V = (m_j g) lam_i + (m_j cc) d
+        am[r] = fmaf(a[u].w, gm, am[r]);
+        const float mc = a[u].w * cc;
+        bx[r] = fmaf(mc, dx, bx[r]);
+        by[r] = fmaf(mc, dy, by[r]);
+        bz[r] = fmaf(mc, dz, bz[r]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < FUNROLL; ++u) {
+      vx[u] = group_sum_last<LP>(vx[u]);
+      vy[u] = group_sum_last<LP>(vy[u]);
+      vz[u] = group_sum_last<LP>(vz[u]);
+    }
+    // each slot was read above by every lane; its group's sum replaces it
+    if (last) {
+#pragma unroll
+      for (int u = 0; u < FUNROLL; ++u) pa[k + u] = make_float4(vx[u], vy[u], vz[u], 0.f);
+    }
+  }
 }
 
 // One pair evaluation for both sides.  Lanes hold targets i (x_i, lambda_i)
 // in groups of lpg lanes (FR per lane); each group takes a slice of the
-// staged entries j (x_j, m_j).  Per pair, with V = g lam_i + 2g'(d.lam_i) d:
-//   target   A_i += m_j g,  B_i += m_j 2g'(d.lam_i) d;  dJ/dx_i = -(A_i lam_i + B_i)
-//   source   W_j += V (summed over the group's lanes, then over passes)
+// staged entries j (x_j, m_j).  Per pair, V = g lam_i + 2g'(d.lam_i) d:
+//   target   A_i += m_j g, B_i += m_j 2g'(d.lam_i) d;  dJ/dx_i = -(A_i lam_i + B_i)
+//   source   W_j += V, summed over the group's lanes (DPP) into the tile slot
+//            the entry was staged in, then over the passes in W itself.
+// Passes over a leaf's targets run one after the other (pass p > 0 adds to W;
+// the barrier between passes orders those global writes within the group).
 template <int ORDER>
-__global__ void __launch_bounds__(64)
-k_adj_fused(const hsr_node *__restrict__ cn, const AItem *__restrict__ items,
-            const int64_t *__restrict__ nitems_p, const int32_t *__restrict__ jidx,
-            const PJ *__restrict__ pj, const float *__restrict__ lx,
-            const float *__restrict__ ly, const float *__restrict__ lz, AdjLaw L,
-            unsigned long long *__restrict__ qctr, float *__restrict__ gx,
-            float *__restrict__ gy, float *__restrict__ gz, float *__restrict__ part,
-            float *__restrict__ wx, float *__restrict__ wy, float *__restrict__ wz) {
+__device__ __forceinline__ void
+adj_fused_body(const hsr_node *__restrict__ cn, const AItem *__restrict__ items,
+               const int64_t *__restrict__ nitems_p, const int32_t *__restrict__ jidx,
+               const PJ *__restrict__ pj, const float *__restrict__ lx,
+               const float *__restrict__ ly, const float *__restrict__ lz, const AdjLaw &L,
+               unsigned long long *__restrict__ qctr, float *__restrict__ gx,
+               float *__restrict__ gy, float *__restrict__ gz, float *__restrict__ part,
+               float *__restrict__ wx, float *__restrict__ wy, float *__restrict__ wz) {
   __shared__ float4 tile[LDS_ENT];
-  __shared__ float jacc[3][FTILE];
-  float *red = reinterpret_cast<float *>(&tile[0]);  // [4][64 * FR] after the tiles
+  __shared__ int16_t qmap[FTILE];
+  float (*red)[64 * FR] = reinterpret_cast<float (*)[64 * FR]>(&tile[0]);
   const int l = lane_id();
   const int64_t nitems = *nitems_p;
   for (;;) {
@@ -367,131 +444,95 @@ k_adj_fused(const hsr_node *__restrict__ cn, const AItem *__restrict__ items,
     if (it >= nitems) break;
     const AItem I = items[it];
     const hsr_node &nd = cn[I.leaf];
-    const int64_t off = nd.offset;
-    const int cnt = (int)nd.count;
+    const int64_t off = nd.offset, cnt = nd.count;
     if (cnt == 0) {  // an empty leaf contributes nothing to its entries
       for (int64_t e = I.e0 + l; e < I.e1; e += 64) wx[e] = wy[e] = wz[e] = 0.f;
       continue;
     }
-    const int np = (cnt + 64 * FR - 1) / (64 * FR);
     const float4 padJ = make_float4(nd.xc[0] + 1000.f, nd.xc[1], nd.xc[2], 0.f);
     const float padI = nd.xc[0] + 3000.f;
-    int lpg[FMAXP];
-    float xi[FMAXP][FR], yi[FMAXP][FR], zi[FMAXP][FR];
-    float la[FMAXP][FR], lb[FMAXP][FR], lc[FMAXP][FR];
-    float A[FMAXP][FR], Bx[FMAXP][FR], By[FMAXP][FR], Bz[FMAXP][FR];
-#pragma unroll
-    for (int p = 0; p < FMAXP; ++p) {
-      const int ni = min(64 * FR, cnt - p * 64 * FR);
+    for (int64_t ib = 0; ib < cnt; ib += 64 * FR) {
+      const bool first = ib == 0;
+      const int ni = (int)min<int64_t>(64 * FR, cnt - ib);
       const int need = max((ni + FR - 1) / FR, 4);
       int lp = 4;
       while (lp < need) lp <<= 1;
-      lpg[p] = lp;
-      const int tq = l & (lp - 1);
+      const int G = 64 / lp;
+      const int g = l / lp, tq = l & (lp - 1);
+      float xi[FR], yi[FR], zi[FR], la[FR], lb[FR], lc[FR], am[FR], bx[FR], by[FR], bz[FR];
 #pragma unroll
       for (int r = 0; r < FR; ++r) {
         const int ii = tq + r * lp;
         const bool v = ii < ni;
-        const int64_t q = off + p * 64 * FR + (v ? ii : 0);
-        const PJ P = pj[v ? q : off];
-        xi[p][r] = v ? P.x : padI;
-        yi[p][r] = v ? P.y : nd.xc[1];
-        zi[p][r] = v ? P.z : nd.xc[2];
-        la[p][r] = v ? lx[q] : 0.f;
-        lb[p][r] = v ? ly[q] : 0.f;
-        lc[p][r] = v ? lz[q] : 0.f;
-        A[p][r] = Bx[p][r] = By[p][r] = Bz[p][r] = 0.f;
+        const int64_t q = off + ib + (v ? ii : 0);
+        const PJ P = pj[q];
+        xi[r] = v ? P.x : padI;
+        yi[r] = v ? P.y : nd.xc[1];
+        zi[r] = v ? P.z : nd.xc[2];
+        la[r] = v ? lx[q] : 0.f;
+        lb[r] = v ? ly[q] : 0.f;
+        lc[r] = v ? lz[q] : 0.f;
+        am[r] = bx[r] = by[r] = bz[r] = 0.f;
       }
-    }
-    for (int64_t t0 = I.e0; t0 < I.e1; t0 += FTILE) {
-      const int nt = (int)min<int64_t>(FTILE, I.e1 - t0);
-#pragma unroll
-      for (int p = 0; p < FMAXP; ++p) {
-        if (p >= np) break;
-        const int lp = lpg[p], G = 64 / lp;
-        const int g = l / lp, tq = l & (lp - 1);
-        int tg = (nt + G - 1) / G;
-        tg = (tg + FUNROLL - 1) / FUNROLL * FUNROLL;
-        const int stride = tg + ((9 - (tg & 7)) & 7);
+      for (int64_t t0 = I.e0; t0 < I.e1; t0 += FTILE) {
+        const int nt = (int)min<int64_t>(FTILE, I.e1 - t0);
+        int tgmax = (nt + G - 1) / G;
+        tgmax = (tgmax + FUNROLL - 1) / FUNROLL * FUNROLL;
+        const int stride = tgmax + ((9 - (tgmax & 7)) & 7);
         __syncthreads();
-        for (int q = l; q < G * tg; q += 64) {
-          const int gq = q / tg, kq = q - gq * tg;
+        // entries beyond the cutoff of the whole leaf have W = 0 and are
+        // dropped; the kept ones go round-robin to the groups (c -> group
+        // c % G, slot c / G), qmap remembering their place in the tile
+        int nk = 0;
+        for (int q0 = 0; q0 < nt; q0 += 64) {
+          const int q = q0 + l;
           float4 a = padJ;
+          bool keep = false;
           if (q < nt) {
             const PJ P = pj[jidx[t0 + q]];
             a = make_float4(P.x, P.y, P.z, P.m);
+            keep = !L.cull || box_d2(nd, P.x, P.y, P.z) <= L.cull2;
+            if (!keep && first) wx[t0 + q] = wy[t0 + q] = wz[t0 + q] = 0.f;
           }
-          tile[gq * stride + kq] = a;
+          const uint64_t bal = __ballot(keep);
+          if (keep) {
+            const int c = nk + __popcll(bal & lanemask_lt());
+            tile[(c % G) * stride + c / G] = a;
+            qmap[c] = (int16_t)q;
+          }
+          nk += __popcll(bal);
+        }
+        int tg = (nk + G - 1) / G;
+        tg = (tg + FUNROLL - 1) / FUNROLL * FUNROLL;
+        for (int c = nk + l; c < G * tg; c += 64) tile[(c % G) * stride + c / G] = padJ;
+        __syncthreads();
+        float4 *pa = tile + g * stride;
+        const bool last = tq == lp - 1;
+        switch (lp) {  // the group width fixes the DPP sequence
+          case 4: fused_slice<ORDER, 4>(L, pa, tg, last, xi, yi, zi, la, lb, lc, am, bx, by, bz); break;
+          case 8: fused_slice<ORDER, 8>(L, pa, tg, last, xi, yi, zi, la, lb, lc, am, bx, by, bz); break;
+          case 16: fused_slice<ORDER, 16>(L, pa, tg, last, xi, yi, zi, la, lb, lc, am, bx, by, bz); break;
+          case 32: fused_slice<ORDER, 32>(L, pa, tg, last, xi, yi, zi, la, lb, lc, am, bx, by, bz); break;
+          default: fused_slice<ORDER, 64>(L, pa, tg, last, xi, yi, zi, la, lb, lc, am, bx, by, bz); break;
         }
         __syncthreads();
-        const float4 *pa = tile + g * stride;
-        for (int k = 0; k < tg; k += FUNROLL) {
-#pragma unroll
-          for (int u = 0; u < FUNROLL; ++u) {
-            const float4 a = pa[k + u];
-            float vx = 0.f, vy = 0.f, vz = 0.f;
-#pragma unroll
-            for (int r = 0; r < FR; ++r) {
-              const float dx = a.x - xi[p][r];
-              const float dy = a.y - yi[p][r];
-              const float dz = a.z - zi[p][r];
-              float s = fmaf(dx, dx, L.eps2);
-              s = fmaf(dy, dy, s);
-              s = fmaf(dz, dz, s);
-              float gg, g2p;
-              pair_g<ORDER>(L, s, gg, g2p);
-              const bool in = s < L.rcs;
-              float dl = dx * la[p][r];
-              dl = fmaf(dy, lb[p][r], dl);
-              dl = fmaf(dz, lc[p][r], dl);
-              const float gm = in ? gg : 0.f;
-              const float cc = in ? g2p * dl : 0.f;
-              vx = fmaf(gm, la[p][r], vx);
-              vy = fmaf(gm, lb[p][r], vy);
-              vz = fmaf(gm, lc[p][r], vz);
-              vx = fmaf(cc, dx, vx);
-              vy = fmaf(cc, dy, vy);
-              vz = fmaf(cc, dz, vz);
-              A[p][r] = fmaf(a.w, gm, A[p][r]);
-              const float mc = a.w * cc;
-              Bx[p][r] = fmaf(mc, dx, Bx[p][r]);
-              By[p][r] = fmaf(mc, dy, By[p][r]);
-              Bz[p][r] = fmaf(mc, dz, Bz[p][r]);
-            }
-            vx = group_sum_last(vx, lp);
-            vy = group_sum_last(vy, lp);
-            vz = group_sum_last(vz, lp);
-            const int e = g * tg + k + u;
-            if (tq == lp - 1 && e < nt) {
-              if (p == 0) {
-                jacc[0][e] = vx;
-                jacc[1][e] = vy;
-                jacc[2][e] = vz;
-              } else {
-                jacc[0][e] += vx;
-                jacc[1][e] += vy;
-                jacc[2][e] += vz;
-              }
-            }
+        for (int c = l; c < nk; c += 64) {
+          const float4 w = tile[(c % G) * stride + c / G];
+          const int64_t e = t0 + qmap[c];
+          if (first) {
+            wx[e] = w.x;
+            wy[e] = w.y;
+            wz[e] = w.z;
+          } else {
+            wx[e] += w.x;
+            wy[e] += w.y;
+            wz[e] += w.z;
           }
         }
       }
+      // target side: sum the groups in order
       __syncthreads();
-      for (int q = l; q < nt; q += 64) {
-        wx[t0 + q] = jacc[0][q];
-        wy[t0 + q] = jacc[1][q];
-        wz[t0 + q] = jacc[2][q];
-      }
-    }
-    // target side: sum the groups in order, then -(A lam + B)
-#pragma unroll
-    for (int p = 0; p < FMAXP; ++p) {
-      if (p >= np) break;
-      const int lp = lpg[p], G = 64 / lp;
-      const int g = l / lp, tq = l & (lp - 1);
-      const int ni = min(64 * FR, cnt - p * 64 * FR);
-      __syncthreads();
-      for (int q = l; q < 4 * 64 * FR; q += 64) red[q] = 0.f;
+      for (int q = l; q < 64 * FR; q += 64) red[0][q] = red[1][q] = red[2][q] = red[3][q] = 0.f;
       __syncthreads();
       for (int gg = 0; gg < G; ++gg) {
         if (g == gg) {
@@ -499,27 +540,27 @@ k_adj_fused(const hsr_node *__restrict__ cn, const AItem *__restrict__ items,
           for (int r = 0; r < FR; ++r) {
             const int ii = tq + r * lp;
             if (ii < ni) {
-              red[ii] += A[p][r];
-              red[64 * FR + ii] += Bx[p][r];
-              red[2 * 64 * FR + ii] += By[p][r];
-              red[3 * 64 * FR + ii] += Bz[p][r];
+              red[0][ii] += bx[r];
+              red[1][ii] += by[r];
+              red[2][ii] += bz[r];
+              red[3][ii] += am[r];
             }
           }
         }
         __syncthreads();
       }
       for (int ii = l; ii < ni; ii += 64) {
-        const int64_t q = off + p * 64 * FR + ii;
-        const float a = red[ii];
-        const float vx = -fmaf(a, lx[q], red[64 * FR + ii]);
-        const float vy = -fmaf(a, ly[q], red[2 * 64 * FR + ii]);
-        const float vz = -fmaf(a, lz[q], red[3 * 64 * FR + ii]);
+        const int64_t q = off + ib + ii;
+        const float a = red[3][ii];
+        const float vx = -fmaf(a, lx[q], red[0][ii]);
+        const float vy = -fmaf(a, ly[q], red[1][ii]);
+        const float vz = -fmaf(a, lz[q], red[2][ii]);
         if (I.pbase < 0) {
           gx[q] = vx;
           gy[q] = vy;
           gz[q] = vz;
         } else {
-          float *pp = part + I.pbase + 3 * (p * 64 * FR + ii);
+          float *pp = part + I.pbase + 3 * (ib + ii);
           pp[0] = vx;
           pp[1] = vy;
           pp[2] = vz;
@@ -527,6 +568,33 @@ k_adj_fused(const hsr_node *__restrict__ cn, const AItem *__restrict__ items,
       }
     }
   }
+}
+
+// Waves per SIMD the register allocation of the 5th-order body targets (0:
+// the compiler's choice).
+#ifndef HSR_ADJ_WAVES
+#define HSR_ADJ_WAVES 0
+#endif
+#if HSR_ADJ_WAVES > 0
+#define HSR_ADJ_ATTR __attribute__((amdgpu_waves_per_eu(HSR_ADJ_WAVES, HSR_ADJ_WAVES)))
+#else
+#define HSR_ADJ_ATTR
+#endif
+#define HSR_ADJ_FUSED_ARGS                                                             \
+  const hsr_node *__restrict__ cn, const AItem *__restrict__ items,                    \
+      const int64_t *__restrict__ nitems_p, const int32_t *__restrict__ jidx,          \
+      const PJ *__restrict__ pj, const float *__restrict__ lx,                         \
+      const float *__restrict__ ly, const float *__restrict__ lz, AdjLaw L,            \
+      unsigned long long *__restrict__ qctr, float *__restrict__ gx,                   \
+      float *__restrict__ gy, float *__restrict__ gz, float *__restrict__ part,        \
+      float *__restrict__ wx, float *__restrict__ wy, float *__restrict__ wz
+__global__ void __launch_bounds__(64) HSR_ADJ_ATTR
+k_adj_fused5(HSR_ADJ_FUSED_ARGS) {
+  adj_fused_body<5>(cn, items, nitems_p, jidx, pj, lx, ly, lz, L, qctr, gx, gy, gz, part, wx, wy, wz);
+}
+__global__ void __launch_bounds__(64)
+k_adj_fused6(HSR_ADJ_FUSED_ARGS) {
+  adj_fused_body<6>(cn, items, nitems_p, jidx, pj, lx, ly, lz, L, qctr, gx, gy, gz, part, wx, wy, wz);
 }
 
 // Fused schedule, source side: one wave per source leaf (a run of sorted
@@ -573,13 +641,6 @@ __global__ void k_pseudo_from_w(int64_t np, int64_t p0, const int32_t *val,
   gseg[3 * q + 2] = wz[e];
 }
 
-// Largest listed leaf.
-__global__ void k_max_leaf(int64_t nl, const int32_t *leaves, const hsr_node *cn,
-                           unsigned long long *out) {
-  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (k >= nl) return;
-  atomicMax(out, (unsigned long long)cn[leaves[k]].count);
-}
 
 // ---------------------------------------------------------- transposition
 __global__ void k_seg_target(int64_t nl, const int64_t *seg_ptr,
@@ -971,26 +1032,16 @@ hsr_status adjoint_plan(hsr_ctx *c) {
   A.key = key2;
   A.val = val2;
   A.seg_t = seg_t;
-  // ---- schedule: fused needs every listed leaf within FMAX_LEAF and room for W
+  // ---- schedule: fused unless W (12 B per list entry) does not fit
   {
-    HSR_HIP(dbuf_reserve(A.counts, sizeof(int64_t) * 4));
-    unsigned long long *mx = reinterpret_cast<unsigned long long *>(dptr<int64_t>(A.counts) + 2);
-    k_zero_u64a<<<1, 64, 0, st>>>(mx);
-    if (nl) k_max_leaf<<<nblk(nl, 256), 256, 0, st>>>(nl, leaves, cn, mx);
-    HSR_HIP(hipMemcpyAsync(&tail_items[0], mx, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    HSR_HIP(hipStreamSynchronize(st));
-    const int64_t max_leaf = tail_items[0];
     const size_t wbytes = sizeof(float) * 3 * (size_t)(LS.nentries + 1);
     size_t free_b = 0, total_b = 0;
     HSR_HIP(hipMemGetInfo(&free_b, &total_b));
     const bool fits = wbytes <= A.W.bytes || (double)wbytes <= 0.7 * (double)free_b;
     const int mode = c->tuning.adjoint_mode;
-    if (mode == 1 && max_leaf > FMAX_LEAF) {
-      hsr_set_error("hsr_adjoint: the fused schedule takes leaves of at most 256 particles");
-      return HSR_ERR_UNSUPPORTED;
-    }
-    A.fused = mode == 1 || (mode == 0 && max_leaf <= FMAX_LEAF && fits);
+    A.fused = mode == 1 || (mode == 0 && fits);
     if (A.fused) HSR_HIP(dbuf_reserve(A.W, wbytes));
+    HSR_HIP(dbuf_reserve(A.counts, sizeof(int64_t) * 4));
   }
   if (!A.fused) {
     k_tr_counts<<<nblk(ns, 256), 256, 0, st>>>(ns, nn, key2, val2, seg_t, cn, leaves, tcnt, start);
@@ -1091,7 +1142,8 @@ hsr_status adjoint_run(hsr_ctx *c, const float *lx, const float *ly,
   const int64_t n = c->tree.n, nn = c->tree.nnodes;
   const hsr_node *cn = dptr<hsr_node>(c->tree.canon_nodes);
   const PJ *pj = dptr<PJ>(c->tree.pj);
-  const AdjLaw L = adj_law(c->law);
+  AdjLaw L = adj_law(c->law);
+  L.cull = c->tuning.cull_off ? 0 : 1;
   if (c->law.guard) {
     hsr_set_error("hsr_adjoint: needs rsm > 0");
     return HSR_ERR_UNSUPPORTED;
@@ -1122,10 +1174,10 @@ hsr_status adjoint_run(hsr_ctx *c, const float *lx, const float *ly,
       float *W = dptr<float>(A.W);
       const int64_t ne = c->lists.nentries + 1;
       if (L.order == 6)
-        k_adj_fused<6><<<(unsigned)grid, 64, 0, st>>>(cn, dptr<AItem>(A.fw_items), counts,
+        k_adj_fused6<<<(unsigned)grid, 64, 0, st>>>(cn, dptr<AItem>(A.fw_items), counts,
             dptr<int32_t>(c->lists.jidx), pj, lx, ly, lz, L, q, gx, gy, gz, part, W, W + ne, W + 2 * ne);
       else
-        k_adj_fused<5><<<(unsigned)grid, 64, 0, st>>>(cn, dptr<AItem>(A.fw_items), counts,
+        k_adj_fused5<<<(unsigned)grid, 64, 0, st>>>(cn, dptr<AItem>(A.fw_items), counts,
             dptr<int32_t>(c->lists.jidx), pj, lx, ly, lz, L, q, gx, gy, gz, part, W, W + ne, W + 2 * ne);
     } else if (L.order == 6) {
       LA(0, 6, dptr<AItem>(A.fw_items), counts);

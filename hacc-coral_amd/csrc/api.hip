@@ -408,6 +408,14 @@ hsr_status hsr_lists_many(hsr_ctx *c, const float *min_force,
   return HSR_OK;
 }
 
+// Pairs the last force phase evaluated (its kernel's counter, qctr[1]).
+static hipError_t read_evaluated(hsr_ctx *c) {
+  c->evaluated_pairs = 0;
+  if (c->lists.nleaves == 0 || c->lists.nitems == 0 || !c->s.qctr.p) return hipSuccess;
+  return hipMemcpyAsync(&c->evaluated_pairs, dptr<unsigned long long>(c->s.qctr) + 1,
+                        sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
+}
+
 hsr_status hsr_forces(hsr_ctx *c, float *ax, float *ay, float *az,
                       int ptr_flags) {
   hsr_status s = check_ctx(c);
@@ -432,6 +440,7 @@ hsr_status hsr_forces(hsr_ctx *c, float *ax, float *ay, float *az,
   s = force_run(c, fx, fy, fz);
   if (s) return s;
   HSR_HIP(hipEventRecord(c->ev[5], c->stream));
+  HSR_HIP(read_evaluated(c));
   if (!(ptr_flags & HSR_DEVICE_PTRS) && n) {
     HSR_HIP(hipMemcpyAsync(ax, fx, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
     HSR_HIP(hipMemcpyAsync(ay, fy, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
@@ -465,6 +474,7 @@ hsr_status hsr_kick(hsr_ctx *c, float fcoeff) {
   s = kick_run(c, fcoeff, c->user_dv[0], c->user_dv[1], c->user_dv[2], dptr<float>(c->tree.ms));
   if (s) return s;
   HSR_HIP(hipEventRecord(c->ev[5], c->stream));
+  HSR_HIP(read_evaluated(c));
   if (!(c->user_flags & HSR_DEVICE_PTRS) && n) {
     for (int k = 0; k < 3; ++k)
       HSR_HIP(hipMemcpyAsync(c->user_v[k], c->user_dv[k], sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
@@ -622,6 +632,7 @@ hsr_status hsr_get_stats(hsr_ctx *c, hsr_stats *st) {
   st->segments = c->lists.nsegs;
   st->entries = c->lists.nentries;
   st->pairs = c->lists.pairs;
+  st->evaluated_pairs = c->evaluated_pairs;
   st->build_ms = c->build_ms;
   st->lists_ms = c->lists_ms;
   st->force_ms = c->force_ms;
@@ -707,6 +718,7 @@ hsr_status hsr_set_tuning(hsr_ctx *c, const hsr_tuning *t) {
     return HSR_ERR_ARG;
   }
   c->tuning = *t;
+  c->graph_gen = -1;  // a captured force phase holds the old settings
   return HSR_OK;
 }
 

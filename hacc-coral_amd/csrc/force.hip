@@ -47,7 +47,17 @@ struct LawArgs {
   float rcs;       // s cutoff: pairs with s < rcs contribute
   float eps2;
   int guard;
+  int cull;        // drop entries farther than the cutoff from the leaf's box
+  float cull2;     // squared box distance beyond which every pair is masked
 };
+
+// Squared distance from p to the box [lo, hi] (0 inside).
+__device__ __forceinline__ float box_d2(const hsr_node &nd, float x, float y, float z) {
+  const float dx = fmaxf(0.f, fmaxf(nd.xmin[0] - x, x - nd.xmax[0]));
+  const float dy = fmaxf(0.f, fmaxf(nd.xmin[1] - y, y - nd.xmax[1]));
+  const float dz = fmaxf(0.f, fmaxf(nd.xmin[2] - z, z - nd.xmax[2]));
+  return dx * dx + dy * dy + dz * dz;
+}
 
 template <int ORDER>
 __device__ __forceinline__ float poly_tail(const LawArgs &L, float s) {
@@ -66,13 +76,25 @@ __device__ __forceinline__ float poly_tail(const LawArgs &L, float s) {
 }
 
 template <int ORDER, bool GUARD>
-__global__ void __launch_bounds__(64)
+// Waves per SIMD the register allocation of the leaf kernel targets (0: the
+// compiler's choice).
+#ifndef HSR_FORCE_WAVES
+#define HSR_FORCE_WAVES 0
+#endif
+#if HSR_FORCE_WAVES > 0
+#define HSR_FORCE_ATTR __attribute__((amdgpu_waves_per_eu(HSR_FORCE_WAVES, HSR_FORCE_WAVES)))
+#else
+#define HSR_FORCE_ATTR
+#endif
+
+__global__ void __launch_bounds__(64) HSR_FORCE_ATTR
 k_force(const hsr_node *__restrict__ cn, const int32_t *__restrict__ leaves,
         const Item *__restrict__ items, const int64_t *__restrict__ nitems_p,
         const int32_t *__restrict__ jidx, const PJ *__restrict__ pj,
         LawArgs L, unsigned long long *__restrict__ qctr,
         float *__restrict__ fx, float *__restrict__ fy,
-        float *__restrict__ fz, float *__restrict__ part) {
+        float *__restrict__ fz, float *__restrict__ part,
+        unsigned long long *__restrict__ evals) {
   // the tile doubles as the reduction buffer once the j-walk is done
   __shared__ float4 tile[LDS_ENT];
   static_assert(3 * 64 * R <= 4 * LDS_ENT, "reduction buffer fits the tile");
@@ -91,6 +113,7 @@ k_force(const hsr_node *__restrict__ cn, const int32_t *__restrict__ leaves,
     // padding entries: far enough to be masked, near enough to keep every
     // intermediate finite
     const float4 pad = make_float4(nd.xc[0] + 1000.f, nd.xc[1], nd.xc[2], 0.f);
+    int64_t ev = 0;  // pairs evaluated for this item (culled entries excluded)
     for (int64_t ib = 0; ib < cnt; ib += 64 * R) {
       const int ni = (int)min<int64_t>(64 * R, cnt - ib);
       const int lpg = max((ni + R - 1) / R, 64 / GMAX);
@@ -110,20 +133,35 @@ k_force(const hsr_node *__restrict__ cn, const int32_t *__restrict__ leaves,
       }
       for (int64_t t0 = I.e0; t0 < I.e1; t0 += TILE) {
         const int nt = (int)min<int64_t>(TILE, I.e1 - t0);
-        // per-group slice: tg entries (multiple of UNROLL), stride = 1 mod 8
-        int tg = (nt + G - 1) / G;
-        tg = (tg + UNROLL - 1) / UNROLL * UNROLL;
-        const int stride = tg + ((9 - (tg & 7)) & 7);
+        // per-group slices of at most tgmax entries, stride = 1 mod 8
+        int tgmax = (nt + G - 1) / G;
+        tgmax = (tgmax + UNROLL - 1) / UNROLL * UNROLL;
+        const int stride = tgmax + ((9 - (tgmax & 7)) & 7);
         __syncthreads();
-        for (int q = l; q < G * tg; q += 64) {
-          const int gq = q / tg, kq = q - gq * tg;
+        // Entries farther than the cutoff from the leaf's box contribute 0 to
+        // every target and are dropped; the kept ones, in list order, go
+        // round-robin to the groups (entry c -> group c % G, slot c / G).
+        int nk = 0;
+        for (int q0 = 0; q0 < nt; q0 += 64) {
+          const int q = q0 + l;
           float4 v = pad;
+          bool keep = false;
           if (q < nt) {
             const PJ p = pj[jidx[t0 + q]];
             v = make_float4(p.x, p.y, p.z, p.m);
+            keep = !L.cull || box_d2(nd, p.x, p.y, p.z) <= L.cull2;
           }
-          tile[gq * stride + kq] = v;
+          const uint64_t bal = __ballot(keep);
+          if (keep) {
+            const int c = nk + __popcll(bal & lanemask_lt());
+            tile[(c % G) * stride + c / G] = v;
+          }
+          nk += __popcll(bal);
         }
+        int tg = (nk + G - 1) / G;
+        tg = (tg + UNROLL - 1) / UNROLL * UNROLL;
+        for (int c = nk + l; c < G * tg; c += 64) tile[(c % G) * stride + c / G] = pad;
+        ev += (int64_t)nk * ni;
         __syncthreads();
         if (active) {
           const float4 *tp = tile + g * stride;
@@ -188,6 +226,7 @@ k_force(const hsr_node *__restrict__ cn, const int32_t *__restrict__ leaves,
         }
       }
     }
+    if (l == 0 && ev > 0) atomicAdd(evals, (unsigned long long)ev);
   }
 }
 
@@ -328,6 +367,10 @@ LawArgs law_args(const ForceLawDev &D) {
   A.rcs = (float)((double)D.rc2 + e);
   A.eps2 = D.eps2;
   A.guard = D.guard;
+  // a relative margin far above the few-ulp rounding of r2 keeps every pair
+  // the kernel's own s < rcs test would accept
+  A.cull = 1;
+  A.cull2 = (float)((double)D.rc2 * (1.0 + 1e-4));
   return A;
 }
 
@@ -462,7 +505,9 @@ static hsr_status force_enqueue(hsr_ctx *c, float *fx, float *fy, float *fz,
   if (LS.nleaves == 0 || LS.nitems == 0) return HSR_OK;
   unsigned long long *qctr = dptr<unsigned long long>(c->s.qctr);
   k_zero_u64<<<1, 64, 0, st>>>(qctr);
-  const LawArgs A = law_args(c->law);
+  k_zero_u64<<<1, 64, 0, st>>>(qctr + 1);
+  LawArgs A = law_args(c->law);
+  A.cull = c->tuning.cull_off ? 0 : 1;
   const int wpc = c->tuning.waves_per_cu > 0 ? c->tuning.waves_per_cu : 32;
   const int64_t grid = std::min<int64_t>(LS.nitems, (int64_t)std::max(1, c->num_cus) * wpc);
   const hsr_node *cn = dptr<hsr_node>(c->tree.canon_nodes);
@@ -473,7 +518,7 @@ static hsr_status force_enqueue(hsr_ctx *c, float *fx, float *fy, float *fz,
   if (timed) HSR_HIP(hipEventRecord(c->ev[6], st));
 #define LAUNCH(ORD, GRD)                                                          \
   k_force<ORD, GRD><<<(unsigned)grid, 64, 0, st>>>(cn, leaves, items, LS.counts, \
-      dptr<int32_t>(LS.jidx), dptr<PJ>(c->tree.pj), A, qctr, fx, fy, fz, part)
+      dptr<int32_t>(LS.jidx), dptr<PJ>(c->tree.pj), A, qctr, fx, fy, fz, part, qctr + 1)
   if (c->law.order == 6) {
     if (A.guard) LAUNCH(6, true); else LAUNCH(6, false);
   } else {

@@ -167,6 +167,7 @@ struct hsr_ctx {
   bool kernel_timed = false;
   // force phase captured as a hipGraph (tuning.use_graph)
   int64_t lists_gen = 0;
+  int64_t evaluated_pairs = 0;  // last force phase, culled entries excluded
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int64_t graph_gen = -1;

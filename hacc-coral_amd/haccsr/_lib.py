@@ -65,13 +65,15 @@ class Stats(C.Structure):
                 ("entries", C.c_int64), ("pairs", C.c_int64),
                 ("max_depth", C.c_int64), ("build_ms", C.c_double),
                 ("lists_ms", C.c_double), ("force_ms", C.c_double),
-                ("kernel_ms", C.c_double), ("adjoint_ms", C.c_double)]
+                ("kernel_ms", C.c_double), ("adjoint_ms", C.c_double),
+                ("evaluated_pairs", C.c_int64)]
 
 
 class Tuning(C.Structure):
     _fields_ = [("i_per_lane", C.c_int), ("tile", C.c_int), ("chunk", C.c_int),
                 ("waves_per_cu", C.c_int), ("use_graph", C.c_int),
-                ("adjoint_mode", C.c_int), ("reserved", C.c_int * 2)]
+                ("adjoint_mode", C.c_int), ("cull_off", C.c_int),
+                ("reserved", C.c_int * 1)]
 
 
 _lib = None
@@ -82,7 +84,7 @@ def load(path: str | None = None):
     global _lib
     if _lib is not None:
         return _lib
-    p = path or LIB_PATH
+    p = path or os.environ.get("HACCSR_LIB") or LIB_PATH
     if not os.path.exists(p):
         raise HsrError(f"{p} is missing: build it with `make -C hacc-coral_amd`")
     L = C.CDLL(p)

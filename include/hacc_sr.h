@@ -139,6 +139,8 @@ typedef struct {
   double build_ms, lists_ms, force_ms; /* device time of the last calls */
   double kernel_ms;       /* the leaf-kernel launch alone (last force/kick) */
   double adjoint_ms;      /* device time of the last hsr_adjoint */
+  int64_t evaluated_pairs;/* pairs the last force/kick evaluated: list pairs
+                             minus those of entries culled beyond the cutoff */
 } hsr_stats;
 
 hsr_status hsr_get_stats(hsr_ctx *ctx, hsr_stats *st);
@@ -193,10 +195,14 @@ typedef struct {
   int waves_per_cu;   /* persistent waves per CU (0 = automatic) */
   int use_graph;      /* capture the force phase in a hipGraph and replay it */
   int adjoint_mode;   /* hsr_adjoint passes: 0 automatic, 1 fused (one pair
-                         evaluation feeds target and source; leaves of at most
-                         256 particles; 12 B of device memory per list entry),
-                         2 separate target and source passes */
-  int reserved[2];
+                         evaluation feeds target and source; 12 B of device
+                         memory per list entry), 2 separate target and source
+                         passes (automatic picks fused when that memory is
+                         free) */
+  int cull_off;       /* 1: evaluate every list entry; 0 (default): skip entries
+                         farther than the cutoff from the target leaf's box,
+                         whose pairs are all masked (results unchanged) */
+  int reserved[1];
 } hsr_tuning;
 
 hsr_status hsr_set_tuning(hsr_ctx *ctx, const hsr_tuning *t);

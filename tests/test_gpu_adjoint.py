@@ -100,10 +100,10 @@ def test_adjoint_config_a(mode):
     assert e < 1e-4
 
 
-def test_adjoint_schedules_agree_and_large_leaves_fall_back():
-    """Fused and two-pass schedules agree to rounding; a leaf above 256
-    particles (nd 300) makes the automatic choice take the two-pass schedule
-    and the fused one refuse."""
+def test_adjoint_schedules_agree_on_large_leaves():
+    """Leaves above 128 particles take several passes over their targets in
+    both schedules (nd 300); fused and two-pass agree with the FP64 oracle
+    and with each other to rounding."""
     say("case adjoint schedules nd=300")
     rng = np.random.default_rng(77)
     n, box = 20000, 12.0
@@ -113,14 +113,14 @@ def test_adjoint_schedules_agree_and_large_leaves_fall_back():
     L32 = lam.astype(np.float32)
     args = (L32[:, 0].copy(), L32[:, 1].copy(), L32[:, 2].copy())
     ref = O.adjoint_f64(tp, tree, lists, lam)
-    got = np.stack(ctx.adjoint(*args), 1)          # automatic -> two-pass
-    assert rel_rms(got, ref) < 1e-4
-    ctx.set_tuning(adjoint_mode=1)
-    with pytest.raises(Exception, match="256"):
-        ctx.adjoint(*args)
-    ctx.set_tuning(adjoint_mode=2)
-    g2 = np.stack(ctx.adjoint(*args), 1)
-    assert np.array_equal(got, g2)
+    out = {}
+    for mode in (1, 2):
+        ctx.set_tuning(adjoint_mode=mode)
+        out[mode] = np.stack(ctx.adjoint(*args), 1).astype(np.float64)
+        e = rel_rms(out[mode], ref)
+        say(f"  mode {mode}: rel rms vs FP64 {e:.3e}")
+        assert e < 1e-4
+    assert rel_rms(out[1], out[2]) < 1e-5
 
 
 @pytest.mark.parametrize("mode", [1, 2])
