@@ -353,6 +353,16 @@ __device__ __forceinline__ float group_sum_last(float v) {
   return v;
 }
 
+// Targets of the next pass over a leaf: 128 while that many remain, then the
+// largest of 64, 32, 16, 8 that fits, then the rest (< 8).  With two targets
+// per lane each chunk fills groups of a power-of-two width exactly.
+__device__ __forceinline__ int64_t pass_size(int64_t rem) {
+  if (rem >= 64 * FR) return 64 * FR;
+  for (int64_t c = 32 * FR; c >= 4 * FR; c >>= 1)
+    if (rem >= c) return c;
+  return rem;
+}
+
 // A group's slice of the staged tile: every pair of its FR targets per lane
 // with each entry, the target sums B in registers, and the entry's source sum
 // W (over the group's lpg = LP lanes) written back into its tile slot.
@@ -451,12 +461,12 @@ adj_fused_body(const hsr_node *__restrict__ cn, const AItem *__restrict__ items,
     }
     const float4 padJ = make_float4(nd.xc[0] + 1000.f, nd.xc[1], nd.xc[2], 0.f);
     const float padI = nd.xc[0] + 3000.f;
-    for (int64_t ib = 0; ib < cnt; ib += 64 * FR) {
+    // passes over the targets in power-of-two chunks (128, 64, ..., 8, rest)
+    // so that every pass fills its groups of lp = chunk / FR lanes
+    for (int64_t ib = 0, ni = 0; ib < cnt; ib += ni) {
       const bool first = ib == 0;
-      const int ni = (int)min<int64_t>(64 * FR, cnt - ib);
-      const int need = max((ni + FR - 1) / FR, 4);
-      int lp = 4;
-      while (lp < need) lp <<= 1;
+      ni = pass_size(cnt - ib);
+      const int lp = max((int)(ni + FR - 1) / FR, 4);
       const int G = 64 / lp;
       const int g = l / lp, tq = l & (lp - 1);
       float xi[FR], yi[FR], zi[FR], la[FR], lb[FR], lc[FR], am[FR], bx[FR], by[FR], bz[FR];

@@ -75,7 +75,15 @@ __device__ __forceinline__ float poly_tail(const LawArgs &L, float s) {
   return q;
 }
 
-template <int ORDER, bool GUARD>
+// Targets of the next pass over a leaf: 128 while that many remain, then the
+// largest of 64, 32, 16, 8 that fits, then the rest (< 8).
+__device__ __forceinline__ int64_t pass_size(int64_t rem) {
+  if (rem >= 64 * R) return 64 * R;
+  for (int64_t c = 32 * R; c >= 4 * R; c >>= 1)
+    if (rem >= c) return c;
+  return rem;
+}
+
 // Waves per SIMD the register allocation of the leaf kernel targets (0: the
 // compiler's choice).
 #ifndef HSR_FORCE_WAVES
@@ -87,6 +95,7 @@ template <int ORDER, bool GUARD>
 #define HSR_FORCE_ATTR
 #endif
 
+template <int ORDER, bool GUARD>
 __global__ void __launch_bounds__(64) HSR_FORCE_ATTR
 k_force(const hsr_node *__restrict__ cn, const int32_t *__restrict__ leaves,
         const Item *__restrict__ items, const int64_t *__restrict__ nitems_p,
@@ -114,8 +123,11 @@ k_force(const hsr_node *__restrict__ cn, const int32_t *__restrict__ leaves,
     // intermediate finite
     const float4 pad = make_float4(nd.xc[0] + 1000.f, nd.xc[1], nd.xc[2], 0.f);
     int64_t ev = 0;  // pairs evaluated for this item (culled entries excluded)
-    for (int64_t ib = 0; ib < cnt; ib += 64 * R) {
-      const int ni = (int)min<int64_t>(64 * R, cnt - ib);
+    // passes over the targets in power-of-two chunks (128, 64, ..., 8, rest)
+    // so that every pass fills its groups of lpg = chunk / R lanes
+    for (int64_t ib = 0, nib = 0; ib < cnt; ib += nib) {
+      nib = pass_size(cnt - ib);
+      const int ni = (int)nib;
       const int lpg = max((ni + R - 1) / R, 64 / GMAX);
       const int G = max(1, 64 / lpg);
       const int g = l / lpg, tq = l - g * lpg;

@@ -9,7 +9,7 @@ from __future__ import annotations
 
 import time
 
-DEFAULT_CANDIDATES = (64, 96, 128, 160, 192, 256)
+DEFAULT_CANDIDATES = (64, 96, 128, 160, 192, 256, 320, 384, 512)
 
 
 def tune_leaf_size(run_step, candidates=DEFAULT_CANDIDATES, repeats=2, log=None):
