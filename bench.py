@@ -44,7 +44,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="B", help="B (256^3 clustered), A (64^3), D (128^3)")
+    ap.add_argument("--config", default="B",
+                    help="B (256^3 clustered), A (64^3), D (128^3), E (ensemble of 64 boxes)")
     ap.add_argument("--n-side", type=int, default=None, help="override the load's n^(1/3)")
     ap.add_argument("--nd", type=int, default=0, help="leaf size (0 = autotuned)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -69,6 +70,14 @@ def workload_name(cfg, load, nd):
              "D": "128^3 clustered (GRF P(k)~k^-2, Zel'dovich rms 2, 62 NFW halos)"}
     return (f"{names.get(cfg, cfg)}, box {load.box:g}, +{load.n - load.n_alive} ghosts "
             f"(shell {load.overload:g}), eps 0.1, fsrrmax 3.12, oa 0.1, nd {nd}")
+
+
+def ensemble_name(ens, n, nd):
+    sides = ens["sides"]
+    return (f"ensemble of {ens['nboxes']} clustered boxes (n_side log-uniform in [32, 128]: "
+            f"{min(sides)}..{max(sides)}; {ens['alive']} alive, {n} with ghosts), eps 0.1, "
+            f"fsrrmax 3.12, oa 0.1, nd {nd}; one build, one list pass, one graph-replayed "
+            f"leaf launch over all boxes")
 
 
 def cpu_baseline(load, nd, seconds):
@@ -165,16 +174,35 @@ def main():
         return
 
     t0 = time.perf_counter()
-    load = ics.config_load(a.config, n_side=a.n_side)
-    log(f"load {a.config}: {load.n} particles ({load.n_alive} alive) in {time.perf_counter() - t0:.1f} s")
-    n = load.n
+    if a.config == "E":
+        loads = ics.ensemble_loads(log=log)
+        ens = {"offsets": np.cumsum([0] + [b.n for b in loads]).astype(np.int64),
+               "lo": np.stack([b.tree_box[0] for b in loads]),
+               "hi": np.stack([b.tree_box[1] for b in loads]),
+               "flo": np.stack([b.force_box[0] for b in loads]),
+               "fhi": np.stack([b.force_box[1] for b in loads]),
+               "alive": sum(b.n_alive for b in loads), "nboxes": len(loads),
+               "sides": [int(round(b.box)) for b in loads]}
+        px = np.concatenate([b.x for b in loads])
+        py = np.concatenate([b.y for b in loads])
+        pz = np.concatenate([b.z for b in loads])
+        n = px.shape[0]
+        load = None
+        log(f"load E: {ens['nboxes']} boxes, {n} particles ({ens['alive']} alive) "
+            f"in {time.perf_counter() - t0:.1f} s")
+    else:
+        ens = None
+        load = ics.config_load(a.config, n_side=a.n_side)
+        log(f"load {a.config}: {load.n} particles ({load.n_alive} alive) in {time.perf_counter() - t0:.1f} s")
+        n = load.n
+        px, py, pz = load.x, load.y, load.z
+        lo, hi = load.tree_box
+        flo, fhi = load.force_box
     dev = torch.device("cuda:0")
-    lo, hi = load.tree_box
-    flo, fhi = load.force_box
 
     def T(arr):
         return torch.from_numpy(np.ascontiguousarray(arr)).to(dev)
-    src = {"x": T(load.x), "y": T(load.y), "z": T(load.z),
+    src = {"x": T(px), "y": T(py), "z": T(pz),
            "m": T(np.ones(n, np.float32)), "vx": T(np.zeros(n, np.float32)),
            "vy": T(np.zeros(n, np.float32)), "vz": T(np.zeros(n, np.float32)),
            "phi": T(np.zeros(n, np.float32)), "id": T(np.arange(n, dtype=np.int64)),
@@ -191,10 +219,17 @@ def main():
         for k in src:
             work[k].copy_(src[k])
         torch.cuda.synchronize()
-        ctx.build(work["x"], work["y"], work["z"], work["m"], work["vx"], work["vy"],
-                  work["vz"], work["phi"], work["id"], work["mask"],
-                  min_loc=lo, max_loc=hi, device_ptrs=True)
-        ctx.lists(flo, fhi)
+        if ens is None:
+            ctx.build(work["x"], work["y"], work["z"], work["m"], work["vx"], work["vy"],
+                      work["vz"], work["phi"], work["id"], work["mask"],
+                      min_loc=lo, max_loc=hi, device_ptrs=True)
+            ctx.lists(flo, fhi)
+        else:
+            # every box's tree, lists and leaf work in the same launches
+            ctx.build_many(ens["offsets"], work["x"], work["y"], work["z"], work["m"],
+                           ens["lo"], ens["hi"], work["vx"], work["vy"], work["vz"],
+                           work["phi"], work["id"], work["mask"], device_ptrs=True)
+            ctx.lists_many(ens["flo"], ens["fhi"])
         ctx.kick(fcoeff)
         return ctx.stats()
 
@@ -241,7 +276,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded clustered generator; unit masses)",
-        "config": {"workload": workload_name(a.config, load, nd), "particles": n,
+        "config": {"workload": (workload_name(a.config, load, nd) if ens is None else
+                                ensemble_name(ens, n, nd)), "particles": n,
                    "pairs_per_step": pairs, "leaf_size": nd,
                    "breakdown_ms": {"build": float(np.mean([s[1] for s in stats])),
                                     "lists": float(np.mean([s[2] for s in stats])),
@@ -260,7 +296,7 @@ def main():
         ctxs[nd].set_tuning(adjoint_mode=a.adjoint_mode)
         out["adjoint"] = adjoint_leg(ctxs[nd], n, dev, a.adjoint_reps, stats)
         log(f"adjoint {out['adjoint']}")
-    if not a.no_cpu:
+    if not a.no_cpu and ens is None:
         log("cpu baseline leg ...")
         out["cpu_baseline"] = cpu_baseline(load, nd, a.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -1136,11 +1136,21 @@ hsr_status adjoint_plan(hsr_ctx *c) {
   HSR_HIP(hipMemcpyAsync(dptr<int64_t>(A.counts), &tail_items[2], 2 * sizeof(int64_t), hipMemcpyHostToDevice, st));
   HSR_HIP(hipStreamSynchronize(st));
   HSR_HIP(hipGetLastError());
+  // buffers of the passes (reserved here so that a captured graph allocates nothing)
+  HSR_HIP(dbuf_reserve(A.qctr, 64));
+  HSR_HIP(dbuf_reserve(A.part, sizeof(float) * (std::max(A.fw_part, A.tr_part) + 1)));
+  HSR_HIP(dbuf_reserve(A.H, sizeof(float) * 3 * (nn + 1) + sizeof(float) * 3 * (A.npseudo + 1)));
   A.gen = c->lists_gen;
   A.planned_mode = c->tuning.adjoint_mode;
   return HSR_OK;
 }
 
+static hsr_status adjoint_enqueue(hsr_ctx *c, const float *lx, const float *ly,
+                                  const float *lz, float *gx, float *gy, float *gz);
+
+// The adjoint passes.  The plan (transposition, item queues; host syncs) is
+// made once per list build; with tuning.use_graph the passes themselves are
+// captured once per plan and argument set and replayed as one graph.
 hsr_status adjoint_run(hsr_ctx *c, const float *lx, const float *ly,
                        const float *lz, float *gx, float *gy, float *gz) {
   hipStream_t st = c->stream;
@@ -1148,26 +1158,70 @@ hsr_status adjoint_run(hsr_ctx *c, const float *lx, const float *ly,
   if (A.gen != c->lists_gen || A.planned_mode != c->tuning.adjoint_mode) {
     hsr_status s = adjoint_plan(c);
     if (s != HSR_OK) return s;
+    A.graph_key[0] = nullptr;  // re-capture
   }
+  if (c->law.guard) {
+    hsr_set_error("hsr_adjoint: needs rsm > 0");
+    return HSR_ERR_UNSUPPORTED;
+  }
+  if (!c->tuning.use_graph) {
+    HSR_HIP(hipEventRecord(c->ev[6], st));
+    hsr_status s = adjoint_enqueue(c, lx, ly, lz, gx, gy, gz);
+    if (s != HSR_OK) return s;
+    HSR_HIP(hipEventRecord(c->ev[7], st));
+    c->kernel_timed = true;
+    return HSR_OK;
+  }
+  const void *key[6] = {lx, ly, lz, gx, gy, gz};
+  bool stale = A.graph_exec == nullptr || A.graph_cull != c->tuning.cull_off ||
+               A.graph_wpc != c->tuning.waves_per_cu;
+  for (int k = 0; k < 6; ++k) stale = stale || A.graph_key[k] != key[k];
+  if (stale) {
+    if (A.graph_exec) {
+      HSR_HIP(hipGraphExecDestroy(A.graph_exec));
+      A.graph_exec = nullptr;
+    }
+    if (A.graph) {
+      HSR_HIP(hipGraphDestroy(A.graph));
+      A.graph = nullptr;
+    }
+    HSR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hsr_status s = adjoint_enqueue(c, lx, ly, lz, gx, gy, gz);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (s != HSR_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return s;
+    }
+    HSR_HIP(e);
+    A.graph = g;
+    HSR_HIP(hipGraphInstantiate(&A.graph_exec, g, nullptr, nullptr, 0));
+    for (int k = 0; k < 6; ++k) A.graph_key[k] = key[k];
+    A.graph_cull = c->tuning.cull_off;
+    A.graph_wpc = c->tuning.waves_per_cu;
+  }
+  HSR_HIP(hipEventRecord(c->ev[6], st));
+  HSR_HIP(hipGraphLaunch(A.graph_exec, st));
+  HSR_HIP(hipEventRecord(c->ev[7], st));
+  c->kernel_timed = true;
+  return HSR_OK;
+}
+
+static hsr_status adjoint_enqueue(hsr_ctx *c, const float *lx, const float *ly,
+                                  const float *lz, float *gx, float *gy, float *gz) {
+  hipStream_t st = c->stream;
+  AdjState &A = c->adj;
   const int64_t n = c->tree.n, nn = c->tree.nnodes;
   const hsr_node *cn = dptr<hsr_node>(c->tree.canon_nodes);
   const PJ *pj = dptr<PJ>(c->tree.pj);
   AdjLaw L = adj_law(c->law);
   L.cull = c->tuning.cull_off ? 0 : 1;
-  if (c->law.guard) {
-    hsr_set_error("hsr_adjoint: needs rsm > 0");
-    return HSR_ERR_UNSUPPORTED;
-  }
-  HSR_HIP(dbuf_reserve(A.qctr, 64));
-  HSR_HIP(dbuf_reserve(A.part, sizeof(float) * (std::max(A.fw_part, A.tr_part) + 1)));
-  HSR_HIP(dbuf_reserve(A.H, sizeof(float) * 3 * (nn + 1) + sizeof(float) * 3 * (A.npseudo + 1)));
   unsigned long long *q = dptr<unsigned long long>(A.qctr);
   float *part = dptr<float>(A.part);
   float *H = dptr<float>(A.H);
   float *gseg = H + 3 * (nn + 1);
   const int64_t *counts = dptr<int64_t>(A.counts);
   const int wpc = c->tuning.waves_per_cu > 0 ? c->tuning.waves_per_cu : 32;
-  HSR_HIP(hipEventRecord(c->ev[6], st));
   if (n) {
     k_fill<<<nblk(n, 256), 256, 0, st>>>(n, gx);
     k_fill<<<nblk(n, 256), 256, 0, st>>>(n, gy);
@@ -1235,8 +1289,6 @@ hsr_status adjoint_run(hsr_ctx *c, const float *lx, const float *ly,
     k_pseudo_node<<<nblk(A.npseudo, 256), 256, 0, st>>>(A.npseudo, A.nsp, nn, A.key, gseg, cn, A.mpre, 0, H);
     k_pseudo_push<<<(unsigned)nn, 64, 0, st>>>(nn, cn, dptr<int32_t>(A.par), H, pj, gx, gy, gz);
   }
-  HSR_HIP(hipEventRecord(c->ev[7], st));
-  c->kernel_timed = true;
   HSR_HIP(hipGetLastError());
   return HSR_OK;
 }

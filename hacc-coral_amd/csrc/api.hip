@@ -309,6 +309,8 @@ hsr_status hsr_destroy(hsr_ctx *c) {
                   &c->s.cub, &c->s.x1, &c->s.y1, &c->s.z1, &c->s.m1, &c->s.p1,
                   &c->s.x0, &c->s.y0, &c->s.z0, &c->s.m0, &c->s.p0, &c->s.blocks, &c->s.rights,
                   &c->s.pblocks, &c->s.counters, &c->dev_v, &c->force};
+  if (c->adj.graph_exec) (void)hipGraphExecDestroy(c->adj.graph_exec);
+  if (c->adj.graph) (void)hipGraphDestroy(c->adj.graph);
   for (DBuf *b : bufs) dbuf_free(*b);
   for (int k = 0; k < 10; ++k) dbuf_free(c->s.host_in[k]);
   for (int k = 0; k < 8; ++k) (void)hipEventDestroy(c->ev[k]);

@@ -129,6 +129,11 @@ struct AdjState {
   int64_t nsp = 0, npseudo = 0, fw_part = 0, tr_part = 0;
   bool fused = false;        // one pair evaluation feeds both sides (W per entry)
   int planned_mode = -1;     // tuning.adjoint_mode the plan was made for
+  // captured passes (tuning.use_graph) and what they were captured for
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  const void *graph_key[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  int graph_cull = -1, graph_wpc = -1;
   int64_t nruns = 0;         // source leaves among the sorted particle segments
   int64_t *run_first = nullptr;
   int64_t *fw_first = nullptr, *tr_first = nullptr;

@@ -289,23 +289,46 @@ k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, const CmBlock *blk,
       }
     }
     const int cnt = (int)min<int64_t>(64, nb - bb);
-    for (int i = 0; i < cnt; ++i) {
-      for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < 4; ++q) {
+      for (int i = 0; i < cnt;) {  // uniform
+        // a run of blocks from i whose maps share i's grid: compose them in
+        // order (maps compose associatively) and apply once
+        const int32_t Ei = readlane_i32(mE[q], i);
+        const bool same = l >= i && l < cnt && mE[q] == Ei && !mbad[q];
+        const uint64_t sh = (uint64_t)__ballot(same) >> i;
+        const int run = (~sh == 0ull) ? 64 - i : (int)__builtin_ctzll(~sh);
+        if (run >= 2) {
+          sd_map m = sd_map_identity();
+          if (l >= i && l < i + run) {
+            m.d0 = md0[q];
+            m.d1 = md1[q];
+          }
+          const sd_map g = wave_compose_maps(m);
+          sd_t S2 = S[q];
+          if (sd_map_apply(&S2, g, Ei)) {
+            S[q] = S2;
+            i += run;
+            continue;
+          }
+        }
+        // block i alone: its map, else its addends chunk by chunk
         sd_map g;
         g.d0 = readlane_u64(md0[q], i);
         g.d1 = readlane_u64(md1[q], i);
         g.bad = readlane_i32(mbad[q], i);
-        if (sd_map_apply(&S[q], g, readlane_i32(mE[q], i))) continue;  // uniform
-        const CmBlock &B = blk[b0 + bb + i];
-        const int64_t start = B.start, len = B.len;
-        __syncthreads();
-        for (int k = l; k < len; k += 64) buf[k] = addend(q, X, Y, Z, M, start + k);
-        __syncthreads();
-        for (int64_t c0 = 0; c0 < len; c0 += 64) {
-          int nvalid = (int)min<int64_t>(64, len - c0);
-          float v = (c0 + l < len) ? buf[c0 + l] : 0.f;
-          sd_chunk(S[q], v, nvalid);
+        if (!sd_map_apply(&S[q], g, Ei)) {
+          const CmBlock &B = blk[b0 + bb + i];
+          const int64_t start = B.start, len = B.len;
+          __syncthreads();
+          for (int k = l; k < len; k += 64) buf[k] = addend(q, X, Y, Z, M, start + k);
+          __syncthreads();
+          for (int64_t c0 = 0; c0 < len; c0 += 64) {
+            int nvalid = (int)min<int64_t>(64, len - c0);
+            float v = (c0 + l < len) ? buf[c0 + l] : 0.f;
+            sd_chunk(S[q], v, nvalid);
+          }
         }
+        ++i;
       }
     }
   }

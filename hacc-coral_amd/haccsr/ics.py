@@ -264,3 +264,22 @@ def config_load(name: str, seed: int | None = None, sort: bool = True,
     if sort:
         load, _ = cell_sort(load)
     return load
+
+
+def ensemble_loads(nboxes: int = 64, seed0: int = 1100, n_min: int = 32, n_max: int = 128,
+                   overload: float = 3.2, log=None):
+    """Independent periodic boxes: n_side ~ log-uniform in [n_min, n_max]
+    (particle counts log-uniform between n_min^3 and n_max^3), box side
+    n_side (mean density 1), the clustered generator with seed seed0 + b and
+    halos scaled by volume (500 per 256^3); each box with its ghost shell and
+    cell sort."""
+    rng = np.random.default_rng(seed0)
+    sides = np.rint(np.exp(rng.uniform(math.log(n_min), math.log(n_max), nboxes))).astype(int)
+    loads = []
+    for b, ns in enumerate(sides):
+        x, y, z = clustered(int(ns), float(ns), max(1, round(500 * (ns / 256) ** 3)), seed0 + b)
+        load, _ = cell_sort(add_ghosts(x, y, z, float(ns), overload))
+        loads.append(load)
+        if log and b % 16 == 15:
+            log(f"ensemble: {b + 1}/{nboxes} boxes generated")
+    return loads

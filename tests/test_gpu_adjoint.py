@@ -143,3 +143,27 @@ def test_adjoint_degenerate_tree(mode):
     e = rel_rms(got, ref)
     say(f"  rel rms vs FP64 {e:.3e}")
     assert e < 1e-4
+
+
+def test_adjoint_graph_replay_matches_eager():
+    """With tuning.use_graph the adjoint passes are captured once per plan and
+    replayed: every replay (with eager work in between) equals the eager
+    result bitwise, and a new list build re-captures."""
+    say("case adjoint graph replay")
+    rng = np.random.default_rng(31)
+    n, box = 20000, 14.0
+    p = (rng.random((n, 3)) * box).astype(np.float32)
+    tp, tree, lists, ctx, lam = setup(p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), box, 64, 0.5)
+    L32 = lam.astype(np.float32)
+    args = (L32[:, 0].copy(), L32[:, 1].copy(), L32[:, 2].copy())
+    eager = np.stack(ctx.adjoint(*args), 1)
+    ctx.set_tuning(use_graph=1)
+    for rep in range(4):
+        got = np.stack(ctx.adjoint(*args), 1)
+        assert np.array_equal(got.view(np.uint32), eager.view(np.uint32)), f"replay {rep}"
+        ctx.forces()                      # eager work between replays
+    ctx.lists(tp.min_force, tp.max_force)  # new plan -> new capture
+    again = np.stack(ctx.adjoint(*args), 1)
+    assert np.array_equal(again.view(np.uint32), eager.view(np.uint32))
+    ref = O.adjoint_f64(tp, tree, lists, lam)
+    assert rel_rms(eager.astype(np.float64), ref) < 1e-4
