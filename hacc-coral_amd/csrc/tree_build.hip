@@ -23,8 +23,9 @@
 
 namespace {
 
-constexpr int CM_BLK = 4096;        // elements per block of a big node
-constexpr int64_t CM_SMALL = 16384; // nodes up to this size: one wave each
+constexpr int CM_BLK = 1024;        // elements per block of a big node
+constexpr int CM_PER = CM_BLK / 64; // consecutive addends each lane composes
+constexpr int64_t CM_SMALL = 2048;  // nodes up to this size: one wave each
 constexpr int PB = 1024;            // partition block (256 threads x 4)
 constexpr int MAXHOP = 256;         // swap-chain hops before pointer jumping
 
@@ -183,30 +184,49 @@ k_cm_blk_stats(CmBlock *blk, const Counters *ctr, const float *X,
   }
 }
 
-// Per big node: exclusive prefix of the approximate sums (binade guesses)
-// and the node bbox.
-__global__ void k_cm_blk_prefix(BNode *nodes, int64_t lb, int64_t le,
-                                CmBlock *blk) {
-  int64_t t = lb + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+// Per big node (one wave): exclusive prefix of the approximate block sums
+// (only a guess of the running sum's binade, so any summation order serves)
+// and the node bbox (min/max: order-free).
+__global__ void __launch_bounds__(64)
+k_cm_blk_prefix(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk) {
+  const int64_t t = lb + blockIdx.x;
   if (t >= le) return;
   if (nodes[t].count <= CM_SMALL) return;
-  int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
-  float run[4] = {0.f, 0.f, 0.f, 0.f};
+  const int l = lane_id();
+  const int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
+  float base[4] = {0.f, 0.f, 0.f, 0.f};
   float mn[3] = {INFINITY, INFINITY, INFINITY};
   float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int64_t b = b0; b < b0 + nb; ++b) {
+  for (int64_t c0 = 0; c0 < nb; c0 += 64) {
+    const int64_t b = b0 + c0 + l;
+    const bool v = c0 + l < nb;
+#pragma unroll
     for (int q = 0; q < 4; ++q) {
-      blk[b].ein[q] = run[q];
-      run[q] += blk[b].approx[q];
+      const float a = v ? blk[b].approx[q] : 0.f;
+      float inc = a;  // inclusive scan over the lanes
+      for (int o = 1; o < 64; o <<= 1) {
+        const float y = __shfl_up(inc, o);
+        if (l >= o) inc += y;
+      }
+      if (v) blk[b].ein[q] = base[q] + (inc - a);
+      base[q] += __shfl(inc, 63);
     }
-    for (int d = 0; d < 3; ++d) {
-      mn[d] = fminf(mn[d], blk[b].bmin[d]);
-      mx[d] = fmaxf(mx[d], blk[b].bmax[d]);
+    if (v) {
+      for (int d = 0; d < 3; ++d) {
+        mn[d] = fminf(mn[d], blk[b].bmin[d]);
+        mx[d] = fmaxf(mx[d], blk[b].bmax[d]);
+      }
     }
   }
   for (int d = 0; d < 3; ++d) {
-    nodes[t].xmin[d] = mn[d];
-    nodes[t].xmax[d] = mx[d];
+    mn[d] = wave_min_f32(mn[d]);
+    mx[d] = wave_max_f32(mx[d]);
+  }
+  if (l == 0) {
+    for (int d = 0; d < 3; ++d) {
+      nodes[t].xmin[d] = mn[d];
+      nodes[t].xmax[d] = mx[d];
+    }
   }
 }
 
@@ -225,12 +245,12 @@ __device__ __forceinline__ float addend(int q, const float *X, const float *Y,
 }
 
 // Fixed-grid maps of each block for its guessed binade: the block is staged
-// in LDS, each lane composes 64 consecutive addends in order, then the 64
+// in LDS, each lane composes CM_PER consecutive addends in order, then the 64
 // lane maps are composed in lane order.
 __global__ void __launch_bounds__(64)
 k_cm_blk_maps(CmBlock *blk, const Counters *ctr, const float *X,
               const float *Y, const float *Z, const float *M) {
-  __shared__ float buf[64 * 65];
+  __shared__ float buf[64 * (CM_PER + 1)];
   int64_t b = blockIdx.x;
   if (b >= ctr->cm_blocks) return;
   CmBlock &B = blk[b];
@@ -245,11 +265,11 @@ k_cm_blk_maps(CmBlock *blk, const Counters *ctr, const float *X,
       __syncthreads();
       for (int k = l; k < CM_BLK; k += 64) {
         float v = k < len ? addend(q, X, Y, Z, M, start + k) : 0.f;
-        buf[(k >> 6) * 65 + (k & 63)] = v;
+        buf[(k / CM_PER) * (CM_PER + 1) + (k % CM_PER)] = v;
       }
       __syncthreads();
-      for (int k = 0; k < 64; ++k)
-        acc = sd_map_compose(acc, sd_map_of(buf[l * 65 + k], E));
+      for (int k = 0; k < CM_PER; ++k)
+        acc = sd_map_compose(acc, sd_map_of(buf[l * (CM_PER + 1) + k], E));
       acc = wave_compose_maps(acc);
     }
     if (l == 0) {
@@ -268,7 +288,7 @@ k_cm_blk_maps(CmBlock *blk, const Counters *ctr, const float *X,
 __global__ void __launch_bounds__(64)
 k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, const CmBlock *blk,
                const float *X, const float *Y, const float *Z,
-               const float *M) {
+               const float *M, Counters *ctr) {
   __shared__ float buf[CM_BLK];
   int64_t t = lb + blockIdx.x;
   if (t >= le) return;
@@ -308,6 +328,9 @@ k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, const CmBlock *blk,
           if (sd_map_apply(&S2, g, Ei)) {
             S[q] = S2;
             i += run;
+#ifdef HSR_CM_STATS
+            if (l == 0) atomicAdd((unsigned long long *)&ctr->pad[0], 1ull);
+#endif
             continue;
           }
         }
@@ -316,7 +339,13 @@ k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, const CmBlock *blk,
         g.d0 = readlane_u64(md0[q], i);
         g.d1 = readlane_u64(md1[q], i);
         g.bad = readlane_i32(mbad[q], i);
+#ifdef HSR_CM_STATS
+        if (l == 0) atomicAdd((unsigned long long *)&ctr->pad[1], 1ull);
+#endif
         if (!sd_map_apply(&S[q], g, Ei)) {
+#ifdef HSR_CM_STATS
+          if (l == 0) atomicAdd((unsigned long long *)&ctr->pad[2], 1ull);
+#endif
           const CmBlock &B = blk[b0 + bb + i];
           const int64_t start = B.start, len = B.len;
           __syncthreads();
@@ -903,9 +932,9 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     k_cm_plan<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, cmb, ctr);
     unsigned cmg = (unsigned)std::max<int64_t>(1, active / CM_BLK + active / CM_SMALL + 1);
     k_cm_blk_stats<<<cmg, 64, 0, st>>>(cmb, ctr, X[cur], Y[cur], Z[cur], Mm[cur]);
-    k_cm_blk_prefix<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, cmb);
+    k_cm_blk_prefix<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, cmb);
     k_cm_blk_maps<<<cmg, 64, 0, st>>>(cmb, ctr, X[cur], Y[cur], Z[cur], Mm[cur]);
-    k_cm_blk_chain<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, cmb, X[cur], Y[cur], Z[cur], Mm[cur]);
+    k_cm_blk_chain<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, cmb, X[cur], Y[cur], Z[cur], Mm[cur], ctr);
     // ---- classify, finalize plain leaves
     k_classify<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, nd, c->p.ppc, Mm[cur]);
     k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_LEAF, P[cur], Pfin);
@@ -934,6 +963,11 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
                                         X[cur ^ 1], Y[cur ^ 1], Z[cur ^ 1], Mm[cur ^ 1], P[cur ^ 1]);
     k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_DEGEN, P[cur ^ 1], Pfin);
     HSR_HIP(hipGetLastError());
+#ifdef HSR_CM_STATS
+    fprintf(stderr, "[cm] level %d nodes %lld blocks %lld composed %lld single %lld fallback %lld\n", depth,
+            (long long)nl, (long long)hc.cm_blocks, (long long)hc.pad[0], (long long)hc.pad[1],
+            (long long)hc.pad[2]);
+#endif
     T.level_start.push_back(le + 2 * hc.nsplit);
     active = hc.active_next;
     cur ^= 1;
