@@ -58,52 +58,39 @@ __device__ __forceinline__ const float *coord(int d, const float *X,
 }
 
 // ------------------------------------------------------------ cm (small)
-// One wave per node with count <= CM_SMALL (BGQCM.c:181-212).  The next
-// chunk's loads are issued before the current chunk is accumulated.
+__device__ __forceinline__ float addend(int q, const float *X, const float *Y,
+                                        const float *Z, const float *M,
+                                        int64_t e) {
+  float w = M[e];
+  return q == 3 ? w : w * (q == 0 ? X[e] : (q == 1 ? Y[e] : Z[e]));
+}
+
+// One wave per node with count <= CM_SMALL (BGQCM.c:181-212): bbox by wave
+// min/max, each of the four sums staged in LDS and added by sd_block.
 __global__ void __launch_bounds__(64)
 k_cm_small(BNode *nodes, int64_t lb, int64_t le, const float *X,
            const float *Y, const float *Z, const float *M) {
+  __shared__ float buf[CM_SMALL];
   int64_t t = lb + blockIdx.x;
   if (t >= le) return;
-  int64_t cnt = nodes[t].count;
+  const int64_t cnt = nodes[t].count;
   if (cnt > CM_SMALL || cnt == 0) return;
-  int64_t off = nodes[t].offset;
-  int l = lane_id();
-  sd_t S[4] = {sd_zero(), sd_zero(), sd_zero(), sd_zero()};
+  const int64_t off = nodes[t].offset;
+  const int l = lane_id();
   float mn[3] = {INFINITY, INFINITY, INFINITY};
   float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-  float nw = 0.f, nx = 0.f, ny = 0.f, nz = 0.f;
-  bool nvld = l < cnt;
-  if (nvld) {
-    nw = M[off + l];
-    nx = X[off + l];
-    ny = Y[off + l];
-    nz = Z[off + l];
+  for (int64_t k = l; k < cnt; k += 64) {
+    const float px = X[off + k], py = Y[off + k], pz = Z[off + k];
+    mn[0] = fminf(mn[0], px); mx[0] = fmaxf(mx[0], px);
+    mn[1] = fminf(mn[1], py); mx[1] = fmaxf(mx[1], py);
+    mn[2] = fminf(mn[2], pz); mx[2] = fmaxf(mx[2], pz);
   }
-  for (int64_t c0 = 0; c0 < cnt; c0 += 64) {
-    const int nvalid = (int)min<int64_t>(64, cnt - c0);
-    const bool vld = nvld;
-    const float w = nw, px = nx, py = ny, pz = nz;
-    const int64_t k1 = c0 + 64 + l;
-    nvld = k1 < cnt;
-    if (nvld) {
-      nw = M[off + k1];
-      nx = X[off + k1];
-      ny = Y[off + k1];
-      nz = Z[off + k1];
-    }
-    float v[4] = {0.f, 0.f, 0.f, 0.f};
-    if (vld) {
-      mn[0] = fminf(mn[0], px); mx[0] = fmaxf(mx[0], px);
-      mn[1] = fminf(mn[1], py); mx[1] = fmaxf(mx[1], py);
-      mn[2] = fminf(mn[2], pz); mx[2] = fmaxf(mx[2], pz);
-      v[0] = w * px;
-      v[1] = w * py;
-      v[2] = w * pz;
-      v[3] = w;
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) sd_chunk(S[q], v[q], nvalid);
+  sd_t S[4] = {sd_zero(), sd_zero(), sd_zero(), sd_zero()};
+  for (int q = 0; q < 4; ++q) {
+    __syncthreads();
+    for (int64_t k = l; k < cnt; k += 64) buf[k] = addend(q, X, Y, Z, M, off + k);
+    __syncthreads();
+    sd_block(S[q], buf, (int)cnt);
   }
   for (int d = 0; d < 3; ++d) {
     mn[d] = wave_min_f32(mn[d]);
@@ -237,12 +224,6 @@ __device__ __forceinline__ int32_t grid_exp_guess(float ein) {
   return (ef - 127) - 52;
 }
 
-__device__ __forceinline__ float addend(int q, const float *X, const float *Y,
-                                        const float *Z, const float *M,
-                                        int64_t e) {
-  float w = M[e];
-  return q == 3 ? w : w * (q == 0 ? X[e] : (q == 1 ? Y[e] : Z[e]));
-}
 
 // Fixed-grid maps of each block for its guessed binade: the block is staged
 // in LDS, each lane composes CM_PER consecutive addends in order, then the 64
@@ -351,11 +332,7 @@ k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, const CmBlock *blk,
           __syncthreads();
           for (int k = l; k < len; k += 64) buf[k] = addend(q, X, Y, Z, M, start + k);
           __syncthreads();
-          for (int64_t c0 = 0; c0 < len; c0 += 64) {
-            int nvalid = (int)min<int64_t>(64, len - c0);
-            float v = (c0 + l < len) ? buf[c0 + l] : 0.f;
-            sd_chunk(S[q], v, nvalid);
-          }
+          sd_block(S[q], buf, (int)len);
         }
         ++i;
       }

@@ -66,6 +66,66 @@ __device__ __forceinline__ sd_map wave_compose_maps(sd_map m) {
   return m;
 }
 
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int lane) {
+  uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+  uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ int readlane_i32(int v, int lane) {
+  return __builtin_amdgcn_readlane(v, lane);
+}
+
+// Sequential correctly rounded double sum S += buf[0] + ... + buf[len-1]
+// (addends staged in LDS, every lane sees them), reproduced exactly and with
+// all 64 lanes working: lane l composes the fixed-grid map of a contiguous
+// run of the remaining addends for S's binade, an ordered inclusive scan of
+// the lane maps gives every prefix's map, and the longest prefix that keeps
+// S in its binade is applied at once.  The run that leaves the binade is
+// then added serially and the rest resumes in the new binade, so a block
+// costs one scan per binade it crosses.
+__device__ __forceinline__ void sd_block(sd_t &S, const float *buf, int len) {
+  const int l = lane_id();
+  int start = 0;
+  while (start < len) {  // uniform
+    if (S.m == 0 || S.s) {  // from zero (or a negative sum): one addend exactly
+      S = sd_add_f32(S, buf[start]);
+      ++start;
+      continue;
+    }
+    const int32_t E = S.e;
+    const int per = (len - start + 63) / 64;
+    const int lo = start + l * per, hi = min(lo + per, len);
+    sd_map m = sd_map_identity();
+    for (int e = lo; e < hi; ++e) m = sd_map_compose(m, sd_map_of(buf[e], E));
+    for (int o = 1; o < 64; o <<= 1) {  // ordered inclusive scan
+      sd_map h;
+      h.d0 = (uint64_t)__shfl_up((unsigned long long)m.d0, o);
+      h.d1 = (uint64_t)__shfl_up((unsigned long long)m.d1, o);
+      h.bad = __shfl_up(m.bad, o);
+      if (l >= o) m = sd_map_compose(h, m);
+    }
+    sd_t S2 = S;
+    const bool ok = sd_map_apply(&S2, m, E) != 0;
+    const uint64_t bal = (uint64_t)__ballot(ok);
+    const int j = (~bal == 0ull) ? 64 : (int)__builtin_ctzll(~bal);  // ok lanes: a prefix
+    if (j == 64) {
+      S.m = readlane_u64(S2.m, 63);
+      S.e = readlane_i32(S2.e, 63);
+      S.s = readlane_i32(S2.s, 63);
+      return;
+    }
+    if (j > 0) {
+      S.m = readlane_u64(S2.m, j - 1);
+      S.e = readlane_i32(S2.e, j - 1);
+      S.s = readlane_i32(S2.s, j - 1);
+    }
+    const int jlo = start + j * per, jhi = min(jlo + per, len);
+    for (int e = jlo; e < jhi; ++e) S = sd_add_f32(S, buf[e]);
+    start = jhi;
+  }
+}
+
 // Accumulate one 64-addend chunk (lane k holds addend k; lanes >= nvalid
 // hold 0) into the wave-uniform exact double S, reproducing k sequential
 // correctly rounded double additions.  The exact-sum test is cheapest and
@@ -99,12 +159,3 @@ __device__ __forceinline__ void sd_chunk(sd_t &S, float v, int nvalid) {
   }
 }
 
-__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int lane) {
-  uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
-  uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
-  return ((uint64_t)hi << 32) | lo;
-}
-
-__device__ __forceinline__ int readlane_i32(int v, int lane) {
-  return __builtin_amdgcn_readlane(v, lane);
-}
