@@ -45,6 +45,11 @@ struct CmBlock {
   int32_t E[4];            // grid exponent each map was built for
   int32_t bad[4];
   float bmin[3], bmax[3];
+  // the leader block (first of each 64-block window of a node) also holds the
+  // window's composed map, valid when all its blocks share one grid
+  uint64_t wd0[4], wd1[4];
+  int32_t wE[4];
+  int32_t wok[4];
 };
 
 struct PartBlock {

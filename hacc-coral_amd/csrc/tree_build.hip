@@ -262,6 +262,42 @@ k_cm_blk_maps(CmBlock *blk, const Counters *ctr, const float *X,
   }
 }
 
+// Window maps: the first block of every 64-block window of a node composes
+// the window's block maps in order when they share one grid (one wave per
+// block; waves of other blocks return at once).
+__global__ void __launch_bounds__(64)
+k_cm_win_maps(const BNode *nodes, CmBlock *blk, const Counters *ctr) {
+  const int64_t b = blockIdx.x;
+  if (b >= ctr->cm_blocks) return;
+  const int64_t t = blk[b].node;
+  const int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
+  if ((b - b0) % 64 != 0) return;
+  const int l = lane_id();
+  const int cnt = (int)min<int64_t>(64, b0 + nb - b);
+  for (int q = 0; q < 4; ++q) {
+    sd_map m = sd_map_identity();
+    int32_t E = 0;
+    int bad = 1;
+    if (l < cnt) {
+      const CmBlock &B = blk[b + l];
+      m.d0 = B.d0[q];
+      m.d1 = B.d1[q];
+      E = B.E[q];
+      bad = B.bad[q];
+    }
+    const int32_t E0 = readlane_i32(E, 0);
+    const int ok = !__any(l < cnt && (bad || E != E0));
+    if (l >= cnt) m = sd_map_identity();
+    m = wave_compose_maps(m);
+    if (l == 0) {
+      blk[b].wd0[q] = m.d0;
+      blk[b].wd1[q] = m.d1;
+      blk[b].wE[q] = E0;
+      blk[b].wok[q] = ok && !m.bad;
+    }
+  }
+}
+
 // Chain the block maps of each big node in order (64 blocks' maps are read
 // at once, one per lane, then applied serially); a block whose map does not
 // apply (binade guess wrong or binade crossed) is staged in LDS and
@@ -278,6 +314,19 @@ k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, const CmBlock *blk,
   const int l = lane_id();
   sd_t S[4] = {sd_zero(), sd_zero(), sd_zero(), sd_zero()};
   for (int64_t bb = 0; bb < nb; bb += 64) {
+    // the whole window at once when its blocks share a grid and S stays in it
+    int done = 0;
+    {
+      const CmBlock &W = blk[b0 + bb];
+      for (int q = 0; q < 4; ++q) {
+        sd_map g;
+        g.d0 = W.wd0[q];
+        g.d1 = W.wd1[q];
+        g.bad = !W.wok[q];
+        if (sd_map_apply(&S[q], g, W.wE[q])) done |= 1 << q;
+      }
+    }
+    if (done == 15) continue;
     uint64_t md0[4] = {0, 0, 0, 0}, md1[4] = {0, 0, 0, 0};
     int mbad[4] = {1, 1, 1, 1}, mE[4] = {0, 0, 0, 0};
     if (bb + l < nb) {
@@ -291,6 +340,7 @@ k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, const CmBlock *blk,
     }
     const int cnt = (int)min<int64_t>(64, nb - bb);
     for (int q = 0; q < 4; ++q) {
+      if (done & (1 << q)) continue;
       for (int i = 0; i < cnt;) {  // uniform
         // a run of blocks from i whose maps share i's grid: compose them in
         // order (maps compose associatively) and apply once
@@ -911,6 +961,7 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     k_cm_blk_stats<<<cmg, 64, 0, st>>>(cmb, ctr, X[cur], Y[cur], Z[cur], Mm[cur]);
     k_cm_blk_prefix<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, cmb);
     k_cm_blk_maps<<<cmg, 64, 0, st>>>(cmb, ctr, X[cur], Y[cur], Z[cur], Mm[cur]);
+    k_cm_win_maps<<<cmg, 64, 0, st>>>(nodes, cmb, ctr);
     k_cm_blk_chain<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, cmb, X[cur], Y[cur], Z[cur], Mm[cur], ctr);
     // ---- classify, finalize plain leaves
     k_classify<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, nd, c->p.ppc, Mm[cur]);

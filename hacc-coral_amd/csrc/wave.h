@@ -78,7 +78,8 @@ __device__ __forceinline__ int readlane_i32(int v, int lane) {
 
 // Sequential correctly rounded double sum S += buf[0] + ... + buf[len-1]
 // (addends staged in LDS, every lane sees them), reproduced exactly and with
-// all 64 lanes working: lane l composes the fixed-grid map of a contiguous
+// all 64 lanes working.  When no addition of the rest can round, the exact
+// integer sum is the answer.  Otherwise lane l composes the fixed-grid map of a contiguous
 // run of the remaining addends for S's binade, an ordered inclusive scan of
 // the lane maps gives every prefix's map, and the longest prefix that keeps
 // S in its binade is applied at once.  The run that leaves the binade is
@@ -88,14 +89,36 @@ __device__ __forceinline__ void sd_block(sd_t &S, const float *buf, int len) {
   const int l = lane_id();
   int start = 0;
   while (start < len) {  // uniform
+    const int per = (len - start + 63) / 64;
+    const int lo = start + l * per, hi = min(lo + per, len);
+    if (!S.s) {
+      // no rounding anywhere in the rest (every partial sum a multiple of the
+      // finest lsb below 2^53 of them): the sum is the exact integer sum
+      int32_t t = S.m ? sd_lsb(S) : (1 << 20);
+      for (int e = lo; e < hi; ++e) t = min(t, sd_lsb_f32(buf[e]));
+      t = wave_min_i32(t);
+      uint64_t u = 0;
+      int bad = 0;
+      for (int e = lo; e < hi; ++e) {
+        const uint64_t uu = sd_units_f32(buf[e], t);
+        if (uu == ~0ull) bad = 1;
+        else u += uu;
+      }
+      const uint64_t su = sd_units(S, t);
+      if (!__any(bad) && su != ~0ull) {
+        const uint64_t tot = wave_sum_u64(u) + su;
+        if (tot < SD_M53) {
+          S = sd_from_units(tot, t);
+          return;
+        }
+      }
+    }
     if (S.m == 0 || S.s) {  // from zero (or a negative sum): one addend exactly
       S = sd_add_f32(S, buf[start]);
       ++start;
       continue;
     }
     const int32_t E = S.e;
-    const int per = (len - start + 63) / 64;
-    const int lo = start + l * per, hi = min(lo + per, len);
     sd_map m = sd_map_identity();
     for (int e = lo; e < hi; ++e) m = sd_map_compose(m, sd_map_of(buf[e], E));
     for (int o = 1; o < 64; o <<= 1) {  // ordered inclusive scan
