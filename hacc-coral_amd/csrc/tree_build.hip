@@ -342,48 +342,48 @@ k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, const CmBlock *blk,
     for (int q = 0; q < 4; ++q) {
       if (done & (1 << q)) continue;
       for (int i = 0; i < cnt;) {  // uniform
-        // a run of blocks from i whose maps share i's grid: compose them in
-        // order (maps compose associatively) and apply once
-        const int32_t Ei = readlane_i32(mE[q], i);
-        const bool same = l >= i && l < cnt && mE[q] == Ei && !mbad[q];
-        const uint64_t sh = (uint64_t)__ballot(same) >> i;
-        const int run = (~sh == 0ull) ? 64 - i : (int)__builtin_ctzll(~sh);
-        if (run >= 2) {
+        if (S[q].m != 0 && !S[q].s) {
+          // ordered scan of the block maps from i built for S's binade; the
+          // longest prefix that keeps S in the binade is applied at once
+          const int32_t E = S[q].e;
           sd_map m = sd_map_identity();
-          if (l >= i && l < i + run) {
+          if (l >= i && l < cnt) {
             m.d0 = md0[q];
             m.d1 = md1[q];
+            m.bad = mbad[q] || mE[q] != E;
           }
-          const sd_map g = wave_compose_maps(m);
+          for (int o = 1; o < 64; o <<= 1) {
+            sd_map h;
+            h.d0 = (uint64_t)__shfl_up((unsigned long long)m.d0, o);
+            h.d1 = (uint64_t)__shfl_up((unsigned long long)m.d1, o);
+            h.bad = __shfl_up(m.bad, o);
+            if (l >= o) m = sd_map_compose(h, m);
+          }
           sd_t S2 = S[q];
-          if (sd_map_apply(&S2, g, Ei)) {
-            S[q] = S2;
-            i += run;
+          const bool ok = l >= i && l < cnt && sd_map_apply(&S2, m, E);
+          const uint64_t bal = ((uint64_t)__ballot(ok) >> i) | (cnt - i < 64 ? ~0ull << (cnt - i) : 0ull);
+          const int j = min(cnt, i + ((~bal == 0ull) ? 64 - i : (int)__builtin_ctzll(~bal)));
 #ifdef HSR_CM_STATS
-            if (l == 0) atomicAdd((unsigned long long *)&ctr->pad[0], 1ull);
+          if (l == 0) atomicAdd((unsigned long long *)&ctr->pad[0], 1ull);
 #endif
+          if (j > i) {
+            S[q].m = readlane_u64(S2.m, j - 1);
+            S[q].e = readlane_i32(S2.e, j - 1);
+            S[q].s = readlane_i32(S2.s, j - 1);
+            i = j;
             continue;
           }
         }
-        // block i alone: its map, else its addends chunk by chunk
-        sd_map g;
-        g.d0 = readlane_u64(md0[q], i);
-        g.d1 = readlane_u64(md1[q], i);
-        g.bad = readlane_i32(mbad[q], i);
+        // block i does not fit one grid: its addends, exactly, from LDS
 #ifdef HSR_CM_STATS
-        if (l == 0) atomicAdd((unsigned long long *)&ctr->pad[1], 1ull);
+        if (l == 0) atomicAdd((unsigned long long *)&ctr->pad[2], 1ull);
 #endif
-        if (!sd_map_apply(&S[q], g, Ei)) {
-#ifdef HSR_CM_STATS
-          if (l == 0) atomicAdd((unsigned long long *)&ctr->pad[2], 1ull);
-#endif
-          const CmBlock &B = blk[b0 + bb + i];
-          const int64_t start = B.start, len = B.len;
-          __syncthreads();
-          for (int k = l; k < len; k += 64) buf[k] = addend(q, X, Y, Z, M, start + k);
-          __syncthreads();
-          sd_block(S[q], buf, (int)len);
-        }
+        const CmBlock &B = blk[b0 + bb + i];
+        const int64_t start = B.start, len = B.len;
+        __syncthreads();
+        for (int k = l; k < len; k += 64) buf[k] = addend(q, X, Y, Z, M, start + k);
+        __syncthreads();
+        sd_block(S[q], buf, (int)len);
         ++i;
       }
     }
@@ -992,7 +992,7 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_DEGEN, P[cur ^ 1], Pfin);
     HSR_HIP(hipGetLastError());
 #ifdef HSR_CM_STATS
-    fprintf(stderr, "[cm] level %d nodes %lld blocks %lld composed %lld single %lld fallback %lld\n", depth,
+    fprintf(stderr, "[cm] level %d nodes %lld blocks %lld scans %lld single %lld fallback %lld\n", depth,
             (long long)nl, (long long)hc.cm_blocks, (long long)hc.pad[0], (long long)hc.pad[1],
             (long long)hc.pad[2]);
 #endif
