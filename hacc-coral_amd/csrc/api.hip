@@ -6,6 +6,7 @@
 // RCBForceTree's constructor does (RCBForceTree.cxx:336-450).
 #include "hsr_internal.h"
 
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -208,6 +209,82 @@ hsr_status build_common(hsr_ctx *c, int64_t nboxes, const int64_t *box_off,
   return HSR_OK;
 }
 
+// Particles::map1's step factor prefactor * tau in the reference's float
+// arithmetic (Particles.cxx:746-747): powf of pp by the double exponent
+// 1 + 1/alpha rounded to float, then 1.0 / (alpha * adot * pf) in double,
+// stored as float.
+float map1_factor(float pp, float tau, float adot, float alpha) {
+  const float pf = powf(pp, (float)(1.0 + 1.0 / (double)alpha));
+  const float prefactor = (float)(1.0 / (double)(alpha * adot * pf));
+  return prefactor * tau;
+}
+
+// Particles::map2's kick coefficient (Particles.cxx:1230-1233): float
+// gpscal^3 and pi = 4 atanf(1), the rest in double, stored as float.
+float map2_coeff(float gpscal, double fscal, double tau, double stepFraction) {
+  const float divscal = gpscal * gpscal * gpscal;
+  const float pi = (float)(4.0 * (double)atanf(1.0f));
+  return (float)((double)divscal / 4.0 / (double)pi * fscal * tau * stepFraction);
+}
+
+hsr_status sub_events(hsr_ctx *c) {
+  for (int k = 0; k < 6; ++k)
+    if (!c->sub.ev[k]) HSR_HIP(hipEventCreate(&c->sub.ev[k]));
+  return HSR_OK;
+}
+
+hsr_status check_count_grid(const char *who, int64_t count, const int32_t *ng) {
+  if (count < 0 || count >= (int64_t)INT32_MAX) {
+    g_err = std::string(who) + ": particle count must be in [0, 2^31)";
+    return HSR_ERR_ARG;
+  }
+  if (!ng || ng[0] < 1 || ng[1] < 1 || ng[2] < 1 || ng[0] > (1 << 24) || ng[1] > (1 << 24) ||
+      ng[2] > (1 << 24) ||
+      (int64_t)ng[0] * ng[1] * ng[2] >= (int64_t)INT32_MAX) {
+    g_err = std::string(who) + ": ng must be positive with ng[0]*ng[1]*ng[2] < 2^31 - 1";
+    return HSR_ERR_ARG;
+  }
+  return HSR_OK;
+}
+
+// Copy the present arrays among the ten into device buffers buf[0..9].
+hsr_status stage_in(hsr_ctx *c, DBuf *buf, int64_t n, float *const f[8], const int64_t *id,
+                    const uint16_t *mask, float *df[8], int64_t **did, uint16_t **dmask) {
+  hipStream_t st = c->stream;
+  const size_t nn = (size_t)std::max<int64_t>(n, 1);
+  for (int a = 0; a < 8; ++a) {
+    df[a] = nullptr;
+    if (!f[a]) continue;
+    HSR_HIP(dbuf_reserve(buf[a], sizeof(float) * nn));
+    df[a] = dptr<float>(buf[a]);
+    if (n) HSR_HIP(hipMemcpyAsync(df[a], f[a], sizeof(float) * n, hipMemcpyHostToDevice, st));
+  }
+  *did = nullptr;
+  *dmask = nullptr;
+  if (id) {
+    HSR_HIP(dbuf_reserve(buf[8], sizeof(int64_t) * nn));
+    *did = dptr<int64_t>(buf[8]);
+    if (n) HSR_HIP(hipMemcpyAsync(*did, id, sizeof(int64_t) * n, hipMemcpyHostToDevice, st));
+  }
+  if (mask) {
+    HSR_HIP(dbuf_reserve(buf[9], sizeof(uint16_t) * nn));
+    *dmask = dptr<uint16_t>(buf[9]);
+    if (n) HSR_HIP(hipMemcpyAsync(*dmask, mask, sizeof(uint16_t) * n, hipMemcpyHostToDevice, st));
+  }
+  return HSR_OK;
+}
+
+hsr_status stage_out(hsr_ctx *c, int64_t n, float *const f[8], int64_t *id, uint16_t *mask,
+                     float *const df[8], const int64_t *did, const uint16_t *dmask) {
+  if (!n) return HSR_OK;
+  hipStream_t st = c->stream;
+  for (int a = 0; a < 8; ++a)
+    if (f[a]) HSR_HIP(hipMemcpyAsync(f[a], df[a], sizeof(float) * n, hipMemcpyDeviceToHost, st));
+  if (id) HSR_HIP(hipMemcpyAsync(id, did, sizeof(int64_t) * n, hipMemcpyDeviceToHost, st));
+  if (mask) HSR_HIP(hipMemcpyAsync(mask, dmask, sizeof(uint16_t) * n, hipMemcpyDeviceToHost, st));
+  return HSR_OK;
+}
+
 }  // namespace
 
 void hsr_set_error(const std::string &msg) { g_err = msg; }
@@ -313,6 +390,11 @@ hsr_status hsr_destroy(hsr_ctx *c) {
   if (c->adj.graph) (void)hipGraphDestroy(c->adj.graph);
   for (DBuf *b : bufs) dbuf_free(*b);
   for (int k = 0; k < 10; ++k) dbuf_free(c->s.host_in[k]);
+  dbuf_free(c->sub.keys);
+  dbuf_free(c->sub.stage);
+  for (int k = 0; k < 10; ++k) dbuf_free(c->sub.arr[k]);
+  for (int k = 0; k < 6; ++k)
+    if (c->sub.ev[k]) (void)hipEventDestroy(c->sub.ev[k]);
   for (int k = 0; k < 8; ++k) (void)hipEventDestroy(c->ev[k]);
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
@@ -640,6 +722,9 @@ hsr_status hsr_get_stats(hsr_ctx *c, hsr_stats *st) {
   st->force_ms = c->force_ms;
   st->kernel_ms = c->kernel_ms;
   st->adjoint_ms = c->adjoint_ms;
+  st->drift_ms = c->sub.drift_ms;
+  st->resort_ms = c->sub.resort_ms;
+  st->subcycle_ms = c->sub.subcycle_ms;
   return HSR_OK;
 }
 
@@ -730,6 +815,166 @@ hsr_status hsr_get_tuning(hsr_ctx *c, hsr_tuning *t) {
     return HSR_ERR_ARG;
   }
   *t = c->tuning;
+  return HSR_OK;
+}
+
+
+hsr_status hsr_drift(hsr_ctx *c, int64_t count, float *x, float *y, float *z,
+                     const float *vx, const float *vy, const float *vz, float pp,
+                     float tau, float adot, float alpha, int ptr_flags) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (count < 0 || !x || !y || !z || !vx || !vy || !vz) {
+    g_err = "hsr_drift: null array or negative count";
+    return HSR_ERR_ARG;
+  }
+  if ((s = sub_events(c))) return s;
+  const float k = map1_factor(pp, tau, adot, alpha);
+  float *d[6] = {x, y, z, (float *)vx, (float *)vy, (float *)vz};
+  const bool host = !(ptr_flags & HSR_DEVICE_PTRS);
+  hipStream_t st = c->stream;
+  if (host) {
+    for (int a = 0; a < 6; ++a) {
+      HSR_HIP(dbuf_reserve(c->s.host_in[a], sizeof(float) * (size_t)std::max<int64_t>(count, 1)));
+      float *b = dptr<float>(c->s.host_in[a]);
+      if (count) HSR_HIP(hipMemcpyAsync(b, d[a], sizeof(float) * count, hipMemcpyHostToDevice, st));
+      d[a] = b;
+    }
+  }
+  HSR_HIP(hipEventRecord(c->sub.ev[0], st));
+  if ((s = drift_run(c, count, k, d[0], d[1], d[2], d[3], d[4], d[5]))) return s;
+  HSR_HIP(hipEventRecord(c->sub.ev[1], st));
+  if (host && count) {
+    HSR_HIP(hipMemcpyAsync(x, d[0], sizeof(float) * count, hipMemcpyDeviceToHost, st));
+    HSR_HIP(hipMemcpyAsync(y, d[1], sizeof(float) * count, hipMemcpyDeviceToHost, st));
+    HSR_HIP(hipMemcpyAsync(z, d[2], sizeof(float) * count, hipMemcpyDeviceToHost, st));
+  }
+  HSR_HIP(hipStreamSynchronize(st));
+  float ms = 0.f;
+  HSR_HIP(hipEventElapsedTime(&ms, c->sub.ev[0], c->sub.ev[1]));
+  c->sub.drift_ms = ms;
+  return HSR_OK;
+}
+
+hsr_status hsr_resort(hsr_ctx *c, int64_t count, const int32_t ng[3], float *x,
+                      float *y, float *z, float *vx, float *vy, float *vz,
+                      float *mass, float *phi, int64_t *id, uint16_t *mask,
+                      int64_t *inside, int ptr_flags) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (!x || !y || !z || !inside) {
+    g_err = "hsr_resort: null positions or output";
+    return HSR_ERR_ARG;
+  }
+  if ((s = check_count_grid("hsr_resort", count, ng))) return s;
+  if ((s = sub_events(c))) return s;
+  float *f[8] = {x, y, z, vx, vy, vz, mass, phi};
+  float *df[8];
+  int64_t *did = id;
+  uint16_t *dmask = mask;
+  const bool host = !(ptr_flags & HSR_DEVICE_PTRS);
+  if (host) {
+    if ((s = stage_in(c, c->s.host_in, count, f, id, mask, df, &did, &dmask))) return s;
+  } else {
+    for (int a = 0; a < 8; ++a) df[a] = f[a];
+  }
+  HSR_HIP(hipEventRecord(c->sub.ev[2], c->stream));
+  if ((s = resort_run(c, count, ng, df, did, dmask, inside))) return s;
+  HSR_HIP(hipEventRecord(c->sub.ev[3], c->stream));
+  if (host && (s = stage_out(c, count, f, id, mask, df, did, dmask))) return s;
+  HSR_HIP(hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  HSR_HIP(hipEventElapsedTime(&ms, c->sub.ev[2], c->sub.ev[3]));
+  c->sub.resort_ms = ms;
+  return HSR_OK;
+}
+
+hsr_status hsr_subcycle(hsr_ctx *c, const hsr_subcycle_params *sp, int64_t count,
+                        float *x, float *y, float *z, float *vx, float *vy,
+                        float *vz, float *mass, float *phi, int64_t *id,
+                        uint16_t *mask, int64_t *inside, int ptr_flags) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (!sp || !x || !y || !z || !vx || !vy || !vz || !mass || !inside) {
+    g_err = "hsr_subcycle: null parameters, positions, velocities, masses or output";
+    return HSR_ERR_ARG;
+  }
+  if (sp->nsub < 1) {
+    g_err = "hsr_subcycle: nsub must be >= 1";
+    return HSR_ERR_ARG;
+  }
+  if ((s = check_count_grid("hsr_subcycle", count, sp->ng))) return s;
+  if ((s = sub_events(c))) return s;
+  const auto t0 = std::chrono::steady_clock::now();
+  hipStream_t st = c->stream;
+  float *f[8] = {x, y, z, vx, vy, vz, mass, phi};
+  float *df[8];
+  int64_t *did = id;
+  uint16_t *dmask = mask;
+  const bool host = !(ptr_flags & HSR_DEVICE_PTRS);
+  if (host) {
+    if ((s = stage_in(c, c->sub.arr, count, f, id, mask, df, &did, &dmask))) return s;
+  } else {
+    for (int a = 0; a < 8; ++a) df[a] = f[a];
+  }
+  // Particles::subCycle (Particles.cxx:1176-1201): the TimeStepper is not
+  // advanced inside the cycle, so every sub-step uses the same factors.
+  const double stepFraction = 1.0 / sp->nsub;
+  const float k = map1_factor((float)sp->pp, (float)(stepFraction * sp->tau2), (float)sp->adot,
+                              sp->alpha);
+  const float cc = map2_coeff(sp->gpscal, sp->fscal, sp->tau, stepFraction);
+  // map2's boxes (Particles.cxx:1210-1228)
+  const int ngmax = std::max(std::max(sp->ng[0], sp->ng[1]), sp->ng[2]);
+  const float tmin[3] = {0.f, 0.f, 0.f};
+  const float tmax[3] = {(float)(1.0 * ngmax), (float)(1.0 * ngmax), (float)(1.0 * ngmax)};
+  float flo[3], fhi[3];
+  for (int d = 0; d < 3; ++d) {
+    flo[d] = sp->edge;
+    fhi[d] = (float)(1.0 * sp->ng[d] - sp->edge);
+  }
+  double drift_ms = 0, resort_ms = 0;
+  int64_t np_last = -1;
+  auto timed = [&](int e0, int e1, double &acc) -> hsr_status {
+    float ms = 0.f;
+    HSR_HIP(hipEventElapsedTime(&ms, c->sub.ev[e0], c->sub.ev[e1]));
+    acc += ms;
+    return HSR_OK;
+  };
+  for (int step = 0; step < sp->nsub; ++step) {
+    if (!sp->skip_stream) {
+      HSR_HIP(hipEventRecord(c->sub.ev[0], st));
+      if ((s = drift_run(c, count, k, df[0], df[1], df[2], df[3], df[4], df[5]))) return s;
+      HSR_HIP(hipEventRecord(c->sub.ev[1], st));
+    }
+    if (!sp->skip_kick) {
+      HSR_HIP(hipEventRecord(c->sub.ev[2], st));
+      if ((s = resort_run(c, count, sp->ng, df, did, dmask, &np_last))) return s;  // synchronizes
+      HSR_HIP(hipEventRecord(c->sub.ev[3], st));
+      if ((s = fill_run(c, count, 1.0f, df[6]))) return s;
+      if ((s = hsr_build(c, np_last, df[0], df[1], df[2], df[3], df[4], df[5], df[6], df[7], did,
+                         dmask, tmin, tmax, HSR_DEVICE_PTRS)))
+        return s;
+      if ((s = hsr_lists(c, flo, fhi))) return s;
+      if ((s = hsr_kick(c, cc))) return s;
+    }
+    if (!sp->skip_stream) {
+      HSR_HIP(hipEventRecord(c->sub.ev[4], st));
+      if ((s = drift_run(c, count, k, df[0], df[1], df[2], df[3], df[4], df[5]))) return s;
+      HSR_HIP(hipEventRecord(c->sub.ev[5], st));
+    }
+    HSR_HIP(hipStreamSynchronize(st));
+    if (!sp->skip_stream) {
+      if ((s = timed(0, 1, drift_ms)) || (s = timed(4, 5, drift_ms))) return s;
+    }
+    if (!sp->skip_kick && (s = timed(2, 3, resort_ms))) return s;
+  }
+  if (host && (s = stage_out(c, count, f, id, mask, df, did, dmask))) return s;
+  HSR_HIP(hipStreamSynchronize(st));
+  *inside = np_last;
+  c->sub.drift_ms = drift_ms;
+  c->sub.resort_ms = resort_ms;
+  c->sub.subcycle_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return HSR_OK;
 }
 

@@ -153,6 +153,16 @@ struct Scratch {
   DBuf host_in[10];
 };
 
+// Buffers of the sub-cycle steps (drift, resort) and the device-resident
+// particle arrays hsr_subcycle keeps for host-pointer callers.
+struct SubState {
+  DBuf keys;       // cell keys and indices of the sort (16 B per particle)
+  DBuf stage;      // gather staging of the ten arrays
+  DBuf arr[10];    // x y z vx vy vz mass phi id mask on the device
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  double drift_ms = 0, resort_ms = 0, subcycle_ms = 0;
+};
+
 struct hsr_ctx {
   hsr_params p;
   ForceLawDev law;
@@ -165,6 +175,7 @@ struct hsr_ctx {
   ListState lists;
   AdjState adj;
   Scratch s;
+  SubState sub;
   // caller arrays of the last build (for hsr_kick)
   float *user_v[3] = {nullptr, nullptr, nullptr};
   float *user_mass = nullptr;
@@ -213,3 +224,8 @@ hsr_status adjoint_run(hsr_ctx *c, const float *lx, const float *ly,
 hsr_status forces_f64_run(hsr_ctx *c, double *ax, double *ay, double *az, double *mom);
 hsr_status leaf_momentum_run(hsr_ctx *c, const float *ax, const float *ay, const float *az,
                              double *mom);
+hsr_status drift_run(hsr_ctx *c, int64_t n, float k, float *x, float *y, float *z,
+                     const float *vx, const float *vy, const float *vz);
+hsr_status resort_run(hsr_ctx *c, int64_t n, const int32_t ng[3], float *f[8],
+                      int64_t *id, uint16_t *mask, int64_t *inside);
+hsr_status fill_run(hsr_ctx *c, int64_t n, float v, float *a);

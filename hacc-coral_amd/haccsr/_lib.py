@@ -27,7 +27,8 @@ EXPORTS = [
     "hsr_get_stream", "hsr_build", "hsr_lists", "hsr_forces", "hsr_kick",
     "hsr_adjoint", "hsr_get_stats", "hsr_get_nodes", "hsr_get_perm",
     "hsr_get_lists", "hsr_build_many", "hsr_lists_many", "hsr_set_tuning",
-    "hsr_get_tuning", "hsr_forces_f64", "hsr_leaf_momentum",
+    "hsr_get_tuning", "hsr_forces_f64", "hsr_leaf_momentum", "hsr_drift",
+    "hsr_resort", "hsr_subcycle",
 ]
 
 
@@ -66,7 +67,18 @@ class Stats(C.Structure):
                 ("max_depth", C.c_int64), ("build_ms", C.c_double),
                 ("lists_ms", C.c_double), ("force_ms", C.c_double),
                 ("kernel_ms", C.c_double), ("adjoint_ms", C.c_double),
-                ("evaluated_pairs", C.c_int64)]
+                ("evaluated_pairs", C.c_int64), ("drift_ms", C.c_double),
+                ("resort_ms", C.c_double), ("subcycle_ms", C.c_double)]
+
+
+class SubcycleParams(C.Structure):
+    """hsr_subcycle_params: Particles' sub-cycle settings and the
+    TimeStepper values map1/map2 read (Particles.cxx:1176-1233)."""
+    _fields_ = [("nsub", C.c_int), ("skip_stream", C.c_int), ("skip_kick", C.c_int),
+                ("ng", C.c_int32 * 3), ("alpha", C.c_float), ("gpscal", C.c_float),
+                ("edge", C.c_float), ("reserved0", C.c_float), ("pp", C.c_double),
+                ("tau", C.c_double), ("tau2", C.c_double), ("adot", C.c_double),
+                ("fscal", C.c_double)]
 
 
 class Tuning(C.Structure):
@@ -87,6 +99,14 @@ def load(path: str | None = None):
     p = path or os.environ.get("HACCSR_LIB") or LIB_PATH
     if not os.path.exists(p):
         raise HsrError(f"{p} is missing: build it with `make -C hacc-coral_amd`")
+    # PyTorch ships its own HIP runtime.  Loaded first, its symbols are the
+    # ones libhaccsr.so binds to, so the process has one runtime and device
+    # tensors' pointers are valid here; loaded after the library, torch finds
+    # no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(p)
     vp, i64, f32p = C.c_void_p, C.c_int64, C.c_void_p
     L.hsr_last_error.restype = C.c_char_p
@@ -103,6 +123,9 @@ def load(path: str | None = None):
     L.hsr_adjoint.argtypes = [vp] + [vp] * 6 + [C.c_int]
     L.hsr_forces_f64.argtypes = [vp] + [vp] * 4 + [C.c_int]
     L.hsr_leaf_momentum.argtypes = [vp] + [vp] * 4 + [C.c_int]
+    L.hsr_drift.argtypes = [vp, i64] + [vp] * 6 + [C.c_float] * 4 + [C.c_int]
+    L.hsr_resort.argtypes = [vp, i64, vp] + [vp] * 10 + [vp, C.c_int]
+    L.hsr_subcycle.argtypes = [vp, C.POINTER(SubcycleParams), i64] + [vp] * 10 + [vp, C.c_int]
     L.hsr_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.hsr_get_nodes.argtypes = [vp, vp, i64]
     L.hsr_get_perm.argtypes = [vp, vp, i64]
@@ -259,6 +282,40 @@ class Context:
         check(self.L.hsr_leaf_momentum(self.h, _ptr(ax), _ptr(ay), _ptr(az), _ptr(leaf_mom), fl),
               "hsr_leaf_momentum")
         return leaf_mom
+
+    # -- sub-cycle
+    def drift(self, x, y, z, vx, vy, vz, pp, tau, adot, alpha, device_ptrs=False):
+        """Particles::map1 in place on x, y, z (float32 arrays or tensors)."""
+        n = int(x.shape[0])
+        fl = HSR_DEVICE_PTRS if device_ptrs else HSR_HOST_PTRS
+        check(self.L.hsr_drift(self.h, n, _ptr(x), _ptr(y), _ptr(z), _ptr(vx), _ptr(vy),
+                               _ptr(vz), C.c_float(pp), C.c_float(tau), C.c_float(adot),
+                               C.c_float(alpha), fl), "hsr_drift")
+
+    def resort(self, ng, x, y, z, vx=None, vy=None, vz=None, mass=None, phi=None,
+               ids=None, mask=None, device_ptrs=False) -> int:
+        """Particles::resortParticles in place; returns m_Np_last."""
+        n = int(x.shape[0])
+        g = np.ascontiguousarray(ng, np.int32)
+        inside = C.c_int64(0)
+        fl = HSR_DEVICE_PTRS if device_ptrs else HSR_HOST_PTRS
+        check(self.L.hsr_resort(self.h, n, g.ctypes.data, _ptr(x), _ptr(y), _ptr(z),
+                                _ptr(vx), _ptr(vy), _ptr(vz), _ptr(mass), _ptr(phi),
+                                _ptr(ids), _ptr(mask), C.byref(inside), fl), "hsr_resort")
+        return int(inside.value)
+
+    def subcycle(self, params: "SubcycleParams", x, y, z, vx, vy, vz, mass, phi=None,
+                 ids=None, mask=None, device_ptrs=False) -> int:
+        """Particles::subCycle with map2's RCB monopole kick; returns the
+        particle count of the last tree (m_Np_last, -1 without kicks)."""
+        n = int(x.shape[0])
+        inside = C.c_int64(0)
+        fl = HSR_DEVICE_PTRS if device_ptrs else HSR_HOST_PTRS
+        check(self.L.hsr_subcycle(self.h, C.byref(params), n, _ptr(x), _ptr(y), _ptr(z),
+                                  _ptr(vx), _ptr(vy), _ptr(vz), _ptr(mass), _ptr(phi),
+                                  _ptr(ids), _ptr(mask), C.byref(inside), fl), "hsr_subcycle")
+        self.n = int(inside.value) if inside.value >= 0 else self.n
+        return int(inside.value)
 
     # -- introspection
     def stats(self) -> Stats:

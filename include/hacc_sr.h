@@ -16,6 +16,11 @@
  *       = device centre of mass inside hsr_build
  *   extern "C" Step16_int(...)                RCBForceTree.cxx:572
  *       = device leaf kernel inside hsr_forces / hsr_kick
+ * and its caller in the sub-cycle (src/cpu):
+ *   Particles::map1 / resortParticles         Particles.cxx:732-758, 402-488
+ *       = hsr_drift / hsr_resort
+ *   Particles::subCycle + map2 (RCB monopole) Particles.cxx:1176-1280
+ *       = hsr_subcycle
  */
 #ifndef HACC_SR_H
 #define HACC_SR_H
@@ -141,6 +146,10 @@ typedef struct {
   double adjoint_ms;      /* device time of the last hsr_adjoint */
   int64_t evaluated_pairs;/* pairs the last force/kick evaluated: list pairs
                              minus those of entries culled beyond the cutoff */
+  double drift_ms;        /* device time of the last hsr_drift (hsr_subcycle:
+                             all its drifts) */
+  double resort_ms;       /* the same for hsr_resort */
+  double subcycle_ms;     /* host wall time of the last hsr_subcycle */
 } hsr_stats;
 
 hsr_status hsr_get_stats(hsr_ctx *ctx, hsr_stats *st);
@@ -186,6 +195,55 @@ hsr_status hsr_forces_f64(hsr_ctx *ctx, double *ax, double *ay, double *az,
                           double *leaf_mom, int ptr_flags);
 hsr_status hsr_leaf_momentum(hsr_ctx *ctx, const float *ax, const float *ay,
                              const float *az, double *leaf_mom, int ptr_flags);
+
+/* ------------------------------------------------------------ sub-cycle */
+/* The drift of the sub-cycle, Particles::map1(pp, tau, adot)
+ * (Particles.cxx:732-758): x += prefactor * tau * v over `count` particles
+ * with prefactor = 1 / (alpha * adot * powf(pp, 1 + 1/alpha)) evaluated on
+ * the host as there (alpha = m_alpha) and every element rounded as the
+ * reference's float expression x + (prefactor * tau) * v. */
+hsr_status hsr_drift(hsr_ctx *ctx, int64_t count, float *x, float *y, float *z,
+                     const float *vx, const float *vy, const float *vz,
+                     float pp, float tau, float adot, float alpha,
+                     int ptr_flags);
+
+/* Particles::resortParticles (Particles.cxx:402-488): stable counting sort
+ * of the particles by unit cell, cell = (floor x * ng[1] + floor y) * ng[2] +
+ * floor z for particles inside [0, ng) per dimension, all others after the
+ * last cell, input order kept within a cell.  Permutes the ten arrays in
+ * place (vx .. mask may be NULL) and writes the number of particles inside
+ * the grid (m_Np_last) to *inside.  ng[0]*ng[1]*ng[2] must stay below 2^31. */
+hsr_status hsr_resort(hsr_ctx *ctx, int64_t count, const int32_t ng[3],
+                      float *x, float *y, float *z, float *vx, float *vy,
+                      float *vz, float *mass, float *phi, int64_t *id,
+                      uint16_t *mask, int64_t *inside, int ptr_flags);
+
+/* Particles::subCycle with its short-range kick Particles::map2
+ * (Particles.cxx:1176-1201, 1205-1280; RCB monopole tree): for each of nsub
+ * sub-steps, drift by stepFraction * tau2 (map1), then resort, set every
+ * mass to 1, build the tree over the particles inside the grid with the tree
+ * box [0, max ng) and list and kick the leaves inside the force box
+ * [edge, ng - edge) with c = gpscal^3/(4 pi) * fscal * tau * stepFraction,
+ * then drift again.  The particles stay on the device for the whole cycle
+ * (host arrays are copied in once and out once).  Tree and kernel parameters
+ * (nd, fsrrmax, rsm, oa, law) are the context's. */
+typedef struct {
+  int nsub;              /* m_nsub (stepFraction = 1.0 / nsub) */
+  int skip_stream;       /* m_skipStreamQ: no drifts */
+  int skip_kick;         /* m_skipKickSRQ: no map2 */
+  int32_t ng[3];         /* Domain::ng_local_total */
+  float alpha;           /* m_alpha */
+  float gpscal;          /* m_gpscal */
+  float edge;            /* m_edge */
+  float reserved0;
+  double pp, tau, tau2, adot, fscal;  /* the TimeStepper's values */
+} hsr_subcycle_params;
+
+hsr_status hsr_subcycle(hsr_ctx *ctx, const hsr_subcycle_params *sp,
+                        int64_t count, float *x, float *y, float *z,
+                        float *vx, float *vy, float *vz, float *mass,
+                        float *phi, int64_t *id, uint16_t *mask,
+                        int64_t *inside, int ptr_flags);
 
 /* ------------------------------------------------------------- tuning */
 typedef struct {

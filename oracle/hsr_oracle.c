@@ -762,6 +762,37 @@ int64_t hsro_cell_sort_perm(int64_t n, const float *x, const float *y,
   return inside;
 }
 
+/* ------------------------------------------------------------ map1/map2 */
+/* Particles.cxx:746-747: pf = powf(pp, 1.0 + 1.0/m_alpha) (double exponent
+ * passed to powf as float); prefactor = 1.0 / (m_alpha * adot * pf) with a
+ * float product and a double quotient stored as float. */
+float hsro_map1_factor(float pp, float tau, float adot, float alpha) {
+  float pf = powf(pp, (float)(1.0 + 1.0 / (double)alpha));
+  float prefactor = (float)(1.0 / (double)(alpha * adot * pf));
+  return prefactor * tau;
+}
+
+/* Particles.cxx:753-757: x[i] = x[i] + prefactor * tau * vx[i]. */
+void hsro_drift(int64_t n, float *x, float *y, float *z, const float *vx,
+                const float *vy, const float *vz, float factor) {
+  for (int64_t i = 0; i < n; ++i) {
+    x[i] = x[i] + factor * vx[i];
+    y[i] = y[i] + factor * vy[i];
+    z[i] = z[i] + factor * vz[i];
+  }
+}
+
+/* Particles.cxx:1230-1233: divscal = gpscal^3 and pi = 4.0*atanf(1.0) stored
+ * as float; c = divscal/4.0/pi*fscal*tau*stepFraction in double, stored as
+ * float. */
+float hsro_map2_coeff(float gpscal, double fscal, double tau,
+                      double step_fraction) {
+  float divscal = gpscal * gpscal * gpscal;
+  float pi = (float)(4.0 * (double)atanf(1.0f));
+  return (float)((double)divscal / 4.0 / (double)pi * fscal * tau *
+                 step_fraction);
+}
+
 /* -------------------------------------------------------------- adjoint */
 /* g(r2) and g'(r2) of the Step16 law.  With rsm > 0 the pair term
  * m g(r2 + rsm2) d is smooth through d = 0, so a coincident pair (r2 == 0,

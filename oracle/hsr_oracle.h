@@ -13,6 +13,9 @@
  *   hsro_leaf_order      RCBForceTree.cxx:1151-1197
  *   hsro_list            RCBForceTree.cxx:924-1140
  *   hsro_force_*         RCBForceTree.cxx:575-620 / BGQStep16.c:167-187
+ *   hsro_cell_sort_perm  Particles.cxx:402-488 (resortParticles)
+ *   hsro_map1_factor, hsro_drift   Particles.cxx:732-758 (map1)
+ *   hsro_map2_coeff      Particles.cxx:1230-1233 (map2)
  *
  * Parity pins (see tests/test_oracle_*.py):
  *   cm            bit-exact against the reference's own BGQCM.c compiled
@@ -22,6 +25,9 @@
  *   tree          tiling/tight-bbox/split-rule/leaf-size invariants.
  *   lists         mass-accounting invariant and oa->0 direct-sum limit.
  *   kernels       FP64 brute force over the same lists.
+ *   cell sort     numpy's stable argsort of the cell keys.
+ *   map1, map2    closed forms (prefactor 1 exactly, double-precision
+ *                 formula to float rounding, two half drifts vs one).
  */
 #ifndef HSR_ORACLE_H
 #define HSR_ORACLE_H
@@ -176,6 +182,17 @@ double hsro_objective_f64(const hsro_params *p, const hsro_node *nodes,
 int64_t hsro_cell_sort_perm(int64_t n, const float *x, const float *y,
                             const float *z, const int32_t ng[3],
                             int64_t *perm);
+
+/* Particles::map1 (Particles.cxx:732-758): the step factor prefactor * tau
+ * in the reference's float/double mix, and the drift x += factor * v with
+ * every element rounded as x + (factor * v). */
+float hsro_map1_factor(float pp, float tau, float adot, float alpha);
+void hsro_drift(int64_t n, float *x, float *y, float *z, const float *vx,
+                const float *vy, const float *vz, float factor);
+
+/* Particles::map2's kick coefficient (Particles.cxx:1230-1233). */
+float hsro_map2_coeff(float gpscal, double fscal, double tau,
+                      double step_fraction);
 
 #ifdef __cplusplus
 }

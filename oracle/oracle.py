@@ -92,6 +92,11 @@ def lib():
         L.hsro_force_f64d.argtypes = [C.POINTER(Params), nodes, f64, f64, f64, f64, i64, C.c_int64, i64, i64, i64, f64, f64, f64]
         L.hsro_cell_sort_perm.argtypes = [C.c_int64, f32, f32, f32, P(np.int32, flags="C"), i64]
         L.hsro_cell_sort_perm.restype = C.c_int64
+        L.hsro_map1_factor.argtypes = [C.c_float] * 4
+        L.hsro_map1_factor.restype = C.c_float
+        L.hsro_drift.argtypes = [C.c_int64] + [f32] * 6 + [C.c_float]
+        L.hsro_map2_coeff.argtypes = [C.c_float, C.c_double, C.c_double, C.c_double]
+        L.hsro_map2_coeff.restype = C.c_float
         _lib = L
     return _lib
 
@@ -324,6 +329,77 @@ def cell_sort_perm(x, y, z, ng):
                                        np.ascontiguousarray(z, np.float32),
                                        np.asarray(ng, np.int32), perm)
     return perm, inside
+
+
+def map1_factor(pp, tau, adot, alpha) -> float:
+    return float(lib().hsro_map1_factor(C.c_float(pp), C.c_float(tau), C.c_float(adot),
+                                        C.c_float(alpha)))
+
+
+def drift(x, y, z, vx, vy, vz, factor):
+    """Particles::map1's update on copies; returns (x, y, z)."""
+    X, Y, Z = (np.ascontiguousarray(a, np.float32).copy() for a in (x, y, z))
+    V = [np.ascontiguousarray(a, np.float32) for a in (vx, vy, vz)]
+    lib().hsro_drift(X.shape[0], X, Y, Z, V[0], V[1], V[2], C.c_float(factor))
+    return X, Y, Z
+
+
+def map2_coeff(gpscal, fscal, tau, step_fraction) -> float:
+    return float(lib().hsro_map2_coeff(C.c_float(gpscal), fscal, tau, step_fraction))
+
+
+POSVEL = ("x", "y", "z", "vx", "vy", "vz", "mass", "phi")
+
+
+def resort(state: dict, ng):
+    """Particles::resortParticles on a dict of the ten arrays (copies);
+    returns (new state, m_Np_last)."""
+    perm, inside = cell_sort_perm(state["x"], state["y"], state["z"], ng)
+    out = {}
+    for k, a in state.items():
+        b = np.empty_like(a)
+        b[perm] = a
+        out[k] = b
+    return out, int(inside)
+
+
+def substep(state: dict, sp: dict, tp_kw: dict):
+    """One sub-step of Particles::subCycle with map2's RCB monopole kick
+    (Particles.cxx:1188-1198, 1205-1280) on copies of the ten arrays:
+    drift, resort, masses set to 1, tree over the particles inside the
+    grid, lists and Step16 kick of the force box, drift.  sp holds nsub,
+    ng, alpha, gpscal, edge, pp, tau, tau2, adot, fscal; tp_kw the tree's
+    fsrrmax, rsm, oa, nd.  Returns (state, m_Np_last, tree, lists)."""
+    sf = 1.0 / sp["nsub"]
+    k = map1_factor(np.float32(sp["pp"]), np.float32(sf * sp["tau2"]),
+                    np.float32(sp["adot"]), sp["alpha"])
+    c = map2_coeff(sp["gpscal"], sp["fscal"], sp["tau"], sf)
+    st = {a: b.copy() for a, b in state.items()}
+    st["x"], st["y"], st["z"] = drift(st["x"], st["y"], st["z"], st["vx"], st["vy"], st["vz"], k)
+    st, inside = resort(st, sp["ng"])
+    st["mass"][:] = 1.0
+    ng = sp["ng"]
+    gmax = np.float32(max(ng))
+    edge = np.float32(sp["edge"])
+    tp = TreeParams(np.zeros(3, np.float32), np.full(3, gmax, np.float32),
+                    np.full(3, edge, np.float32),
+                    np.array([np.float32(1.0 * g - float(edge)) for g in ng], np.float32),
+                    **tp_kw)
+    sl = slice(0, inside)
+    tree = build_tree(tp, st["x"][sl], st["y"][sl], st["z"][sl], st["mass"][sl],
+                      st["vx"][sl], st["vy"][sl], st["vz"][sl], st["phi"][sl],
+                      st["id"][sl], st["mask"][sl])
+    for a in ("x", "y", "z", "vx", "vy", "vz", "mass", "phi", "id", "mask"):
+        st[a][sl] = getattr(tree, a)
+    lists = build_lists(tp, tree)
+    fx, fy, fz = forces_step16(tp, tree, lists)
+    cf = np.float32(c)
+    for a, f in (("vx", fx), ("vy", fy), ("vz", fz)):
+        v = st[a][sl]
+        v += cf * f   # zero force outside the listed leaves leaves v unchanged
+        st[a][sl] = v
+    st["x"], st["y"], st["z"] = drift(st["x"], st["y"], st["z"], st["vx"], st["vy"], st["vz"], k)
+    return st, inside, tree, lists
 
 
 def _d(a):

@@ -301,6 +301,79 @@ def test_cell_sort_is_stable_counting_sort():
     assert inside == ok.sum()
 
 
+def test_map1_drift_closed_forms():
+    """Particles::map1 restated: with alpha = adot = pp = 1 the prefactor is
+    exactly 1 and the drift is x + tau * v in float; elsewhere the step factor
+    follows 1 / (alpha adot pp^(1 + 1/alpha)) * tau to float rounding (an
+    exponent of 1/alpha or 1 + alpha, or a dropped adot, fails this), and two
+    drifts of tau/2 agree with one of tau to rounding."""
+    rng = np.random.default_rng(21)
+    n = 4097
+    x, y, z = (rng.uniform(-4, 60, n).astype(np.float32) for _ in range(3))
+    vx, vy, vz = (rng.normal(0, 3, n).astype(np.float32) for _ in range(3))
+    assert O.map1_factor(1.0, 0.37, 1.0, 1.0) == float(np.float32(0.37))
+    X, Y, Z = O.drift(x, y, z, vx, vy, vz, O.map1_factor(1.0, 0.37, 1.0, 1.0))
+    t = np.float32(0.37)
+    assert np.array_equal(X, x + t * vx) and np.array_equal(Z, z + t * vz)
+    for pp, tau, adot, alpha in [(0.5, 0.02, 1.3, 1.0), (0.13, 0.004, 4.2, 1.5),
+                                 (0.9, 0.1, 0.7, 0.75), (2.0, 0.01, 0.2, 2.0)]:
+        want = tau / (alpha * adot * pp ** (1.0 + 1.0 / alpha))
+        got = O.map1_factor(pp, tau, adot, alpha)
+        assert abs(got - want) <= 4e-7 * abs(want), (pp, tau, adot, alpha)
+    k = O.map1_factor(0.5, 0.02, 1.3, 1.0)
+    h = O.map1_factor(0.5, 0.01, 1.3, 1.0)
+    X1, Y1, Z1 = O.drift(x, y, z, vx, vy, vz, k)
+    X2, Y2, Z2 = O.drift(*O.drift(x, y, z, vx, vy, vz, h), vx, vy, vz, h)
+    assert np.allclose(X1, X2, rtol=0, atol=4e-6 * 64)
+
+
+def test_map2_coeff_closed_form():
+    """map2's kick coefficient gpscal^3 / (4 pi) * fscal * tau * stepFraction
+    to float rounding (a dropped or doubled factor fails)."""
+    import math
+    for g, fs, tau, sf in [(1.0, 1.0, 1.0, 1.0), (2.0, 1.7, 0.03, 0.2),
+                           (0.5, 0.8, 0.11, 1.0 / 3), (1.25, 2.3, 0.004, 0.5)]:
+        want = g ** 3 / (4 * math.pi) * fs * tau * sf
+        got = O.map2_coeff(g, fs, tau, sf)
+        assert abs(got - want) <= 3e-7 * want, (g, fs, tau, sf)
+
+
+def test_host_step_factors_match_oracle():
+    """The library's Python mirror of map1/map2's host arithmetic agrees with
+    the C restatement bit for bit (the C ABI computes the same in C++)."""
+    from haccsr import particles as P
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        pp, tau, adot, alpha = rng.uniform(0.05, 3.0), rng.uniform(1e-4, 0.2), \
+            rng.uniform(0.1, 5.0), rng.choice([0.5, 1.0, 1.5, 2.0])
+        assert P.map1_factor(pp, tau, adot, alpha) == \
+            O.map1_factor(np.float32(pp), np.float32(tau), np.float32(adot), alpha)
+        g, fs, sf = rng.uniform(0.5, 3.0), rng.uniform(0.1, 3.0), 1.0 / rng.integers(1, 9)
+        assert P.map2_coeff(g, fs, tau, sf) == O.map2_coeff(np.float32(g), fs, tau, sf)
+
+
+def test_resort_oracle_moves_all_arrays_and_keeps_order():
+    """oracle.resort applies the cell permutation to all ten arrays; within a
+    cell the input order is kept and the tail holds every particle outside
+    the grid."""
+    rng = np.random.default_rng(12)
+    n, ng = 3000, [5, 6, 7]
+    st = {a: rng.uniform(-1, 8, n).astype(np.float32) for a in O.POSVEL}
+    st["id"] = np.arange(n, dtype=np.int64)
+    st["mask"] = rng.integers(0, 60000, n).astype(np.uint16)
+    out, inside = O.resort(st, ng)
+    fx, fy, fz = (np.floor(out[a]) for a in ("x", "y", "z"))
+    ok = (fx >= 0) & (fx < 5) & (fy >= 0) & (fy < 6) & (fz >= 0) & (fz < 7)
+    assert ok[:inside].all() and not ok[inside:].any()
+    cell = ((fx * 6 + fy) * 7 + fz)[:inside]
+    assert np.all(np.diff(cell) >= 0)
+    same = np.diff(cell) == 0
+    assert np.all(np.diff(out["id"][:inside])[same] > 0)
+    assert np.all(np.diff(out["id"][inside:]) > 0)
+    for a in list(O.POSVEL) + ["mask"]:
+        assert np.array_equal(out[a], st[a][out["id"]])
+
+
 # ------------------------------------------------- drop-in host force laws
 def test_dropin_forcelaw_host_matches_reference_bitwise():
     """hacc-coral_amd/host/ForceLaw.cxx (the drop-in's host ForceLaw
