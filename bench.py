@@ -54,6 +54,8 @@ def parse():
                     help="timed hsr_adjoint calls on the last step's tree (0 = skip)")
     ap.add_argument("--adjoint-mode", type=int, default=0,
                     help="hsr_tuning.adjoint_mode: 0 automatic, 1 fused, 2 two-pass")
+    ap.add_argument("--subcycle-nsub", type=int, default=5,
+                    help="sub-steps of the device-resident sub-cycle leg (0: skip it)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run one plain device-to-device copy of BYTES (the "
                          "reference kernel for scaling FETCH_SIZE/WRITE_SIZE counters)")
@@ -65,10 +67,16 @@ def log(msg):
 
 
 def workload_name(cfg, load, nd):
-    names = {"A": "64^3 Plummer mix (70% uniform, 30% in 20 clumps)",
-             "B": "256^3 clustered (GRF P(k)~k^-2, Zel'dovich rms 2, 500 NFW halos)",
-             "D": "128^3 clustered (GRF P(k)~k^-2, Zel'dovich rms 2, 62 NFW halos)"}
-    return (f"{names.get(cfg, cfg)}, box {load.box:g}, +{load.n - load.n_alive} ghosts "
+    ns = round(load.n_alive ** (1 / 3))
+    halos = {"B": 500 * (ns / 256) ** 3, "C": 2000 * (ns / 512) ** 3, "D": 500 * (ns / 256) ** 3}
+    if cfg == "A":
+        name = f"{ns}^3 Plummer mix (70% uniform, 30% in 20 clumps)"
+    elif cfg in halos:
+        name = (f"{ns}^3 clustered (GRF P(k)~k^-2, Zel'dovich rms 2, "
+                f"{max(1, round(halos[cfg]))} NFW halos)")
+    else:
+        name = cfg
+    return (f"{name}, box {load.box:g}, +{load.n - load.n_alive} ghosts "
             f"(shell {load.overload:g}), eps 0.1, fsrrmax 3.12, oa 0.1, nd {nd}")
 
 
@@ -152,6 +160,59 @@ def adjoint_leg(ctx, n, dev, reps, fw_stats):
             "ratio": adj_ms / fw_ms, "first_call_ms_with_transpose": first_ms,
             "note": "adjoint_ms: target, source and pseudo passes (HIP events); forward_ms: "
                     "force phase of the same lists (fill, leaf kernel, chunk reduction)"}
+
+
+def subcycle_leg(load, nd, nsub, dev):
+    """Particles::subCycle on the device (hsr_subcycle) over the same load:
+    nsub x (drift, resort, unit masses, tree, lists, kick, drift) with the
+    particles resident for the whole cycle.  Seeded Gaussian velocities (sigma
+    5 grid units per unit time) with a TimeStepper of pp 0.5, tau = tau2 =
+    0.02, adot 1.3, fscal 1.1, alpha 1; grid ng = ceil(box + 2 overload) per
+    side, edge = overload.  Drift and resort are HBM passes: drift moves 36 B
+    per particle (reads x, y, z, vx, vy, vz; writes x, y, z), the resort at
+    least 84 B (reads and writes the ten arrays, 42 B per particle)."""
+    import math
+    import torch
+    from haccsr.particles import Particles, TimeStepperValues
+    n = load.n
+    ng = int(math.ceil(load.box + 2 * load.overload))
+    g = torch.Generator().manual_seed(23)
+    d = {"x": torch.from_numpy(load.x).to(dev), "y": torch.from_numpy(load.y).to(dev),
+         "z": torch.from_numpy(load.z).to(dev)}
+    for k in ("vx", "vy", "vz"):
+        d[k] = (5.0 * torch.randn(n, generator=g)).to(dev)
+    d["mass"] = torch.ones(n, device=dev)
+    d["phi"] = torch.zeros(n, device=dev)
+    d["id"] = torch.arange(n, dtype=torch.int64, device=dev)
+    d["mask"] = torch.zeros(n, dtype=torch.int16, device=dev)
+    ts = TimeStepperValues(pp=0.5, tau=0.02, tau2=0.02, adot=1.3, fscal=1.1)
+    P = Particles([ng] * 3, alpha=1.0, gpscal=1.0, edge=load.overload, nsub=nsub,
+                  rcb_tree_ppn=nd)
+    P.ctx.set_tuning(use_graph=1)
+    P.set_particles(d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["mass"], d["phi"],
+                    d["id"], d["mask"])
+    P.nsub = 1
+    P.subCycle(ts)          # allocations and plans
+    P.nsub = nsub
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    P.subCycle(ts)
+    torch.cuda.synchronize()
+    wall = 1e3 * (time.perf_counter() - t0)
+    st = P.ctx.stats()
+    P.close()
+    drift_bytes, resort_bytes = 36 * n, 84 * n
+    dr_ms, rs_ms = st.drift_ms / (2 * nsub), st.resort_ms / nsub
+    return {"nsub": nsub, "ms": wall, "ms_per_substep": wall / nsub, "particles": n,
+            "inside_last": P.Np_last, "ng": ng,
+            "drift": {"ms": dr_ms, "algorithmic_bytes": drift_bytes,
+                      "GB/s": drift_bytes / (dr_ms * 1e-3) / 1e9,
+                      "frac": drift_bytes / (dr_ms * 1e-3) / 1e9 / PEAK_HBM_GBPS},
+            "resort": {"ms": rs_ms, "algorithmic_bytes": resort_bytes,
+                       "GB/s": resort_bytes / (rs_ms * 1e-3) / 1e9,
+                       "frac": resort_bytes / (rs_ms * 1e-3) / 1e9 / PEAK_HBM_GBPS},
+            "note": "one hsr_subcycle call, particles resident on the device; drift and "
+                    "resort times are HIP-event means per pass"}
 
 
 def main():
@@ -296,6 +357,11 @@ def main():
         ctxs[nd].set_tuning(adjoint_mode=a.adjoint_mode)
         out["adjoint"] = adjoint_leg(ctxs[nd], n, dev, a.adjoint_reps, stats)
         log(f"adjoint {out['adjoint']}")
+    if a.subcycle_nsub > 0 and ens is None:
+        for k in list(ctxs):
+            ctxs.pop(k).close()
+        out["subcycle"] = subcycle_leg(load, nd, a.subcycle_nsub, dev)
+        log(f"subcycle {out['subcycle']}")
     if not a.no_cpu and ens is None:
         log("cpu baseline leg ...")
         out["cpu_baseline"] = cpu_baseline(load, nd, a.cpu_seconds)

@@ -725,6 +725,8 @@ hsr_status hsr_get_stats(hsr_ctx *c, hsr_stats *st) {
   st->drift_ms = c->sub.drift_ms;
   st->resort_ms = c->sub.resort_ms;
   st->subcycle_ms = c->sub.subcycle_ms;
+  st->drift_factor = c->sub.drift_factor;
+  st->kick_coeff = c->sub.kick_coeff;
   return HSR_OK;
 }
 
@@ -830,6 +832,7 @@ hsr_status hsr_drift(hsr_ctx *c, int64_t count, float *x, float *y, float *z,
   }
   if ((s = sub_events(c))) return s;
   const float k = map1_factor(pp, tau, adot, alpha);
+  c->sub.drift_factor = k;
   float *d[6] = {x, y, z, (float *)vx, (float *)vy, (float *)vz};
   const bool host = !(ptr_flags & HSR_DEVICE_PTRS);
   hipStream_t st = c->stream;
@@ -889,6 +892,73 @@ hsr_status hsr_resort(hsr_ctx *c, int64_t count, const int32_t ng[3], float *x,
   return HSR_OK;
 }
 
+// Particles::map2, RCB monopole branch (Particles.cxx:1205-1280), on device
+// arrays: resort all particles, set every mass to 1, build the tree over the
+// m_Np_last particles inside the grid with the tree box [0, max ng), list the
+// force box [edge, ng - edge) and kick with
+// c = gpscal^3 / (4 pi) * fscal * tau * stepFraction.
+static hsr_status map2_device(hsr_ctx *c, const hsr_subcycle_params *sp, double stepFraction,
+                              int64_t count, float *const df[8], int64_t *did,
+                              uint16_t *dmask, int64_t *np_last, double *resort_ms) {
+  hsr_status s;
+  hipStream_t st = c->stream;
+  const float cc = map2_coeff(sp->gpscal, sp->fscal, sp->tau, stepFraction);
+  c->sub.kick_coeff = cc;
+  const int ngmax = std::max(std::max(sp->ng[0], sp->ng[1]), sp->ng[2]);
+  const float tmin[3] = {0.f, 0.f, 0.f};
+  const float tmax[3] = {(float)(1.0 * ngmax), (float)(1.0 * ngmax), (float)(1.0 * ngmax)};
+  float flo[3], fhi[3];
+  for (int d = 0; d < 3; ++d) {
+    flo[d] = sp->edge;
+    fhi[d] = (float)(1.0 * sp->ng[d] - sp->edge);
+  }
+  float *f[8];
+  for (int a = 0; a < 8; ++a) f[a] = df[a];
+  HSR_HIP(hipEventRecord(c->sub.ev[2], st));
+  if ((s = resort_run(c, count, sp->ng, f, did, dmask, np_last))) return s;  // synchronizes
+  HSR_HIP(hipEventRecord(c->sub.ev[3], st));
+  if ((s = fill_run(c, count, 1.0f, f[6]))) return s;
+  if ((s = hsr_build(c, *np_last, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], did, dmask,
+                     tmin, tmax, HSR_DEVICE_PTRS)))
+    return s;
+  if ((s = hsr_lists(c, flo, fhi))) return s;
+  if ((s = hsr_kick(c, cc))) return s;  // synchronizes
+  float ms = 0.f;
+  HSR_HIP(hipEventElapsedTime(&ms, c->sub.ev[2], c->sub.ev[3]));
+  *resort_ms += ms;
+  return HSR_OK;
+}
+
+hsr_status hsr_map2(hsr_ctx *c, const hsr_subcycle_params *sp, double stepFraction,
+                    int64_t count, float *x, float *y, float *z, float *vx, float *vy,
+                    float *vz, float *mass, float *phi, int64_t *id, uint16_t *mask,
+                    int64_t *inside, int ptr_flags) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (!sp || !x || !y || !z || !vx || !vy || !vz || !mass || !inside) {
+    g_err = "hsr_map2: null parameters, positions, velocities, masses or output";
+    return HSR_ERR_ARG;
+  }
+  if ((s = check_count_grid("hsr_map2", count, sp->ng))) return s;
+  if ((s = sub_events(c))) return s;
+  float *f[8] = {x, y, z, vx, vy, vz, mass, phi};
+  float *df[8];
+  int64_t *did = id;
+  uint16_t *dmask = mask;
+  const bool host = !(ptr_flags & HSR_DEVICE_PTRS);
+  if (host) {
+    if ((s = stage_in(c, c->sub.arr, count, f, id, mask, df, &did, &dmask))) return s;
+  } else {
+    for (int a = 0; a < 8; ++a) df[a] = f[a];
+  }
+  double resort_ms = 0;
+  if ((s = map2_device(c, sp, stepFraction, count, df, did, dmask, inside, &resort_ms))) return s;
+  if (host && (s = stage_out(c, count, f, id, mask, df, did, dmask))) return s;
+  HSR_HIP(hipStreamSynchronize(c->stream));
+  c->sub.resort_ms = resort_ms;
+  return HSR_OK;
+}
+
 hsr_status hsr_subcycle(hsr_ctx *c, const hsr_subcycle_params *sp, int64_t count,
                         float *x, float *y, float *z, float *vx, float *vy,
                         float *vz, float *mass, float *phi, int64_t *id,
@@ -918,20 +988,12 @@ hsr_status hsr_subcycle(hsr_ctx *c, const hsr_subcycle_params *sp, int64_t count
     for (int a = 0; a < 8; ++a) df[a] = f[a];
   }
   // Particles::subCycle (Particles.cxx:1176-1201): the TimeStepper is not
-  // advanced inside the cycle, so every sub-step uses the same factors.
+  // advanced inside the cycle, so every sub-step uses the same factors;
+  // map1 receives pp, stepFraction * tau2 and adot as floats.
   const double stepFraction = 1.0 / sp->nsub;
   const float k = map1_factor((float)sp->pp, (float)(stepFraction * sp->tau2), (float)sp->adot,
                               sp->alpha);
-  const float cc = map2_coeff(sp->gpscal, sp->fscal, sp->tau, stepFraction);
-  // map2's boxes (Particles.cxx:1210-1228)
-  const int ngmax = std::max(std::max(sp->ng[0], sp->ng[1]), sp->ng[2]);
-  const float tmin[3] = {0.f, 0.f, 0.f};
-  const float tmax[3] = {(float)(1.0 * ngmax), (float)(1.0 * ngmax), (float)(1.0 * ngmax)};
-  float flo[3], fhi[3];
-  for (int d = 0; d < 3; ++d) {
-    flo[d] = sp->edge;
-    fhi[d] = (float)(1.0 * sp->ng[d] - sp->edge);
-  }
+  c->sub.drift_factor = k;
   double drift_ms = 0, resort_ms = 0;
   int64_t np_last = -1;
   auto timed = [&](int e0, int e1, double &acc) -> hsr_status {
@@ -946,17 +1008,9 @@ hsr_status hsr_subcycle(hsr_ctx *c, const hsr_subcycle_params *sp, int64_t count
       if ((s = drift_run(c, count, k, df[0], df[1], df[2], df[3], df[4], df[5]))) return s;
       HSR_HIP(hipEventRecord(c->sub.ev[1], st));
     }
-    if (!sp->skip_kick) {
-      HSR_HIP(hipEventRecord(c->sub.ev[2], st));
-      if ((s = resort_run(c, count, sp->ng, df, did, dmask, &np_last))) return s;  // synchronizes
-      HSR_HIP(hipEventRecord(c->sub.ev[3], st));
-      if ((s = fill_run(c, count, 1.0f, df[6]))) return s;
-      if ((s = hsr_build(c, np_last, df[0], df[1], df[2], df[3], df[4], df[5], df[6], df[7], did,
-                         dmask, tmin, tmax, HSR_DEVICE_PTRS)))
-        return s;
-      if ((s = hsr_lists(c, flo, fhi))) return s;
-      if ((s = hsr_kick(c, cc))) return s;
-    }
+    if (!sp->skip_kick &&
+        (s = map2_device(c, sp, stepFraction, count, df, did, dmask, &np_last, &resort_ms)))
+      return s;
     if (!sp->skip_stream) {
       HSR_HIP(hipEventRecord(c->sub.ev[4], st));
       if ((s = drift_run(c, count, k, df[0], df[1], df[2], df[3], df[4], df[5]))) return s;
@@ -966,7 +1020,6 @@ hsr_status hsr_subcycle(hsr_ctx *c, const hsr_subcycle_params *sp, int64_t count
     if (!sp->skip_stream) {
       if ((s = timed(0, 1, drift_ms)) || (s = timed(4, 5, drift_ms))) return s;
     }
-    if (!sp->skip_kick && (s = timed(2, 3, resort_ms))) return s;
   }
   if (host && (s = stage_out(c, count, f, id, mask, df, did, dmask))) return s;
   HSR_HIP(hipStreamSynchronize(st));

@@ -161,6 +161,7 @@ struct SubState {
   DBuf arr[10];    // x y z vx vy vz mass phi id mask on the device
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   double drift_ms = 0, resort_ms = 0, subcycle_ms = 0;
+  float drift_factor = 0.f, kick_coeff = 0.f;  // last map1 / map2 factors
 };
 
 struct hsr_ctx {

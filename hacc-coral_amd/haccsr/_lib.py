@@ -28,7 +28,7 @@ EXPORTS = [
     "hsr_adjoint", "hsr_get_stats", "hsr_get_nodes", "hsr_get_perm",
     "hsr_get_lists", "hsr_build_many", "hsr_lists_many", "hsr_set_tuning",
     "hsr_get_tuning", "hsr_forces_f64", "hsr_leaf_momentum", "hsr_drift",
-    "hsr_resort", "hsr_subcycle",
+    "hsr_resort", "hsr_map2", "hsr_subcycle",
 ]
 
 
@@ -68,7 +68,8 @@ class Stats(C.Structure):
                 ("lists_ms", C.c_double), ("force_ms", C.c_double),
                 ("kernel_ms", C.c_double), ("adjoint_ms", C.c_double),
                 ("evaluated_pairs", C.c_int64), ("drift_ms", C.c_double),
-                ("resort_ms", C.c_double), ("subcycle_ms", C.c_double)]
+                ("resort_ms", C.c_double), ("subcycle_ms", C.c_double),
+                ("drift_factor", C.c_double), ("kick_coeff", C.c_double)]
 
 
 class SubcycleParams(C.Structure):
@@ -125,6 +126,8 @@ def load(path: str | None = None):
     L.hsr_leaf_momentum.argtypes = [vp] + [vp] * 4 + [C.c_int]
     L.hsr_drift.argtypes = [vp, i64] + [vp] * 6 + [C.c_float] * 4 + [C.c_int]
     L.hsr_resort.argtypes = [vp, i64, vp] + [vp] * 10 + [vp, C.c_int]
+    L.hsr_map2.argtypes = [vp, C.POINTER(SubcycleParams), C.c_double, i64] + [vp] * 10 + \
+        [vp, C.c_int]
     L.hsr_subcycle.argtypes = [vp, C.POINTER(SubcycleParams), i64] + [vp] * 10 + [vp, C.c_int]
     L.hsr_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.hsr_get_nodes.argtypes = [vp, vp, i64]
@@ -302,6 +305,20 @@ class Context:
         check(self.L.hsr_resort(self.h, n, g.ctypes.data, _ptr(x), _ptr(y), _ptr(z),
                                 _ptr(vx), _ptr(vy), _ptr(vz), _ptr(mass), _ptr(phi),
                                 _ptr(ids), _ptr(mask), C.byref(inside), fl), "hsr_resort")
+        return int(inside.value)
+
+    def map2(self, params: "SubcycleParams", step_fraction, x, y, z, vx, vy, vz, mass,
+             phi=None, ids=None, mask=None, device_ptrs=False) -> int:
+        """Particles::map2 (resort, unit masses, tree, lists, kick); returns
+        m_Np_last."""
+        n = int(x.shape[0])
+        inside = C.c_int64(0)
+        fl = HSR_DEVICE_PTRS if device_ptrs else HSR_HOST_PTRS
+        check(self.L.hsr_map2(self.h, C.byref(params), float(step_fraction), n, _ptr(x),
+                              _ptr(y), _ptr(z), _ptr(vx), _ptr(vy), _ptr(vz), _ptr(mass),
+                              _ptr(phi), _ptr(ids), _ptr(mask), C.byref(inside), fl),
+              "hsr_map2")
+        self.n = int(inside.value)
         return int(inside.value)
 
     def subcycle(self, params: "SubcycleParams", x, y, z, vx, vy, vz, mass, phi=None,

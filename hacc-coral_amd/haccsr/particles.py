@@ -9,50 +9,15 @@ the grid and its kick, Particles.cxx:1205-1280) and ``subCycle``
 arrays (host memory, staged by the library) or float32/int64/int16 device
 tensors (used in place).  ``subCycle`` is one hsr_subcycle call that keeps the
 particles on the device for all nsub sub-steps; ``map1`` and ``map2`` are the
-single steps, each one call or a short sequence of calls of the C ABI.
+single steps (hsr_drift, hsr_map2), one C ABI call each.
 """
 from __future__ import annotations
 
-import ctypes as C
 import dataclasses
 
 import numpy as np
 
 from ._lib import Context, SubcycleParams
-
-_libm = None
-
-
-def _m():
-    global _libm
-    if _libm is None:
-        _libm = C.CDLL("libm.so.6")
-        _libm.powf.argtypes = [C.c_float, C.c_float]
-        _libm.powf.restype = C.c_float
-        _libm.atanf.argtypes = [C.c_float]
-        _libm.atanf.restype = C.c_float
-    return _libm
-
-
-def map1_factor(pp: float, tau: float, adot: float, alpha: float) -> float:
-    """prefactor * tau of Particles::map1 (Particles.cxx:746-747) in its
-    float/double mix: powf(pp, 1 + 1/alpha) with the double exponent passed
-    as float; 1.0 / (alpha * adot * pf) with a float product and a double
-    quotient stored as float."""
-    f = np.float32
-    pf = f(_m().powf(f(pp), f(1.0 + 1.0 / float(f(alpha)))))
-    prefactor = f(1.0 / float(f(alpha) * f(adot) * pf))
-    return float(prefactor * f(tau))
-
-
-def map2_coeff(gpscal: float, fscal: float, tau: float, step_fraction: float) -> float:
-    """Kick coefficient of Particles::map2 (Particles.cxx:1230-1233)."""
-    f = np.float32
-    g = f(gpscal)
-    divscal = g * g * g
-    pi = f(4.0 * float(f(_m().atanf(f(1.0)))))
-    return float(f(float(divscal) / 4.0 / float(pi) * fscal * tau * step_fraction))
-
 
 @dataclasses.dataclass
 class TimeStepperValues:
@@ -111,32 +76,12 @@ class Particles:
                                        a["vz"], a["mass"], a["phi"], a["id"], a["mask"],
                                        device_ptrs=self.device_ptrs)
 
-    def _boxes(self):
-        gmax = np.float32(max(self.ng))
-        lo_t = np.zeros(3, np.float32)
-        hi_t = np.full(3, gmax, np.float32)
-        lo_f = np.full(3, np.float32(self.edge), np.float32)
-        hi_f = np.array([np.float32(1.0 * g - float(np.float32(self.edge))) for g in self.ng],
-                        np.float32)
-        return lo_t, hi_t, lo_f, hi_f
-
     # Particles::map2 (RCB monopole branch)
     def map2(self, ts: TimeStepperValues, stepFraction: float = 1.0):
-        c = map2_coeff(self.gpscal, ts.fscal, ts.tau, stepFraction)
-        self.resortParticles()
-        m = self.arrays["mass"]
-        if isinstance(m, np.ndarray):
-            m[:] = 1.0
-        else:
-            m.fill_(1.0)
-        n = self.Np_last
-        lo_t, hi_t, lo_f, hi_f = self._boxes()
-        sl = {k: (None if v is None else v[:n]) for k, v in self.arrays.items()}
-        self.ctx.build(sl["x"], sl["y"], sl["z"], sl["mass"], sl["vx"], sl["vy"], sl["vz"],
-                       sl["phi"], sl["id"], sl["mask"], min_loc=lo_t, max_loc=hi_t,
-                       device_ptrs=self.device_ptrs)
-        self.ctx.lists(lo_f, hi_f)
-        self.ctx.kick(c)
+        a = self.arrays
+        self.Np_last = self.ctx.map2(self.params(ts), stepFraction, a["x"], a["y"], a["z"],
+                                     a["vx"], a["vy"], a["vz"], a["mass"], a["phi"], a["id"],
+                                     a["mask"], device_ptrs=self.device_ptrs)
 
     def params(self, ts: TimeStepperValues) -> SubcycleParams:
         p = SubcycleParams()

@@ -19,7 +19,9 @@
  * and its caller in the sub-cycle (src/cpu):
  *   Particles::map1 / resortParticles         Particles.cxx:732-758, 402-488
  *       = hsr_drift / hsr_resort
- *   Particles::subCycle + map2 (RCB monopole) Particles.cxx:1176-1280
+ *   Particles::map2 (RCB monopole branch)     Particles.cxx:1205-1280
+ *       = hsr_map2
+ *   Particles::subCycle                       Particles.cxx:1176-1201
  *       = hsr_subcycle
  */
 #ifndef HACC_SR_H
@@ -150,6 +152,8 @@ typedef struct {
                              all its drifts) */
   double resort_ms;       /* the same for hsr_resort */
   double subcycle_ms;     /* host wall time of the last hsr_subcycle */
+  double drift_factor;    /* prefactor * tau of the last drift (a float) */
+  double kick_coeff;      /* c of the last map2 kick (a float) */
 } hsr_stats;
 
 hsr_status hsr_get_stats(hsr_ctx *ctx, hsr_stats *st);
@@ -238,6 +242,16 @@ typedef struct {
   float reserved0;
   double pp, tau, tau2, adot, fscal;  /* the TimeStepper's values */
 } hsr_subcycle_params;
+
+/* Particles::map2 alone (RCB monopole branch, Particles.cxx:1205-1280):
+ * resort, unit masses, tree, lists and kick as one sub-step of hsr_subcycle
+ * without its drifts, with the caller's stepFraction (sp->nsub, skip_* and
+ * the drift's TimeStepper values are not read). */
+hsr_status hsr_map2(hsr_ctx *ctx, const hsr_subcycle_params *sp,
+                    double stepFraction, int64_t count, float *x, float *y,
+                    float *z, float *vx, float *vy, float *vz, float *mass,
+                    float *phi, int64_t *id, uint16_t *mask, int64_t *inside,
+                    int ptr_flags);
 
 hsr_status hsr_subcycle(hsr_ctx *ctx, const hsr_subcycle_params *sp,
                         int64_t count, float *x, float *y, float *z,

@@ -26,7 +26,8 @@ def declared_functions():
 def test_header_declares_the_entry_points():
     names = declared_functions()
     for must in ("hsr_create", "hsr_destroy", "hsr_build", "hsr_lists", "hsr_forces",
-                 "hsr_kick", "hsr_adjoint", "hsr_build_many", "hsr_lists_many"):
+                 "hsr_kick", "hsr_adjoint", "hsr_build_many", "hsr_lists_many", "hsr_drift",
+                 "hsr_resort", "hsr_map2", "hsr_subcycle"):
         assert must in names
 
 
@@ -54,6 +55,9 @@ int main(void) {
   printf("%zu %zu %zu %zu\n", offsetof(hsr_params, nd), offsetof(hsr_params, fsrrmax),
          offsetof(hsr_params, ppc), offsetof(hsr_node, xc));
   printf("%zu %zu\n", offsetof(hsr_stats, build_ms), offsetof(hsr_tuning, use_graph));
+  printf("%zu %zu %zu %zu %zu\n", sizeof(hsr_subcycle_params), offsetof(hsr_subcycle_params, ng),
+         offsetof(hsr_subcycle_params, pp), offsetof(hsr_subcycle_params, fscal),
+         offsetof(hsr_stats, kick_coeff));
   return 0;
 }
 """)
@@ -64,7 +68,9 @@ int main(void) {
     want = [C.sizeof(_lib.Params), C.sizeof(_lib.Node), C.sizeof(_lib.Stats),
             C.sizeof(_lib.Tuning), _lib.Params.nd.offset, _lib.Params.fsrrmax.offset,
             _lib.Params.ppc.offset, _lib.Node.xc.offset, _lib.Stats.build_ms.offset,
-            _lib.Tuning.use_graph.offset]
+            _lib.Tuning.use_graph.offset, C.sizeof(_lib.SubcycleParams),
+            _lib.SubcycleParams.ng.offset, _lib.SubcycleParams.pp.offset,
+            _lib.SubcycleParams.fscal.offset, _lib.Stats.kick_coeff.offset]
     assert got == want
     assert _lib.NODE_DTYPE.itemsize == C.sizeof(_lib.Node)
 

@@ -220,6 +220,36 @@ def test_subcycle_skips():
     P.close()
 
 
+def test_step_factors_bitexact():
+    """The drift factor and kick coefficient the library derives on the host
+    from the TimeStepper values equal the oracle's bit for bit."""
+    ng, nd = (12, 12, 12), 64
+    st = particles(3000, ng, 19)
+    rng = np.random.default_rng(4)
+    for trial in range(6):
+        ts = TimeStepperValues(pp=rng.uniform(0.05, 2.0), tau=rng.uniform(1e-3, 0.1),
+                               tau2=rng.uniform(1e-3, 0.1), adot=rng.uniform(0.1, 4.0),
+                               fscal=rng.uniform(0.2, 3.0))
+        nsub = int(rng.integers(1, 7))
+        alpha, gpscal = float(rng.choice([0.5, 1.0, 1.5, 2.0])), float(rng.uniform(0.5, 2.5))
+        say(f"case step factors trial {trial} nsub {nsub}")
+        a = {k: v.copy() for k, v in st.items()}
+        P = make(ng, nd, nsub=nsub, alpha=alpha, gpscal=gpscal)
+        P.set_particles(a["x"], a["y"], a["z"], a["vx"], a["vy"], a["vz"], a["mass"], a["phi"],
+                        a["id"], a["mask"])
+        P.subCycle(ts)
+        s = P.ctx.stats()
+        sf = 1.0 / nsub
+        k = O.map1_factor(np.float32(ts.pp), np.float32(sf * ts.tau2), np.float32(ts.adot), alpha)
+        c = O.map2_coeff(np.float32(gpscal), ts.fscal, ts.tau, sf)
+        assert np.float32(s.drift_factor) == np.float32(k)
+        assert np.float32(s.kick_coeff) == np.float32(c)
+        P.map2(ts, 0.3)
+        assert np.float32(P.ctx.stats().kick_coeff) == np.float32(
+            O.map2_coeff(np.float32(gpscal), ts.fscal, ts.tau, 0.3))
+        P.close()
+
+
 def test_subcycle_steps_against_oracle():
     """Each sub-step of the device cycle against the oracle's sub-step from
     the same starting state: the resort and the tree permutation are exact
@@ -261,3 +291,46 @@ def test_subcycle_steps_against_oracle():
 
     P.subCycle_steps(TS, after_step=check)
     P.close()
+
+
+def _read_cxx(path, n):
+    raw = np.fromfile(path, dtype=np.uint8)
+    half = raw.shape[0] // 2
+    out = []
+    for part in (raw[:half], raw[half:]):
+        f = part[: 32 * n].view(np.float32).reshape(8, n)
+        ids = part[32 * n: 40 * n].view(np.int64)
+        mask = part[40 * n: 42 * n].view(np.uint16)
+        d = dict(zip(("x", "y", "z", "vx", "vy", "vz", "phi", "mass"), f))
+        d["id"], d["mask"] = ids, mask
+        out.append(d)
+    return out
+
+
+def test_cxx_subcycle_dropin(tmp_path):
+    """The C++ drop-in (ParticleSubCycle, reference caller pattern
+    subCycle(gts)) equals its own map1/map2 steps and the Python front end on
+    the same particles, bit for bit."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "hacc-coral_amd", "bin", "test_subcycle")
+    n, g, nd = 30000, 20, 64
+    res = []
+    for mode in (0, 1):
+        say(f"case C++ subcycle mode {mode}")
+        out = tmp_path / f"sub{mode}.bin"
+        subprocess.run([exe, str(n), str(g), str(nd), "77", str(mode), str(out)], check=True,
+                       timeout=120)
+        res.append(_read_cxx(out, n))
+    (init, fin0), (_, fin1) = res
+    for k in ARR:
+        assert bits_equal(fin0[k], fin1[k]), k
+    a = {k: v.copy() for k, v in init.items()}
+    P = make((g, g, g), nd)
+    P.set_particles(a["x"], a["y"], a["z"], a["vx"], a["vy"], a["vz"], a["mass"], a["phi"],
+                    a["id"], a["mask"])
+    P.subCycle(TS)
+    P.close()
+    for k in ARR:
+        assert bits_equal(a[k], fin0[k]), k

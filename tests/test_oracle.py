@@ -338,20 +338,6 @@ def test_map2_coeff_closed_form():
         assert abs(got - want) <= 3e-7 * want, (g, fs, tau, sf)
 
 
-def test_host_step_factors_match_oracle():
-    """The library's Python mirror of map1/map2's host arithmetic agrees with
-    the C restatement bit for bit (the C ABI computes the same in C++)."""
-    from haccsr import particles as P
-    rng = np.random.default_rng(5)
-    for _ in range(200):
-        pp, tau, adot, alpha = rng.uniform(0.05, 3.0), rng.uniform(1e-4, 0.2), \
-            rng.uniform(0.1, 5.0), rng.choice([0.5, 1.0, 1.5, 2.0])
-        assert P.map1_factor(pp, tau, adot, alpha) == \
-            O.map1_factor(np.float32(pp), np.float32(tau), np.float32(adot), alpha)
-        g, fs, sf = rng.uniform(0.5, 3.0), rng.uniform(0.1, 3.0), 1.0 / rng.integers(1, 9)
-        assert P.map2_coeff(g, fs, tau, sf) == O.map2_coeff(np.float32(g), fs, tau, sf)
-
-
 def test_resort_oracle_moves_all_arrays_and_keeps_order():
     """oracle.resort applies the cell permutation to all ten arrays; within a
     cell the input order is kept and the tail holds every particle outside
