@@ -339,7 +339,7 @@ def main():
         "data": "synthetic (seeded clustered generator; unit masses)",
         "config": {"workload": (workload_name(a.config, load, nd) if ens is None else
                                 ensemble_name(ens, n, nd)), "particles": n,
-                   "pairs_per_step": pairs, "leaf_size": nd,
+                   "pairs_per_step": pairs, "list_entries": int(st.entries), "leaf_size": nd,
                    "breakdown_ms": {"build": float(np.mean([s[1] for s in stats])),
                                     "lists": float(np.mean([s[2] for s in stats])),
                                     "kick": float(np.mean([s[3] for s in stats])),

@@ -90,36 +90,80 @@ static float pptdr(const float *xmin, const float *xmax, const float *xc) {
   return r;
 }
 
-/* RCBForceTree.cxx:525-534 with the TDPTS = 1 design (0,0,0). */
-static void pppts1(float tdr, const float *xc, float *px, float *py, float *pz) {
-  *px = tdr * 0.0f + xc[0];
-  *py = tdr * 0.0f + xc[1];
-  *pz = tdr * 0.0f + xc[2];
+/* The spherical designs of RCBForceTree.cxx:134-272 that this path uses:
+ * TDPTS = 1, the single point (0,0,0) (the monopole at the centre of mass);
+ * TDPTS = 12, Hardin and Sloane's 12-point 4-design (icosahedron vertices),
+ * in the reference's order, rounded to float as POSVEL_T arrays are. */
+static const float SPH12_X[12] = {
+    0.0f, 0.0f, (float)0.525731112119134, (float)-0.525731112119134,
+    (float)0.85065080835204, (float)-0.85065080835204, 0.0f, 0.0f,
+    (float)-0.525731112119134, (float)0.525731112119134,
+    (float)-0.85065080835204, (float)0.85065080835204};
+static const float SPH12_Y[12] = {
+    (float)0.85065080835204, (float)0.85065080835204, 0.0f, 0.0f,
+    (float)0.525731112119134, (float)0.525731112119134,
+    (float)-0.85065080835204, (float)-0.85065080835204, 0.0f, 0.0f,
+    (float)-0.525731112119134, (float)-0.525731112119134};
+static const float SPH12_Z[12] = {
+    (float)0.525731112119134, (float)-0.525731112119134,
+    (float)0.85065080835204, (float)0.85065080835204, 0.0f, 0.0f,
+    (float)-0.525731112119134, (float)0.525731112119134,
+    (float)-0.85065080835204, (float)-0.85065080835204, 0.0f, 0.0f};
+
+static int tdpts_of(const hsro_params *p) { return p->tdpts > 1 ? p->tdpts : 1; }
+
+/* The pseudo masses of node k. */
+static float *ppm_of(const hsro_params *p, hsro_node *t, int64_t k) {
+  return tdpts_of(p) == 1 ? t[k].ppm : p->ppm_t + k * (int64_t)tdpts_of(p);
+}
+static const float *ppm_of_c(const hsro_params *p, const hsro_node *t, int64_t k) {
+  return tdpts_of(p) == 1 ? t[k].ppm : p->ppm_t + k * (int64_t)tdpts_of(p);
 }
 
-/* RCBForceTree.cxx:536-569 for TDPTS = 1: K = 1, odr0 = 1; the single
- * design point sits at the centre (rj2 == 0), so odr1 = odr2 = 0 and each
- * member adds mass*(1+0+0). */
-static void pp1(int64_t count, const float *xx, const float *yy,
+/* RCBForceTree.cxx:525-534: ppx[i] = tdr*sphdesign<T>::x[i] + xc[0]. */
+static void pppts(int T, float tdr, const float *xc, float *px, float *py, float *pz) {
+  for (int i = 0; i < T; ++i) {
+    float sx = T == 1 ? 0.0f : SPH12_X[i];
+    float sy = T == 1 ? 0.0f : SPH12_Y[i];
+    float sz = T == 1 ? 0.0f : SPH12_Z[i];
+    px[i] = tdr * sx + xc[0];
+    py[i] = tdr * sy + xc[1];
+    pz[i] = tdr * sz + xc[2];
+  }
+}
+
+void hsro_pppts(int tdpts, float tdr, const float *xc, float *px, float *py,
+                float *pz) {
+  pppts(tdpts > 1 ? tdpts : 1, tdr, xc, px, py, pz);
+}
+
+/* RCBForceTree.cxx:536-569: ppm[j] += mass[i]*(odr0 + odr1 + odr2) with
+ * K = T, odr0 = 1/K, odr1 = (3/K)(ri/tdr) aij and
+ * odr2 = (5/K)(ri/tdr)(ri/tdr) 0.5 (3 aij^2 - 1), the product with 0.5
+ * taken in double as the double literal makes it; odr1 = odr2 = 0 where the
+ * design point sits at the centre (T = 1). */
+static void ppT(int T, int64_t count, const float *xx, const float *yy,
                 const float *zz, const float *mass, const float *xc,
                 const float *ppx, const float *ppy, const float *ppz,
                 float *ppm, float tdr) {
-  const float K = 1.0f;
+  const float K = (float)T;
   const float odr0 = 1 / K;
   for (int64_t i = 0; i < count; ++i) {
     float xi = xx[i] - xc[0], yi = yy[i] - xc[1], zi = zz[i] - xc[2];
     float ri = sqrtf(xi * xi + yi * yi + zi * zi);
-    float xj = ppx[0] - xc[0], yj = ppy[0] - xc[1], zj = ppz[0] - xc[2];
-    float rj2 = xj * xj + yj * yj + zj * zj;
-    float odr1 = 0, odr2 = 0;
-    if (rj2 != 0) {
-      float rj = sqrtf(rj2);
-      float aij = (xi * xj + yi * yj + zi * zj) / (ri * rj);
-      odr1 = (3 / K) * (ri / tdr) * aij;
-      odr2 = (float)((double)((5 / K) * (ri / tdr) * (ri / tdr)) * 0.5 *
-                     (double)(3 * aij * aij - 1));
+    for (int j = 0; j < T; ++j) {
+      float xj = ppx[j] - xc[0], yj = ppy[j] - xc[1], zj = ppz[j] - xc[2];
+      float rj2 = xj * xj + yj * yj + zj * zj;
+      float odr1 = 0, odr2 = 0;
+      if (rj2 != 0) {
+        float rj = sqrtf(rj2);
+        float aij = (xi * xj + yi * yj + zi * zj) / (ri * rj);
+        odr1 = (3 / K) * (ri / tdr) * aij;
+        odr2 = (float)((double)((5 / K) * (ri / tdr) * (ri / tdr)) * 0.5 *
+                       (double)(3 * aij * aij - 1));
+      }
+      ppm[j] += mass[i] * (odr0 + odr1 + odr2);
     }
-    ppm[0] += mass[i] * (odr0 + odr1 + odr2);
   }
 }
 
@@ -175,17 +219,20 @@ static void build_node(builder *b, int64_t tl) {
   hsro_node *t = b->t;
   int64_t cnt = t[tl].count, off = t[tl].offset;
   float **a = b->a;
+  const int T = tdpts_of(b->p);
   hsro_cm(cnt, a[0] + off, a[1] + off, a[2] + off, a[6] + off, t[tl].xmin,
           t[tl].xmax, t[tl].xc);
   if (cnt <= b->p->nd) {
     t[tl].tdr = b->p->ppc * pptdr(t[tl].xmin, t[tl].xmax, t[tl].xc);
-    t[tl].ppm[0] = 0;
-    if (cnt > 1) {
-      float px, py, pz;
-      pppts1(t[tl].tdr, t[tl].xc, &px, &py, &pz);
-      pp1(cnt, a[0] + off, a[1] + off, a[2] + off, a[6] + off, t[tl].xc,
-          &px, &py, &pz, t[tl].ppm, t[tl].tdr);
+    float *pm = ppm_of(b->p, t, tl);
+    memset(pm, 0, sizeof(float) * T);
+    if (cnt > T) {
+      float px[12], py[12], pz[12];
+      pppts(T, t[tl].tdr, t[tl].xc, px, py, pz);
+      ppT(T, cnt, a[0] + off, a[1] + off, a[2] + off, a[6] + off, t[tl].xc,
+          px, py, pz, pm, t[tl].tdr);
     }
+    t[tl].ppm[0] = pm[0];
     return;
   }
   int64_t tlcl = b->size, tlcr = tlcl + 1;
@@ -195,6 +242,7 @@ static void build_node(builder *b, int64_t tl) {
   }
   b->size = tlcr + 1;
   memset(&t[tlcl], 0, sizeof(hsro_node) * 2);
+  if (T > 1) memset(b->p->ppm_t + tlcl * (int64_t)T, 0, sizeof(float) * 2 * T);
   for (int i = 0; i < 3; ++i) {
     t[tlcl].xmin[i] = t[tl].xmin[i];
     t[tlcr].xmin[i] = t[tl].xmin[i];
@@ -210,26 +258,28 @@ static void build_node(builder *b, int64_t tl) {
   build_subtree(b, d, tl, tlcl, tlcr);
   if (b->overflow) return;
 
-  float ppx, ppy, ppz;
+  float ppx[12], ppy[12], ppz[12];
   t[tl].tdr = b->p->ppc * pptdr(t[tl].xmin, t[tl].xmax, t[tl].xc);
-  pppts1(t[tl].tdr, t[tl].xc, &ppx, &ppy, &ppz);
-  t[tl].ppm[0] = 0;
+  pppts(T, t[tl].tdr, t[tl].xc, ppx, ppy, ppz);
+  float *pm = ppm_of(b->p, t, tl);
+  memset(pm, 0, sizeof(float) * T);
   const int64_t ch[2] = {tlcl, tlcr};
   for (int k = 0; k < 2; ++k) {
     int64_t c = ch[k];
     if (t[c].count > 0) {
-      if (t[c].count <= 1) {
+      if (t[c].count <= T) {
         int64_t offc = t[c].offset;
-        pp1(t[c].count, a[0] + offc, a[1] + offc, a[2] + offc, a[6] + offc,
-            t[tl].xc, &ppx, &ppy, &ppz, t[tl].ppm, t[tl].tdr);
+        ppT(T, t[c].count, a[0] + offc, a[1] + offc, a[2] + offc, a[6] + offc,
+            t[tl].xc, ppx, ppy, ppz, pm, t[tl].tdr);
       } else {
-        float cx, cy, cz;
-        pppts1(t[c].tdr, t[c].xc, &cx, &cy, &cz);
-        pp1(1, &cx, &cy, &cz, t[c].ppm, t[tl].xc, &ppx, &ppy, &ppz,
-            t[tl].ppm, t[tl].tdr);
+        float cx[12], cy[12], cz[12];
+        pppts(T, t[c].tdr, t[c].xc, cx, cy, cz);
+        ppT(T, T, cx, cy, cz, ppm_of(b->p, t, c), t[tl].xc, ppx, ppy, ppz, pm,
+            t[tl].tdr);
       }
     }
   }
+  t[tl].ppm[0] = pm[0];
 }
 
 int64_t hsro_build(const hsro_params *p, int64_t n, float *x, float *y,
@@ -243,7 +293,11 @@ int64_t hsro_build(const hsro_params *p, int64_t n, float *x, float *y,
   b.a[5] = vz; b.a[6] = mass; b.a[7] = phi;
   b.id = id; b.mask = mask; b.t = nodes; b.cap = cap; b.size = 1;
   b.overflow = 0;
+  if (!(p->tdpts == 0 || p->tdpts == 1 || p->tdpts == 12) ||
+      (p->tdpts > 1 && !p->ppm_t))
+    return -2;
   memset(&nodes[0], 0, sizeof(hsro_node));
+  if (p->tdpts > 1) memset(p->ppm_t, 0, sizeof(float) * p->tdpts);
   nodes[0].count = n;
   nodes[0].offset = 0;
   for (int i = 0; i < 3; ++i) {
@@ -386,11 +440,11 @@ int64_t hsro_list(const hsro_params *p, const hsro_node *t, int64_t nnodes,
     }
     if (!looksBig) {
       if (dist2 > fsrrmax2) continue;
-      if (t[tln].count <= 1) {
+      if (t[tln].count <= tdpts_of(p)) {
         EMIT(t[tln].offset, t[tln].count, t[tln].count);
         continue;
       }
-      EMIT(tln, -1, 1);
+      EMIT(tln, -1, tdpts_of(p));
       continue;
     } else if (t[tln].cr == 0 && t[tln].cl == 0) {
       EMIT(t[tln].offset, t[tln].count, t[tln].count);
@@ -524,12 +578,13 @@ float hsro_law_f32(int law, float r2, float rsm, float fsrrmax) {
 
 /* ------------------------------------------------------------- kernels */
 /* Gather the list of one leaf into SoA (the reference's nx/ny/nz/nm). */
-static int64_t gather_list(const hsro_node *t, const float *x, const float *y,
-                           const float *z, const float *m, const int64_t *sa,
-                           const int64_t *sb, int64_t ns, float **bufs,
-                           int64_t *bcap) {
+static int64_t gather_list(const hsro_params *p, const hsro_node *t,
+                           const float *x, const float *y, const float *z,
+                           const float *m, const int64_t *sa, const int64_t *sb,
+                           int64_t ns, float **bufs, int64_t *bcap) {
+  const int T = tdpts_of(p);
   int64_t len = 0;
-  for (int64_t s = 0; s < ns; ++s) len += sb[s] >= 0 ? sb[s] : 1;
+  for (int64_t s = 0; s < ns; ++s) len += sb[s] >= 0 ? sb[s] : T;
   if (len > *bcap) {
     *bcap = len * 2;
     for (int k = 0; k < 4; ++k)
@@ -546,10 +601,12 @@ static int64_t gather_list(const hsro_node *t, const float *x, const float *y,
         ++o;
       }
     } else {
+      /* RCBForceTree.cxx:1051-1060: the node's T pseudo-particles */
       const hsro_node *n = &t[sa[s]];
-      pppts1(n->tdr, n->xc, &bufs[0][o], &bufs[1][o], &bufs[2][o]);
-      bufs[3][o] = n->ppm[0];
-      ++o;
+      const float *pm = ppm_of_c(p, t, sa[s]);
+      pppts(T, n->tdr, n->xc, &bufs[0][o], &bufs[1][o], &bufs[2][o]);
+      for (int j = 0; j < T; ++j) bufs[3][o + j] = pm[j];
+      o += T;
     }
   }
   return len;
@@ -570,7 +627,7 @@ void hsro_force_f32(int law, const hsro_params *p, const hsro_node *t,
 #pragma omp for schedule(dynamic, 4)
     for (int64_t k = 0; k < nleaves; ++k) {
       const hsro_node *n = &t[leaves[k]];
-      int64_t len = gather_list(t, x, y, z, m, seg_a + seg_ptr[k],
+      int64_t len = gather_list(p, t, x, y, z, m, seg_a + seg_ptr[k],
                                 seg_b + seg_ptr[k], seg_ptr[k + 1] - seg_ptr[k],
                                 bufs, &bcap);
       for (int64_t i = n->offset; i < n->offset + n->count; ++i) {
@@ -618,7 +675,7 @@ void hsro_force_step16(const hsro_params *p, const hsro_node *t,
 #pragma omp for schedule(dynamic, 4)
     for (int64_t k = 0; k < nleaves; ++k) {
       const hsro_node *n = &t[leaves[k]];
-      int64_t len = gather_list(t, x, y, z, m, seg_a + seg_ptr[k],
+      int64_t len = gather_list(p, t, x, y, z, m, seg_a + seg_ptr[k],
                                 seg_b + seg_ptr[k], seg_ptr[k + 1] - seg_ptr[k],
                                 bufs, &bcap);
       for (int64_t i = n->offset; i < n->offset + n->count; ++i) {
@@ -669,7 +726,7 @@ void hsro_force_f64(const hsro_params *p, const hsro_node *t,
 #pragma omp for schedule(dynamic, 4)
     for (int64_t k = 0; k < nleaves; ++k) {
       const hsro_node *n = &t[leaves[k]];
-      int64_t len = gather_list(t, x, y, z, m, seg_a + seg_ptr[k],
+      int64_t len = gather_list(p, t, x, y, z, m, seg_a + seg_ptr[k],
                                 seg_b + seg_ptr[k], seg_ptr[k + 1] - seg_ptr[k],
                                 bufs, &bcap);
       double mx = 0, my = 0, mz = 0;
@@ -838,6 +895,7 @@ double hsro_objective_f64(const hsro_params *p, const hsro_node *t,
                           const int64_t *seg_a, const int64_t *seg_b,
                           const double *lx, const double *ly,
                           const double *lz) {
+  if (tdpts_of(p) != 1) return NAN; /* the adjoint covers the monopole tree */
   const double rc2 = (double)(p->fsrrmax * p->fsrrmax);
   const double rsm2 = (double)p->rsm * (double)p->rsm;
   double J = 0.0;
@@ -879,6 +937,7 @@ void hsro_adjoint_f64(const hsro_params *p, const hsro_node *t,
                       const int64_t *seg_ptr, const int64_t *seg_a,
                       const int64_t *seg_b, const double *lx, const double *ly,
                       const double *lz, double *gx, double *gy, double *gz) {
+  if (tdpts_of(p) != 1) return; /* the adjoint covers the monopole tree */
   const double rc2 = (double)(p->fsrrmax * p->fsrrmax);
   const double rsm2 = (double)p->rsm * (double)p->rsm;
   /* serial on purpose: source-side accumulations cross leaves */
@@ -937,6 +996,7 @@ void hsro_force_f64d(const hsro_params *p, const hsro_node *t,
                      const double *m, const int64_t *leaves, int64_t nleaves,
                      const int64_t *seg_ptr, const int64_t *seg_a,
                      const int64_t *seg_b, double *ax, double *ay, double *az) {
+  if (tdpts_of(p) != 1) return; /* the adjoint covers the monopole tree */
   const double rc2 = (double)(p->fsrrmax * p->fsrrmax);
   const double rsm2 = (double)p->rsm * (double)p->rsm;
 #pragma omp parallel for schedule(dynamic, 4)

@@ -13,6 +13,7 @@
  *   hsro_leaf_order      RCBForceTree.cxx:1151-1197
  *   hsro_list            RCBForceTree.cxx:924-1140
  *   hsro_force_*         RCBForceTree.cxx:575-620 / BGQStep16.c:167-187
+ *   hsro_pppts, TDPTS 12 RCBForceTree.cxx:229-272, 519-569, 1051-1060
  *   hsro_cell_sort_perm  Particles.cxx:402-488 (resortParticles)
  *   hsro_map1_factor, hsro_drift   Particles.cxx:732-758 (map1)
  *   hsro_map2_coeff      Particles.cxx:1230-1233 (map2)
@@ -25,6 +26,10 @@
  *   tree          tiling/tight-bbox/split-rule/leaf-size invariants.
  *   lists         mass-accounting invariant and oa->0 direct-sum limit.
  *   kernels       FP64 brute force over the same lists.
+ *   TDPTS = 12    the design is the icosahedron's 12 vertices; every node's
+ *                 pseudo-particles reproduce its members' mass, dipole and
+ *                 traceless quadrupole about xc; the tree force is closer
+ *                 to the direct sum than the monopole tree's.
  *   cell sort     numpy's stable argsort of the cell keys.
  *   map1, map2    closed forms (prefactor 1 exactly, double-precision
  *                 formula to float rounding, two half drifts vs one).
@@ -41,7 +46,8 @@ extern "C" {
 #define HSRO_MAX_TDPTS 1
 
 /* Node layout follows RCBForceTree<TDPTS>::TreeNode (RCBForceTree.h:131-147)
- * with TDPTS = 1 (RCBMonopoleForceTree). */
+ * with TDPTS = 1 (RCBMonopoleForceTree); with TDPTS = 12
+ * (RCBQuadrupoleForceTree) the twelve masses live in hsro_params.ppm_t. */
 typedef struct {
   int64_t count, offset, cl, cr;
   float ppm[HSRO_MAX_TDPTS];
@@ -57,6 +63,11 @@ typedef struct {
   float ppc;           /* pseudo-particle contraction factor (0.9) */
   float min_loc[3], max_loc[3];     /* tree box (root bbox seed) */
   float min_force[3], max_force[3]; /* force box */
+  int32_t tdpts;       /* pseudo-particles per node: 1 (also when 0) or 12 */
+  int32_t reserved;
+  float *ppm_t;        /* tdpts > 1: pseudo masses, tdpts per node in node
+                          order (the build writes them, lists and kernels
+                          read them); node.ppm[0] then holds the first */
 } hsro_params;
 
 /* force-law kinds */
@@ -182,6 +193,11 @@ double hsro_objective_f64(const hsro_params *p, const hsro_node *nodes,
 int64_t hsro_cell_sort_perm(int64_t n, const float *x, const float *y,
                             const float *z, const int32_t ng[3],
                             int64_t *perm);
+
+/* RCBForceTree.cxx:525-534: the positions of a node's T pseudo-particles,
+ * tdr * design + xc (T = 1 or 12). */
+void hsro_pppts(int tdpts, float tdr, const float *xc, float *px, float *py,
+                float *pz);
 
 /* Particles::map1 (Particles.cxx:732-758): the step factor prefactor * tau
  * in the reference's float/double mix, and the drift x += factor * v with

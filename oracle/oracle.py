@@ -42,7 +42,8 @@ class Params(C.Structure):
     _fields_ = [("nd", C.c_int64), ("fsrrmax", C.c_float), ("rsm", C.c_float),
                 ("tan_oa", C.c_float), ("ppc", C.c_float),
                 ("min_loc", C.c_float * 3), ("max_loc", C.c_float * 3),
-                ("min_force", C.c_float * 3), ("max_force", C.c_float * 3)]
+                ("min_force", C.c_float * 3), ("max_force", C.c_float * 3),
+                ("tdpts", C.c_int32), ("reserved", C.c_int32), ("ppm_t", C.c_void_p)]
 
 
 _lib = None
@@ -92,6 +93,7 @@ def lib():
         L.hsro_force_f64d.argtypes = [C.POINTER(Params), nodes, f64, f64, f64, f64, i64, C.c_int64, i64, i64, i64, f64, f64, f64]
         L.hsro_cell_sort_perm.argtypes = [C.c_int64, f32, f32, f32, P(np.int32, flags="C"), i64]
         L.hsro_cell_sort_perm.restype = C.c_int64
+        L.hsro_pppts.argtypes = [C.c_int, C.c_float, f32, f32, f32, f32]
         L.hsro_map1_factor.argtypes = [C.c_float] * 4
         L.hsro_map1_factor.restype = C.c_float
         L.hsro_drift.argtypes = [C.c_int64] + [f32] * 6 + [C.c_float]
@@ -138,13 +140,18 @@ class TreeParams:
     oa: float = 0.1
     nd: int = 128
     ppc: float = 0.9
+    tdpts: int = 1          # 1: RCBMonopoleForceTree, 12: RCBQuadrupoleForceTree
 
     def tan_oa(self) -> float:
         # tanf(oa) in float, as RCBForceTree.cxx:381 evaluates it
         return float(np.float32(math.tan(float(np.float32(self.oa)))))
 
-    def ctypes(self) -> Params:
+    def ctypes(self, tree=None) -> Params:
+        """The C parameter block; a tree with T > 1 lends its pseudo masses."""
         p = Params()
+        p.tdpts = int(self.tdpts)
+        if tree is not None and getattr(tree, "ppm_t", None) is not None:
+            p.ppm_t = tree.ppm_t.ctypes.data
         p.nd = int(self.nd)
         p.fsrrmax = float(np.float32(self.fsrrmax))
         p.rsm = float(np.float32(self.rsm))
@@ -180,6 +187,11 @@ class Tree:
     phi: np.ndarray
     id: np.ndarray
     mask: np.ndarray
+    ppm_t: np.ndarray = None   # T > 1: (nodes, T) pseudo masses
+
+    def ppm(self, T=1) -> np.ndarray:
+        """Pseudo masses as (nodes, T)."""
+        return self.nodes["ppm"].reshape(-1, 1) if self.ppm_t is None else self.ppm_t
 
 
 def build_tree(tp: TreeParams, x, y, z, mass=None, vx=None, vy=None, vz=None,
@@ -196,15 +208,19 @@ def build_tree(tp: TreeParams, x, y, z, mass=None, vx=None, vy=None, vz=None,
     cap = max(16, 4 * (n // max(1, tp.nd // 2) + 2) + 64)
     while True:
         nodes = np.zeros(cap, NODE_DTYPE)
+        ppm_t = np.zeros((cap, tp.tdpts), np.float32) if tp.tdpts > 1 else None
         p = tp.ctypes()
+        if ppm_t is not None:
+            p.ppm_t = ppm_t.ctypes.data
         # rebuild from pristine inputs each attempt
         X2, Y2, Z2, M2 = X.copy(), Y.copy(), Z.copy(), M.copy()
         VX2, VY2, VZ2, PHI2, ID2, MASK2 = VX.copy(), VY.copy(), VZ.copy(), PHI.copy(), ID.copy(), MASK.copy()
         nn = lib().hsro_build(C.byref(p), n, X2, Y2, Z2, VX2, VY2, VZ2, M2,
                               PHI2, ID2, MASK2, nodes, cap)
+        assert nn != -2, "unsupported tdpts"
         if nn >= 0:
             return Tree(nodes[:nn].copy(), X2, Y2, Z2, VX2, VY2, VZ2, M2,
-                        PHI2, ID2, MASK2)
+                        PHI2, ID2, MASK2, None if ppm_t is None else ppm_t[:nn].copy())
         cap *= 2
 
 
@@ -251,7 +267,7 @@ def build_lists(tp: TreeParams, tree: Tree, leaves=None) -> Lists:
 def forces_f32(tp: TreeParams, tree: Tree, lists: Lists, law: int = LAW_STEP16):
     n = tree.x.shape[0]
     ax = np.zeros(n, np.float32); ay = np.zeros(n, np.float32); az = np.zeros(n, np.float32)
-    p = tp.ctypes()
+    p = tp.ctypes(tree)
     lib().hsro_force_f32(law, C.byref(p), tree.nodes, tree.x, tree.y, tree.z,
                          tree.mass, lists.leaves, lists.leaves.shape[0],
                          lists.seg_ptr, lists.seg_a, lists.seg_b, ax, ay, az)
@@ -261,7 +277,7 @@ def forces_f32(tp: TreeParams, tree: Tree, lists: Lists, law: int = LAW_STEP16):
 def forces_step16(tp: TreeParams, tree: Tree, lists: Lists):
     n = tree.x.shape[0]
     ax = np.zeros(n, np.float32); ay = np.zeros(n, np.float32); az = np.zeros(n, np.float32)
-    p = tp.ctypes()
+    p = tp.ctypes(tree)
     lib().hsro_force_step16(C.byref(p), tree.nodes, tree.x, tree.y, tree.z,
                             tree.mass, lists.leaves, lists.leaves.shape[0],
                             lists.seg_ptr, lists.seg_a, lists.seg_b, ax, ay, az)
@@ -272,7 +288,7 @@ def forces_f64(tp: TreeParams, tree: Tree, lists: Lists, with_momentum=False):
     n = tree.x.shape[0]
     ax = np.zeros(n); ay = np.zeros(n); az = np.zeros(n)
     mom = np.zeros((lists.leaves.shape[0], 3)) if with_momentum else None
-    p = tp.ctypes()
+    p = tp.ctypes(tree)
     lib().hsro_force_f64(C.byref(p), tree.nodes, tree.x, tree.y, tree.z,
                          tree.mass, lists.leaves, lists.leaves.shape[0],
                          lists.seg_ptr, lists.seg_a, lists.seg_b, ax, ay, az,
@@ -329,6 +345,13 @@ def cell_sort_perm(x, y, z, ng):
                                        np.ascontiguousarray(z, np.float32),
                                        np.asarray(ng, np.int32), perm)
     return perm, inside
+
+
+def pppts(tdpts, tdr, xc):
+    """(T, 3) positions of a node's pseudo-particles."""
+    out = [np.zeros(tdpts, np.float32) for _ in range(3)]
+    lib().hsro_pppts(tdpts, C.c_float(tdr), np.ascontiguousarray(xc, np.float32), *out)
+    return np.stack(out, 1)
 
 
 def map1_factor(pp, tau, adot, alpha) -> float:

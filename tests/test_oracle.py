@@ -287,6 +287,118 @@ def test_float_kernel_vs_fp64():
     assert rel(s16[sel], f64[sel]) < 1e-5
 
 
+# ------------------------------------------------- quadrupole tree (T = 12)
+def _icosahedron():
+    """Cyclic permutations of (0, +-phi, +-1), normalised."""
+    phi = (1 + 5 ** 0.5) / 2
+    v = []
+    for a in (-phi, phi):
+        for b in (-1, 1):
+            v += [(0, a, b), (a, b, 0), (b, 0, a)]
+    v = np.array(v) / np.sqrt(1 + phi ** 2)
+    return v
+
+
+def test_quadrupole_design_is_the_icosahedron():
+    """The 12 pseudo-particle directions are the icosahedron's vertices (a
+    spherical 4-design): unit vectors, sum zero, second moment K/3 I."""
+    P = O.pppts(12, 1.0, np.zeros(3, np.float32)).astype(np.float64)
+    assert np.allclose(np.linalg.norm(P, axis=1), 1, atol=1e-6)
+    ico = _icosahedron()
+    d = np.linalg.norm(P[:, None, :] - ico[None, :, :], axis=2)
+    assert np.all(d.min(1) < 1e-6) and len(set(d.argmin(1))) == 12
+    assert np.allclose(P.sum(0), 0, atol=1e-6)
+    assert np.allclose(P.T @ P, 4 * np.eye(3), atol=1e-5)
+    # a 4-design: the fourth moment along any axis is K/5
+    a = np.random.default_rng(0).normal(size=3)
+    a /= np.linalg.norm(a)
+    assert abs(np.sum((P @ a) ** 4) - 12 / 5) < 1e-5
+
+
+def _moments(pos, m, c):
+    d = pos.astype(np.float64) - c
+    m = m.astype(np.float64)
+    M0 = m.sum()
+    M1 = (m[:, None] * d).sum(0)
+    Q = (m[:, None, None] * (3 * d[:, :, None] * d[:, None, :]
+                             - (d ** 2).sum(1)[:, None, None] * np.eye(3))).sum(0)
+    return M0, M1, Q
+
+
+@pytest.mark.parametrize("n,nd,seed", [(6000, 64, 31), (20000, 32, 32)])
+def test_quadrupole_pseudo_particles_reproduce_moments(n, nd, seed):
+    """Every node's 12 pseudo-particles carry its members' mass, dipole and
+    traceless quadrupole about xc (the point of RCBQuadrupoleForceTree); a
+    wrong design constant, weight (1/K, 3/K, 5/K, the 0.5) or sign fails."""
+    rng = np.random.default_rng(seed)
+    x, y, z = uniform(n, 10.0, seed)
+    m = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    tp = small_params(10.0, nd)
+    tp.tdpts = 12
+    tree = O.build_tree(tp, x, y, z, m)
+    t, ppm = tree.nodes, tree.ppm_t
+    assert ppm.shape == (t.shape[0], 12)
+    assert np.array_equal(t["ppm"][:, 0], ppm[:, 0])
+    pos = np.stack([tree.x, tree.y, tree.z], 1)
+    checked = 0
+    worst = [0.0, 0.0, 0.0]
+    for k in range(t.shape[0]):
+        cnt = t["count"][k]
+        if cnt <= 12:
+            assert np.all(ppm[k] == 0)
+            continue
+        leaf = t["cl"][k] == 0 and t["cr"][k] == 0
+        if leaf and cnt > nd:          # degenerate: no pseudo mass
+            assert np.all(ppm[k] == 0)
+            continue
+        c = t["xc"][k].astype(np.float64)
+        sl = slice(t["offset"][k], t["offset"][k] + cnt)
+        M0, M1, Q = _moments(pos[sl], tree.mass[sl], c)
+        tdr = float(t["tdr"][k])
+        P = O.pppts(12, tdr, t["xc"][k])
+        P0, P1, PQ = _moments(P, ppm[k], c)
+        # float rounding scales with the weights' magnitude: members far
+        # outside the tdr sphere carry weights ~ (r/tdr)^2 of either sign
+        rho = np.sqrt(((pos[sl] - c) ** 2).sum(1)) / tdr
+        W = float(np.sum(tree.mass[sl] * (1 + 3 * rho + 5 * rho ** 2)))
+        worst[0] = max(worst[0], abs(P0 - M0) / W)
+        worst[1] = max(worst[1], np.abs(P1 - M1).max() / (W * tdr))
+        worst[2] = max(worst[2], np.abs(PQ - Q).max() / (W * tdr * tdr))
+        checked += 1
+    assert checked > 20
+    # rounding-level agreement (a wrong weight or design point gives O(1))
+    assert worst[0] < 2e-5 and worst[1] < 2e-5 and worst[2] < 2e-5, worst
+
+
+def test_quadrupole_tree_force_beats_monopole():
+    """At a wide opening angle the quadrupole tree's force is much closer to
+    the direct sum than the monopole tree's, and its lists count 12 entries
+    per pseudo-particle and cover every particle at most once."""
+    x, y, z = uniform(8000, 12.0, 33)
+    errs = {}
+    for T in (1, 12):
+        tp = small_params(12.0, 8, oa=0.8)
+        tp.tdpts = T
+        tree = O.build_tree(tp, x, y, z)
+        lists = O.build_lists(tp, tree)
+        sb = lists.seg_b
+        assert np.any(sb < 0)
+        for k in range(lists.leaves.shape[0]):
+            a, b = lists.seg_ptr[k], lists.seg_ptr[k + 1]
+            assert lists.entries[k] == np.sum(np.where(sb[a:b] >= 0, sb[a:b], T))
+            assert np.all(tree.nodes["count"][lists.seg_a[a:b][sb[a:b] < 0]] > T)
+        ax, ay, az = O.forces_f64(tp, tree, lists)
+        targets = np.concatenate([np.arange(tree.nodes[l]["offset"], tree.nodes[l]["offset"] + tree.nodes[l]["count"])
+                                  for l in lists.leaves])
+        dx, dy, dz = O.direct_f64(tree.x, tree.y, tree.z, tree.mass, targets)
+        G = np.stack([ax[targets], ay[targets], az[targets]], 1)
+        D = np.stack([dx, dy, dz], 1)
+        errs[T] = np.sqrt(np.sum((G - D) ** 2) / np.sum(D ** 2))
+        f32 = np.stack(O.forces_step16(tp, tree, lists), 1)[targets]
+        assert np.sqrt(np.sum((f32 - G) ** 2) / np.sum(G ** 2)) < 1e-5
+    assert errs[12] < 0.5 * errs[1], errs
+
+
 def test_cell_sort_is_stable_counting_sort():
     rng = np.random.default_rng(10)
     x = (rng.random(5000) * 9.0 - 0.5).astype(np.float32)
