@@ -325,8 +325,8 @@ hsr_status hsr_create(hsr_ctx **out, const hsr_params *p) {
     return HSR_ERR_ARG;
   }
   *out = nullptr;
-  if (p->tdpts != 1) {
-    g_err = "hsr_create: only TDPTS = 1 (RCBMonopoleForceTree) is implemented on the device";
+  if (p->tdpts != 1 && p->tdpts != 12) {
+    g_err = "hsr_create: TDPTS must be 1 (RCBMonopoleForceTree) or 12 (RCBQuadrupoleForceTree)";
     return HSR_ERR_UNSUPPORTED;
   }
   if (p->nd < 1) {
@@ -374,7 +374,7 @@ hsr_status hsr_destroy(hsr_ctx *c) {
   if (!c) return HSR_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
-  DBuf *bufs[] = {&c->tree.bnodes, &c->tree.canon_nodes, &c->tree.pfin, &c->tree.pj,
+  DBuf *bufs[] = {&c->tree.bnodes, &c->tree.canon_nodes, &c->tree.pfin, &c->tree.pj, &c->tree.ppm12,
                   &c->tree.xs, &c->tree.ys, &c->tree.zs, &c->tree.ms,
                   &c->lists.leaves, &c->lists.seg_ptr, &c->lists.seg, &c->lists.ent_ptr,
                   &c->lists.jidx, &c->lists.items, &c->lists.stack, &c->lists.item_keys,
@@ -588,6 +588,10 @@ hsr_status hsr_adjoint(hsr_ctx *c, const float *lx, const float *ly,
     g_err = "hsr_adjoint: null argument";
     return HSR_ERR_ARG;
   }
+  if (c->p.tdpts != 1) {
+    g_err = "hsr_adjoint: the adjoint covers the monopole tree (TDPTS = 1)";
+    return HSR_ERR_UNSUPPORTED;
+  }
   const int64_t n = c->tree.n;
   const float *dl[3] = {lx, ly, lz};
   float *dg[3] = {gx, gy, gz};
@@ -727,6 +731,29 @@ hsr_status hsr_get_stats(hsr_ctx *c, hsr_stats *st) {
   st->subcycle_ms = c->sub.subcycle_ms;
   st->drift_factor = c->sub.drift_factor;
   st->kick_coeff = c->sub.kick_coeff;
+  return HSR_OK;
+}
+
+hsr_status hsr_get_pseudo_masses(hsr_ctx *c, float *ppm, int64_t cap) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (!c->built) {
+    g_err = "hsr_get_pseudo_masses: no tree";
+    return HSR_ERR_STATE;
+  }
+  const int64_t nn = c->tree.nnodes, T = c->p.tdpts;
+  if (!ppm || cap < nn * T) {
+    g_err = "hsr_get_pseudo_masses: output holds fewer than nodes * TDPTS floats";
+    return HSR_ERR_ARG;
+  }
+  if (nn == 0) return HSR_OK;
+  if (T == 12) {
+    HSR_HIP(hipMemcpy(ppm, c->tree.ppm12.p, sizeof(float) * 12 * nn, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<hsr_node> nodes(nn);
+    HSR_HIP(hipMemcpy(nodes.data(), c->tree.canon_nodes.p, sizeof(hsr_node) * nn, hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < nn; ++k) ppm[k] = nodes[k].ppm[0];
+  }
   return HSR_OK;
 }
 

@@ -73,8 +73,8 @@ T *dptr(DBuf &b) {
   return reinterpret_cast<T *>(b.p);
 }
 
-// Sorted j-particle record (x, y, z, m) for the leaf kernel; pseudo-particles
-// of node k follow the particles at index count + k.
+// Sorted j-particle record (x, y, z, m) for the leaf kernel; the TDPTS
+// pseudo-particles of node k follow the particles at count + TDPTS k + j.
 struct __align__(16) PJ {
   float x, y, z, m;
 };
@@ -98,7 +98,8 @@ struct TreeState {
   DBuf bnodes;                // BNode[nnodes_bfs]
   DBuf canon_nodes;           // hsr_node[nnodes]
   DBuf pfin;                  // int32 final permutation (sorted -> input)
-  DBuf pj;                    // PJ[n + nnodes]
+  DBuf pj;                    // PJ[n + tdpts * nnodes]
+  DBuf ppm12;                 // float[12 * nnodes]: TDPTS = 12 pseudo masses
   DBuf xs, ys, zs, ms;        // sorted float arrays
   int64_t nboxes = 1;
   std::vector<int64_t> box_root;  // BFS id of each box root

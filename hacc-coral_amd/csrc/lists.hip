@@ -1,12 +1,13 @@
 // lists.hip -- bit-exact per-leaf interaction lists on the GPU.
 //
-// Restates RCBForceTree<1>::calcInternodeForces / calcInternodeForce
+// Restates RCBForceTree<TDPTS>::calcInternodeForces / calcInternodeForce
 // (RCBForceTree.cxx:924-1197, useRealOA = false): for every leaf inside the
 // force box, a right-child-first LIFO walk from the root that
 //   * expands the leaf's ancestors to their children (except the leaf),
 //   * accepts a node whose angular size passes the tan(oa) test and drops
 //     it if its centre is farther than fsrrmax; an accepted node with
-//     count <= TDPTS contributes its particles, else its pseudo-particle,
+//     count <= TDPTS contributes its particles, else its TDPTS
+//     pseudo-particles (TDPTS = 1 or 12),
 //   * takes a not-accepted leaf's particles,
 //   * pushes the children of a not-accepted internal node whose L-infinity
 //     gap to the target leaf is <= fsrrmax,
@@ -28,6 +29,7 @@ namespace {
 
 struct ListParams {
   float fsrrmax, fsrrmax2, tan_oa;
+  int tdpts;                 // pseudo-particles per node (1 or 12)
   int64_t stack_depth;
   int64_t nthreads;
   int64_t *overflow;
@@ -174,11 +176,11 @@ k_walk(const hsr_node *__restrict__ t, const int32_t *__restrict__ leaves,
     }
     if (!looksBig) {
       if (dist2 > fsrrmax2) continue;
-      if (N.count <= 1) {
+      if (N.count <= lp.tdpts) {
         EMIT(N.offset, N.count, N.count);
         continue;
       }
-      EMIT(tln, -1, 1);
+      EMIT(tln, -1, lp.tdpts);
       continue;
     } else if (N.cr == 0 && N.cl == 0) {
       EMIT(N.offset, N.count, N.count);
@@ -213,9 +215,10 @@ k_walk(const hsr_node *__restrict__ t, const int32_t *__restrict__ leaves,
 }
 
 // Expand segments into j indices (one wave per listed leaf): particle
-// ranges by all lanes, pseudo-particles as index count + node.
+// ranges by all lanes, the T pseudo-particles of node a as indices
+// count + T a + j (RCBForceTree.cxx:1051-1060).
 __global__ void __launch_bounds__(64)
-k_expand(int64_t nleaves, int64_t n, const int64_t *seg_ptr,
+k_expand(int64_t nleaves, int64_t n, int tdpts, const int64_t *seg_ptr,
          const int64_t *seg_a, const int64_t *seg_b, const int64_t *ent_ptr,
          int32_t *jidx) {
   int64_t k = blockIdx.x;
@@ -225,8 +228,8 @@ k_expand(int64_t nleaves, int64_t n, const int64_t *seg_ptr,
   for (int64_t s = seg_ptr[k]; s < seg_ptr[k + 1]; ++s) {
     int64_t a = seg_a[s], b = seg_b[s];
     if (b < 0) {
-      if (l == 0) jidx[e] = (int32_t)(n + a);
-      e += 1;
+      if (l < tdpts) jidx[e + l] = (int32_t)(n + tdpts * a + l);
+      e += tdpts;
     } else {
       for (int64_t i = l; i < b; i += 64) jidx[e + i] = (int32_t)(a + i);
       e += b;
@@ -331,6 +334,7 @@ hsr_status lists_build_device(hsr_ctx *c, const float *min_force_h,
   lp.fsrrmax = c->p.fsrrmax;
   lp.fsrrmax2 = c->p.fsrrmax * c->p.fsrrmax;
   lp.tan_oa = c->p.tan_oa;
+  lp.tdpts = c->p.tdpts;
   lp.stack_depth = depth;
   lp.nthreads = std::max<int64_t>(1, nl);
   HSR_HIP(dbuf_reserve(S.counters, 256));
@@ -363,7 +367,7 @@ hsr_status lists_build_device(hsr_ctx *c, const float *min_force_h,
   if (nl > 0) {
     k_walk<true><<<nblk(nl, 64), 64, 0, st>>>(cn, leaves, leaf_box, d_cbase, nl, lp, stk, nullptr, nullptr,
                                                dptr<int64_t>(LS.seg_ptr), seg_a, seg_b, seg_e);
-    k_expand<<<(unsigned)nl, 64, 0, st>>>(nl, n, dptr<int64_t>(LS.seg_ptr), seg_a, seg_b,
+    k_expand<<<(unsigned)nl, 64, 0, st>>>(nl, n, c->p.tdpts, dptr<int64_t>(LS.seg_ptr), seg_a, seg_b,
                                           dptr<int64_t>(LS.ent_ptr), dptr<int32_t>(LS.jidx));
   }
   // pairs = sum count * list length

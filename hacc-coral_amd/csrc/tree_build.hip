@@ -813,6 +813,116 @@ __global__ void k_gather_sorted(int64_t n, const int32_t *Pfin,
   pj[k] = PJ{a, b, c, d};
 }
 
+// ------------------------------------------------- quadrupole (TDPTS = 12)
+// Hardin and Sloane's 12-point spherical 4-design (the icosahedron's
+// vertices) as the reference tabulates it (RCBForceTree.cxx:229-272), in
+// its order and rounded to float as its POSVEL_T arrays are.
+#define A12 ((float)0.525731112119134)
+#define B12 ((float)0.85065080835204)
+__constant__ float kSph12X[12] = {0.f, 0.f, A12, -A12, B12, -B12, 0.f, 0.f, -A12, A12, -B12, B12};
+__constant__ float kSph12Y[12] = {B12, B12, 0.f, 0.f, A12, A12, -B12, -B12, 0.f, 0.f, -A12, -A12};
+__constant__ float kSph12Z[12] = {A12, -A12, B12, B12, 0.f, 0.f, -A12, A12, -B12, -B12, 0.f, 0.f};
+#undef A12
+#undef B12
+
+// One term of pp<12> (RCBForceTree.cxx:549-565): the weight of a member at
+// (x, y, z) with mass w for design point (pjx, pjy, pjz) of a node centred
+// at xc with radius tdr.  odr2's product with the double literal 0.5 is one
+// rounding of t2 * u / 2 in the reference; (t2 * 0.5f) * u rounds the same
+// exact value once whenever t2 / 2 is a normal float, and when it is not,
+// |odr2| < 2^-100 is absorbed by odr0 + odr1 ~ 1/12 either way.
+__device__ __forceinline__ float pp12_term(float x, float y, float z, float w,
+                                           const float *xc, float pjx, float pjy,
+                                           float pjz, float tdr) {
+  const float K = 12.0f;
+  const float odr0 = 1 / K;
+  const float xi = x - xc[0], yi = y - xc[1], zi = z - xc[2];
+  const float ri = sqrtf(xi * xi + yi * yi + zi * zi);
+  const float xj = pjx - xc[0], yj = pjy - xc[1], zj = pjz - xc[2];
+  const float rj2 = xj * xj + yj * yj + zj * zj;
+  float odr1 = 0.f, odr2 = 0.f;
+  if (rj2 != 0.f) {
+    const float rj = sqrtf(rj2);
+    const float aij = (xi * xj + yi * yj + zi * zj) / (ri * rj);
+    odr1 = (3 / K) * (ri / tdr) * aij;
+    const float t2 = (5 / K) * (ri / tdr) * (ri / tdr);
+    const float u = 3 * aij * aij - 1;
+    odr2 = (t2 * 0.5f) * u;
+  }
+  return w * (odr0 + odr1 + odr2);
+}
+
+// pp<12> of every node of one BFS level (deepest level first), one thread
+// per (node, design point): leaves with count > 12 from their members
+// (RCBForceTree.cxx:787-797), split nodes from each child, left then right
+// -- its members when count <= 12, else its 12 pseudo-particles
+// (:856-889) -- accumulated in that order; degenerate nodes and leaves with
+// count <= 12 keep 0.  Reads the canonical nodes and the particles in tree
+// order; writes ppm12[12 * canonical id + j].
+__global__ void k_pp12_level(const BNode *nodes, int64_t lb, int64_t le,
+                             const int64_t *box_cbase, const int32_t *box_of,
+                             hsr_node *cn, const float *xs, const float *ys,
+                             const float *zs, const float *ms, float *ppm12) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t t = lb + g / 12;
+  const int j = (int)(g % 12);
+  if (t >= le) return;
+  const BNode &b = nodes[t];
+  const int64_t k = box_cbase[box_of[t]] + b.canon;
+  const hsr_node &o = cn[k];
+  const float pjx = o.tdr * kSph12X[j] + o.xc[0];
+  const float pjy = o.tdr * kSph12Y[j] + o.xc[1];
+  const float pjz = o.tdr * kSph12Z[j] + o.xc[2];
+  float acc = 0.f;
+  if (b.kind == NODE_LEAF) {
+    if (o.count > 12)
+      for (int64_t i = o.offset; i < o.offset + o.count; ++i)
+        acc += pp12_term(xs[i], ys[i], zs[i], ms[i], o.xc, pjx, pjy, pjz, o.tdr);
+  } else if (b.kind == NODE_SPLIT) {
+    const int64_t ch[2] = {o.cl, o.cr};
+    for (int q = 0; q < 2; ++q) {
+      const hsr_node &c = cn[ch[q]];
+      if (c.count <= 0) continue;
+      if (c.count <= 12) {
+        for (int64_t i = c.offset; i < c.offset + c.count; ++i)
+          acc += pp12_term(xs[i], ys[i], zs[i], ms[i], o.xc, pjx, pjy, pjz, o.tdr);
+      } else {
+        const float *cm = ppm12 + 12 * ch[q];
+        for (int i = 0; i < 12; ++i) {
+          const float cx = c.tdr * kSph12X[i] + c.xc[0];
+          const float cy = c.tdr * kSph12Y[i] + c.xc[1];
+          const float cz = c.tdr * kSph12Z[i] + c.xc[2];
+          acc += pp12_term(cx, cy, cz, cm[i], o.xc, pjx, pjy, pjz, o.tdr);
+        }
+      }
+    }
+  }
+  ppm12[12 * k + j] = acc;
+  if (j == 0) cn[k].ppm[0] = acc;  // the node table's mass slot holds ppm[0] of the 12
+}
+
+// Pseudo-particle records of TDPTS = 12: node k's design point j at
+// tdr * design + xc with mass ppm12, at index count + 12 k + j.
+__global__ void k_pseudo_pj12(int64_t n, int64_t nc, const hsr_node *cn, const float *ppm12,
+                              PJ *pj) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= 12 * nc) return;
+  const int64_t k = g / 12;
+  const int j = (int)(g % 12);
+  const hsr_node &o = cn[k];
+  PJ r;
+  r.x = o.tdr * kSph12X[j] + o.xc[0];
+  r.y = o.tdr * kSph12Y[j] + o.xc[1];
+  r.z = o.tdr * kSph12Z[j] + o.xc[2];
+  r.m = ppm12[g];
+  pj[n + g] = r;
+}
+
+__global__ void k_zero_f32(int64_t n, float *a) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < n) a[k] = 0.f;
+}
+
 // Pseudo-particle records (pppts with the TDPTS = 1 design point (0,0,0):
 // position tdr*0 + xc, mass ppm), RCBForceTree.cxx:525-534, 1057-1060.
 __global__ void k_pseudo_pj(int64_t n, int64_t nc, const hsr_node *cn,
@@ -1045,12 +1155,29 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
   // ---- sorted arrays and j-particle records
   for (DBuf *bb : {&T.xs, &T.ys, &T.zs, &T.ms})
     HSR_HIP(dbuf_reserve(*bb, sizeof(float) * nn));
-  HSR_HIP(dbuf_reserve(T.pj, sizeof(PJ) * (n + T.nnodes + 1)));
+  const int tdpts = c->p.tdpts;
+  HSR_HIP(dbuf_reserve(T.pj, sizeof(PJ) * (n + tdpts * T.nnodes + 1)));
   if (n > 0)
     k_gather_sorted<<<nblk_for(n, 256), 256, 0, st>>>(n, Pfin, x, y, z, m, dptr<float>(T.xs), dptr<float>(T.ys),
                                                      dptr<float>(T.zs), dptr<float>(T.ms), dptr<PJ>(T.pj));
-  if (T.nnodes > 0)
+  if (tdpts == 12) {
+    // quadrupole pseudo-particles, deepest BFS level first; the orphan
+    // slots of degenerate nodes keep the zeros
+    HSR_HIP(dbuf_reserve(T.ppm12, sizeof(float) * 12 * std::max<int64_t>(T.nnodes, 1)));
+    float *ppm12 = dptr<float>(T.ppm12);
+    if (T.nnodes > 0) k_zero_f32<<<nblk_for(12 * T.nnodes, 256), 256, 0, st>>>(12 * T.nnodes, ppm12);
+    for (int l = nlev - 1; l >= 0; --l) {
+      int64_t lb = T.level_start[l], le = T.level_start[l + 1];
+      if (le > lb)
+        k_pp12_level<<<nblk_for(12 * (le - lb), 256), 256, 0, st>>>(
+            nodes, lb, le, d_cbase, box_of, cn, dptr<float>(T.xs), dptr<float>(T.ys), dptr<float>(T.zs),
+            dptr<float>(T.ms), ppm12);
+    }
+    if (T.nnodes > 0)
+      k_pseudo_pj12<<<nblk_for(12 * T.nnodes, 256), 256, 0, st>>>(n, T.nnodes, cn, ppm12, dptr<PJ>(T.pj));
+  } else if (T.nnodes > 0) {
     k_pseudo_pj<<<nblk_for(T.nnodes, 256), 256, 0, st>>>(n, T.nnodes, cn, dptr<PJ>(T.pj));
+  }
   HSR_HIP(hipGetLastError());
   return HSR_OK;
 }

@@ -28,7 +28,7 @@ EXPORTS = [
     "hsr_adjoint", "hsr_get_stats", "hsr_get_nodes", "hsr_get_perm",
     "hsr_get_lists", "hsr_build_many", "hsr_lists_many", "hsr_set_tuning",
     "hsr_get_tuning", "hsr_forces_f64", "hsr_leaf_momentum", "hsr_drift",
-    "hsr_resort", "hsr_map2", "hsr_subcycle",
+    "hsr_resort", "hsr_map2", "hsr_subcycle", "hsr_get_pseudo_masses",
 ]
 
 
@@ -131,6 +131,7 @@ def load(path: str | None = None):
     L.hsr_subcycle.argtypes = [vp, C.POINTER(SubcycleParams), i64] + [vp] * 10 + [vp, C.c_int]
     L.hsr_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.hsr_get_nodes.argtypes = [vp, vp, i64]
+    L.hsr_get_pseudo_masses.argtypes = [vp, vp, i64]
     L.hsr_get_perm.argtypes = [vp, vp, i64]
     L.hsr_get_lists.argtypes = [vp, vp, vp, vp, vp, vp, i64, i64]
     L.hsr_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
@@ -344,6 +345,15 @@ class Context:
         st = self.stats()
         out = np.zeros(max(st.nodes, 1), NODE_DTYPE)
         check(self.L.hsr_get_nodes(self.h, out.ctypes.data, out.shape[0]), "hsr_get_nodes")
+        return out[: st.nodes]
+
+    def pseudo_masses(self) -> np.ndarray:
+        """(nodes, TDPTS) pseudo-particle masses in canonical node order."""
+        st = self.stats()
+        T = int(self.params.tdpts)
+        out = np.zeros((max(st.nodes, 1), T), np.float32)
+        check(self.L.hsr_get_pseudo_masses(self.h, out.ctypes.data, out.size),
+              "hsr_get_pseudo_masses")
         return out[: st.nodes]
 
     def perm(self) -> np.ndarray:

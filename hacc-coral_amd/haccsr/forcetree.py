@@ -92,9 +92,6 @@ class RCBForceTree:
                  yLoc, zLoc, xVel, yVel, zVel, mass, phiLoc, idLoc, maskLoc,
                  avgMass, fsm, r, oa, nd=1, ds=1, tmin=128, fl=None,
                  fcoeff=0.0, ppc=0.9, kernel: str = "step16", device: int = 0):
-        if self.TDPTS != 1:
-            raise _lib.HsrError("RCBQuadrupoleForceTree (TDPTS = 12) is not "
-                                "implemented by the device path")
         if fl is None:
             fcoeff = 1.0  # RCBForceTree.cxx:399-404
         law = _law_id(fl, kernel)
@@ -102,7 +99,8 @@ class RCBForceTree:
         if isinstance(fl, ForceLawSR) and kernel != "step16":
             rsm = fl.rsm
         self.ctx = _lib.Context(nd=nd, fsrrmax=fsm, rsm=rsm, oa=oa, ppc=ppc,
-                                law=law, device=device, ds=ds, tmin=tmin)
+                                law=law, device=device, ds=ds, tmin=tmin,
+                                tdpts=self.TDPTS)
         arrs = [xLoc, yLoc, zLoc, xVel, yVel, zVel, mass, phiLoc]
         for a in arrs:
             if a.dtype != np.float32 or not a.flags["C_CONTIGUOUS"]:

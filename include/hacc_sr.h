@@ -9,7 +9,8 @@
  * Reference surface replaced (BenWibking/hacc-coral, src/halo_finder):
  *   RCBForceTree<TDPTS>::RCBForceTree(...)   RCBForceTree.h:96-124,
  *                                             RCBForceTree.cxx:336-450
- *       = hsr_create + hsr_build + hsr_lists + hsr_kick (+ hsr_destroy)
+ *       = hsr_create + hsr_build + hsr_lists + hsr_kick (+ hsr_destroy),
+ *         TDPTS 1 (RCBMonopoleForceTree) and 12 (RCBQuadrupoleForceTree)
  *   ForceLaw / ForceLawSR / FGridEvalPoly    ForceLaw.h:95-127
  *       = hsr_params.law (+ coefficients)
  *   extern "C" cm(...)                        RCBForceTree.cxx:514-516
@@ -61,7 +62,8 @@ enum {
 
 typedef struct {
   int device;            /* HIP device ordinal */
-  int tdpts;             /* pseudo-particles per node: 1 (RCBMonopoleForceTree) */
+  int tdpts;             /* pseudo-particles per node: 1 (RCBMonopoleForceTree)
+                            or 12 (RCBQuadrupoleForceTree; no hsr_adjoint) */
   int law;               /* HSR_LAW_* */
   int reserved0;
   int64_t nd;            /* leaf size ("nDirect", ctor arg nd) */
@@ -159,6 +161,10 @@ typedef struct {
 hsr_status hsr_get_stats(hsr_ctx *ctx, hsr_stats *st);
 /* Tree in canonical numbering (host buffer of stats.nodes entries). */
 hsr_status hsr_get_nodes(hsr_ctx *ctx, hsr_node *out, int64_t cap);
+/* Pseudo-particle masses, TDPTS per node in canonical node order
+ * (RCBForceTree.h:139, ppm[TDPTS]); cap counts floats.  hsr_node.ppm[0]
+ * holds the first of them. */
+hsr_status hsr_get_pseudo_masses(hsr_ctx *ctx, float *ppm, int64_t cap);
 /* perm[k] = input index of the particle now at position k. */
 hsr_status hsr_get_perm(hsr_ctx *ctx, int64_t *perm, int64_t cap);
 /* Lists in the reference's order: listed leaves in the order

@@ -2,7 +2,7 @@
 // (Particles::map2, Particles.cxx:1311-1335) against the MI355X
 // RCBForceTree.  Writes the permuted arrays and kicked velocities to a raw
 // file for comparison with the oracle:
-//   test_dropin <n> <box> <nd> <seed> <out.bin>
+//   test_dropin <n> <box> <nd> <seed> <out.bin> [tdpts: 1 (default) or 12]
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -42,11 +42,20 @@ int main(int argc, char **argv) {
   ForceLawSR fl(&poly, 0.1f);
   const float fsrrmax = fg.rmax();
   const float c = 0.5f;  // fcoeff
-  RCBMonopoleForceTree *sft = new RCBMonopoleForceTree(
-      zero, hi, lo_f, hi_f, n, x.data(), y.data(), z.data(), vx.data(),
-      vy.data(), vz.data(), m.data(), phi.data(), id.data(), mask.data(), 1.0,
-      fsrrmax, 0.1f, 0.1f, nd, 2, 128, &fl, c);
-  delete sft;
+  const int tdpts = argc > 6 ? atoi(argv[6]) : 1;
+  if (tdpts == 12) {
+    RCBQuadrupoleForceTree *sft = new RCBQuadrupoleForceTree(
+        zero, hi, lo_f, hi_f, n, x.data(), y.data(), z.data(), vx.data(),
+        vy.data(), vz.data(), m.data(), phi.data(), id.data(), mask.data(), 1.0,
+        fsrrmax, 0.1f, 0.1f, nd, 2, 128, &fl, c);
+    delete sft;
+  } else {
+    RCBMonopoleForceTree *sft = new RCBMonopoleForceTree(
+        zero, hi, lo_f, hi_f, n, x.data(), y.data(), z.data(), vx.data(),
+        vy.data(), vz.data(), m.data(), phi.data(), id.data(), mask.data(), 1.0,
+        fsrrmax, 0.1f, 0.1f, nd, 2, 128, &fl, c);
+    delete sft;
+  }
   FILE *f = fopen(argv[5], "wb");
   if (!f) return 1;
   fwrite(x0.data(), 4, n, f); fwrite(y0.data(), 4, n, f); fwrite(z0.data(), 4, n, f);

@@ -80,7 +80,7 @@ def test_create_validates_arguments_before_device_work():
     h = C.c_void_p()
     assert L.hsr_create(None, None) == _lib.HSR_ERR_ARG
     p = _lib.Params()
-    p.tdpts, p.nd, p.fsrrmax, p.rsm, p.ppc = 12, 128, 3.12, 0.1, 0.9
+    p.tdpts, p.nd, p.fsrrmax, p.rsm, p.ppc = 2, 128, 3.12, 0.1, 0.9
     assert L.hsr_create(C.byref(h), C.byref(p)) == _lib.HSR_ERR_UNSUPPORTED
     assert b"TDPTS" in L.hsr_last_error()
     p.tdpts, p.nd = 1, 0

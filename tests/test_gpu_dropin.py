@@ -15,19 +15,21 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "hacc-coral_amd", "bin", "test_dropin")
 
 
-def test_cxx_dropin_constructor(tmp_path):
+@pytest.mark.parametrize("tdpts", [1, 12])
+def test_cxx_dropin_constructor(tmp_path, tdpts):
     n, box, nd, seed, fcoeff = 30000, 14.0, 64, 7, 0.5
-    print(f"case C++ drop-in n={n} nd={nd}", flush=True)
+    print(f"case C++ drop-in n={n} nd={nd} TDPTS={tdpts}", flush=True)
     out = tmp_path / "dropin.bin"
-    subprocess.run([EXE, str(n), str(box), str(nd), str(seed), str(out)], check=True,
-                   timeout=120)
+    subprocess.run([EXE, str(n), str(box), str(nd), str(seed), str(out), str(tdpts)],
+                   check=True, timeout=120)
     raw = np.fromfile(out, dtype=np.uint8)
     f = raw[: 9 * 4 * n].view(np.float32).reshape(9, n)
     ids = raw[9 * 4 * n:].view(np.int64)
     x0, y0, z0, x, y, z, vx, vy, vz = f
     tp = O.TreeParams(np.zeros(3, np.float32), np.full(3, box, np.float32),
                       np.full(3, 1.0, np.float32), np.full(3, box - 1.0, np.float32),
-                      fsrrmax=float(np.float32(3.116326355)), rsm=0.1, oa=0.1, nd=nd)
+                      fsrrmax=float(np.float32(3.116326355)), rsm=0.1, oa=0.1, nd=nd,
+                      tdpts=tdpts)
     tree = O.build_tree(tp, x0, y0, z0)
     assert np.array_equal(ids, tree.id)
     assert np.array_equal(x.view(np.uint32), tree.x.view(np.uint32))
