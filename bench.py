@@ -32,7 +32,7 @@ PEAK_HBM_GBPS = 8000.0       # MI355X HBM3E
 # HBM bytes per k_force launch measured with rocprofv3 (FETCH_SIZE x 2.0 + WRITE_SIZE
 # x 1.0, factors from a 256 MiB device copy) on the 256^3 clustered load at leaf size
 # 512: profiles/k_force_256c_traffic.md.  Reported only for that workload.
-MEASURED_TRAFFIC = {("B", 256, 512): 149282159677}
+MEASURED_TRAFFIC = {("B", 256, 512, 1): 149282159677}
 # Algorithmic bytes of the tree build: 48 B per particle per level it is split at
 # (cm reads x, y, z, m; partition reads and writes key + index) plus one 84 B
 # gather of the ten payload arrays.
@@ -54,6 +54,9 @@ def parse():
                     help="timed hsr_adjoint calls on the last step's tree (0 = skip)")
     ap.add_argument("--adjoint-mode", type=int, default=0,
                     help="hsr_tuning.adjoint_mode: 0 automatic, 1 fused, 2 two-pass")
+    ap.add_argument("--tdpts", type=int, default=1, choices=(1, 12),
+                    help="pseudo-particles per node: 1 (monopole, the bench line) or 12 "
+                         "(quadrupole; no adjoint leg)")
     ap.add_argument("--subcycle-nsub", type=int, default=5,
                     help="sub-steps of the device-resident sub-cycle leg (0: skip it)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -275,7 +278,7 @@ def main():
     def step(nd):
         ctx = ctxs.get(nd)
         if ctx is None:
-            ctx = ctxs[nd] = Context(nd=nd)
+            ctx = ctxs[nd] = Context(nd=nd, tdpts=a.tdpts)
             ctx.set_tuning(use_graph=1)
         for k in src:
             work[k].copy_(src[k])
@@ -347,13 +350,15 @@ def main():
         "roofline": {"bound": "mfma", "pipe": "FP32 VALU (no MFMA: the pair force is not a contraction)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": MEASURED_TRAFFIC.get((a.config, a.n_side or 256, nd)),
+                     "traffic": MEASURED_TRAFFIC.get((a.config, a.n_side or 256, nd, a.tdpts)),
                      "kernel": "k_force (leaf kernel)", "flops_per_pair": FLOPS_PER_PAIR,
                      "evaluated_pairs": evaluated,
                      "culled_fraction_of_list_pairs": 1.0 - evaluated / pairs if pairs else 0.0},
         "hbm_passes": build_roofline(ctxs[nd], n, float(np.mean([s[1] for s in stats]))),
     }
-    if a.adjoint_reps > 0:
+    if a.tdpts != 1:
+        out["config"]["workload"] += "; quadrupole tree (TDPTS 12)"
+    if a.adjoint_reps > 0 and a.tdpts == 1:
         ctxs[nd].set_tuning(adjoint_mode=a.adjoint_mode)
         out["adjoint"] = adjoint_leg(ctxs[nd], n, dev, a.adjoint_reps, stats)
         log(f"adjoint {out['adjoint']}")
