@@ -731,6 +731,8 @@ hsr_status hsr_get_stats(hsr_ctx *c, hsr_stats *st) {
   st->subcycle_ms = c->sub.subcycle_ms;
   st->drift_factor = c->sub.drift_factor;
   st->kick_coeff = c->sub.kick_coeff;
+  st->cic_ms = c->sub.cic_ms;
+  st->inverse_cic_ms = c->sub.inverse_cic_ms;
   return HSR_OK;
 }
 
@@ -916,6 +918,92 @@ hsr_status hsr_resort(hsr_ctx *c, int64_t count, const int32_t ng[3], float *x,
   float ms = 0.f;
   HSR_HIP(hipEventElapsedTime(&ms, c->sub.ev[2], c->sub.ev[3]));
   c->sub.resort_ms = ms;
+  return HSR_OK;
+}
+
+hsr_status hsr_cic(hsr_ctx *c, int64_t count, const float *x, const float *y,
+                   const float *z, const int32_t ng[3], float gpscal, float *rho,
+                   int ptr_flags) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (!x || !y || !z || !rho) {
+    g_err = "hsr_cic: null positions or grid";
+    return HSR_ERR_ARG;
+  }
+  if ((s = check_count_grid("hsr_cic", count, ng))) return s;
+  if (count > (int64_t)(INT32_MAX / 8)) {
+    g_err = "hsr_cic: at most 2^28 particles per call";
+    return HSR_ERR_ARG;
+  }
+  if ((s = sub_events(c))) return s;
+  hipStream_t st = c->stream;
+  const int64_t ncell = (int64_t)ng[0] * ng[1] * ng[2] + 1;
+  const float *d[3] = {x, y, z};
+  float *drho = rho;
+  const bool host = !(ptr_flags & HSR_DEVICE_PTRS);
+  if (host) {
+    const size_t nn = (size_t)std::max<int64_t>(count, 1);
+    for (int a = 0; a < 3; ++a) {
+      HSR_HIP(dbuf_reserve(c->s.host_in[a], sizeof(float) * nn));
+      float *b = dptr<float>(c->s.host_in[a]);
+      if (count) HSR_HIP(hipMemcpyAsync(b, d[a], sizeof(float) * count, hipMemcpyHostToDevice, st));
+      d[a] = b;
+    }
+    HSR_HIP(dbuf_reserve(c->s.host_in[3], sizeof(float) * ncell));
+    drho = dptr<float>(c->s.host_in[3]);
+  }
+  HSR_HIP(hipEventRecord(c->sub.ev[0], st));
+  if ((s = cic_run(c, count, d[0], d[1], d[2], ng, gpscal, drho))) return s;
+  HSR_HIP(hipEventRecord(c->sub.ev[1], st));
+  if (host) HSR_HIP(hipMemcpyAsync(rho, drho, sizeof(float) * ncell, hipMemcpyDeviceToHost, st));
+  HSR_HIP(hipStreamSynchronize(st));
+  float ms = 0.f;
+  HSR_HIP(hipEventElapsedTime(&ms, c->sub.ev[0], c->sub.ev[1]));
+  c->sub.cic_ms = ms;
+  return HSR_OK;
+}
+
+hsr_status hsr_inverse_cic(hsr_ctx *c, int64_t count, const float *x, const float *y,
+                           const float *z, const int32_t ng[3], const float *grid, float tau,
+                           float fscal, float *v, int ptr_flags) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (!x || !y || !z || !grid || !v) {
+    g_err = "hsr_inverse_cic: null positions, grid or velocities";
+    return HSR_ERR_ARG;
+  }
+  if ((s = check_count_grid("hsr_inverse_cic", count, ng))) return s;
+  if ((s = sub_events(c))) return s;
+  hipStream_t st = c->stream;
+  const int64_t ncell = (int64_t)ng[0] * ng[1] * ng[2] + 1;
+  const float *d[3] = {x, y, z};
+  const float *dg = grid;
+  float *dv = v;
+  const bool host = !(ptr_flags & HSR_DEVICE_PTRS);
+  if (host) {
+    const size_t nn = (size_t)std::max<int64_t>(count, 1);
+    for (int a = 0; a < 3; ++a) {
+      HSR_HIP(dbuf_reserve(c->s.host_in[a], sizeof(float) * nn));
+      float *b = dptr<float>(c->s.host_in[a]);
+      if (count) HSR_HIP(hipMemcpyAsync(b, d[a], sizeof(float) * count, hipMemcpyHostToDevice, st));
+      d[a] = b;
+    }
+    HSR_HIP(dbuf_reserve(c->s.host_in[3], sizeof(float) * ncell));
+    float *bg = dptr<float>(c->s.host_in[3]);
+    HSR_HIP(hipMemcpyAsync(bg, grid, sizeof(float) * ncell, hipMemcpyHostToDevice, st));
+    dg = bg;
+    HSR_HIP(dbuf_reserve(c->s.host_in[4], sizeof(float) * nn));
+    dv = dptr<float>(c->s.host_in[4]);
+    if (count) HSR_HIP(hipMemcpyAsync(dv, v, sizeof(float) * count, hipMemcpyHostToDevice, st));
+  }
+  HSR_HIP(hipEventRecord(c->sub.ev[2], st));
+  if ((s = inverse_cic_run(c, count, d[0], d[1], d[2], ng, dg, tau, fscal, dv))) return s;
+  HSR_HIP(hipEventRecord(c->sub.ev[3], st));
+  if (host && count) HSR_HIP(hipMemcpyAsync(v, dv, sizeof(float) * count, hipMemcpyDeviceToHost, st));
+  HSR_HIP(hipStreamSynchronize(st));
+  float ms = 0.f;
+  HSR_HIP(hipEventElapsedTime(&ms, c->sub.ev[2], c->sub.ev[3]));
+  c->sub.inverse_cic_ms = ms;
   return HSR_OK;
 }
 

@@ -161,7 +161,7 @@ struct SubState {
   DBuf stage;      // gather staging of the ten arrays
   DBuf arr[10];    // x y z vx vy vz mass phi id mask on the device
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-  double drift_ms = 0, resort_ms = 0, subcycle_ms = 0;
+  double drift_ms = 0, resort_ms = 0, subcycle_ms = 0, cic_ms = 0, inverse_cic_ms = 0;
   float drift_factor = 0.f, kick_coeff = 0.f;  // last map1 / map2 factors
 };
 
@@ -231,3 +231,8 @@ hsr_status drift_run(hsr_ctx *c, int64_t n, float k, float *x, float *y, float *
 hsr_status resort_run(hsr_ctx *c, int64_t n, const int32_t ng[3], float *f[8],
                       int64_t *id, uint16_t *mask, int64_t *inside);
 hsr_status fill_run(hsr_ctx *c, int64_t n, float v, float *a);
+hsr_status cic_run(hsr_ctx *c, int64_t n, const float *x, const float *y, const float *z,
+                   const int32_t ng[3], float gpscal, float *rho);
+hsr_status inverse_cic_run(hsr_ctx *c, int64_t n, const float *x, const float *y,
+                           const float *z, const int32_t ng[3], const float *grid, float tau,
+                           float fscal, float *v);

@@ -1,0 +1,271 @@
+// pm.hip -- the particle-grid steps either side of the PM solve, on the
+// device: the cloud-in-cell deposit (Particles::cic, Particles.cxx:589-641)
+// and its interpolation back to the particles (Particles::inverse_cic,
+// Particles.cxx:645-714).
+//
+// Both follow the reference's arithmetic exactly: the weights ab, de, gh
+// are 1.0 + (ix - x) with a float difference and a double sum stored as
+// float; of the eight corner terms the first is a float product and the
+// other seven are double from their first (1.0 - w) factor on, and each is
+// added to its float accumulator in that precision.  That is the
+// reference's own use of double (its literals 1.0 promote the arithmetic);
+// compiled without FMA contraction.
+//
+// The deposit is a scatter whose result depends on the order of the float
+// additions into each cell.  To reproduce the reference's sequential order
+// in parallel it writes the 8 contributions of every particle (cell, term)
+// in (particle, corner) order, sorts them stably by cell, and adds each
+// cell's run in that order: one thread per short run, one wave per long run
+// (a dense clump), streaming the run 64 contributions at a time.  The
+// interpolation is a gather, one thread per particle.
+#include "hsr_internal.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <climits>
+
+namespace {
+
+inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+// (int)floorf(x) as the reference's x86 build converts it: NaN and values
+// outside the int range become INT_MIN (the conversion's indefinite value).
+__device__ __forceinline__ int x86_floor_int(float x) {
+  const float f = floorf(x);
+  return (f >= -2147483648.0f && f < 2147483648.0f) ? (int)f : INT_MIN;
+}
+
+struct Grid {
+  int ng0, ng1, ng2;
+  uint32_t safe;  // ng0 * ng1 * ng2: the slot of corners outside the grid
+};
+
+// Particles.cxx:371-395 with lo = 0, hi = ng.
+__device__ __forceinline__ uint32_t cell_of(int ix, int iy, int iz, const Grid &g) {
+  if (ix >= 0 && ix < g.ng0 && iy >= 0 && iy < g.ng1 && iz >= 0 && iz < g.ng2)
+    return ((uint32_t)ix * (uint32_t)g.ng1 + (uint32_t)iy) * (uint32_t)g.ng2 + (uint32_t)iz;
+  return g.safe;
+}
+
+struct Corners {
+  uint32_t cell[8];
+  float ab, de, gh;
+};
+
+// Particles.cxx:617-640: corners in the reference's order.
+__device__ __forceinline__ Corners corners(float xx, float yy, float zz, const Grid &g) {
+  Corners c;
+  const int ix = x86_floor_int(xx), iy = x86_floor_int(yy), iz = x86_floor_int(zz);
+  const int ip = ix + 1, jp = iy + 1, kp = iz + 1;
+  c.ab = (float)(1.0 + (double)((float)ix - xx));
+  c.de = (float)(1.0 + (double)((float)iy - yy));
+  c.gh = (float)(1.0 + (double)((float)iz - zz));
+  c.cell[0] = cell_of(ix, iy, iz, g);
+  c.cell[1] = cell_of(ix, jp, iz, g);
+  c.cell[2] = cell_of(ix, jp, kp, g);
+  c.cell[3] = cell_of(ix, iy, kp, g);
+  c.cell[4] = cell_of(ip, iy, kp, g);
+  c.cell[5] = cell_of(ip, jp, kp, g);
+  c.cell[6] = cell_of(ip, jp, iz, g);
+  c.cell[7] = cell_of(ip, iy, iz, g);
+  return c;
+}
+
+// Corner q's term with lead factor s, evaluated as the reference's
+// expression is: float products until the first (1.0 - w), double after.
+__device__ __forceinline__ double corner_term(int q, float s, float ab, float de, float gh) {
+  switch (q) {
+  case 0: return (double)(s * ab * de * gh);
+  case 1: return (double)(s * ab) * (1.0 - (double)de) * (double)gh;
+  case 2: return (double)(s * ab) * (1.0 - (double)de) * (1.0 - (double)gh);
+  case 3: return (double)(s * ab * de) * (1.0 - (double)gh);
+  case 4: return (double)s * (1.0 - (double)ab) * (double)de * (1.0 - (double)gh);
+  case 5: return (double)s * (1.0 - (double)ab) * (1.0 - (double)de) * (1.0 - (double)gh);
+  case 6: return (double)s * (1.0 - (double)ab) * (1.0 - (double)de) * (double)gh;
+  default: return (double)s * (1.0 - (double)ab) * (double)de * (double)gh;
+  }
+}
+
+// Contributions of particle nn at 8 nn + q: its corner cell (sort key), its
+// position in the sequential order (sort value) and its term.
+__global__ void __launch_bounds__(256)
+k_cic_contrib(int64_t n, const float *__restrict__ x, const float *__restrict__ y,
+              const float *__restrict__ z, Grid g, float c, uint32_t *__restrict__ key,
+              int32_t *__restrict__ val, double *__restrict__ term) {
+  const int64_t nn = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (nn >= n) return;
+  const Corners cc = corners(x[nn], y[nn], z[nn], g);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int64_t k = 8 * nn + q;
+    key[k] = cc.cell[q];
+    val[k] = (int32_t)k;
+    term[k] = corner_term(q, c, cc.ab, cc.de, cc.gh);
+  }
+}
+
+// Terms in sorted order, with the corner-0 flag folded into a byte.
+__global__ void __launch_bounds__(256)
+k_cic_gather(int64_t m, const int32_t *__restrict__ val, const double *__restrict__ term,
+             double *__restrict__ ts, uint8_t *__restrict__ is0) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= m) return;
+  const int32_t k = val[p];
+  ts[p] = term[k];
+  is0[p] = (k & 7) == 0;
+}
+
+// The reference's accumulation step for one contribution
+// (Particles.cxx:632-639: rhoArr[...] += float or double).
+__device__ __forceinline__ float cic_add(float r, double t, bool first) {
+  return first ? r + (float)t : (float)((double)r + t);
+}
+
+constexpr int RUN_SHORT = 64;
+
+// One thread per run of equal cells in the sorted contributions: runs of at
+// most RUN_SHORT are added in order here; longer runs are queued for a wave
+// each.
+__global__ void __launch_bounds__(256)
+k_cic_runs(int64_t m, const uint32_t *__restrict__ key, const double *__restrict__ ts,
+           const uint8_t *__restrict__ is0, float *__restrict__ rho,
+           int64_t *__restrict__ long_runs, unsigned long long *__restrict__ nlong) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= m) return;
+  const uint32_t cell = key[p];
+  if (p > 0 && key[p - 1] == cell) return;
+  const int64_t lim = min(m, p + RUN_SHORT + 1);
+  int64_t e = p;
+  while (e < lim && key[e] == cell) ++e;
+  if (e - p > RUN_SHORT) {
+    const unsigned long long slot = atomicAdd(nlong, 1ull);
+    long_runs[slot] = p;
+    return;
+  }
+  float r = 0.f;
+  for (int64_t q = p; q < e; ++q) r = cic_add(r, ts[q], is0[q]);
+  rho[cell] = r;
+}
+
+// One wave per long run: the lanes load 64 consecutive contributions at a
+// time and the wave adds them in order (every lane carries the same sum),
+// so a cell holding a dense clump streams its terms instead of chasing one
+// element at a time.  Waves loop over the queue; each exits when the queue
+// is drained.
+__global__ void __launch_bounds__(64)
+k_cic_long(int64_t m, const uint32_t *__restrict__ key, const double *__restrict__ ts,
+           const uint8_t *__restrict__ is0, float *__restrict__ rho,
+           const int64_t *__restrict__ long_runs, const unsigned long long *__restrict__ nlong) {
+  const int l = threadIdx.x;
+  const int64_t nruns = (int64_t)*nlong;
+  for (int64_t run = blockIdx.x; run < nruns; run += gridDim.x) {
+    const int64_t p0 = long_runs[run];
+    const uint32_t cell = key[p0];
+    float r = 0.f;
+    bool done = false;
+    for (int64_t base = p0; !done; base += 64) {
+      const int64_t q = base + l;
+      const bool in = q < m && key[q] == cell;
+      const double t = in ? ts[q] : 0.0;
+      const int f0 = in ? (int)is0[q] : 0;
+      const unsigned long long live = __ballot(in);
+      // contributions of one cell are contiguous: the live lanes are a prefix
+      const int cnt = __popcll(live);
+      for (int i = 0; i < cnt; ++i) r = cic_add(r, __shfl(t, i), __shfl(f0, i) != 0);
+      done = cnt < 64;
+    }
+    if (l == 0) rho[cell] = r;
+  }
+}
+
+// Particles.cxx:682-711, one component: f accumulated as the deposit's runs
+// are (grid[safe] read as 0), then v += f * fscal * tau in float.
+__global__ void __launch_bounds__(256)
+k_inverse_cic(int64_t n, const float *__restrict__ x, const float *__restrict__ y,
+              const float *__restrict__ z, Grid g, const float *__restrict__ grid,
+              float tau, float fscal, float *__restrict__ v) {
+  const int64_t nn = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (nn >= n) return;
+  const Corners cc = corners(x[nn], y[nn], z[nn], g);
+  float f = 0.f;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const float s = cc.cell[q] == g.safe ? 0.f : grid[cc.cell[q]];
+    const double t = corner_term(q, s, cc.ab, cc.de, cc.gh);
+    if (q == 0) f = f + (float)t;
+    else f = (float)((double)f + t);
+  }
+  v[nn] = v[nn] + f * fscal * tau;
+}
+
+__global__ void k_zero_u64(unsigned long long *p) {
+  if (threadIdx.x == 0) p[0] = 0ull;
+}
+
+__global__ void k_fill0(int64_t n, float *a) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < n) a[k] = 0.f;
+}
+
+Grid make_grid(const int32_t ng[3]) {
+  Grid g;
+  g.ng0 = ng[0];
+  g.ng1 = ng[1];
+  g.ng2 = ng[2];
+  g.safe = (uint32_t)((uint64_t)ng[0] * ng[1] * ng[2]);
+  return g;
+}
+
+}  // namespace
+
+hsr_status cic_run(hsr_ctx *c, int64_t n, const float *x, const float *y, const float *z,
+                   const int32_t ng[3], float gpscal, float *rho) {
+  hipStream_t st = c->stream;
+  const Grid g = make_grid(ng);
+  const int64_t ncell = (int64_t)g.safe + 1;
+  k_fill0<<<nblk(ncell, 256), 256, 0, st>>>(ncell, rho);
+  HSR_HIP(hipGetLastError());
+  if (n <= 0) return HSR_OK;
+  const float cc = gpscal * gpscal * gpscal;  // Particles.cxx:606
+  const int64_t m = 8 * n;
+  int bits = 1;
+  while (bits < 32 && ((uint64_t)g.safe >> bits) != 0) ++bits;
+  // per contribution: key, key', val, val' (4 x 4 B), its term and the
+  // sorted terms (2 x 8 B), the corner-0 flag (1 B) and the long-run queue
+  // (8 B per run at most one per contribution)
+  const size_t mm = (size_t)m;
+  HSR_HIP(dbuf_reserve(c->sub.keys, 41 * mm + 512));
+  uint32_t *key = dptr<uint32_t>(c->sub.keys);
+  uint32_t *key2 = key + mm;
+  int32_t *val = (int32_t *)(key2 + mm);
+  int32_t *val2 = val + mm;
+  double *term = (double *)(val2 + mm);  // 16 m bytes in: 8-byte aligned
+  double *ts = term + mm;
+  int64_t *long_runs = (int64_t *)(ts + mm);
+  unsigned long long *nlong = (unsigned long long *)(long_runs + mm);
+  uint8_t *is0 = (uint8_t *)(nlong + 1);
+  k_zero_u64<<<1, 64, 0, st>>>(nlong);
+  k_cic_contrib<<<nblk(n, 256), 256, 0, st>>>(n, x, y, z, g, cc, key, val, term);
+  HSR_HIP(hipGetLastError());
+  size_t tmp = 0;
+  HSR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, key, key2, val, val2, (int)m, 0, bits, st));
+  HSR_HIP(dbuf_reserve(c->s.cub, tmp));
+  HSR_HIP(hipcub::DeviceRadixSort::SortPairs(c->s.cub.p, tmp, key, key2, val, val2, (int)m, 0, bits, st));
+  k_cic_gather<<<nblk(m, 256), 256, 0, st>>>(m, val2, term, ts, is0);
+  k_cic_runs<<<nblk(m, 256), 256, 0, st>>>(m, key2, ts, is0, rho, long_runs, nlong);
+  const int waves = std::max(1, c->num_cus) * 8;
+  k_cic_long<<<waves, 64, 0, st>>>(m, key2, ts, is0, rho, long_runs, nlong);
+  HSR_HIP(hipGetLastError());
+  return HSR_OK;
+}
+
+hsr_status inverse_cic_run(hsr_ctx *c, int64_t n, const float *x, const float *y,
+                           const float *z, const int32_t ng[3], const float *grid, float tau,
+                           float fscal, float *v) {
+  if (n <= 0) return HSR_OK;
+  const Grid g = make_grid(ng);
+  k_inverse_cic<<<nblk(n, 256), 256, 0, c->stream>>>(n, x, y, z, g, grid, tau, fscal, v);
+  HSR_HIP(hipGetLastError());
+  return HSR_OK;
+}

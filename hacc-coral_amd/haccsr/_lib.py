@@ -28,7 +28,8 @@ EXPORTS = [
     "hsr_adjoint", "hsr_get_stats", "hsr_get_nodes", "hsr_get_perm",
     "hsr_get_lists", "hsr_build_many", "hsr_lists_many", "hsr_set_tuning",
     "hsr_get_tuning", "hsr_forces_f64", "hsr_leaf_momentum", "hsr_drift",
-    "hsr_resort", "hsr_map2", "hsr_subcycle", "hsr_get_pseudo_masses",
+    "hsr_resort", "hsr_map2", "hsr_subcycle", "hsr_get_pseudo_masses", "hsr_cic",
+    "hsr_inverse_cic",
 ]
 
 
@@ -69,7 +70,8 @@ class Stats(C.Structure):
                 ("kernel_ms", C.c_double), ("adjoint_ms", C.c_double),
                 ("evaluated_pairs", C.c_int64), ("drift_ms", C.c_double),
                 ("resort_ms", C.c_double), ("subcycle_ms", C.c_double),
-                ("drift_factor", C.c_double), ("kick_coeff", C.c_double)]
+                ("drift_factor", C.c_double), ("kick_coeff", C.c_double),
+                ("cic_ms", C.c_double), ("inverse_cic_ms", C.c_double)]
 
 
 class SubcycleParams(C.Structure):
@@ -129,6 +131,9 @@ def load(path: str | None = None):
     L.hsr_map2.argtypes = [vp, C.POINTER(SubcycleParams), C.c_double, i64] + [vp] * 10 + \
         [vp, C.c_int]
     L.hsr_subcycle.argtypes = [vp, C.POINTER(SubcycleParams), i64] + [vp] * 10 + [vp, C.c_int]
+    L.hsr_cic.argtypes = [vp, i64, vp, vp, vp, vp, C.c_float, vp, C.c_int]
+    L.hsr_inverse_cic.argtypes = [vp, i64, vp, vp, vp, vp, vp, C.c_float, C.c_float, vp,
+                                  C.c_int]
     L.hsr_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.hsr_get_nodes.argtypes = [vp, vp, i64]
     L.hsr_get_pseudo_masses.argtypes = [vp, vp, i64]
@@ -334,6 +339,27 @@ class Context:
                                   _ptr(ids), _ptr(mask), C.byref(inside), fl), "hsr_subcycle")
         self.n = int(inside.value) if inside.value >= 0 else self.n
         return int(inside.value)
+
+    # -- PM particle-grid steps
+    def cic(self, x, y, z, ng, gpscal=1.0, rho=None, device_ptrs=False):
+        """Particles::cic: rho of ng0*ng1*ng2 + 1 floats (the last slot
+        collects corners outside the grid)."""
+        g = np.ascontiguousarray(ng, np.int32)
+        if rho is None:
+            rho = np.zeros(int(np.prod(g.astype(np.int64))) + 1, np.float32)
+        fl = HSR_DEVICE_PTRS if device_ptrs else HSR_HOST_PTRS
+        check(self.L.hsr_cic(self.h, int(x.shape[0]), _ptr(x), _ptr(y), _ptr(z), g.ctypes.data,
+                             C.c_float(gpscal), _ptr(rho), fl), "hsr_cic")
+        return rho
+
+    def inverse_cic(self, x, y, z, ng, grid, tau, fscal, v, device_ptrs=False):
+        """Particles::inverse_cic for one component, v updated in place."""
+        g = np.ascontiguousarray(ng, np.int32)
+        fl = HSR_DEVICE_PTRS if device_ptrs else HSR_HOST_PTRS
+        check(self.L.hsr_inverse_cic(self.h, int(x.shape[0]), _ptr(x), _ptr(y), _ptr(z),
+                                     g.ctypes.data, _ptr(grid), C.c_float(tau),
+                                     C.c_float(fscal), _ptr(v), fl), "hsr_inverse_cic")
+        return v
 
     # -- introspection
     def stats(self) -> Stats:

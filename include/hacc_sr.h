@@ -24,6 +24,8 @@
  *       = hsr_map2
  *   Particles::subCycle                       Particles.cxx:1176-1201
  *       = hsr_subcycle
+ *   Particles::cic / inverse_cic              Particles.cxx:589-714
+ *       = hsr_cic / hsr_inverse_cic
  */
 #ifndef HACC_SR_H
 #define HACC_SR_H
@@ -156,6 +158,8 @@ typedef struct {
   double subcycle_ms;     /* host wall time of the last hsr_subcycle */
   double drift_factor;    /* prefactor * tau of the last drift (a float) */
   double kick_coeff;      /* c of the last map2 kick (a float) */
+  double cic_ms;          /* device time of the last hsr_cic */
+  double inverse_cic_ms;  /* device time of the last hsr_inverse_cic */
 } hsr_stats;
 
 hsr_status hsr_get_stats(hsr_ctx *ctx, hsr_stats *st);
@@ -264,6 +268,25 @@ hsr_status hsr_subcycle(hsr_ctx *ctx, const hsr_subcycle_params *sp,
                         float *vx, float *vy, float *vz, float *mass,
                         float *phi, int64_t *id, uint16_t *mask,
                         int64_t *inside, int ptr_flags);
+
+/* ------------------------------------------------------------ PM steps */
+/* Particles::cic (Particles.cxx:589-641): cloud-in-cell deposit of
+ * c = gpscal^3 per particle into rho, which holds ng[0]*ng[1]*ng[2] + 1
+ * floats (row-major cells, then the slot that collects corners outside the
+ * grid); rho is zeroed first.  Bit-exact with the reference's sequential
+ * order of float additions into each cell. */
+hsr_status hsr_cic(hsr_ctx *ctx, int64_t count, const float *x, const float *y,
+                   const float *z, const int32_t ng[3], float gpscal, float *rho,
+                   int ptr_flags);
+
+/* Particles::inverse_cic(tau, fscal, comp) (Particles.cxx:645-714) for one
+ * component: v += fscal * tau * (CIC interpolation of grid at each
+ * particle), grid holding ng[0]*ng[1]*ng[2] + 1 floats whose last slot is
+ * read as 0 (the reference zeroes it; here it is left untouched). */
+hsr_status hsr_inverse_cic(hsr_ctx *ctx, int64_t count, const float *x,
+                           const float *y, const float *z, const int32_t ng[3],
+                           const float *grid, float tau, float fscal, float *v,
+                           int ptr_flags);
 
 /* ------------------------------------------------------------- tuning */
 typedef struct {

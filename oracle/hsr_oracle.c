@@ -850,6 +850,101 @@ float hsro_map2_coeff(float gpscal, double fscal, double tau,
                  step_fraction);
 }
 
+/* ------------------------------------------------------------------ CIC */
+/* Particles.cxx:371-395 with lo = 0, hi = ng: the cell's row-major index,
+ * or `safe` outside the grid. */
+static int64_t cic_index(int ix, int iy, int iz, const int32_t *ng, int64_t safe) {
+  if (ix >= 0 && ix < ng[0] && iy >= 0 && iy < ng[1] && iz >= 0 && iz < ng[2])
+    return ((int64_t)ix * ng[1] + iy) * ng[2] + iz;
+  return safe;
+}
+
+/* The eight corner cells and weights of Particles.cxx:617-640 / 685-711.
+ * ab = 1.0 + (ix - xx): the float difference, the sum in double, stored as
+ * float.  Corner q's weight is returned as the reference's expression
+ * evaluates it with lead factor s (c for the deposit, the grid value for the
+ * interpolation): corner 0 in float, the others in double through the
+ * (1.0 - w) factors; *is_float tells which. */
+typedef struct {
+  int64_t cell[8];
+} cic_cells;
+
+static void cic_setup(float xx, float yy, float zz, const int32_t *ng, int64_t safe,
+                      cic_cells *cc, float *ab, float *de, float *gh) {
+  int ix = (int)floorf(xx), iy = (int)floorf(yy), iz = (int)floorf(zz);
+  int ip = ix + 1, jp = iy + 1, kp = iz + 1;
+  *ab = (float)(1.0 + (double)((float)ix - xx));
+  *de = (float)(1.0 + (double)((float)iy - yy));
+  *gh = (float)(1.0 + (double)((float)iz - zz));
+  cc->cell[0] = cic_index(ix, iy, iz, ng, safe);
+  cc->cell[1] = cic_index(ix, jp, iz, ng, safe);
+  cc->cell[2] = cic_index(ix, jp, kp, ng, safe);
+  cc->cell[3] = cic_index(ix, iy, kp, ng, safe);
+  cc->cell[4] = cic_index(ip, iy, kp, ng, safe);
+  cc->cell[5] = cic_index(ip, jp, kp, ng, safe);
+  cc->cell[6] = cic_index(ip, jp, iz, ng, safe);
+  cc->cell[7] = cic_index(ip, iy, iz, ng, safe);
+}
+
+/* s*ab*de*gh and its seven siblings, each as C evaluates the reference's
+ * expression left to right (float until the first (1.0 - w) factor, double
+ * from there). */
+static double cic_term(int q, float s, float ab, float de, float gh) {
+  switch (q) {
+  case 0: return (double)(s * ab * de * gh);
+  case 1: return (double)(s * ab) * (1.0 - de) * gh;
+  case 2: return (double)(s * ab) * (1.0 - de) * (1.0 - gh);
+  case 3: return (double)(s * ab * de) * (1.0 - gh);
+  case 4: return (double)s * (1.0 - ab) * de * (1.0 - gh);
+  case 5: return (double)s * (1.0 - ab) * (1.0 - de) * (1.0 - gh);
+  case 6: return (double)s * (1.0 - ab) * (1.0 - de) * gh;
+  default: return (double)s * (1.0 - ab) * de * gh;
+  }
+}
+
+/* Particles::cic (Particles.cxx:589-641): rho[Ng + 1] zeroed, then every
+ * particle adds c = gpscal^3 times its eight weights in order; corner 0 is a
+ * float addition, the others a double addition rounded to float. */
+void hsro_cic(int64_t np, const float *x, const float *y, const float *z,
+              const int32_t ng[3], float gpscal, float *rho) {
+  const int64_t Ng = (int64_t)ng[0] * ng[1] * ng[2];
+  const float c = gpscal * gpscal * gpscal;
+  memset(rho, 0, sizeof(float) * (size_t)(Ng + 1));
+  for (int64_t nn = 0; nn < np; ++nn) {
+    cic_cells cc;
+    float ab, de, gh;
+    cic_setup(x[nn], y[nn], z[nn], ng, Ng, &cc, &ab, &de, &gh);
+    for (int q = 0; q < 8; ++q) {
+      double t = cic_term(q, c, ab, de, gh);
+      float *r = &rho[cc.cell[q]];
+      if (q == 0) *r = *r + (float)t;
+      else *r = (float)((double)*r + t);
+    }
+  }
+}
+
+/* Particles::inverse_cic (Particles.cxx:645-714) for one component:
+ * f = sum of the eight grid values times their weights (same arithmetic as
+ * the deposit, grid[Ng] read as 0), then v[nn] += f * fscal * tau in float. */
+void hsro_inverse_cic(int64_t np, const float *x, const float *y, const float *z,
+                      const int32_t ng[3], const float *grid, float tau, float fscal,
+                      float *v) {
+  const int64_t Ng = (int64_t)ng[0] * ng[1] * ng[2];
+  for (int64_t nn = 0; nn < np; ++nn) {
+    cic_cells cc;
+    float ab, de, gh;
+    cic_setup(x[nn], y[nn], z[nn], ng, Ng, &cc, &ab, &de, &gh);
+    float f = 0;
+    for (int q = 0; q < 8; ++q) {
+      float g = cc.cell[q] == Ng ? 0.0f : grid[cc.cell[q]];
+      double t = cic_term(q, g, ab, de, gh);
+      if (q == 0) f = f + (float)t;
+      else f = (float)((double)f + t);
+    }
+    v[nn] += f * fscal * tau;
+  }
+}
+
 /* -------------------------------------------------------------- adjoint */
 /* g(r2) and g'(r2) of the Step16 law.  With rsm > 0 the pair term
  * m g(r2 + rsm2) d is smooth through d = 0, so a coincident pair (r2 == 0,

@@ -17,6 +17,7 @@
  *   hsro_cell_sort_perm  Particles.cxx:402-488 (resortParticles)
  *   hsro_map1_factor, hsro_drift   Particles.cxx:732-758 (map1)
  *   hsro_map2_coeff      Particles.cxx:1230-1233 (map2)
+ *   hsro_cic, hsro_inverse_cic     Particles.cxx:589-714
  *
  * Parity pins (see tests/test_oracle_*.py):
  *   cm            bit-exact against the reference's own BGQCM.c compiled
@@ -31,6 +32,9 @@
  *                 traceless quadrupole about xc; the tree force is closer
  *                 to the direct sum than the monopole tree's.
  *   cell sort     numpy's stable argsort of the cell keys.
+ *   CIC           weights sum to one per particle (mass conservation),
+ *                 linear fields interpolate exactly, a particle on a grid
+ *                 point deposits into that cell alone.
  *   map1, map2    closed forms (prefactor 1 exactly, double-precision
  *                 formula to float rounding, two half drifts vs one).
  */
@@ -205,6 +209,17 @@ void hsro_pppts(int tdpts, float tdr, const float *xc, float *px, float *py,
 float hsro_map1_factor(float pp, float tau, float adot, float alpha);
 void hsro_drift(int64_t n, float *x, float *y, float *z, const float *vx,
                 const float *vy, const float *vz, float factor);
+
+/* Particles::cic (Particles.cxx:589-641): cloud-in-cell deposit of c =
+ * gpscal^3 per particle into rho[ng0*ng1*ng2 + 1] (the last slot collects
+ * the corners outside the grid). */
+void hsro_cic(int64_t np, const float *x, const float *y, const float *z,
+              const int32_t ng[3], float gpscal, float *rho);
+/* Particles::inverse_cic (Particles.cxx:645-714), one component:
+ * v += fscal * tau * CIC interpolation of grid (grid[Ng] read as 0). */
+void hsro_inverse_cic(int64_t np, const float *x, const float *y, const float *z,
+                      const int32_t ng[3], const float *grid, float tau, float fscal,
+                      float *v);
 
 /* Particles::map2's kick coefficient (Particles.cxx:1230-1233). */
 float hsro_map2_coeff(float gpscal, double fscal, double tau,

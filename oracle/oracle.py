@@ -94,6 +94,10 @@ def lib():
         L.hsro_cell_sort_perm.argtypes = [C.c_int64, f32, f32, f32, P(np.int32, flags="C"), i64]
         L.hsro_cell_sort_perm.restype = C.c_int64
         L.hsro_pppts.argtypes = [C.c_int, C.c_float, f32, f32, f32, f32]
+        i32 = P(np.int32, flags="C")
+        L.hsro_cic.argtypes = [C.c_int64, f32, f32, f32, i32, C.c_float, f32]
+        L.hsro_inverse_cic.argtypes = [C.c_int64, f32, f32, f32, i32, f32, C.c_float,
+                                       C.c_float, f32]
         L.hsro_map1_factor.argtypes = [C.c_float] * 4
         L.hsro_map1_factor.restype = C.c_float
         L.hsro_drift.argtypes = [C.c_int64] + [f32] * 6 + [C.c_float]
@@ -352,6 +356,25 @@ def pppts(tdpts, tdr, xc):
     out = [np.zeros(tdpts, np.float32) for _ in range(3)]
     lib().hsro_pppts(tdpts, C.c_float(tdr), np.ascontiguousarray(xc, np.float32), *out)
     return np.stack(out, 1)
+
+
+def cic(x, y, z, ng, gpscal=1.0):
+    """Particles::cic: rho of ng0*ng1*ng2 + 1 cells (last = outside)."""
+    g = np.asarray(ng, np.int32)
+    rho = np.zeros(int(np.prod(g.astype(np.int64))) + 1, np.float32)
+    f = lambda a: np.ascontiguousarray(a, np.float32)
+    lib().hsro_cic(x.shape[0], f(x), f(y), f(z), g, C.c_float(gpscal), rho)
+    return rho
+
+
+def inverse_cic(x, y, z, ng, grid, tau, fscal, v):
+    """Particles::inverse_cic for one component; returns v + the kick."""
+    g = np.asarray(ng, np.int32)
+    out = np.ascontiguousarray(v, np.float32).copy()
+    f = lambda a: np.ascontiguousarray(a, np.float32)
+    lib().hsro_inverse_cic(x.shape[0], f(x), f(y), f(z), g, f(grid), C.c_float(tau),
+                           C.c_float(fscal), out)
+    return out
 
 
 def map1_factor(pp, tau, adot, alpha) -> float:

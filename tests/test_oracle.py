@@ -399,6 +399,56 @@ def test_quadrupole_tree_force_beats_monopole():
     assert errs[12] < 0.5 * errs[1], errs
 
 
+# -------------------------------------------------------------------- CIC
+def test_cic_weights_partition_unity_and_grid_points():
+    """Each particle's eight CIC weights are in [0, 1] and sum to one; a
+    particle on a grid point deposits into its own cell alone; the total
+    deposit is c = gpscal^3 per particle, the outside slot taking the corners
+    beyond the grid."""
+    rng = np.random.default_rng(61)
+    ng = [7, 6, 5]
+    Ng = 7 * 6 * 5
+    for _ in range(50):
+        p = (rng.random(3) * (np.array(ng) - 1)).astype(np.float32)
+        rho = O.cic(p[:1], p[1:2], p[2:], ng)
+        w = rho[:Ng]
+        assert np.all(w >= 0) and np.all(w <= 1) and rho[Ng] == 0
+        assert abs(w.sum() - 1.0) < 1e-6
+        assert np.count_nonzero(w) <= 8
+    q = np.array([3.0], np.float32), np.array([2.0], np.float32), np.array([4.0], np.float32)
+    rho = O.cic(*q, ng, gpscal=2.0)
+    assert rho[(3 * 6 + 2) * 5 + 4] == 8.0 and np.count_nonzero(rho) == 1
+    n = 4000
+    x, y, z = (rng.uniform(-1.5, g + 0.5, n).astype(np.float32) for g in ng)
+    rho = O.cic(x, y, z, ng, gpscal=1.5)
+    assert abs(rho.astype(np.float64).sum() - 1.5 ** 3 * n) < 1e-4 * n
+
+
+def test_inverse_cic_is_the_deposit_transpose_and_exact_on_linear_fields():
+    """Interpolating a grid G at a particle equals the dot product of G with
+    that particle's deposit weights (a swapped corner in either direction
+    fails), and a linear field interpolates to its value at the particle."""
+    rng = np.random.default_rng(62)
+    ng = [8, 9, 10]
+    Ng = 8 * 9 * 10
+    G = rng.normal(size=Ng + 1).astype(np.float32)
+    G[Ng] = 123.0                  # the outside slot reads as 0
+    for _ in range(40):
+        p = [np.array([rng.uniform(-0.9, g - 0.1)], np.float32) for g in ng]
+        w = O.cic(*p, ng)
+        want = float(np.dot(w[:Ng].astype(np.float64), G[:Ng]))
+        got = O.inverse_cic(*p, ng, G, 1.0, 1.0, np.zeros(1, np.float32))[0]
+        assert abs(got - want) <= 1e-5 * (1 + abs(want))
+    i, j, k = np.meshgrid(np.arange(8), np.arange(9), np.arange(10), indexing="ij")
+    lin = np.append((0.5 * i - 1.25 * j + 2.0 * k + 3.0).astype(np.float32).ravel(), 0)
+    n = 500
+    x, y, z = (rng.uniform(0, g - 1, n).astype(np.float32) for g in ng)
+    v0 = rng.normal(size=n).astype(np.float32)
+    v = O.inverse_cic(x, y, z, ng, lin, 0.25, 2.0, v0)
+    exact = 0.5 * x.astype(np.float64) - 1.25 * y + 2.0 * z + 3.0
+    assert np.allclose((v - v0) / 0.5, exact, rtol=0, atol=2e-5 * 30)
+
+
 def test_cell_sort_is_stable_counting_sort():
     rng = np.random.default_rng(10)
     x = (rng.random(5000) * 9.0 - 0.5).astype(np.float32)
