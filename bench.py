@@ -59,6 +59,8 @@ def parse():
                          "(quadrupole; no adjoint leg)")
     ap.add_argument("--subcycle-nsub", type=int, default=5,
                     help="sub-steps of the device-resident sub-cycle leg (0: skip it)")
+    ap.add_argument("--pm-reps", type=int, default=3,
+                    help="timed CIC deposit/interpolation pairs of the PM leg (0: skip it)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run one plain device-to-device copy of BYTES (the "
                          "reference kernel for scaling FETCH_SIZE/WRITE_SIZE counters)")
@@ -218,6 +220,44 @@ def subcycle_leg(load, nd, nsub, dev):
                     "resort times are HIP-event means per pass"}
 
 
+def pm_leg(load, dev, reps=3):
+    """Particles::cic and Particles::inverse_cic on the same load (hsr_cic,
+    hsr_inverse_cic; device arrays; grid ng = ceil(box + 2 overload) per
+    side).  Algorithmic bytes: the deposit reads 12 B per particle and writes
+    4 B per cell; the interpolation reads 12 B of position and 4 B of
+    velocity and writes 4 B per particle, and reads the grid (4 B per cell)."""
+    import math
+    import torch
+    from haccsr import Context
+    n = load.n
+    ng = int(math.ceil(load.box + 2 * load.overload))
+    ncell = ng ** 3 + 1
+    d = [torch.from_numpy(a).to(dev) for a in (load.x, load.y, load.z)]
+    rho = torch.empty(ncell, dtype=torch.float32, device=dev)
+    v = torch.zeros(n, dtype=torch.float32, device=dev)
+    ctx = Context()
+    ctx.cic(*d, [ng] * 3, 1.0, rho, device_ptrs=True)      # allocations
+    ctx.inverse_cic(*d, [ng] * 3, rho, 0.01, 1.0, v, device_ptrs=True)
+    cic_ms, icic_ms = [], []
+    for _ in range(reps):
+        ctx.cic(*d, [ng] * 3, 1.0, rho, device_ptrs=True)
+        cic_ms.append(ctx.stats().cic_ms)
+        ctx.inverse_cic(*d, [ng] * 3, rho, 0.01, 1.0, v, device_ptrs=True)
+        icic_ms.append(ctx.stats().inverse_cic_ms)
+    ctx.close()
+    c_ms, i_ms = float(np.mean(cic_ms)), float(np.mean(icic_ms))
+    cb = 12 * n + 4 * ncell
+    ib = 20 * n + 4 * ncell
+    return {"ng": ng, "particles": n,
+            "cic": {"ms": c_ms, "algorithmic_bytes": cb, "GB/s": cb / (c_ms * 1e-3) / 1e9,
+                    "frac": cb / (c_ms * 1e-3) / 1e9 / PEAK_HBM_GBPS},
+            "inverse_cic": {"ms": i_ms, "algorithmic_bytes": ib,
+                            "GB/s": ib / (i_ms * 1e-3) / 1e9,
+                            "frac": ib / (i_ms * 1e-3) / 1e9 / PEAK_HBM_GBPS},
+            "note": "deposit bit-exact with the reference's sequential float order: "
+                    "8 contributions per particle sorted by cell, runs added in order"}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -367,6 +407,9 @@ def main():
             ctxs.pop(k).close()
         out["subcycle"] = subcycle_leg(load, nd, a.subcycle_nsub, dev)
         log(f"subcycle {out['subcycle']}")
+    if a.pm_reps > 0 and ens is None:
+        out["pm"] = pm_leg(load, dev, a.pm_reps)
+        log(f"pm {out['pm']}")
     if not a.no_cpu and ens is None:
         log("cpu baseline leg ...")
         out["cpu_baseline"] = cpu_baseline(load, nd, a.cpu_seconds)

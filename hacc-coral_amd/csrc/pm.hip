@@ -105,21 +105,29 @@ k_cic_contrib(int64_t n, const float *__restrict__ x, const float *__restrict__ 
   }
 }
 
-// Terms in sorted order, with the corner-0 flag folded into a byte.
+// Terms in sorted order.
 __global__ void __launch_bounds__(256)
 k_cic_gather(int64_t m, const int32_t *__restrict__ val, const double *__restrict__ term,
-             double *__restrict__ ts, uint8_t *__restrict__ is0) {
+             double *__restrict__ ts) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= m) return;
-  const int32_t k = val[p];
-  ts[p] = term[k];
-  is0[p] = (k & 7) == 0;
+  ts[p] = term[val[p]];
 }
 
 // The reference's accumulation step for one contribution
-// (Particles.cxx:632-639: rhoArr[...] += float or double).
-__device__ __forceinline__ float cic_add(float r, double t, bool first) {
-  return first ? r + (float)t : (float)((double)r + t);
+// (Particles.cxx:632-639): corner 0 adds a float term in float, the others
+// add a double term and round to float.  The float addition r + t equals
+// (float)((double)r + t) for float r and t (double rounding through a
+// format of at least 2 * 24 + 2 bits is innocuous for a sum), so every
+// corner takes the one form below.
+__device__ __forceinline__ float cic_add(float r, double t) {
+  return (float)((double)r + t);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int i) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), i);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), i);
+  return __hiloint2double(hi, lo);
 }
 
 constexpr int RUN_SHORT = 64;
@@ -129,8 +137,8 @@ constexpr int RUN_SHORT = 64;
 // each.
 __global__ void __launch_bounds__(256)
 k_cic_runs(int64_t m, const uint32_t *__restrict__ key, const double *__restrict__ ts,
-           const uint8_t *__restrict__ is0, float *__restrict__ rho,
-           int64_t *__restrict__ long_runs, unsigned long long *__restrict__ nlong) {
+           float *__restrict__ rho, int64_t *__restrict__ long_runs,
+           unsigned long long *__restrict__ nlong) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= m) return;
   const uint32_t cell = key[p];
@@ -144,19 +152,20 @@ k_cic_runs(int64_t m, const uint32_t *__restrict__ key, const double *__restrict
     return;
   }
   float r = 0.f;
-  for (int64_t q = p; q < e; ++q) r = cic_add(r, ts[q], is0[q]);
+  for (int64_t q = p; q < e; ++q) r = cic_add(r, ts[q]);
   rho[cell] = r;
 }
 
 // One wave per long run: the lanes load 64 consecutive contributions at a
-// time and the wave adds them in order (every lane carries the same sum),
-// so a cell holding a dense clump streams its terms instead of chasing one
-// element at a time.  Waves loop over the queue; each exits when the queue
-// is drained.
+// time and the wave adds them in order, reading lane i's term with
+// v_readlane (every lane carries the same sum), so a cell holding a dense
+// clump streams its terms and the serial chain is three dependent ALU
+// operations per contribution.  Waves loop over the queue; each exits when
+// the queue is drained.
 __global__ void __launch_bounds__(64)
 k_cic_long(int64_t m, const uint32_t *__restrict__ key, const double *__restrict__ ts,
-           const uint8_t *__restrict__ is0, float *__restrict__ rho,
-           const int64_t *__restrict__ long_runs, const unsigned long long *__restrict__ nlong) {
+           float *__restrict__ rho, const int64_t *__restrict__ long_runs,
+           const unsigned long long *__restrict__ nlong) {
   const int l = threadIdx.x;
   const int64_t nruns = (int64_t)*nlong;
   for (int64_t run = blockIdx.x; run < nruns; run += gridDim.x) {
@@ -168,11 +177,14 @@ k_cic_long(int64_t m, const uint32_t *__restrict__ key, const double *__restrict
       const int64_t q = base + l;
       const bool in = q < m && key[q] == cell;
       const double t = in ? ts[q] : 0.0;
-      const int f0 = in ? (int)is0[q] : 0;
-      const unsigned long long live = __ballot(in);
       // contributions of one cell are contiguous: the live lanes are a prefix
-      const int cnt = __popcll(live);
-      for (int i = 0; i < cnt; ++i) r = cic_add(r, __shfl(t, i), __shfl(f0, i) != 0);
+      const int cnt = __popcll(__ballot(in));
+      if (cnt == 64) {
+#pragma unroll 16
+        for (int i = 0; i < 64; ++i) r = cic_add(r, readlane_f64(t, i));
+      } else {
+        for (int i = 0; i < cnt; ++i) r = cic_add(r, readlane_f64(t, i));
+      }
       done = cnt < 64;
     }
     if (l == 0) rho[cell] = r;
@@ -232,10 +244,10 @@ hsr_status cic_run(hsr_ctx *c, int64_t n, const float *x, const float *y, const 
   int bits = 1;
   while (bits < 32 && ((uint64_t)g.safe >> bits) != 0) ++bits;
   // per contribution: key, key', val, val' (4 x 4 B), its term and the
-  // sorted terms (2 x 8 B), the corner-0 flag (1 B) and the long-run queue
-  // (8 B per run at most one per contribution)
+  // sorted terms (2 x 8 B) and the long-run queue (8 B; at most one run per
+  // contribution)
   const size_t mm = (size_t)m;
-  HSR_HIP(dbuf_reserve(c->sub.keys, 41 * mm + 512));
+  HSR_HIP(dbuf_reserve(c->sub.keys, 40 * mm + 512));
   uint32_t *key = dptr<uint32_t>(c->sub.keys);
   uint32_t *key2 = key + mm;
   int32_t *val = (int32_t *)(key2 + mm);
@@ -244,7 +256,6 @@ hsr_status cic_run(hsr_ctx *c, int64_t n, const float *x, const float *y, const 
   double *ts = term + mm;
   int64_t *long_runs = (int64_t *)(ts + mm);
   unsigned long long *nlong = (unsigned long long *)(long_runs + mm);
-  uint8_t *is0 = (uint8_t *)(nlong + 1);
   k_zero_u64<<<1, 64, 0, st>>>(nlong);
   k_cic_contrib<<<nblk(n, 256), 256, 0, st>>>(n, x, y, z, g, cc, key, val, term);
   HSR_HIP(hipGetLastError());
@@ -252,10 +263,10 @@ hsr_status cic_run(hsr_ctx *c, int64_t n, const float *x, const float *y, const 
   HSR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, key, key2, val, val2, (int)m, 0, bits, st));
   HSR_HIP(dbuf_reserve(c->s.cub, tmp));
   HSR_HIP(hipcub::DeviceRadixSort::SortPairs(c->s.cub.p, tmp, key, key2, val, val2, (int)m, 0, bits, st));
-  k_cic_gather<<<nblk(m, 256), 256, 0, st>>>(m, val2, term, ts, is0);
-  k_cic_runs<<<nblk(m, 256), 256, 0, st>>>(m, key2, ts, is0, rho, long_runs, nlong);
+  k_cic_gather<<<nblk(m, 256), 256, 0, st>>>(m, val2, term, ts);
+  k_cic_runs<<<nblk(m, 256), 256, 0, st>>>(m, key2, ts, rho, long_runs, nlong);
   const int waves = std::max(1, c->num_cus) * 8;
-  k_cic_long<<<waves, 64, 0, st>>>(m, key2, ts, is0, rho, long_runs, nlong);
+  k_cic_long<<<waves, 64, 0, st>>>(m, key2, ts, rho, long_runs, nlong);
   HSR_HIP(hipGetLastError());
   return HSR_OK;
 }

@@ -120,3 +120,27 @@ void ParticleSubCycle::subCycleSteps(double pp, double tau, double tau2, double 
     if (!m_skipStreamQ) map1((float)pp, (float)(stepFraction * tau2), (float)adot);
   }
 }
+
+void ParticleSubCycle::cic(float *rho) {
+  const int32_t ng[3] = {m_ng[0], m_ng[1], m_ng[2]};
+  check(hsr_cic(m_ctx, m_Np, m_x, m_y, m_z, ng, m_gpscal, rho, m_flags), "hsr_cic");
+}
+
+void ParticleSubCycle::inverse_cic(float tau, float fscal, int comp, const float *grad) {
+  POSVEL_T *v[4] = {m_vx, m_vy, m_vz, m_phi};
+  if (comp < 0 || comp > 3 || !v[comp])
+    throw std::runtime_error("inverse_cic: component must be 0..3 (vx, vy, vz, phi)");
+  const int32_t ng[3] = {m_ng[0], m_ng[1], m_ng[2]};
+  check(hsr_inverse_cic(m_ctx, m_Np, m_x, m_y, m_z, ng, grad, tau, fscal, v[comp], m_flags),
+        "hsr_inverse_cic");
+}
+
+// Particles.cxx:718-726: phi = 0, then the interpolation with tau = fscal = 1.
+void ParticleSubCycle::inverse_cic_potential(const float *phiGrid) {
+  if (!m_phi) throw std::runtime_error("inverse_cic_potential: no phi array");
+  if (m_flags == HSR_DEVICE_PTRS)
+    throw std::runtime_error("inverse_cic_potential: host arrays only (zero a device phi, "
+                             "then call inverse_cic(1, 1, 3, grid))");
+  for (ID_T i = 0; i < m_Np; ++i) m_phi[i] = 0.0f;
+  inverse_cic(1.0f, 1.0f, 3, phiGrid);
+}

@@ -1,8 +1,9 @@
-// ParticleSubCycle.h -- drop-in for the sub-cycle methods of the reference's
-// Particles class (Particles.h:160-175): map1 (drift, Particles.cxx:732-758),
-// resortParticles (Particles.cxx:402-488), map2 with the RCB monopole tree
-// (Particles.cxx:1205-1280) and subCycle (Particles.cxx:1176-1201), over the
-// C ABI of libhaccsr.so.  The ten particle arrays stay with the caller (host
+// ParticleSubCycle.h -- drop-in for the methods of the reference's Particles
+// class around the short-range kick (Particles.h:160-175): map1 (drift,
+// Particles.cxx:732-758), resortParticles (Particles.cxx:402-488), map2 with
+// the RCB monopole tree (Particles.cxx:1205-1280), subCycle
+// (Particles.cxx:1176-1201), and the PM particle-grid steps cic and
+// inverse_cic (Particles.cxx:589-725), over the C ABI of libhaccsr.so.  The ten particle arrays stay with the caller (host
 // or device memory); subCycle is one hsr_subcycle call, so the particles stay
 // on the device for all nsub sub-steps.  The TimeStepper is any type with the
 // reference's accessors pp(), tau(), tau2(), adot() and fscal()
@@ -48,6 +49,14 @@ class ParticleSubCycle {
   void subCycle(double pp, double tau, double tau2, double adot, double fscal);
   // the same cycle through map1 / map2 / map1 per sub-step
   void subCycleSteps(double pp, double tau, double tau2, double adot, double fscal);
+
+  // Particles::cic / inverse_cic (Particles.cxx:589-714) with the grid
+  // passed in (the reference's m_field): ng[0]*ng[1]*ng[2] + 1 floats, the
+  // last the slot for corners outside the grid.  comp 0, 1, 2 kick vx, vy,
+  // vz; 3 adds to phi (inverse_cic_potential zeroes phi first).
+  void cic(float *rho);
+  void inverse_cic(float tau, float fscal, int comp, const float *grad);
+  void inverse_cic_potential(const float *phiGrid);
 
   void setSkip(bool skipStream, bool skipKickSR) {
     m_skipStreamQ = skipStream;
