@@ -334,3 +334,29 @@ def test_cxx_subcycle_dropin(tmp_path):
     P.close()
     for k in ARR:
         assert bits_equal(a[k], fin0[k]), k
+
+
+def test_subcycle_from_restart_file(tmp_path):
+    """Particles read from a restart file (serial layout) cycle exactly like
+    the arrays they were written from (readRestart sets unit masses)."""
+    from haccsr import restart as R
+    ng, nd = (16, 16, 16), 64
+    st = particles(20000, ng, 21)
+    st["mass"][:] = 1.0
+    say("case subcycle from a restart file")
+    p = R.RestartParticles(**{k: st[k] for k in ("x", "y", "z", "vx", "vy", "vz", "phi")},
+                           id=st["id"], mask=st["mask"], mass=st["mass"])
+    f = tmp_path / "restart.bin"
+    R.write_restart(f, p)
+    q = R.read_restart(f)
+    a = {k: v.copy() for k, v in st.items()}
+    outs = []
+    for src in (a, {k: getattr(q, k) for k in ARR}):
+        P = make(ng, nd)
+        P.set_particles(src["x"], src["y"], src["z"], src["vx"], src["vy"], src["vz"],
+                        src["mass"], src["phi"], src["id"], src["mask"])
+        P.subCycle(TS)
+        P.close()
+        outs.append(src)
+    for k in ARR:
+        assert bits_equal(outs[0][k], outs[1][k]), k
