@@ -16,8 +16,9 @@
 // in parallel it writes the 8 contributions of every particle (cell, term)
 // in (particle, corner) order, sorts them stably by cell, and adds each
 // cell's run in that order: one thread per short run, one wave per long run
-// (a dense clump), streaming the run 64 contributions at a time.  The
-// interpolation is a gather, one thread per particle.
+// (a dense clump), streaming the run 64 contributions at a time through
+// exact fixed-grid maps of the float accumulation.  The interpolation is a
+// gather, one thread per particle.
 #include "hsr_internal.h"
 
 #include <hipcub/hipcub.hpp>
@@ -134,11 +135,13 @@ constexpr int RUN_SHORT = 64;
 
 // One thread per run of equal cells in the sorted contributions: runs of at
 // most RUN_SHORT are added in order here; longer runs are queued for a wave
-// each.
+// each, except the last run of the sorted order, which is the outside slot
+// whenever any corner fell outside the grid -- typically the longest run by
+// far -- and goes to a workgroup of its own.
 __global__ void __launch_bounds__(256)
 k_cic_runs(int64_t m, const uint32_t *__restrict__ key, const double *__restrict__ ts,
            float *__restrict__ rho, int64_t *__restrict__ long_runs,
-           unsigned long long *__restrict__ nlong) {
+           unsigned long long *__restrict__ nlong, int64_t *__restrict__ tail_start) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= m) return;
   const uint32_t cell = key[p];
@@ -147,6 +150,11 @@ k_cic_runs(int64_t m, const uint32_t *__restrict__ key, const double *__restrict
   int64_t e = p;
   while (e < lim && key[e] == cell) ++e;
   if (e - p > RUN_SHORT) {
+    if (key[m - 1] == cell) {
+      // the last run (the outside slot, or the largest cell): a workgroup
+      *tail_start = p;
+      return;
+    }
     const unsigned long long slot = atomicAdd(nlong, 1ull);
     long_runs[slot] = p;
     return;
@@ -156,12 +164,56 @@ k_cic_runs(int64_t m, const uint32_t *__restrict__ key, const double *__restrict
   rho[cell] = r;
 }
 
+// Fixed-grid maps of the accumulation r <- (float)((double)r + t), t >= 0.
+// While r stays in one binade [2^E, 2^(E+1)) (E >= -125) it is R u with
+// u = 2^(E-23) and R an integer below 2^24, and one step is R <- R + D where
+// D depends on t and on the parity of R alone: the double sum rounds to the
+// grid u 2^-29 (its ulp in that binade, ties to even independent of R), the
+// float to the grid u with ties to the even significand.  Maps (d0, d1) =
+// (D for even R, D for odd R) compose associatively, so a wave scans the
+// maps of 64 terms and applies the longest prefix that stays in the binade;
+// the term that leaves it is added directly.  Totals past 2^26 saturate
+// (they leave the binade anyway).
+struct PMap {
+  int d0, d1;
+};
+constexpr int PMAP_SAT = 1 << 26;
+
+__device__ __forceinline__ int pmap_add(int a, int b) {
+  const long long s = (long long)a + b;
+  return s > PMAP_SAT ? PMAP_SAT : (int)s;
+}
+
+// a, then b
+__device__ __forceinline__ PMap pmap_compose(PMap a, PMap b) {
+  PMap m;
+  m.d0 = a.d0 >= PMAP_SAT ? PMAP_SAT : pmap_add(a.d0, (a.d0 & 1) ? b.d1 : b.d0);
+  m.d1 = a.d1 >= PMAP_SAT ? PMAP_SAT : pmap_add(a.d1, ((a.d1 + 1) & 1) ? b.d1 : b.d0);
+  return m;
+}
+
+__device__ __forceinline__ PMap pmap_term(double t, int E) {
+  PMap m;
+  const double q = ldexp(t, 23 - E);                 // t / u, exact
+  if (!(q < 16777216.0)) {
+    m.d0 = m.d1 = PMAP_SAT;
+    return m;
+  }
+  const double qq = ldexp(rint(ldexp(q, 29)), -29);  // the double sum's rounding
+  const double a = floor(qq), f = qq - a;
+  const int ai = (int)a;
+  const int up = f > 0.5 ? 1 : 0;
+  const bool tie = f == 0.5;
+  m.d0 = ai + up + ((tie && (ai & 1)) ? 1 : 0);
+  m.d1 = ai + up + ((tie && !(ai & 1)) ? 1 : 0);
+  return m;
+}
+
 // One wave per long run: the lanes load 64 consecutive contributions at a
-// time and the wave adds them in order, reading lane i's term with
-// v_readlane (every lane carries the same sum), so a cell holding a dense
-// clump streams its terms and the serial chain is three dependent ALU
-// operations per contribution.  Waves loop over the queue; each exits when
-// the queue is drained.
+// time.  With all terms nonnegative (the deposit's weights are) the wave
+// applies them by the map scan above; otherwise it adds them in order,
+// reading lane i's term with v_readlane.  Every lane carries the same sum.
+// Waves loop over the queue; each exits when the queue is drained.
 __global__ void __launch_bounds__(64)
 k_cic_long(int64_t m, const uint32_t *__restrict__ key, const double *__restrict__ ts,
            float *__restrict__ rho, const int64_t *__restrict__ long_runs,
@@ -179,16 +231,139 @@ k_cic_long(int64_t m, const uint32_t *__restrict__ key, const double *__restrict
       const double t = in ? ts[q] : 0.0;
       // contributions of one cell are contiguous: the live lanes are a prefix
       const int cnt = __popcll(__ballot(in));
-      if (cnt == 64) {
-#pragma unroll 16
-        for (int i = 0; i < 64; ++i) r = cic_add(r, readlane_f64(t, i));
-      } else {
-        for (int i = 0; i < cnt; ++i) r = cic_add(r, readlane_f64(t, i));
-      }
       done = cnt < 64;
+      if (__ballot(in && !(t >= 0.0)) != 0ull) {
+        for (int i = 0; i < cnt; ++i) r = cic_add(r, readlane_f64(t, i));
+        continue;
+      }
+      int s = 0;
+      while (s < cnt) {
+        if (!(r >= 0x1p-125f)) {  // zero or subnormal: the double sum's grid varies
+          r = cic_add(r, readlane_f64(t, s));
+          ++s;
+          continue;
+        }
+        const int E = ilogbf(r);
+        const double u = ldexp(1.0, E - 23);
+        const int R = (int)((double)r / u);
+        const bool live = l >= s && l < cnt;
+        PMap mp = live ? pmap_term(t, E) : PMap{0, 0};
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          PMap pv;
+          pv.d0 = __shfl_up(mp.d0, off);
+          pv.d1 = __shfl_up(mp.d1, off);
+          if (l >= off) mp = pmap_compose(pv, mp);
+        }
+        const int d = (R & 1) ? mp.d1 : mp.d0;
+        const int Rl = d >= PMAP_SAT ? PMAP_SAT : R + d;
+        const unsigned long long cross = __ballot(live && Rl >= (1 << 24));
+        const int j = cross ? (__ffsll((long long)cross) - 1) : cnt;
+        if (j > s) r = (float)((double)__shfl(Rl, j - 1) * u);
+        if (j < cnt) {
+          r = cic_add(r, readlane_f64(t, j));
+          s = j + 1;
+        } else {
+          s = cnt;
+        }
+      }
     }
     if (l == 0) rho[cell] = r;
   }
+}
+
+constexpr int TAIL_THREADS = 1024;
+
+// The last run [p0, m) by one workgroup: 1024 terms per step, maps scanned
+// within each wave and across the 16 waves through LDS, the longest prefix
+// that stays in the binade applied, the crossing term added by thread 0.
+__global__ void __launch_bounds__(TAIL_THREADS)
+k_cic_tail(int64_t m, const uint32_t *__restrict__ key, const double *__restrict__ ts,
+           float *__restrict__ rho, const int64_t *__restrict__ tail_start) {
+  __shared__ PMap wtot[TAIL_THREADS / 64];
+  __shared__ float r_sh;
+  __shared__ int j_sh, neg_sh;
+  const int64_t p0 = *tail_start;
+  if (p0 < 0) return;
+  const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
+  if (tid == 0) r_sh = 0.f;
+  __syncthreads();
+  for (int64_t base = p0; base < m; base += TAIL_THREADS) {
+    const int cnt = (int)min<int64_t>(TAIL_THREADS, m - base);
+    const bool in = tid < cnt;
+    const double t = in ? ts[base + tid] : 0.0;
+    if (tid == 0) neg_sh = 0;
+    __syncthreads();
+    if (in && !(t >= 0.0)) neg_sh = 1;
+    __syncthreads();
+    if (neg_sh) {
+      if (tid == 0) {
+        float r = r_sh;
+        for (int i = 0; i < cnt; ++i) r = cic_add(r, ts[base + i]);
+        r_sh = r;
+      }
+      __syncthreads();
+      continue;
+    }
+    int s = 0;
+    while (s < cnt) {
+      const float r = r_sh;
+      if (!(r >= 0x1p-125f)) {
+        __syncthreads();
+        if (tid == 0) r_sh = cic_add(r, ts[base + s]);
+        __syncthreads();
+        ++s;
+        continue;
+      }
+      const int E = ilogbf(r);
+      const double u = ldexp(1.0, E - 23);
+      const int R = (int)((double)r / u);
+      const bool live = tid >= s && tid < cnt;
+      PMap mp = live ? pmap_term(t, E) : PMap{0, 0};
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        PMap pv;
+        pv.d0 = __shfl_up(mp.d0, off);
+        pv.d1 = __shfl_up(mp.d1, off);
+        if (l >= off) mp = pmap_compose(pv, mp);
+      }
+      if (l == 63) wtot[w] = mp;
+      if (tid == 0) j_sh = cnt;
+      __syncthreads();
+      if (w == 0) {  // exclusive prefix of the wave totals, in wave order
+        PMap tv = l < TAIL_THREADS / 64 ? wtot[l] : PMap{0, 0};
+#pragma unroll
+        for (int off = 1; off < TAIL_THREADS / 64; off <<= 1) {
+          PMap pv;
+          pv.d0 = __shfl_up(tv.d0, off);
+          pv.d1 = __shfl_up(tv.d1, off);
+          if (l >= off) tv = pmap_compose(pv, tv);
+        }
+        PMap ex;
+        ex.d0 = __shfl_up(tv.d0, 1);
+        ex.d1 = __shfl_up(tv.d1, 1);
+        if (l == 0) ex = PMap{0, 0};
+        if (l < TAIL_THREADS / 64) wtot[l] = ex;
+      }
+      __syncthreads();
+      mp = pmap_compose(wtot[w], mp);
+      const int d = (R & 1) ? mp.d1 : mp.d0;
+      const int Rl = d >= PMAP_SAT ? PMAP_SAT : R + d;
+      if (live && Rl >= (1 << 24)) atomicMin(&j_sh, tid);
+      __syncthreads();
+      const int j = j_sh;
+      if (j > s && tid == j - 1) r_sh = (float)((double)Rl * u);
+      __syncthreads();
+      if (j < cnt) {
+        if (tid == 0) r_sh = cic_add(r_sh, ts[base + j]);
+        s = j + 1;
+      } else {
+        s = cnt;
+      }
+      __syncthreads();
+    }
+  }
+  if (tid == 0) rho[key[p0]] = r_sh;
 }
 
 // Particles.cxx:682-711, one component: f accumulated as the deposit's runs
@@ -211,8 +386,11 @@ k_inverse_cic(int64_t n, const float *__restrict__ x, const float *__restrict__ 
   v[nn] = v[nn] + f * fscal * tau;
 }
 
-__global__ void k_zero_u64(unsigned long long *p) {
-  if (threadIdx.x == 0) p[0] = 0ull;
+__global__ void k_zero_u64(unsigned long long *p, int64_t *tail) {
+  if (threadIdx.x == 0) {
+    p[0] = 0ull;
+    tail[0] = -1;
+  }
 }
 
 __global__ void k_fill0(int64_t n, float *a) {
@@ -247,7 +425,7 @@ hsr_status cic_run(hsr_ctx *c, int64_t n, const float *x, const float *y, const 
   // sorted terms (2 x 8 B) and the long-run queue (8 B; at most one run per
   // contribution)
   const size_t mm = (size_t)m;
-  HSR_HIP(dbuf_reserve(c->sub.keys, 40 * mm + 512));
+  HSR_HIP(dbuf_reserve(c->sub.keys, 40 * mm + 1024));
   uint32_t *key = dptr<uint32_t>(c->sub.keys);
   uint32_t *key2 = key + mm;
   int32_t *val = (int32_t *)(key2 + mm);
@@ -256,7 +434,8 @@ hsr_status cic_run(hsr_ctx *c, int64_t n, const float *x, const float *y, const 
   double *ts = term + mm;
   int64_t *long_runs = (int64_t *)(ts + mm);
   unsigned long long *nlong = (unsigned long long *)(long_runs + mm);
-  k_zero_u64<<<1, 64, 0, st>>>(nlong);
+  int64_t *tail_start = (int64_t *)(nlong + 1);
+  k_zero_u64<<<1, 64, 0, st>>>(nlong, tail_start);
   k_cic_contrib<<<nblk(n, 256), 256, 0, st>>>(n, x, y, z, g, cc, key, val, term);
   HSR_HIP(hipGetLastError());
   size_t tmp = 0;
@@ -264,9 +443,10 @@ hsr_status cic_run(hsr_ctx *c, int64_t n, const float *x, const float *y, const 
   HSR_HIP(dbuf_reserve(c->s.cub, tmp));
   HSR_HIP(hipcub::DeviceRadixSort::SortPairs(c->s.cub.p, tmp, key, key2, val, val2, (int)m, 0, bits, st));
   k_cic_gather<<<nblk(m, 256), 256, 0, st>>>(m, val2, term, ts);
-  k_cic_runs<<<nblk(m, 256), 256, 0, st>>>(m, key2, ts, rho, long_runs, nlong);
-  const int waves = std::max(1, c->num_cus) * 8;
+  k_cic_runs<<<nblk(m, 256), 256, 0, st>>>(m, key2, ts, rho, long_runs, nlong, tail_start);
+  const int waves = std::max(1, c->num_cus) * 32;  // latency-bound: many waves in flight
   k_cic_long<<<waves, 64, 0, st>>>(m, key2, ts, rho, long_runs, nlong);
+  k_cic_tail<<<1, TAIL_THREADS, 0, st>>>(m, key2, ts, rho, tail_start);
   HSR_HIP(hipGetLastError());
   return HSR_OK;
 }
