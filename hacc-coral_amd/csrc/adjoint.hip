@@ -55,14 +55,16 @@ struct AdjLaw {
   float dp[6];  // 2 * k * c_k: 2 P'(s) coefficients (dp[k-1] for s^(k-1))
   float rcs, eps2;
   int order;
-  int cull;     // drop entries farther than the cutoff from the target leaf's box
+  int cull;     // drop entries farther than the cutoff from a pass's target box
   float cull2;  // squared box distance beyond which every pair is masked
 };
 
-__device__ __forceinline__ float box_d2(const hsr_node &nd, float x, float y, float z) {
-  const float dx = fmaxf(0.f, fmaxf(nd.xmin[0] - x, x - nd.xmax[0]));
-  const float dy = fmaxf(0.f, fmaxf(nd.xmin[1] - y, y - nd.xmax[1]));
-  const float dz = fmaxf(0.f, fmaxf(nd.xmin[2] - z, z - nd.xmax[2]));
+// Squared distance from p to the box [lo, hi] (0 inside).
+__device__ __forceinline__ float box_d2(const float *lo, const float *hi, float x, float y,
+                                        float z) {
+  const float dx = fmaxf(0.f, fmaxf(lo[0] - x, x - hi[0]));
+  const float dy = fmaxf(0.f, fmaxf(lo[1] - y, y - hi[1]));
+  const float dz = fmaxf(0.f, fmaxf(lo[2] - z, z - hi[2]));
   return dx * dx + dy * dy + dz * dz;
 }
 
@@ -296,8 +298,8 @@ k_adj(const hsr_node *__restrict__ cn, const AItem *__restrict__ items,
 // Chunked items: partials summed in chunk order.  MODE 0 writes, MODE 1 adds.
 __global__ void __launch_bounds__(64)
 k_adj_reduce(const hsr_node *cn, const AItem *items, const int64_t *first,
-             int64_t nmulti, int add, const float *part, float *gx, float *gy,
-             float *gz) {
+             int64_t nmulti, int add, const int32_t *iperm, const float *part, float *gx,
+             float *gy, float *gz) {
   const int64_t k = blockIdx.x;
   if (k >= nmulti) return;
   const int64_t i0 = first[k], i1 = first[k + 1];
@@ -311,7 +313,8 @@ k_adj_reduce(const hsr_node *cn, const AItem *items, const int64_t *first,
       sy += p[1];
       sz += p[2];
     }
-    const int64_t q = nd.offset + i;
+    // partials of the fused schedule are kept in target order
+    const int64_t q = iperm ? (int64_t)iperm[nd.offset + i] : nd.offset + i;
     if (add) {
       gx[q] += sx;
       gy[q] += sy;
@@ -437,8 +440,9 @@ template <int ORDER>
 __device__ __forceinline__ void
 adj_fused_body(const hsr_node *__restrict__ cn, const AItem *__restrict__ items,
                const int64_t *__restrict__ nitems_p, const int32_t *__restrict__ jidx,
-               const PJ *__restrict__ pj, const float *__restrict__ lx,
-               const float *__restrict__ ly, const float *__restrict__ lz, const AdjLaw &L,
+               const PJ *__restrict__ pj, const int32_t *__restrict__ iperm,
+               const float *__restrict__ lx, const float *__restrict__ ly,
+               const float *__restrict__ lz, const AdjLaw &L,
                unsigned long long *__restrict__ qctr, float *__restrict__ gx,
                float *__restrict__ gy, float *__restrict__ gz, float *__restrict__ part,
                float *__restrict__ wx, float *__restrict__ wy, float *__restrict__ wz) {
@@ -470,11 +474,13 @@ adj_fused_body(const hsr_node *__restrict__ cn, const AItem *__restrict__ items,
       const int G = 64 / lp;
       const int g = l / lp, tq = l & (lp - 1);
       float xi[FR], yi[FR], zi[FR], la[FR], lb[FR], lc[FR], am[FR], bx[FR], by[FR], bz[FR];
+      // the pass's targets (in the leaf's target order) and their box
+      float bmin[3] = {INFINITY, INFINITY, INFINITY}, bmax[3] = {-INFINITY, -INFINITY, -INFINITY};
 #pragma unroll
       for (int r = 0; r < FR; ++r) {
         const int ii = tq + r * lp;
         const bool v = ii < ni;
-        const int64_t q = off + ib + (v ? ii : 0);
+        const int64_t q = iperm[off + ib + (v ? ii : 0)];
         const PJ P = pj[q];
         xi[r] = v ? P.x : padI;
         yi[r] = v ? P.y : nd.xc[1];
@@ -483,6 +489,19 @@ adj_fused_body(const hsr_node *__restrict__ cn, const AItem *__restrict__ items,
         lb[r] = v ? ly[q] : 0.f;
         lc[r] = v ? lz[q] : 0.f;
         am[r] = bx[r] = by[r] = bz[r] = 0.f;
+        if (v) {
+          bmin[0] = fminf(bmin[0], P.x); bmax[0] = fmaxf(bmax[0], P.x);
+          bmin[1] = fminf(bmin[1], P.y); bmax[1] = fmaxf(bmax[1], P.y);
+          bmin[2] = fminf(bmin[2], P.z); bmax[2] = fmaxf(bmax[2], P.z);
+        }
+      }
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          bmin[d] = fminf(bmin[d], __shfl_xor(bmin[d], o));
+          bmax[d] = fmaxf(bmax[d], __shfl_xor(bmax[d], o));
+        }
       }
       for (int64_t t0 = I.e0; t0 < I.e1; t0 += FTILE) {
         const int nt = (int)min<int64_t>(FTILE, I.e1 - t0);
@@ -490,9 +509,10 @@ adj_fused_body(const hsr_node *__restrict__ cn, const AItem *__restrict__ items,
         tgmax = (tgmax + FUNROLL - 1) / FUNROLL * FUNROLL;
         const int stride = tgmax + ((9 - (tgmax & 7)) & 7);
         __syncthreads();
-        // entries beyond the cutoff of the whole leaf have W = 0 and are
-        // dropped; the kept ones go round-robin to the groups (c -> group
-        // c % G, slot c / G), qmap remembering their place in the tile
+        // entries beyond the cutoff of the pass's targets add nothing to W in
+        // this pass and are dropped (the first pass zeroes their W); the
+        // kept ones go round-robin to the groups (c -> group c % G, slot
+        // c / G), qmap remembering their place in the tile
         int nk = 0;
         for (int q0 = 0; q0 < nt; q0 += 64) {
           const int q = q0 + l;
@@ -501,7 +521,7 @@ adj_fused_body(const hsr_node *__restrict__ cn, const AItem *__restrict__ items,
           if (q < nt) {
             const PJ P = pj[jidx[t0 + q]];
             a = make_float4(P.x, P.y, P.z, P.m);
-            keep = !L.cull || box_d2(nd, P.x, P.y, P.z) <= L.cull2;
+            keep = !L.cull || box_d2(bmin, bmax, P.x, P.y, P.z) <= L.cull2;
             if (!keep && first) wx[t0 + q] = wy[t0 + q] = wz[t0 + q] = 0.f;
           }
           const uint64_t bal = __ballot(keep);
@@ -560,7 +580,7 @@ adj_fused_body(const hsr_node *__restrict__ cn, const AItem *__restrict__ items,
         __syncthreads();
       }
       for (int ii = l; ii < ni; ii += 64) {
-        const int64_t q = off + ib + ii;
+        const int64_t q = iperm[off + ib + ii];
         const float a = red[3][ii];
         const float vx = -fmaf(a, lx[q], red[0][ii]);
         const float vy = -fmaf(a, ly[q], red[1][ii]);
@@ -593,18 +613,21 @@ adj_fused_body(const hsr_node *__restrict__ cn, const AItem *__restrict__ items,
 #define HSR_ADJ_FUSED_ARGS                                                             \
   const hsr_node *__restrict__ cn, const AItem *__restrict__ items,                    \
       const int64_t *__restrict__ nitems_p, const int32_t *__restrict__ jidx,          \
-      const PJ *__restrict__ pj, const float *__restrict__ lx,                         \
+      const PJ *__restrict__ pj, const int32_t *__restrict__ iperm,                    \
+      const float *__restrict__ lx,                                                    \
       const float *__restrict__ ly, const float *__restrict__ lz, AdjLaw L,            \
       unsigned long long *__restrict__ qctr, float *__restrict__ gx,                   \
       float *__restrict__ gy, float *__restrict__ gz, float *__restrict__ part,        \
       float *__restrict__ wx, float *__restrict__ wy, float *__restrict__ wz
 __global__ void __launch_bounds__(64) HSR_ADJ_ATTR
 k_adj_fused5(HSR_ADJ_FUSED_ARGS) {
-  adj_fused_body<5>(cn, items, nitems_p, jidx, pj, lx, ly, lz, L, qctr, gx, gy, gz, part, wx, wy, wz);
+  adj_fused_body<5>(cn, items, nitems_p, jidx, pj, iperm, lx, ly, lz, L, qctr, gx, gy, gz, part, wx, wy,
+                    wz);
 }
 __global__ void __launch_bounds__(64)
 k_adj_fused6(HSR_ADJ_FUSED_ARGS) {
-  adj_fused_body<6>(cn, items, nitems_p, jidx, pj, lx, ly, lz, L, qctr, gx, gy, gz, part, wx, wy, wz);
+  adj_fused_body<6>(cn, items, nitems_p, jidx, pj, iperm, lx, ly, lz, L, qctr, gx, gy, gz, part, wx, wy,
+                    wz);
 }
 
 // Fused schedule, source side: one wave per source leaf (a run of sorted
@@ -1218,6 +1241,7 @@ static hsr_status adjoint_enqueue(hsr_ctx *c, const float *lx, const float *ly,
   L.cull = c->tuning.cull_off ? 0 : 1;
   unsigned long long *q = dptr<unsigned long long>(A.qctr);
   float *part = dptr<float>(A.part);
+  const int32_t *iperm = dptr<int32_t>(c->lists.iperm);
   float *H = dptr<float>(A.H);
   float *gseg = H + 3 * (nn + 1);
   const int64_t *counts = dptr<int64_t>(A.counts);
@@ -1239,10 +1263,12 @@ static hsr_status adjoint_enqueue(hsr_ctx *c, const float *lx, const float *ly,
       const int64_t ne = c->lists.nentries + 1;
       if (L.order == 6)
         k_adj_fused6<<<(unsigned)grid, 64, 0, st>>>(cn, dptr<AItem>(A.fw_items), counts,
-            dptr<int32_t>(c->lists.jidx), pj, lx, ly, lz, L, q, gx, gy, gz, part, W, W + ne, W + 2 * ne);
+            dptr<int32_t>(c->lists.jidx), pj, iperm, lx, ly, lz, L, q, gx, gy, gz, part, W, W + ne,
+            W + 2 * ne);
       else
         k_adj_fused5<<<(unsigned)grid, 64, 0, st>>>(cn, dptr<AItem>(A.fw_items), counts,
-            dptr<int32_t>(c->lists.jidx), pj, lx, ly, lz, L, q, gx, gy, gz, part, W, W + ne, W + 2 * ne);
+            dptr<int32_t>(c->lists.jidx), pj, iperm, lx, ly, lz, L, q, gx, gy, gz, part, W, W + ne,
+            W + 2 * ne);
     } else if (L.order == 6) {
       LA(0, 6, dptr<AItem>(A.fw_items), counts);
     } else {
@@ -1250,7 +1276,8 @@ static hsr_status adjoint_enqueue(hsr_ctx *c, const float *lx, const float *ly,
     }
     if (A.fw_part)
       k_adj_reduce<<<(unsigned)A.n_fw_leaves, 64, 0, st>>>(cn, dptr<AItem>(A.fw_items), A.fw_first,
-                                                            A.n_fw_leaves, 0, part, gx, gy, gz);
+                                                            A.n_fw_leaves, 0, A.fused ? iperm : nullptr,
+                                                            part, gx, gy, gz);
   }
   const int64_t ns = c->lists.nsegs;
   const int64_t *seg_e = dptr<int64_t>(c->lists.seg) + 2 * (ns + 1);
@@ -1270,7 +1297,7 @@ static hsr_status adjoint_enqueue(hsr_ctx *c, const float *lx, const float *ly,
 #undef LA
     if (A.tr_part)
       k_adj_reduce<<<(unsigned)A.n_tr_leaves, 64, 0, st>>>(cn, dptr<AItem>(A.tr_items), A.tr_first,
-                                                            A.n_tr_leaves, 1, part, gx, gy, gz);
+                                                            A.n_tr_leaves, 1, nullptr, part, gx, gy, gz);
   }
   // pseudo pass
   if (nn) k_fill<<<nblk(3 * nn, 256), 256, 0, st>>>(3 * nn, H);

@@ -835,8 +835,15 @@ hsr_status hsr_set_tuning(hsr_ctx *c, const hsr_tuning *t) {
     g_err = "hsr_set_tuning: null argument";
     return HSR_ERR_ARG;
   }
+  const bool reorder = t->tree_order_targets != c->tuning.tree_order_targets;
   c->tuning = *t;
   c->graph_gen = -1;  // a captured force phase holds the old settings
+  if (reorder && c->listed) {
+    // the target order is part of the force plan
+    hsr_status s = force_plan(c);
+    if (s) return s;
+    c->lists_gen++;
+  }
   return HSR_OK;
 }
 

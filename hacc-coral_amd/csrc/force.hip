@@ -52,10 +52,11 @@ struct LawArgs {
 };
 
 // Squared distance from p to the box [lo, hi] (0 inside).
-__device__ __forceinline__ float box_d2(const hsr_node &nd, float x, float y, float z) {
-  const float dx = fmaxf(0.f, fmaxf(nd.xmin[0] - x, x - nd.xmax[0]));
-  const float dy = fmaxf(0.f, fmaxf(nd.xmin[1] - y, y - nd.xmax[1]));
-  const float dz = fmaxf(0.f, fmaxf(nd.xmin[2] - z, z - nd.xmax[2]));
+__device__ __forceinline__ float box_d2(const float *lo, const float *hi, float x, float y,
+                                        float z) {
+  const float dx = fmaxf(0.f, fmaxf(lo[0] - x, x - hi[0]));
+  const float dy = fmaxf(0.f, fmaxf(lo[1] - y, y - hi[1]));
+  const float dz = fmaxf(0.f, fmaxf(lo[2] - z, z - hi[2]));
   return dx * dx + dy * dy + dz * dz;
 }
 
@@ -100,9 +101,9 @@ __global__ void __launch_bounds__(64) HSR_FORCE_ATTR
 k_force(const hsr_node *__restrict__ cn, const int32_t *__restrict__ leaves,
         const Item *__restrict__ items, const int64_t *__restrict__ nitems_p,
         const int32_t *__restrict__ jidx, const PJ *__restrict__ pj,
-        LawArgs L, unsigned long long *__restrict__ qctr,
-        float *__restrict__ fx, float *__restrict__ fy,
-        float *__restrict__ fz, float *__restrict__ part,
+        const int32_t *__restrict__ iperm, LawArgs L,
+        unsigned long long *__restrict__ qctr, float *__restrict__ fx,
+        float *__restrict__ fy, float *__restrict__ fz, float *__restrict__ part,
         unsigned long long *__restrict__ evals) {
   // the tile doubles as the reduction buffer once the j-walk is done
   __shared__ float4 tile[LDS_ENT];
@@ -133,15 +134,30 @@ k_force(const hsr_node *__restrict__ cn, const int32_t *__restrict__ leaves,
       const int g = l / lpg, tq = l - g * lpg;
       const bool active = g < G;
       float xi[R], yi[R], zi[R], ax[R], ay[R], az[R];
+      // the pass's targets (in the leaf's target order) and their box
+      float bmin[3] = {INFINITY, INFINITY, INFINITY}, bmax[3] = {-INFINITY, -INFINITY, -INFINITY};
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int ii = tq + r * lpg;
         const bool v = active && ii < ni;
-        const PJ p = pj[off + ib + (v ? ii : 0)];
+        const PJ p = pj[iperm[off + ib + (v ? ii : 0)]];
         xi[r] = v ? p.x : 1e30f;
         yi[r] = v ? p.y : 1e30f;
         zi[r] = v ? p.z : 1e30f;
         ax[r] = ay[r] = az[r] = 0.f;
+        if (v) {
+          bmin[0] = fminf(bmin[0], p.x); bmax[0] = fmaxf(bmax[0], p.x);
+          bmin[1] = fminf(bmin[1], p.y); bmax[1] = fmaxf(bmax[1], p.y);
+          bmin[2] = fminf(bmin[2], p.z); bmax[2] = fmaxf(bmax[2], p.z);
+        }
+      }
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          bmin[d] = fminf(bmin[d], __shfl_xor(bmin[d], o));
+          bmax[d] = fmaxf(bmax[d], __shfl_xor(bmax[d], o));
+        }
       }
       for (int64_t t0 = I.e0; t0 < I.e1; t0 += TILE) {
         const int nt = (int)min<int64_t>(TILE, I.e1 - t0);
@@ -150,9 +166,10 @@ k_force(const hsr_node *__restrict__ cn, const int32_t *__restrict__ leaves,
         tgmax = (tgmax + UNROLL - 1) / UNROLL * UNROLL;
         const int stride = tgmax + ((9 - (tgmax & 7)) & 7);
         __syncthreads();
-        // Entries farther than the cutoff from the leaf's box contribute 0 to
-        // every target and are dropped; the kept ones, in list order, go
-        // round-robin to the groups (entry c -> group c % G, slot c / G).
+        // Entries farther than the cutoff from the pass's box contribute 0 to
+        // every target of the pass and are dropped; the kept ones, in list
+        // order, go round-robin to the groups (entry c -> group c % G, slot
+        // c / G).
         int nk = 0;
         for (int q0 = 0; q0 < nt; q0 += 64) {
           const int q = q0 + l;
@@ -161,7 +178,7 @@ k_force(const hsr_node *__restrict__ cn, const int32_t *__restrict__ leaves,
           if (q < nt) {
             const PJ p = pj[jidx[t0 + q]];
             v = make_float4(p.x, p.y, p.z, p.m);
-            keep = !L.cull || box_d2(nd, p.x, p.y, p.z) <= L.cull2;
+            keep = !L.cull || box_d2(bmin, bmax, p.x, p.y, p.z) <= L.cull2;
           }
           const uint64_t bal = __ballot(keep);
           if (keep) {
@@ -225,9 +242,10 @@ k_force(const hsr_node *__restrict__ cn, const int32_t *__restrict__ leaves,
       }
       if (I.pbase < 0) {
         for (int ii = l; ii < ni; ii += 64) {
-          fx[off + ib + ii] = red[0][ii];
-          fy[off + ib + ii] = red[1][ii];
-          fz[off + ib + ii] = red[2][ii];
+          const int32_t q = iperm[off + ib + ii];
+          fx[q] = red[0][ii];
+          fy[q] = red[1][ii];
+          fz[q] = red[2][ii];
         }
       } else {
         float *pp = part + I.pbase;
@@ -247,7 +265,7 @@ __global__ void __launch_bounds__(64)
 k_reduce_chunks(const hsr_node *cn, const int32_t *leaves,
                 const int64_t *multi, const int64_t *nmulti_p,
                 const int64_t *leaf_pbase, const int32_t *leaf_nchunk,
-                const float *part, float *fx, float *fy, float *fz) {
+                const int32_t *iperm, const float *part, float *fx, float *fy, float *fz) {
   const int64_t k = blockIdx.x;
   if (k >= *nmulti_p) return;
   const int64_t lf = multi[k];
@@ -263,9 +281,10 @@ k_reduce_chunks(const hsr_node *cn, const int32_t *leaves,
       sy += p[1];
       sz += p[2];
     }
-    fx[off + i] = sx;
-    fy[off + i] = sy;
-    fz[off + i] = sz;
+    const int32_t q = iperm[off + i];  // partials are kept in target order
+    fx[q] = sx;
+    fy[q] = sy;
+    fz[q] = sz;
   }
 }
 
@@ -389,6 +408,67 @@ LawArgs law_args(const ForceLawDev &D) {
 }  // namespace
 
 // Work items of the current lists (rebuilt after every hsr_lists call).
+// Target order inside each listed leaf: positions sorted along the leaf's
+// longest box edge (ties x > y > z; equal coordinates by position), so that
+// a pass's 128 targets span a slab of the leaf and the entries its box is
+// beyond the cutoff of can be skipped.  iperm[off + k] = position of the
+// k-th target; leaves above SORT_MAX keep their order.
+constexpr int SORT_MAX = 1024;
+
+__global__ void k_iota32(int64_t n, int32_t *a) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < n) a[k] = (int32_t)k;
+}
+
+__global__ void __launch_bounds__(256)
+k_leaf_sort(int64_t nl, const int32_t *__restrict__ leaves, const hsr_node *__restrict__ cn,
+            const PJ *__restrict__ pj, int32_t *__restrict__ iperm) {
+  __shared__ float key[SORT_MAX];
+  __shared__ int32_t idx[SORT_MAX];
+  const int64_t k = blockIdx.x;
+  if (k >= nl) return;
+  const hsr_node &nd = cn[leaves[k]];
+  const int64_t off = nd.offset, cnt = nd.count;
+  if (cnt < 2 || cnt > SORT_MAX) return;
+  const float e0 = nd.xmax[0] - nd.xmin[0], e1 = nd.xmax[1] - nd.xmin[1],
+              e2 = nd.xmax[2] - nd.xmin[2];
+  const int d = (e0 >= e1 && e0 >= e2) ? 0 : (e1 >= e2 ? 1 : 2);
+  int np2 = 2;
+  while (np2 < cnt) np2 <<= 1;
+  for (int i = threadIdx.x; i < np2; i += blockDim.x) {
+    if (i < cnt) {
+      const PJ p = pj[off + i];
+      key[i] = d == 0 ? p.x : (d == 1 ? p.y : p.z);
+    } else {
+      key[i] = INFINITY;
+    }
+    idx[i] = i;
+  }
+  __syncthreads();
+  // bitonic sort of (key, idx), ascending and lexicographic
+  for (int size = 2; size <= np2; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = threadIdx.x; i < np2; i += blockDim.x) {
+        const int j = i ^ stride;
+        if (j > i) {
+          const bool up = (i & size) == 0;
+          const float ki = key[i], kj = key[j];
+          const int32_t ii = idx[i], ij = idx[j];
+          const bool gt = ki > kj || (ki == kj && ii > ij);
+          if (gt == up) {
+            key[i] = kj;
+            key[j] = ki;
+            idx[i] = ij;
+            idx[j] = ii;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < cnt; i += blockDim.x) iperm[off + i] = (int32_t)(off + idx[i]);
+}
+
 hsr_status force_plan(hsr_ctx *c) {
   hipStream_t st = c->stream;
   ListState &LS = c->lists;
@@ -457,6 +537,13 @@ hsr_status force_plan(hsr_ctx *c) {
   LS.counts = counts;
   LS.leaf_pbase = pbase;
   LS.leaf_nchunk = nchunk;
+  // target order inside the listed leaves
+  const int64_t n = c->tree.n;
+  HSR_HIP(dbuf_reserve(LS.iperm, sizeof(int32_t) * std::max<int64_t>(n, 1)));
+  int32_t *iperm = dptr<int32_t>(LS.iperm);
+  if (n) k_iota32<<<nblk(n, 256), 256, 0, st>>>(n, iperm);
+  if (!c->tuning.tree_order_targets)
+    k_leaf_sort<<<(unsigned)nl, 256, 0, st>>>(nl, leaves, cn, dptr<PJ>(c->tree.pj), iperm);
   HSR_HIP(hipGetLastError());
   return HSR_OK;
 }
@@ -530,7 +617,8 @@ static hsr_status force_enqueue(hsr_ctx *c, float *fx, float *fy, float *fz,
   if (timed) HSR_HIP(hipEventRecord(c->ev[6], st));
 #define LAUNCH(ORD, GRD)                                                          \
   k_force<ORD, GRD><<<(unsigned)grid, 64, 0, st>>>(cn, leaves, items, LS.counts, \
-      dptr<int32_t>(LS.jidx), dptr<PJ>(c->tree.pj), A, qctr, fx, fy, fz, part, qctr + 1)
+      dptr<int32_t>(LS.jidx), dptr<PJ>(c->tree.pj), dptr<int32_t>(LS.iperm), A, qctr, fx, fy, fz, \
+      part, qctr + 1)
   if (c->law.order == 6) {
     if (A.guard) LAUNCH(6, true); else LAUNCH(6, false);
   } else {
@@ -544,7 +632,8 @@ static hsr_status force_enqueue(hsr_ctx *c, float *fx, float *fy, float *fz,
   }
   if (LS.nmulti > 0)
     k_reduce_chunks<<<(unsigned)LS.nmulti, 64, 0, st>>>(cn, leaves, LS.multi, LS.counts + 1, LS.leaf_pbase,
-                                                       LS.leaf_nchunk, part, fx, fy, fz);
+                                                       LS.leaf_nchunk, dptr<int32_t>(LS.iperm), part,
+                                                       fx, fy, fz);
   HSR_HIP(hipGetLastError());
   return HSR_OK;
 }

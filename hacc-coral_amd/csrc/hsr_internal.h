@@ -123,6 +123,7 @@ struct ListState {
   int64_t *multi = nullptr, *counts = nullptr, *leaf_pbase = nullptr;
   int32_t *leaf_nchunk = nullptr;
   DBuf stack;                 // DFS stacks
+  DBuf iperm;                 // int32 [n]: target order inside each listed leaf
 };
 
 // Adjoint plan: item queues of the target and source passes, the

@@ -88,7 +88,7 @@ class Tuning(C.Structure):
     _fields_ = [("i_per_lane", C.c_int), ("tile", C.c_int), ("chunk", C.c_int),
                 ("waves_per_cu", C.c_int), ("use_graph", C.c_int),
                 ("adjoint_mode", C.c_int), ("cull_off", C.c_int),
-                ("reserved", C.c_int * 1)]
+                ("tree_order_targets", C.c_int)]
 
 
 _lib = None

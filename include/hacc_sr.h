@@ -301,9 +301,12 @@ typedef struct {
                          passes (automatic picks fused when that memory is
                          free) */
   int cull_off;       /* 1: evaluate every list entry; 0 (default): skip entries
-                         farther than the cutoff from the target leaf's box,
-                         whose pairs are all masked (results unchanged) */
-  int reserved[1];
+                         farther than the cutoff from the box of the targets
+                         of a pass, whose pairs are all masked */
+  int tree_order_targets; /* 1: take a leaf's targets in tree order; 0
+                         (default): sorted along the leaf's longest edge, so
+                         that each pass of 128 targets spans a slab and culls
+                         more entries (sums in another order: rounding only) */
 } hsr_tuning;
 
 hsr_status hsr_set_tuning(hsr_ctx *ctx, const hsr_tuning *t);
