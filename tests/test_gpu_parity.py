@@ -271,3 +271,31 @@ def test_graph_replay_matches_eager():
     ctx.lists([1.0] * 3, [15.0] * 3)
     e2 = check_forces(tp, tree, lists, ctx)
     say(f"  replays ok; after re-listing rel rms {e2:.3e}")
+
+
+def test_culling_and_target_order_change_rounding_only():
+    """Skipping list entries beyond the cutoff of a pass's targets, and
+    sorting a leaf's targets along its longest edge, leave every pair that
+    the cutoff keeps: forces agree with the unculled tree-order kernel to
+    float rounding of the reordered sums (5e-6 relative RMS, below the
+    1e-5 bar each setting meets against the oracle), and the evaluated pair
+    counts order as expected."""
+    say("case culling / target order on clustered 48^3, nd=256")
+    load = ics.config_load("B", n_side=48)
+    lo, hi = load.tree_box
+    flo, fhi = load.force_box
+    m = np.ones(load.n, np.float32)
+    tp, tree, lists, ctx, arrs = run_both(load.x, load.y, load.z, m, lo, hi, flo, fhi, 256)
+    out, ev = {}, {}
+    for cull_off, tree_order in ((1, 1), (0, 1), (0, 0)):
+        ctx.set_tuning(cull_off=cull_off, tree_order_targets=tree_order)
+        out[(cull_off, tree_order)] = np.stack(ctx.forces(), 1)
+        ev[(cull_off, tree_order)] = ctx.stats().evaluated_pairs
+        check_forces(tp, tree, lists, ctx)
+    ref = out[(1, 1)]
+    for k in ((0, 1), (0, 0)):
+        assert rel_rms(out[k], ref) < 5e-6, (k, rel_rms(out[k], ref))
+    pairs = ctx.stats().pairs
+    say(f"  evaluated pairs: all {ev[(1, 1)]}, leaf-box culled {ev[(0, 1)]}, "
+        f"sorted targets {ev[(0, 0)]} of {pairs}")
+    assert ev[(1, 1)] == pairs and ev[(0, 0)] <= ev[(0, 1)] < ev[(1, 1)]
