@@ -61,6 +61,8 @@ def parse():
                     help="sub-steps of the device-resident sub-cycle leg (0: skip it)")
     ap.add_argument("--pm-reps", type=int, default=3,
                     help="timed CIC deposit/interpolation pairs of the PM leg (0: skip it)")
+    ap.add_argument("--refresh-reps", type=int, default=3,
+                    help="timed one-rank overload refreshes of the refresh leg (0: skip it)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run one plain device-to-device copy of BYTES (the "
                          "reference kernel for scaling FETCH_SIZE/WRITE_SIZE counters)")
@@ -258,6 +260,58 @@ def pm_leg(load, dev, reps=3):
                     "8 contributions per particle sorted by cell, runs added in order"}
 
 
+def refresh_leg(load, dev, reps=3):
+    """MC3Extras' particle refresh on one rank (hsr_refresh: alive particles
+    kept, the 26 neighbours' overload copies appended, unit masses, resort;
+    device arrays) on the same particles, whose old ghosts it drops and
+    regenerates.  Domain: rL = ng = L, oL = the overload width (rounded up to
+    whole cells by Domain::initialize).  Algorithmic bytes: the exchange
+    reads 12 B per particle to classify and 38 B to pack, and writes 42 B per
+    refreshed particle; the resort reads 12 B per refreshed particle for its
+    keys and moves 84 B in its gather (the radix passes over the 8 B
+    (cell, index) pairs are not counted)."""
+    import torch
+    from haccsr import Context, _lib
+    ng = int(round(load.box))
+    dom = _lib.domain(load.box, load.overload, ng)
+    shift = float(dom.ng_overload) - float(load.overload)
+    n = load.n
+    cap = n + n // 4
+    rng = np.random.default_rng(7)
+    host = {"x": (load.x + shift).astype(np.float32), "y": (load.y + shift).astype(np.float32),
+            "z": (load.z + shift).astype(np.float32)}
+    for k in ("vx", "vy", "vz"):
+        host[k] = rng.normal(size=n).astype(np.float32)
+    host["mass"] = np.ones(n, np.float32)
+    host["phi"] = np.zeros(n, np.float32)
+    host["id"] = np.arange(n, dtype=np.int64)
+    host["mask"] = np.zeros(n, np.uint16)
+    names = ("x", "y", "z", "vx", "vy", "vz", "mass", "phi", "id", "mask")
+    src = {k: torch.from_numpy(host[k]).to(dev) for k in names}
+    buf = {k: torch.empty(cap, dtype=src[k].dtype, device=dev) for k in names}
+    ctx = Context()
+    ms, res = [], None
+    for r in range(reps + 1):       # the first call allocates
+        for k in names:
+            buf[k][:n].copy_(src[k])
+        torch.cuda.synchronize()
+        res = ctx.refresh(dom, 1.0, n, *(buf[k] for k in names), device_ptrs=True)
+        if r:
+            ms.append(ctx.stats().refresh_ms)
+    st = ctx.stats()
+    ctx.close()
+    m = res[0]
+    t = float(np.mean(ms))
+    b = 50 * n + 138 * m
+    return {"ms": t, "particles_in": n, "alive": int(st.refresh_alive),
+            "copies": int(st.refresh_dead), "inside": res[1], "ng_local_total": dom.ng_local_total,
+            "algorithmic_bytes": b, "GB/s": b / (t * 1e-3) / 1e9,
+            "frac": b / (t * 1e-3) / 1e9 / PEAK_HBM_GBPS,
+            "note": "bit-exact with the oracle's restatement of copyAliveIntoVectors, "
+                    "ParticleExchange (every neighbour this rank), copyAllFromVectors and "
+                    "resortParticles"}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -410,6 +464,9 @@ def main():
     if a.pm_reps > 0 and ens is None:
         out["pm"] = pm_leg(load, dev, a.pm_reps)
         log(f"pm {out['pm']}")
+    if a.refresh_reps > 0 and ens is None:
+        out["refresh"] = refresh_leg(load, dev, a.refresh_reps)
+        log(f"refresh {out['refresh']}")
     if not a.no_cpu and ens is None:
         log("cpu baseline leg ...")
         out["cpu_baseline"] = cpu_baseline(load, nd, a.cpu_seconds)

@@ -392,6 +392,9 @@ hsr_status hsr_destroy(hsr_ctx *c) {
   for (int k = 0; k < 10; ++k) dbuf_free(c->s.host_in[k]);
   dbuf_free(c->sub.keys);
   dbuf_free(c->sub.stage);
+  dbuf_free(c->sub.xcnt);
+  dbuf_free(c->sub.xstage);
+  dbuf_free(c->sub.xout);
   for (int k = 0; k < 10; ++k) dbuf_free(c->sub.arr[k]);
   for (int k = 0; k < 6; ++k)
     if (c->sub.ev[k]) (void)hipEventDestroy(c->sub.ev[k]);
@@ -733,6 +736,9 @@ hsr_status hsr_get_stats(hsr_ctx *c, hsr_stats *st) {
   st->kick_coeff = c->sub.kick_coeff;
   st->cic_ms = c->sub.cic_ms;
   st->inverse_cic_ms = c->sub.inverse_cic_ms;
+  st->refresh_ms = c->sub.refresh_ms;
+  st->refresh_alive = c->sub.refresh_alive;
+  st->refresh_dead = c->sub.refresh_dead;
   return HSR_OK;
 }
 
@@ -1154,3 +1160,97 @@ hsr_status hsr_subcycle(hsr_ctx *c, const hsr_subcycle_params *sp, int64_t count
 }
 
 }  // extern "C"
+
+static hsr_status check_domain(const char *who, const hsr_domain *d) {
+  if (!d || !(d->rL > 0.f) || !std::isfinite(d->rL) || d->ng < 1 || !(d->oL_in >= 0.f) ||
+      !(d->oL_in <= d->rL)) {
+    g_err = std::string(who) + ": domain needs rL > 0, ng >= 1 and 0 <= oL <= rL";
+    return HSR_ERR_ARG;
+  }
+  return HSR_OK;
+}
+
+hsr_status hsr_domain_init(hsr_domain *d) {
+  hsr_status s = check_domain("hsr_domain_init", d);
+  if (s) return s;
+  domain_derive(d);
+  const int64_t ngt = (int64_t)d->ng + 2 * (int64_t)d->ng_overload + 1;
+  if (d->ng_overload < 0 || ngt > (1 << 24) || ngt * ngt * ngt >= (int64_t)INT32_MAX) {
+    g_err = "hsr_domain_init: ng + 2 ng_overload + 1 must stay below 1291 (cells < 2^31)";
+    return HSR_ERR_ARG;
+  }
+  return HSR_OK;
+}
+
+hsr_status hsr_refresh(hsr_ctx *c, const hsr_domain *dom_in, float anow, int64_t *count,
+                       int64_t capacity, float *x, float *y, float *z, float *vx, float *vy,
+                       float *vz, float *mass, float *phi, int64_t *id, uint16_t *mask,
+                       int64_t *inside, int ptr_flags) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (!count || !inside || !x || !y || !z || !vx || !vy || !vz || !mass || !phi || !id ||
+      !mask) {
+    g_err = "hsr_refresh: every particle array, count and inside are required";
+    return HSR_ERR_ARG;
+  }
+  hsr_domain dom = dom_in ? *dom_in : hsr_domain{};
+  if ((s = hsr_domain_init(&dom))) return s;
+  if (!(anow > 0.f) || !std::isfinite(anow)) {
+    g_err = "hsr_refresh: anow must be positive and finite";
+    return HSR_ERR_ARG;
+  }
+  const int64_t n = *count;
+  if (n < 0 || n >= (int64_t)INT32_MAX || capacity < n) {
+    g_err = "hsr_refresh: need 0 <= count <= capacity and count < 2^31";
+    return HSR_ERR_ARG;
+  }
+  if ((s = sub_events(c))) return s;
+  hipStream_t st = c->stream;
+  float *f[8] = {x, y, z, vx, vy, vz, mass, phi};
+  PArrays in{};
+  const bool host = !(ptr_flags & HSR_DEVICE_PTRS);
+  if (host) {
+    if ((s = stage_in(c, c->s.host_in, n, f, id, mask, in.f, &in.id, &in.mask))) return s;
+  } else {
+    for (int a = 0; a < 8; ++a) in.f[a] = f[a];
+    in.id = id;
+    in.mask = mask;
+  }
+  HSR_HIP(hipEventRecord(c->sub.ev[4], st));
+  int64_t n_out = 0, alive = 0;
+  if ((s = refresh_count(c, &dom, anow, n, in.f[0], in.f[1], in.f[2], &n_out, &alive))) return s;
+  if (n_out > capacity || n_out >= (int64_t)INT32_MAX) {
+    g_err = "hsr_refresh: " + std::to_string(n_out) + " particles after the exchange exceed the capacity " +
+            std::to_string(capacity) + " (or 2^31)";
+    *count = n_out;
+    return HSR_ERR_ARG;
+  }
+  PArrays out{};
+  if (host) {
+    const size_t no = (size_t)std::max<int64_t>(n_out, 1);
+    HSR_HIP(dbuf_reserve(c->sub.xout, (8 + 8 * 4 + 2) * no + 256));
+    char *p = (char *)c->sub.xout.p;
+    out.id = (int64_t *)p;
+    p += sizeof(int64_t) * no;
+    for (int a = 0; a < 8; ++a) {
+      out.f[a] = (float *)p;
+      p += sizeof(float) * no;
+    }
+    out.mask = (uint16_t *)p;
+  } else {
+    out = in;
+  }
+  int64_t ins = 0;
+  if ((s = refresh_finish(c, &dom, anow, n, in, n_out, out, &ins))) return s;
+  HSR_HIP(hipEventRecord(c->sub.ev[5], st));
+  if (host && (s = stage_out(c, n_out, f, id, mask, out.f, out.id, out.mask))) return s;
+  HSR_HIP(hipStreamSynchronize(st));
+  float ms = 0.f;
+  HSR_HIP(hipEventElapsedTime(&ms, c->sub.ev[4], c->sub.ev[5]));
+  c->sub.refresh_ms = ms;
+  c->sub.refresh_alive = alive;
+  c->sub.refresh_dead = n_out - alive;
+  *count = n_out;
+  *inside = ins;
+  return HSR_OK;
+}

@@ -161,8 +161,13 @@ struct SubState {
   DBuf keys;       // cell keys and indices of the sort (16 B per particle)
   DBuf stage;      // gather staging of the ten arrays
   DBuf arr[10];    // x y z vx vy vz mass phi id mask on the device
+  DBuf xcnt;       // refresh: per-tile category counts, their offsets, totals
+  DBuf xstage;     // refresh: the exchanged particles before the resort
+  DBuf xout;       // refresh: device output of host-pointer calls
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   double drift_ms = 0, resort_ms = 0, subcycle_ms = 0, cic_ms = 0, inverse_cic_ms = 0;
+  double refresh_ms = 0;
+  int64_t refresh_alive = 0, refresh_dead = 0;
   float drift_factor = 0.f, kick_coeff = 0.f;  // last map1 / map2 factors
 };
 
@@ -232,6 +237,29 @@ hsr_status drift_run(hsr_ctx *c, int64_t n, float k, float *x, float *y, float *
 hsr_status resort_run(hsr_ctx *c, int64_t n, const int32_t ng[3], float *f[8],
                       int64_t *id, uint16_t *mask, int64_t *inside);
 hsr_status fill_run(hsr_ctx *c, int64_t n, float v, float *a);
+
+// The ten particle arrays (x y z vx vy vz mass phi, id, mask); entries may
+// be null where an operation allows it.
+struct PArrays {
+  float *f[8];
+  int64_t *id;
+  uint16_t *mask;
+};
+// Stable cell sort of n particles from `in` into `out` (distinct arrays).
+hsr_status resort_into(hsr_ctx *c, int64_t n, const int32_t ng[3], const PArrays &in,
+                       const PArrays &out, int64_t *inside);
+
+// One-rank overload refresh, in two halves so the caller can check the
+// output size: refresh_count classifies the n particles (x, y, z of `in`)
+// and returns the refreshed total and the alive count; refresh_finish packs
+// them into staging and resorts them into `out` (room for n_out).
+hsr_status refresh_count(hsr_ctx *c, const hsr_domain *dom, float anow, int64_t n,
+                         const float *x, const float *y, const float *z, int64_t *n_out,
+                         int64_t *alive);
+hsr_status refresh_finish(hsr_ctx *c, const hsr_domain *dom, float anow, int64_t n,
+                          const PArrays &in, int64_t n_out, const PArrays &out,
+                          int64_t *inside);
+void domain_derive(hsr_domain *d);
 hsr_status cic_run(hsr_ctx *c, int64_t n, const float *x, const float *y, const float *z,
                    const int32_t ng[3], float gpscal, float *rho);
 hsr_status inverse_cic_run(hsr_ctx *c, int64_t n, const float *x, const float *y,

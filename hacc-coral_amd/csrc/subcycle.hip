@@ -131,11 +131,13 @@ hsr_status drift_run(hsr_ctx *c, int64_t n, float k, float *x, float *y, float *
   return HSR_OK;
 }
 
-hsr_status resort_run(hsr_ctx *c, int64_t n, const int32_t ng[3], float *f[8],
-                      int64_t *id, uint16_t *mask, int64_t *inside) {
+// Cell keys of n particles and their stable radix sort; leaves the source
+// index of each destination in the returned array and the number of
+// particles outside the grid in *outside (device).
+static hsr_status sort_cells(hsr_ctx *c, int64_t n, const int32_t ng[3], const float *x,
+                             const float *y, const float *z, const int32_t **src,
+                             unsigned long long **outside_dev) {
   hipStream_t st = c->stream;
-  *inside = 0;
-  if (n <= 0) return HSR_OK;
   const uint64_t Ng = (uint64_t)ng[0] * (uint64_t)ng[1] * (uint64_t)ng[2];
   int bits = 1;
   while (bits < 32 && (Ng >> bits) != 0) ++bits;  // keys 0..Ng need this many bits
@@ -148,13 +150,55 @@ hsr_status resort_run(hsr_ctx *c, int64_t n, const int32_t ng[3], float *f[8],
   int32_t *val2 = val + nn;
   unsigned long long *outside = (unsigned long long *)(val2 + nn);  // 16n bytes in: aligned
   k_zero_ull<<<1, 64, 0, st>>>(outside);
-  k_cell_keys<<<nblk(n, 256), 256, 0, st>>>(n, f[0], f[1], f[2], ng[0], ng[1], ng[2],
-                                          (uint32_t)Ng, key, val, outside);
+  k_cell_keys<<<nblk(n, 256), 256, 0, st>>>(n, x, y, z, ng[0], ng[1], ng[2], (uint32_t)Ng,
+                                          key, val, outside);
   HSR_HIP(hipGetLastError());
   size_t tmp = 0;
   HSR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, key, key2, val, val2, (int)n, 0, bits, st));
   HSR_HIP(dbuf_reserve(c->s.cub, tmp));
   HSR_HIP(hipcub::DeviceRadixSort::SortPairs(c->s.cub.p, tmp, key, key2, val, val2, (int)n, 0, bits, st));
+  *src = val2;
+  *outside_dev = outside;
+  return HSR_OK;
+}
+
+hsr_status resort_into(hsr_ctx *c, int64_t n, const int32_t ng[3], const PArrays &in,
+                       const PArrays &out, int64_t *inside) {
+  hipStream_t st = c->stream;
+  *inside = 0;
+  if (n <= 0) return HSR_OK;
+  const int32_t *src = nullptr;
+  unsigned long long *outside = nullptr;
+  hsr_status s = sort_cells(c, n, ng, in.f[0], in.f[1], in.f[2], &src, &outside);
+  if (s) return s;
+  Ten ti{}, to{};
+  for (int a = 0; a < 8; ++a) {
+    ti.f[a] = in.f[a];
+    to.f[a] = out.f[a];
+  }
+  ti.id = in.id;
+  to.id = out.id;
+  ti.mask = in.mask;
+  to.mask = out.mask;
+  k_gather_ten<<<nblk(n, 256), 256, 0, st>>>(n, src, ti, to);
+  HSR_HIP(hipGetLastError());
+  unsigned long long h_out = 0;
+  HSR_HIP(hipMemcpyAsync(&h_out, outside, sizeof(h_out), hipMemcpyDeviceToHost, st));
+  HSR_HIP(hipStreamSynchronize(st));
+  *inside = n - (int64_t)h_out;
+  return HSR_OK;
+}
+
+hsr_status resort_run(hsr_ctx *c, int64_t n, const int32_t ng[3], float *f[8],
+                      int64_t *id, uint16_t *mask, int64_t *inside) {
+  hipStream_t st = c->stream;
+  *inside = 0;
+  if (n <= 0) return HSR_OK;
+  const size_t nn = (size_t)n;
+  const int32_t *val2 = nullptr;
+  unsigned long long *outside = nullptr;
+  hsr_status s = sort_cells(c, n, ng, f[0], f[1], f[2], &val2, &outside);
+  if (s) return s;
   // gather into staging, then copy back in place
   size_t bytes = 0;
   for (int a = 0; a < 8; ++a)

@@ -29,12 +29,20 @@ EXPORTS = [
     "hsr_get_lists", "hsr_build_many", "hsr_lists_many", "hsr_set_tuning",
     "hsr_get_tuning", "hsr_forces_f64", "hsr_leaf_momentum", "hsr_drift",
     "hsr_resort", "hsr_map2", "hsr_subcycle", "hsr_get_pseudo_masses", "hsr_cic",
-    "hsr_inverse_cic",
+    "hsr_inverse_cic", "hsr_domain_init", "hsr_refresh",
 ]
 
 
 class HsrError(RuntimeError):
     pass
+
+
+class CapacityError(HsrError):
+    """hsr_refresh needs room for `required` particles."""
+
+    def __init__(self, msg, required):
+        super().__init__(msg)
+        self.required = required
 
 
 class Params(C.Structure):
@@ -71,7 +79,19 @@ class Stats(C.Structure):
                 ("evaluated_pairs", C.c_int64), ("drift_ms", C.c_double),
                 ("resort_ms", C.c_double), ("subcycle_ms", C.c_double),
                 ("drift_factor", C.c_double), ("kick_coeff", C.c_double),
-                ("cic_ms", C.c_double), ("inverse_cic_ms", C.c_double)]
+                ("cic_ms", C.c_double), ("inverse_cic_ms", C.c_double),
+                ("refresh_ms", C.c_double), ("refresh_alive", C.c_int64),
+                ("refresh_dead", C.c_int64)]
+
+
+class Domain(C.Structure):
+    """hsr_domain: a one-rank Domain (Domain.cxx:34-84) from the deck's rL,
+    oL and ng; the other fields are derived by hsr_domain_init."""
+    _fields_ = [("rL", C.c_float), ("oL_in", C.c_float), ("ng", C.c_int32),
+                ("ng_overload", C.c_int32), ("ng_local_total", C.c_int32),
+                ("oL", C.c_float), ("phys2grid_pos", C.c_float),
+                ("grid2phys_pos", C.c_float), ("phys2grid_vel", C.c_float),
+                ("grid2phys_vel", C.c_float)]
 
 
 class SubcycleParams(C.Structure):
@@ -134,6 +154,9 @@ def load(path: str | None = None):
     L.hsr_cic.argtypes = [vp, i64, vp, vp, vp, vp, C.c_float, vp, C.c_int]
     L.hsr_inverse_cic.argtypes = [vp, i64, vp, vp, vp, vp, vp, C.c_float, C.c_float, vp,
                                   C.c_int]
+    L.hsr_domain_init.argtypes = [C.POINTER(Domain)]
+    L.hsr_refresh.argtypes = [vp, C.POINTER(Domain), C.c_float, C.POINTER(C.c_int64), i64] + \
+        [vp] * 10 + [vp, C.c_int]
     L.hsr_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.hsr_get_nodes.argtypes = [vp, vp, i64]
     L.hsr_get_pseudo_masses.argtypes = [vp, vp, i64]
@@ -340,6 +363,25 @@ class Context:
         self.n = int(inside.value) if inside.value >= 0 else self.n
         return int(inside.value)
 
+    def refresh(self, dom: "Domain", anow, count, x, y, z, vx, vy, vz, mass, phi, ids, mask,
+                device_ptrs=False):
+        """The one-rank overload refresh (MC3Extras::refreshParticles after
+        copyAliveIntoVectors) in place on arrays of capacity len(x) holding
+        `count` particles; returns (refreshed count, m_Np_last)."""
+        arrs = (x, y, z, vx, vy, vz, mass, phi, ids, mask)
+        cap = int(x.shape[0])
+        if any(a is None or int(a.shape[0]) != cap for a in arrs):
+            raise HsrError("refresh: all ten arrays are required, with one length (the capacity)")
+        n = C.c_int64(int(count))
+        inside = C.c_int64(0)
+        fl = HSR_DEVICE_PTRS if device_ptrs else HSR_HOST_PTRS
+        st = self.L.hsr_refresh(self.h, C.byref(dom), C.c_float(anow), C.byref(n), cap,
+                                *(_ptr(a) for a in arrs), C.byref(inside), fl)
+        if st == HSR_ERR_ARG and n.value > cap:
+            raise CapacityError(f"hsr_refresh: {self.L.hsr_last_error().decode()}", int(n.value))
+        check(st, "hsr_refresh")
+        return int(n.value), int(inside.value)
+
     # -- PM particle-grid steps
     def cic(self, x, y, z, ng, gpscal=1.0, rho=None, device_ptrs=False):
         """Particles::cic: rho of ng0*ng1*ng2 + 1 floats (the last slot
@@ -411,3 +453,10 @@ class Context:
         for k, v in kw.items():
             setattr(t, k, int(v))
         check(self.L.hsr_set_tuning(self.h, C.byref(t)), "hsr_set_tuning")
+
+
+def domain(rL, oL, ng) -> Domain:
+    """hsr_domain_init: Domain::initialize for one rank (no device needed)."""
+    d = Domain(rL=float(rL), oL_in=float(oL), ng=int(ng))
+    check(load().hsr_domain_init(C.byref(d)), "hsr_domain_init")
+    return d

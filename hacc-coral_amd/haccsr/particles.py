@@ -10,6 +10,7 @@ arrays (host memory, staged by the library) or float32/int64/int16 device
 tensors (used in place).  ``subCycle`` is one hsr_subcycle call that keeps the
 particles on the device for all nsub sub-steps; ``map1`` and ``map2`` are the
 single steps (hsr_drift, hsr_map2), one C ABI call each.
+``refreshParticles`` is the one-rank overload refresh (hsr_refresh).
 """
 from __future__ import annotations
 
@@ -17,7 +18,7 @@ import dataclasses
 
 import numpy as np
 
-from ._lib import Context, SubcycleParams
+from ._lib import CapacityError, Context, SubcycleParams, domain
 
 @dataclasses.dataclass
 class TimeStepperValues:
@@ -113,3 +114,42 @@ class Particles:
                 self.map1(ts.pp, np.float32(sf * ts.tau2), ts.adot)
             if after_step is not None:
                 after_step(step)
+
+    # MC3Extras::particleExtras' refresh on one rank (MC3Extras.cxx:330-338,
+    # 660-700): copyAliveIntoVectors, ParticleExchange, copyAllFromVectors,
+    # resortParticles
+    def refreshParticles(self, rL: float, oL: float, ng: int, anow: float,
+                         capacity: int | None = None) -> int:
+        """Refresh the overload copies for the domain of the deck's rL, oL and
+        ng.  The particle arrays are replaced by arrays of the refreshed
+        length (views of new buffers of `capacity`, grown if the exchange
+        needs more); returns the new particle count."""
+        dom = domain(rL, oL, ng)
+        if any(g != dom.ng_local_total for g in self.ng):
+            raise ValueError(f"refreshParticles: the grid {self.ng} is not the domain's "
+                             f"ng_local_total {dom.ng_local_total}")
+        n = self.Np_local_total
+        cap = max(int(capacity) if capacity else n + n // 4 + 1024, n)
+        for attempt in range(2):
+            bufs = {k: self._grown(k, cap) for k in self.ARRAYS}
+            try:
+                count, inside = self.ctx.refresh(dom, anow, n, *(bufs[k] for k in self.ARRAYS),
+                                                 device_ptrs=self.device_ptrs)
+                break
+            except CapacityError as e:
+                if attempt:
+                    raise
+                cap = e.required
+        self.arrays = {k: b[:count] for k, b in bufs.items()}
+        self.Np_last = inside
+        return count
+
+    def _grown(self, name: str, cap: int):
+        a = self.arrays[name]
+        n = int(a.shape[0])
+        if isinstance(a, np.ndarray):
+            b = np.empty(cap, a.dtype)
+        else:
+            b = a.new_empty(cap)
+        b[:n] = a
+        return b

@@ -91,6 +91,26 @@ void ParticleSubCycle::resortParticles() {
   m_Np_last = inside;
 }
 
+ID_T ParticleSubCycle::refreshParticles(float rL, float oL, int ng, float anow,
+                                        ID_T capacity) {
+  hsr_domain dom{};
+  dom.rL = rL;
+  dom.oL_in = oL;
+  dom.ng = ng;
+  check(hsr_domain_init(&dom), "hsr_domain_init");
+  for (int k = 0; k < DIMENSION; ++k)
+    if (m_ng[k] != dom.ng_local_total)
+      throw std::runtime_error("refreshParticles: the grid is not the domain's ng_local_total (" +
+                               std::to_string(dom.ng_local_total) + ")");
+  int64_t count = m_Np, inside = 0;
+  check(hsr_refresh(m_ctx, &dom, anow, &count, capacity, m_x, m_y, m_z, m_vx, m_vy, m_vz,
+                    m_mass, m_phi, m_id, m_mask, &inside, m_flags),
+        "hsr_refresh");
+  m_Np = count;
+  m_Np_last = inside;
+  return m_Np;
+}
+
 void ParticleSubCycle::map2(double fscal, double tau, double stepFraction) {
   const hsr_subcycle_params sp = params(0.0, tau, 0.0, 0.0, fscal);
   int64_t inside = 0;

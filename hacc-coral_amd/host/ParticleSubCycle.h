@@ -2,8 +2,10 @@
 // class around the short-range kick (Particles.h:160-175): map1 (drift,
 // Particles.cxx:732-758), resortParticles (Particles.cxx:402-488), map2 with
 // the RCB monopole tree (Particles.cxx:1205-1280), subCycle
-// (Particles.cxx:1176-1201), and the PM particle-grid steps cic and
-// inverse_cic (Particles.cxx:589-725), over the C ABI of libhaccsr.so.  The ten particle arrays stay with the caller (host
+// (Particles.cxx:1176-1201), the PM particle-grid steps cic and
+// inverse_cic (Particles.cxx:589-725) and the one-rank particle refresh
+// (MC3Extras.cxx:660-700), over the C ABI of libhaccsr.so.  The ten particle
+// arrays stay with the caller (host
 // or device memory); subCycle is one hsr_subcycle call, so the particles stay
 // on the device for all nsub sub-steps.  The TimeStepper is any type with the
 // reference's accessors pp(), tau(), tau2(), adot() and fscal()
@@ -62,8 +64,18 @@ class ParticleSubCycle {
     m_skipStreamQ = skipStream;
     m_skipKickSRQ = skipKickSR;
   }
+  // The particle refresh of MC3Extras::particleExtras on one rank
+  // (MC3Extras.cxx:330-338, 660-700) for the domain of the deck's rL, oL and
+  // ng (Domain::initialize): alive particles kept in order, the overload
+  // copies of ParticleExchange appended (every neighbour is this rank), unit
+  // masses, grid units, resortParticles.  The arrays given to setParticles
+  // must have room for `capacity` particles; the grid of this object must be
+  // the domain's ng_local_total.  Returns the new particle count, which the
+  // object keeps (m_Np_local_total).
+  ID_T refreshParticles(float rL, float oL, int ng, float anow, ID_T capacity);
   // m_Np_last: particles inside the grid after the last resort
   ID_T Np_last() const { return m_Np_last; }
+  ID_T Np() const { return m_Np; }
 
  private:
   hsr_subcycle_params params(double pp, double tau, double tau2, double adot,

@@ -26,6 +26,10 @@
  *       = hsr_subcycle
  *   Particles::cic / inverse_cic              Particles.cxx:589-714
  *       = hsr_cic / hsr_inverse_cic
+ *   MC3Extras refresh: Particles::copyAliveIntoVectors, ParticleExchange,
+ *   copyAllFromVectors, resortParticles       MC3Extras.cxx:330-338,660-700,
+ *                                             ParticleExchange.cxx:488-728
+ *       = hsr_domain_init + hsr_refresh (one-rank decomposition)
  */
 #ifndef HACC_SR_H
 #define HACC_SR_H
@@ -160,6 +164,9 @@ typedef struct {
   double kick_coeff;      /* c of the last map2 kick (a float) */
   double cic_ms;          /* device time of the last hsr_cic */
   double inverse_cic_ms;  /* device time of the last hsr_inverse_cic */
+  double refresh_ms;      /* device time of the last hsr_refresh */
+  int64_t refresh_alive;  /* particles it kept alive */
+  int64_t refresh_dead;   /* overload copies it appended */
 } hsr_stats;
 
 hsr_status hsr_get_stats(hsr_ctx *ctx, hsr_stats *st);
@@ -287,6 +294,56 @@ hsr_status hsr_inverse_cic(hsr_ctx *ctx, int64_t count, const float *x,
                            const float *y, const float *z, const int32_t ng[3],
                            const float *grid, float tau, float fscal, float *v,
                            int ptr_flags);
+
+/* ------------------------------------------------------ overload refresh */
+/* The rank's domain (Domain::initialize, Domain.cxx:34-84) for a one-rank
+ * decomposition.  The caller sets rL, oL_in and ng (the input deck's rL, oL
+ * and ng); hsr_domain_init derives the rest with the reference's float and
+ * double operations: phys2grid_pos = ng/rL, grid2phys_pos = rL/ng,
+ * phys2grid_vel = ng/(100 rL), grid2phys_vel = 100 rL/ng, ng_overload =
+ * ceilf(oL_in * phys2grid_pos), oL = ng_overload * grid2phys_pos and
+ * ng_local_total = ng + 2 ng_overload + 1 (the grid of hsr_resort, hsr_map2
+ * and hsr_subcycle). */
+typedef struct {
+  float rL;              /* box side, physical units */
+  float oL_in;           /* requested overload width, physical units */
+  int32_t ng;            /* grid points per side */
+  int32_t ng_overload;   /* derived */
+  int32_t ng_local_total;/* derived */
+  float oL;              /* derived: overload width actually used */
+  float phys2grid_pos, grid2phys_pos, phys2grid_vel, grid2phys_vel; /* derived */
+} hsr_domain;
+
+hsr_status hsr_domain_init(hsr_domain *dom);
+
+/* The particle refresh of MC3Extras::particleExtras (MC3Extras.cxx:330-338,
+ * 660-700) on one rank, with the particles on the device throughout:
+ *  1. Particles::copyAliveIntoVectors (Particles.cxx:945-1030): local grid
+ *     units to physical ((x + corner) * grid2phys_pos, v * grid2phys_vel /
+ *     anow / anow, corner = -ng_overload) and keep, in order, the particles
+ *     with every coordinate in [0, rL);
+ *  2. ParticleExchange::exchangeParticles (ParticleExchange.cxx:488-728)
+ *     with every neighbour this rank: a kept particle outside the open
+ *     interior (oL, rL - oL)^3 is copied, for each of the 26 neighbours in
+ *     the order of BasicDefinition.h:224-264 whose closed region holds it,
+ *     with its position shifted by -rL, 0 or +rL per dimension; copies are
+ *     appended neighbour by neighbour, in particle order within one;
+ *  3. Particles::copyAllFromVectors (Particles.cxx:298-366): every mass set
+ *     to 1, back to local grid units (x * phys2grid_pos - corner,
+ *     v * phys2grid_vel * anow * anow);
+ *  4. Particles::resortParticles on the ng_local_total grid (as hsr_resort).
+ * The ten arrays hold *count particles on entry and have room for capacity;
+ * on return *count is the refreshed number and *inside the particles inside
+ * the grid.  When the refreshed number exceeds capacity the call fails with
+ * HSR_ERR_ARG, leaves the arrays as they were and writes the number it needs
+ * to *count.  With more ranks step 2 sends the 26
+ * packed sets to the neighbour ranks; that transport is not part of this
+ * library. */
+hsr_status hsr_refresh(hsr_ctx *ctx, const hsr_domain *dom, float anow,
+                       int64_t *count, int64_t capacity, float *x, float *y,
+                       float *z, float *vx, float *vy, float *vz, float *mass,
+                       float *phi, int64_t *id, uint16_t *mask,
+                       int64_t *inside, int ptr_flags);
 
 /* ------------------------------------------------------------- tuning */
 typedef struct {

@@ -1121,3 +1121,120 @@ void hsro_force_f64d(const hsro_params *p, const hsro_node *t,
     }
   }
 }
+
+
+/* ------------------------------------------------------- overload refresh */
+
+/* Domain.cxx:42-54, 65-69 with decomposition 1 and position 0. */
+void hsro_domain_init(hsro_domain *d) {
+  d->phys2grid_pos = d->ng / d->rL;
+  d->grid2phys_pos = d->rL / d->ng;
+  d->phys2grid_vel = d->ng / (100.0 * d->rL);
+  d->grid2phys_vel = 100.0 * d->rL / d->ng;
+  d->ng_overload = (int)ceilf(d->oL_in * d->phys2grid_pos);
+  d->oL = d->ng_overload * d->grid2phys_pos;
+  d->ng_local_total = d->ng + 2 * d->ng_overload + 1;
+}
+
+/* The enum NEIGHBOR of BasicDefinition.h:224-264, by name; a name's parts
+ * say which face of which dimension the neighbour lies across. */
+static const char *const kNeighborNames[26] = {
+    "X0",       "X1",       "Y0",       "Y1",       "Z0",       "Z1",
+    "X0_Y0",    "X1_Y1",    "X0_Y1",    "X1_Y0",    "Y0_Z0",    "Y1_Z1",
+    "Y0_Z1",    "Y1_Z0",    "Z0_X0",    "Z1_X1",    "Z0_X1",    "Z1_X0",
+    "X0_Y0_Z0", "X1_Y1_Z1", "X0_Y0_Z1", "X1_Y1_Z0", "X0_Y1_Z0", "X1_Y0_Z1",
+    "X0_Y1_Z1", "X1_Y0_Z0"};
+
+/* side[d] = 0 (low face), 1 (high face) or -1 (dimension not named). */
+static void neighbor_sides(int n, int side[3]) {
+  side[0] = side[1] = side[2] = -1;
+  for (const char *c = kNeighborNames[n]; *c; ++c)
+    if (*c == 'X' || *c == 'Y' || *c == 'Z') side[*c - 'X'] = c[1] - '0';
+}
+
+int64_t hsro_refresh(const hsro_domain *d, float anow, int64_t n, int64_t cap,
+                     float *x, float *y, float *z, float *vx, float *vy, float *vz,
+                     float *mass, float *phi, int64_t *id, uint16_t *mask,
+                     int64_t *alive) {
+  float *pos[3] = {x, y, z}, *vel[3] = {vx, vy, vz};
+  const int corner_alive = 0 * d->ng;               /* Domain.cxx:74 */
+  const int corner = corner_alive - d->ng_overload;  /* Domain.cxx:75 */
+  /* Particles::coords_local2global (Particles.cxx:798-830) */
+  float pre_vel = d->grid2phys_vel / anow / anow;
+  for (int64_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      pos[k][i] += corner;
+      pos[k][i] *= d->grid2phys_pos;
+      vel[k][i] *= pre_vel;
+    }
+  /* copyAliveIntoVectors (Particles.cxx:959-1030): keep [ca, ca + rla) */
+  float ca = corner_alive * d->grid2phys_pos;
+  float rla = d->rL / 1;
+  float lo = ca, hi = ca + rla;
+  int64_t na = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const int aa = (x[i] >= lo) * (x[i] < hi) * (y[i] >= lo) * (y[i] < hi) * (z[i] >= lo) *
+                   (z[i] < hi);
+    if (!aa) continue;
+    x[na] = x[i]; y[na] = y[i]; z[na] = z[i];
+    vx[na] = vx[i]; vy[na] = vy[i]; vz[na] = vz[i];
+    phi[na] = phi[i]; id[na] = id[i]; mask[na] = mask[i];
+    mass[na] = 1.0f; /* MC3Extras::refreshParticles pushes 1.0 per particle */
+    ++na;
+  }
+  *alive = na;
+  /* ParticleExchange::setParameters / initialize (ParticleExchange.cxx:103,
+   * 243-274) with boxSize rL and deadSize oL */
+  float box = d->rL, dead = d->oL;
+  float step = box / 1;
+  float minShare = 0 * step, maxShare = minShare + step;
+  if (maxShare > box) maxShare = box;
+  float minMine = minShare + dead, maxMine = maxShare - dead;
+  /* calculateExchangeRegions / calculateOffsetFactor: a named low face
+   * takes [minShare, minMine] and is shifted by +box, a named high face
+   * [maxMine, maxShare] and -box, an unnamed dimension the whole share */
+  float rmin[26][3], rmax[26][3], off[26][3];
+  for (int nb = 0; nb < 26; ++nb) {
+    int side[3];
+    neighbor_sides(nb, side);
+    for (int k = 0; k < 3; ++k) {
+      int factor = side[k] == 0 ? 1 : (side[k] == 1 ? -1 : 0);
+      rmin[nb][k] = side[k] == 1 ? maxMine : minShare;
+      rmax[nb][k] = side[k] == 0 ? minMine : maxShare;
+      off[nb][k] = factor * box;
+    }
+  }
+  /* identifyExchangeParticles (ParticleExchange.cxx:542-571), then the
+   * exchanges in pairs (n, n+1), (n+1, n), each appending the particles
+   * listed for its send side (ParticleExchange.cxx:620-625, 676-695) */
+  int64_t total = na;
+  for (int nb = 0; nb < 26; ++nb) {
+    for (int64_t i = 0; i < na; ++i) {
+      if (x[i] > minMine && x[i] < maxMine && y[i] > minMine && y[i] < maxMine &&
+          z[i] > minMine && z[i] < maxMine)
+        continue;
+      if (!(x[i] >= rmin[nb][0] && x[i] <= rmax[nb][0] && y[i] >= rmin[nb][1] &&
+            y[i] <= rmax[nb][1] && z[i] >= rmin[nb][2] && z[i] <= rmax[nb][2]))
+        continue;
+      if (total >= cap) return -1;
+      x[total] = x[i] + off[nb][0];
+      y[total] = y[i] + off[nb][1];
+      z[total] = z[i] + off[nb][2];
+      vx[total] = vx[i]; vy[total] = vy[i]; vz[total] = vz[i];
+      mass[total] = mass[i]; phi[total] = phi[i]; id[total] = id[i]; mask[total] = mask[i];
+      ++total;
+    }
+  }
+  /* copyAllFromVectors (Particles.cxx:309-364): unit masses, then
+   * coords_global2local (Particles.cxx:762-796) */
+  float pre2 = d->phys2grid_vel * anow * anow;
+  for (int64_t i = 0; i < total; ++i) {
+    mass[i] = 1.0f;
+    for (int k = 0; k < 3; ++k) {
+      pos[k][i] *= d->phys2grid_pos;
+      pos[k][i] -= corner;
+      vel[k][i] *= pre2;
+    }
+  }
+  return total;
+}

@@ -18,6 +18,11 @@
  *   hsro_map1_factor, hsro_drift   Particles.cxx:732-758 (map1)
  *   hsro_map2_coeff      Particles.cxx:1230-1233 (map2)
  *   hsro_cic, hsro_inverse_cic     Particles.cxx:589-714
+ *   hsro_domain_init     Domain.cxx:34-84 (one rank)
+ *   hsro_refresh         MC3Extras.cxx:330-338, 660-700 with
+ *                        Particles.cxx:298-366, 762-830, 945-1030 and
+ *                        ParticleExchange.cxx:103-728 (src/simulation,
+ *                        src/cpu, src/halo_finder)
  *
  * Parity pins (see tests/test_oracle_*.py):
  *   cm            bit-exact against the reference's own BGQCM.c compiled
@@ -37,6 +42,11 @@
  *                 point deposits into that cell alone.
  *   map1, map2    closed forms (prefactor 1 exactly, double-precision
  *                 formula to float rounding, two half drifts vs one).
+ *   domain        known answers from exact rational arithmetic.
+ *   refresh       the copies are exactly the periodic images k*rL,
+ *                 k in {-1,0,1}^3 \ 0, of the alive particles whose
+ *                 shift is allowed by the closed overload shell, enumerated
+ *                 by brute force; neighbour order from the enum's names.
  */
 #ifndef HSR_ORACLE_H
 #define HSR_ORACLE_H
@@ -224,6 +234,24 @@ void hsro_inverse_cic(int64_t np, const float *x, const float *y, const float *z
 /* Particles::map2's kick coefficient (Particles.cxx:1230-1233). */
 float hsro_map2_coeff(float gpscal, double fscal, double tau,
                       double step_fraction);
+
+/* Domain::initialize for a one-rank decomposition (Domain.cxx:34-84). */
+typedef struct {
+  float rL, oL_in;
+  int32_t ng, ng_overload, ng_local_total;
+  float oL, phys2grid_pos, grid2phys_pos, phys2grid_vel, grid2phys_vel;
+} hsro_domain;
+void hsro_domain_init(hsro_domain *d);
+
+/* The refresh of MC3Extras::particleExtras on one rank, before its
+ * resortParticles: copyAliveIntoVectors, ParticleExchange with every
+ * neighbour this rank, copyAllFromVectors.  The ten arrays hold n particles
+ * and have room for cap; returns the refreshed count (-1 if it exceeds cap,
+ * the arrays then hold an unspecified mix) and the alive count in *alive. */
+int64_t hsro_refresh(const hsro_domain *d, float anow, int64_t n, int64_t cap,
+                     float *x, float *y, float *z, float *vx, float *vy, float *vz,
+                     float *mass, float *phi, int64_t *id, uint16_t *mask,
+                     int64_t *alive);
 
 #ifdef __cplusplus
 }

@@ -38,6 +38,14 @@ NODE_DTYPE = np.dtype([("count", "<i8"), ("offset", "<i8"), ("cl", "<i8"),
 assert NODE_DTYPE.itemsize == C.sizeof(Node)
 
 
+class Domain(C.Structure):
+    _fields_ = [("rL", C.c_float), ("oL_in", C.c_float), ("ng", C.c_int32),
+                ("ng_overload", C.c_int32), ("ng_local_total", C.c_int32),
+                ("oL", C.c_float), ("phys2grid_pos", C.c_float),
+                ("grid2phys_pos", C.c_float), ("phys2grid_vel", C.c_float),
+                ("grid2phys_vel", C.c_float)]
+
+
 class Params(C.Structure):
     _fields_ = [("nd", C.c_int64), ("fsrrmax", C.c_float), ("rsm", C.c_float),
                 ("tan_oa", C.c_float), ("ppc", C.c_float),
@@ -103,6 +111,10 @@ def lib():
         L.hsro_drift.argtypes = [C.c_int64] + [f32] * 6 + [C.c_float]
         L.hsro_map2_coeff.argtypes = [C.c_float, C.c_double, C.c_double, C.c_double]
         L.hsro_map2_coeff.restype = C.c_float
+        L.hsro_domain_init.argtypes = [C.POINTER(Domain)]
+        L.hsro_refresh.argtypes = [C.POINTER(Domain), C.c_float, C.c_int64, C.c_int64] + \
+            [f32] * 8 + [i64, u16, C.POINTER(C.c_int64)]
+        L.hsro_refresh.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -407,6 +419,35 @@ def resort(state: dict, ng):
         b[perm] = a
         out[k] = b
     return out, int(inside)
+
+
+def domain(rL, oL, ng) -> Domain:
+    """Domain::initialize for one rank (Domain.cxx:34-84)."""
+    d = Domain(rL=float(rL), oL_in=float(oL), ng=int(ng))
+    lib().hsro_domain_init(C.byref(d))
+    return d
+
+
+def refresh(state: dict, count: int, rL, oL, ng, anow, sort=True):
+    """MC3Extras' particle refresh on one rank (copyAliveIntoVectors,
+    ParticleExchange, copyAllFromVectors, then resortParticles unless sort
+    is False) on copies of the ten arrays, whose length is the capacity.
+    Returns (state, count, m_Np_last, alive); m_Np_last is -1 unsorted."""
+    d = domain(rL, oL, ng)
+    st = {a: np.ascontiguousarray(b).copy() for a, b in state.items()}
+    cap = st["x"].shape[0]
+    alive = C.c_int64(0)
+    n = lib().hsro_refresh(C.byref(d), C.c_float(anow), int(count), cap,
+                           *(st[a] for a in POSVEL), st["id"], st["mask"], C.byref(alive))
+    if n < 0:
+        raise ValueError("refreshed particles exceed the capacity")
+    if not sort:
+        return st, int(n), -1, int(alive.value)
+    sub = {a: b[:n].copy() for a, b in st.items()}
+    sub, inside = resort(sub, [d.ng_local_total] * 3)
+    for a in st:
+        st[a][:n] = sub[a]
+    return st, int(n), inside, int(alive.value)
 
 
 def substep(state: dict, sp: dict, tp_kw: dict):

@@ -1,8 +1,13 @@
 // GPU test driver of the C++ sub-cycle drop-in (ParticleSubCycle): the
 // reference's caller pattern, Particles::subCycle(gts) with a TimeStepper
-// exposing pp(), tau(), tau2(), adot(), fscal().  Writes the initial and the
-// final ten arrays to a raw file:
-//   test_subcycle <n> <ng> <nd> <seed> <mode: 0 subCycle, 1 map1/map2 steps> <out.bin>
+// exposing pp(), tau(), tau2(), adot(), fscal(); and the one-rank particle
+// refresh, refreshParticles.  Writes the initial and the final ten arrays to
+// a raw file:
+//   test_subcycle <n> <ng> <nd> <seed> <mode> <out.bin>
+// mode 0: subCycle, 1: map1/map2 steps (n particles on an ng^3 grid);
+// mode 2: refreshParticles(rL = ng, oL = 2.5, ng, anow = 0.8) on arrays with
+// room for 3n particles (the grid is the domain's ng_local_total), the final
+// arrays written at that capacity and the new count printed.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -32,12 +37,19 @@ int main(int argc, char **argv) {
   const long nd = atol(argv[3]);
   uint64_t s = (uint64_t)atoll(argv[4]);
   const int mode = atoi(argv[5]);
+  hsr_domain dom{};
+  dom.rL = (float)g;
+  dom.oL_in = 2.5f;
+  dom.ng = g;
+  if (mode == 2 && hsr_domain_init(&dom) != HSR_OK) return 1;
+  const int gl = mode == 2 ? dom.ng_local_total : g;  // the grid of the particles
+  const long cap = mode == 2 ? 3 * n : n;
   std::vector<float> f[8];
-  for (auto &a : f) a.resize(n);
-  std::vector<ID_T> id(n);
-  std::vector<MASK_T> mask(n);
+  for (auto &a : f) a.resize(cap);
+  std::vector<ID_T> id(cap);
+  std::vector<MASK_T> mask(cap);
   for (long i = 0; i < n; ++i) {
-    for (int d = 0; d < 3; ++d) f[d][i] = (float)(u01(s) * (g + 2.0) - 1.0);
+    for (int d = 0; d < 3; ++d) f[d][i] = (float)(u01(s) * (gl + 2.0) - 1.0);
     for (int d = 3; d < 6; ++d) f[d][i] = (float)(8.0 * (u01(s) - 0.5));
     f[6][i] = (float)u01(s);  // phi
     f[7][i] = 0.5f;           // mass (map2 resets it to 1)
@@ -49,19 +61,21 @@ int main(int argc, char **argv) {
   for (auto &a : f) fwrite(a.data(), 4, n, out);
   fwrite(id.data(), 8, n, out);
   fwrite(mask.data(), 2, n, out);
-  const int ng[3] = {g, g, g};
+  const int ng[3] = {gl, gl, gl};
   Stepper ts;
   ParticleSubCycle P(ng, 1.0f, 1.0f, 2.0f, 5, 3.12f, 0.1f, 0.1f, nd);
   P.setParticles(n, f[0].data(), f[1].data(), f[2].data(), f[3].data(), f[4].data(),
                  f[5].data(), f[6].data(), f[7].data(), id.data(), mask.data());
   if (mode == 0)
     P.subCycle(&ts);
-  else
+  else if (mode == 1)
     P.subCycleSteps(ts.pp(), ts.tau(), ts.tau2(), ts.adot(), ts.fscal());
-  for (auto &a : f) fwrite(a.data(), 4, n, out);
-  fwrite(id.data(), 8, n, out);
-  fwrite(mask.data(), 2, n, out);
+  else
+    P.refreshParticles(dom.rL, dom.oL_in, dom.ng, 0.8f, cap);
+  for (auto &a : f) fwrite(a.data(), 4, cap, out);
+  fwrite(id.data(), 8, cap, out);
+  fwrite(mask.data(), 2, cap, out);
   fclose(out);
-  printf("subcycle ok n=%ld Np_last=%ld\n", n, (long)P.Np_last());
+  printf("subcycle ok n=%ld count=%ld Np_last=%ld\n", n, (long)P.Np(), (long)P.Np_last());
   return 0;
 }

@@ -58,6 +58,8 @@ int main(void) {
   printf("%zu %zu %zu %zu %zu\n", sizeof(hsr_subcycle_params), offsetof(hsr_subcycle_params, ng),
          offsetof(hsr_subcycle_params, pp), offsetof(hsr_subcycle_params, fscal),
          offsetof(hsr_stats, kick_coeff));
+  printf("%zu %zu %zu %zu\n", sizeof(hsr_domain), offsetof(hsr_domain, ng_local_total),
+         offsetof(hsr_domain, grid2phys_vel), offsetof(hsr_stats, refresh_dead));
   return 0;
 }
 """)
@@ -70,7 +72,9 @@ int main(void) {
             _lib.Params.ppc.offset, _lib.Node.xc.offset, _lib.Stats.build_ms.offset,
             _lib.Tuning.use_graph.offset, C.sizeof(_lib.SubcycleParams),
             _lib.SubcycleParams.ng.offset, _lib.SubcycleParams.pp.offset,
-            _lib.SubcycleParams.fscal.offset, _lib.Stats.kick_coeff.offset]
+            _lib.SubcycleParams.fscal.offset, _lib.Stats.kick_coeff.offset,
+            C.sizeof(_lib.Domain), _lib.Domain.ng_local_total.offset,
+            _lib.Domain.grid2phys_vel.offset, _lib.Stats.refresh_dead.offset]
     assert got == want
     assert _lib.NODE_DTYPE.itemsize == C.sizeof(_lib.Node)
 
@@ -100,3 +104,21 @@ def test_no_device_is_an_error_not_a_fallback():
         pass
     with pytest.raises(_lib.HsrError, match="no HIP device"):
         _lib.Context()
+
+
+def test_domain_init_matches_the_oracle_and_validates():
+    """hsr_domain_init (host arithmetic, no device) derives the one-rank
+    Domain exactly as the oracle's restatement of Domain::initialize."""
+    from oracle import oracle as O
+    fields = ["ng_overload", "ng_local_total", "oL", "phys2grid_pos", "grid2phys_pos",
+              "phys2grid_vel", "grid2phys_vel"]
+    for rL, oL, ng in [(256, 3.2, 256), (1000, 10, 512), (100, 8, 64), (64, 0, 64),
+                       (3.7, 0.3, 17), (512, 12.5, 1024), (1.0, 1.0, 3)]:
+        a, b = _lib.domain(rL, oL, ng), O.domain(rL, oL, ng)
+        for f in fields:
+            assert getattr(a, f) == getattr(b, f), (rL, oL, ng, f)
+    L = _lib.load()
+    for rL, oL, ng in [(0, 1, 64), (64, -1, 64), (64, 65, 64), (64, 1, 0),
+                       (float("nan"), 1, 64), (64, float("nan"), 64), (2000, 2, 2000)]:
+        d = _lib.Domain(rL=rL, oL_in=oL, ng=ng)
+        assert L.hsr_domain_init(C.byref(d)) == _lib.HSR_ERR_ARG, (rL, oL, ng)
