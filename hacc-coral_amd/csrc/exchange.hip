@@ -112,13 +112,23 @@ k_xchg_count(int64_t n, int64_t ntiles, const float *__restrict__ x,
   int c[NCAT];
 #pragma unroll
   for (int k = 0; k < NCAT; ++k) c[k] = 0;
+  // all of the tile's positions in flight at once (one memory round trip)
+  float px[XITEMS], py[XITEMS], pz[XITEMS];
+#pragma unroll
+  for (int it = 0; it < XITEMS; ++it) {
+    const int64_t i = w * XTILE + it * 64 + l;
+    px[it] = i < n ? x[i] : 0.f;
+    py[it] = i < n ? y[i] : 0.f;
+    pz[it] = i < n ? z[i] : 0.f;
+  }
+#pragma unroll
   for (int it = 0; it < XITEMS; ++it) {
     const int64_t i = w * XTILE + it * 64 + l;
     bool al = false;
     uint32_t bits = 0;
     if (i < n) {
       float gx, gy, gz;
-      to_phys(T, x[i], y[i], z[i], gx, gy, gz);
+      to_phys(T, px[it], py[it], pz[it], gx, gy, gz);
       al = classify(T, gx, gy, gz, bits);
     }
     c[0] += __popcll(__ballot(al));
@@ -178,6 +188,7 @@ k_xchg_scatter(int64_t n, int64_t ntiles, XIn in, XOut out, XTable T,
   int64_t base[NCAT];
 #pragma unroll
   for (int k = 0; k < NCAT; ++k) base[k] = off[k * ntiles + w];
+#pragma unroll 2
   for (int it = 0; it < XITEMS; ++it) {
     const int64_t i = w * XTILE + it * 64 + l;
     bool al = false;
@@ -186,20 +197,22 @@ k_xchg_scatter(int64_t n, int64_t ntiles, XIn in, XOut out, XTable T,
     int64_t id = 0;
     uint16_t mk = 0;
     if (i < n) {
-      to_phys(T, in.f[0][i], in.f[1][i], in.f[2][i], gx, gy, gz);
+      // every field loaded up front (almost all particles are kept), so the
+      // item costs one memory round trip
+      const float x0 = in.f[0][i], y0 = in.f[1][i], z0 = in.f[2][i];
+      const float u = in.f[3][i], v = in.f[4][i], t = in.f[5][i];
+      phi = in.f[7][i];
+      id = in.id[i];
+      mk = in.mask[i];
+      to_phys(T, x0, y0, z0, gx, gy, gz);
       al = classify(T, gx, gy, gz, bits);
-      if (al) {
-        // physical velocities, then back to grid units: two roundings each
-        vx = in.f[3][i] * T.vl2g;
-        vx = vx * T.vg2l;
-        vy = in.f[4][i] * T.vl2g;
-        vy = vy * T.vg2l;
-        vz = in.f[5][i] * T.vl2g;
-        vz = vz * T.vg2l;
-        phi = in.f[7][i];
-        id = in.id[i];
-        mk = in.mask[i];
-      }
+      // physical velocities, then back to grid units: two roundings each
+      vx = u * T.vl2g;
+      vx = vx * T.vg2l;
+      vy = v * T.vl2g;
+      vy = vy * T.vg2l;
+      vz = t * T.vl2g;
+      vz = vz * T.vg2l;
     }
     const uint64_t ba = __ballot(al);
     if (al)
