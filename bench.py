@@ -61,6 +61,8 @@ def parse():
                     help="sub-steps of the device-resident sub-cycle leg (0: skip it)")
     ap.add_argument("--pm-reps", type=int, default=3,
                     help="timed CIC deposit/interpolation pairs of the PM leg (0: skip it)")
+    ap.add_argument("--force-pass", type=int, default=0, choices=(0, 64, 128),
+                    help="hsr_tuning.force_pass: targets per leaf-kernel pass (0: default)")
     ap.add_argument("--refresh-reps", type=int, default=3,
                     help="timed one-rank overload refreshes of the refresh leg (0: skip it)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -373,7 +375,7 @@ def main():
         ctx = ctxs.get(nd)
         if ctx is None:
             ctx = ctxs[nd] = Context(nd=nd, tdpts=a.tdpts)
-            ctx.set_tuning(use_graph=1)
+            ctx.set_tuning(use_graph=1, force_pass=a.force_pass)
         for k in src:
             work[k].copy_(src[k])
         torch.cuda.synchronize()

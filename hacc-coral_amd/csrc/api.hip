@@ -383,7 +383,7 @@ hsr_status hsr_destroy(hsr_ctx *c) {
                   &c->adj.tj, &c->adj.par, &c->adj.counts, &c->adj.qctr, &c->adj.part, &c->adj.H,
                   &c->adj.W, &c->adj.msum,
                   &c->s.a, &c->s.b, &c->s.c, &c->s.d, &c->s.e, &c->s.f, &c->s.g, &c->s.h,
-                  &c->s.cub, &c->s.x1, &c->s.y1, &c->s.z1, &c->s.m1, &c->s.p1,
+                  &c->s.cub, &c->s.cms, &c->s.x1, &c->s.y1, &c->s.z1, &c->s.m1, &c->s.p1,
                   &c->s.x0, &c->s.y0, &c->s.z0, &c->s.m0, &c->s.p0, &c->s.blocks, &c->s.rights,
                   &c->s.pblocks, &c->s.counters, &c->dev_v, &c->force};
   if (c->adj.graph_exec) (void)hipGraphExecDestroy(c->adj.graph_exec);
@@ -839,6 +839,10 @@ hsr_status hsr_get_lists(hsr_ctx *c, int64_t *leaves, int64_t *seg_ptr,
 hsr_status hsr_set_tuning(hsr_ctx *c, const hsr_tuning *t) {
   if (!c || !t) {
     g_err = "hsr_set_tuning: null argument";
+    return HSR_ERR_ARG;
+  }
+  if (t->force_pass != 0 && t->force_pass != 64 && t->force_pass != 128) {
+    g_err = "hsr_set_tuning: force_pass must be 0, 64 or 128";
     return HSR_ERR_ARG;
   }
   const bool reorder = t->tree_order_targets != c->tuning.tree_order_targets;

@@ -49,6 +49,7 @@ struct LawArgs {
   int guard;
   int cull;        // drop entries farther than the cutoff from the leaf's box
   float cull2;     // squared box distance beyond which every pair is masked
+  int pmax;        // targets per pass (64 R or 32 R)
 };
 
 // Squared distance from p to the box [lo, hi] (0 inside).
@@ -78,9 +79,9 @@ __device__ __forceinline__ float poly_tail(const LawArgs &L, float s) {
 
 // Targets of the next pass over a leaf: 128 while that many remain, then the
 // largest of 64, 32, 16, 8 that fits, then the rest (< 8).
-__device__ __forceinline__ int64_t pass_size(int64_t rem) {
-  if (rem >= 64 * R) return 64 * R;
-  for (int64_t c = 32 * R; c >= 4 * R; c >>= 1)
+__device__ __forceinline__ int64_t pass_size(int64_t rem, int pmax) {
+  if (rem >= pmax) return pmax;
+  for (int64_t c = pmax >> 1; c >= 4 * R; c >>= 1)
     if (rem >= c) return c;
   return rem;
 }
@@ -127,7 +128,7 @@ k_force(const hsr_node *__restrict__ cn, const int32_t *__restrict__ leaves,
     // passes over the targets in power-of-two chunks (128, 64, ..., 8, rest)
     // so that every pass fills its groups of lpg = chunk / R lanes
     for (int64_t ib = 0, nib = 0; ib < cnt; ib += nib) {
-      nib = pass_size(cnt - ib);
+      nib = pass_size(cnt - ib, L.pmax);
       const int ni = (int)nib;
       const int lpg = max((ni + R - 1) / R, 64 / GMAX);
       const int G = max(1, 64 / lpg);
@@ -402,6 +403,7 @@ LawArgs law_args(const ForceLawDev &D) {
   // the kernel's own s < rcs test would accept
   A.cull = 1;
   A.cull2 = (float)((double)D.rc2 * (1.0 + 1e-4));
+  A.pmax = 64 * R;
   return A;
 }
 
@@ -607,6 +609,7 @@ static hsr_status force_enqueue(hsr_ctx *c, float *fx, float *fy, float *fz,
   k_zero_u64<<<1, 64, 0, st>>>(qctr + 1);
   LawArgs A = law_args(c->law);
   A.cull = c->tuning.cull_off ? 0 : 1;
+  A.pmax = c->tuning.force_pass == 32 * R ? 32 * R : 64 * R;
   const int wpc = c->tuning.waves_per_cu > 0 ? c->tuning.waves_per_cu : 32;
   const int64_t grid = std::min<int64_t>(LS.nitems, (int64_t)std::max(1, c->num_cus) * wpc);
   const hsr_node *cn = dptr<hsr_node>(c->tree.canon_nodes);

@@ -65,44 +65,56 @@ __device__ __forceinline__ float addend(int q, const float *X, const float *Y,
   return q == 3 ? w : w * (q == 0 ? X[e] : (q == 1 ? Y[e] : Z[e]));
 }
 
-// One wave per node with count <= CM_SMALL (BGQCM.c:181-212): bbox by wave
-// min/max, each of the four sums staged in LDS and added by sd_block.
+// One wave per (node, sum) for nodes with count <= CM_SMALL
+// (BGQCM.c:181-212): sum q (x, y, z, mass; blockIdx.y) staged in LDS and
+// added by sd_block into cms[4 (t - lb) + q]; the q = 0 wave also takes the
+// bbox by wave min/max.  k_cm_finish divides.
 __global__ void __launch_bounds__(64)
 k_cm_small(BNode *nodes, int64_t lb, int64_t le, const float *X,
-           const float *Y, const float *Z, const float *M) {
+           const float *Y, const float *Z, const float *M, sd_t *cms) {
   __shared__ float buf[CM_SMALL];
   int64_t t = lb + blockIdx.x;
   if (t >= le) return;
+  const int q = blockIdx.y;
   const int64_t cnt = nodes[t].count;
   if (cnt > CM_SMALL || cnt == 0) return;
   const int64_t off = nodes[t].offset;
   const int l = lane_id();
-  float mn[3] = {INFINITY, INFINITY, INFINITY};
-  float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int64_t k = l; k < cnt; k += 64) {
-    const float px = X[off + k], py = Y[off + k], pz = Z[off + k];
-    mn[0] = fminf(mn[0], px); mx[0] = fmaxf(mx[0], px);
-    mn[1] = fminf(mn[1], py); mx[1] = fmaxf(mx[1], py);
-    mn[2] = fminf(mn[2], pz); mx[2] = fmaxf(mx[2], pz);
-  }
-  sd_t S[4] = {sd_zero(), sd_zero(), sd_zero(), sd_zero()};
-  for (int q = 0; q < 4; ++q) {
-    __syncthreads();
-    for (int64_t k = l; k < cnt; k += 64) buf[k] = addend(q, X, Y, Z, M, off + k);
-    __syncthreads();
-    sd_block(S[q], buf, (int)cnt);
-  }
-  for (int d = 0; d < 3; ++d) {
-    mn[d] = wave_min_f32(mn[d]);
-    mx[d] = wave_max_f32(mx[d]);
-  }
-  if (l == 0) {
+  if (q == 0) {
+    float mn[3] = {INFINITY, INFINITY, INFINITY};
+    float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t k = l; k < cnt; k += 64) {
+      const float px = X[off + k], py = Y[off + k], pz = Z[off + k];
+      mn[0] = fminf(mn[0], px); mx[0] = fmaxf(mx[0], px);
+      mn[1] = fminf(mn[1], py); mx[1] = fmaxf(mx[1], py);
+      mn[2] = fminf(mn[2], pz); mx[2] = fmaxf(mx[2], pz);
+    }
     for (int d = 0; d < 3; ++d) {
-      nodes[t].xmin[d] = mn[d];
-      nodes[t].xmax[d] = mx[d];
-      nodes[t].xc[d] = sd_div_to_f32(S[d], S[3]);
+      mn[d] = wave_min_f32(mn[d]);
+      mx[d] = wave_max_f32(mx[d]);
+    }
+    if (l == 0) {
+      for (int d = 0; d < 3; ++d) {
+        nodes[t].xmin[d] = mn[d];
+        nodes[t].xmax[d] = mx[d];
+      }
     }
   }
+  sd_t S = sd_zero();
+  for (int64_t k = l; k < cnt; k += 64) buf[k] = addend(q, X, Y, Z, M, off + k);
+  __syncthreads();
+  sd_block(S, buf, (int)cnt);
+  if (l == 0) cms[4 * (t - lb) + q] = S;
+}
+
+// xc = (float)(sum w x / sum w) per dimension for every non-empty node of
+// the level, from the four exact sums (BGQCM.c:208-210).
+__global__ void k_cm_finish(BNode *nodes, int64_t lb, int64_t le, const sd_t *cms) {
+  const int64_t t = lb + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= le) return;
+  if (nodes[t].count == 0) return;
+  const sd_t *S = cms + 4 * (t - lb);
+  for (int d = 0; d < 3; ++d) nodes[t].xc[d] = sd_div_to_f32(S[d], S[3]);
 }
 
 // ------------------------------------------------------------ cm (big)
@@ -305,52 +317,46 @@ k_cm_win_maps(const BNode *nodes, CmBlock *blk, const Counters *ctr) {
 __global__ void __launch_bounds__(64)
 k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, const CmBlock *blk,
                const float *X, const float *Y, const float *Z,
-               const float *M, Counters *ctr) {
+               const float *M, Counters *ctr, sd_t *cms) {
   __shared__ float buf[CM_BLK];
   int64_t t = lb + blockIdx.x;
   if (t >= le) return;
   if (nodes[t].count <= CM_SMALL) return;
+  const int q = blockIdx.y;  // one wave per (node, sum): x, y, z, mass
   const int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
   const int l = lane_id();
-  sd_t S[4] = {sd_zero(), sd_zero(), sd_zero(), sd_zero()};
+  sd_t Sq = sd_zero();
   for (int64_t bb = 0; bb < nb; bb += 64) {
     // the whole window at once when its blocks share a grid and S stays in it
-    int done = 0;
     {
       const CmBlock &W = blk[b0 + bb];
-      for (int q = 0; q < 4; ++q) {
-        sd_map g;
-        g.d0 = W.wd0[q];
-        g.d1 = W.wd1[q];
-        g.bad = !W.wok[q];
-        if (sd_map_apply(&S[q], g, W.wE[q])) done |= 1 << q;
-      }
+      sd_map g;
+      g.d0 = W.wd0[q];
+      g.d1 = W.wd1[q];
+      g.bad = !W.wok[q];
+      if (sd_map_apply(&Sq, g, W.wE[q])) continue;
     }
-    if (done == 15) continue;
-    uint64_t md0[4] = {0, 0, 0, 0}, md1[4] = {0, 0, 0, 0};
-    int mbad[4] = {1, 1, 1, 1}, mE[4] = {0, 0, 0, 0};
+    uint64_t md0 = 0, md1 = 0;
+    int mbad = 1, mE = 0;
     if (bb + l < nb) {
       const CmBlock &B = blk[b0 + bb + l];
-      for (int q = 0; q < 4; ++q) {
-        md0[q] = B.d0[q];
-        md1[q] = B.d1[q];
-        mbad[q] = B.bad[q];
-        mE[q] = B.E[q];
-      }
+      md0 = B.d0[q];
+      md1 = B.d1[q];
+      mbad = B.bad[q];
+      mE = B.E[q];
     }
     const int cnt = (int)min<int64_t>(64, nb - bb);
-    for (int q = 0; q < 4; ++q) {
-      if (done & (1 << q)) continue;
+    {
       for (int i = 0; i < cnt;) {  // uniform
-        if (S[q].m != 0 && !S[q].s) {
+        if (Sq.m != 0 && !Sq.s) {
           // ordered scan of the block maps from i built for S's binade; the
           // longest prefix that keeps S in the binade is applied at once
-          const int32_t E = S[q].e;
+          const int32_t E = Sq.e;
           sd_map m = sd_map_identity();
           if (l >= i && l < cnt) {
-            m.d0 = md0[q];
-            m.d1 = md1[q];
-            m.bad = mbad[q] || mE[q] != E;
+            m.d0 = md0;
+            m.d1 = md1;
+            m.bad = mbad || mE != E;
           }
           for (int o = 1; o < 64; o <<= 1) {
             sd_map h;
@@ -359,7 +365,7 @@ k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, const CmBlock *blk,
             h.bad = __shfl_up(m.bad, o);
             if (l >= o) m = sd_map_compose(h, m);
           }
-          sd_t S2 = S[q];
+          sd_t S2 = Sq;
           const bool ok = l >= i && l < cnt && sd_map_apply(&S2, m, E);
           const uint64_t bal = ((uint64_t)__ballot(ok) >> i) | (cnt - i < 64 ? ~0ull << (cnt - i) : 0ull);
           const int j = min(cnt, i + ((~bal == 0ull) ? 64 - i : (int)__builtin_ctzll(~bal)));
@@ -367,9 +373,9 @@ k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, const CmBlock *blk,
           if (l == 0) atomicAdd((unsigned long long *)&ctr->pad[0], 1ull);
 #endif
           if (j > i) {
-            S[q].m = readlane_u64(S2.m, j - 1);
-            S[q].e = readlane_i32(S2.e, j - 1);
-            S[q].s = readlane_i32(S2.s, j - 1);
+            Sq.m = readlane_u64(S2.m, j - 1);
+            Sq.e = readlane_i32(S2.e, j - 1);
+            Sq.s = readlane_i32(S2.s, j - 1);
             i = j;
             continue;
           }
@@ -383,14 +389,12 @@ k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, const CmBlock *blk,
         __syncthreads();
         for (int k = l; k < len; k += 64) buf[k] = addend(q, X, Y, Z, M, start + k);
         __syncthreads();
-        sd_block(S[q], buf, (int)len);
+        sd_block(Sq, buf, (int)len);
         ++i;
       }
     }
   }
-  if (l == 0) {
-    for (int d = 0; d < 3; ++d) nodes[t].xc[d] = sd_div_to_f32(S[d], S[3]);
-  }
+  if (l == 0) cms[4 * (t - lb) + q] = Sq;
 }
 
 // ------------------------------------------------------------ classify
@@ -1065,14 +1069,19 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     int64_t cm_cap = active / CM_BLK + active / CM_SMALL + 2;
     HSR_HIP(dbuf_reserve(S.blocks, sizeof(CmBlock) * cm_cap));
     CmBlock *cmb = dptr<CmBlock>(S.blocks);
-    k_cm_small<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, X[cur], Y[cur], Z[cur], Mm[cur]);
+    HSR_HIP(dbuf_reserve(S.cms, sizeof(sd_t) * 4 * (size_t)nl));
+    sd_t *cms = dptr<sd_t>(S.cms);
+    k_cm_small<<<dim3((unsigned)nl, 4), 64, 0, st>>>(nodes, lb, le, X[cur], Y[cur], Z[cur], Mm[cur],
+                                                    cms);
     k_cm_plan<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, cmb, ctr);
     unsigned cmg = (unsigned)std::max<int64_t>(1, active / CM_BLK + active / CM_SMALL + 1);
     k_cm_blk_stats<<<cmg, 64, 0, st>>>(cmb, ctr, X[cur], Y[cur], Z[cur], Mm[cur]);
     k_cm_blk_prefix<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, cmb);
     k_cm_blk_maps<<<cmg, 64, 0, st>>>(cmb, ctr, X[cur], Y[cur], Z[cur], Mm[cur]);
     k_cm_win_maps<<<cmg, 64, 0, st>>>(nodes, cmb, ctr);
-    k_cm_blk_chain<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, cmb, X[cur], Y[cur], Z[cur], Mm[cur], ctr);
+    k_cm_blk_chain<<<dim3((unsigned)nl, 4), 64, 0, st>>>(nodes, lb, le, cmb, X[cur], Y[cur], Z[cur],
+                                                        Mm[cur], ctr, cms);
+    k_cm_finish<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, cms);
     // ---- classify, finalize plain leaves
     k_classify<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, nd, c->p.ppc, Mm[cur]);
     k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_LEAF, P[cur], Pfin);

@@ -364,6 +364,9 @@ typedef struct {
                          (default): sorted along the leaf's longest edge, so
                          that each pass of 128 targets spans a slab and culls
                          more entries (sums in another order: rounding only) */
+  int force_pass;     /* targets per pass of the leaf kernel: 128 (0 = default)
+                         or 64 (smaller target boxes cull more list entries;
+                         sums in another order: rounding only) */
 } hsr_tuning;
 
 hsr_status hsr_set_tuning(hsr_ctx *ctx, const hsr_tuning *t);
