@@ -65,6 +65,28 @@ __device__ __forceinline__ float addend(int q, const float *X, const float *Y,
   return q == 3 ? w : w * (q == 0 ? X[e] : (q == 1 ? Y[e] : Z[e]));
 }
 
+// buf[k] = addend q of element off + k for k < n, U loads in flight per lane
+// (the staging is latency-bound otherwise).
+template <int U>
+__device__ __forceinline__ void stage_addends(float *buf, int q, const float *X, const float *Y,
+                                              const float *Z, const float *M, int64_t off,
+                                              int64_t n) {
+  const int l = lane_id();
+  for (int64_t k0 = 0; k0 < n; k0 += 64 * U) {
+    float v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t k = k0 + u * 64 + l;
+      v[u] = k < n ? addend(q, X, Y, Z, M, off + k) : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t k = k0 + u * 64 + l;
+      if (k < n) buf[k] = v[u];
+    }
+  }
+}
+
 // One wave per (node, sum) for nodes with count <= CM_SMALL
 // (BGQCM.c:181-212): sum q (x, y, z, mass; blockIdx.y) staged in LDS and
 // added by sd_block into cms[4 (t - lb) + q]; the q = 0 wave also takes the
@@ -83,11 +105,22 @@ k_cm_small(BNode *nodes, int64_t lb, int64_t le, const float *X,
   if (q == 0) {
     float mn[3] = {INFINITY, INFINITY, INFINITY};
     float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int64_t k = l; k < cnt; k += 64) {
-      const float px = X[off + k], py = Y[off + k], pz = Z[off + k];
-      mn[0] = fminf(mn[0], px); mx[0] = fmaxf(mx[0], px);
-      mn[1] = fminf(mn[1], py); mx[1] = fmaxf(mx[1], py);
-      mn[2] = fminf(mn[2], pz); mx[2] = fmaxf(mx[2], pz);
+    for (int64_t k0 = 0; k0 < cnt; k0 += 64 * 4) {
+      float px[4], py[4], pz[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int64_t k = k0 + u * 64 + l;
+        const bool v = k < cnt;
+        px[u] = v ? X[off + k] : X[off];
+        py[u] = v ? Y[off + k] : Y[off];
+        pz[u] = v ? Z[off + k] : Z[off];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        mn[0] = fminf(mn[0], px[u]); mx[0] = fmaxf(mx[0], px[u]);
+        mn[1] = fminf(mn[1], py[u]); mx[1] = fmaxf(mx[1], py[u]);
+        mn[2] = fminf(mn[2], pz[u]); mx[2] = fmaxf(mx[2], pz[u]);
+      }
     }
     for (int d = 0; d < 3; ++d) {
       mn[d] = wave_min_f32(mn[d]);
@@ -101,7 +134,7 @@ k_cm_small(BNode *nodes, int64_t lb, int64_t le, const float *X,
     }
   }
   sd_t S = sd_zero();
-  for (int64_t k = l; k < cnt; k += 64) buf[k] = addend(q, X, Y, Z, M, off + k);
+  stage_addends<8>(buf, q, X, Y, Z, M, off, cnt);
   __syncthreads();
   sd_block(S, buf, (int)cnt);
   if (l == 0) cms[4 * (t - lb) + q] = S;
@@ -158,6 +191,7 @@ k_cm_blk_stats(CmBlock *blk, const Counters *ctr, const float *X,
   float s[4] = {0.f, 0.f, 0.f, 0.f};
   float mn[3] = {INFINITY, INFINITY, INFINITY};
   float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+#pragma unroll 4
   for (int64_t k = l; k < B.len; k += 64) {
     int64_t e = B.start + k;
     float w = M[e], px = X[e], py = Y[e], pz = Z[e];
@@ -183,33 +217,48 @@ k_cm_blk_stats(CmBlock *blk, const Counters *ctr, const float *X,
   }
 }
 
-// Per big node (one wave): exclusive prefix of the approximate block sums
-// (only a guess of the running sum's binade, so any summation order serves)
-// and the node bbox (min/max: order-free).
-__global__ void __launch_bounds__(64)
+// Per big node (one workgroup of 4 waves): exclusive prefix of the
+// approximate block sums (only a guess of the running sum's binade, so any
+// summation order serves) and the node bbox (min/max: order-free).
+__global__ void __launch_bounds__(256)
 k_cm_blk_prefix(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk) {
+  __shared__ float wtot[4][4];
+  __shared__ float wmn[4][3], wmx[4][3];
   const int64_t t = lb + blockIdx.x;
   if (t >= le) return;
   if (nodes[t].count <= CM_SMALL) return;
-  const int l = lane_id();
+  const int l = lane_id(), w = threadIdx.x >> 6;
   const int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
   float base[4] = {0.f, 0.f, 0.f, 0.f};
   float mn[3] = {INFINITY, INFINITY, INFINITY};
   float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int64_t c0 = 0; c0 < nb; c0 += 64) {
-    const int64_t b = b0 + c0 + l;
-    const bool v = c0 + l < nb;
+  for (int64_t c0 = 0; c0 < nb; c0 += 256) {
+    const int64_t b = b0 + c0 + threadIdx.x;
+    const bool v = c0 + threadIdx.x < nb;
+    float a[4], inc[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const float a = v ? blk[b].approx[q] : 0.f;
-      float inc = a;  // inclusive scan over the lanes
+      a[q] = v ? blk[b].approx[q] : 0.f;
+      inc[q] = a[q];  // inclusive scan over the wave's lanes
       for (int o = 1; o < 64; o <<= 1) {
-        const float y = __shfl_up(inc, o);
-        if (l >= o) inc += y;
+        const float y = __shfl_up(inc[q], o);
+        if (l >= o) inc[q] += y;
       }
-      if (v) blk[b].ein[q] = base[q] + (inc - a);
-      base[q] += __shfl(inc, 63);
     }
+    if (l == 63)
+      for (int q = 0; q < 4; ++q) wtot[w][q] = inc[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      float before = 0.f, all = 0.f;
+      for (int k = 0; k < 4; ++k) {
+        if (k < w) before += wtot[k][q];
+        all += wtot[k][q];
+      }
+      if (v) blk[b].ein[q] = base[q] + before + (inc[q] - a[q]);
+      base[q] += all;
+    }
+    __syncthreads();
     if (v) {
       for (int d = 0; d < 3; ++d) {
         mn[d] = fminf(mn[d], blk[b].bmin[d]);
@@ -221,10 +270,21 @@ k_cm_blk_prefix(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk) {
     mn[d] = wave_min_f32(mn[d]);
     mx[d] = wave_max_f32(mx[d]);
   }
-  if (l == 0) {
+  if (l == 0)
     for (int d = 0; d < 3; ++d) {
-      nodes[t].xmin[d] = mn[d];
-      nodes[t].xmax[d] = mx[d];
+      wmn[w][d] = mn[d];
+      wmx[w][d] = mx[d];
+    }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int d = 0; d < 3; ++d) {
+      float a = wmn[0][d], b = wmx[0][d];
+      for (int k = 1; k < 4; ++k) {
+        a = fminf(a, wmn[k][d]);
+        b = fmaxf(b, wmx[k][d]);
+      }
+      nodes[t].xmin[d] = a;
+      nodes[t].xmax[d] = b;
     }
   }
 }
@@ -246,31 +306,30 @@ k_cm_blk_maps(CmBlock *blk, const Counters *ctr, const float *X,
   __shared__ float buf[64 * (CM_PER + 1)];
   int64_t b = blockIdx.x;
   if (b >= ctr->cm_blocks) return;
+  const int q = blockIdx.y;  // one wave per (block, sum)
   CmBlock &B = blk[b];
   const int l = lane_id();
   const int64_t start = B.start, len = B.len;
-  for (int q = 0; q < 4; ++q) {
-    const int32_t E = grid_exp_guess(B.ein[q]);
-    sd_map acc = sd_map_identity();
-    if (E == INT32_MIN) {
-      acc.bad = 1;
-    } else {
-      __syncthreads();
-      for (int k = l; k < CM_BLK; k += 64) {
-        float v = k < len ? addend(q, X, Y, Z, M, start + k) : 0.f;
-        buf[(k / CM_PER) * (CM_PER + 1) + (k % CM_PER)] = v;
-      }
-      __syncthreads();
-      for (int k = 0; k < CM_PER; ++k)
-        acc = sd_map_compose(acc, sd_map_of(buf[l * (CM_PER + 1) + k], E));
-      acc = wave_compose_maps(acc);
+  const int32_t E = grid_exp_guess(B.ein[q]);
+  sd_map acc = sd_map_identity();
+  if (E == INT32_MIN) {
+    acc.bad = 1;
+  } else {
+#pragma unroll
+    for (int k = l; k < CM_BLK; k += 64) {
+      float v = k < len ? addend(q, X, Y, Z, M, start + k) : 0.f;
+      buf[(k / CM_PER) * (CM_PER + 1) + (k % CM_PER)] = v;
     }
-    if (l == 0) {
-      B.d0[q] = acc.d0;
-      B.d1[q] = acc.d1;
-      B.bad[q] = acc.bad;
-      B.E[q] = E;
-    }
+    __syncthreads();
+    for (int k = 0; k < CM_PER; ++k)
+      acc = sd_map_compose(acc, sd_map_of(buf[l * (CM_PER + 1) + k], E));
+    acc = wave_compose_maps(acc);
+  }
+  if (l == 0) {
+    B.d0[q] = acc.d0;
+    B.d1[q] = acc.d1;
+    B.bad[q] = acc.bad;
+    B.E[q] = E;
   }
 }
 
@@ -387,7 +446,7 @@ k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, const CmBlock *blk,
         const CmBlock &B = blk[b0 + bb + i];
         const int64_t start = B.start, len = B.len;
         __syncthreads();
-        for (int k = l; k < len; k += 64) buf[k] = addend(q, X, Y, Z, M, start + k);
+        stage_addends<8>(buf, q, X, Y, Z, M, start, len);
         __syncthreads();
         sd_block(Sq, buf, (int)len);
         ++i;
@@ -448,25 +507,39 @@ k_finalize_leaves(const BNode *nodes, int64_t lb, int64_t le, int kind,
 __global__ void __launch_bounds__(64)
 k_part_plan(BNode *nodes, int64_t lb, int64_t le, PartBlock *pb,
             Counters *ctr) {
-  int64_t t = lb + blockIdx.x;
-  if (t >= le) return;
-  if (nodes[t].kind != NODE_SPLIT) return;
-  int64_t cnt = nodes[t].count;
-  int64_t nb = (cnt + PB - 1) / PB;
-  int64_t b0 = 0;
-  if (lane_id() == 0) {
-    b0 = (int64_t)atomicAdd((unsigned long long *)&ctr->part_blocks,
-                            (unsigned long long)nb);
+  // one lane per node: a wave scan of the block counts and one atomic per
+  // wave reserve the blocks; then the wave fills them node by node
+  const int l = lane_id();
+  const int64_t t0 = lb + (int64_t)blockIdx.x * 64;
+  const int64_t t = t0 + l;
+  int64_t nb = 0;
+  if (t < le && nodes[t].kind == NODE_SPLIT) nb = (nodes[t].count + PB - 1) / PB;
+  int64_t inc = nb;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int64_t y = (int64_t)__shfl_up((long long)inc, o);
+    if (l >= o) inc += y;
+  }
+  int64_t base = 0;
+  if (l == 63 && inc > 0)
+    base = (int64_t)atomicAdd((unsigned long long *)&ctr->part_blocks, (unsigned long long)inc);
+  base = (int64_t)__shfl((long long)base, 63);
+  const int64_t b0 = base + inc - nb;
+  if (nb > 0) {
     nodes[t].blk0 = b0;
     nodes[t].nblk = nb;
   }
-  b0 = (int64_t)__shfl((long long)b0, 0);
-  int64_t off = nodes[t].offset;
-  for (int64_t b = lane_id(); b < nb; b += 64) {
-    PartBlock &B = pb[b0 + b];
-    B.node = t;
-    B.start = off + b * PB;
-    B.len = min<int64_t>(PB, cnt - b * PB);
+  for (int j = 0; j < 64; ++j) {
+    const int64_t nbj = (int64_t)__shfl((long long)nb, j);
+    if (nbj == 0) continue;
+    const int64_t b0j = (int64_t)__shfl((long long)b0, j);
+    const int64_t tj = t0 + j;
+    const int64_t off = nodes[tj].offset, cnt = nodes[tj].count;
+    for (int64_t b = l; b < nbj; b += 64) {
+      PartBlock &B = pb[b0j + b];
+      B.node = tj;
+      B.start = off + b * PB;
+      B.len = min<int64_t>(PB, cnt - b * PB);
+    }
   }
 }
 
@@ -1076,8 +1149,8 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     k_cm_plan<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, cmb, ctr);
     unsigned cmg = (unsigned)std::max<int64_t>(1, active / CM_BLK + active / CM_SMALL + 1);
     k_cm_blk_stats<<<cmg, 64, 0, st>>>(cmb, ctr, X[cur], Y[cur], Z[cur], Mm[cur]);
-    k_cm_blk_prefix<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, cmb);
-    k_cm_blk_maps<<<cmg, 64, 0, st>>>(cmb, ctr, X[cur], Y[cur], Z[cur], Mm[cur]);
+    k_cm_blk_prefix<<<(unsigned)nl, 256, 0, st>>>(nodes, lb, le, cmb);
+    k_cm_blk_maps<<<dim3(cmg, 4), 64, 0, st>>>(cmb, ctr, X[cur], Y[cur], Z[cur], Mm[cur]);
     k_cm_win_maps<<<cmg, 64, 0, st>>>(nodes, cmb, ctr);
     k_cm_blk_chain<<<dim3((unsigned)nl, 4), 64, 0, st>>>(nodes, lb, le, cmb, X[cur], Y[cur], Z[cur],
                                                         Mm[cur], ctr, cms);
@@ -1089,7 +1162,7 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     int64_t pcap = active / PB + nl + 2;
     HSR_HIP(dbuf_reserve(S.pblocks, sizeof(PartBlock) * pcap));
     PartBlock *pb = dptr<PartBlock>(S.pblocks);
-    k_part_plan<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, pb, ctr);
+    k_part_plan<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, pb, ctr);
     unsigned pg = (unsigned)std::max<int64_t>(1, pcap);
     k_part_count<<<pg, 256, 0, st>>>(nodes, pb, ctr, X[cur], Y[cur], Z[cur]);
     k_part_scan<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, pb);
