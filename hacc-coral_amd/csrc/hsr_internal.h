@@ -150,6 +150,7 @@ struct AdjState {
 struct Scratch {
   DBuf a, b, c, d, e, f, g, h, cub;
   DBuf cms;        // tree build: the four exact sums of each node of a level
+  DBuf cmfail;     // tree build: small-node sums left to the ordered path
   DBuf x1, y1, z1, m1, p1, x0, y0, z0, m0, p0;
   DBuf blocks, pblocks, rights;
   DBuf counters, qctr;

@@ -285,17 +285,24 @@ SD_FN int32_t sd_lsb(sd_t S) {
   return S.e + sd_ctz64(S.m);
 }
 
-// v as an integer multiple of 2^t; returns UINT64_MAX when v >= 2^(t+53) or
-// negative.
+// v as an integer multiple of 2^t; returns UINT64_MAX when v >= 2^(t+53),
+// negative, or not a multiple of 2^t.  The mantissa is not normalised to an
+// odd integer, so t may lie above the split exponent (1.0 = 2^23 * 2^-23 is
+// a multiple of 2^0): the low mantissa bits below 2^t must then be zero.
 SD_FN uint64_t sd_units_f32(float v, int32_t t) {
   uint32_t fm;
   int32_t fe, fs;
   sd_split_f32(v, &fm, &fe, &fs);
   if (fm == 0) return 0;
   if (fs) return ~0ull;
-  int sh = fe - t;  // >= 0 by choice of t
+  int sh = fe - t;
+  if (sh < 0) {
+    const int k = -sh;
+    if (k >= 32 || (fm & ((1u << k) - 1u))) return ~0ull;
+    return (uint64_t)(fm >> k);
+  }
   int bits = 64 - sd_clz64((uint64_t)fm);
-  if (sh < 0 || sh + bits > 53) return ~0ull;
+  if (sh + bits > 53) return ~0ull;
   return (uint64_t)fm << sh;
 }
 

@@ -35,7 +35,8 @@ struct Counters {
   int64_t nsplit;         // non-degenerate splits this level
   int64_t active_next;    // particles in the next level's nodes
   int64_t unresolved;     // swap chains longer than MAXHOP
-  int64_t pad[3];
+  int64_t cm_fail;        // small-node sums left to the ordered path
+  int64_t pad[2];
 };
 
 // ------------------------------------------------------------ helpers
@@ -88,14 +89,18 @@ __device__ __forceinline__ void stage_addends(float *buf, int q, const float *X,
 }
 
 // One wave per (node, sum) for nodes with count <= CM_SMALL
-// (BGQCM.c:181-212): sum q (x, y, z, mass; blockIdx.y) staged in LDS and
-// added by sd_block into cms[4 (t - lb) + q]; the q = 0 wave also takes the
-// bbox by wave min/max.  k_cm_finish divides.
+// (BGQCM.c:181-212), sum q (x, y, z, mass) = blockIdx.y; the q = 0 wave also
+// takes the bbox by wave min/max.  The addends stay in registers and are
+// summed exactly when every partial sum is a multiple of the finest lsb
+// below 2^53 of them (no rounding anywhere, so the sequential double sum is
+// that integer sum; the first test of sd_block).  Sums that round go to
+// cm_list for k_cm_small_ordered.  k_cm_finish divides.
+constexpr int SM_PER = CM_SMALL / 64;
 __global__ void __launch_bounds__(64)
 k_cm_small(BNode *nodes, int64_t lb, int64_t le, const float *X,
-           const float *Y, const float *Z, const float *M, sd_t *cms) {
-  __shared__ float buf[CM_SMALL];
-  int64_t t = lb + blockIdx.x;
+           const float *Y, const float *Z, const float *M, sd_t *cms,
+           int32_t *cm_list, Counters *ctr) {
+  const int64_t t = lb + blockIdx.x;
   if (t >= le) return;
   const int q = blockIdx.y;
   const int64_t cnt = nodes[t].count;
@@ -133,11 +138,69 @@ k_cm_small(BNode *nodes, int64_t lb, int64_t le, const float *X,
       }
     }
   }
-  sd_t S = sd_zero();
-  stage_addends<8>(buf, q, X, Y, Z, M, off, cnt);
-  __syncthreads();
-  sd_block(S, buf, (int)cnt);
-  if (l == 0) cms[4 * (t - lb) + q] = S;
+  // two passes over the addends, 8 loads in flight per lane (the second
+  // pass reads them again from L2)
+  int32_t tl = 1 << 20;  // the lsb exponent of zero (sd_lsb_f32)
+  for (int i0 = 0; i0 < SM_PER; i0 += 8) {
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int64_t k = (i0 + i) * 64 + l;
+      v[i] = k < cnt ? addend(q, X, Y, Z, M, off + k) : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) tl = min(tl, sd_lsb_f32(v[i]));
+  }
+  tl = wave_min_i32(tl);
+  uint64_t u = 0;  // < 2048 addends below 2^53 units each: no wrap
+  int bad = 0;
+  for (int i0 = 0; i0 < SM_PER; i0 += 8) {
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int64_t k = (i0 + i) * 64 + l;
+      v[i] = k < cnt ? addend(q, X, Y, Z, M, off + k) : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const uint64_t uu = sd_units_f32(v[i], tl);
+      if (uu == ~0ull) bad = 1;
+      else u += uu;
+    }
+  }
+  if (!__any(bad)) {
+    const uint64_t tot = wave_sum_u64(u);
+    if (tot < SD_M53) {
+      if (l == 0) cms[4 * (t - lb) + q] = sd_from_units(tot, tl);
+      return;
+    }
+  }
+  if (l == 0) {
+    const int64_t k = (int64_t)atomicAdd((unsigned long long *)&ctr->cm_fail, 1ull);
+    cm_list[k] = (int32_t)(4 * (t - lb) + q);
+  }
+}
+
+// The small-node sums that round: the addends staged in LDS in order and
+// added by sd_block (exact sequential double additions), a wave each.
+__global__ void __launch_bounds__(64)
+k_cm_small_ordered(const BNode *nodes, int64_t lb, const float *X, const float *Y,
+                   const float *Z, const float *M, sd_t *cms, const int32_t *cm_list,
+                   const Counters *ctr) {
+  __shared__ float buf[CM_SMALL];
+  const int l = lane_id();
+  for (int64_t k = blockIdx.x; k < ctr->cm_fail; k += gridDim.x) {
+    const int32_t item = cm_list[k];
+    const int64_t t = lb + item / 4;
+    const int q = item % 4;
+    const int64_t cnt = nodes[t].count, off = nodes[t].offset;
+    sd_t S = sd_zero();
+    __syncthreads();  // the previous sum has read buf
+    stage_addends<8>(buf, q, X, Y, Z, M, off, cnt);
+    __syncthreads();
+    sd_block(S, buf, (int)cnt);
+    if (l == 0) cms[item] = S;
+  }
 }
 
 // xc = (float)(sum w x / sum w) per dimension for every non-empty node of
@@ -155,28 +218,28 @@ __global__ void k_cm_finish(BNode *nodes, int64_t lb, int64_t le, const sd_t *cm
 __global__ void __launch_bounds__(64)
 k_cm_plan(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk,
           Counters *ctr) {
-  int64_t t = lb + blockIdx.x;
-  if (t >= le) return;
-  int64_t cnt = nodes[t].count;
-  if (cnt <= CM_SMALL) {
-    if (lane_id() == 0) nodes[t].nblk = 0;
-    return;
-  }
-  int64_t nb = (cnt + CM_BLK - 1) / CM_BLK;
-  int64_t b0 = 0;
-  if (lane_id() == 0) {
-    b0 = (int64_t)atomicAdd((unsigned long long *)&ctr->cm_blocks,
-                            (unsigned long long)nb);
-    nodes[t].blk0 = b0;
-    nodes[t].nblk = nb;
-  }
-  b0 = (int64_t)__shfl((long long)b0, 0);
-  int64_t off = nodes[t].offset;
-  for (int64_t b = lane_id(); b < nb; b += 64) {
-    CmBlock &B = blk[b0 + b];
-    B.node = t;
-    B.start = off + b * CM_BLK;
-    B.len = min<int64_t>(CM_BLK, cnt - b * CM_BLK);
+  for (int64_t t = lb + blockIdx.x; t < le; t += gridDim.x) {
+    int64_t cnt = nodes[t].count;
+    if (cnt <= CM_SMALL) {
+      if (lane_id() == 0) nodes[t].nblk = 0;
+      continue;
+    }
+    int64_t nb = (cnt + CM_BLK - 1) / CM_BLK;
+    int64_t b0 = 0;
+    if (lane_id() == 0) {
+      b0 = (int64_t)atomicAdd((unsigned long long *)&ctr->cm_blocks,
+                              (unsigned long long)nb);
+      nodes[t].blk0 = b0;
+      nodes[t].nblk = nb;
+    }
+    b0 = (int64_t)__shfl((long long)b0, 0);
+    int64_t off = nodes[t].offset;
+    for (int64_t b = lane_id(); b < nb; b += 64) {
+      CmBlock &B = blk[b0 + b];
+      B.node = t;
+      B.start = off + b * CM_BLK;
+      B.len = min<int64_t>(CM_BLK, cnt - b * CM_BLK);
+    }
   }
 }
 
@@ -184,35 +247,35 @@ k_cm_plan(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk,
 __global__ void __launch_bounds__(64)
 k_cm_blk_stats(CmBlock *blk, const Counters *ctr, const float *X,
                const float *Y, const float *Z, const float *M) {
-  int64_t b = blockIdx.x;
-  if (b >= ctr->cm_blocks) return;
-  CmBlock &B = blk[b];
-  int l = lane_id();
-  float s[4] = {0.f, 0.f, 0.f, 0.f};
-  float mn[3] = {INFINITY, INFINITY, INFINITY};
-  float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t b = blockIdx.x; b < ctr->cm_blocks; b += gridDim.x) {
+    CmBlock &B = blk[b];
+    int l = lane_id();
+    float s[4] = {0.f, 0.f, 0.f, 0.f};
+    float mn[3] = {INFINITY, INFINITY, INFINITY};
+    float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
 #pragma unroll 4
-  for (int64_t k = l; k < B.len; k += 64) {
-    int64_t e = B.start + k;
-    float w = M[e], px = X[e], py = Y[e], pz = Z[e];
-    mn[0] = fminf(mn[0], px); mx[0] = fmaxf(mx[0], px);
-    mn[1] = fminf(mn[1], py); mx[1] = fmaxf(mx[1], py);
-    mn[2] = fminf(mn[2], pz); mx[2] = fmaxf(mx[2], pz);
-    s[0] += w * px;
-    s[1] += w * py;
-    s[2] += w * pz;
-    s[3] += w;
-  }
-  for (int q = 0; q < 4; ++q) s[q] = wave_sum_f32(s[q]);
-  for (int d = 0; d < 3; ++d) {
-    mn[d] = wave_min_f32(mn[d]);
-    mx[d] = wave_max_f32(mx[d]);
-  }
-  if (l == 0) {
-    for (int q = 0; q < 4; ++q) B.approx[q] = s[q];
+    for (int64_t k = l; k < B.len; k += 64) {
+      int64_t e = B.start + k;
+      float w = M[e], px = X[e], py = Y[e], pz = Z[e];
+      mn[0] = fminf(mn[0], px); mx[0] = fmaxf(mx[0], px);
+      mn[1] = fminf(mn[1], py); mx[1] = fmaxf(mx[1], py);
+      mn[2] = fminf(mn[2], pz); mx[2] = fmaxf(mx[2], pz);
+      s[0] += w * px;
+      s[1] += w * py;
+      s[2] += w * pz;
+      s[3] += w;
+    }
+    for (int q = 0; q < 4; ++q) s[q] = wave_sum_f32(s[q]);
     for (int d = 0; d < 3; ++d) {
-      B.bmin[d] = mn[d];
-      B.bmax[d] = mx[d];
+      mn[d] = wave_min_f32(mn[d]);
+      mx[d] = wave_max_f32(mx[d]);
+    }
+    if (l == 0) {
+      for (int q = 0; q < 4; ++q) B.approx[q] = s[q];
+      for (int d = 0; d < 3; ++d) {
+        B.bmin[d] = mn[d];
+        B.bmax[d] = mx[d];
+      }
     }
   }
 }
@@ -224,67 +287,67 @@ __global__ void __launch_bounds__(256)
 k_cm_blk_prefix(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk) {
   __shared__ float wtot[4][4];
   __shared__ float wmn[4][3], wmx[4][3];
-  const int64_t t = lb + blockIdx.x;
-  if (t >= le) return;
-  if (nodes[t].count <= CM_SMALL) return;
   const int l = lane_id(), w = threadIdx.x >> 6;
-  const int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
-  float base[4] = {0.f, 0.f, 0.f, 0.f};
-  float mn[3] = {INFINITY, INFINITY, INFINITY};
-  float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int64_t c0 = 0; c0 < nb; c0 += 256) {
-    const int64_t b = b0 + c0 + threadIdx.x;
-    const bool v = c0 + threadIdx.x < nb;
-    float a[4], inc[4];
+  for (int64_t t = lb + blockIdx.x; t < le; t += gridDim.x) {
+    if (nodes[t].count <= CM_SMALL) continue;
+    const int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
+    float base[4] = {0.f, 0.f, 0.f, 0.f};
+    float mn[3] = {INFINITY, INFINITY, INFINITY};
+    float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t c0 = 0; c0 < nb; c0 += 256) {
+      const int64_t b = b0 + c0 + threadIdx.x;
+      const bool v = c0 + threadIdx.x < nb;
+      float a[4], inc[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      a[q] = v ? blk[b].approx[q] : 0.f;
-      inc[q] = a[q];  // inclusive scan over the wave's lanes
-      for (int o = 1; o < 64; o <<= 1) {
-        const float y = __shfl_up(inc[q], o);
-        if (l >= o) inc[q] += y;
+      for (int q = 0; q < 4; ++q) {
+        a[q] = v ? blk[b].approx[q] : 0.f;
+        inc[q] = a[q];  // inclusive scan over the wave's lanes
+        for (int o = 1; o < 64; o <<= 1) {
+          const float y = __shfl_up(inc[q], o);
+          if (l >= o) inc[q] += y;
+        }
+      }
+      if (l == 63)
+        for (int q = 0; q < 4; ++q) wtot[w][q] = inc[q];
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        float before = 0.f, all = 0.f;
+        for (int k = 0; k < 4; ++k) {
+          if (k < w) before += wtot[k][q];
+          all += wtot[k][q];
+        }
+        if (v) blk[b].ein[q] = base[q] + before + (inc[q] - a[q]);
+        base[q] += all;
+      }
+      __syncthreads();
+      if (v) {
+        for (int d = 0; d < 3; ++d) {
+          mn[d] = fminf(mn[d], blk[b].bmin[d]);
+          mx[d] = fmaxf(mx[d], blk[b].bmax[d]);
+        }
       }
     }
-    if (l == 63)
-      for (int q = 0; q < 4; ++q) wtot[w][q] = inc[q];
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      float before = 0.f, all = 0.f;
-      for (int k = 0; k < 4; ++k) {
-        if (k < w) before += wtot[k][q];
-        all += wtot[k][q];
-      }
-      if (v) blk[b].ein[q] = base[q] + before + (inc[q] - a[q]);
-      base[q] += all;
+    for (int d = 0; d < 3; ++d) {
+      mn[d] = wave_min_f32(mn[d]);
+      mx[d] = wave_max_f32(mx[d]);
     }
-    __syncthreads();
-    if (v) {
+    if (l == 0)
       for (int d = 0; d < 3; ++d) {
-        mn[d] = fminf(mn[d], blk[b].bmin[d]);
-        mx[d] = fmaxf(mx[d], blk[b].bmax[d]);
+        wmn[w][d] = mn[d];
+        wmx[w][d] = mx[d];
       }
-    }
-  }
-  for (int d = 0; d < 3; ++d) {
-    mn[d] = wave_min_f32(mn[d]);
-    mx[d] = wave_max_f32(mx[d]);
-  }
-  if (l == 0)
-    for (int d = 0; d < 3; ++d) {
-      wmn[w][d] = mn[d];
-      wmx[w][d] = mx[d];
-    }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int d = 0; d < 3; ++d) {
-      float a = wmn[0][d], b = wmx[0][d];
-      for (int k = 1; k < 4; ++k) {
-        a = fminf(a, wmn[k][d]);
-        b = fmaxf(b, wmx[k][d]);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int d = 0; d < 3; ++d) {
+        float a = wmn[0][d], b = wmx[0][d];
+        for (int k = 1; k < 4; ++k) {
+          a = fminf(a, wmn[k][d]);
+          b = fmaxf(b, wmx[k][d]);
+        }
+        nodes[t].xmin[d] = a;
+        nodes[t].xmax[d] = b;
       }
-      nodes[t].xmin[d] = a;
-      nodes[t].xmax[d] = b;
     }
   }
 }
@@ -304,32 +367,33 @@ __global__ void __launch_bounds__(64)
 k_cm_blk_maps(CmBlock *blk, const Counters *ctr, const float *X,
               const float *Y, const float *Z, const float *M) {
   __shared__ float buf[64 * (CM_PER + 1)];
-  int64_t b = blockIdx.x;
-  if (b >= ctr->cm_blocks) return;
   const int q = blockIdx.y;  // one wave per (block, sum)
-  CmBlock &B = blk[b];
-  const int l = lane_id();
-  const int64_t start = B.start, len = B.len;
-  const int32_t E = grid_exp_guess(B.ein[q]);
-  sd_map acc = sd_map_identity();
-  if (E == INT32_MIN) {
-    acc.bad = 1;
-  } else {
+  for (int64_t b = blockIdx.x; b < ctr->cm_blocks; b += gridDim.x) {
+    CmBlock &B = blk[b];
+    const int l = lane_id();
+    const int64_t start = B.start, len = B.len;
+    const int32_t E = grid_exp_guess(B.ein[q]);
+    sd_map acc = sd_map_identity();
+    if (E == INT32_MIN) {
+      acc.bad = 1;
+    } else {
+      __syncthreads();  // the previous block's maps have read buf
 #pragma unroll
-    for (int k = l; k < CM_BLK; k += 64) {
-      float v = k < len ? addend(q, X, Y, Z, M, start + k) : 0.f;
-      buf[(k / CM_PER) * (CM_PER + 1) + (k % CM_PER)] = v;
+      for (int k = l; k < CM_BLK; k += 64) {
+        float v = k < len ? addend(q, X, Y, Z, M, start + k) : 0.f;
+        buf[(k / CM_PER) * (CM_PER + 1) + (k % CM_PER)] = v;
+      }
+      __syncthreads();
+      for (int k = 0; k < CM_PER; ++k)
+        acc = sd_map_compose(acc, sd_map_of(buf[l * (CM_PER + 1) + k], E));
+      acc = wave_compose_maps(acc);
     }
-    __syncthreads();
-    for (int k = 0; k < CM_PER; ++k)
-      acc = sd_map_compose(acc, sd_map_of(buf[l * (CM_PER + 1) + k], E));
-    acc = wave_compose_maps(acc);
-  }
-  if (l == 0) {
-    B.d0[q] = acc.d0;
-    B.d1[q] = acc.d1;
-    B.bad[q] = acc.bad;
-    B.E[q] = E;
+    if (l == 0) {
+      B.d0[q] = acc.d0;
+      B.d1[q] = acc.d1;
+      B.bad[q] = acc.bad;
+      B.E[q] = E;
+    }
   }
 }
 
@@ -338,33 +402,33 @@ k_cm_blk_maps(CmBlock *blk, const Counters *ctr, const float *X,
 // block; waves of other blocks return at once).
 __global__ void __launch_bounds__(64)
 k_cm_win_maps(const BNode *nodes, CmBlock *blk, const Counters *ctr) {
-  const int64_t b = blockIdx.x;
-  if (b >= ctr->cm_blocks) return;
-  const int64_t t = blk[b].node;
-  const int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
-  if ((b - b0) % 64 != 0) return;
-  const int l = lane_id();
-  const int cnt = (int)min<int64_t>(64, b0 + nb - b);
-  for (int q = 0; q < 4; ++q) {
-    sd_map m = sd_map_identity();
-    int32_t E = 0;
-    int bad = 1;
-    if (l < cnt) {
-      const CmBlock &B = blk[b + l];
-      m.d0 = B.d0[q];
-      m.d1 = B.d1[q];
-      E = B.E[q];
-      bad = B.bad[q];
-    }
-    const int32_t E0 = readlane_i32(E, 0);
-    const int ok = !__any(l < cnt && (bad || E != E0));
-    if (l >= cnt) m = sd_map_identity();
-    m = wave_compose_maps(m);
-    if (l == 0) {
-      blk[b].wd0[q] = m.d0;
-      blk[b].wd1[q] = m.d1;
-      blk[b].wE[q] = E0;
-      blk[b].wok[q] = ok && !m.bad;
+  for (int64_t b = blockIdx.x; b < ctr->cm_blocks; b += gridDim.x) {
+    const int64_t t = blk[b].node;
+    const int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
+    if ((b - b0) % 64 != 0) continue;
+    const int l = lane_id();
+    const int cnt = (int)min<int64_t>(64, b0 + nb - b);
+    for (int q = 0; q < 4; ++q) {
+      sd_map m = sd_map_identity();
+      int32_t E = 0;
+      int bad = 1;
+      if (l < cnt) {
+        const CmBlock &B = blk[b + l];
+        m.d0 = B.d0[q];
+        m.d1 = B.d1[q];
+        E = B.E[q];
+        bad = B.bad[q];
+      }
+      const int32_t E0 = readlane_i32(E, 0);
+      const int ok = !__any(l < cnt && (bad || E != E0));
+      if (l >= cnt) m = sd_map_identity();
+      m = wave_compose_maps(m);
+      if (l == 0) {
+        blk[b].wd0[q] = m.d0;
+        blk[b].wd1[q] = m.d1;
+        blk[b].wE[q] = E0;
+        blk[b].wok[q] = ok && !m.bad;
+      }
     }
   }
 }
@@ -378,82 +442,82 @@ k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, const CmBlock *blk,
                const float *X, const float *Y, const float *Z,
                const float *M, Counters *ctr, sd_t *cms) {
   __shared__ float buf[CM_BLK];
-  int64_t t = lb + blockIdx.x;
-  if (t >= le) return;
-  if (nodes[t].count <= CM_SMALL) return;
   const int q = blockIdx.y;  // one wave per (node, sum): x, y, z, mass
-  const int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
   const int l = lane_id();
-  sd_t Sq = sd_zero();
-  for (int64_t bb = 0; bb < nb; bb += 64) {
-    // the whole window at once when its blocks share a grid and S stays in it
-    {
-      const CmBlock &W = blk[b0 + bb];
-      sd_map g;
-      g.d0 = W.wd0[q];
-      g.d1 = W.wd1[q];
-      g.bad = !W.wok[q];
-      if (sd_map_apply(&Sq, g, W.wE[q])) continue;
-    }
-    uint64_t md0 = 0, md1 = 0;
-    int mbad = 1, mE = 0;
-    if (bb + l < nb) {
-      const CmBlock &B = blk[b0 + bb + l];
-      md0 = B.d0[q];
-      md1 = B.d1[q];
-      mbad = B.bad[q];
-      mE = B.E[q];
-    }
-    const int cnt = (int)min<int64_t>(64, nb - bb);
-    {
-      for (int i = 0; i < cnt;) {  // uniform
-        if (Sq.m != 0 && !Sq.s) {
-          // ordered scan of the block maps from i built for S's binade; the
-          // longest prefix that keeps S in the binade is applied at once
-          const int32_t E = Sq.e;
-          sd_map m = sd_map_identity();
-          if (l >= i && l < cnt) {
-            m.d0 = md0;
-            m.d1 = md1;
-            m.bad = mbad || mE != E;
-          }
-          for (int o = 1; o < 64; o <<= 1) {
-            sd_map h;
-            h.d0 = (uint64_t)__shfl_up((unsigned long long)m.d0, o);
-            h.d1 = (uint64_t)__shfl_up((unsigned long long)m.d1, o);
-            h.bad = __shfl_up(m.bad, o);
-            if (l >= o) m = sd_map_compose(h, m);
-          }
-          sd_t S2 = Sq;
-          const bool ok = l >= i && l < cnt && sd_map_apply(&S2, m, E);
-          const uint64_t bal = ((uint64_t)__ballot(ok) >> i) | (cnt - i < 64 ? ~0ull << (cnt - i) : 0ull);
-          const int j = min(cnt, i + ((~bal == 0ull) ? 64 - i : (int)__builtin_ctzll(~bal)));
+  for (int64_t t = lb + blockIdx.x; t < le; t += gridDim.x) {
+    if (nodes[t].count <= CM_SMALL) continue;
+    const int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
+    sd_t Sq = sd_zero();
+    for (int64_t bb = 0; bb < nb; bb += 64) {
+      // the whole window at once when its blocks share a grid and S stays in it
+      {
+        const CmBlock &W = blk[b0 + bb];
+        sd_map g;
+        g.d0 = W.wd0[q];
+        g.d1 = W.wd1[q];
+        g.bad = !W.wok[q];
+        if (sd_map_apply(&Sq, g, W.wE[q])) continue;
+      }
+      uint64_t md0 = 0, md1 = 0;
+      int mbad = 1, mE = 0;
+      if (bb + l < nb) {
+        const CmBlock &B = blk[b0 + bb + l];
+        md0 = B.d0[q];
+        md1 = B.d1[q];
+        mbad = B.bad[q];
+        mE = B.E[q];
+      }
+      const int cnt = (int)min<int64_t>(64, nb - bb);
+      {
+        for (int i = 0; i < cnt;) {  // uniform
+          if (Sq.m != 0 && !Sq.s) {
+            // ordered scan of the block maps from i built for S's binade; the
+            // longest prefix that keeps S in the binade is applied at once
+            const int32_t E = Sq.e;
+            sd_map m = sd_map_identity();
+            if (l >= i && l < cnt) {
+              m.d0 = md0;
+              m.d1 = md1;
+              m.bad = mbad || mE != E;
+            }
+            for (int o = 1; o < 64; o <<= 1) {
+              sd_map h;
+              h.d0 = (uint64_t)__shfl_up((unsigned long long)m.d0, o);
+              h.d1 = (uint64_t)__shfl_up((unsigned long long)m.d1, o);
+              h.bad = __shfl_up(m.bad, o);
+              if (l >= o) m = sd_map_compose(h, m);
+            }
+            sd_t S2 = Sq;
+            const bool ok = l >= i && l < cnt && sd_map_apply(&S2, m, E);
+            const uint64_t bal = ((uint64_t)__ballot(ok) >> i) | (cnt - i < 64 ? ~0ull << (cnt - i) : 0ull);
+            const int j = min(cnt, i + ((~bal == 0ull) ? 64 - i : (int)__builtin_ctzll(~bal)));
 #ifdef HSR_CM_STATS
-          if (l == 0) atomicAdd((unsigned long long *)&ctr->pad[0], 1ull);
+            if (l == 0) atomicAdd((unsigned long long *)&ctr->pad[0], 1ull);
 #endif
-          if (j > i) {
-            Sq.m = readlane_u64(S2.m, j - 1);
-            Sq.e = readlane_i32(S2.e, j - 1);
-            Sq.s = readlane_i32(S2.s, j - 1);
-            i = j;
-            continue;
+            if (j > i) {
+              Sq.m = readlane_u64(S2.m, j - 1);
+              Sq.e = readlane_i32(S2.e, j - 1);
+              Sq.s = readlane_i32(S2.s, j - 1);
+              i = j;
+              continue;
+            }
           }
+          // block i does not fit one grid: its addends, exactly, from LDS
+#ifdef HSR_CM_STATS
+          if (l == 0) atomicAdd((unsigned long long *)&ctr->pad[1], 1ull);
+#endif
+          const CmBlock &B = blk[b0 + bb + i];
+          const int64_t start = B.start, len = B.len;
+          __syncthreads();
+          stage_addends<8>(buf, q, X, Y, Z, M, start, len);
+          __syncthreads();
+          sd_block(Sq, buf, (int)len);
+          ++i;
         }
-        // block i does not fit one grid: its addends, exactly, from LDS
-#ifdef HSR_CM_STATS
-        if (l == 0) atomicAdd((unsigned long long *)&ctr->pad[2], 1ull);
-#endif
-        const CmBlock &B = blk[b0 + bb + i];
-        const int64_t start = B.start, len = B.len;
-        __syncthreads();
-        stage_addends<8>(buf, q, X, Y, Z, M, start, len);
-        __syncthreads();
-        sd_block(Sq, buf, (int)len);
-        ++i;
       }
     }
+    if (l == 0) cms[4 * (t - lb) + q] = Sq;
   }
-  if (l == 0) cms[4 * (t - lb) + q] = Sq;
 }
 
 // ------------------------------------------------------------ classify
@@ -496,11 +560,11 @@ __global__ void k_classify(BNode *nodes, int64_t lb, int64_t le, int64_t nd,
 __global__ void __launch_bounds__(64)
 k_finalize_leaves(const BNode *nodes, int64_t lb, int64_t le, int kind,
                   const int32_t *P, int32_t *Pfin) {
-  int64_t t = lb + blockIdx.x;
-  if (t >= le) return;
-  if (nodes[t].kind != kind) return;
-  int64_t off = nodes[t].offset, cnt = nodes[t].count;
-  for (int64_t k = lane_id(); k < cnt; k += 64) Pfin[off + k] = P[off + k];
+  for (int64_t t = lb + blockIdx.x; t < le; t += gridDim.x) {
+    if (nodes[t].kind != kind) continue;
+    int64_t off = nodes[t].offset, cnt = nodes[t].count;
+    for (int64_t k = lane_id(); k < cnt; k += 64) Pfin[off + k] = P[off + k];
+  }
 }
 
 // ------------------------------------------------------------ partition
@@ -547,46 +611,47 @@ k_part_plan(BNode *nodes, int64_t lb, int64_t le, PartBlock *pb,
 __global__ void __launch_bounds__(256)
 k_part_count(const BNode *nodes, PartBlock *pb, const Counters *ctr,
              const float *X, const float *Y, const float *Z) {
-  int64_t b = blockIdx.x;
-  if (b >= ctr->part_blocks) return;
-  const PartBlock &B = pb[b];
-  const BNode &n = nodes[B.node];
-  const float *key = coord(n.split_d, X, Y, Z);
-  float pv = n.xc[n.split_d];
-  int c = 0;
-  for (int k = threadIdx.x; k < B.len; k += 256) c += key[B.start + k] < pv;
-  __shared__ int red[4];
-  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-  if (lane_id() == 0) red[threadIdx.x >> 6] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) pb[b].cnt = red[0] + red[1] + red[2] + red[3];
+  for (int64_t b = blockIdx.x; b < ctr->part_blocks; b += gridDim.x) {
+    const PartBlock &B = pb[b];
+    const BNode &n = nodes[B.node];
+    const float *key = coord(n.split_d, X, Y, Z);
+    float pv = n.xc[n.split_d];
+    int c = 0;
+    for (int k = threadIdx.x; k < B.len; k += 256) c += key[B.start + k] < pv;
+    __shared__ int red[4];
+    __syncthreads();  // thread 0 has read the previous block's red
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if (lane_id() == 0) red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) pb[b].cnt = red[0] + red[1] + red[2] + red[3];
+  }
 }
 
 // In-order exclusive scan of block counts per node; degenerate splits
 // (is == 0 or is == count, RCBForceTree.cxx:727-729) are marked.
 __global__ void __launch_bounds__(64)
 k_part_scan(BNode *nodes, int64_t lb, int64_t le, PartBlock *pb) {
-  int64_t t = lb + blockIdx.x;
-  if (t >= le) return;
-  if (nodes[t].kind != NODE_SPLIT) return;
-  int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
-  int64_t run = 0;
-  int l = lane_id();
-  for (int64_t c0 = 0; c0 < nb; c0 += 64) {
-    int64_t b = c0 + l;
-    int64_t v = b < nb ? pb[b0 + b].cnt : 0;
-    // inclusive scan
-    int64_t s = v;
-    for (int o = 1; o < 64; o <<= 1) {
-      int64_t u = (int64_t)__shfl_up((long long)s, o);
-      if (l >= o) s += u;
+  for (int64_t t = lb + blockIdx.x; t < le; t += gridDim.x) {
+    if (nodes[t].kind != NODE_SPLIT) continue;
+    int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
+    int64_t run = 0;
+    int l = lane_id();
+    for (int64_t c0 = 0; c0 < nb; c0 += 64) {
+      int64_t b = c0 + l;
+      int64_t v = b < nb ? pb[b0 + b].cnt : 0;
+      // inclusive scan
+      int64_t s = v;
+      for (int o = 1; o < 64; o <<= 1) {
+        int64_t u = (int64_t)__shfl_up((long long)s, o);
+        if (l >= o) s += u;
+      }
+      if (b < nb) pb[b0 + b].base = run + s - v;
+      run += (int64_t)__shfl((long long)s, 63);
     }
-    if (b < nb) pb[b0 + b].base = run + s - v;
-    run += (int64_t)__shfl((long long)s, 63);
-  }
-  if (l == 0) {
-    nodes[t].is = run;
-    if (run == 0 || run == nodes[t].count) nodes[t].kind = NODE_DEGEN;
+    if (l == 0) {
+      nodes[t].is = run;
+      if (run == 0 || run == nodes[t].count) nodes[t].kind = NODE_DEGEN;
+    }
   }
 }
 
@@ -597,43 +662,43 @@ __global__ void __launch_bounds__(256)
 k_part_rank(const BNode *nodes, const PartBlock *pb, const Counters *ctr,
             const float *X, const float *Y, const float *Z, int32_t *idx,
             int32_t *rights, int32_t *dest) {
-  int64_t b = blockIdx.x;
-  if (b >= ctr->part_blocks) return;
-  const PartBlock &B = pb[b];
-  const BNode &n = nodes[B.node];
-  const float *key = coord(n.split_d, X, Y, Z);
-  float pv = n.xc[n.split_d];
-  int64_t off = n.offset, is = n.is;
-  __shared__ int wsum[4];
-  int64_t run = B.base;
-  int w = threadIdx.x >> 6;
-  for (int r = 0; r < PB / 256; ++r) {
-    int k = r * 256 + threadIdx.x;
-    bool valid = k < B.len;
-    int64_t e = B.start + k;
-    bool f = valid && key[e] < pv;
-    uint64_t bal = __ballot(f);
-    int pre = __popcll(bal & lanemask_lt());
-    if (lane_id() == 0) wsum[w] = __popcll(bal);
-    __syncthreads();
-    int wbase = 0, tot = 0;
-    for (int i = 0; i < 4; ++i) {
-      if (i < w) wbase += wsum[i];
-      tot += wsum[i];
-    }
-    if (valid) {
-      int64_t p = e - off;
-      int64_t lefts_before = run + wbase + pre;
-      if (f) {
-        idx[off + lefts_before] = (int32_t)p;
-        dest[e] = (int32_t)lefts_before;
-      } else {
-        rights[off + (p - lefts_before)] = (int32_t)p;
-        dest[e] = p >= is ? (int32_t)p : -1;
+  for (int64_t b = blockIdx.x; b < ctr->part_blocks; b += gridDim.x) {
+    const PartBlock &B = pb[b];
+    const BNode &n = nodes[B.node];
+    const float *key = coord(n.split_d, X, Y, Z);
+    float pv = n.xc[n.split_d];
+    int64_t off = n.offset, is = n.is;
+    __shared__ int wsum[4];
+    int64_t run = B.base;
+    int w = threadIdx.x >> 6;
+    for (int r = 0; r < PB / 256; ++r) {
+      int k = r * 256 + threadIdx.x;
+      bool valid = k < B.len;
+      int64_t e = B.start + k;
+      bool f = valid && key[e] < pv;
+      uint64_t bal = __ballot(f);
+      int pre = __popcll(bal & lanemask_lt());
+      if (lane_id() == 0) wsum[w] = __popcll(bal);
+      __syncthreads();
+      int wbase = 0, tot = 0;
+      for (int i = 0; i < 4; ++i) {
+        if (i < w) wbase += wsum[i];
+        tot += wsum[i];
       }
+      if (valid) {
+        int64_t p = e - off;
+        int64_t lefts_before = run + wbase + pre;
+        if (f) {
+          idx[off + lefts_before] = (int32_t)p;
+          dest[e] = (int32_t)lefts_before;
+        } else {
+          rights[off + (p - lefts_before)] = (int32_t)p;
+          dest[e] = p >= is ? (int32_t)p : -1;
+        }
+      }
+      run += tot;
+      __syncthreads();
     }
-    run += tot;
-    __syncthreads();
   }
 }
 
@@ -647,36 +712,36 @@ __global__ void __launch_bounds__(256)
 k_part_chain(const BNode *nodes, const PartBlock *pb, const Counters *ctrc,
              const int32_t *idx, const int32_t *rights, int32_t *dest,
              Counters *ctr) {
-  int64_t b = blockIdx.x;
-  if (b >= ctrc->part_blocks) return;
-  const PartBlock &B = pb[b];
-  const BNode &n = nodes[B.node];
-  const int64_t off = n.offset, is = n.is, cnt = n.count, nr = cnt - is;
-  for (int k = threadIdx.x; k < B.len; k += 256) {
-    int64_t e = B.start + k;
-    if (dest[e] != -1) continue;
-    int64_t j = e - off;
-    bool done = false;
-    for (int it = 0; it < MAXHOP; ++it) {
-      const int64_t f = idx[off + j];  // f > j
-      if (f >= is) {
-        dest[e] = (int32_t)f;
-        done = true;
-        break;
+  for (int64_t b = blockIdx.x; b < ctrc->part_blocks; b += gridDim.x) {
+    const PartBlock &B = pb[b];
+    const BNode &n = nodes[B.node];
+    const int64_t off = n.offset, is = n.is, cnt = n.count, nr = cnt - is;
+    for (int k = threadIdx.x; k < B.len; k += 256) {
+      int64_t e = B.start + k;
+      if (dest[e] != -1) continue;
+      int64_t j = e - off;
+      bool done = false;
+      for (int it = 0; it < MAXHOP; ++it) {
+        const int64_t f = idx[off + j];  // f > j
+        if (f >= is) {
+          dest[e] = (int32_t)f;
+          done = true;
+          break;
+        }
+        const int64_t step = f - j;  // keys >= pivot below f
+        const int64_t lim = step < nr ? rights[off + step] : cnt;
+        const int64_t v = f + ((is - f + step - 1) / step) * step;
+        if (v < lim) {
+          dest[e] = (int32_t)v;
+          done = true;
+          break;
+        }
+        j = f + ((lim - 1 - f) / step) * step;
       }
-      const int64_t step = f - j;  // keys >= pivot below f
-      const int64_t lim = step < nr ? rights[off + step] : cnt;
-      const int64_t v = f + ((is - f + step - 1) / step) * step;
-      if (v < lim) {
-        dest[e] = (int32_t)v;
-        done = true;
-        break;
+      if (!done) {
+        dest[e] = -2 - (int32_t)j;  // unresolved; continue from j
+        atomicAdd((unsigned long long *)&ctr->unresolved, 1ull);
       }
-      j = f + ((lim - 1 - f) / step) * step;
-    }
-    if (!done) {
-      dest[e] = -2 - (int32_t)j;  // unresolved; continue from j
-      atomicAdd((unsigned long long *)&ctr->unresolved, 1ull);
     }
   }
 }
@@ -686,28 +751,28 @@ k_part_chain(const BNode *nodes, const PartBlock *pb, const Counters *ctrc,
 __global__ void __launch_bounds__(256)
 k_part_jump_init(const BNode *nodes, const PartBlock *pb, const Counters *ctr,
                  const int32_t *idx, int32_t *jmp) {
-  int64_t b = blockIdx.x;
-  if (b >= ctr->part_blocks) return;
-  const PartBlock &B = pb[b];
-  const BNode &n = nodes[B.node];
-  for (int k = threadIdx.x; k < B.len; k += 256) {
-    int64_t p = B.start + k - n.offset;
-    if (p < n.is) jmp[n.offset + p] = idx[n.offset + p];
+  for (int64_t b = blockIdx.x; b < ctr->part_blocks; b += gridDim.x) {
+    const PartBlock &B = pb[b];
+    const BNode &n = nodes[B.node];
+    for (int k = threadIdx.x; k < B.len; k += 256) {
+      int64_t p = B.start + k - n.offset;
+      if (p < n.is) jmp[n.offset + p] = idx[n.offset + p];
+    }
   }
 }
 
 __global__ void __launch_bounds__(256)
 k_part_jump_round(const BNode *nodes, const PartBlock *pb,
                   const Counters *ctr, int32_t *jmp) {
-  int64_t b = blockIdx.x;
-  if (b >= ctr->part_blocks) return;
-  const PartBlock &B = pb[b];
-  const BNode &n = nodes[B.node];
-  for (int k = threadIdx.x; k < B.len; k += 256) {
-    int64_t p = B.start + k - n.offset;
-    if (p < n.is) {
-      int32_t j = jmp[n.offset + p];
-      if (j < n.is) jmp[n.offset + p] = jmp[n.offset + j];
+  for (int64_t b = blockIdx.x; b < ctr->part_blocks; b += gridDim.x) {
+    const PartBlock &B = pb[b];
+    const BNode &n = nodes[B.node];
+    for (int k = threadIdx.x; k < B.len; k += 256) {
+      int64_t p = B.start + k - n.offset;
+      if (p < n.is) {
+        int32_t j = jmp[n.offset + p];
+        if (j < n.is) jmp[n.offset + p] = jmp[n.offset + j];
+      }
     }
   }
 }
@@ -715,14 +780,14 @@ k_part_jump_round(const BNode *nodes, const PartBlock *pb,
 __global__ void __launch_bounds__(256)
 k_part_jump_final(const BNode *nodes, const PartBlock *pb,
                   const Counters *ctr, const int32_t *jmp, int32_t *dest) {
-  int64_t b = blockIdx.x;
-  if (b >= ctr->part_blocks) return;
-  const PartBlock &B = pb[b];
-  const BNode &n = nodes[B.node];
-  for (int k = threadIdx.x; k < B.len; k += 256) {
-    int64_t e = B.start + k;
-    int32_t d = dest[e];
-    if (d <= -2) dest[e] = jmp[n.offset + (-2 - d)];
+  for (int64_t b = blockIdx.x; b < ctr->part_blocks; b += gridDim.x) {
+    const PartBlock &B = pb[b];
+    const BNode &n = nodes[B.node];
+    for (int k = threadIdx.x; k < B.len; k += 256) {
+      int64_t e = B.start + k;
+      int32_t d = dest[e];
+      if (d <= -2) dest[e] = jmp[n.offset + (-2 - d)];
+    }
   }
 }
 
@@ -731,18 +796,18 @@ k_part_scatter(const BNode *nodes, const PartBlock *pb, const Counters *ctr,
                const int32_t *dest, const float *X, const float *Y,
                const float *Z, const float *M, const int32_t *P, float *X2,
                float *Y2, float *Z2, float *M2, int32_t *P2) {
-  int64_t b = blockIdx.x;
-  if (b >= ctr->part_blocks) return;
-  const PartBlock &B = pb[b];
-  int64_t off = nodes[B.node].offset;
-  for (int k = threadIdx.x; k < B.len; k += 256) {
-    int64_t e = B.start + k;
-    int64_t o = off + dest[e];
-    X2[o] = X[e];
-    Y2[o] = Y[e];
-    Z2[o] = Z[e];
-    M2[o] = M[e];
-    P2[o] = P[e];
+  for (int64_t b = blockIdx.x; b < ctr->part_blocks; b += gridDim.x) {
+    const PartBlock &B = pb[b];
+    int64_t off = nodes[B.node].offset;
+    for (int k = threadIdx.x; k < B.len; k += 256) {
+      int64_t e = B.start + k;
+      int64_t o = off + dest[e];
+      X2[o] = X[e];
+      Y2[o] = Y[e];
+      Z2[o] = Z[e];
+      M2[o] = M[e];
+      P2[o] = P[e];
+    }
   }
 }
 
@@ -1144,15 +1209,25 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     CmBlock *cmb = dptr<CmBlock>(S.blocks);
     HSR_HIP(dbuf_reserve(S.cms, sizeof(sd_t) * 4 * (size_t)nl));
     sd_t *cms = dptr<sd_t>(S.cms);
+    // Kernels over the big nodes (count > CM_SMALL) and their blocks loop
+    // over a grid of at most gbig workgroups: deep levels hold many nodes
+    // but few big ones, and empty workgroups would cost their dispatch.
+    const unsigned gbig = (unsigned)std::max(1, c->num_cus) * 32;
+    const unsigned gnl = (unsigned)std::min<int64_t>(nl, gbig);
+    HSR_HIP(dbuf_reserve(S.cmfail, sizeof(int32_t) * 4 * (size_t)nl));
+    int32_t *cm_list = dptr<int32_t>(S.cmfail);
     k_cm_small<<<dim3((unsigned)nl, 4), 64, 0, st>>>(nodes, lb, le, X[cur], Y[cur], Z[cur], Mm[cur],
-                                                    cms);
-    k_cm_plan<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, cmb, ctr);
+                                                    cms, cm_list, ctr);
+    k_cm_small_ordered<<<std::min<unsigned>((unsigned)std::max<int64_t>(1, nl), 4 * gbig), 64, 0, st>>>(
+        nodes, lb, X[cur], Y[cur], Z[cur], Mm[cur], cms, cm_list, ctr);
+    k_cm_plan<<<gnl, 64, 0, st>>>(nodes, lb, le, cmb, ctr);
     unsigned cmg = (unsigned)std::max<int64_t>(1, active / CM_BLK + active / CM_SMALL + 1);
+    cmg = std::min(cmg, gbig);
     k_cm_blk_stats<<<cmg, 64, 0, st>>>(cmb, ctr, X[cur], Y[cur], Z[cur], Mm[cur]);
-    k_cm_blk_prefix<<<(unsigned)nl, 256, 0, st>>>(nodes, lb, le, cmb);
+    k_cm_blk_prefix<<<gnl, 256, 0, st>>>(nodes, lb, le, cmb);
     k_cm_blk_maps<<<dim3(cmg, 4), 64, 0, st>>>(cmb, ctr, X[cur], Y[cur], Z[cur], Mm[cur]);
     k_cm_win_maps<<<cmg, 64, 0, st>>>(nodes, cmb, ctr);
-    k_cm_blk_chain<<<dim3((unsigned)nl, 4), 64, 0, st>>>(nodes, lb, le, cmb, X[cur], Y[cur], Z[cur],
+    k_cm_blk_chain<<<dim3(gnl, 4), 64, 0, st>>>(nodes, lb, le, cmb, X[cur], Y[cur], Z[cur],
                                                         Mm[cur], ctr, cms);
     k_cm_finish<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, cms);
     // ---- classify, finalize plain leaves
@@ -1163,7 +1238,7 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     HSR_HIP(dbuf_reserve(S.pblocks, sizeof(PartBlock) * pcap));
     PartBlock *pb = dptr<PartBlock>(S.pblocks);
     k_part_plan<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, pb, ctr);
-    unsigned pg = (unsigned)std::max<int64_t>(1, pcap);
+    const unsigned pg = (unsigned)std::max<int64_t>(1, pcap);
     k_part_count<<<pg, 256, 0, st>>>(nodes, pb, ctr, X[cur], Y[cur], Z[cur]);
     k_part_scan<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, pb);
     // children depend only on the partition counts: create them now so a
@@ -1184,9 +1259,9 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_DEGEN, P[cur ^ 1], Pfin);
     HSR_HIP(hipGetLastError());
 #ifdef HSR_CM_STATS
-    fprintf(stderr, "[cm] level %d nodes %lld blocks %lld scans %lld single %lld fallback %lld\n", depth,
-            (long long)nl, (long long)hc.cm_blocks, (long long)hc.pad[0], (long long)hc.pad[1],
-            (long long)hc.pad[2]);
+    fprintf(stderr, "[cm] level %d nodes %lld blocks %lld scans %lld fallback %lld small ordered %lld\n",
+            depth, (long long)nl, (long long)hc.cm_blocks, (long long)hc.pad[0], (long long)hc.pad[1],
+            (long long)hc.cm_fail);
 #endif
     T.level_start.push_back(le + 2 * hc.nsplit);
     active = hc.active_next;

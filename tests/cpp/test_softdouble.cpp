@@ -29,6 +29,7 @@ static float rand_float(std::mt19937_64 &g, int mode) {
             if (((b >> 23) & 255) == 255) b &= 0x7f7fffffu;
             float f; memcpy(&f, &b, 4); return (g() & 1) ? -f : f; }
   case 4: return (float)((g() % 4096) * 0.25);                  // many ties
+  case 6: return (float)(1 + g() % 8);                          // unit-like masses
   default: return (float)(u(g) * 4.0);
   }
 }
@@ -53,7 +54,7 @@ int main() {
   // 2. chunk paths (exact, fixed grid) agree with serial when they apply
   long applied_exact = 0, applied_map = 0;
   for (int trial = 0; trial < 20000; ++trial) {
-    int mode = (trial % 3 == 0) ? 4 : (trial % 3 == 1 ? 0 : 5);
+    int mode = (trial % 4 == 0) ? 4 : (trial % 4 == 1 ? 0 : (trial % 4 == 2 ? 5 : 6));
     sd_t S = sd_zero();
     int pre = (int)(g() % 5000);
     for (int i = 0; i < pre; ++i) S = sd_add_f32(S, rand_float(g, mode));
