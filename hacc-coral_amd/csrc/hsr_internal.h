@@ -50,6 +50,8 @@ struct CmBlock {
   uint64_t wd0[4], wd1[4];
   int32_t wE[4];
   int32_t wok[4];
+  int32_t lsb[4];          // finest lsb exponent of the addends (INT32_MIN: a negative one)
+  uint64_t units[4];       // their sum in units of 2^lsb (~0: not an integer there)
 };
 
 struct PartBlock {
@@ -151,6 +153,7 @@ struct Scratch {
   DBuf a, b, c, d, e, f, g, h, cub;
   DBuf cms;        // tree build: the four exact sums of each node of a level
   DBuf cmfail;     // tree build: small-node sums left to the ordered path
+  DBuf cmx;        // tree build: big nodes' sums done by the integer path (bit q)
   DBuf x1, y1, z1, m1, p1, x0, y0, z0, m0, p0;
   DBuf blocks, pblocks, rights;
   DBuf counters, qctr;

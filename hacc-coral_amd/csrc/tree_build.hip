@@ -247,31 +247,75 @@ k_cm_plan(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk,
 __global__ void __launch_bounds__(64)
 k_cm_blk_stats(CmBlock *blk, const Counters *ctr, const float *X,
                const float *Y, const float *Z, const float *M) {
+  const int l = lane_id();
   for (int64_t b = blockIdx.x; b < ctr->cm_blocks; b += gridDim.x) {
     CmBlock &B = blk[b];
-    int l = lane_id();
+    const int64_t start = B.start;
+    const int len = (int)B.len;
+    // the block's elements in registers, CM_PER per lane, one load round
+    float xv[CM_PER], yv[CM_PER], zv[CM_PER], wv[CM_PER];
+#pragma unroll
+    for (int i = 0; i < CM_PER; ++i) {
+      const int k = i * 64 + l;
+      const int64_t e = start + (k < len ? k : 0);
+      wv[i] = M[e];
+      xv[i] = X[e];
+      yv[i] = Y[e];
+      zv[i] = Z[e];
+    }
     float s[4] = {0.f, 0.f, 0.f, 0.f};
     float mn[3] = {INFINITY, INFINITY, INFINITY};
     float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-#pragma unroll 4
-    for (int64_t k = l; k < B.len; k += 64) {
-      int64_t e = B.start + k;
-      float w = M[e], px = X[e], py = Y[e], pz = Z[e];
-      mn[0] = fminf(mn[0], px); mx[0] = fmaxf(mx[0], px);
-      mn[1] = fminf(mn[1], py); mx[1] = fmaxf(mx[1], py);
-      mn[2] = fminf(mn[2], pz); mx[2] = fmaxf(mx[2], pz);
-      s[0] += w * px;
-      s[1] += w * py;
-      s[2] += w * pz;
-      s[3] += w;
+    int32_t lsb[4] = {1 << 20, 1 << 20, 1 << 20, 1 << 20};
+#pragma unroll
+    for (int i = 0; i < CM_PER; ++i) {
+      if (i * 64 + l >= len) continue;
+      mn[0] = fminf(mn[0], xv[i]); mx[0] = fmaxf(mx[0], xv[i]);
+      mn[1] = fminf(mn[1], yv[i]); mx[1] = fmaxf(mx[1], yv[i]);
+      mn[2] = fminf(mn[2], zv[i]); mx[2] = fmaxf(mx[2], zv[i]);
+      const float w = wv[i];
+      const float a[4] = {w * xv[i], w * yv[i], w * zv[i], w};  // the addends of BGQCM.c
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        s[q] += a[q];
+        lsb[q] = a[q] < 0.f ? INT32_MIN : min(lsb[q], sd_lsb_f32(a[q]));
+      }
     }
-    for (int q = 0; q < 4; ++q) s[q] = wave_sum_f32(s[q]);
+    for (int q = 0; q < 4; ++q) {
+      s[q] = wave_sum_f32(s[q]);
+      lsb[q] = wave_min_i32(lsb[q]);
+    }
+    // the block's integer sums in units of its finest lsb; fewer than 1024
+    // addends below 2^53 units each
+    uint64_t un[4] = {0, 0, 0, 0};
+    int bad[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < CM_PER; ++i) {
+      if (i * 64 + l >= len) continue;
+      const float w = wv[i];
+      const float a[4] = {w * xv[i], w * yv[i], w * zv[i], w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (lsb[q] == INT32_MIN) continue;
+        const uint64_t uu = sd_units_f32(a[q], lsb[q]);
+        if (uu == ~0ull) bad[q] = 1;
+        else un[q] += uu;
+      }
+    }
+    for (int q = 0; q < 4; ++q) {
+      bad[q] = __any(bad[q]) || lsb[q] == INT32_MIN;
+      un[q] = wave_sum_u64(un[q]);
+    }
     for (int d = 0; d < 3; ++d) {
       mn[d] = wave_min_f32(mn[d]);
       mx[d] = wave_max_f32(mx[d]);
     }
     if (l == 0) {
-      for (int q = 0; q < 4; ++q) B.approx[q] = s[q];
+      for (int q = 0; q < 4; ++q) {
+        B.approx[q] = s[q];
+        B.lsb[q] = lsb[q];
+        B.units[q] = bad[q] ? ~0ull : un[q];
+      }
       for (int d = 0; d < 3; ++d) {
         B.bmin[d] = mn[d];
         B.bmax[d] = mx[d];
@@ -282,11 +326,28 @@ k_cm_blk_stats(CmBlock *blk, const Counters *ctr, const float *X,
 
 // Per big node (one workgroup of 4 waves): exclusive prefix of the
 // approximate block sums (only a guess of the running sum's binade, so any
-// summation order serves) and the node bbox (min/max: order-free).
+// summation order serves), the node bbox (min/max: order-free), and the
+// integer-sum path.  With t the finest lsb of a sum's addends over the node,
+// every addend a nonnegative multiple of 2^t and their total below 2^(t+53),
+// no partial sum of the sequential double additions rounds (each is such a
+// multiple below 2^(t+53)), so the sum is that integer times 2^t: written to
+// cms and flagged done (bit q of cmflags) for the ordered path to skip.
+__device__ __forceinline__ uint64_t units_add_sat(uint64_t s, uint64_t ub, int32_t tb, int32_t tq) {
+  if (s >= SD_M53 || ub == ~0ull) return SD_M53;
+  if (ub == 0) return s;
+  const int32_t sh = tb - tq;  // >= 0
+  if (sh >= 53 || (ub >> (53 - sh)) != 0) return SD_M53;
+  const uint64_t v = s + (ub << sh);
+  return v >= SD_M53 ? SD_M53 : v;
+}
+
 __global__ void __launch_bounds__(256)
-k_cm_blk_prefix(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk) {
+k_cm_blk_prefix(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk, int32_t *cmflags,
+                sd_t *cms) {
   __shared__ float wtot[4][4];
   __shared__ float wmn[4][3], wmx[4][3];
+  __shared__ int32_t wlsb[4][4];
+  __shared__ uint64_t wun[4][4];
   const int l = lane_id(), w = threadIdx.x >> 6;
   for (int64_t t = lb + blockIdx.x; t < le; t += gridDim.x) {
     if (nodes[t].count <= CM_SMALL) continue;
@@ -294,6 +355,7 @@ k_cm_blk_prefix(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk) {
     float base[4] = {0.f, 0.f, 0.f, 0.f};
     float mn[3] = {INFINITY, INFINITY, INFINITY};
     float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    int32_t lsb[4] = {1 << 20, 1 << 20, 1 << 20, 1 << 20};
     for (int64_t c0 = 0; c0 < nb; c0 += 256) {
       const int64_t b = b0 + c0 + threadIdx.x;
       const bool v = c0 + threadIdx.x < nb;
@@ -326,17 +388,37 @@ k_cm_blk_prefix(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk) {
           mn[d] = fminf(mn[d], blk[b].bmin[d]);
           mx[d] = fmaxf(mx[d], blk[b].bmax[d]);
         }
+        for (int q = 0; q < 4; ++q) lsb[q] = min(lsb[q], blk[b].lsb[q]);
       }
     }
     for (int d = 0; d < 3; ++d) {
       mn[d] = wave_min_f32(mn[d]);
       mx[d] = wave_max_f32(mx[d]);
     }
-    if (l == 0)
+    for (int q = 0; q < 4; ++q) lsb[q] = wave_min_i32(lsb[q]);
+    if (l == 0) {
       for (int d = 0; d < 3; ++d) {
         wmn[w][d] = mn[d];
         wmx[w][d] = mx[d];
       }
+      for (int q = 0; q < 4; ++q) wlsb[w][q] = lsb[q];
+    }
+    __syncthreads();
+    int32_t tq[4];
+    for (int q = 0; q < 4; ++q) {
+      tq[q] = wlsb[0][q];
+      for (int k = 1; k < 4; ++k) tq[q] = min(tq[q], wlsb[k][q]);
+    }
+    // the node's integer sums: block sums rescaled to 2^tq, saturating at 2^53
+    uint64_t su[4] = {0, 0, 0, 0};
+    for (int64_t c = threadIdx.x; c < nb; c += 256) {
+      const CmBlock &B = blk[b0 + c];
+      for (int q = 0; q < 4; ++q)
+        if (tq[q] != INT32_MIN) su[q] = units_add_sat(su[q], B.units[q], B.lsb[q], tq[q]);
+    }
+    for (int q = 0; q < 4; ++q) su[q] = wave_sum_u64(su[q]);  // 64 values below 2^54
+    if (l == 0)
+      for (int q = 0; q < 4; ++q) wun[w][q] = su[q];
     __syncthreads();
     if (threadIdx.x == 0) {
       for (int d = 0; d < 3; ++d) {
@@ -348,6 +430,15 @@ k_cm_blk_prefix(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk) {
         nodes[t].xmin[d] = a;
         nodes[t].xmax[d] = b;
       }
+      int32_t f = 0;
+      for (int q = 0; q < 4; ++q) {
+        const uint64_t tot = wun[0][q] + wun[1][q] + wun[2][q] + wun[3][q];
+        // a negative addend, all addends zero, or a total that rounds
+        if (tq[q] == INT32_MIN || tq[q] == (1 << 20) || tot >= SD_M53) continue;
+        cms[4 * (t - lb) + q] = sd_from_units(tot, tq[q]);
+        f |= 1 << q;
+      }
+      cmflags[t - lb] = f;
     }
   }
 }
@@ -365,11 +456,13 @@ __device__ __forceinline__ int32_t grid_exp_guess(float ein) {
 // lane maps are composed in lane order.
 __global__ void __launch_bounds__(64)
 k_cm_blk_maps(CmBlock *blk, const Counters *ctr, const float *X,
-              const float *Y, const float *Z, const float *M) {
+              const float *Y, const float *Z, const float *M, int64_t lb,
+              const int32_t *cmflags) {
   __shared__ float buf[64 * (CM_PER + 1)];
   const int q = blockIdx.y;  // one wave per (block, sum)
   for (int64_t b = blockIdx.x; b < ctr->cm_blocks; b += gridDim.x) {
     CmBlock &B = blk[b];
+    if (cmflags[B.node - lb] & (1 << q)) continue;  // summed exactly
     const int l = lane_id();
     const int64_t start = B.start, len = B.len;
     const int32_t E = grid_exp_guess(B.ein[q]);
@@ -440,12 +533,13 @@ k_cm_win_maps(const BNode *nodes, CmBlock *blk, const Counters *ctr) {
 __global__ void __launch_bounds__(64)
 k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, const CmBlock *blk,
                const float *X, const float *Y, const float *Z,
-               const float *M, Counters *ctr, sd_t *cms) {
+               const float *M, Counters *ctr, sd_t *cms, const int32_t *cmflags) {
   __shared__ float buf[CM_BLK];
   const int q = blockIdx.y;  // one wave per (node, sum): x, y, z, mass
   const int l = lane_id();
   for (int64_t t = lb + blockIdx.x; t < le; t += gridDim.x) {
     if (nodes[t].count <= CM_SMALL) continue;
+    if (cmflags[t - lb] & (1 << q)) continue;  // summed exactly
     const int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
     sd_t Sq = sd_zero();
     for (int64_t bb = 0; bb < nb; bb += 64) {
@@ -1224,11 +1318,13 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     unsigned cmg = (unsigned)std::max<int64_t>(1, active / CM_BLK + active / CM_SMALL + 1);
     cmg = std::min(cmg, gbig);
     k_cm_blk_stats<<<cmg, 64, 0, st>>>(cmb, ctr, X[cur], Y[cur], Z[cur], Mm[cur]);
-    k_cm_blk_prefix<<<gnl, 256, 0, st>>>(nodes, lb, le, cmb);
-    k_cm_blk_maps<<<dim3(cmg, 4), 64, 0, st>>>(cmb, ctr, X[cur], Y[cur], Z[cur], Mm[cur]);
+    HSR_HIP(dbuf_reserve(S.cmx, sizeof(int32_t) * (size_t)nl));
+    int32_t *cmflags = dptr<int32_t>(S.cmx);
+    k_cm_blk_prefix<<<gnl, 256, 0, st>>>(nodes, lb, le, cmb, cmflags, cms);
+    k_cm_blk_maps<<<dim3(cmg, 4), 64, 0, st>>>(cmb, ctr, X[cur], Y[cur], Z[cur], Mm[cur], lb, cmflags);
     k_cm_win_maps<<<cmg, 64, 0, st>>>(nodes, cmb, ctr);
     k_cm_blk_chain<<<dim3(gnl, 4), 64, 0, st>>>(nodes, lb, le, cmb, X[cur], Y[cur], Z[cur],
-                                                        Mm[cur], ctr, cms);
+                                                        Mm[cur], ctr, cms, cmflags);
     k_cm_finish<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, cms);
     // ---- classify, finalize plain leaves
     k_classify<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, nd, c->p.ppc, Mm[cur]);
