@@ -125,6 +125,7 @@ struct ListState {
   int64_t *multi = nullptr, *counts = nullptr, *leaf_pbase = nullptr;
   int32_t *leaf_nchunk = nullptr;
   DBuf stack;                 // DFS stacks
+  DBuf walk;                  // per (leaf, sibling walk): counts and offsets
   DBuf iperm;                 // int32 [n]: target order inside each listed leaf
 };
 

@@ -11,9 +11,10 @@
 //   * takes a not-accepted leaf's particles,
 //   * pushes the children of a not-accepted internal node whose L-infinity
 //     gap to the target leaf is <= fsrrmax,
-// and finally appends the leaf's own particles.  One thread walks one leaf
-// (two passes: count, then fill), with its LIFO stack in HBM laid out
-// level-major so the threads of a wave touch neighbouring words.
+// and finally appends the leaf's own particles.  One thread walks one
+// (leaf, ancestor sibling) subtree (two passes: count, then fill), with its
+// LIFO stack in HBM laid out level-major so the threads of a wave touch
+// neighbouring words.
 //
 // Compiled with -ffp-contract=off so dist2, l2, dtt2 and the corner
 // diagonals round exactly as the reference's x86 float code does.
@@ -80,31 +81,83 @@ __global__ void k_slot_box(int64_t nboxes, const int64_t *leaf_base,
     slot_box[k] = (int32_t)b;
 }
 
+// The walk of one leaf is a concatenation of independent walks.  Popping an
+// ancestor of the leaf pushes its children except the leaf (left, then
+// right), so below the ancestor at depth d - 1 the LIFO finishes the right
+// child's subtree before the left one's: the list is the walks of the
+// siblings of the leaf's ancestors (the other child at each depth d), right
+// siblings in increasing d, then left siblings in decreasing d, then the
+// leaf's own particles.  One thread walks one (leaf, d) sibling subtree.
+// Returns the sibling at depth d (-1 if none) and whether it is the right
+// child.
+__device__ __forceinline__ int64_t ancestor_sibling(const hsr_node *__restrict__ t, int64_t root,
+                                                    int64_t tl, int64_t toff, int64_t tend, int d,
+                                                    bool *right) {
+  int64_t a = root;
+  int64_t sib = -1;
+  *right = true;
+  // a leaf that is the root is no ancestor of itself: its walk starts at
+  // the root like any other (RCBForceTree.cxx:936-962) and takes the root's
+  // particles before the own ones are appended
+  if (root == tl) return d == 1 ? root : -1;
+  for (int i = 1; i <= d; ++i) {
+    if (a == tl) return -1;
+    const hsr_node &A = t[a];
+    const int64_t cl = A.cl, cr = A.cr;
+    const bool in_l = cl > 0 && t[cl].count > 0 && t[cl].offset <= toff &&
+                      tend <= t[cl].offset + t[cl].count;
+    if (in_l) {
+      sib = cr;
+      *right = true;
+      a = cl;
+    } else {
+      if (cr <= 0) return -1;
+      sib = cl;
+      *right = false;
+      a = cr;
+    }
+  }
+  if (sib <= 0 || t[sib].count <= 0) return -1;
+  return sib;
+}
+
 template <bool FILL>
 __global__ void __launch_bounds__(64)
 k_walk(const hsr_node *__restrict__ t, const int32_t *__restrict__ leaves,
        const int32_t *__restrict__ leaf_box, const int64_t *__restrict__ cbase,
-       int64_t nleaves, ListParams lp, int32_t *stack, int64_t *nseg_out,
-       int64_t *nent_out, const int64_t *seg_ptr, int64_t *seg_a,
-       int64_t *seg_b, int64_t *seg_e) {
-  int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (k >= nleaves) return;
+       int64_t nleaves, int ndepth, ListParams lp, int32_t *stack, int64_t *nseg_out,
+       int64_t *nent_out, uint8_t *side_out, const int64_t *seg_ptr, const int64_t *slot_seg,
+       const int64_t *slot_ent, int64_t *seg_a, int64_t *seg_b, int64_t *seg_e) {
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (gid >= nleaves * ndepth) return;
+  const int64_t k = gid / ndepth;
+  const int d = (int)(gid - k * ndepth) + 1;
   const int64_t tl = leaves[k];
   const int64_t root = cbase[leaf_box[k]];
   const float fsrrmax = lp.fsrrmax, fsrrmax2 = lp.fsrrmax2;
   const float tanOA = lp.tan_oa;
   const hsr_node &L = t[tl];
   const int64_t toff = L.offset, tend = L.offset + L.count;
+  bool right;
+  const int64_t sib = ancestor_sibling(t, root, tl, toff, tend, d, &right);
+  if (!FILL) side_out[gid] = right ? 1 : 0;
+  if (sib < 0) {
+    if (!FILL) {
+      nseg_out[gid] = 0;
+      nent_out[gid] = 0;
+    }
+    return;
+  }
   const float lxc0 = L.xc[0], lxc1 = L.xc[1], lxc2 = L.xc[2];
   float lmin[3] = {L.xmin[0], L.xmin[1], L.xmin[2]};
   float lmax[3] = {L.xmax[0], L.xmax[1], L.xmax[2]};
-  int32_t *stk = stack + k;
+  int32_t *stk = stack + gid;
   const int64_t sstride = lp.nthreads;
   int64_t sp = 0;
-  int64_t nseg = 0, nent = 0;
-  int64_t so = FILL ? seg_ptr[k] : 0;
+  int64_t nseg = 0, nent = FILL ? slot_ent[gid] : 0;
+  const int64_t so = FILL ? seg_ptr[k] + slot_seg[gid] : 0;
   bool dead = false;
-  stk[0] = (int32_t)root;
+  stk[0] = (int32_t)sib;
   sp = 1;
   // the LIFO never holds more than (tree depth + 2) nodes; the guard only
   // protects the stack buffer
@@ -205,13 +258,54 @@ k_walk(const hsr_node *__restrict__ t, const int32_t *__restrict__ leaves,
       }
     }
   }
-  EMIT(L.offset, L.count, L.count);
 #undef PUSH
 #undef EMIT
   if (!FILL) {
-    nseg_out[k] = nseg;
-    nent_out[k] = nent;
+    nseg_out[gid] = nseg;
+    nent_out[gid] = nent;
   }
+}
+
+// Per leaf: the order of its sibling walks (right siblings by increasing
+// depth, left ones by decreasing depth; depths past the leaf's hold empty
+// walks), their segment and entry offsets in the leaf's list, and the
+// leaf's totals with its own particles as the last segment.
+__global__ void k_walk_order(const hsr_node *__restrict__ t, const int32_t *__restrict__ leaves,
+                             int64_t nleaves, int ndepth, const int64_t *cnt_seg,
+                             const int64_t *cnt_ent, const uint8_t *side, int64_t *slot_seg,
+                             int64_t *slot_ent, int64_t *nseg, int64_t *nent) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= nleaves) return;
+  const int64_t base = k * ndepth;
+  int64_t so = 0, eo = 0;
+  for (int i = 0; i < ndepth; ++i)
+    if (side[base + i]) {
+      slot_seg[base + i] = so;
+      slot_ent[base + i] = eo;
+      so += cnt_seg[base + i];
+      eo += cnt_ent[base + i];
+    }
+  for (int i = ndepth - 1; i >= 0; --i)
+    if (!side[base + i]) {
+      slot_seg[base + i] = so;
+      slot_ent[base + i] = eo;
+      so += cnt_seg[base + i];
+      eo += cnt_ent[base + i];
+    }
+  nseg[k] = so + 1;  // the leaf's own particles
+  nent[k] = eo + t[leaves[k]].count;
+}
+
+__global__ void k_walk_own(const hsr_node *__restrict__ t, const int32_t *__restrict__ leaves,
+                           int64_t nleaves, const int64_t *seg_ptr, const int64_t *ent_ptr,
+                           int64_t *seg_a, int64_t *seg_b, int64_t *seg_e) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= nleaves) return;
+  const hsr_node &L = t[leaves[k]];
+  const int64_t s = seg_ptr[k + 1] - 1;
+  seg_a[s] = L.offset;
+  seg_b[s] = L.count;
+  seg_e[s] = ent_ptr[k + 1] - ent_ptr[k] - L.count;
 }
 
 // Expand segments into j indices (one wave per listed leaf): particle
@@ -329,22 +423,34 @@ hsr_status lists_build_device(hsr_ctx *c, const float *min_force_h,
   int64_t *nseg = dptr<int64_t>(S.h);
   int64_t *nent = nseg + nl + 1;
   const int64_t depth = T.max_depth + 8;
-  HSR_HIP(dbuf_reserve(LS.stack, sizeof(int32_t) * (size_t)std::max<int64_t>(1, nl) * depth));
+  // one walk per (leaf, ancestor sibling at depth 1 .. max_depth)
+  const int ndepth = (int)std::max<int64_t>(1, T.max_depth);
+  const int64_t nwalk = std::max<int64_t>(1, nl) * ndepth;
+  HSR_HIP(dbuf_reserve(LS.stack, sizeof(int32_t) * (size_t)nwalk * depth));
+  HSR_HIP(dbuf_reserve(LS.walk, (sizeof(int64_t) * 4 + 1) * (size_t)nwalk + 64));
+  int64_t *cnt_seg = dptr<int64_t>(LS.walk);
+  int64_t *cnt_ent = cnt_seg + nwalk;
+  int64_t *slot_seg = cnt_ent + nwalk;
+  int64_t *slot_ent = slot_seg + nwalk;
+  uint8_t *side = reinterpret_cast<uint8_t *>(slot_ent + nwalk);
   ListParams lp;
   lp.fsrrmax = c->p.fsrrmax;
   lp.fsrrmax2 = c->p.fsrrmax * c->p.fsrrmax;
   lp.tan_oa = c->p.tan_oa;
   lp.tdpts = c->p.tdpts;
   lp.stack_depth = depth;
-  lp.nthreads = std::max<int64_t>(1, nl);
+  lp.nthreads = nwalk;
   HSR_HIP(dbuf_reserve(S.counters, 256));
   lp.overflow = dptr<int64_t>(S.counters) + 20;
   k_zero_i64<<<1, 64, 0, st>>>(lp.overflow, 1);
   const hsr_node *cn = dptr<hsr_node>(T.canon_nodes);
   int32_t *stk = dptr<int32_t>(LS.stack);
   if (nl > 0) {
-    k_walk<false><<<nblk(nl, 64), 64, 0, st>>>(cn, leaves, leaf_box, d_cbase, nl, lp, stk, nseg, nent,
-                                                nullptr, nullptr, nullptr, nullptr);
+    k_walk<false><<<nblk(nl * ndepth, 64), 64, 0, st>>>(cn, leaves, leaf_box, d_cbase, nl, ndepth, lp,
+                                                          stk, cnt_seg, cnt_ent, side, nullptr, nullptr,
+                                                          nullptr, nullptr, nullptr, nullptr);
+    k_walk_order<<<nblk(nl, 64), 64, 0, st>>>(cn, leaves, nl, ndepth, cnt_seg, cnt_ent, side,
+                                              slot_seg, slot_ent, nseg, nent);
     k_zero_i64<<<1, 64, 0, st>>>(nseg + nl, 1);
     k_zero_i64<<<1, 64, 0, st>>>(nent + nl, 1);
     if (excl_scan(c, nseg, dptr<int64_t>(LS.seg_ptr), nl + 1) != HSR_OK) return HSR_ERR_HIP;
@@ -365,8 +471,12 @@ hsr_status lists_build_device(hsr_ctx *c, const float *min_force_h,
   int64_t *seg_e = seg_b + LS.nsegs + 1;
   HSR_HIP(dbuf_reserve(LS.jidx, sizeof(int32_t) * (LS.nentries + 1)));
   if (nl > 0) {
-    k_walk<true><<<nblk(nl, 64), 64, 0, st>>>(cn, leaves, leaf_box, d_cbase, nl, lp, stk, nullptr, nullptr,
-                                               dptr<int64_t>(LS.seg_ptr), seg_a, seg_b, seg_e);
+    k_walk<true><<<nblk(nl * ndepth, 64), 64, 0, st>>>(cn, leaves, leaf_box, d_cbase, nl, ndepth, lp,
+                                                         stk, nullptr, nullptr, nullptr,
+                                                         dptr<int64_t>(LS.seg_ptr),
+                                                         slot_seg, slot_ent, seg_a, seg_b, seg_e);
+    k_walk_own<<<nblk(nl, 64), 64, 0, st>>>(cn, leaves, nl, dptr<int64_t>(LS.seg_ptr),
+                                            dptr<int64_t>(LS.ent_ptr), seg_a, seg_b, seg_e);
     k_expand<<<(unsigned)nl, 64, 0, st>>>(nl, n, c->p.tdpts, dptr<int64_t>(LS.seg_ptr), seg_a, seg_b,
                                           dptr<int64_t>(LS.ent_ptr), dptr<int32_t>(LS.jidx));
   }
