@@ -507,7 +507,7 @@ hsr_status force_plan(hsr_ctx *c) {
   HSR_HIP(hipMemcpyAsync(&tail[3], mflag + nl - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   HSR_HIP(hipMemcpyAsync(&tail[4], pbase + nl - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   HSR_HIP(hipMemcpyAsync(&tail[5], psize + nl - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  HSR_HIP(hipStreamSynchronize(st));
+  HSR_HIP(stream_wait(st));
   const int64_t nitems = tail[0] + tail[1];
   const int64_t nmulti = tail[2] + tail[3];
   const int64_t npart = tail[4] + tail[5];
@@ -533,7 +533,7 @@ hsr_status force_plan(hsr_ctx *c) {
   // device-resident counts for the persistent launch
   int64_t hc[2] = {nitems, nmulti};
   HSR_HIP(hipMemcpyAsync(counts, hc, sizeof(hc), hipMemcpyHostToDevice, st));
-  HSR_HIP(hipStreamSynchronize(st));
+  HSR_HIP(stream_wait(st));
   HSR_HIP(dbuf_reserve(c->s.qctr, 64));
   LS.multi = multi;
   LS.counts = counts;

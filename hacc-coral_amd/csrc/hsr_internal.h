@@ -244,6 +244,18 @@ hsr_status resort_run(hsr_ctx *c, int64_t n, const int32_t ng[3], float *f[8],
                       int64_t *id, uint16_t *mask, int64_t *inside);
 hsr_status fill_run(hsr_ctx *c, int64_t n, float v, float *a);
 
+// Waits for the stream by polling it for a while before blocking: the
+// tree build reads a few counters back once per level and the list build a
+// few times, and a blocking wait wakes the host tens of microseconds after
+// the copy has landed.
+inline hipError_t stream_wait(hipStream_t s) {
+  for (int i = 0; i < (1 << 20); ++i) {
+    const hipError_t e = hipStreamQuery(s);
+    if (e != hipErrorNotReady) return e;
+  }
+  return hipStreamSynchronize(s);
+}
+
 // The ten particle arrays (x y z vx vy vz mass phi, id, mask); entries may
 // be null where an operation allows it.
 struct PArrays {

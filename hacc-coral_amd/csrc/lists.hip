@@ -373,7 +373,7 @@ hsr_status lists_build_device(hsr_ctx *c, const float *min_force_h,
   // per-box leaf counts: root.nleaf (read back the roots)
   std::vector<BNode> roots(nboxes);
   HSR_HIP(hipMemcpyAsync(roots.data(), nodes, sizeof(BNode) * nboxes, hipMemcpyDeviceToHost, st));
-  HSR_HIP(hipStreamSynchronize(st));
+  HSR_HIP(stream_wait(st));
   std::vector<int64_t> leaf_base(nboxes + 1, 0);
   for (int64_t b = 0; b < nboxes; ++b) leaf_base[b + 1] = leaf_base[b] + roots[b].nleaf;
   const int64_t nslots = leaf_base[nboxes];
@@ -406,7 +406,7 @@ hsr_status lists_build_device(hsr_ctx *c, const float *min_force_h,
   if (nslots > 0) {
     HSR_HIP(hipMemcpyAsync(&last_pos, pos + nslots - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HSR_HIP(hipMemcpyAsync(&last_flag, flag + nslots - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    HSR_HIP(hipStreamSynchronize(st));
+    HSR_HIP(stream_wait(st));
   }
   const int64_t nl = nslots > 0 ? (int64_t)last_pos + last_flag : 0;
   LS.nleaves = nl;
@@ -460,7 +460,7 @@ hsr_status lists_build_device(hsr_ctx *c, const float *min_force_h,
   if (nl > 0) {
     HSR_HIP(hipMemcpyAsync(&tot[0], dptr<int64_t>(LS.seg_ptr) + nl, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     HSR_HIP(hipMemcpyAsync(&tot[1], dptr<int64_t>(LS.ent_ptr) + nl, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    HSR_HIP(hipStreamSynchronize(st));
+    HSR_HIP(stream_wait(st));
   }
   LS.nsegs = tot[0];
   LS.nentries = tot[1];
@@ -487,7 +487,7 @@ hsr_status lists_build_device(hsr_ctx *c, const float *min_force_h,
   int64_t ovf = 0;
   HSR_HIP(hipMemcpyAsync(&LS.pairs, d_pairs, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   HSR_HIP(hipMemcpyAsync(&ovf, lp.overflow, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  HSR_HIP(hipStreamSynchronize(st));
+  HSR_HIP(stream_wait(st));
   HSR_HIP(hipGetLastError());
   if (ovf) {
     hsr_set_error("interaction-list walk exceeded its stack bound");

@@ -1290,7 +1290,7 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
       DBuf nb;
       HSR_HIP(dbuf_reserve(nb, sizeof(BNode) * ncap));
       HSR_HIP(hipMemcpyAsync(nb.p, T.bnodes.p, sizeof(BNode) * le, hipMemcpyDeviceToDevice, st));
-      HSR_HIP(hipStreamSynchronize(st));
+      HSR_HIP(stream_wait(st));
       dbuf_free(T.bnodes);
       T.bnodes = nb;
       cap = ncap;
@@ -1343,7 +1343,7 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     k_part_rank<<<pg, 256, 0, st>>>(nodes, pb, ctr, X[cur], Y[cur], Z[cur], idx, rights, dest);
     k_part_chain<<<pg, 256, 0, st>>>(nodes, pb, ctr, idx, rights, dest, ctr);
     HSR_HIP(hipMemcpyAsync(&hc, ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
-    HSR_HIP(hipStreamSynchronize(st));
+    HSR_HIP(stream_wait(st));
     if (hc.unresolved > 0) {
       k_part_jump_init<<<pg, 256, 0, st>>>(nodes, pb, ctr, idx, jmp);
       for (int r = 0; r < 32; ++r)
@@ -1383,7 +1383,7 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
   // per-box canonical bases: box b holds 1 + 2 * sa(root_b) nodes
   std::vector<BNode> roots(nboxes);
   HSR_HIP(hipMemcpyAsync(roots.data(), nodes, sizeof(BNode) * nboxes, hipMemcpyDeviceToHost, st));
-  HSR_HIP(hipStreamSynchronize(st));
+  HSR_HIP(stream_wait(st));
   std::vector<int64_t> cbase(nboxes + 1, 0);
   for (int64_t b = 0; b < nboxes; ++b) cbase[b + 1] = cbase[b] + 1 + 2 * roots[b].sa;
   T.nnodes = cbase[nboxes];
@@ -1396,7 +1396,7 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     std::vector<int32_t> rb(nboxes);
     for (int64_t b = 0; b < nboxes; ++b) rb[b] = (int32_t)b;
     HSR_HIP(hipMemcpyAsync(root_box, rb.data(), sizeof(int32_t) * nboxes, hipMemcpyHostToDevice, st));
-    HSR_HIP(hipStreamSynchronize(st));
+    HSR_HIP(stream_wait(st));
   }
   HSR_HIP(dbuf_reserve(S.f, sizeof(int32_t) * (nb + 1)));
   int32_t *box_of = dptr<int32_t>(S.f);
