@@ -360,14 +360,31 @@ def main():
         flo, fhi = load.force_box
     dev = torch.device("cuda:0")
 
-    def T(arr):
-        return torch.from_numpy(np.ascontiguousarray(arr)).to(dev)
-    src = {"x": T(px), "y": T(py), "z": T(pz),
-           "m": T(np.ones(n, np.float32)), "vx": T(np.zeros(n, np.float32)),
-           "vy": T(np.zeros(n, np.float32)), "vz": T(np.zeros(n, np.float32)),
-           "phi": T(np.zeros(n, np.float32)), "id": T(np.arange(n, dtype=np.int64)),
-           "mask": torch.zeros(n, dtype=torch.int16, device=dev)}
-    work = {k: v.clone() for k, v in src.items()}
+    # the ten arrays of the unsorted load, and the working copy the step
+    # restores with one device copy on the library's stream (the build
+    # permutes the arrays in place)
+    host = {"x": px, "y": py, "z": pz, "m": np.ones(n, np.float32),
+            "vx": np.zeros(n, np.float32), "vy": np.zeros(n, np.float32),
+            "vz": np.zeros(n, np.float32), "phi": np.zeros(n, np.float32),
+            "id": np.arange(n, dtype=np.int64), "mask": np.zeros(n, np.int16)}
+    layout, nbytes = {}, 0
+    for k, v in host.items():
+        layout[k] = (nbytes, v.dtype)
+        nbytes += (v.nbytes + 255) // 256 * 256
+    src_all = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    work_all = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+
+    def views(buf):
+        out = {}
+        for k, (o, dt) in layout.items():
+            tdt = {np.dtype(np.float32): torch.float32, np.dtype(np.int64): torch.int64,
+                   np.dtype(np.int16): torch.int16}[np.dtype(dt)]
+            out[k] = buf[o:o + n * np.dtype(dt).itemsize].view(tdt)
+        return out
+    src, work = views(src_all), views(work_all)
+    for k, v in host.items():
+        src[k].copy_(torch.from_numpy(np.ascontiguousarray(v)))
+    torch.cuda.synchronize()
     fcoeff = 1.0
     ctxs = {}
 
@@ -376,9 +393,9 @@ def main():
         if ctx is None:
             ctx = ctxs[nd] = Context(nd=nd, tdpts=a.tdpts)
             ctx.set_tuning(use_graph=1, force_pass=a.force_pass)
-        for k in src:
-            work[k].copy_(src[k])
-        torch.cuda.synchronize()
+            ctx.torch_stream = torch.cuda.ExternalStream(ctx.stream())
+        with torch.cuda.stream(ctx.torch_stream):
+            work_all.copy_(src_all)
         if ens is None:
             ctx.build(work["x"], work["y"], work["z"], work["m"], work["vx"], work["vy"],
                       work["vz"], work["phi"], work["id"], work["mask"],
