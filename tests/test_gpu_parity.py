@@ -197,6 +197,35 @@ def test_mixed_magnitudes_and_masses():
     check_forces(tp, tree, lists, ctx)
 
 
+@pytest.mark.parametrize("case", ["negative", "big_nodes"])
+def test_exact_sum_paths(case):
+    """The centre-of-mass sums on every path: negative coordinates (no
+    integer sum: the ordered paths), big nodes (> 2048 particles, the block
+    tables) whose sums are integers in units of the finest lsb, and big
+    nodes whose sums round (coordinates near 0 and masses with full
+    mantissas)."""
+    rng = np.random.default_rng(31 if case == "negative" else 32)
+    n = 60000
+    if case == "negative":
+        say("case exact-sum paths: coordinates in [-16, 16), unit masses")
+        p = (rng.random((n, 3)) * 32.0 - 16.0).astype(np.float32)
+        m = np.ones(n, np.float32)
+        lo, hi, flo, fhi = [-16.0] * 3, [16.0] * 3, [-15.0] * 3, [15.0] * 3
+    else:
+        say("case exact-sum paths: big nodes, integer sums and rounding sums")
+        p = (rng.random((n, 3)) * 40.0).astype(np.float32)
+        p[: n // 3] = np.round(p[: n // 3] * 64.0) / np.float32(64.0)  # coarse lsb
+        p[n // 3: n // 3 + 200] *= np.float32(1e-6)                     # fine lsb near 0
+        m = np.where(rng.random(n) < 0.5, np.float32(1.0),
+                     (0.5 + rng.random(n)).astype(np.float32)).astype(np.float32)
+        lo, hi, flo, fhi = [0.0] * 3, [40.0] * 3, [1.0] * 3, [39.0] * 3
+    tp, tree, lists, ctx, arrs = run_both(p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), m,
+                                          lo, hi, flo, fhi, 64)
+    check_tree(tree, ctx, arrs)
+    check_lists(lists, ctx)
+    check_forces(tp, tree, lists, ctx)
+
+
 def test_ensemble_boxes_one_launch():
     """Independent boxes in one context (hsr_build_many / hsr_lists_many):
     every box's tree, lists and forces equal the oracle run on that box

@@ -206,3 +206,16 @@ def test_cxx_dropin_refresh(tmp_path):
     assert int(fields["count"]) == rc and int(fields["Np_last"]) == ri
     for k in ARR:
         assert bits_equal(fin[k][:rc], ref[k][:rc]), k
+
+
+def test_refresh_twice_reuses_buffers():
+    """Two refreshes on one context, the second with fewer particles: the
+    buffers sized by the first serve the second, results as the oracle's."""
+    rL, oL, ng = 64.0, 3.2, 64
+    d = _lib.domain(rL, oL, ng)
+    ctx = _lib.Context()
+    for n, seed in ((250000, 41), (30000, 42)):
+        say(f"case refresh twice: n={n}")
+        st = state(uniform_local(n, d.ng_local_total, seed), seed + 1)
+        check(st, n, rL, oL, ng, 1.0, 2 * n, device=(n < 100000), ctx=ctx)
+    ctx.close()
