@@ -475,7 +475,7 @@ hsr_status force_plan(hsr_ctx *c) {
   hipStream_t st = c->stream;
   ListState &LS = c->lists;
   const int64_t nl = LS.nleaves;
-  const int64_t chunk = c->tuning.chunk > 0 ? c->tuning.chunk : 16384;
+  const int64_t chunk = c->tuning.chunk > 0 ? c->tuning.chunk : 8192;
   LS.chunk = chunk;
   if (nl == 0) {
     LS.nitems = 0;
