@@ -331,7 +331,7 @@ k_adj_reduce(const hsr_node *cn, const AItem *items, const int64_t *first,
 constexpr int FR = 2;                       // targets per lane
 constexpr int FTILE = 256;                  // entries per LDS tile
 #ifndef HSR_FUSED_UNROLL
-#define HSR_FUSED_UNROLL 2
+#define HSR_FUSED_UNROLL 1
 #endif
 constexpr int FUNROLL = HSR_FUSED_UNROLL;  // entries per step of a slice
 
@@ -603,7 +603,7 @@ adj_fused_body(const hsr_node *__restrict__ cn, const AItem *__restrict__ items,
 // Waves per SIMD the register allocation of the 5th-order body targets (0:
 // the compiler's choice).
 #ifndef HSR_ADJ_WAVES
-#define HSR_ADJ_WAVES 0
+#define HSR_ADJ_WAVES 5
 #endif
 #if HSR_ADJ_WAVES > 0
 #define HSR_ADJ_ATTR __attribute__((amdgpu_waves_per_eu(HSR_ADJ_WAVES, HSR_ADJ_WAVES)))

@@ -27,13 +27,31 @@
 
 namespace {
 
-constexpr int R = 2;          // i-particles per lane
+// i-particles per lane and j entries per unrolled step (build-time choices)
+#ifndef HSR_FORCE_R
+#define HSR_FORCE_R 2
+#endif
+#ifndef HSR_FORCE_UNROLL
+#define HSR_FORCE_UNROLL 2
+#endif
+constexpr int R = HSR_FORCE_R;
+static_assert(R == 1 || R == 2 || R == 4, "i-particles per lane");
 constexpr int TILE = 256;     // j entries staged per tile
 constexpr int GMAX = 16;      // at most 16 lane groups (>= 4 lanes each)
-// staged entries incl. per-group padding: max over G <= 16 of
-// G * stride(G) is 330 (G = 10) for TILE = 256
-constexpr int LDS_ENT = 336;
-constexpr int UNROLL = 4;
+constexpr int LDS_ENT = 336;  // staged entries incl. per-group padding
+constexpr int UNROLL = HSR_FORCE_UNROLL;
+static_assert(UNROLL >= 1 && UNROLL <= 8, "j entries per unrolled step");
+// Passes hold a power of two of at least 4 R targets, or fewer than 4 R, so
+// the group count G = 64 / max(ceil(targets / R), 4) is 1, 2, 4, 8 or 16;
+// the staged slices of a full tile must fit for each of them.
+constexpr int staged_entries(int G) {
+  const int tg = ((TILE + G - 1) / G + UNROLL - 1) / UNROLL * UNROLL;
+  return G * (tg + ((9 - (tg & 7)) & 7));
+}
+static_assert(staged_entries(1) <= LDS_ENT && staged_entries(2) <= LDS_ENT &&
+                  staged_entries(4) <= LDS_ENT && staged_entries(8) <= LDS_ENT &&
+                  staged_entries(16) <= LDS_ENT,
+              "staged tile fits");
 
 struct Item {
   int32_t leaf;     // listed-leaf index
