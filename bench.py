@@ -32,7 +32,7 @@ PEAK_HBM_GBPS = 8000.0       # MI355X HBM3E
 # HBM bytes per k_force launch measured with rocprofv3 (FETCH_SIZE x 2.0 + WRITE_SIZE
 # x 1.0, factors from a 256 MiB device copy) on the 256^3 clustered load at leaf size
 # 512: profiles/k_force_256c_traffic.md.  Reported only for that workload.
-MEASURED_TRAFFIC = {("B", 256, 512, 1): 138908830926}
+MEASURED_TRAFFIC = {("B", 256, 512, 1): 142726894856}
 # Algorithmic bytes of the tree build: 48 B per particle per level it is split at
 # (cm reads x, y, z, m; partition reads and writes key + index) plus one 84 B
 # gather of the ten payload arrays.
