@@ -841,15 +841,27 @@ hsr_status hsr_set_tuning(hsr_ctx *c, const hsr_tuning *t) {
     g_err = "hsr_set_tuning: null argument";
     return HSR_ERR_ARG;
   }
-  if (t->force_pass != 0 && t->force_pass != 64 && t->force_pass != 128) {
-    g_err = "hsr_set_tuning: force_pass must be 0, 64 or 128";
+  const int r = force_targets_per_lane(), tile = force_tile_entries();
+  if (t->force_pass != 0 && t->force_pass != 32 * r && t->force_pass != 64 * r) {
+    g_err = "hsr_set_tuning: force_pass must be 0, " + std::to_string(32 * r) + " or " +
+            std::to_string(64 * r);
     return HSR_ERR_ARG;
   }
-  const bool reorder = t->tree_order_targets != c->tuning.tree_order_targets;
+  if ((t->i_per_lane != 0 && t->i_per_lane != r) || (t->tile != 0 && t->tile != tile)) {
+    g_err = "hsr_set_tuning: i_per_lane and tile are fixed when the library is built (" +
+            std::to_string(r) + ", " + std::to_string(tile) + "); pass 0 or those values";
+    return HSR_ERR_ARG;
+  }
+  if (t->chunk < 0 || t->chunk > (1 << 30) || t->waves_per_cu < 0 || t->waves_per_cu > 256) {
+    g_err = "hsr_set_tuning: chunk must lie in [0, 2^30], waves_per_cu in [0, 256]";
+    return HSR_ERR_ARG;
+  }
+  const bool replan = t->tree_order_targets != c->tuning.tree_order_targets ||
+                      t->chunk != c->tuning.chunk;
   c->tuning = *t;
   c->graph_gen = -1;  // a captured force phase holds the old settings
-  if (reorder && c->listed) {
-    // the target order is part of the force plan
+  if (replan && c->listed) {
+    // the target order and the work items are part of the force plan
     hsr_status s = force_plan(c);
     if (s) return s;
     c->lists_gen++;

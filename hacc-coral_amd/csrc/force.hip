@@ -489,6 +489,9 @@ k_leaf_sort(int64_t nl, const int32_t *__restrict__ leaves, const hsr_node *__re
   for (int i = threadIdx.x; i < cnt; i += blockDim.x) iperm[off + i] = (int32_t)(off + idx[i]);
 }
 
+int force_targets_per_lane() { return R; }
+int force_tile_entries() { return TILE; }
+
 hsr_status force_plan(hsr_ctx *c) {
   hipStream_t st = c->stream;
   ListState &LS = c->lists;

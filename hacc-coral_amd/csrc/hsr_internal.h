@@ -230,6 +230,9 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
 hsr_status lists_build_device(hsr_ctx *c, const float *min_force,
                               const float *max_force);
 hsr_status force_plan(hsr_ctx *c);
+// the leaf kernel's build-time shape: targets per lane, entries per LDS tile
+int force_targets_per_lane();
+int force_tile_entries();
 hsr_status force_run(hsr_ctx *c, float *ax, float *ay, float *az);
 hsr_status kick_run(hsr_ctx *c, float fcoeff, float *vx, float *vy, float *vz,
                     const float *mass_sorted);

@@ -347,10 +347,13 @@ hsr_status hsr_refresh(hsr_ctx *ctx, const hsr_domain *dom, float anow,
 
 /* ------------------------------------------------------------- tuning */
 typedef struct {
-  int i_per_lane;     /* i-particles held per lane (0 = automatic) */
-  int tile;           /* j entries staged per LDS tile (0 = automatic) */
-  int chunk;          /* maximum list entries per work item (0 = automatic) */
-  int waves_per_cu;   /* persistent waves per CU (0 = automatic) */
+  int i_per_lane;     /* i-particles held per lane: fixed when the library is
+                         built (2); 0 or that value */
+  int tile;           /* j entries staged per LDS tile: fixed when the library
+                         is built (256); 0 or that value */
+  int chunk;          /* maximum list entries per work item, 0..2^30 (0 =
+                         8192); a change re-plans lists already built */
+  int waves_per_cu;   /* persistent waves per CU, 0..256 (0 = 32) */
   int use_graph;      /* capture the force phase in a hipGraph and replay it */
   int adjoint_mode;   /* hsr_adjoint passes: 0 automatic, 1 fused (one pair
                          evaluation feeds target and source; 12 B of device
@@ -366,8 +369,11 @@ typedef struct {
                          more entries (sums in another order: rounding only) */
   int force_pass;     /* targets per pass of the leaf kernel: 128 (0 = default)
                          or 64 (smaller target boxes cull more list entries;
-                         sums in another order: rounding only) */
+                         sums in another order: rounding only); 64 and 32
+                         times i_per_lane in general */
 } hsr_tuning;
+
+/* Out-of-range fields give HSR_ERR_ARG and leave the tuning unchanged. */
 
 hsr_status hsr_set_tuning(hsr_ctx *ctx, const hsr_tuning *t);
 hsr_status hsr_get_tuning(hsr_ctx *ctx, hsr_tuning *t);

@@ -6,7 +6,7 @@ import sys
 import numpy as np
 import pytest
 
-from haccsr import Context, ics
+from haccsr import Context, _lib, ics
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -154,6 +154,35 @@ def test_chunked_lists_and_determinism():
     b1 = np.stack(ctx.forces(), 1)
     assert np.array_equal(b1.view(np.uint32), np.stack(ctx.forces(), 1).view(np.uint32))
     say(f"  rel rms unchunked {e1:.3e} chunked {e2:.3e}")
+
+
+def test_tuning_validation_and_chunk_replan():
+    """hsr_set_tuning rejects out-of-range fields and keeps the old tuning;
+    a new chunk size re-plans lists already built, so the next force phase
+    (no new hsr_lists) equals the one after a fresh list build bitwise."""
+    import ctypes
+    say("case tuning validation and re-plan on a chunk change")
+    x, y, z, m = uniform(30000, 14.0, 5)
+    flo, fhi = [1.0] * 3, [13.0] * 3
+    tp, tree, lists, ctx, arrs = run_both(x, y, z, m, [0, 0, 0], [14.0] * 3, flo, fhi, 96)
+    before = _lib.Tuning()
+    assert ctx.L.hsr_get_tuning(ctx.h, ctypes.byref(before)) == 0
+    for bad in (dict(tile=128), dict(i_per_lane=3), dict(chunk=-1), dict(chunk=(1 << 30) + 1),
+                dict(waves_per_cu=-2), dict(waves_per_cu=1000), dict(force_pass=96)):
+        with pytest.raises(_lib.HsrError):
+            ctx.set_tuning(**bad)
+        now = _lib.Tuning()
+        assert ctx.L.hsr_get_tuning(ctx.h, ctypes.byref(now)) == 0
+        assert bytes(now) == bytes(before), bad
+    ctx.set_tuning(tile=256, i_per_lane=2, waves_per_cu=40)
+    e1 = check_forces(tp, tree, lists, ctx)
+    ctx.set_tuning(chunk=256)
+    replanned = np.stack(ctx.forces(), 1).view(np.uint32).copy()
+    ctx.lists(flo, fhi)
+    fresh = np.stack(ctx.forces(), 1).view(np.uint32)
+    assert np.array_equal(replanned, fresh)
+    e2 = check_forces(tp, tree, lists, ctx)
+    say(f"  rel rms chunk 8192 {e1:.3e}, chunk 256 {e2:.3e}")
 
 
 def test_duplicates_and_degenerate_nodes():
