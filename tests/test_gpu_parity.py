@@ -156,6 +156,48 @@ def test_chunked_lists_and_determinism():
     say(f"  rel rms unchunked {e1:.3e} chunked {e2:.3e}")
 
 
+@pytest.mark.parametrize("law", [O.LAW_SR_POLY5, O.LAW_SR_POLY6])
+def test_sr_poly_laws_match_oracle(law):
+    """ForceLawSR with the 5th- and 6th-order grid-force polynomials
+    (ForceLaw.cxx:89-116, 187-192): the device kernel for each order against
+    the oracle's nbody1 with the same law, per unit target mass, on unequal
+    masses; the tree and lists bit-exact as with the default law."""
+    say(f"case SR polynomial law {law}")
+    rng = np.random.default_rng(40 + law)
+    n, box, nd = 20000, 14.0, 64
+    p = (rng.random((n, 3)) * box).astype(np.float32)
+    x, y, z = p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy()
+    m = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    lo, hi = np.zeros(3, np.float32), np.full(3, box, np.float32)
+    flo, fhi = np.full(3, 1.0, np.float32), np.full(3, box - 1.0, np.float32)
+    rmax = 3.116326355  # FGrid::rmax (ForceLaw.cxx:32), the drop-in's fsrrmax
+    tp = O.TreeParams(lo, hi, flo, fhi, nd=nd, fsrrmax=rmax)
+    tree = O.build_tree(tp, x, y, z, m)
+    lists = O.build_lists(tp, tree)
+    ctx = Context(nd=nd, law=law, fsrrmax=rmax)
+    ids = np.arange(n, dtype=np.int64)
+    X, Y, Z, M = x.copy(), y.copy(), z.copy(), m.copy()
+    ctx.build(X, Y, Z, M, ids=ids, min_loc=lo, max_loc=hi)
+    ctx.lists(flo, fhi)
+    check_tree(tree, ctx, (X, Y, Z, M, ids))
+    check_lists(lists, ctx)
+    ax, ay, az = ctx.forces()
+    sel = listed_mask(tree, lists)
+    rx, ry, rz = O.forces_f32(tp, tree, lists, law=law)
+    ref = (np.stack([rx, ry, rz], 1) / tree.mass[:, None])[sel]
+    e = rel_rms(np.stack([ax, ay, az], 1)[sel], ref)
+    say(f"  rel rms {e:.3e}")
+    assert e <= 1e-5
+    ctx.close()
+
+
+def test_newton_law_refused_on_device():
+    """The device implements the Step16 and SR polynomial laws only; any
+    other law is refused at context creation (no CPU fallback)."""
+    with pytest.raises(_lib.HsrError):
+        Context(nd=64, law=_lib.LAW_NEWTON)
+
+
 def test_tuning_validation_and_chunk_replan():
     """hsr_set_tuning rejects out-of-range fields and keeps the old tuning;
     a new chunk size re-plans lists already built, so the next force phase
