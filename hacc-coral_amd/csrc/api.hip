@@ -453,6 +453,10 @@ hsr_status hsr_lists(hsr_ctx *c, const float min_force[3],
     g_err = "hsr_lists: null force box";
     return HSR_ERR_ARG;
   }
+  // the old lists and plan are gone from here on: until the new ones are
+  // complete, every call that needs them fails with HSR_ERR_STATE
+  c->listed = false;
+  c->lists_gen++;
   HSR_HIP(hipEventRecord(c->ev[2], c->stream));
   s = lists_build_device(c, min_force, max_force);
   if (s) return s;
@@ -480,6 +484,10 @@ hsr_status hsr_lists_many(hsr_ctx *c, const float *min_force,
     g_err = "hsr_lists_many: null force boxes";
     return HSR_ERR_ARG;
   }
+  // the old lists and plan are gone from here on: until the new ones are
+  // complete, every call that needs them fails with HSR_ERR_STATE
+  c->listed = false;
+  c->lists_gen++;
   HSR_HIP(hipEventRecord(c->ev[2], c->stream));
   s = lists_build_device(c, min_force, max_force);
   if (s) return s;
@@ -852,19 +860,40 @@ hsr_status hsr_set_tuning(hsr_ctx *c, const hsr_tuning *t) {
             std::to_string(r) + ", " + std::to_string(tile) + "); pass 0 or those values";
     return HSR_ERR_ARG;
   }
-  if (t->chunk < 0 || t->chunk > (1 << 30) || t->waves_per_cu < 0 || t->waves_per_cu > 256) {
-    g_err = "hsr_set_tuning: chunk must lie in [0, 2^30], waves_per_cu in [0, 256]";
+  if ((t->chunk != 0 && (t->chunk < tile || t->chunk > (1 << 30))) || t->waves_per_cu < 0 ||
+      t->waves_per_cu > 256) {
+    g_err = "hsr_set_tuning: chunk must be 0 or lie in [" + std::to_string(tile) +
+            ", 2^30], waves_per_cu in [0, 256]";
+    return HSR_ERR_ARG;
+  }
+  if (t->max_plan_bytes < 0) {
+    g_err = "hsr_set_tuning: max_plan_bytes must be >= 0";
     return HSR_ERR_ARG;
   }
   const bool replan = t->tree_order_targets != c->tuning.tree_order_targets ||
                       t->chunk != c->tuning.chunk;
+  const hsr_tuning old = c->tuning;
   c->tuning = *t;
   c->graph_gen = -1;  // a captured force phase holds the old settings
   if (replan && c->listed) {
-    // the target order and the work items are part of the force plan
-    hsr_status s = force_plan(c);
-    if (s) return s;
+    // The target order and the work items are part of the force plan.  A
+    // failed re-plan leaves the plan half written: the lists count as gone
+    // until a plan with the old tuning is made again.
+    c->listed = false;
     c->lists_gen++;
+    hsr_status s = force_plan(c);
+    if (s) {
+      const std::string why = g_err;
+      c->tuning = old;
+      if (force_plan(c) == HSR_OK) {
+        c->lists_gen++;
+        c->listed = true;
+      }
+      g_err = why;
+      return s;
+    }
+    c->lists_gen++;
+    c->listed = true;
   }
   return HSR_OK;
 }

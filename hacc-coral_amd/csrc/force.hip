@@ -532,6 +532,20 @@ hsr_status force_plan(hsr_ctx *c) {
   const int64_t nitems = tail[0] + tail[1];
   const int64_t nmulti = tail[2] + tail[3];
   const int64_t npart = tail[4] + tail[5];
+  // the item sort and the item ids are 32-bit
+  if (nitems >= (int64_t)INT32_MAX) {
+    hsr_set_error("force plan: " + std::to_string(nitems) +
+                  " work items (2^31 or more); raise tuning.chunk");
+    return HSR_ERR_NOMEM;
+  }
+  const int64_t plan_bytes = (int64_t)sizeof(Item) * 2 * (nitems + 1) +
+                             (int64_t)(sizeof(uint64_t) + sizeof(int32_t)) * 2 * (nitems + 1) +
+                             (int64_t)sizeof(float) * npart;
+  if (c->tuning.max_plan_bytes > 0 && plan_bytes > c->tuning.max_plan_bytes) {
+    hsr_set_error("force plan: needs " + std::to_string(plan_bytes) + " bytes, above tuning.max_plan_bytes (" +
+                  std::to_string(c->tuning.max_plan_bytes) + ")");
+    return HSR_ERR_NOMEM;
+  }
   LS.nitems = nitems;
   LS.nmulti = nmulti;
   HSR_HIP(dbuf_reserve(LS.items, sizeof(Item) * 2 * (nitems + 1)));

@@ -108,7 +108,8 @@ class Tuning(C.Structure):
     _fields_ = [("i_per_lane", C.c_int), ("tile", C.c_int), ("chunk", C.c_int),
                 ("waves_per_cu", C.c_int), ("use_graph", C.c_int),
                 ("adjoint_mode", C.c_int), ("cull_off", C.c_int),
-                ("tree_order_targets", C.c_int), ("force_pass", C.c_int)]
+                ("tree_order_targets", C.c_int), ("force_pass", C.c_int),
+                ("max_plan_bytes", C.c_int64)]
 
 
 _lib = None

@@ -59,11 +59,17 @@ RCBForceTree<TDPTS>::RCBForceTree(
     ID_T ds, ID_T tmin, ForceLaw *fl, float fcoeff, POSVEL_T ppc)
     : particleCount(count), ctx(nullptr), buildTime(0) {
   (void)avgMass;  // an estimate only; unused by the reference's build too
-  // RCBForceTree.cxx:395-404: no ForceLaw -> the tree's own law, fcoeff 1
+  // RCBForceTree.cxx:395-404: no ForceLaw -> the tree's own ForceLawNewton
+  // with fcoeff 1.  The BG/Q build's Step16 kernel never consults the law
+  // (RCBForceTree.cxx:585-597); the generic build's nbody1 would evaluate
+  // Newton's 1/r^3, which the device does not implement.
   RCBKernel k = s_kernel;
   if (!fl) {
     fcoeff = 1.0;
-    k = RCBKernel::Step16;
+    if (k != RCBKernel::Step16)
+      throw std::runtime_error(
+          "RCBForceTree: no ForceLaw with the ForceLaw kernel means ForceLawNewton "
+          "(RCBForceTree.cxx:395-404), which the device leaf kernel does not implement");
   }
   hsr_params p = {};
   p.device = s_device;

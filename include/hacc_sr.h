@@ -351,8 +351,11 @@ typedef struct {
                          built (2); 0 or that value */
   int tile;           /* j entries staged per LDS tile: fixed when the library
                          is built (256); 0 or that value */
-  int chunk;          /* maximum list entries per work item, 0..2^30 (0 =
-                         8192); a change re-plans lists already built */
+  int chunk;          /* maximum list entries per work item: 0 (= 8192) or
+                         tile..2^30; a change re-plans lists already built
+                         (on failure the old tuning and plan are restored
+                         where possible; otherwise the lists must be built
+                         again and calls that need them give HSR_ERR_STATE) */
   int waves_per_cu;   /* persistent waves per CU, 0..256 (0 = 32) */
   int use_graph;      /* capture the force phase in a hipGraph and replay it */
   int adjoint_mode;   /* hsr_adjoint passes: 0 automatic, 1 fused (one pair
@@ -371,6 +374,10 @@ typedef struct {
                          or 64 (smaller target boxes cull more list entries;
                          sums in another order: rounding only); 64 and 32
                          times i_per_lane in general */
+  int64_t max_plan_bytes; /* cap on the device memory of the force plan (work
+                         items, their sort keys, chunk partials); 0: no cap.
+                         A plan above it fails with HSR_ERR_NOMEM before any
+                         launch (so does one of 2^31 work items or more) */
 } hsr_tuning;
 
 /* Out-of-range fields give HSR_ERR_ARG and leave the tuning unchanged. */

@@ -565,15 +565,123 @@ static float step16_f(float r2, float rsm2, float fsrrmax2) {
   return (r2 > 0.0f) ? msk * f : 0.0f;
 }
 
-float hsro_law_f32(int law, float r2, float rsm, float fsrrmax) {
+/* FGrid constants (ForceLaw.cxx:23-33). */
+static const float FG_B = 0.72f, FG_C = 0.01f, FG_D = 0.27f, FG_E = 0.0001f,
+                   FG_F = 360.0f, FG_G = 100.0f, FG_H = 0.67f, FG_L = 17.0f;
+
+/* FGrid::fgor (ForceLaw.cxx:39-51): the fitted grid force over r, with the
+ * reference's mix of float and double operations (C's promotion rules are
+ * C++'s here). */
+static float fgor(float r) {
+  float f0 = FG_C + 2.0 / 3.0 * FG_B * FG_B * FG_B;
+  float r2 = r * r;
+  float r4 = r2 * r2;
+  float r6 = r4 * r2;
+  float coshbr = coshf(FG_B * r);
+  float r3fgor = tanhf(FG_B * r) - FG_B * r / coshbr / coshbr +
+                 FG_C * r * r2 * (1.0 + FG_D * r2) * expf(-1.0 * FG_D * r2) +
+                 FG_E * r2 * (FG_F * r2 + FG_G * r4 + FG_L * r6) * expf(-1.0 * FG_H * r2);
+  float rdiv = r + 1.0 * (r <= 0.0);
+  float rdiv3 = rdiv * rdiv * rdiv;
+  return (r3fgor / rdiv3 + (r <= 0.0) * f0) * (r <= FGRID_RMAX) * (r >= 0.0);
+}
+
+void hsro_fgrid_table(int n, float *r2, float *f) {
+  double dr2 = (FGRID_RMAX * FGRID_RMAX) / (n - 1.0);
+  for (int i = 0; i < n; i++) {
+    r2[i] = i * dr2;
+    f[i] = fgor(sqrt(i * dr2));
+  }
+}
+
+/* FGridEvalInterp (ForceLaw.cxx:145-171) */
+typedef struct {
+  int n;
+  float *r2, *f;
+  float r2min, r2max, dr2, oodr2;
+} interp_t;
+
+static void interp_init(interp_t *I, int n) {
+  I->n = n;
+  I->r2 = (float *)malloc(sizeof(float) * n);
+  I->f = (float *)malloc(sizeof(float) * n);
+  hsro_fgrid_table(n, I->r2, I->f);
+  I->r2min = I->r2[0];
+  I->r2max = I->r2[n - 1];
+  I->dr2 = (I->r2max - I->r2min) / (n - 1.0);
+  I->oodr2 = 1.0 / I->dr2;
+}
+
+static void interp_free(interp_t *I) {
+  free(I->r2);
+  free(I->f);
+}
+
+static float interp_eval(const interp_t *I, float r2) {
+  int inRange = (r2 > I->r2min) * (r2 < I->r2max);
+  float inRangef = 1.0 * inRange;
+  int indx = (int)((r2 - I->r2min) * I->oodr2) * inRange;
+  /* the reference reads m_f[indx + 1] even when r2 rounds onto the last
+   * node (indx = n - 1), one past its table; the oracle reads the last
+   * entry there */
+  int hi = indx + 1 < I->n ? indx + 1 : I->n - 1;
+  return inRangef * (I->f[indx] + (r2 - I->r2[indx]) * I->oodr2 * (I->f[hi] - I->f[indx]));
+}
+
+/* ForceLawSR::f_over_r (ForceLaw.cxx:187-192) with FGridEvalFit
+ * (r2max = rmax^2, ForceLaw.cxx:71-81) or FGridEvalInterp (r2min, r2max from
+ * the table). */
+static float sr_fit_f_over_r(float r2, float rsm2) {
+  float r2max = FGRID_RMAX * FGRID_RMAX;
+  float ret = powf(r2 + rsm2, -1.5f) - fgor(sqrtf(r2));
+  ret *= (float)(r2 >= 0.0f) * (float)(r2 <= r2max);
+  return ret;
+}
+
+static float sr_interp_f_over_r(const interp_t *I, float r2, float rsm2) {
+  float ret = powf(r2 + rsm2, -1.5f) - interp_eval(I, r2);
+  ret *= (float)(r2 >= I->r2min) * (float)(r2 <= I->r2max);
+  return ret;
+}
+
+static float law_f32(int law, float r2, float rsm, float fsrrmax, const interp_t *I) {
   float rsm2 = rsm * rsm;
   switch (law) {
   case HSRO_LAW_STEP16: return step16_f(r2, rsm2, fsrrmax * fsrrmax);
   case HSRO_LAW_SR_POLY5: return sr_f_over_r(5, r2, rsm2);
   case HSRO_LAW_SR_POLY6: return sr_f_over_r(6, r2, rsm2);
   case HSRO_LAW_NEWTON: return (float)(1.0 / (double)r2 / sqrt((double)r2));
+  case HSRO_LAW_SR_FIT: return sr_fit_f_over_r(r2, rsm2);
+  case HSRO_LAW_SR_INTERP: return I ? sr_interp_f_over_r(I, r2, rsm2) : 0.0f;
   }
   return 0.0f;
+}
+
+float hsro_law_f32(int law, float r2, float rsm, float fsrrmax) {
+  return law_f32(law, r2, rsm, fsrrmax, NULL);
+}
+
+void hsro_law_f32_vec(int law, int64_t n, const float *r2, float *out, float rsm,
+                      float fsrrmax, int n_interp) {
+  interp_t I;
+  const int interp = law == HSRO_LAW_SR_INTERP && n_interp >= 2;
+  if (interp) interp_init(&I, n_interp);
+  for (int64_t k = 0; k < n; ++k)
+    out[k] = law_f32(law, r2[k], rsm, fsrrmax, interp ? &I : NULL);
+  if (interp) interp_free(&I);
+}
+
+int hsro_law_coeffs(int law, float *out) {
+  switch (law) {
+  case HSRO_LAW_STEP16:
+  case HSRO_LAW_SR_POLY5:
+    for (int k = 0; k < 6; ++k) out[k] = P5[k];
+    return 6;
+  case HSRO_LAW_SR_POLY6:
+    for (int k = 0; k < 7; ++k) out[k] = P6[k];
+    return 7;
+  }
+  return 0;
 }
 
 /* ------------------------------------------------------------- kernels */
@@ -620,6 +728,9 @@ void hsro_force_f32(int law, const hsro_params *p, const hsro_node *t,
   const float fsrrmax2 = p->fsrrmax * p->fsrrmax;
   const float rsm2 = p->rsm * p->rsm;
   const float fcoeff = 1.0f;
+  interp_t I;
+  const int interp = law == HSRO_LAW_SR_INTERP && p->n_interp >= 2;
+  if (interp) interp_init(&I, p->n_interp);
 #pragma omp parallel
   {
     float *bufs[4] = {0, 0, 0, 0};
@@ -643,7 +754,7 @@ void hsro_force_f32(int law, const hsro_params *p, const hsro_node *t,
              * BGQStep16.c:182 replaces updateq's role for self pairs */
             fl = step16_f(dist2, rsm2, fsrrmax2);
           } else {
-            fl = hsro_law_f32(law, dist2, p->rsm, p->fsrrmax);
+            fl = law_f32(law, dist2, p->rsm, p->fsrrmax, interp ? &I : NULL);
           }
           float f_over_r = m[i] * bufs[3][j] * fl;
           float updateq = 1.0f;
@@ -659,6 +770,7 @@ void hsro_force_f32(int law, const hsro_params *p, const hsro_node *t,
     }
     for (int k = 0; k < 4; ++k) free(bufs[k]);
   }
+  if (interp) interp_free(&I);
 }
 
 void hsro_force_step16(const hsro_params *p, const hsro_node *t,

@@ -78,7 +78,7 @@ typedef struct {
   float min_loc[3], max_loc[3];     /* tree box (root bbox seed) */
   float min_force[3], max_force[3]; /* force box */
   int32_t tdpts;       /* pseudo-particles per node: 1 (also when 0) or 12 */
-  int32_t reserved;
+  int32_t n_interp;    /* HSRO_LAW_SR_INTERP: table size nInterp (>= 2) */
   float *ppm_t;        /* tdpts > 1: pseudo masses, tdpts per node in node
                           order (the build writes them, lists and kernels
                           read them); node.ppm[0] then holds the first */
@@ -89,7 +89,11 @@ enum {
   HSRO_LAW_STEP16 = 0,   /* BGQStep16.c:167-187: P5 law, r2>0 guard, pow() in double */
   HSRO_LAW_SR_POLY5 = 1, /* ForceLawSR + FGridEvalPoly, POLY_ORDER 5 (ForceLaw.cxx:107-116,137-141,187-192) */
   HSRO_LAW_SR_POLY6 = 2, /* ForceLawSR + FGridEvalPoly, POLY_ORDER 6 (stock build, ForceLaw.cxx:96-105) */
-  HSRO_LAW_NEWTON = 3    /* ForceLawNewton (ForceLaw.h:105-110) */
+  HSRO_LAW_NEWTON = 3,   /* ForceLawNewton (ForceLaw.h:105-110) */
+  HSRO_LAW_SR_FIT = 4,   /* ForceLawSR + FGridEvalFit, the reference's default law
+                            (Particles.cxx:173-174; ForceLaw.cxx:39-51,71-81) */
+  HSRO_LAW_SR_INTERP = 5 /* ForceLawSR + FGridEvalInterp(nInterp)
+                            (ForceLaw.cxx:54-67,145-171) */
 };
 
 /* BGQCM.c:181-212, generic branch. */
@@ -164,6 +168,15 @@ void hsro_force_f64(const hsro_params *p, const hsro_node *nodes,
  * reference's ForceLaw.cxx (returns f_over_r without the mass factors and,
  * for STEP16, with the r2 < fsrrmax^2 mask and r2 > 0 guard applied). */
 float hsro_law_f32(int law, float r2, float rsm, float fsrrmax);
+/* hsro_law_f32 over n values (n_interp: table size of HSRO_LAW_SR_INTERP). */
+void hsro_law_f32_vec(int law, int64_t n, const float *r2, float *out, float rsm,
+                      float fsrrmax, int n_interp);
+/* Polynomial coefficients a0.. of a polynomial law (STEP16 and SR_POLY5: the
+ * five-term set of BGQStep16.c:40-45,167 / ForceLaw.cxx:109-114; SR_POLY6:
+ * ForceLaw.cxx:98-104) into out[7]; returns how many (0: not a polynomial law). */
+int hsro_law_coeffs(int law, float *out);
+/* FGrid::fgor_r2_interp (ForceLaw.cxx:54-67): r2[i], f[i] for i < n. */
+void hsro_fgrid_table(int n, float *r2, float *f);
 
 /* Brute-force FP64 short-range force on particle i from all particles j with
  * 0 < r2 < fsrrmax^2 (the oa -> 0 limit of the tree). */

@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libhsrref.so")
 
-LAW_STEP16, LAW_SR_POLY5, LAW_SR_POLY6, LAW_NEWTON = 0, 1, 2, 3
+LAW_STEP16, LAW_SR_POLY5, LAW_SR_POLY6, LAW_NEWTON, LAW_SR_FIT, LAW_SR_INTERP = 0, 1, 2, 3, 4, 5
 
 
 class Node(C.Structure):
@@ -51,7 +51,7 @@ class Params(C.Structure):
                 ("tan_oa", C.c_float), ("ppc", C.c_float),
                 ("min_loc", C.c_float * 3), ("max_loc", C.c_float * 3),
                 ("min_force", C.c_float * 3), ("max_force", C.c_float * 3),
-                ("tdpts", C.c_int32), ("reserved", C.c_int32), ("ppm_t", C.c_void_p)]
+                ("tdpts", C.c_int32), ("n_interp", C.c_int32), ("ppm_t", C.c_void_p)]
 
 
 _lib = None
@@ -71,52 +71,72 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        P = np.ctypeslib.ndpointer
-        f32 = P(np.float32, flags="C")
-        f64 = P(np.float64, flags="C")
-        i64 = P(np.int64, flags="C")
-        u16 = P(np.uint16, flags="C")
-        nodes = P(NODE_DTYPE, flags="C")
-        L.hsro_cm.argtypes = [C.c_int64, f32, f32, f32, f32, f32, f32, f32]
-        L.hsro_partition.argtypes = [C.c_int64] + [f32] * 8 + [i64, u16, C.c_float]
-        L.hsro_partition.restype = C.c_int64
-        L.hsro_build.argtypes = [C.POINTER(Params), C.c_int64] + [f32] * 8 + [i64, u16, nodes, C.c_int64]
-        L.hsro_build.restype = C.c_int64
-        L.hsro_leaf_order.argtypes = [C.POINTER(Params), nodes, C.c_int64, i64, C.c_int64]
-        L.hsro_leaf_order.restype = C.c_int64
-        L.hsro_list.argtypes = [C.POINTER(Params), nodes, C.c_int64, C.c_int64, i64, i64, C.c_int64, C.POINTER(C.c_int64)]
-        L.hsro_list.restype = C.c_int64
-        L.hsro_lists_all.argtypes = [C.POINTER(Params), nodes, C.c_int64, i64, C.c_int64, i64, i64, i64, C.c_int64, i64]
-        L.hsro_lists_all.restype = C.c_int64
-        L.hsro_force_f32.argtypes = [C.c_int, C.POINTER(Params), nodes, f32, f32, f32, f32, i64, C.c_int64, i64, i64, i64, f32, f32, f32]
-        L.hsro_force_step16.argtypes = [C.POINTER(Params), nodes, f32, f32, f32, f32, i64, C.c_int64, i64, i64, i64, f32, f32, f32]
-        L.hsro_force_f64.argtypes = [C.POINTER(Params), nodes, f32, f32, f32, f32, i64, C.c_int64, i64, i64, i64, f64, f64, f64, C.c_void_p]
-        L.hsro_law_f32.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float]
-        L.hsro_law_f32.restype = C.c_float
-        L.hsro_direct_f64.argtypes = [C.c_int64, f32, f32, f32, f32, i64, C.c_int64, C.c_float, C.c_float, f64, f64, f64]
-        L.hsro_adjoint_f64.argtypes = [C.POINTER(Params), nodes, f64, f64, f64, f64, i64, C.c_int64, i64, i64, i64, f64, f64, f64, f64, f64, f64]
-        L.hsro_objective_f64.argtypes = [C.POINTER(Params), nodes, f64, f64, f64, f64, i64, C.c_int64, i64, i64, i64, f64, f64, f64]
-        L.hsro_objective_f64.restype = C.c_double
-        L.hsro_force_f64d.argtypes = [C.POINTER(Params), nodes, f64, f64, f64, f64, i64, C.c_int64, i64, i64, i64, f64, f64, f64]
-        L.hsro_cell_sort_perm.argtypes = [C.c_int64, f32, f32, f32, P(np.int32, flags="C"), i64]
-        L.hsro_cell_sort_perm.restype = C.c_int64
-        L.hsro_pppts.argtypes = [C.c_int, C.c_float, f32, f32, f32, f32]
-        i32 = P(np.int32, flags="C")
-        L.hsro_cic.argtypes = [C.c_int64, f32, f32, f32, i32, C.c_float, f32]
-        L.hsro_inverse_cic.argtypes = [C.c_int64, f32, f32, f32, i32, f32, C.c_float,
-                                       C.c_float, f32]
-        L.hsro_map1_factor.argtypes = [C.c_float] * 4
-        L.hsro_map1_factor.restype = C.c_float
-        L.hsro_drift.argtypes = [C.c_int64] + [f32] * 6 + [C.c_float]
-        L.hsro_map2_coeff.argtypes = [C.c_float, C.c_double, C.c_double, C.c_double]
-        L.hsro_map2_coeff.restype = C.c_float
-        L.hsro_domain_init.argtypes = [C.POINTER(Domain)]
-        L.hsro_refresh.argtypes = [C.POINTER(Domain), C.c_float, C.c_int64, C.c_int64] + \
-            [f32] * 8 + [i64, u16, C.POINTER(C.c_int64)]
-        L.hsro_refresh.restype = C.c_int64
-        _lib = L
+        _lib = load(LIB_PATH)
     return _lib
+
+
+def use(library):
+    """Make the wrappers below call `library` (a handle from load()) and
+    return the one they called before; the mutation tests swap in oracles
+    with planted mistakes this way."""
+    global _lib
+    old = _lib
+    _lib = library
+    return old
+
+
+def load(path):
+    """Load an oracle library built from hsr_oracle.c and declare its
+    functions."""
+    L = C.CDLL(path)
+    P = np.ctypeslib.ndpointer
+    f32 = P(np.float32, flags="C")
+    f64 = P(np.float64, flags="C")
+    i64 = P(np.int64, flags="C")
+    u16 = P(np.uint16, flags="C")
+    nodes = P(NODE_DTYPE, flags="C")
+    L.hsro_cm.argtypes = [C.c_int64, f32, f32, f32, f32, f32, f32, f32]
+    L.hsro_partition.argtypes = [C.c_int64] + [f32] * 8 + [i64, u16, C.c_float]
+    L.hsro_partition.restype = C.c_int64
+    L.hsro_build.argtypes = [C.POINTER(Params), C.c_int64] + [f32] * 8 + [i64, u16, nodes, C.c_int64]
+    L.hsro_build.restype = C.c_int64
+    L.hsro_leaf_order.argtypes = [C.POINTER(Params), nodes, C.c_int64, i64, C.c_int64]
+    L.hsro_leaf_order.restype = C.c_int64
+    L.hsro_list.argtypes = [C.POINTER(Params), nodes, C.c_int64, C.c_int64, i64, i64, C.c_int64, C.POINTER(C.c_int64)]
+    L.hsro_list.restype = C.c_int64
+    L.hsro_lists_all.argtypes = [C.POINTER(Params), nodes, C.c_int64, i64, C.c_int64, i64, i64, i64, C.c_int64, i64]
+    L.hsro_lists_all.restype = C.c_int64
+    L.hsro_force_f32.argtypes = [C.c_int, C.POINTER(Params), nodes, f32, f32, f32, f32, i64, C.c_int64, i64, i64, i64, f32, f32, f32]
+    L.hsro_force_step16.argtypes = [C.POINTER(Params), nodes, f32, f32, f32, f32, i64, C.c_int64, i64, i64, i64, f32, f32, f32]
+    L.hsro_force_f64.argtypes = [C.POINTER(Params), nodes, f32, f32, f32, f32, i64, C.c_int64, i64, i64, i64, f64, f64, f64, C.c_void_p]
+    L.hsro_law_f32.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float]
+    L.hsro_law_f32.restype = C.c_float
+    L.hsro_direct_f64.argtypes = [C.c_int64, f32, f32, f32, f32, i64, C.c_int64, C.c_float, C.c_float, f64, f64, f64]
+    L.hsro_adjoint_f64.argtypes = [C.POINTER(Params), nodes, f64, f64, f64, f64, i64, C.c_int64, i64, i64, i64, f64, f64, f64, f64, f64, f64]
+    L.hsro_objective_f64.argtypes = [C.POINTER(Params), nodes, f64, f64, f64, f64, i64, C.c_int64, i64, i64, i64, f64, f64, f64]
+    L.hsro_objective_f64.restype = C.c_double
+    L.hsro_force_f64d.argtypes = [C.POINTER(Params), nodes, f64, f64, f64, f64, i64, C.c_int64, i64, i64, i64, f64, f64, f64]
+    L.hsro_cell_sort_perm.argtypes = [C.c_int64, f32, f32, f32, P(np.int32, flags="C"), i64]
+    L.hsro_cell_sort_perm.restype = C.c_int64
+    L.hsro_pppts.argtypes = [C.c_int, C.c_float, f32, f32, f32, f32]
+    i32 = P(np.int32, flags="C")
+    L.hsro_cic.argtypes = [C.c_int64, f32, f32, f32, i32, C.c_float, f32]
+    L.hsro_inverse_cic.argtypes = [C.c_int64, f32, f32, f32, i32, f32, C.c_float,
+                                   C.c_float, f32]
+    L.hsro_map1_factor.argtypes = [C.c_float] * 4
+    L.hsro_map1_factor.restype = C.c_float
+    L.hsro_drift.argtypes = [C.c_int64] + [f32] * 6 + [C.c_float]
+    L.hsro_map2_coeff.argtypes = [C.c_float, C.c_double, C.c_double, C.c_double]
+    L.hsro_map2_coeff.restype = C.c_float
+    L.hsro_domain_init.argtypes = [C.POINTER(Domain)]
+    L.hsro_refresh.argtypes = [C.POINTER(Domain), C.c_float, C.c_int64, C.c_int64] + \
+        [f32] * 8 + [i64, u16, C.POINTER(C.c_int64)]
+    L.hsro_refresh.restype = C.c_int64
+    L.hsro_law_f32_vec.argtypes = [C.c_int, C.c_int64, f32, f32, C.c_float, C.c_float, C.c_int]
+    L.hsro_fgrid_table.argtypes = [C.c_int, f32, f32]
+    L.hsro_law_coeffs.argtypes = [C.c_int, P(np.float32, flags="C")]
+    L.hsro_law_coeffs.restype = C.c_int
+    return L
 
 
 def ref():
@@ -139,6 +159,9 @@ def ref():
             getattr(R, name).argtypes = [C.c_float]
             getattr(R, name).restype = C.c_float
         R.hsrref_fgrid_rmax.restype = C.c_float
+        R.hsrref_sr_interp_f_over_r.argtypes = [C.c_float, C.c_float, C.c_int]
+        R.hsrref_sr_interp_f_over_r.restype = C.c_float
+        R.hsrref_interp_table.argtypes = [C.c_int, f32, f32]
         _ref = R
     return _ref
 
@@ -157,21 +180,26 @@ class TreeParams:
     nd: int = 128
     ppc: float = 0.9
     tdpts: int = 1          # 1: RCBMonopoleForceTree, 12: RCBQuadrupoleForceTree
+    n_interp: int = 0       # LAW_SR_INTERP: FGridEvalInterp's table size
+    tan: float = None       # tanOpeningAngle itself, instead of tanf(oa)
 
     def tan_oa(self) -> float:
         # tanf(oa) in float, as RCBForceTree.cxx:381 evaluates it
-        return float(np.float32(math.tan(float(np.float32(self.oa)))))
+        if self.tan is not None:
+            return float(np.float32(self.tan))
+        return tanf(self.oa)
 
     def ctypes(self, tree=None) -> Params:
         """The C parameter block; a tree with T > 1 lends its pseudo masses."""
         p = Params()
         p.tdpts = int(self.tdpts)
+        p.n_interp = int(self.n_interp)
         if tree is not None and getattr(tree, "ppm_t", None) is not None:
             p.ppm_t = tree.ppm_t.ctypes.data
         p.nd = int(self.nd)
         p.fsrrmax = float(np.float32(self.fsrrmax))
         p.rsm = float(np.float32(self.rsm))
-        p.tan_oa = tanf(self.oa)
+        p.tan_oa = self.tan_oa()
         p.ppc = float(np.float32(self.ppc))
         for k in range(3):
             p.min_loc[k] = float(self.min_loc[k])
@@ -325,11 +353,27 @@ def direct_f64(x, y, z, m, targets, rsm=0.1, fsrrmax=3.12):
     return ax, ay, az
 
 
-def law_f32(law: int, r2, rsm=0.1, fsrrmax=3.12):
-    L = lib()
-    return np.array([L.hsro_law_f32(law, float(v), float(np.float32(rsm)),
-                                    float(np.float32(fsrrmax)))
-                     for v in np.asarray(r2, np.float32)], np.float32)
+def law_f32(law: int, r2, rsm=0.1, fsrrmax=3.12, n_interp=0):
+    r2 = np.ascontiguousarray(r2, np.float32).ravel()
+    out = np.zeros_like(r2)
+    lib().hsro_law_f32_vec(law, r2.shape[0], r2, out, float(np.float32(rsm)),
+                           float(np.float32(fsrrmax)), int(n_interp))
+    return out
+
+
+def law_coeffs(law: int) -> np.ndarray:
+    """The polynomial coefficients a0.. the oracle uses for a law."""
+    out = np.zeros(7, np.float32)
+    k = lib().hsro_law_coeffs(law, out)
+    return out[:k].copy()
+
+
+def fgrid_table(n: int):
+    """FGrid::fgor_r2_interp's table (r2, f) of n entries."""
+    r2 = np.zeros(n, np.float32)
+    f = np.zeros(n, np.float32)
+    lib().hsro_fgrid_table(n, r2, f)
+    return r2, f
 
 
 def cm(x, y, z, m):

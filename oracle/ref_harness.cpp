@@ -60,6 +60,27 @@ float hsrref_interp_eval(float r2) {
   return ev.eval(r2);
 }
 
+// ForceLawSR(FGridEvalInterp(FGrid, n), rsm)::f_over_r and FGrid's r^2
+// table (ForceLaw.cxx:54-67, 145-171, 175-192) for any table size n >= 2.
+float hsrref_sr_interp_f_over_r(float r2, float rsm, int n) {
+  static FGrid fg;
+  FGridEvalInterp ev(&fg, n);
+  ForceLawSR fl(&ev, rsm);
+  return fl.f_over_r(r2);
+}
+
+void hsrref_interp_table(int n, float *r2, float *f) {
+  FGrid fg;
+  float *a = nullptr, *b = nullptr;
+  fg.fgor_r2_interp(n, &a, &b);
+  for (int i = 0; i < n; ++i) {
+    r2[i] = a[i];
+    f[i] = b[i];
+  }
+  free(a);
+  free(b);
+}
+
 float hsrref_newton_f_over_r(float r2) {
   ForceLawNewton fl;
   return fl.f_over_r(r2);

@@ -8,7 +8,10 @@ ForceLaw.cxx and BGQCM.c where they lie).  Output:
   ref_forcelaw.npz  ForceLawSR(FGridEvalPoly) and ForceLawSR(FGridEvalFit)
                     f_over_r, FGridEvalFit::eval, FGridEvalPoly::eval and
                     FGridEvalInterp(1024)::eval on an r^2 grid, plus
-                    FGrid::rmax (ForceLaw.cxx)
+                    FGrid::rmax (ForceLaw.cxx); ForceLawSR(FGridEvalInterp(n))
+                    f_over_r for n in INTERP_SIZES on the same grid and on
+                    each table's nodes and midpoints, and FGrid's r^2
+                    tables of those sizes
 """
 import ctypes as C
 import os
@@ -46,6 +49,27 @@ def cm_cases(rng):
     return cases
 
 
+INTERP_SIZES = (2, 97, 1024, 4096)
+
+
+def interp_vectors(R, r2, rsm):
+    """ForceLawSR(FGridEvalInterp(n)) on the r^2 grid and on n's own table
+    nodes and midpoints (where the index rounds), with the tables."""
+    out = {}
+    for n in INTERP_SIZES:
+        tr2 = np.zeros(n, np.float32)
+        tf = np.zeros(n, np.float32)
+        R.hsrref_interp_table(n, tr2, tf)
+        mid = ((tr2[:-1].astype(np.float64) + tr2[1:]) / 2).astype(np.float32)
+        pts = np.concatenate([r2, tr2, mid, np.nextafter(tr2, np.float32(np.inf))]).astype(np.float32)
+        out[f"interp{n}_r2"] = pts
+        out[f"sr_interp{n}"] = np.array([R.hsrref_sr_interp_f_over_r(float(v), float(rsm), n)
+                                         for v in pts], np.float32)
+        out[f"table{n}_r2"] = tr2
+        out[f"table{n}_f"] = tf
+    return out
+
+
 def main():
     R = O.ref()
     if R is None:
@@ -74,7 +98,8 @@ def main():
                         sr_fit=f(R.hsrref_sr_fit_f_over_r, float(rsm)),
                         fgor=f(R.hsrref_fgor_r2), poly6=f(R.hsrref_poly_eval),
                         interp1024=f(R.hsrref_interp_eval),
-                        rmax=np.float32(R.hsrref_fgrid_rmax()))
+                        rmax=np.float32(R.hsrref_fgrid_rmax()),
+                        **interp_vectors(R, r2, rsm))
     print("wrote", os.listdir(HERE))
 
 
