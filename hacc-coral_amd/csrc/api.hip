@@ -362,6 +362,12 @@ hsr_status hsr_create(hsr_ctx **out, const hsr_params *p) {
     delete c;
     return HSR_ERR_HIP;
   }
+  // the allocation-free entry points' scratch address on this device
+  if ((s = noalloc_scratch_init(c->device))) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return s;
+  }
   for (int k = 0; k < 8; ++k) (void)hipEventCreate(&c->ev[k]);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, c->device) == hipSuccess)
@@ -976,6 +982,32 @@ hsr_status hsr_resort(hsr_ctx *c, int64_t count, const int32_t ng[3], float *x,
   float ms = 0.f;
   HSR_HIP(hipEventElapsedTime(&ms, c->sub.ev[2], c->sub.ev[3]));
   c->sub.resort_ms = ms;
+  return HSR_OK;
+}
+
+hsr_status hsr_resort_noalloc(hsr_ctx *c, int64_t count, const int32_t ng[3], float *x,
+                              float *y, float *z, float *vx, float *vy, float *vz,
+                              float *mass, float *phi, int64_t *id, uint16_t *mask,
+                              int64_t *inside, int ptr_flags) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (!x || !y || !z || !inside) {
+    g_err = "hsr_resort_noalloc: null positions or output";
+    return HSR_ERR_ARG;
+  }
+  if (!(ptr_flags & HSR_DEVICE_PTRS)) {
+    g_err = "hsr_resort_noalloc: device pointers only (HSR_DEVICE_PTRS); host arrays need "
+            "staging, which hsr_resort allocates";
+    return HSR_ERR_ARG;
+  }
+  if ((s = check_count_grid("hsr_resort_noalloc", count, ng))) return s;
+  if (count > HSR_RESORT_NOALLOC_MAX) {
+    g_err = "hsr_resort_noalloc: " + std::to_string(count) + " particles, above the scratch " +
+            "array's limit of " + std::to_string(HSR_RESORT_NOALLOC_MAX) + " (use hsr_resort)";
+    return HSR_ERR_ARG;
+  }
+  float *f[8] = {x, y, z, vx, vy, vz, mass, phi};
+  if ((s = resort_noalloc_run(c, count, ng, f, id, mask, inside))) return s;
   return HSR_OK;
 }
 

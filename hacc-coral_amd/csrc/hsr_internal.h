@@ -246,6 +246,12 @@ hsr_status drift_run(hsr_ctx *c, int64_t n, float k, float *x, float *y, float *
 hsr_status resort_run(hsr_ctx *c, int64_t n, const int32_t ng[3], float *f[8],
                       int64_t *id, uint16_t *mask, int64_t *inside);
 hsr_status fill_run(hsr_ctx *c, int64_t n, float v, float *a);
+// Allocation-free resort over the library's static scratch array: the
+// device's scratch address is looked up once per device at context
+// creation (noalloc_scratch_init, device current).
+hsr_status noalloc_scratch_init(int device);
+hsr_status resort_noalloc_run(hsr_ctx *c, int64_t n, const int32_t ng[3], float *f[8],
+                              int64_t *id, uint16_t *mask, int64_t *inside);
 
 // Waits for the stream by polling it for a while before blocking: the
 // tree build reads a few counters back once per level and the list build a

@@ -24,6 +24,8 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 namespace {
 
@@ -131,20 +133,36 @@ hsr_status drift_run(hsr_ctx *c, int64_t n, float k, float *x, float *y, float *
   return HSR_OK;
 }
 
-// Cell keys of n particles and their stable radix sort; leaves the source
-// index of each destination in the returned array and the number of
-// particles outside the grid in *outside (device).
-static hsr_status sort_cells(hsr_ctx *c, int64_t n, const int32_t ng[3], const float *x,
-                             const float *y, const float *z, const int32_t **src,
-                             unsigned long long **outside_dev) {
-  hipStream_t st = c->stream;
+static int cell_key_bits(const int32_t ng[3]) {
   const uint64_t Ng = (uint64_t)ng[0] * (uint64_t)ng[1] * (uint64_t)ng[2];
   int bits = 1;
   while (bits < 32 && (Ng >> bits) != 0) ++bits;  // keys 0..Ng need this many bits
+  return bits;
+}
+
+// Bytes of hipcub radix-sort temporary storage for n (cell, index) pairs.
+static hsr_status sort_tmp_bytes(int64_t n, int bits, hipStream_t st, size_t *tmp) {
+  *tmp = 0;
+  uint32_t *k = nullptr;
+  int32_t *v = nullptr;
+  HSR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, *tmp, k, k, v, v, (int)n, 0, bits, st));
+  return HSR_OK;
+}
+
+// Cell keys of n particles and their stable radix sort, in the caller's
+// scratch: `keys` holds 4n words (key, key', val, val') followed by the
+// 8-byte outside counter; `tmp` holds tmp_bytes of sort storage.  Leaves the
+// source index of each destination in *src and the number of particles
+// outside the grid in *outside_dev (device).
+static hsr_status sort_cells(hsr_ctx *c, int64_t n, const int32_t ng[3], const float *x,
+                             const float *y, const float *z, uint32_t *keys, void *tmp,
+                             size_t tmp_bytes, const int32_t **src,
+                             unsigned long long **outside_dev) {
+  hipStream_t st = c->stream;
+  const uint64_t Ng = (uint64_t)ng[0] * (uint64_t)ng[1] * (uint64_t)ng[2];
+  const int bits = cell_key_bits(ng);
   const size_t nn = (size_t)n;
-  // key, key', val, val' (4 x 4 B) and the outside counter
-  HSR_HIP(dbuf_reserve(c->sub.keys, 16 * nn + 256));
-  uint32_t *key = dptr<uint32_t>(c->sub.keys);
+  uint32_t *key = keys;
   uint32_t *key2 = key + nn;
   int32_t *val = (int32_t *)(key2 + nn);
   int32_t *val2 = val + nn;
@@ -153,13 +171,25 @@ static hsr_status sort_cells(hsr_ctx *c, int64_t n, const int32_t ng[3], const f
   k_cell_keys<<<nblk(n, 256), 256, 0, st>>>(n, x, y, z, ng[0], ng[1], ng[2], (uint32_t)Ng,
                                           key, val, outside);
   HSR_HIP(hipGetLastError());
-  size_t tmp = 0;
-  HSR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, key, key2, val, val2, (int)n, 0, bits, st));
-  HSR_HIP(dbuf_reserve(c->s.cub, tmp));
-  HSR_HIP(hipcub::DeviceRadixSort::SortPairs(c->s.cub.p, tmp, key, key2, val, val2, (int)n, 0, bits, st));
+  size_t t = tmp_bytes;
+  HSR_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, t, key, key2, val, val2, (int)n, 0, bits, st));
   *src = val2;
   *outside_dev = outside;
   return HSR_OK;
+}
+
+// sort_cells over the context's buffers (grown as needed).
+static hsr_status sort_cells_ctx(hsr_ctx *c, int64_t n, const int32_t ng[3], const float *x,
+                                 const float *y, const float *z, const int32_t **src,
+                                 unsigned long long **outside_dev) {
+  size_t tmp = 0;
+  hsr_status s = sort_tmp_bytes(n, cell_key_bits(ng), c->stream, &tmp);
+  if (s) return s;
+  // key, key', val, val' (4 x 4 B) and the outside counter
+  HSR_HIP(dbuf_reserve(c->sub.keys, 16 * (size_t)n + 256));
+  HSR_HIP(dbuf_reserve(c->s.cub, tmp));
+  return sort_cells(c, n, ng, x, y, z, dptr<uint32_t>(c->sub.keys), c->s.cub.p, tmp, src,
+                    outside_dev);
 }
 
 hsr_status resort_into(hsr_ctx *c, int64_t n, const int32_t ng[3], const PArrays &in,
@@ -169,7 +199,7 @@ hsr_status resort_into(hsr_ctx *c, int64_t n, const int32_t ng[3], const PArrays
   if (n <= 0) return HSR_OK;
   const int32_t *src = nullptr;
   unsigned long long *outside = nullptr;
-  hsr_status s = sort_cells(c, n, ng, in.f[0], in.f[1], in.f[2], &src, &outside);
+  hsr_status s = sort_cells_ctx(c, n, ng, in.f[0], in.f[1], in.f[2], &src, &outside);
   if (s) return s;
   Ten ti{}, to{};
   for (int a = 0; a < 8; ++a) {
@@ -189,24 +219,23 @@ hsr_status resort_into(hsr_ctx *c, int64_t n, const int32_t ng[3], const PArrays
   return HSR_OK;
 }
 
-hsr_status resort_run(hsr_ctx *c, int64_t n, const int32_t ng[3], float *f[8],
-                      int64_t *id, uint16_t *mask, int64_t *inside) {
-  hipStream_t st = c->stream;
-  *inside = 0;
-  if (n <= 0) return HSR_OK;
+static size_t stage_bytes(int64_t n, float *const f[8], const int64_t *id, const uint16_t *mask) {
   const size_t nn = (size_t)n;
-  const int32_t *val2 = nullptr;
-  unsigned long long *outside = nullptr;
-  hsr_status s = sort_cells(c, n, ng, f[0], f[1], f[2], &val2, &outside);
-  if (s) return s;
-  // gather into staging, then copy back in place
   size_t bytes = 0;
   for (int a = 0; a < 8; ++a)
     if (f[a]) bytes += sizeof(float) * nn;
   if (id) bytes += sizeof(int64_t) * nn;
   if (mask) bytes += sizeof(uint16_t) * nn;
-  HSR_HIP(dbuf_reserve(c->sub.stage, bytes + 256));
-  char *sp = (char *)c->sub.stage.p;
+  return bytes;
+}
+
+// Gather the present arrays into `stage` in the sorted order, copy them back
+// in place and read the outside count.
+static hsr_status gather_back(hsr_ctx *c, int64_t n, const int32_t *src,
+                              const unsigned long long *outside, float *const f[8], int64_t *id,
+                              uint16_t *mask, char *sp, int64_t *inside) {
+  hipStream_t st = c->stream;
+  const size_t nn = (size_t)n;
   Ten in{}, out{};
   if (id) {  // 8-byte arrays first keeps every staging array aligned
     in.id = id;
@@ -223,7 +252,7 @@ hsr_status resort_run(hsr_ctx *c, int64_t n, const int32_t ng[3], float *f[8],
     in.mask = mask;
     out.mask = (uint16_t *)sp;
   }
-  k_gather_ten<<<nblk(n, 256), 256, 0, st>>>(n, val2, in, out);
+  k_gather_ten<<<nblk(n, 256), 256, 0, st>>>(n, src, in, out);
   HSR_HIP(hipGetLastError());
   for (int a = 0; a < 8; ++a)
     if (f[a]) HSR_HIP(hipMemcpyAsync(f[a], out.f[a], sizeof(float) * nn, hipMemcpyDeviceToDevice, st));
@@ -234,6 +263,88 @@ hsr_status resort_run(hsr_ctx *c, int64_t n, const int32_t ng[3], float *f[8],
   HSR_HIP(hipStreamSynchronize(st));
   *inside = n - (int64_t)h_out;
   return HSR_OK;
+}
+
+hsr_status resort_run(hsr_ctx *c, int64_t n, const int32_t ng[3], float *f[8],
+                      int64_t *id, uint16_t *mask, int64_t *inside) {
+  *inside = 0;
+  if (n <= 0) return HSR_OK;
+  const int32_t *val2 = nullptr;
+  unsigned long long *outside = nullptr;
+  hsr_status s = sort_cells_ctx(c, n, ng, f[0], f[1], f[2], &val2, &outside);
+  if (s) return s;
+  HSR_HIP(dbuf_reserve(c->sub.stage, stage_bytes(n, f, id, mask) + 256));
+  return gather_back(c, n, val2, outside, f, id, mask, (char *)c->sub.stage.p, inside);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free resort (hsr_resort_noalloc).  Its scratch is one
+// file-scope device array, `scratch`, of fixed size, held by every device the
+// library runs on until the process ends:
+//   [keys: 16 B per particle + 256 B][staging: 42 B per particle + 256 B]
+//   [radix-sort storage: the rest]
+// for at most HSR_RESORT_NOALLOC_MAX particles.  Host code never passes the
+// array's name to a kernel or copy (on the host it is only a stand-in
+// symbol): each device's address comes from hipGetSymbolAddress, taken once
+// per device with that device current, and is kept in g_scratch_dev.
+namespace {
+constexpr size_t kNoallocKeys = 16ull * HSR_RESORT_NOALLOC_MAX + 256;
+constexpr size_t kNoallocStage = 42ull * HSR_RESORT_NOALLOC_MAX + 256;
+constexpr size_t kNoallocSort = 64ull << 20;
+constexpr size_t kNoallocBytes = kNoallocKeys + kNoallocStage + kNoallocSort;
+static_assert(kNoallocBytes == HSR_RESORT_NOALLOC_SCRATCH_BYTES, "scratch size");
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char *g_scratch_dev[kMaxDevices];
+}  // namespace
+
+__device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+hsr_status noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) {
+    hsr_set_error("scratch set-up: device ordinal beyond the library's table");
+    return HSR_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[device]) return HSR_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    hsr_set_error(std::string("scratch set-up: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+    return HSR_ERR_HIP;
+  }
+  g_scratch_dev[device] = (char *)p;
+  return HSR_OK;
+}
+
+hsr_status resort_noalloc_run(hsr_ctx *c, int64_t n, const int32_t ng[3], float *f[8],
+                              int64_t *id, uint16_t *mask, int64_t *inside) {
+  *inside = 0;
+  char *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    base = (c->device >= 0 && c->device < kMaxDevices) ? g_scratch_dev[c->device] : nullptr;
+  }
+  if (!base) {
+    hsr_set_error("hsr_resort_noalloc: the device's scratch address was not set up");
+    return HSR_ERR_STATE;
+  }
+  if (n <= 0) return HSR_OK;
+  size_t tmp = 0;
+  hsr_status s = sort_tmp_bytes(n, cell_key_bits(ng), c->stream, &tmp);
+  if (s) return s;
+  if (tmp > kNoallocSort) {
+    hsr_set_error("hsr_resort_noalloc: the radix sort needs more storage than the scratch array holds");
+    return HSR_ERR_UNSUPPORTED;
+  }
+  uint32_t *keys = (uint32_t *)base;
+  char *stage = base + kNoallocKeys;
+  void *sort_tmp = base + kNoallocKeys + kNoallocStage;
+  const int32_t *val2 = nullptr;
+  unsigned long long *outside = nullptr;
+  if ((s = sort_cells(c, n, ng, f[0], f[1], f[2], keys, sort_tmp, tmp, &val2, &outside)))
+    return s;
+  return gather_back(c, n, val2, outside, f, id, mask, stage, inside);
 }
 
 hsr_status fill_run(hsr_ctx *c, int64_t n, float v, float *a) {

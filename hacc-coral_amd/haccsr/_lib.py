@@ -29,7 +29,7 @@ EXPORTS = [
     "hsr_get_lists", "hsr_build_many", "hsr_lists_many", "hsr_set_tuning",
     "hsr_get_tuning", "hsr_forces_f64", "hsr_leaf_momentum", "hsr_drift",
     "hsr_resort", "hsr_map2", "hsr_subcycle", "hsr_get_pseudo_masses", "hsr_cic",
-    "hsr_inverse_cic", "hsr_domain_init", "hsr_refresh",
+    "hsr_inverse_cic", "hsr_domain_init", "hsr_refresh", "hsr_resort_noalloc",
 ]
 
 
@@ -149,6 +149,7 @@ def load(path: str | None = None):
     L.hsr_leaf_momentum.argtypes = [vp] + [vp] * 4 + [C.c_int]
     L.hsr_drift.argtypes = [vp, i64] + [vp] * 6 + [C.c_float] * 4 + [C.c_int]
     L.hsr_resort.argtypes = [vp, i64, vp] + [vp] * 10 + [vp, C.c_int]
+    L.hsr_resort_noalloc.argtypes = [vp, i64, vp] + [vp] * 10 + [vp, C.c_int]
     L.hsr_map2.argtypes = [vp, C.POINTER(SubcycleParams), C.c_double, i64] + [vp] * 10 + \
         [vp, C.c_int]
     L.hsr_subcycle.argtypes = [vp, C.POINTER(SubcycleParams), i64] + [vp] * 10 + [vp, C.c_int]
@@ -335,6 +336,19 @@ class Context:
         check(self.L.hsr_resort(self.h, n, g.ctypes.data, _ptr(x), _ptr(y), _ptr(z),
                                 _ptr(vx), _ptr(vy), _ptr(vz), _ptr(mass), _ptr(phi),
                                 _ptr(ids), _ptr(mask), C.byref(inside), fl), "hsr_resort")
+        return int(inside.value)
+
+    def resort_noalloc(self, ng, x, y, z, vx=None, vy=None, vz=None, mass=None, phi=None,
+                       ids=None, mask=None) -> int:
+        """hsr_resort_noalloc: resortParticles on device arrays over the
+        library's static scratch (no device allocation); returns m_Np_last."""
+        n = int(x.shape[0])
+        g = np.ascontiguousarray(ng, np.int32)
+        inside = C.c_int64(0)
+        check(self.L.hsr_resort_noalloc(self.h, n, g.ctypes.data, _ptr(x), _ptr(y), _ptr(z),
+                                        _ptr(vx), _ptr(vy), _ptr(vz), _ptr(mass), _ptr(phi),
+                                        _ptr(ids), _ptr(mask), C.byref(inside),
+                                        HSR_DEVICE_PTRS), "hsr_resort_noalloc")
         return int(inside.value)
 
     def map2(self, params: "SubcycleParams", step_fraction, x, y, z, vx, vy, vz, mass,

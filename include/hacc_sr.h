@@ -239,6 +239,26 @@ hsr_status hsr_resort(hsr_ctx *ctx, int64_t count, const int32_t ng[3],
                       float *vz, float *mass, float *phi, int64_t *id,
                       uint16_t *mask, int64_t *inside, int ptr_flags);
 
+/* hsr_resort without allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same result as
+ * hsr_resort with HSR_DEVICE_PTRS, bit for bit, but the cell keys, the radix
+ * sort's storage and the staging copy live in one static device array of
+ * HSR_RESORT_NOALLOC_SCRATCH_BYTES that the library holds on every device it
+ * runs on until the process ends; the call makes no hipMalloc or hipFree.
+ * Limits: device pointers only (ptr_flags must be HSR_DEVICE_PTRS) and at
+ * most HSR_RESORT_NOALLOC_MAX particles (HSR_ERR_ARG beyond, before any
+ * launch).  Every call on a device shares that device's array, so calls that
+ * overlap on one device (other contexts, other streams or threads) race: the
+ * caller serialises them. */
+#define HSR_RESORT_NOALLOC_MAX ((int64_t)1 << 22)
+#define HSR_RESORT_NOALLOC_SCRATCH_BYTES \
+  (58ull * (1ull << 22) + 512ull + (64ull << 20))
+hsr_status hsr_resort_noalloc(hsr_ctx *ctx, int64_t count, const int32_t ng[3],
+                              float *x, float *y, float *z, float *vx, float *vy,
+                              float *vz, float *mass, float *phi, int64_t *id,
+                              uint16_t *mask, int64_t *inside, int ptr_flags);
+
 /* Particles::subCycle with its short-range kick Particles::map2
  * (Particles.cxx:1176-1201, 1205-1280; RCB monopole tree): for each of nsub
  * sub-steps, drift by stepFraction * tau2 (map1), then resort, set every
