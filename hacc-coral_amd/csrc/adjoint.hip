@@ -1178,14 +1178,18 @@ hsr_status adjoint_run(hsr_ctx *c, const float *lx, const float *ly,
                        const float *lz, float *gx, float *gy, float *gz) {
   hipStream_t st = c->stream;
   AdjState &A = c->adj;
+  if (c->law.guard) {
+    hsr_set_error("hsr_adjoint: needs rsm > 0");
+    return HSR_ERR_UNSUPPORTED;
+  }
+  if (c->law.kind != LAWK_POLY) {
+    hsr_set_error("hsr_adjoint: implements the polynomial laws (STEP16, SR_POLY5, SR_POLY6)");
+    return HSR_ERR_UNSUPPORTED;
+  }
   if (A.gen != c->lists_gen || A.planned_mode != c->tuning.adjoint_mode) {
     hsr_status s = adjoint_plan(c);
     if (s != HSR_OK) return s;
     A.graph_key[0] = nullptr;  // re-capture
-  }
-  if (c->law.guard) {
-    hsr_set_error("hsr_adjoint: needs rsm > 0");
-    return HSR_ERR_UNSUPPORTED;
   }
   if (!c->tuning.use_graph) {
     HSR_HIP(hipEventRecord(c->ev[6], st));

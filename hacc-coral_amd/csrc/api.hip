@@ -56,8 +56,30 @@ hsr_status make_law(const hsr_params &p, ForceLawDev &L) {
     L.guard = L.eps2 > 0.f ? 0 : 1;
     break;
   }
+  case HSR_LAW_SR_FIT: {
+    // ForceLawSR masks r2 in [0, rmax^2] with FGridEvalFit's r2max()
+    // (ForceLaw.h:47, rmax()*rmax() in float); nbody1 masks r2 < fsrrmax^2
+    L.kind = LAWK_FIT;
+    float incl = std::nextafter(kRmax * kRmax, INFINITY);
+    L.rc2 = std::min(fsr2, incl);
+    L.guard = L.eps2 > 0.f ? 0 : 1;
+    break;
+  }
+  case HSR_LAW_SR_INTERP: {
+    // the cutoff follows the table's last r2 once hsr_set_law_table gives it
+    if (p.n_interp < 2) {
+      g_err = "HSR_LAW_SR_INTERP needs params.n_interp >= 2";
+      return HSR_ERR_ARG;
+    }
+    L.kind = LAWK_INTERP;
+    L.n_interp = p.n_interp;
+    L.rc2 = fsr2;
+    L.guard = L.eps2 > 0.f ? 0 : 1;
+    break;
+  }
   default:
-    g_err = "unsupported force law (device path implements STEP16, SR_POLY5, SR_POLY6)";
+    g_err = "unsupported force law (device path implements STEP16, SR_POLY5, SR_POLY6, "
+            "SR_FIT and SR_INTERP)";
     return HSR_ERR_UNSUPPORTED;
   }
   return HSR_OK;
@@ -376,6 +398,46 @@ hsr_status hsr_create(hsr_ctx **out, const hsr_params *p) {
   return HSR_OK;
 }
 
+hsr_status hsr_set_law_table(hsr_ctx *c, const float *r2, const float *f, int64_t n) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (c->law.kind != LAWK_INTERP) {
+    g_err = "hsr_set_law_table: the context's law is not HSR_LAW_SR_INTERP";
+    return HSR_ERR_STATE;
+  }
+  if (!r2 || !f || n != c->law.n_interp) {
+    g_err = "hsr_set_law_table: null table or n != params.n_interp";
+    return HSR_ERR_ARG;
+  }
+  for (int64_t i = 1; i < n; ++i)
+    if (!(r2[i] > r2[i - 1]) || !std::isfinite(f[i]) || !std::isfinite(f[i - 1])) {
+      g_err = "hsr_set_law_table: r2 must ascend and f be finite";
+      return HSR_ERR_ARG;
+    }
+  // FGridEvalInterp's derived range (ForceLaw.cxx:145-153): float fields
+  // assigned from double expressions
+  const float r2min = r2[0], r2max = r2[n - 1];
+  const float dr2 = (r2max - r2min) / (n - 1.0);
+  const float oodr2 = 1.0 / dr2;
+  std::vector<float4> bins((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    const float fi = f[i], fn = i + 1 < n ? f[i + 1] : f[i];
+    bins[(size_t)i] = make_float4(r2[i], fi, fn - fi, 0.f);
+  }
+  HSR_HIP(dbuf_reserve(c->law_bins, sizeof(float4) * (size_t)n));
+  HSR_HIP(hipMemcpy(c->law_bins.p, bins.data(), sizeof(float4) * (size_t)n, hipMemcpyHostToDevice));
+  c->law.bins = dptr<float4>(c->law_bins);
+  c->law.t_r2min = r2min;
+  c->law.t_r2max = r2max;
+  c->law.t_oodr2 = oodr2;
+  // ForceLawSR's mask r2 in [r2min, r2max] (ForceLaw.cxx:184-190) with
+  // r2min = r2[0] = 0, and nbody1's r2 < fsrrmax^2
+  const float fsr2 = c->p.fsrrmax * c->p.fsrrmax;
+  c->law.rc2 = std::min(fsr2, std::nextafter(r2max, INFINITY));
+  c->graph_gen = -1;  // a captured force phase holds the old law arguments
+  return HSR_OK;
+}
+
 hsr_status hsr_destroy(hsr_ctx *c) {
   if (!c) return HSR_OK;
   (void)hipSetDevice(c->device);
@@ -387,7 +449,7 @@ hsr_status hsr_destroy(hsr_ctx *c) {
                   &c->lists.plan, &c->lists.part, &c->s.qctr,
                   &c->adj.fw, &c->adj.fw_items, &c->adj.tr, &c->adj.tr_items, &c->adj.seg,
                   &c->adj.tj, &c->adj.par, &c->adj.counts, &c->adj.qctr, &c->adj.part, &c->adj.H,
-                  &c->adj.W, &c->adj.msum,
+                  &c->adj.W, &c->adj.msum, &c->law_bins,
                   &c->s.a, &c->s.b, &c->s.c, &c->s.d, &c->s.e, &c->s.f, &c->s.g, &c->s.h,
                   &c->s.cub, &c->s.cms, &c->s.cmfail, &c->s.cmx, &c->s.x1, &c->s.y1, &c->s.z1, &c->s.m1, &c->s.p1,
                   &c->s.x0, &c->s.y0, &c->s.z0, &c->s.m0, &c->s.p0, &c->s.blocks, &c->s.rights,

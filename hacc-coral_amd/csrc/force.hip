@@ -68,7 +68,43 @@ struct LawArgs {
   int cull;        // drop entries farther than the cutoff from the leaf's box
   float cull2;     // squared box distance beyond which every pair is masked
   int pmax;        // targets per pass (64 R or 32 R)
+  // ForceLawSR with FGridEvalFit or FGridEvalInterp: f = s^-3/2 - g(r2),
+  // pairs with r2 < rc2 contribute
+  float rc2;
+  float b, cc, d, e, ff, g, h, l, rmax, f0;  // FGrid constants (ForceLaw.cxx:23-33)
+  const float4 *bins;                        // {r2[i], f[i], f[i+1] - f[i], 0}
+  float t_r2min, t_r2max, t_oodr2;
+  int t_last;                                // last bin index (n_interp - 2)
 };
+
+// FGrid::fgor(r), the fitted grid force over r (ForceLaw.cxx:39-51), in
+// float with the device's coshf, tanhf and expf (the reference mixes a few
+// double products in; the difference is a rounding of fgor, far inside the
+// force tolerance).  Entries padded far away keep every term finite.
+__device__ __forceinline__ float fgor_fit(const LawArgs &L, float r) {
+  const float r2 = r * r, r4 = r2 * r2, r6 = r4 * r2;
+  const float br = L.b * r;
+  const float ch = coshf(br);
+  const float t = tanhf(br) - br / ch / ch +
+                  L.cc * r * r2 * (1.f + L.d * r2) * expf(-L.d * r2) +
+                  L.e * r2 * (L.ff * r2 + L.g * r4 + L.l * r6) * expf(-L.h * r2);
+  const float rdiv = r + (r <= 0.f ? 1.f : 0.f);
+  const float v = t / (rdiv * rdiv * rdiv) + (r <= 0.f ? L.f0 : 0.f);
+  return (r <= L.rmax && r >= 0.f) ? v : 0.f;
+}
+
+// FGridEvalInterp::eval(r2) (ForceLaw.cxx:160-171): linear interpolation in
+// the caller's table, zero outside (r2min, r2max).  The bin index is kept
+// below the last point, which the reference's index can only reach through
+// rounding just under r2max.
+__device__ __forceinline__ float fgor_interp(const LawArgs &L, float r2) {
+  const bool in = r2 > L.t_r2min && r2 < L.t_r2max;
+  int k = in ? (int)((r2 - L.t_r2min) * L.t_oodr2) : 0;
+  k = min(k, L.t_last);
+  const float4 q = L.bins[k];
+  const float v = q.y + (r2 - q.x) * L.t_oodr2 * q.z;
+  return in ? v : 0.f;
+}
 
 // Squared distance from p to the box [lo, hi] (0 inside).
 __device__ __forceinline__ float box_d2(const float *lo, const float *hi, float x, float y,
@@ -115,7 +151,7 @@ __device__ __forceinline__ int64_t pass_size(int64_t rem, int pmax) {
 #define HSR_FORCE_ATTR
 #endif
 
-template <int ORDER, bool GUARD>
+template <int LAWK, int ORDER, bool GUARD>
 __global__ void __launch_bounds__(64) HSR_FORCE_ATTR
 k_force(const hsr_node *__restrict__ cn, const int32_t *__restrict__ leaves,
         const Item *__restrict__ items, const int64_t *__restrict__ nitems_p,
@@ -222,16 +258,28 @@ k_force(const hsr_node *__restrict__ cn, const int32_t *__restrict__ leaves,
                 const float dx = p.x - xi[r];
                 const float dy = p.y - yi[r];
                 const float dz = p.z - zi[r];
-                float s = fmaf(dx, dx, L.eps2);
-                s = fmaf(dy, dy, s);
-                s = fmaf(dz, dz, s);
-                const float y = __builtin_amdgcn_rsqf(s);
-                const float y2 = y * y;
-                const float t3 = fmaf(y2, y, -c0);
-                const float q = poly_tail<ORDER>(L, s);
-                float f = fmaf(-s, q, t3);
-                float mm = (s < rcs) ? p.w : 0.f;
-                if (GUARD) mm = (s > L.eps2) ? mm : 0.f;
+                float f, mm;
+                if constexpr (LAWK == LAWK_POLY) {
+                  float s = fmaf(dx, dx, L.eps2);
+                  s = fmaf(dy, dy, s);
+                  s = fmaf(dz, dz, s);
+                  const float y = __builtin_amdgcn_rsqf(s);
+                  const float y2 = y * y;
+                  const float t3 = fmaf(y2, y, -c0);
+                  const float q = poly_tail<ORDER>(L, s);
+                  f = fmaf(-s, q, t3);
+                  mm = (s < rcs) ? p.w : 0.f;
+                  if (GUARD) mm = (s > L.eps2) ? mm : 0.f;
+                } else {
+                  const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                  const float s = r2 + L.eps2;
+                  const float y = __builtin_amdgcn_rsqf(s);
+                  const float gr = LAWK == LAWK_FIT ? fgor_fit(L, __builtin_sqrtf(r2))
+                                                    : fgor_interp(L, r2);
+                  f = y * y * y - gr;
+                  mm = (r2 < L.rc2) ? p.w : 0.f;
+                  if (GUARD) mm = (r2 > 0.f) ? mm : 0.f;
+                }
                 f *= mm;
                 ax[r] = fmaf(f, dx, ax[r]);
                 ay[r] = fmaf(f, dy, ay[r]);
@@ -422,6 +470,16 @@ LawArgs law_args(const ForceLawDev &D) {
   A.cull = 1;
   A.cull2 = (float)((double)D.rc2 * (1.0 + 1e-4));
   A.pmax = 64 * R;
+  A.rc2 = D.rc2;
+  // FGrid (ForceLaw.cxx:23-33); f0 = c + 2/3 b^3 as the reference rounds it
+  A.b = 0.72f; A.cc = 0.01f; A.d = 0.27f; A.e = 0.0001f; A.ff = 360.0f; A.g = 100.0f;
+  A.h = 0.67f; A.l = 17.0f; A.rmax = 3.116326355f;
+  A.f0 = (float)(A.cc + 2.0 / 3.0 * A.b * A.b * A.b);
+  A.bins = D.bins;
+  A.t_r2min = D.t_r2min;
+  A.t_r2max = D.t_r2max;
+  A.t_oodr2 = D.t_oodr2;
+  A.t_last = std::max(0, D.n_interp - 2);
   return A;
 }
 
@@ -593,6 +651,10 @@ static hsr_status force_enqueue(hsr_ctx *c, float *fx, float *fy, float *fz,
 // once per list build (and output buffers) and replayed as one graph.
 hsr_status force_run(hsr_ctx *c, float *fx, float *fy, float *fz) {
   hipStream_t st = c->stream;
+  if (c->law.kind == LAWK_INTERP && !c->law.bins) {
+    hsr_set_error("HSR_LAW_SR_INTERP: no table (call hsr_set_law_table first)");
+    return HSR_ERR_STATE;
+  }
   if (!c->tuning.use_graph) return force_enqueue(c, fx, fy, fz, true);
   const bool stale = c->graph_exec == nullptr || c->graph_gen != c->lists_gen ||
                      c->graph_out[0] != fx || c->graph_out[1] != fy ||
@@ -653,14 +715,18 @@ static hsr_status force_enqueue(hsr_ctx *c, float *fx, float *fy, float *fz,
   float *part = dptr<float>(LS.part);
   // events bracket the leaf-kernel launch alone (the roofline's duration)
   if (timed) HSR_HIP(hipEventRecord(c->ev[6], st));
-#define LAUNCH(ORD, GRD)                                                          \
-  k_force<ORD, GRD><<<(unsigned)grid, 64, 0, st>>>(cn, leaves, items, LS.counts, \
+#define LAUNCH(LK, ORD, GRD)                                                          \
+  k_force<LK, ORD, GRD><<<(unsigned)grid, 64, 0, st>>>(cn, leaves, items, LS.counts, \
       dptr<int32_t>(LS.jidx), dptr<PJ>(c->tree.pj), dptr<int32_t>(LS.iperm), A, qctr, fx, fy, fz, \
       part, qctr + 1)
-  if (c->law.order == 6) {
-    if (A.guard) LAUNCH(6, true); else LAUNCH(6, false);
+  if (c->law.kind == LAWK_FIT) {
+    if (A.guard) LAUNCH(LAWK_FIT, 0, true); else LAUNCH(LAWK_FIT, 0, false);
+  } else if (c->law.kind == LAWK_INTERP) {
+    if (A.guard) LAUNCH(LAWK_INTERP, 0, true); else LAUNCH(LAWK_INTERP, 0, false);
+  } else if (c->law.order == 6) {
+    if (A.guard) LAUNCH(LAWK_POLY, 6, true); else LAUNCH(LAWK_POLY, 6, false);
   } else {
-    if (A.guard) LAUNCH(5, true); else LAUNCH(5, false);
+    if (A.guard) LAUNCH(LAWK_POLY, 5, true); else LAUNCH(LAWK_POLY, 5, false);
   }
 #undef LAUNCH
   HSR_HIP(hipGetLastError());

@@ -135,7 +135,7 @@ hsr_status forces_f64_run(hsr_ctx *c, double *ax, double *ay, double *az, double
   L.rsm2 = (double)c->p.rsm * (double)c->p.rsm;
   L.rc2 = (double)c->law.rc2;
   L.guard = 1;
-  if (c->law.newton) {
+  if (c->law.newton || c->law.kind != LAWK_POLY) {
     hsr_set_error("hsr_forces_f64: the FP64 evaluation covers the polynomial laws");
     return HSR_ERR_UNSUPPORTED;
   }

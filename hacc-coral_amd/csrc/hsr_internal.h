@@ -81,6 +81,9 @@ struct __align__(16) PJ {
   float x, y, z, m;
 };
 
+// Kinds of grid-force subtraction the leaf kernel evaluates.
+enum { LAWK_POLY = 0, LAWK_FIT = 1, LAWK_INTERP = 2 };
+
 struct ForceLawDev {
   float c[7];     // polynomial P(r2) coefficients a0..a6
   int order;      // 5 or 6 (0 = no polynomial)
@@ -88,6 +91,12 @@ struct ForceLawDev {
   float rc2;      // strict cutoff: pairs with r2 < rc2 contribute
   int guard;      // 1: r2 > 0 guard (needed when eps2 == 0)
   int newton;     // 1: f = r^-3 evaluated without softening
+  int kind;       // LAWK_*: polynomial, FGrid::fgor per pair, or its table
+  int n_interp;   // LAWK_INTERP: table size
+  // LAWK_INTERP: the caller's table, device bins {r2[i], f[i], f[i+1]-f[i], 0}
+  // for i < n_interp - 1, and the range FGridEvalInterp derives from it
+  const float4 *bins;
+  float t_r2min, t_r2max, t_oodr2;
 };
 
 // Tree-build state kept in the context between calls.
@@ -180,6 +189,7 @@ struct SubState {
 struct hsr_ctx {
   hsr_params p;
   ForceLawDev law;
+  DBuf law_bins;  // LAWK_INTERP table on the device
   hsr_tuning tuning;
   hipStream_t stream = nullptr;
   hipEvent_t ev[8];

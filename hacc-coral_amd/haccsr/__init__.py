@@ -8,15 +8,15 @@ kick; ``forcetree.RCBMonopoleForceTree`` mirrors it over the C ABI, and
 mirrors the sub-cycle and PM particle-grid methods of the reference's
 Particles class, and ``restart`` reads and writes its restart files.
 """
-from ._lib import (Context, HsrError, LAW_NEWTON, LAW_SR_POLY5, LAW_SR_POLY6,
-                   LAW_STEP16, NODE_DTYPE, load)
-from .forcetree import (FGrid, FGridEvalFit, FGridEvalPoly, ForceLaw,
-                        ForceLawNewton, ForceLawSR, RCBMonopoleForceTree,
+from ._lib import (Context, HsrError, LAW_NEWTON, LAW_SR_FIT, LAW_SR_INTERP,
+                   LAW_SR_POLY5, LAW_SR_POLY6, LAW_STEP16, NODE_DTYPE, load)
+from .forcetree import (FGrid, FGridEvalFit, FGridEvalInterp, FGridEvalPoly,
+                        ForceLaw, ForceLawNewton, ForceLawSR, RCBMonopoleForceTree,
                         RCBQuadrupoleForceTree)
 
 __all__ = [
-    "Context", "HsrError", "LAW_NEWTON", "LAW_SR_POLY5", "LAW_SR_POLY6",
-    "LAW_STEP16", "NODE_DTYPE", "load", "FGrid", "FGridEvalFit",
-    "FGridEvalPoly", "ForceLaw", "ForceLawNewton", "ForceLawSR",
+    "Context", "HsrError", "LAW_NEWTON", "LAW_SR_FIT", "LAW_SR_INTERP",
+    "LAW_SR_POLY5", "LAW_SR_POLY6", "LAW_STEP16", "NODE_DTYPE", "load", "FGrid",
+    "FGridEvalFit", "FGridEvalInterp", "FGridEvalPoly", "ForceLaw", "ForceLawNewton", "ForceLawSR",
     "RCBMonopoleForceTree", "RCBQuadrupoleForceTree",
 ]

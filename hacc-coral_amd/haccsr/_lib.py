@@ -14,9 +14,11 @@ import numpy as np
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_ROOT, "libhaccsr.so")
+CXX_LIB_PATH = os.path.join(PKG_ROOT, "libhaccsr_cxx.so")  # C++ drop-ins (ForceLaw, ...)
 
 HSR_HOST_PTRS, HSR_DEVICE_PTRS = 0, 1
 LAW_STEP16, LAW_SR_POLY5, LAW_SR_POLY6, LAW_NEWTON = 0, 1, 2, 3
+LAW_SR_FIT, LAW_SR_INTERP = 4, 5
 
 # hsr_status codes (include/hacc_sr.h)
 HSR_OK, HSR_ERR_ARG, HSR_ERR_HIP, HSR_ERR_UNSUPPORTED = 0, 1, 2, 3
@@ -30,6 +32,7 @@ EXPORTS = [
     "hsr_get_tuning", "hsr_forces_f64", "hsr_leaf_momentum", "hsr_drift",
     "hsr_resort", "hsr_map2", "hsr_subcycle", "hsr_get_pseudo_masses", "hsr_cic",
     "hsr_inverse_cic", "hsr_domain_init", "hsr_refresh", "hsr_resort_noalloc",
+    "hsr_set_law_table",
 ]
 
 
@@ -47,7 +50,7 @@ class CapacityError(HsrError):
 
 class Params(C.Structure):
     _fields_ = [("device", C.c_int), ("tdpts", C.c_int), ("law", C.c_int),
-                ("reserved0", C.c_int), ("nd", C.c_int64), ("ds", C.c_int64),
+                ("n_interp", C.c_int), ("nd", C.c_int64), ("ds", C.c_int64),
                 ("tmin", C.c_int64), ("fsrrmax", C.c_float), ("rsm", C.c_float),
                 ("oa", C.c_float), ("tan_oa", C.c_float), ("ppc", C.c_float),
                 ("reserved1", C.c_float * 3)]
@@ -150,6 +153,7 @@ def load(path: str | None = None):
     L.hsr_drift.argtypes = [vp, i64] + [vp] * 6 + [C.c_float] * 4 + [C.c_int]
     L.hsr_resort.argtypes = [vp, i64, vp] + [vp] * 10 + [vp, C.c_int]
     L.hsr_resort_noalloc.argtypes = [vp, i64, vp] + [vp] * 10 + [vp, C.c_int]
+    L.hsr_set_law_table.argtypes = [vp, vp, vp, i64]
     L.hsr_map2.argtypes = [vp, C.POINTER(SubcycleParams), C.c_double, i64] + [vp] * 10 + \
         [vp, C.c_int]
     L.hsr_subcycle.argtypes = [vp, C.POINTER(SubcycleParams), i64] + [vp] * 10 + [vp, C.c_int]
@@ -202,12 +206,21 @@ class Context:
 
     def __init__(self, nd: int = 128, fsrrmax: float = 3.12, rsm: float = 0.1,
                  oa: float = 0.1, ppc: float = 0.9, law: int = LAW_STEP16,
-                 device: int = 0, ds: int = 2, tmin: int = 128, tdpts: int = 1):
+                 device: int = 0, ds: int = 2, tmin: int = 128, tdpts: int = 1,
+                 interp_table=None):
+        """interp_table: (r2, f) of an FGridEvalInterp for LAW_SR_INTERP
+        (forcetree.FGridEvalInterp(...).table())."""
         self.L = load()
         p = Params()
         p.device = device
         p.tdpts = tdpts
         p.law = law
+        if law == LAW_SR_INTERP:
+            if interp_table is None:
+                raise HsrError("LAW_SR_INTERP needs interp_table=(r2, f)")
+            r2t = np.ascontiguousarray(interp_table[0], np.float32)
+            ft = np.ascontiguousarray(interp_table[1], np.float32)
+            p.n_interp = int(r2t.shape[0])
         p.nd = int(nd)
         p.ds = int(ds)
         p.tmin = int(tmin)
@@ -221,6 +234,9 @@ class Context:
         check(self.L.hsr_create(C.byref(h), C.byref(p)), "hsr_create")
         self.h = h
         self.n = 0
+        if law == LAW_SR_INTERP:
+            check(self.L.hsr_set_law_table(self.h, r2t.ctypes.data, ft.ctypes.data,
+                                           r2t.shape[0]), "hsr_set_law_table")
 
     def close(self):
         if getattr(self, "h", None):

@@ -9,10 +9,13 @@ builds the tree (permuting all ten arrays in place), builds the interaction
 lists of the leaves inside the force box and applies the short-range kick
 v += fcoeff * F -- all inside the constructor (RCBForceTree.cxx:336-450).
 The ForceLaw hierarchy (ForceLaw.h:10-127) keeps its names and
-constructors; on the device a law is an evaluator, so only the laws the
-leaf kernel implements are accepted (no CPU fallback).
+constructors; on the device a law is an evaluator: ForceLawSR with
+FGridEvalPoly (orders 5, 6), FGridEvalFit or FGridEvalInterp.
+ForceLawNewton raises (no CPU fallback).
 """
 from __future__ import annotations
+
+import ctypes
 
 import numpy as np
 
@@ -31,11 +34,36 @@ class FGridEval:
 
 
 class FGridEvalFit(FGridEval):
-    """FGridEvalFit (ForceLaw.cxx:71-81): the exact fitted grid force.  The
-    device kernel does not evaluate it; a tree built with it raises."""
+    """FGridEvalFit (ForceLaw.cxx:71-81): the fitted grid force FGrid::fgor,
+    evaluated per pair by the device kernel (LAW_SR_FIT)."""
 
     def __init__(self, fg: FGrid):
         self.fg = fg
+
+
+class FGridEvalInterp(FGridEval):
+    """FGridEvalInterp(FGrid*, nInterp) (ForceLaw.cxx:145-171): FGrid::fgor
+    tabulated on nInterp points of r^2 (FGrid::fgor_r2_interp, computed by
+    the C++ drop-in library libhaccsr_cxx.so) and interpolated linearly by the
+    device kernel (LAW_SR_INTERP)."""
+
+    def __init__(self, fg: FGrid, nInterp: int):
+        if int(nInterp) < 2:
+            raise ValueError("FGridEvalInterp: nInterp must be >= 2")
+        self.fg = fg
+        self.n = int(nInterp)
+        self._r2 = np.zeros(self.n, np.float32)
+        self._f = np.zeros(self.n, np.float32)
+        lib = ctypes.CDLL(_lib.CXX_LIB_PATH)
+        lib.hsr_cxx_fgrid_table.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        lib.hsr_cxx_fgrid_table.restype = None
+        lib.hsr_cxx_fgrid_table(self.n, self._r2.ctypes.data, self._f.ctypes.data)
+
+    def nInterp(self) -> int:
+        return self.n
+
+    def table(self):
+        return self._r2, self._f
 
 
 class FGridEvalPoly(FGridEval):
@@ -76,6 +104,10 @@ def _law_id(fl, kernel: str) -> int:
         raise _lib.HsrError("ForceLawNewton is not implemented by the device kernel")
     if isinstance(fl, ForceLawSR) and isinstance(fl.fgore, FGridEvalPoly):
         return _lib.LAW_SR_POLY5 if fl.fgore.order == 5 else _lib.LAW_SR_POLY6
+    if isinstance(fl, ForceLawSR) and isinstance(fl.fgore, FGridEvalFit):
+        return _lib.LAW_SR_FIT
+    if isinstance(fl, ForceLawSR) and isinstance(fl.fgore, FGridEvalInterp):
+        return _lib.LAW_SR_INTERP
     raise _lib.HsrError(f"{type(fl).__name__}/{type(getattr(fl, 'fgore', None)).__name__}"
                         " is not implemented by the device kernel")
 
@@ -98,9 +130,10 @@ class RCBForceTree:
         rsm = float(r)
         if isinstance(fl, ForceLawSR) and kernel != "step16":
             rsm = fl.rsm
+        table = fl.fgore.table() if law == _lib.LAW_SR_INTERP else None
         self.ctx = _lib.Context(nd=nd, fsrrmax=fsm, rsm=rsm, oa=oa, ppc=ppc,
                                 law=law, device=device, ds=ds, tmin=tmin,
-                                tdpts=self.TDPTS)
+                                tdpts=self.TDPTS, interp_table=table)
         arrs = [xLoc, yLoc, zLoc, xVel, yVel, zVel, mass, phiLoc]
         for a in arrs:
             if a.dtype != np.float32 or not a.flags["C_CONTIGUOUS"]:

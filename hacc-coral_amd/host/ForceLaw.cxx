@@ -104,3 +104,17 @@ float ForceLawSR::f_over_r(float r2) {
   ret *= (r2 >= m_r2min) * (r2 <= m_r2max);
   return ret;
 }
+
+// FGridEvalInterp's table for callers that hold no FGrid object (the Python
+// front end): FGrid::fgor_r2_interp into caller-owned arrays of n floats.
+extern "C" void hsr_cxx_fgrid_table(int n, float *r2, float *f) {
+  FGrid fg;
+  float *a = nullptr, *b = nullptr;
+  fg.fgor_r2_interp(n, &a, &b);
+  for (int i = 0; i < n; ++i) {
+    r2[i] = a[i];
+    f[i] = b[i];
+  }
+  free(a);
+  free(b);
+}

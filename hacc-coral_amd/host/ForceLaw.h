@@ -120,4 +120,8 @@ class ForceLawSR : public ForceLaw {
   FGridEval *m_fgore;
 };
 
+// FGrid::fgor_r2_interp into caller-owned arrays (for front ends without
+// the classes, e.g. Python through ctypes).
+extern "C" void hsr_cxx_fgrid_table(int n, float *r2, float *f);
+
 #endif  // FORCELAW_H

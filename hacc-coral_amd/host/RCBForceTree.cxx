@@ -16,20 +16,32 @@ void check(hsr_status s, const char *what) {
     throw std::runtime_error(std::string(what) + ": " + hsr_last_error());
 }
 
-// Device law for a ForceLaw object (ForceLaw.h:95-127).
-int device_law(ForceLaw *fl, RCBKernel k, float *rsm) {
+// Device law for a ForceLaw object (ForceLaw.h:95-127); *interp receives
+// the FGridEvalInterp whose table the device needs.
+int device_law(ForceLaw *fl, RCBKernel k, float *rsm, FGridEvalInterp **interp) {
+  *interp = nullptr;
   if (k == RCBKernel::Step16) return HSR_LAW_STEP16;
   ForceLawSR *sr = dynamic_cast<ForceLawSR *>(fl);
   if (sr) {
-    FGridEvalPoly *poly = dynamic_cast<FGridEvalPoly *>(sr->grid_eval());
+    FGridEval *ge = sr->grid_eval();
+    FGridEvalPoly *poly = dynamic_cast<FGridEvalPoly *>(ge);
     if (poly && (poly->order() == 5 || poly->order() == 6)) {
       *rsm = sr->rsm();
       return poly->order() == 5 ? HSR_LAW_SR_POLY5 : HSR_LAW_SR_POLY6;
     }
+    if (dynamic_cast<FGridEvalFit *>(ge)) {
+      *rsm = sr->rsm();
+      return HSR_LAW_SR_FIT;
+    }
+    if ((*interp = dynamic_cast<FGridEvalInterp *>(ge))) {
+      *rsm = sr->rsm();
+      return HSR_LAW_SR_INTERP;
+    }
   }
   throw std::runtime_error(
       "RCBForceTree: this ForceLaw is not implemented by the device leaf kernel "
-      "(supported: ForceLawSR with FGridEvalPoly of order 5 or 6, or the Step16 kernel)");
+      "(supported: ForceLawSR with FGridEvalPoly of order 5 or 6, FGridEvalFit or "
+      "FGridEvalInterp, or the Step16 kernel)");
 }
 
 }  // namespace
@@ -75,7 +87,9 @@ RCBForceTree<TDPTS>::RCBForceTree(
   p.device = s_device;
   p.tdpts = TDPTS;
   float rsm = r;
-  p.law = device_law(fl, k, &rsm);
+  FGridEvalInterp *interp = nullptr;
+  p.law = device_law(fl, k, &rsm, &interp);
+  if (interp) p.n_interp = interp->nInterp();
   p.nd = nd;
   p.ds = ds;
   p.tmin = tmin;
@@ -86,6 +100,9 @@ RCBForceTree<TDPTS>::RCBForceTree(
   p.ppc = ppc;
   check(hsr_create(&ctx, &p), "hsr_create");
   try {
+    if (interp)
+      check(hsr_set_law_table(ctx, interp->r2(), interp->f(), interp->nInterp()),
+            "hsr_set_law_table");
     check(hsr_build(ctx, count, xLoc, yLoc, zLoc, xVel, yVel, zVel, mass,
                     phiLoc, idLoc, maskLoc, minLoc, maxLoc, HSR_HOST_PTRS),
           "hsr_build");

@@ -57,7 +57,14 @@ enum {
   HSR_LAW_STEP16 = 0,    /* (r^2+rsm^2)^-3/2 - P5(r^2), r^2 < fsrrmax^2, r^2 > 0 */
   HSR_LAW_SR_POLY5 = 1,  /* ForceLawSR(FGridEvalPoly), POLY_ORDER 5 */
   HSR_LAW_SR_POLY6 = 2,  /* ForceLawSR(FGridEvalPoly), POLY_ORDER 6 (stock) */
-  HSR_LAW_NEWTON = 3     /* ForceLawNewton, 1/r^3 */
+  HSR_LAW_NEWTON = 3,    /* ForceLawNewton, 1/r^3 (not on the device) */
+  HSR_LAW_SR_FIT = 4,    /* ForceLawSR(FGridEvalFit): the fitted grid force
+                            FGrid::fgor (ForceLaw.cxx:39-51) per pair; the
+                            reference's default law (Particles.cxx:173-174) */
+  HSR_LAW_SR_INTERP = 5  /* ForceLawSR(FGridEvalInterp): FGrid::fgor tabulated
+                            on n_interp points of r^2 and interpolated linearly
+                            (ForceLaw.cxx:54-67,145-171); the table comes from
+                            the caller's FGridEvalInterp (hsr_set_law_table) */
 };
 
 /* pointer-location flags */
@@ -71,7 +78,7 @@ typedef struct {
   int tdpts;             /* pseudo-particles per node: 1 (RCBMonopoleForceTree)
                             or 12 (RCBQuadrupoleForceTree; no hsr_adjoint) */
   int law;               /* HSR_LAW_* */
-  int reserved0;
+  int n_interp;          /* HSR_LAW_SR_INTERP: table size nInterp (>= 2) */
   int64_t nd;            /* leaf size ("nDirect", ctor arg nd) */
   int64_t ds;            /* depth safety (ctor arg ds): initial node reservation */
   int64_t tmin;          /* ctor arg tmin (task threshold; no effect on results) */
@@ -100,6 +107,14 @@ const char *hsr_version(void);
 
 hsr_status hsr_create(hsr_ctx **ctx, const hsr_params *p);
 hsr_status hsr_destroy(hsr_ctx *ctx);
+
+/* The table of an HSR_LAW_SR_INTERP context: FGridEvalInterp's r2() and f()
+ * arrays of nInterp entries (ForceLaw.h:73-91, filled by
+ * FGrid::fgor_r2_interp, ForceLaw.cxx:54-67), copied to the device; n must
+ * equal params.n_interp and r2 must ascend from r2[0].  Required before the
+ * first hsr_forces / hsr_kick of such a context (HSR_ERR_STATE otherwise);
+ * other laws refuse it. */
+hsr_status hsr_set_law_table(hsr_ctx *ctx, const float *r2, const float *f, int64_t n);
 
 /* The HIP stream the context launches on (hipStream_t as void*). */
 hsr_status hsr_get_stream(hsr_ctx *ctx, void **stream);

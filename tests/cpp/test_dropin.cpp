@@ -3,6 +3,9 @@
 // RCBForceTree.  Writes the permuted arrays and kicked velocities to a raw
 // file for comparison with the oracle:
 //   test_dropin <n> <box> <nd> <seed> <out.bin> [tdpts: 1 (default) or 12]
+//               [law: step16 (default; the BG/Q kernel) | poly5 | poly6 | fit |
+//                interp<N> (ForceLawSR over FGridEvalInterp with N points),
+//                the last four through the generic ForceLaw kernel]
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -38,8 +41,20 @@ int main(int argc, char **argv) {
   float zero[3] = {0.f, 0.f, 0.f}, hi[3] = {box, box, box};
   float lo_f[3] = {1.f, 1.f, 1.f}, hi_f[3] = {box - 1.f, box - 1.f, box - 1.f};
   FGrid fg;
-  FGridEvalPoly poly(&fg);
-  ForceLawSR fl(&poly, 0.1f);
+  const char *law = argc > 7 ? argv[7] : "step16";
+  FGridEval *ge = nullptr;
+  if (!strcmp(law, "step16") || !strcmp(law, "poly6")) ge = new FGridEvalPoly(&fg, 6);
+  else if (!strcmp(law, "poly5")) ge = new FGridEvalPoly(&fg, 5);
+  else if (!strcmp(law, "fit")) ge = new FGridEvalFit(&fg);
+  else if (!strncmp(law, "interp", 6)) ge = new FGridEvalInterp(&fg, atoi(law + 6));
+  else {
+    fprintf(stderr, "unknown law %s\n", law);
+    return 2;
+  }
+  ForceLawSR fl(ge, 0.1f);
+  const RCBKernel kern = strcmp(law, "step16") ? RCBKernel::ForceLaw : RCBKernel::Step16;
+  RCBMonopoleForceTree::setKernel(kern);
+  RCBQuadrupoleForceTree::setKernel(kern);
   const float fsrrmax = fg.rmax();
   const float c = 0.5f;  // fcoeff
   const int tdpts = argc > 6 ? atoi(argv[6]) : 1;
@@ -63,6 +78,7 @@ int main(int argc, char **argv) {
   fwrite(vx.data(), 4, n, f); fwrite(vy.data(), 4, n, f); fwrite(vz.data(), 4, n, f);
   fwrite(id.data(), 8, n, f);
   fclose(f);
-  printf("dropin ok n=%ld\n", n);
+  delete ge;
+  printf("dropin ok n=%ld law=%s\n", n, law);
   return 0;
 }

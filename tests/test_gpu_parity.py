@@ -156,13 +156,19 @@ def test_chunked_lists_and_determinism():
     say(f"  rel rms unchunked {e1:.3e} chunked {e2:.3e}")
 
 
-@pytest.mark.parametrize("law", [O.LAW_SR_POLY5, O.LAW_SR_POLY6])
-def test_sr_poly_laws_match_oracle(law):
-    """ForceLawSR with the 5th- and 6th-order grid-force polynomials
-    (ForceLaw.cxx:89-116, 187-192): the device kernel for each order against
-    the oracle's nbody1 with the same law, per unit target mass, on unequal
-    masses; the tree and lists bit-exact as with the default law."""
-    say(f"case SR polynomial law {law}")
+@pytest.mark.parametrize("law,n_interp", [(O.LAW_SR_POLY5, 0), (O.LAW_SR_POLY6, 0),
+                                          (O.LAW_SR_FIT, 0), (O.LAW_SR_INTERP, 1024),
+                                          (O.LAW_SR_INTERP, 97), (O.LAW_SR_INTERP, 2)])
+def test_sr_laws_match_oracle(law, n_interp):
+    """ForceLawSR over each grid-force evaluator (ForceLaw.cxx:39-192): the
+    5th- and 6th-order polynomials, FGridEvalFit (FGrid::fgor per pair; the
+    reference's default law) and FGridEvalInterp with 1024, 97 and 2 table
+    points (the table from the C++ drop-in's FGrid::fgor_r2_interp): the
+    device kernel against the oracle's nbody1 with the same law, per unit
+    target mass, on unequal masses; the tree and lists bit-exact as with the
+    default law."""
+    from haccsr import FGrid, FGridEvalInterp
+    say(f"case SR law {law} n_interp={n_interp}")
     rng = np.random.default_rng(40 + law)
     n, box, nd = 20000, 14.0, 64
     p = (rng.random((n, 3)) * box).astype(np.float32)
@@ -171,10 +177,11 @@ def test_sr_poly_laws_match_oracle(law):
     lo, hi = np.zeros(3, np.float32), np.full(3, box, np.float32)
     flo, fhi = np.full(3, 1.0, np.float32), np.full(3, box - 1.0, np.float32)
     rmax = 3.116326355  # FGrid::rmax (ForceLaw.cxx:32), the drop-in's fsrrmax
-    tp = O.TreeParams(lo, hi, flo, fhi, nd=nd, fsrrmax=rmax)
+    tp = O.TreeParams(lo, hi, flo, fhi, nd=nd, fsrrmax=rmax, n_interp=n_interp)
     tree = O.build_tree(tp, x, y, z, m)
     lists = O.build_lists(tp, tree)
-    ctx = Context(nd=nd, law=law, fsrrmax=rmax)
+    table = FGridEvalInterp(FGrid(), n_interp).table() if n_interp else None
+    ctx = Context(nd=nd, law=law, fsrrmax=rmax, interp_table=table)
     ids = np.arange(n, dtype=np.int64)
     X, Y, Z, M = x.copy(), y.copy(), z.copy(), m.copy()
     ctx.build(X, Y, Z, M, ids=ids, min_loc=lo, max_loc=hi)
@@ -192,10 +199,24 @@ def test_sr_poly_laws_match_oracle(law):
 
 
 def test_newton_law_refused_on_device():
-    """The device implements the Step16 and SR polynomial laws only; any
-    other law is refused at context creation (no CPU fallback)."""
+    """ForceLawNewton is not implemented on the device: refused at context
+    creation (no CPU fallback).  An interpolation context without its table
+    refuses to compute forces, and the adjoint and FP64 paths refuse the
+    laws they do not implement."""
     with pytest.raises(_lib.HsrError):
         Context(nd=64, law=_lib.LAW_NEWTON)
+    with pytest.raises(_lib.HsrError, match="interp_table"):
+        Context(nd=64, law=_lib.LAW_SR_INTERP)
+    x, y, z, m = uniform(3000, 8.0, 9)
+    ctx = Context(nd=32, law=_lib.LAW_SR_FIT)
+    ctx.build(x, y, z, m, min_loc=(0, 0, 0), max_loc=(8, 8, 8))
+    ctx.lists((1, 1, 1), (7, 7, 7))
+    ctx.forces()
+    with pytest.raises(_lib.HsrError, match="polynomial laws"):
+        ctx.adjoint(*[np.zeros(3000, np.float32)] * 3)
+    with pytest.raises(_lib.HsrError, match="polynomial laws"):
+        ctx.forces_f64()
+    ctx.close()
 
 
 def test_tuning_validation_and_chunk_replan():

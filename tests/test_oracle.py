@@ -542,6 +542,24 @@ def test_dropin_forcelaw_host_matches_reference_bitwise():
         assert np.array_equal(got[:, k], f32bits(d[name])), name
 
 
+@pytest.mark.parametrize("n", [2, 97, 1024, 4096])
+def test_dropin_fgrid_table_matches_reference_bitwise(n):
+    """The table the drop-in library gives the device for FGridEvalInterp
+    (hsr_cxx_fgrid_table over FGrid::fgor_r2_interp, ForceLaw.cxx:54-67) is
+    the reference's table bit for bit."""
+    import ctypes
+    from haccsr import _lib
+    lib = ctypes.CDLL(_lib.CXX_LIB_PATH)
+    lib.hsr_cxx_fgrid_table.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    lib.hsr_cxx_fgrid_table.restype = None
+    r2 = np.zeros(n, np.float32)
+    f = np.zeros(n, np.float32)
+    lib.hsr_cxx_fgrid_table(n, r2.ctypes.data, f.ctypes.data)
+    d = np.load(os.path.join(GOLDEN, "ref_forcelaw.npz"))
+    assert np.array_equal(f32bits(r2), f32bits(d[f"table{n}_r2"]))
+    assert np.array_equal(f32bits(f), f32bits(d[f"table{n}_f"]))
+
+
 # ------------------------------------------------------------------ adjoint
 def test_adjoint_f64_matches_finite_differences():
     """The FP64 adjoint (including the pseudo-particle centre-of-mass term)
