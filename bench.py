@@ -61,8 +61,13 @@ def parse():
                     help="sub-steps of the device-resident sub-cycle leg (0: skip it)")
     ap.add_argument("--pm-reps", type=int, default=3,
                     help="timed CIC deposit/interpolation pairs of the PM leg (0: skip it)")
-    ap.add_argument("--force-pass", type=int, default=0, choices=(0, 64, 128),
+    ap.add_argument("--force-pass", type=int, default=0, choices=(0, 32, 64, 128, 256),
                     help="hsr_tuning.force_pass: targets per leaf-kernel pass (0: default)")
+    ap.add_argument("--i-per-lane", type=int, default=0,
+                    help="leaf-kernel shape: i-particles per lane (0: autotuned with the "
+                         "unroll and tile)")
+    ap.add_argument("--unroll", type=int, default=0, help="leaf-kernel shape: j entries per step")
+    ap.add_argument("--tile", type=int, default=0, help="leaf-kernel shape: entries per LDS tile")
     ap.add_argument("--refresh-reps", type=int, default=3,
                     help="timed one-rank overload refreshes of the refresh leg (0: skip it)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -387,13 +392,17 @@ def main():
     torch.cuda.synchronize()
     fcoeff = 1.0
     ctxs = {}
+    shape = {"i_per_lane": a.i_per_lane, "unroll": a.unroll, "tile": a.tile}
 
     def step(nd):
         ctx = ctxs.get(nd)
         if ctx is None:
             ctx = ctxs[nd] = Context(nd=nd, tdpts=a.tdpts)
-            ctx.set_tuning(use_graph=1, force_pass=a.force_pass)
             ctx.torch_stream = torch.cuda.ExternalStream(ctx.stream())
+        want = dict(use_graph=1, force_pass=a.force_pass, **shape)
+        if getattr(ctx, "applied", None) != want:  # a change drops the captured graph
+            ctx.set_tuning(**want)
+            ctx.applied = want
         with torch.cuda.stream(ctx.torch_stream):
             work_all.copy_(src_all)
         if ens is None:
@@ -411,8 +420,23 @@ def main():
         return ctx.stats()
 
     nd = a.nd
+    tuned = {}
+    if a.tdpts == 1 and not (a.i_per_lane or a.unroll or a.tile):
+        # the kernel shape on the lists of one leaf size, then the leaf size
+        # with that shape (the two interact weakly: the shape sets the
+        # leaf kernel's speed per pair, the leaf size the pairs and the rest)
+        nd0 = nd if nd > 0 else 512
+        step(nd0)
+        best, times = autotune.tune_kernel_shape(ctxs[nd0], lambda: ctxs[nd0].kick(fcoeff), log=log)
+        shape.update(best)
+        ctxs[nd0].applied = None
+        tuned["kernel_shape"] = dict(best, tuned_at_nd=nd0,
+                                     ms={f"{r},{u},{t}": round(v, 3) for (r, u, t), v in times.items()})
+        log(f"autotuned kernel shape {best}")
     if nd <= 0:
-        nd, _ = autotune.tune_leaf_size(step, log=log)
+        nd, times = autotune.tune_leaf_size(step, log=log)
+        tuned["leaf_size"] = {"nd": nd, "ms_per_step": {str(k): round(1e3 * v, 2)
+                                                        for k, v in sorted(times.items())}}
         log(f"autotuned leaf size nd={nd}")
     for k in list(ctxs):
         if k != nd:
@@ -456,6 +480,9 @@ def main():
         "config": {"workload": (workload_name(a.config, load, nd) if ens is None else
                                 ensemble_name(ens, n, nd)), "particles": n,
                    "pairs_per_step": pairs, "list_entries": int(st.entries), "leaf_size": nd,
+                   "kernel_shape": {"i_per_lane": shape["i_per_lane"] or 2,
+                                    "unroll": shape["unroll"] or 2, "tile": shape["tile"] or 256},
+                   "autotune": tuned,
                    "breakdown_ms": {"build": float(np.mean([s[1] for s in stats])),
                                     "lists": float(np.mean([s[2] for s in stats])),
                                     "kick": float(np.mean([s[3] for s in stats])),

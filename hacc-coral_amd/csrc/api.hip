@@ -573,9 +573,12 @@ hsr_status hsr_lists_many(hsr_ctx *c, const float *min_force,
 
 // Pairs the last force phase evaluated (its kernel's counter, qctr[1]).
 static hipError_t read_evaluated(hsr_ctx *c) {
-  c->evaluated_pairs = 0;
+  c->evaluated_pairs = c->unmasked_pairs = 0;
   if (c->lists.nleaves == 0 || c->lists.nitems == 0 || !c->s.qctr.p) return hipSuccess;
-  return hipMemcpyAsync(&c->evaluated_pairs, dptr<unsigned long long>(c->s.qctr) + 1,
+  hipError_t e = hipMemcpyAsync(&c->evaluated_pairs, dptr<unsigned long long>(c->s.qctr) + 1,
+                                sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
+  if (e != hipSuccess) return e;
+  return hipMemcpyAsync(&c->unmasked_pairs, dptr<unsigned long long>(c->s.qctr) + 2,
                         sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
 }
 
@@ -800,6 +803,7 @@ hsr_status hsr_get_stats(hsr_ctx *c, hsr_stats *st) {
   st->entries = c->lists.nentries;
   st->pairs = c->lists.pairs;
   st->evaluated_pairs = c->evaluated_pairs;
+  st->unmasked_pairs = c->unmasked_pairs;
   st->build_ms = c->build_ms;
   st->lists_ms = c->lists_ms;
   st->force_ms = c->force_ms;
@@ -917,15 +921,19 @@ hsr_status hsr_set_tuning(hsr_ctx *c, const hsr_tuning *t) {
     g_err = "hsr_set_tuning: null argument";
     return HSR_ERR_ARG;
   }
-  const int r = force_targets_per_lane(), tile = force_tile_entries();
+  int r, u, tile;
+  force_shape(*t, &r, &u, &tile);
+  if (t->i_per_lane < 0 || t->unroll < 0 || t->tile < 0 ||
+      !force_shape_built(c->law.kind, c->law.guard != 0, r, u, tile)) {
+    g_err = "hsr_set_tuning: (i_per_lane, unroll, tile) = (" + std::to_string(r) + ", " +
+            std::to_string(u) + ", " + std::to_string(tile) + ") is not built for this law; "
+            "built: " + force_shapes_built() + " (the polynomial laws with rsm > 0), the "
+            "first for every law";
+    return HSR_ERR_ARG;
+  }
   if (t->force_pass != 0 && t->force_pass != 32 * r && t->force_pass != 64 * r) {
     g_err = "hsr_set_tuning: force_pass must be 0, " + std::to_string(32 * r) + " or " +
             std::to_string(64 * r);
-    return HSR_ERR_ARG;
-  }
-  if ((t->i_per_lane != 0 && t->i_per_lane != r) || (t->tile != 0 && t->tile != tile)) {
-    g_err = "hsr_set_tuning: i_per_lane and tile are fixed when the library is built (" +
-            std::to_string(r) + ", " + std::to_string(tile) + "); pass 0 or those values";
     return HSR_ERR_ARG;
   }
   if ((t->chunk != 0 && (t->chunk < tile || t->chunk > (1 << 30))) || t->waves_per_cu < 0 ||

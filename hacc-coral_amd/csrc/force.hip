@@ -27,31 +27,30 @@
 
 namespace {
 
-// i-particles per lane and j entries per unrolled step (build-time choices)
-#ifndef HSR_FORCE_R
-#define HSR_FORCE_R 2
-#endif
-#ifndef HSR_FORCE_UNROLL
-#define HSR_FORCE_UNROLL 2
-#endif
-constexpr int R = HSR_FORCE_R;
-static_assert(R == 1 || R == 2 || R == 4, "i-particles per lane");
-constexpr int TILE = 256;     // j entries staged per tile
+// Kernel shapes: RR i-particles per lane, UU j entries per unrolled step,
+// TT j entries staged per LDS tile.  The default (2, 2, 256) is the fastest
+// measured at 256^3; the others are built for the tuner (hsr_tuning
+// i_per_lane, unroll, tile).
+constexpr int DEF_R = 2, DEF_U = 2, DEF_T = 256;
 constexpr int GMAX = 16;      // at most 16 lane groups (>= 4 lanes each)
-constexpr int LDS_ENT = 336;  // staged entries incl. per-group padding
-constexpr int UNROLL = HSR_FORCE_UNROLL;
-static_assert(UNROLL >= 1 && UNROLL <= 8, "j entries per unrolled step");
 // Passes hold a power of two of at least 4 R targets, or fewer than 4 R, so
 // the group count G = 64 / max(ceil(targets / R), 4) is 1, 2, 4, 8 or 16;
-// the staged slices of a full tile must fit for each of them.
+// the staged slices of a full tile must fit for each of them.  A group's
+// slice holds its unmasked entries (a multiple of UU) and its masked ones
+// rounded up to UU: at most ceil(entries / G) + UU - 1 slots.
+template <int UU>
+__host__ __device__ constexpr int slice_slots(int nt, int G) { return (nt + G - 1) / G + UU; }
+template <int UU, int TT>
 constexpr int staged_entries(int G) {
-  const int tg = ((TILE + G - 1) / G + UNROLL - 1) / UNROLL * UNROLL;
+  const int tg = slice_slots<UU>(TT, G);
   return G * (tg + ((9 - (tg & 7)) & 7));
 }
-static_assert(staged_entries(1) <= LDS_ENT && staged_entries(2) <= LDS_ENT &&
-                  staged_entries(4) <= LDS_ENT && staged_entries(8) <= LDS_ENT &&
-                  staged_entries(16) <= LDS_ENT,
-              "staged tile fits");
+template <int UU, int TT>
+constexpr int lds_entries() {
+  int m = 0;
+  for (int G = 1; G <= GMAX; G *= 2) m = staged_entries<UU, TT>(G) > m ? staged_entries<UU, TT>(G) : m;
+  return m;
+}
 
 struct Item {
   int32_t leaf;     // listed-leaf index
@@ -68,6 +67,9 @@ struct LawArgs {
   int cull;        // drop entries farther than the cutoff from the leaf's box
   float cull2;     // squared box distance beyond which every pair is masked
   int pmax;        // targets per pass (64 R or 32 R)
+  float in2;       // squared farthest box distance below which every pair of
+                   // the pass is inside the cutoff (the unmasked class)
+  int unmasked;    // 1: unit-mass entries inside the cutoff skip mask and mass
   // ForceLawSR with FGridEvalFit or FGridEvalInterp: f = s^-3/2 - g(r2),
   // pairs with r2 < rc2 contribute
   float rc2;
@@ -115,6 +117,15 @@ __device__ __forceinline__ float box_d2(const float *lo, const float *hi, float 
   return dx * dx + dy * dy + dz * dz;
 }
 
+// Squared distance from p to the farthest point of the box [lo, hi].
+__device__ __forceinline__ float box_far2(const float *lo, const float *hi, float x, float y,
+                                          float z) {
+  const float dx = fmaxf(fabsf(x - lo[0]), fabsf(x - hi[0]));
+  const float dy = fmaxf(fabsf(y - lo[1]), fabsf(y - hi[1]));
+  const float dz = fmaxf(fabsf(z - lo[2]), fabsf(z - hi[2]));
+  return dx * dx + dy * dy + dz * dz;
+}
+
 template <int ORDER>
 __device__ __forceinline__ float poly_tail(const LawArgs &L, float s) {
   // q(s) = c1 + c2 s + ... ; f = y^3 - c0 - s q(s)
@@ -131,8 +142,9 @@ __device__ __forceinline__ float poly_tail(const LawArgs &L, float s) {
   return q;
 }
 
-// Targets of the next pass over a leaf: 128 while that many remain, then the
-// largest of 64, 32, 16, 8 that fits, then the rest (< 8).
+// Targets of the next pass over a leaf: 64 R while that many remain, then
+// the largest power of two of at least 4 R that fits, then the rest.
+template <int R>
 __device__ __forceinline__ int64_t pass_size(int64_t rem, int pmax) {
   if (rem >= pmax) return pmax;
   for (int64_t c = pmax >> 1; c >= 4 * R; c >>= 1)
@@ -141,7 +153,9 @@ __device__ __forceinline__ int64_t pass_size(int64_t rem, int pmax) {
 }
 
 // Waves per SIMD the register allocation of the leaf kernel targets (0: the
-// compiler's choice).
+// compiler's choice: 94 VGPRs, 5 waves, with the four pairs of an unrolled
+// step interleaved; a cap of 6 waves fits 80 VGPRs but serialises them and
+// measured 2.7% slower at 256^3).
 #ifndef HSR_FORCE_WAVES
 #define HSR_FORCE_WAVES 0
 #endif
@@ -151,7 +165,49 @@ __device__ __forceinline__ int64_t pass_size(int64_t rem, int pmax) {
 #define HSR_FORCE_ATTR
 #endif
 
-template <int LAWK, int ORDER, bool GUARD>
+// One pair: F_i += m_j f(r2) (x_j - x_i).  MASKED applies the cutoff (and
+// the r2 > 0 guard) and the source mass; unmasked entries are known to have
+// unit mass and every pair inside the cutoff, where both are exact no-ops.
+template <int LAWK, int ORDER, bool GUARD, bool MASKED>
+__device__ __forceinline__ void pair(const LawArgs &L, float c0, float rcs, const float4 p,
+                                     float xi, float yi, float zi, float &ax, float &ay,
+                                     float &az) {
+  const float dx = p.x - xi;
+  const float dy = p.y - yi;
+  const float dz = p.z - zi;
+  float f;
+  if constexpr (LAWK == LAWK_POLY) {
+    float s = fmaf(dx, dx, L.eps2);
+    s = fmaf(dy, dy, s);
+    s = fmaf(dz, dz, s);
+    const float y = __builtin_amdgcn_rsqf(s);
+    const float y2 = y * y;
+    const float t3 = fmaf(y2, y, -c0);
+    const float q = poly_tail<ORDER>(L, s);
+    f = fmaf(-s, q, t3);
+    if constexpr (MASKED) {
+      float mm = (s < rcs) ? p.w : 0.f;
+      if (GUARD) mm = (s > L.eps2) ? mm : 0.f;
+      f *= mm;
+    }
+  } else {
+    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    const float s = r2 + L.eps2;
+    const float y = __builtin_amdgcn_rsqf(s);
+    const float gr = LAWK == LAWK_FIT ? fgor_fit(L, __builtin_sqrtf(r2)) : fgor_interp(L, r2);
+    f = y * y * y - gr;
+    if constexpr (MASKED) {
+      float mm = (r2 < L.rc2) ? p.w : 0.f;
+      if (GUARD) mm = (r2 > 0.f) ? mm : 0.f;
+      f *= mm;
+    }
+  }
+  ax = fmaf(f, dx, ax);
+  ay = fmaf(f, dy, ay);
+  az = fmaf(f, dz, az);
+}
+
+template <int LAWK, int ORDER, bool GUARD, int R, int UNROLL, int TILE>
 __global__ void __launch_bounds__(64) HSR_FORCE_ATTR
 k_force(const hsr_node *__restrict__ cn, const int32_t *__restrict__ leaves,
         const Item *__restrict__ items, const int64_t *__restrict__ nitems_p,
@@ -160,6 +216,11 @@ k_force(const hsr_node *__restrict__ cn, const int32_t *__restrict__ leaves,
         unsigned long long *__restrict__ qctr, float *__restrict__ fx,
         float *__restrict__ fy, float *__restrict__ fz, float *__restrict__ part,
         unsigned long long *__restrict__ evals) {
+  static_assert(R == 1 || R == 2 || R == 4, "i-particles per lane");
+  static_assert(UNROLL >= 1 && UNROLL <= 8, "j entries per unrolled step");
+  static_assert(TILE % 64 == 0 && TILE <= 512, "tile of 64-entry chunks");
+  constexpr int NCH = TILE / 64;  // 64-entry chunks per tile
+  constexpr int LDS_ENT = lds_entries<UNROLL, TILE>();
   // the tile doubles as the reduction buffer once the j-walk is done
   __shared__ float4 tile[LDS_ENT];
   static_assert(3 * 64 * R <= 4 * LDS_ENT, "reduction buffer fits the tile");
@@ -178,11 +239,12 @@ k_force(const hsr_node *__restrict__ cn, const int32_t *__restrict__ leaves,
     // padding entries: far enough to be masked, near enough to keep every
     // intermediate finite
     const float4 pad = make_float4(nd.xc[0] + 1000.f, nd.xc[1], nd.xc[2], 0.f);
-    int64_t ev = 0;  // pairs evaluated for this item (culled entries excluded)
+    int64_t ev = 0;   // pairs evaluated for this item (culled entries excluded)
+    int64_t evu = 0;  // of which unmasked
     // passes over the targets in power-of-two chunks (128, 64, ..., 8, rest)
     // so that every pass fills its groups of lpg = chunk / R lanes
     for (int64_t ib = 0, nib = 0; ib < cnt; ib += nib) {
-      nib = pass_size(cnt - ib, L.pmax);
+      nib = pass_size<R>(cnt - ib, L.pmax);
       const int ni = (int)nib;
       const int lpg = max((ni + R - 1) / R, 64 / GMAX);
       const int G = max(1, 64 / lpg);
@@ -214,77 +276,100 @@ k_force(const hsr_node *__restrict__ cn, const int32_t *__restrict__ leaves,
           bmax[d] = fmaxf(bmax[d], __shfl_xor(bmax[d], o));
         }
       }
+      // the box is uniform across the wave: keep it in scalar registers
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        bmin[d] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, bmin[d])));
+        bmax[d] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, bmax[d])));
+      }
+      const int lg = __builtin_ctz(G);  // G is a power of two
       for (int64_t t0 = I.e0; t0 < I.e1; t0 += TILE) {
         const int nt = (int)min<int64_t>(TILE, I.e1 - t0);
-        // per-group slices of at most tgmax entries, stride = 1 mod 8
-        int tgmax = (nt + G - 1) / G;
-        tgmax = (tgmax + UNROLL - 1) / UNROLL * UNROLL;
+        int tgmax = slice_slots<UNROLL>(nt, G);
         const int stride = tgmax + ((9 - (tgmax & 7)) & 7);
-        __syncthreads();
-        // Entries farther than the cutoff from the pass's box contribute 0 to
-        // every target of the pass and are dropped; the kept ones, in list
-        // order, go round-robin to the groups (entry c -> group c % G, slot
-        // c / G).
-        int nk = 0;
-        for (int q0 = 0; q0 < nt; q0 += 64) {
-          const int q = q0 + l;
-          float4 v = pad;
-          bool keep = false;
-          if (q < nt) {
-            const PJ p = pj[jidx[t0 + q]];
-            v = make_float4(p.x, p.y, p.z, p.m);
-            keep = !L.cull || box_d2(bmin, bmax, p.x, p.y, p.z) <= L.cull2;
+        // Classify the tile's entries.  Culled: farther than the cutoff from
+        // the pass's target box, every pair masked.  Unmasked: unit mass and
+        // the farthest point of the box inside the cutoff, so every pair
+        // passes the cutoff test and m_j = 1.  Masked: the rest.
+        float4 v[NCH];
+        uint64_t bk[NCH], ba[NCH];
+#pragma unroll
+        for (int k = 0; k < NCH; ++k) {
+          v[k] = pad;
+          bk[k] = ba[k] = 0;
+          if (k * 64 < nt) {
+            const int q = k * 64 + l;
+            bool keep = false, um = false;
+            if (q < nt) {
+              const PJ p = pj[jidx[t0 + q]];
+              v[k] = make_float4(p.x, p.y, p.z, p.m);
+              keep = !L.cull || box_d2(bmin, bmax, p.x, p.y, p.z) <= L.cull2;
+              um = keep && L.unmasked && p.m == 1.f &&
+                   box_far2(bmin, bmax, p.x, p.y, p.z) < L.in2;
+            }
+            bk[k] = __ballot(keep);
+            ba[k] = __ballot(um);
           }
-          const uint64_t bal = __ballot(keep);
-          if (keep) {
-            const int c = nk + __popcll(bal & lanemask_lt());
-            tile[(c % G) * stride + c / G] = v;
-          }
-          nk += __popcll(bal);
         }
-        int tg = (nk + G - 1) / G;
-        tg = (tg + UNROLL - 1) / UNROLL * UNROLL;
-        for (int c = nk + l; c < G * tg; c += 64) tile[(c % G) * stride + c / G] = pad;
-        ev += (int64_t)nk * ni;
+        int nA = 0, nK = 0;
+#pragma unroll
+        for (int k = 0; k < NCH; ++k) {
+          nA += __popcll(ba[k]);
+          nK += __popcll(bk[k]);
+        }
+        // Unmasked entries fill the first tA slots of every group (a multiple
+        // of UNROLL, entry a -> group a % G, slot a / G); the unmasked ones
+        // beyond G tA and the masked ones follow as masked entries m -> group
+        // m % G, slot tA + m / G, padded to tM slots.
+        const int tA = (nA >> lg) / UNROLL * UNROLL;
+        const int splitA = tA << lg;
+        const int nM = nK - splitA;
+        const int tM = (((nM + G - 1) >> lg) + UNROLL - 1) / UNROLL * UNROLL;
+        __syncthreads();
+        const uint64_t lt = lanemask_lt();
+        int pa = 0, pb = nA - splitA;
+#pragma unroll
+        for (int k = 0; k < NCH; ++k) {
+          const uint64_t bm = bk[k] & ~ba[k];
+          if ((bk[k] >> l) & 1ull) {
+            int slot, grp;
+            if ((ba[k] >> l) & 1ull) {
+              const int a = pa + __popcll(ba[k] & lt);
+              const int m = a - splitA;
+              grp = (a < splitA ? a : m) & (G - 1);
+              slot = a < splitA ? (a >> lg) : tA + (m >> lg);
+            } else {
+              const int m = pb + __popcll(bm & lt);
+              grp = m & (G - 1);
+              slot = tA + (m >> lg);
+            }
+            tile[grp * stride + slot] = v[k];
+          }
+          pa += __popcll(ba[k]);
+          pb += __popcll(bm);
+        }
+        for (int m = nM + l; m < (tM << lg); m += 64) tile[(m & (G - 1)) * stride + tA + (m >> lg)] = pad;
+        ev += (int64_t)nK * ni;
+        evu += (int64_t)splitA * ni;
         __syncthreads();
         if (active) {
           const float4 *tp = tile + g * stride;
-          for (int k = 0; k < tg; k += UNROLL) {
+          for (int k = 0; k < tA; k += UNROLL) {
 #pragma unroll
             for (int u = 0; u < UNROLL; ++u) {
               const float4 p = tp[k + u];
 #pragma unroll
-              for (int r = 0; r < R; ++r) {
-                const float dx = p.x - xi[r];
-                const float dy = p.y - yi[r];
-                const float dz = p.z - zi[r];
-                float f, mm;
-                if constexpr (LAWK == LAWK_POLY) {
-                  float s = fmaf(dx, dx, L.eps2);
-                  s = fmaf(dy, dy, s);
-                  s = fmaf(dz, dz, s);
-                  const float y = __builtin_amdgcn_rsqf(s);
-                  const float y2 = y * y;
-                  const float t3 = fmaf(y2, y, -c0);
-                  const float q = poly_tail<ORDER>(L, s);
-                  f = fmaf(-s, q, t3);
-                  mm = (s < rcs) ? p.w : 0.f;
-                  if (GUARD) mm = (s > L.eps2) ? mm : 0.f;
-                } else {
-                  const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                  const float s = r2 + L.eps2;
-                  const float y = __builtin_amdgcn_rsqf(s);
-                  const float gr = LAWK == LAWK_FIT ? fgor_fit(L, __builtin_sqrtf(r2))
-                                                    : fgor_interp(L, r2);
-                  f = y * y * y - gr;
-                  mm = (r2 < L.rc2) ? p.w : 0.f;
-                  if (GUARD) mm = (r2 > 0.f) ? mm : 0.f;
-                }
-                f *= mm;
-                ax[r] = fmaf(f, dx, ax[r]);
-                ay[r] = fmaf(f, dy, ay[r]);
-                az[r] = fmaf(f, dz, az[r]);
-              }
+              for (int r = 0; r < R; ++r)
+                pair<LAWK, ORDER, GUARD, false>(L, c0, rcs, p, xi[r], yi[r], zi[r], ax[r], ay[r], az[r]);
+            }
+          }
+          for (int k = tA; k < tA + tM; k += UNROLL) {
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+              const float4 p = tp[k + u];
+#pragma unroll
+              for (int r = 0; r < R; ++r)
+                pair<LAWK, ORDER, GUARD, true>(L, c0, rcs, p, xi[r], yi[r], zi[r], ax[r], ay[r], az[r]);
             }
           }
         }
@@ -324,6 +409,7 @@ k_force(const hsr_node *__restrict__ cn, const int32_t *__restrict__ leaves,
       }
     }
     if (l == 0 && ev > 0) atomicAdd(evals, (unsigned long long)ev);
+    if (l == 0 && evu > 0) atomicAdd(evals + 1, (unsigned long long)evu);
   }
 }
 
@@ -392,7 +478,7 @@ __global__ void k_item_fill(int64_t nleaves, const int64_t *ent_ptr,
     I.pbase = nc > 1 ? pbase[k] + (int64_t)c * 3 * cnt : -1;
     items[j] = I;
     // work ~ i-passes * entries
-    uint64_t work = (uint64_t)((cnt + 64 * R - 1) / (64 * R)) * (uint64_t)(I.e1 - I.e0);
+    uint64_t work = (uint64_t)((cnt + 64 * DEF_R - 1) / (64 * DEF_R)) * (uint64_t)(I.e1 - I.e0);
     keys[j] = work;
     vals[j] = (int32_t)j;
   }
@@ -469,7 +555,11 @@ LawArgs law_args(const ForceLawDev &D) {
   // the kernel's own s < rcs test would accept
   A.cull = 1;
   A.cull2 = (float)((double)D.rc2 * (1.0 + 1e-4));
-  A.pmax = 64 * R;
+  A.pmax = 64 * DEF_R;
+  // the same relative margin the other way: every pair of an entry whose
+  // farthest box point is inside in2 passes the kernel's cutoff test
+  A.in2 = (float)((double)D.rc2 * (1.0 - 1e-4));
+  A.unmasked = D.guard ? 0 : 1;
   A.rc2 = D.rc2;
   // FGrid (ForceLaw.cxx:23-33); f0 = c + 2/3 b^3 as the reference rounds it
   A.b = 0.72f; A.cc = 0.01f; A.d = 0.27f; A.e = 0.0001f; A.ff = 360.0f; A.g = 100.0f;
@@ -547,8 +637,35 @@ k_leaf_sort(int64_t nl, const int32_t *__restrict__ leaves, const hsr_node *__re
   for (int i = threadIdx.x; i < cnt; i += blockDim.x) iperm[off + i] = (int32_t)(off + idx[i]);
 }
 
-int force_targets_per_lane() { return R; }
-int force_tile_entries() { return TILE; }
+// Shapes built for every polynomial law without the r2 > 0 guard (the
+// production laws); the other laws run the default shape only.
+struct Shape {
+  int r, u, t;
+};
+constexpr Shape kShapes[] = {{2, 2, 256}, {2, 1, 256}, {2, 4, 256}, {1, 2, 256},
+                             {1, 4, 256}, {4, 1, 256}, {2, 2, 128}, {2, 2, 512}};
+
+void force_shape(const hsr_tuning &t, int *r, int *u, int *tile) {
+  *r = t.i_per_lane ? t.i_per_lane : DEF_R;
+  *u = t.unroll ? t.unroll : DEF_U;
+  *tile = t.tile ? t.tile : DEF_T;
+}
+
+bool force_shape_built(int law_kind, bool guard, int r, int u, int tile) {
+  if (r == DEF_R && u == DEF_U && tile == DEF_T) return true;
+  if (law_kind != LAWK_POLY || guard) return false;
+  for (const Shape &sh : kShapes)
+    if (sh.r == r && sh.u == u && sh.t == tile) return true;
+  return false;
+}
+
+std::string force_shapes_built() {
+  std::string out;
+  for (const Shape &sh : kShapes)
+    out += (out.empty() ? "" : ", ") + std::string("(") + std::to_string(sh.r) + ", " +
+           std::to_string(sh.u) + ", " + std::to_string(sh.t) + ")";
+  return out;
+}
 
 hsr_status force_plan(hsr_ctx *c) {
   hipStream_t st = c->stream;
@@ -704,8 +821,11 @@ static hsr_status force_enqueue(hsr_ctx *c, float *fx, float *fy, float *fz,
   unsigned long long *qctr = dptr<unsigned long long>(c->s.qctr);
   k_zero_u64<<<1, 64, 0, st>>>(qctr);
   k_zero_u64<<<1, 64, 0, st>>>(qctr + 1);
+  k_zero_u64<<<1, 64, 0, st>>>(qctr + 2);
   LawArgs A = law_args(c->law);
   A.cull = c->tuning.cull_off ? 0 : 1;
+  int R, U, T;
+  force_shape(c->tuning, &R, &U, &T);
   A.pmax = c->tuning.force_pass == 32 * R ? 32 * R : 64 * R;
   const int wpc = c->tuning.waves_per_cu > 0 ? c->tuning.waves_per_cu : 32;
   const int64_t grid = std::min<int64_t>(LS.nitems, (int64_t)std::max(1, c->num_cus) * wpc);
@@ -715,19 +835,33 @@ static hsr_status force_enqueue(hsr_ctx *c, float *fx, float *fy, float *fz,
   float *part = dptr<float>(LS.part);
   // events bracket the leaf-kernel launch alone (the roofline's duration)
   if (timed) HSR_HIP(hipEventRecord(c->ev[6], st));
-#define LAUNCH(LK, ORD, GRD)                                                          \
-  k_force<LK, ORD, GRD><<<(unsigned)grid, 64, 0, st>>>(cn, leaves, items, LS.counts, \
+#define LAUNCH(LK, ORD, GRD, RR, UU, TT)                                              \
+  k_force<LK, ORD, GRD, RR, UU, TT><<<(unsigned)grid, 64, 0, st>>>(cn, leaves, items, LS.counts, \
       dptr<int32_t>(LS.jidx), dptr<PJ>(c->tree.pj), dptr<int32_t>(LS.iperm), A, qctr, fx, fy, fz, \
       part, qctr + 1)
+#define SHAPES(ORD)                                                                      \
+  if (R == 2 && U == 1 && T == 256) LAUNCH(LAWK_POLY, ORD, false, 2, 1, 256);            \
+  else if (R == 2 && U == 4 && T == 256) LAUNCH(LAWK_POLY, ORD, false, 2, 4, 256);       \
+  else if (R == 1 && U == 2 && T == 256) LAUNCH(LAWK_POLY, ORD, false, 1, 2, 256);       \
+  else if (R == 1 && U == 4 && T == 256) LAUNCH(LAWK_POLY, ORD, false, 1, 4, 256);       \
+  else if (R == 4 && U == 1 && T == 256) LAUNCH(LAWK_POLY, ORD, false, 4, 1, 256);       \
+  else if (R == 2 && U == 2 && T == 128) LAUNCH(LAWK_POLY, ORD, false, 2, 2, 128);       \
+  else if (R == 2 && U == 2 && T == 512) LAUNCH(LAWK_POLY, ORD, false, 2, 2, 512);       \
+  else LAUNCH(LAWK_POLY, ORD, false, DEF_R, DEF_U, DEF_T)
   if (c->law.kind == LAWK_FIT) {
-    if (A.guard) LAUNCH(LAWK_FIT, 0, true); else LAUNCH(LAWK_FIT, 0, false);
+    if (A.guard) LAUNCH(LAWK_FIT, 0, true, DEF_R, DEF_U, DEF_T);
+    else LAUNCH(LAWK_FIT, 0, false, DEF_R, DEF_U, DEF_T);
   } else if (c->law.kind == LAWK_INTERP) {
-    if (A.guard) LAUNCH(LAWK_INTERP, 0, true); else LAUNCH(LAWK_INTERP, 0, false);
+    if (A.guard) LAUNCH(LAWK_INTERP, 0, true, DEF_R, DEF_U, DEF_T);
+    else LAUNCH(LAWK_INTERP, 0, false, DEF_R, DEF_U, DEF_T);
   } else if (c->law.order == 6) {
-    if (A.guard) LAUNCH(LAWK_POLY, 6, true); else LAUNCH(LAWK_POLY, 6, false);
+    if (A.guard) LAUNCH(LAWK_POLY, 6, true, DEF_R, DEF_U, DEF_T);
+    else { SHAPES(6); }
   } else {
-    if (A.guard) LAUNCH(LAWK_POLY, 5, true); else LAUNCH(LAWK_POLY, 5, false);
+    if (A.guard) LAUNCH(LAWK_POLY, 5, true, DEF_R, DEF_U, DEF_T);
+    else { SHAPES(5); }
   }
+#undef SHAPES
 #undef LAUNCH
   HSR_HIP(hipGetLastError());
   if (timed) {

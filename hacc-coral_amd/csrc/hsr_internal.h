@@ -213,6 +213,7 @@ struct hsr_ctx {
   // force phase captured as a hipGraph (tuning.use_graph)
   int64_t lists_gen = 0;
   int64_t evaluated_pairs = 0;  // last force phase, culled entries excluded
+  int64_t unmasked_pairs = 0;   // of which unit-mass entries inside the cutoff
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int64_t graph_gen = -1;
@@ -240,9 +241,12 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
 hsr_status lists_build_device(hsr_ctx *c, const float *min_force,
                               const float *max_force);
 hsr_status force_plan(hsr_ctx *c);
-// the leaf kernel's build-time shape: targets per lane, entries per LDS tile
-int force_targets_per_lane();
-int force_tile_entries();
+// the leaf kernel's shape (targets per lane, entries per unrolled step,
+// entries per LDS tile) for a tuning, 0 fields taking the defaults; whether
+// that shape is built for a law; the built shapes as text
+void force_shape(const hsr_tuning &t, int *r, int *u, int *tile);
+bool force_shape_built(int law_kind, bool guard, int r, int u, int tile);
+std::string force_shapes_built();
 hsr_status force_run(hsr_ctx *c, float *ax, float *ay, float *az);
 hsr_status kick_run(hsr_ctx *c, float fcoeff, float *vx, float *vy, float *vz,
                     const float *mass_sorted);

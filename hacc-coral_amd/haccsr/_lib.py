@@ -13,7 +13,9 @@ import os
 import numpy as np
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_ROOT, "libhaccsr.so")
+# HSR_LIBRARY overrides the library path (e.g. a build with other kernel
+# shapes, make KDEFS=...)
+LIB_PATH = os.environ.get("HSR_LIBRARY") or os.path.join(PKG_ROOT, "libhaccsr.so")
 CXX_LIB_PATH = os.path.join(PKG_ROOT, "libhaccsr_cxx.so")  # C++ drop-ins (ForceLaw, ...)
 
 HSR_HOST_PTRS, HSR_DEVICE_PTRS = 0, 1
@@ -84,7 +86,7 @@ class Stats(C.Structure):
                 ("drift_factor", C.c_double), ("kick_coeff", C.c_double),
                 ("cic_ms", C.c_double), ("inverse_cic_ms", C.c_double),
                 ("refresh_ms", C.c_double), ("refresh_alive", C.c_int64),
-                ("refresh_dead", C.c_int64)]
+                ("refresh_dead", C.c_int64), ("unmasked_pairs", C.c_int64)]
 
 
 class Domain(C.Structure):
@@ -112,7 +114,7 @@ class Tuning(C.Structure):
                 ("waves_per_cu", C.c_int), ("use_graph", C.c_int),
                 ("adjoint_mode", C.c_int), ("cull_off", C.c_int),
                 ("tree_order_targets", C.c_int), ("force_pass", C.c_int),
-                ("max_plan_bytes", C.c_int64)]
+                ("max_plan_bytes", C.c_int64), ("unroll", C.c_int), ("reserved0", C.c_int)]
 
 
 _lib = None

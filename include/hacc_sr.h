@@ -182,6 +182,9 @@ typedef struct {
   double refresh_ms;      /* device time of the last hsr_refresh */
   int64_t refresh_alive;  /* particles it kept alive */
   int64_t refresh_dead;   /* overload copies it appended */
+  int64_t unmasked_pairs; /* of evaluated_pairs, those of unit-mass entries
+                             whose every pair lies inside the cutoff (the leaf
+                             kernel skips their mask and mass) */
 } hsr_stats;
 
 hsr_status hsr_get_stats(hsr_ctx *ctx, hsr_stats *st);
@@ -382,10 +385,14 @@ hsr_status hsr_refresh(hsr_ctx *ctx, const hsr_domain *dom, float anow,
 
 /* ------------------------------------------------------------- tuning */
 typedef struct {
-  int i_per_lane;     /* i-particles held per lane: fixed when the library is
-                         built (2); 0 or that value */
-  int tile;           /* j entries staged per LDS tile: fixed when the library
-                         is built (256); 0 or that value */
+  int i_per_lane;     /* leaf kernel shape: i-particles held per lane (0 = 2);
+                         with unroll and tile one of the built shapes
+                         (i_per_lane, unroll, tile): (2,2,256) the default,
+                         (2,1,256) (2,4,256) (1,2,256) (1,4,256) (4,1,256)
+                         (2,2,128) (2,2,512), the non-default ones for the
+                         polynomial laws with rsm > 0; sums in another order:
+                         rounding only */
+  int tile;           /* j entries staged per LDS tile (0 = 256) */
   int chunk;          /* maximum list entries per work item: 0 (= 8192) or
                          tile..2^30; a change re-plans lists already built
                          (on failure the old tuning and plan are restored
@@ -413,6 +420,8 @@ typedef struct {
                          items, their sort keys, chunk partials); 0: no cap.
                          A plan above it fails with HSR_ERR_NOMEM before any
                          launch (so does one of 2^31 work items or more) */
+  int unroll;         /* leaf kernel shape: j entries per unrolled step (0 = 2) */
+  int reserved0;
 } hsr_tuning;
 
 /* Out-of-range fields give HSR_ERR_ARG and leave the tuning unchanged. */

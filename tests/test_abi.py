@@ -60,6 +60,8 @@ int main(void) {
          offsetof(hsr_stats, kick_coeff));
   printf("%zu %zu %zu %zu\n", sizeof(hsr_domain), offsetof(hsr_domain, ng_local_total),
          offsetof(hsr_domain, grid2phys_vel), offsetof(hsr_stats, refresh_dead));
+  printf("%zu %zu %zu\n", offsetof(hsr_stats, unmasked_pairs), offsetof(hsr_tuning, unroll),
+         offsetof(hsr_params, n_interp));
   return 0;
 }
 """)
@@ -74,7 +76,9 @@ int main(void) {
             _lib.SubcycleParams.ng.offset, _lib.SubcycleParams.pp.offset,
             _lib.SubcycleParams.fscal.offset, _lib.Stats.kick_coeff.offset,
             C.sizeof(_lib.Domain), _lib.Domain.ng_local_total.offset,
-            _lib.Domain.grid2phys_vel.offset, _lib.Stats.refresh_dead.offset]
+            _lib.Domain.grid2phys_vel.offset, _lib.Stats.refresh_dead.offset,
+            _lib.Stats.unmasked_pairs.offset, _lib.Tuning.unroll.offset,
+            _lib.Params.n_interp.offset]
     assert got == want
     assert _lib.NODE_DTYPE.itemsize == C.sizeof(_lib.Node)
 

@@ -230,7 +230,8 @@ def test_tuning_validation_and_chunk_replan():
     tp, tree, lists, ctx, arrs = run_both(x, y, z, m, [0, 0, 0], [14.0] * 3, flo, fhi, 96)
     before = _lib.Tuning()
     assert ctx.L.hsr_get_tuning(ctx.h, ctypes.byref(before)) == 0
-    for bad in (dict(tile=128), dict(i_per_lane=3), dict(chunk=-1), dict(chunk=(1 << 30) + 1),
+    for bad in (dict(tile=64), dict(i_per_lane=3), dict(unroll=3), dict(tile=128, i_per_lane=4),
+                dict(chunk=-1), dict(chunk=(1 << 30) + 1),
                 dict(waves_per_cu=-2), dict(waves_per_cu=1000), dict(force_pass=96)):
         with pytest.raises(_lib.HsrError):
             ctx.set_tuning(**bad)
@@ -246,6 +247,37 @@ def test_tuning_validation_and_chunk_replan():
     assert np.array_equal(replanned, fresh)
     e2 = check_forces(tp, tree, lists, ctx)
     say(f"  rel rms chunk 8192 {e1:.3e}, chunk 256 {e2:.3e}")
+
+
+SHAPES = [(2, 2, 256), (2, 1, 256), (2, 4, 256), (1, 2, 256), (1, 4, 256), (4, 1, 256),
+          (2, 2, 128), (2, 2, 512)]
+
+
+def test_every_built_kernel_shape_matches_oracle():
+    """Each leaf-kernel shape the tuner can pick (i-particles per lane, j
+    entries per unrolled step, entries per LDS tile) gives forces within the
+    tolerance of the oracle on a clustered load with long, chunked lists;
+    shapes change only the order of the float sums."""
+    say("case every built kernel shape, clustered 40^3, nd=256")
+    load = ics.config_load("B", n_side=40)
+    m = np.ones(load.n, np.float32)
+    tp, tree, lists, ctx, arrs = run_both(load.x, load.y, load.z, m, *load.tree_box,
+                                          *load.force_box, 256)
+    check_lists(lists, ctx)
+    ctx.set_tuning(chunk=1024)
+    base = None
+    for r, u, t in SHAPES:
+        ctx.set_tuning(i_per_lane=r, unroll=u, tile=t, chunk=1024)
+        e = check_forces(tp, tree, lists, ctx)
+        st = ctx.stats()
+        f = np.stack(ctx.forces(), 1)
+        if base is None:
+            base = f
+        d = rel_rms(f, base)
+        say(f"  shape {(r, u, t)}: rel rms vs oracle {e:.3e}, vs default shape {d:.3e}, "
+            f"unmasked {st.unmasked_pairs / max(st.evaluated_pairs, 1):.3f} of evaluated")
+        assert d <= 2e-6
+    ctx.close()
 
 
 def test_duplicates_and_degenerate_nodes():
