@@ -29,10 +29,27 @@ sys.path.insert(0, os.path.join(ROOT, "hacc-coral_amd"))
 FLOPS_PER_PAIR = 30          # d 3, r2 5, +eps2 1, rsqrt 1, cube 2, P5 10, sub 1, *m 1, acc 6
 PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 vector peak (packed); non-packed 78.6
 PEAK_HBM_GBPS = 8000.0       # MI355X HBM3E
-# HBM bytes per k_force launch measured with rocprofv3 (FETCH_SIZE x 2.0 + WRITE_SIZE
-# x 1.0, factors from a 256 MiB device copy) on the 256^3 clustered load at leaf size
-# 512: profiles/k_force_256c_traffic.md.  Reported only for that workload.
-MEASURED_TRAFFIC = {("B", 256, 512, 1): 142726894856}
+# HBM bytes per k_force launch measured with rocprofv3 (FETCH_SIZE and WRITE_SIZE,
+# scaled by what they read for a plain device copy of known size), keyed by the
+# workload (config, n_side, leaf size, TDPTS), the kernel shape and the first 16
+# hex digits of the SHA-256 of the leaf kernel's source (csrc/force.hip): a
+# figure is reported only for the build it was measured on.
+MEASURED_TRAFFIC = {
+    # profiles/k_force_256c_traffic.md
+    ("B", 256, 640, 1, (2, 2, 128), "6237f4f277288489"): 147129793408,
+}
+
+
+def _kernel_src_hash():
+    import hashlib
+    path = os.path.join(ROOT, "hacc-coral_amd", "csrc", "force.hip")
+    try:
+        return hashlib.sha256(open(path, "rb").read()).hexdigest()[:16]
+    except OSError:
+        return None
+
+
+KERNEL_SRC = _kernel_src_hash()
 # Algorithmic bytes of the tree build: 48 B per particle per level it is split at
 # (cm reads x, y, z, m; partition reads and writes key + index) plus one 84 B
 # gather of the ten payload arrays.
@@ -49,7 +66,6 @@ def parse():
     ap.add_argument("--n-side", type=int, default=None, help="override the load's n^(1/3)")
     ap.add_argument("--nd", type=int, default=0, help="leaf size (0 = autotuned)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--adjoint-reps", type=int, default=3,
                     help="timed hsr_adjoint calls on the last step's tree (0 = skip)")
     ap.add_argument("--adjoint-mode", type=int, default=0,
@@ -102,9 +118,33 @@ def ensemble_name(ens, n, nd):
             f"leaf launch over all boxes")
 
 
-def cpu_baseline(load, nd, seconds):
-    """The oracle (CPU restatement of the reference's float kernel, OpenMP
-    over leaves) on a sample of the same tree's leaves."""
+def host_cpu():
+    """lscpu's model name and physical core count of this host."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except Exception:
+        pass
+    try:
+        phys = int(info.get("Core(s) per socket", "0")) * int(info.get("Socket(s)", "0"))
+    except ValueError:
+        phys = 0
+    return {"cpu_model": info.get("Model name", "unknown"), "physical_cores": phys or None,
+            "logical_cpus": os.cpu_count()}
+
+
+def cpu_baseline(load, nd):
+    """The oracle (CPU restatement of the reference's tree build, lists and
+    float kernel; OpenMP over leaves for lists and kernel) on the host:
+    the tree build of the whole load once, then lists + kernel on a random
+    sample of the same tree's listed leaves, median of 5 after a warm-up, on
+    all the threads this process may use and on one thread.  The sample is
+    8 leaves per thread (at least 64) for all threads and 8 leaves for one
+    thread: about 20-30 s of host time at 256^3."""
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     lo, hi = load.tree_box
@@ -114,28 +154,74 @@ def cpu_baseline(load, nd, seconds):
     tree = O.build_tree(tp, load.x, load.y, load.z)
     t_tree = time.perf_counter() - t0
     leaves = O.leaf_order(tp, tree)
+    total_pairs = O.build_lists(tp, tree, leaves).pairs(tree)
     rng = np.random.default_rng(7)
-    # grow a random leaf sample until its kernel time reaches the budget
     order = rng.permutation(leaves.shape[0])
-    k = 64
-    done_pairs, done_t = 0, 0.0
-    used = 0
-    while used < order.shape[0] and done_t < seconds:
-        take = order[used:used + k]
-        used += take.shape[0]
-        sub = np.sort(leaves[take])
+
+    def run(sub):
+        t = time.perf_counter()
         L = O.build_lists(tp, tree, sub)
-        t0 = time.perf_counter()
         O.forces_f32(tp, tree, L)
-        done_t += time.perf_counter() - t0
-        done_pairs += L.pairs(tree)
-        k = min(k * 2, 4096)
-    rate = done_pairs / done_t
+        return time.perf_counter() - t, L.pairs(tree)
+
+    def median5(sub):
+        run(sub)  # warm-up
+        ts = sorted(run(sub)[0] for _ in range(5))
+        return ts[2]
+
+    # enough leaves that dynamic scheduling balances the threads (list
+    # lengths spread by 100x between voids and halo cores), and for one
+    # thread the same number of leaves per thread
+    O.set_threads(threads)
+    k = min(leaves.shape[0], max(64, 8 * threads))
+    sub = np.sort(leaves[order[:k]])
+    t_all = median5(sub)
+    pairs = run(sub)[1]
+    k1 = max(1, k // max(threads, 1))
+    sub1 = np.sort(leaves[order[:k1]])
+    pairs1 = run(sub1)[1]
+    O.set_threads(1)
+    t_one = median5(sub1)
+    O.set_threads(threads)
+    rate = pairs / t_all
+    rate1 = pairs1 / t_one
     return {"value": rate, "unit": "pairs/s", "cores": threads, "kind": "port",
-            "sample": (f"oracle float kernel (nbody1 restatement, powf law) on {used} random "
-                       f"listed leaves of the same tree ({done_pairs:.3e} pairs in {done_t:.1f} s, "
-                       f"{threads} OpenMP threads); oracle tree build of the whole load took "
-                       f"{t_tree:.1f} s single-threaded")}
+            **host_cpu(),
+            "one_thread": {"value": rate1, "unit": "pairs/s", "sample_leaves": int(k1),
+                           "sample_pairs": int(pairs1), "median_s": t_one},
+            "tree_build_s": t_tree, "tree_build_threads": 1,
+            "lists_kernel_median_s": t_all,
+            "full_step_estimate_s": t_tree + total_pairs / rate,
+            "sampled_fraction_of_pairs": pairs / total_pairs if total_pairs else 0.0,
+            "sample": (f"oracle (CPU restatement of the reference path): lists + float kernel "
+                       f"(nbody1 restatement, powf law) on {k} random listed leaves of the "
+                       f"same tree ({pairs:.3e} pairs = {100 * pairs / total_pairs:.2f}% of the "
+                       f"step's), median of 5 after a warm-up on {threads} OpenMP threads; "
+                       f"one-thread figure on {k1} leaves; tree build of the whole load once, "
+                       f"single-threaded ({t_tree:.1f} s)")}
+
+
+def fp64_error(ctx, n):
+    """The float forces of the last step's lists against the device FP64
+    evaluation of the same lists (equal to the CPU FP64 oracle): relative
+    RMS over the listed particles and the largest relative error where |F|
+    exceeds 1e-3 of its RMS."""
+    fx, fy, fz = ctx.forces()
+    dx, dy, dz = ctx.forces_f64()
+    nodes = ctx.nodes()
+    leaves = ctx.get_lists()[0]
+    listed = np.zeros(n + 1, np.int32)
+    np.add.at(listed, nodes["offset"][leaves], 1)
+    np.add.at(listed, nodes["offset"][leaves] + nodes["count"][leaves], -1)
+    sel = np.cumsum(listed[:n]) > 0
+    f = np.stack([fx, fy, fz], 1)[sel].astype(np.float64)
+    d = np.stack([dx, dy, dz], 1)[sel]
+    e = np.linalg.norm(f - d, axis=1)
+    nrm = np.linalg.norm(d, axis=1)
+    big = nrm > 1e-3 * np.sqrt(np.mean(nrm ** 2))
+    return {"rel_rms": float(np.sqrt(np.sum(e ** 2) / np.sum(nrm ** 2))),
+            "max_rel": float((e[big] / nrm[big]).max()) if big.any() else 0.0,
+            "particles": int(sel.sum()), "reference": "device FP64 evaluation of the same lists"}
 
 
 def build_roofline(ctx, n, build_ms):
@@ -464,6 +550,8 @@ def main():
         f"  lists {np.mean([s[2] for s in stats]):.2f}  kick {np.mean([s[3] for s in stats]):.2f}"
         f"  leaf kernel {kern_ms:.2f} ms  nodes {st.nodes} listed leaves {st.listed_leaves}"
         f"  max depth {st.max_depth}")
+    out_shape = {"i_per_lane": shape["i_per_lane"] or 2, "unroll": shape["unroll"] or 2,
+                 "tile": shape["tile"] or 256}
     out = {
         "metric": "pair interactions/s & % FP32 VALU peak at 256^3; adjoint/forward time",
         "value": pairs / (ms_per_step * 1e-3),
@@ -480,24 +568,27 @@ def main():
         "config": {"workload": (workload_name(a.config, load, nd) if ens is None else
                                 ensemble_name(ens, n, nd)), "particles": n,
                    "pairs_per_step": pairs, "list_entries": int(st.entries), "leaf_size": nd,
-                   "kernel_shape": {"i_per_lane": shape["i_per_lane"] or 2,
-                                    "unroll": shape["unroll"] or 2, "tile": shape["tile"] or 256},
+                   "kernel_shape": out_shape,
                    "autotune": tuned,
                    "breakdown_ms": {"build": float(np.mean([s[1] for s in stats])),
                                     "lists": float(np.mean([s[2] for s in stats])),
                                     "kick": float(np.mean([s[3] for s in stats])),
                                     "leaf_kernel": kern_ms}},
-        "roofline": {"bound": "mfma", "pipe": "FP32 VALU (no MFMA: the pair force is not a contraction)",
+        "roofline": {"bound": "valu", "pipe": "FP32 VALU (no MFMA: the pair force is not a contraction)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": MEASURED_TRAFFIC.get((a.config, a.n_side or 256, nd, a.tdpts)),
+                     "traffic": MEASURED_TRAFFIC.get((a.config, a.n_side or 256, nd, a.tdpts,
+                                                      tuple(out_shape.values()), KERNEL_SRC)),
                      "kernel": "k_force (leaf kernel)", "flops_per_pair": FLOPS_PER_PAIR,
-                     "evaluated_pairs": evaluated,
+                     "evaluated_pairs": evaluated, "unmasked_pairs": int(st.unmasked_pairs),
                      "culled_fraction_of_list_pairs": 1.0 - evaluated / pairs if pairs else 0.0},
         "hbm_passes": build_roofline(ctxs[nd], n, float(np.mean([s[1] for s in stats]))),
     }
     if a.tdpts != 1:
         out["config"]["workload"] += "; quadrupole tree (TDPTS 12)"
+    out["roofline"]["kernel_source_sha16"] = KERNEL_SRC
+    out["fp64_error"] = fp64_error(ctxs[nd], n)
+    log(f"float vs FP64 {out['fp64_error']}")
     if a.adjoint_reps > 0 and a.tdpts == 1:
         ctxs[nd].set_tuning(adjoint_mode=a.adjoint_mode)
         out["adjoint"] = adjoint_leg(ctxs[nd], n, dev, a.adjoint_reps, stats)
@@ -515,7 +606,7 @@ def main():
         log(f"refresh {out['refresh']}")
     if not a.no_cpu and ens is None:
         log("cpu baseline leg ...")
-        out["cpu_baseline"] = cpu_baseline(load, nd, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(load, nd)
     print(json.dumps(out), flush=True)
 
 

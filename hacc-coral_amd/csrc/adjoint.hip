@@ -862,82 +862,42 @@ k_pseudo_grad(int64_t np, int64_t p0, int64_t nn, int64_t n,
 }
 
 // Per node: H_n = (sum of its pseudo segment terms, in order) / S_n, the push
-// factor of its members (S_n its member mass, see below).  The two-pass
-// schedule's terms already carry the pseudo mass ppm; the fused schedule's W
-// sums do not, so they are scaled by ppm here.
-__global__ void k_pseudo_node(int64_t np, int64_t p0, int64_t nn,
-                              const int32_t *key, const float *gseg,
-                              const hsr_node *cn, const double *mpre, int with_ppm,
-                              float *H) {
-  int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+// factor of its members.  S_n, the member mass sum over [offset, offset +
+// count), is the denominator of the node's centre of mass xc (cm,
+// BGQCM.c:181-212), so a member k moves the pseudo-particle by m_k / S_n.
+// S_n differs from the pseudo mass ppm when the node holds an unsplittable
+// leaf, whose ppm the reference builds from two empty child slots
+// (RCBForceTree.cxx:727-729, 856-889) and leaves 0.  One wave per run of a
+// node's sorted pseudo segments sums S_n in float, lanes strided over the
+// members and then a fixed xor tree (deterministic; a float sum suffices for
+// a derivative factor).  The two-pass schedule's terms already carry the
+// pseudo mass ppm; the fused schedule's W sums do not, so they are scaled by
+// ppm here.
+__global__ void __launch_bounds__(64)
+k_pseudo_node(int64_t np, int64_t p0, int64_t nn, const int32_t *key, const float *gseg,
+              const hsr_node *cn, const PJ *pj, int with_ppm, float *H) {
+  const int64_t q = blockIdx.x;
   if (q >= np) return;
   const int64_t p = p0 + q;
   if (q > 0 && key[p - 1] == key[p]) return;  // not a run start
   const int64_t node = key[p] - nn;
+  const hsr_node &nd = cn[node];
+  const int l = threadIdx.x;
+  float m = 0.f;
+  for (int64_t i = nd.offset + l; i < nd.offset + nd.count; i += 64) m += pj[i].m;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m += __shfl_xor(m, o);
+  if (l != 0) return;
   float sx = 0.f, sy = 0.f, sz = 0.f;
   for (int64_t r = q; r < np && key[p0 + r] == key[p]; ++r) {
     sx += gseg[3 * r + 0];
     sy += gseg[3 * r + 1];
     sz += gseg[3 * r + 2];
   }
-  const hsr_node &nd = cn[node];
-  const double S = mpre[nd.offset + nd.count] - mpre[nd.offset];
-  const float f = S > 0.0 ? (float)((with_ppm ? (double)nd.ppm[0] : 1.0) / S) : 0.f;
+  const float f = m > 0.f ? (with_ppm ? nd.ppm[0] : 1.f) / m : 0.f;
   H[3 * node + 0] = sx * f;
   H[3 * node + 1] = sy * f;
   H[3 * node + 2] = sz * f;
-}
-
-// Member mass of a node, S_n = sum of m over [offset, offset + count): the
-// denominator of its centre of mass xc (cm, BGQCM.c:181-212), so a member k
-// moves the pseudo-particle by m_k / S_n.  S_n differs from the pseudo mass
-// ppm when the node holds an unsplittable leaf, whose ppm the reference builds
-// from two empty child slots (RCBForceTree.cxx:727-729, 856-889) and leaves 0.
-// S_n comes from an FP64 prefix of the sorted masses, summed in a fixed order:
-// per block of MASS_BLK elements (k_mass_blk), blocks scanned on the host, then
-// the per-element prefix (k_mass_pre).
-constexpr int MASS_BLK = 2048;  // 256 threads x 8 elements
-
-__global__ void __launch_bounds__(256)
-k_mass_blk(int64_t n, const PJ *pj, double *bsum) {
-  __shared__ double sh[256];
-  const int t = threadIdx.x;
-  const int64_t base = (int64_t)blockIdx.x * MASS_BLK + 8 * t;
-  double s = 0.0;
-  for (int j = 0; j < 8; ++j)
-    if (base + j < n) s += (double)pj[base + j].m;
-  sh[t] = s;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (t < o) sh[t] += sh[t + o];
-    __syncthreads();
-  }
-  if (t == 0) bsum[blockIdx.x] = sh[0];
-}
-
-__global__ void __launch_bounds__(256)
-k_mass_pre(int64_t n, const PJ *pj, const double *bpre, double *pre) {
-  __shared__ double sh[256];
-  const int t = threadIdx.x;
-  const int64_t base = (int64_t)blockIdx.x * MASS_BLK + 8 * t;
-  double s = 0.0;
-  for (int j = 0; j < 8; ++j)
-    if (base + j < n) s += (double)pj[base + j].m;
-  sh[t] = s;
-  __syncthreads();
-  for (int o = 1; o < 256; o <<= 1) {  // inclusive scan, fixed order
-    const double v = t >= o ? sh[t - o] : 0.0;
-    __syncthreads();
-    sh[t] += v;
-    __syncthreads();
-  }
-  double run = bpre[blockIdx.x] + (t > 0 ? sh[t - 1] : 0.0);
-  for (int j = 0; j < 8; ++j) {
-    const int64_t i = base + j;
-    if (i > n) break;
-    pre[i] = run;
-    if (i < n) run += (double)pj[i].m;
-  }
 }
 
 __global__ void k_fill(int64_t n, float *a) {
@@ -1126,27 +1086,6 @@ hsr_status adjoint_plan(hsr_ctx *c) {
   A.n_tr_leaves = nruns;
   tail_items[1] = A.n_fw_items;
   HSR_HIP(hipMemcpyAsync(ffirst + nl, &tail_items[1], sizeof(int64_t), hipMemcpyHostToDevice, st));
-  // member-mass prefix (FP64, fixed summation order) for the pseudo push-down
-  {
-    const int64_t nb = (n + 1 + MASS_BLK - 1) / MASS_BLK;
-    HSR_HIP(dbuf_reserve(A.msum, sizeof(double) * (2 * nb + n + 2)));
-    double *bsum = dptr<double>(A.msum), *bpre = bsum + nb, *pre = bpre + nb;
-    const PJ *pj = dptr<PJ>(T.pj);
-    k_mass_blk<<<(unsigned)nb, 256, 0, st>>>(n, pj, bsum);
-    std::vector<double> hb(nb);
-    HSR_HIP(hipMemcpyAsync(hb.data(), bsum, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
-    HSR_HIP(hipStreamSynchronize(st));
-    double acc = 0.0;
-    for (int64_t b = 0; b < nb; ++b) {
-      const double v = hb[b];
-      hb[b] = acc;
-      acc += v;
-    }
-    HSR_HIP(hipMemcpyAsync(bpre, hb.data(), sizeof(double) * nb, hipMemcpyHostToDevice, st));
-    k_mass_pre<<<(unsigned)nb, 256, 0, st>>>(n, pj, bpre, pre);
-    HSR_HIP(hipStreamSynchronize(st));
-    A.mpre = pre;
-  }
   // canonical parents (for the pseudo push-down)
   HSR_HIP(dbuf_reserve(A.par, sizeof(int32_t) * (nn + 1) + sizeof(int64_t) * (T.nboxes + 1)));
   int32_t *par = dptr<int32_t>(A.par);
@@ -1308,7 +1247,7 @@ static hsr_status adjoint_enqueue(hsr_ctx *c, const float *lx, const float *ly,
   if (A.npseudo && A.fused) {
     k_pseudo_from_w<<<nblk(A.npseudo, 256), 256, 0, st>>>(A.npseudo, A.nsp, A.val, A.seg_t, ent_ptr, seg_e,
                                                           W, W + ne, W + 2 * ne, gseg);
-    k_pseudo_node<<<nblk(A.npseudo, 256), 256, 0, st>>>(A.npseudo, A.nsp, nn, A.key, gseg, cn, A.mpre, 1, H);
+    k_pseudo_node<<<(unsigned)A.npseudo, 64, 0, st>>>(A.npseudo, A.nsp, nn, A.key, gseg, cn, pj, 1, H);
     k_pseudo_push<<<(unsigned)nn, 64, 0, st>>>(nn, cn, dptr<int32_t>(A.par), H, pj, gx, gy, gz);
   } else if (A.npseudo) {
     if (L.order == 6)
@@ -1317,7 +1256,7 @@ static hsr_status adjoint_enqueue(hsr_ctx *c, const float *lx, const float *ly,
     else
       k_pseudo_grad<5><<<(unsigned)A.npseudo, 64, 0, st>>>(A.npseudo, A.nsp, nn, n, A.key, A.val, A.seg_t,
                                                            dptr<int32_t>(c->lists.leaves), cn, pj, lx, ly, lz, L, gseg);
-    k_pseudo_node<<<nblk(A.npseudo, 256), 256, 0, st>>>(A.npseudo, A.nsp, nn, A.key, gseg, cn, A.mpre, 0, H);
+    k_pseudo_node<<<(unsigned)A.npseudo, 64, 0, st>>>(A.npseudo, A.nsp, nn, A.key, gseg, cn, pj, 0, H);
     k_pseudo_push<<<(unsigned)nn, 64, 0, st>>>(nn, cn, dptr<int32_t>(A.par), H, pj, gx, gy, gz);
   }
   HSR_HIP(hipGetLastError());

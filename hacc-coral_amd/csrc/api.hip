@@ -449,7 +449,7 @@ hsr_status hsr_destroy(hsr_ctx *c) {
                   &c->lists.plan, &c->lists.part, &c->s.qctr,
                   &c->adj.fw, &c->adj.fw_items, &c->adj.tr, &c->adj.tr_items, &c->adj.seg,
                   &c->adj.tj, &c->adj.par, &c->adj.counts, &c->adj.qctr, &c->adj.part, &c->adj.H,
-                  &c->adj.W, &c->adj.msum, &c->law_bins,
+                  &c->adj.W, &c->law_bins,
                   &c->s.a, &c->s.b, &c->s.c, &c->s.d, &c->s.e, &c->s.f, &c->s.g, &c->s.h,
                   &c->s.cub, &c->s.cms, &c->s.cmfail, &c->s.cmx, &c->s.x1, &c->s.y1, &c->s.z1, &c->s.m1, &c->s.p1,
                   &c->s.x0, &c->s.y0, &c->s.z0, &c->s.m0, &c->s.p0, &c->s.blocks, &c->s.rights,

@@ -142,8 +142,7 @@ struct ListState {
 // transposed (source-major) lists and the pseudo-particle segments.
 struct AdjState {
   int64_t gen = -1;
-  DBuf fw, fw_items, tr, tr_items, seg, tj, par, counts, qctr, part, H, W, msum;
-  const double *mpre = nullptr;  // FP64 prefix of the sorted masses [n+1]
+  DBuf fw, fw_items, tr, tr_items, seg, tj, par, counts, qctr, part, H, W;
   int64_t n_fw_items = 0, n_fw_leaves = 0, n_tr_items = 0, n_tr_leaves = 0;
   int64_t nsp = 0, npseudo = 0, fw_part = 0, tr_part = 0;
   bool fused = false;        // one pair evaluation feeds both sides (W per entry)

@@ -10,8 +10,17 @@
 #include "hsr_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* OpenMP threads of the parallel loops below (the CPU baseline times 1 and
+ * all threads); returns the previous setting. */
+int hsro_set_threads(int n) {
+  const int prev = omp_get_max_threads();
+  if (n > 0) omp_set_num_threads(n);
+  return prev;
+}
 
 /* ------------------------------------------------------------------ cm */
 /* BGQCM.c:181-212 (generic branch): tight bbox via fminf/fmaxf seeded with

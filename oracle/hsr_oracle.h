@@ -131,6 +131,7 @@ int64_t hsro_list(const hsro_params *p, const hsro_node *nodes,
 
 /* All lists for the leaves in `leaves` (CSR over segments). seg_ptr has
  * nleaves+1 entries.  Returns total segments or -1. Uses OpenMP. */
+int hsro_set_threads(int n);
 int64_t hsro_lists_all(const hsro_params *p, const hsro_node *nodes,
                        int64_t nnodes, const int64_t *leaves, int64_t nleaves,
                        int64_t *seg_ptr, int64_t *seg_a, int64_t *seg_b,

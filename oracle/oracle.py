@@ -66,6 +66,11 @@ def build():
         subprocess.check_call(["make", "-s", "-C", HERE, "ref"])
 
 
+def set_threads(n: int) -> int:
+    """OpenMP threads of the oracle's parallel loops; returns the previous."""
+    return int(lib().hsro_set_threads(int(n)))
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -95,6 +100,8 @@ def load(path):
     i64 = P(np.int64, flags="C")
     u16 = P(np.uint16, flags="C")
     nodes = P(NODE_DTYPE, flags="C")
+    L.hsro_set_threads.argtypes = [C.c_int]
+    L.hsro_set_threads.restype = C.c_int
     L.hsro_cm.argtypes = [C.c_int64, f32, f32, f32, f32, f32, f32, f32]
     L.hsro_partition.argtypes = [C.c_int64] + [f32] * 8 + [i64, u16, C.c_float]
     L.hsro_partition.restype = C.c_int64
