@@ -449,7 +449,7 @@ hsr_status hsr_destroy(hsr_ctx *c) {
                   &c->lists.plan, &c->lists.part, &c->s.qctr,
                   &c->adj.fw, &c->adj.fw_items, &c->adj.tr, &c->adj.tr_items, &c->adj.seg,
                   &c->adj.tj, &c->adj.par, &c->adj.counts, &c->adj.qctr, &c->adj.part, &c->adj.H,
-                  &c->adj.W, &c->law_bins,
+                  &c->adj.W, &c->law_bins, &c->f64_at, &c->f64_g,
                   &c->s.a, &c->s.b, &c->s.c, &c->s.d, &c->s.e, &c->s.f, &c->s.g, &c->s.h,
                   &c->s.cub, &c->s.cms, &c->s.cmfail, &c->s.cmx, &c->s.x1, &c->s.y1, &c->s.z1, &c->s.m1, &c->s.p1,
                   &c->s.x0, &c->s.y0, &c->s.z0, &c->s.m0, &c->s.p0, &c->s.blocks, &c->s.rights,
@@ -734,6 +734,40 @@ hsr_status hsr_forces_f64(hsr_ctx *c, double *ax, double *ay, double *az,
     if (leaf_mom && nl)
       HSR_HIP(hipMemcpyAsync(leaf_mom, d[3], sizeof(double) * 3 * nl, hipMemcpyDeviceToHost, c->stream));
   }
+  HSR_HIP(hipStreamSynchronize(c->stream));
+  return HSR_OK;
+}
+
+hsr_status hsr_forces_f64_at(hsr_ctx *c, const double *x, const double *y, const double *z,
+                             double *ax, double *ay, double *az) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (!x || !y || !z || !ax || !ay || !az) {
+    g_err = "hsr_forces_f64_at: null argument";
+    return HSR_ERR_ARG;
+  }
+  if ((s = forces_f64_at_run(c, x, y, z, ax, ay, az))) return s;
+  HSR_HIP(hipStreamSynchronize(c->stream));
+  return HSR_OK;
+}
+
+hsr_status hsr_adjoint_f64_at(hsr_ctx *c, const double *x, const double *y, const double *z,
+                              const double *lx, const double *ly, const double *lz, double *gx,
+                              double *gy, double *gz) {
+  hsr_status s = check_ctx(c);
+  if (s) return s;
+  if (!x || !y || !z || !lx || !ly || !lz || !gx || !gy || !gz) {
+    g_err = "hsr_adjoint_f64_at: null argument";
+    return HSR_ERR_ARG;
+  }
+  const int64_t n = c->tree.n, ng = n + c->tree.nnodes + 1;
+  HSR_HIP(dbuf_reserve(c->f64_g, sizeof(double) * 3 * (size_t)ng));
+  double *g = dptr<double>(c->f64_g);
+  if ((s = adjoint_f64_at_run(c, x, y, z, lx, ly, lz, g, g + ng, g + 2 * ng))) return s;
+  double *out[3] = {gx, gy, gz};
+  if (n)
+    for (int k = 0; k < 3; ++k)
+      HSR_HIP(hipMemcpyAsync(out[k], g + k * ng, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
   HSR_HIP(hipStreamSynchronize(c->stream));
   return HSR_OK;
 }

@@ -189,6 +189,8 @@ struct hsr_ctx {
   hsr_params p;
   ForceLawDev law;
   DBuf law_bins;  // LAWK_INTERP table on the device
+  DBuf f64_at;    // FP64-at-positions source records and node masses
+  DBuf f64_g;     // FP64-at-positions VJP with its node part
   hsr_tuning tuning;
   hipStream_t stream = nullptr;
   hipEvent_t ev[8];
@@ -252,6 +254,11 @@ hsr_status kick_run(hsr_ctx *c, float fcoeff, float *vx, float *vy, float *vz,
 hsr_status adjoint_run(hsr_ctx *c, const float *lx, const float *ly,
                        const float *lz, float *gx, float *gy, float *gz);
 hsr_status forces_f64_run(hsr_ctx *c, double *ax, double *ay, double *az, double *mom);
+hsr_status forces_f64_at_run(hsr_ctx *c, const double *x, const double *y, const double *z,
+                             double *ax, double *ay, double *az);
+hsr_status adjoint_f64_at_run(hsr_ctx *c, const double *x, const double *y, const double *z,
+                              const double *lx, const double *ly, const double *lz, double *gx,
+                              double *gy, double *gz);
 hsr_status leaf_momentum_run(hsr_ctx *c, const float *ax, const float *ay, const float *az,
                              double *mom);
 hsr_status drift_run(hsr_ctx *c, int64_t n, float k, float *x, float *y, float *z,

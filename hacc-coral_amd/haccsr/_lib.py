@@ -34,7 +34,7 @@ EXPORTS = [
     "hsr_get_tuning", "hsr_forces_f64", "hsr_leaf_momentum", "hsr_drift",
     "hsr_resort", "hsr_map2", "hsr_subcycle", "hsr_get_pseudo_masses", "hsr_cic",
     "hsr_inverse_cic", "hsr_domain_init", "hsr_refresh", "hsr_resort_noalloc",
-    "hsr_set_law_table",
+    "hsr_set_law_table", "hsr_forces_f64_at", "hsr_adjoint_f64_at",
 ]
 
 
@@ -156,6 +156,8 @@ def load(path: str | None = None):
     L.hsr_resort.argtypes = [vp, i64, vp] + [vp] * 10 + [vp, C.c_int]
     L.hsr_resort_noalloc.argtypes = [vp, i64, vp] + [vp] * 10 + [vp, C.c_int]
     L.hsr_set_law_table.argtypes = [vp, vp, vp, i64]
+    L.hsr_forces_f64_at.argtypes = [vp] + [vp] * 6
+    L.hsr_adjoint_f64_at.argtypes = [vp] + [vp] * 9
     L.hsr_map2.argtypes = [vp, C.POINTER(SubcycleParams), C.c_double, i64] + [vp] * 10 + \
         [vp, C.c_int]
     L.hsr_subcycle.argtypes = [vp, C.POINTER(SubcycleParams), i64] + [vp] * 10 + [vp, C.c_int]
@@ -325,6 +327,25 @@ class Context:
         check(self.L.hsr_forces_f64(self.h, _ptr(ax), _ptr(ay), _ptr(az), _ptr(leaf_mom), fl),
               "hsr_forces_f64")
         return (ax, ay, az, leaf_mom) if with_momentum else (ax, ay, az)
+
+    def forces_f64_at(self, x, y, z):
+        """FP64 forces at double positions (tree order, cuda float64
+        tensors) with the current tree and lists frozen; returns (ax, ay, az)
+        as cuda float64 tensors."""
+        import torch
+        out = [torch.empty_like(x) for _ in range(3)]
+        check(self.L.hsr_forces_f64_at(self.h, _ptr(x), _ptr(y), _ptr(z), *map(_ptr, out)),
+              "hsr_forces_f64_at")
+        return tuple(out)
+
+    def adjoint_f64_at(self, x, y, z, lx, ly, lz):
+        """FP64 d(sum lambda . F)/dx at double positions (tree order, cuda
+        float64 tensors), topology frozen; returns (gx, gy, gz)."""
+        import torch
+        out = [torch.empty_like(x) for _ in range(3)]
+        check(self.L.hsr_adjoint_f64_at(self.h, _ptr(x), _ptr(y), _ptr(z), _ptr(lx), _ptr(ly),
+                                        _ptr(lz), *map(_ptr, out)), "hsr_adjoint_f64_at")
+        return tuple(out)
 
     def leaf_momentum(self, ax, ay, az, leaf_mom=None, device_ptrs=False):
         """sum_i m_i a_i per listed leaf (FP64) of float accelerations."""

@@ -34,6 +34,7 @@ class Topology:
     seg_a: np.ndarray
     seg_b: np.ndarray
     entries: np.ndarray
+    pos: np.ndarray | None = None  # the float positions the tree was built from (input order)
 
 
 class ForceSolver:
@@ -93,7 +94,7 @@ class ForceSolver:
         if self.record if record is None else record:
             leaves, seg_ptr, sa, sb, ent = self.ctx.get_lists()
             self.topologies.append(Topology(perm.cpu().numpy(), self.ctx.nodes(), leaves,
-                                            seg_ptr, sa, sb, ent))
+                                            seg_ptr, sa, sb, ent, pos.detach().cpu().numpy()))
         return out
 
     def vjp(self, pos: torch.Tensor, lam: torch.Tensor) -> torch.Tensor:

@@ -235,6 +235,22 @@ hsr_status hsr_forces_f64(hsr_ctx *ctx, double *ax, double *ay, double *az,
 hsr_status hsr_leaf_momentum(hsr_ctx *ctx, const float *ax, const float *ay,
                              const float *az, double *leaf_mom, int ptr_flags);
 
+/* FP64 oracle of the differentiation checks (no counterpart in the
+ * reference): the force and its vector-Jacobian product at given double
+ * positions with the tree and lists of the last hsr_build / hsr_lists
+ * frozen.  Positions are in tree order; pseudo-particles sit at the FP64
+ * centre of mass of their members' given positions with mass ppm; the law is
+ * the context's polynomial in r2 in double (the Step16 law: r2 < fsrrmax^2,
+ * pairs at r2 = 0 skipped only when rsm = 0).  hsr_adjoint_f64_at gives
+ * g = d(sum_i lambda_i . F_i)/dx including the pseudo-particle centres.
+ * Device pointers only, arrays of stats.count doubles; a single monopole
+ * tree (hsr_build, TDPTS 1). */
+hsr_status hsr_forces_f64_at(hsr_ctx *ctx, const double *x, const double *y,
+                             const double *z, double *ax, double *ay, double *az);
+hsr_status hsr_adjoint_f64_at(hsr_ctx *ctx, const double *x, const double *y,
+                              const double *z, const double *lx, const double *ly,
+                              const double *lz, double *gx, double *gy, double *gz);
+
 /* ------------------------------------------------------------ sub-cycle */
 /* The drift of the sub-cycle, Particles::map1(pp, tau, adot)
  * (Particles.cxx:732-758): x += prefactor * tau * v over `count` particles
