@@ -329,7 +329,10 @@ k_adj_reduce(const hsr_node *cn, const AItem *items, const int64_t *first,
 
 // ------------------------------------------------------------- fused pass
 constexpr int FR = 2;                       // targets per lane
-constexpr int FTILE = 256;                  // entries per LDS tile
+#ifndef HSR_FUSED_TILE
+#define HSR_FUSED_TILE 256
+#endif
+constexpr int FTILE = HSR_FUSED_TILE;       // entries per LDS tile
 #ifndef HSR_FUSED_UNROLL
 #define HSR_FUSED_UNROLL 1
 #endif

@@ -85,22 +85,6 @@ hsr_status make_law(const hsr_params &p, ForceLawDev &L) {
   return HSR_OK;
 }
 
-__global__ void k_gather_f32(int64_t n, const int32_t *perm, const float *in,
-                             float *out) {
-  int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (k < n) out[k] = in[perm[k]];
-}
-__global__ void k_gather_i64(int64_t n, const int32_t *perm, const int64_t *in,
-                             int64_t *out) {
-  int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (k < n) out[k] = in[perm[k]];
-}
-__global__ void k_gather_u16(int64_t n, const int32_t *perm,
-                             const uint16_t *in, uint16_t *out) {
-  int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (k < n) out[k] = in[perm[k]];
-}
-
 inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 hsr_status check_ctx(hsr_ctx *c) {
@@ -116,30 +100,70 @@ hsr_status check_ctx(hsr_ctx *c) {
   return HSR_OK;
 }
 
+struct Gather6 {
+  const float *in[4];
+  float *out[4];
+  const int64_t *id_in;
+  int64_t *id_out;
+  const uint16_t *mask_in;
+  uint16_t *mask_out;
+};
+
+// out[k] = in[P[k]] for the present arrays of vx, vy, vz, phi, id, mask in
+// one pass over the permutation.
+__global__ void k_gather_rest(int64_t n, const int32_t *__restrict__ P, Gather6 g) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int64_t s = P[k];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+    if (g.in[a]) g.out[a][k] = g.in[a][s];
+  if (g.id_in) g.id_out[k] = g.id_in[s];
+  if (g.mask_in) g.mask_out[k] = g.mask_in[s];
+}
+
 // Permute the ten arrays (device memory) in place with the build's final
-// permutation.
+// permutation.  x, y, z and mass come from the sorted copies the build made
+// (the same gather); the other six are gathered in one pass into staging
+// and copied back.
 hsr_status permute_in_place(hsr_ctx *c, int64_t n, float *f[8], int64_t *id,
                             uint16_t *mask) {
   hipStream_t st = c->stream;
   const int32_t *P = dptr<int32_t>(c->tree.pfin);
-  HSR_HIP(dbuf_reserve(c->s.x1, sizeof(int64_t) * std::max<int64_t>(n, 1)));
-  float *tmp = dptr<float>(c->s.x1);
-  for (int k = 0; k < 8; ++k) {
-    if (!f[k]) continue;
-    k_gather_f32<<<nblk(n, 256), 256, 0, st>>>(n, P, f[k], tmp);
-    HSR_HIP(hipMemcpyAsync(f[k], tmp, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+  const float *sorted[4] = {dptr<float>(c->tree.xs), dptr<float>(c->tree.ys),
+                            dptr<float>(c->tree.zs), dptr<float>(c->tree.ms)};
+  const int own[4] = {0, 1, 2, 6};
+  for (int k = 0; k < 4; ++k)
+    HSR_HIP(hipMemcpyAsync(f[own[k]], sorted[k], sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+  const int rest[4] = {3, 4, 5, 7};
+  const size_t nn = (size_t)std::max<int64_t>(n, 1);
+  HSR_HIP(dbuf_reserve(c->s.x1, (sizeof(int64_t) + 4 * sizeof(float) + sizeof(uint16_t)) * nn + 256));
+  char *sp = (char *)c->s.x1.p;
+  Gather6 g{};
+  if (id) {  // 8-byte array first keeps every staging array aligned
+    g.id_in = id;
+    g.id_out = (int64_t *)sp;
+    sp += sizeof(int64_t) * nn;
   }
-  if (id) {
-    int64_t *t64 = dptr<int64_t>(c->s.x1);
-    k_gather_i64<<<nblk(n, 256), 256, 0, st>>>(n, P, id, t64);
-    HSR_HIP(hipMemcpyAsync(id, t64, sizeof(int64_t) * n, hipMemcpyDeviceToDevice, st));
-  }
+  for (int a = 0; a < 4; ++a)
+    if (f[rest[a]]) {
+      g.in[a] = f[rest[a]];
+      g.out[a] = (float *)sp;
+      sp += sizeof(float) * nn;
+    }
   if (mask) {
-    uint16_t *t16 = dptr<uint16_t>(c->s.x1);
-    k_gather_u16<<<nblk(n, 256), 256, 0, st>>>(n, P, mask, t16);
-    HSR_HIP(hipMemcpyAsync(mask, t16, sizeof(uint16_t) * n, hipMemcpyDeviceToDevice, st));
+    g.mask_in = mask;
+    g.mask_out = (uint16_t *)sp;
   }
-  HSR_HIP(hipGetLastError());
+  if (g.id_in || g.mask_in || g.in[0] || g.in[1] || g.in[2] || g.in[3]) {
+    k_gather_rest<<<nblk(n, 256), 256, 0, st>>>(n, P, g);
+    HSR_HIP(hipGetLastError());
+  }
+  for (int a = 0; a < 4; ++a)
+    if (g.in[a])
+      HSR_HIP(hipMemcpyAsync(f[rest[a]], g.out[a], sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+  if (id) HSR_HIP(hipMemcpyAsync(id, g.id_out, sizeof(int64_t) * n, hipMemcpyDeviceToDevice, st));
+  if (mask) HSR_HIP(hipMemcpyAsync(mask, g.mask_out, sizeof(uint16_t) * n, hipMemcpyDeviceToDevice, st));
   return HSR_OK;
 }
 

@@ -631,11 +631,7 @@ __global__ void k_classify(BNode *nodes, int64_t lb, int64_t le, int64_t nd,
       n.xc[0] = n.xc[1] = n.xc[2] = __int_as_float(0x7fc00000);
     }
     n.tdr = ppc * pptdr_dev(n.xmin, n.xmax, n.xc);
-    float ppm = 0.f;
-    if (n.count > 1) {
-      for (int64_t i = 0; i < n.count; ++i) ppm += M[n.offset + i] * 1.0f;
-    }
-    n.ppm = ppm;
+    n.ppm = 0.f;  // leaves with more than one member: k_leaf_ppm
     return;
   }
   n.kind = NODE_SPLIT;
@@ -647,6 +643,70 @@ __global__ void k_classify(BNode *nodes, int64_t lb, int64_t le, int64_t nd,
   else if (xl1 > xl2) d = 1;
   else d = 2;
   n.split_d = d;
+}
+
+// Lowest set bit of a finite nonzero float, as a power-of-two exponent.
+__device__ __forceinline__ int f32_lsb_exp(uint32_t b) {
+  const int e = (int)((b >> 23) & 0xffu);
+  const uint32_t mant = b & 0x7fffffu;
+  return e == 0 ? -149 + __builtin_ctz(mant) : e - 150 + __builtin_ctz(mant | 0x800000u);
+}
+
+// Leaf pseudo mass (pp with TDPTS = 1: the members' masses times
+// (1 + 0 + 0), summed in member order in float; RCBForceTree.cxx:536-569,
+// 787-797), one wave per leaf with more than one member.  When every mass is
+// a multiple of the finest lsb u among them and their magnitudes sum below
+// 2^24 u, every partial sum of the sequential order is exact, so that sum is
+// the exact integer sum in units of u, which the wave forms in any order
+// (unit masses always take this path).  Otherwise lane 0 adds them in order.
+__global__ void __launch_bounds__(64)
+k_leaf_ppm(BNode *nodes, int64_t lb, int64_t le, int64_t nd, const float *M) {
+  const int l = lane_id();
+  for (int64_t t = lb + blockIdx.x; t < le; t += gridDim.x) {
+    BNode &n = nodes[t];
+    const int64_t cnt = n.count, off = n.offset;
+    if (cnt > nd || cnt <= 1) continue;
+    int emin = 1 << 20;
+    bool bad = false;
+    for (int64_t i = l; i < cnt; i += 64) {
+      const float m = M[off + i] * 1.0f;
+      const uint32_t b = __float_as_uint(m);
+      if (((b >> 23) & 0xffu) == 0xffu) bad = true;  // inf or nan
+      else if ((b & 0x7fffffffu) != 0u) emin = min(emin, f32_lsb_exp(b));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) emin = min(emin, __shfl_xor(emin, o));
+    int64_t sum = 0;
+    uint64_t mag = 0;
+    for (int64_t i = l; i < cnt && !bad; i += 64) {
+      const uint32_t b = __float_as_uint(M[off + i] * 1.0f);
+      if ((b & 0x7fffffffu) == 0u) continue;
+      const int e = (int)((b >> 23) & 0xffu);
+      const uint32_t mant = e == 0 ? (b & 0x7fffffu) : ((b & 0x7fffffu) | 0x800000u);
+      const int sh = f32_lsb_exp(b) - emin;  // |m| = odd * 2^(lsb), odd = mant >> ctz
+      if (sh > 32) {
+        bad = true;
+        break;
+      }
+      const uint64_t v = (uint64_t)(mant >> __builtin_ctz(mant)) << sh;
+      mag += v;
+      sum += (b >> 31) ? -(int64_t)v : (int64_t)v;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      sum += __shfl_xor(sum, o);
+      mag += __shfl_xor(mag, o);
+      bad = __shfl_xor((int)bad, o) || bad;
+    }
+    if (l != 0) continue;
+    if (!bad && mag < (1ull << 24)) {
+      n.ppm = sum == 0 ? 0.f : ldexpf((float)sum, emin);
+    } else {
+      float ppm = 0.f;
+      for (int64_t i = 0; i < cnt; ++i) ppm += M[off + i] * 1.0f;
+      n.ppm = ppm;
+    }
+  }
 }
 
 // Copy the final permutation of nodes that stop here (leaves at classify
@@ -1328,6 +1388,7 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     k_cm_finish<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, cms);
     // ---- classify, finalize plain leaves
     k_classify<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, nd, c->p.ppc, Mm[cur]);
+    k_leaf_ppm<<<gnl, 64, 0, st>>>(nodes, lb, le, nd, Mm[cur]);
     k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_LEAF, P[cur], Pfin);
     // ---- partition of splitting nodes
     int64_t pcap = active / PB + nl + 2;
