@@ -133,8 +133,10 @@ hsr_status permute_in_place(hsr_ctx *c, int64_t n, float *f[8], int64_t *id,
   const float *sorted[4] = {dptr<float>(c->tree.xs), dptr<float>(c->tree.ys),
                             dptr<float>(c->tree.zs), dptr<float>(c->tree.ms)};
   const int own[4] = {0, 1, 2, 6};
-  for (int k = 0; k < 4; ++k)
+  for (int k = 0; k < 4; ++k) {
+    if (k == 3 && c->tree.unit) continue;  // all ones: any permutation leaves them
     HSR_HIP(hipMemcpyAsync(f[own[k]], sorted[k], sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+  }
   const int rest[4] = {3, 4, 5, 7};
   const size_t nn = (size_t)std::max<int64_t>(n, 1);
   HSR_HIP(dbuf_reserve(c->s.x1, (sizeof(int64_t) + 4 * sizeof(float) + sizeof(uint16_t)) * nn + 256));

@@ -112,6 +112,7 @@ struct TreeState {
   DBuf pj;                    // PJ[n + tdpts * nnodes]
   DBuf ppm12;                 // float[12 * nnodes]: TDPTS = 12 pseudo masses
   DBuf xs, ys, zs, ms;        // sorted float arrays
+  bool unit = false;          // every mass of the last build was exactly 1.0f
   int64_t nboxes = 1;
   std::vector<int64_t> box_root;  // BFS id of each box root
 };

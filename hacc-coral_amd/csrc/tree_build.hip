@@ -59,17 +59,21 @@ __device__ __forceinline__ const float *coord(int d, const float *X,
 }
 
 // ------------------------------------------------------------ cm (small)
-__device__ __forceinline__ float addend(int q, const float *X, const float *Y,
+// With unit masses (every mass exactly 1.0f) the addend w * x is x exactly,
+// the mass sum is the count, and M is not read (the partition then stops
+// moving masses, so M may hold stale values).
+__device__ __forceinline__ float addend(int q, bool unit, const float *X, const float *Y,
                                         const float *Z, const float *M,
                                         int64_t e) {
-  float w = M[e];
-  return q == 3 ? w : w * (q == 0 ? X[e] : (q == 1 ? Y[e] : Z[e]));
+  if (q == 3) return unit ? 1.f : M[e];
+  const float c = q == 0 ? X[e] : (q == 1 ? Y[e] : Z[e]);
+  return unit ? c : M[e] * c;
 }
 
 // buf[k] = addend q of element off + k for k < n, U loads in flight per lane
 // (the staging is latency-bound otherwise).
 template <int U>
-__device__ __forceinline__ void stage_addends(float *buf, int q, const float *X, const float *Y,
+__device__ __forceinline__ void stage_addends(float *buf, int q, bool unit, const float *X, const float *Y,
                                               const float *Z, const float *M, int64_t off,
                                               int64_t n) {
   const int l = lane_id();
@@ -78,7 +82,7 @@ __device__ __forceinline__ void stage_addends(float *buf, int q, const float *X,
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int64_t k = k0 + u * 64 + l;
-      v[u] = k < n ? addend(q, X, Y, Z, M, off + k) : 0.f;
+      v[u] = k < n ? addend(q, unit, X, Y, Z, M, off + k) : 0.f;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -97,7 +101,7 @@ __device__ __forceinline__ void stage_addends(float *buf, int q, const float *X,
 // cm_list for k_cm_small_ordered.  k_cm_finish divides.
 constexpr int SM_PER = CM_SMALL / 64;
 __global__ void __launch_bounds__(64)
-k_cm_small(BNode *nodes, int64_t lb, int64_t le, const float *X,
+k_cm_small(BNode *nodes, int64_t lb, int64_t le, bool unit, const float *X,
            const float *Y, const float *Z, const float *M, sd_t *cms,
            int32_t *cm_list, Counters *ctr) {
   const int64_t t = lb + blockIdx.x;
@@ -146,7 +150,7 @@ k_cm_small(BNode *nodes, int64_t lb, int64_t le, const float *X,
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const int64_t k = (i0 + i) * 64 + l;
-      v[i] = k < cnt ? addend(q, X, Y, Z, M, off + k) : 0.f;
+      v[i] = k < cnt ? addend(q, unit, X, Y, Z, M, off + k) : 0.f;
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) tl = min(tl, sd_lsb_f32(v[i]));
@@ -159,7 +163,7 @@ k_cm_small(BNode *nodes, int64_t lb, int64_t le, const float *X,
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const int64_t k = (i0 + i) * 64 + l;
-      v[i] = k < cnt ? addend(q, X, Y, Z, M, off + k) : 0.f;
+      v[i] = k < cnt ? addend(q, unit, X, Y, Z, M, off + k) : 0.f;
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
@@ -184,7 +188,7 @@ k_cm_small(BNode *nodes, int64_t lb, int64_t le, const float *X,
 // The small-node sums that round: the addends staged in LDS in order and
 // added by sd_block (exact sequential double additions), a wave each.
 __global__ void __launch_bounds__(64)
-k_cm_small_ordered(const BNode *nodes, int64_t lb, const float *X, const float *Y,
+k_cm_small_ordered(const BNode *nodes, int64_t lb, bool unit, const float *X, const float *Y,
                    const float *Z, const float *M, sd_t *cms, const int32_t *cm_list,
                    const Counters *ctr) {
   __shared__ float buf[CM_SMALL];
@@ -196,7 +200,7 @@ k_cm_small_ordered(const BNode *nodes, int64_t lb, const float *X, const float *
     const int64_t cnt = nodes[t].count, off = nodes[t].offset;
     sd_t S = sd_zero();
     __syncthreads();  // the previous sum has read buf
-    stage_addends<8>(buf, q, X, Y, Z, M, off, cnt);
+    stage_addends<8>(buf, q, unit, X, Y, Z, M, off, cnt);
     __syncthreads();
     sd_block(S, buf, (int)cnt);
     if (l == 0) cms[item] = S;
@@ -205,12 +209,16 @@ k_cm_small_ordered(const BNode *nodes, int64_t lb, const float *X, const float *
 
 // xc = (float)(sum w x / sum w) per dimension for every non-empty node of
 // the level, from the four exact sums (BGQCM.c:208-210).
-__global__ void k_cm_finish(BNode *nodes, int64_t lb, int64_t le, const sd_t *cms) {
+__global__ void k_cm_finish(BNode *nodes, int64_t lb, int64_t le, bool unit, const sd_t *cms) {
   const int64_t t = lb + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= le) return;
-  if (nodes[t].count == 0) return;
+  const int64_t cnt = nodes[t].count;
+  if (cnt == 0) return;
   const sd_t *S = cms + 4 * (t - lb);
-  for (int d = 0; d < 3; ++d) nodes[t].xc[d] = sd_div_to_f32(S[d], S[3]);
+  // unit masses: the mass sum is the count, exactly (in units of 2^0, as the
+  // integer-sum path forms it from addends 1.0)
+  const sd_t Sm = unit ? sd_from_units((uint64_t)cnt, 0) : S[3];
+  for (int d = 0; d < 3; ++d) nodes[t].xc[d] = sd_div_to_f32(S[d], Sm);
 }
 
 // ------------------------------------------------------------ cm (big)
@@ -243,10 +251,14 @@ k_cm_plan(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk,
   }
 }
 
-// Approximate sums and bbox per block (one wave per block).
+// Approximate sums and bbox per block (one wave per block).  With unit
+// masses (UNIT) the mass sum's record is written directly: approximate sum
+// and integer sum len in units of 2^0.
+template <bool UNIT>
 __global__ void __launch_bounds__(64)
 k_cm_blk_stats(CmBlock *blk, const Counters *ctr, const float *X,
                const float *Y, const float *Z, const float *M) {
+  constexpr int NQ = UNIT ? 3 : 4;  // sums formed from the addends
   const int l = lane_id();
   for (int64_t b = blockIdx.x; b < ctr->cm_blocks; b += gridDim.x) {
     CmBlock &B = blk[b];
@@ -258,7 +270,7 @@ k_cm_blk_stats(CmBlock *blk, const Counters *ctr, const float *X,
     for (int i = 0; i < CM_PER; ++i) {
       const int k = i * 64 + l;
       const int64_t e = start + (k < len ? k : 0);
-      wv[i] = M[e];
+      wv[i] = UNIT ? 1.f : M[e];
       xv[i] = X[e];
       yv[i] = Y[e];
       zv[i] = Z[e];
@@ -274,14 +286,16 @@ k_cm_blk_stats(CmBlock *blk, const Counters *ctr, const float *X,
       mn[1] = fminf(mn[1], yv[i]); mx[1] = fmaxf(mx[1], yv[i]);
       mn[2] = fminf(mn[2], zv[i]); mx[2] = fmaxf(mx[2], zv[i]);
       const float w = wv[i];
-      const float a[4] = {w * xv[i], w * yv[i], w * zv[i], w};  // the addends of BGQCM.c
+      // the addends of BGQCM.c (w * x is x exactly when w = 1)
+      const float a[4] = {UNIT ? xv[i] : w * xv[i], UNIT ? yv[i] : w * yv[i],
+                          UNIT ? zv[i] : w * zv[i], w};
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < NQ; ++q) {
         s[q] += a[q];
         lsb[q] = a[q] < 0.f ? INT32_MIN : min(lsb[q], sd_lsb_f32(a[q]));
       }
     }
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NQ; ++q) {
       s[q] = wave_sum_f32(s[q]);
       lsb[q] = wave_min_i32(lsb[q]);
     }
@@ -293,18 +307,26 @@ k_cm_blk_stats(CmBlock *blk, const Counters *ctr, const float *X,
     for (int i = 0; i < CM_PER; ++i) {
       if (i * 64 + l >= len) continue;
       const float w = wv[i];
-      const float a[4] = {w * xv[i], w * yv[i], w * zv[i], w};
+      const float a[4] = {UNIT ? xv[i] : w * xv[i], UNIT ? yv[i] : w * yv[i],
+                          UNIT ? zv[i] : w * zv[i], w};
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < NQ; ++q) {
         if (lsb[q] == INT32_MIN) continue;
         const uint64_t uu = sd_units_f32(a[q], lsb[q]);
         if (uu == ~0ull) bad[q] = 1;
         else un[q] += uu;
       }
     }
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NQ; ++q) {
       bad[q] = __any(bad[q]) || lsb[q] == INT32_MIN;
       un[q] = wave_sum_u64(un[q]);
+    }
+    if (UNIT) {
+      // len ones: the float sum, the lsb of 1.0 and the count are exact
+      s[3] = (float)len;
+      lsb[3] = 0;
+      un[3] = (uint64_t)len;
+      bad[3] = 0;
     }
     for (int d = 0; d < 3; ++d) {
       mn[d] = wave_min_f32(mn[d]);
@@ -324,7 +346,7 @@ k_cm_blk_stats(CmBlock *blk, const Counters *ctr, const float *X,
   }
 }
 
-// Per big node (one workgroup of 4 waves): exclusive prefix of the
+// Per big node (one workgroup of 16 waves): exclusive prefix of the
 // approximate block sums (only a guess of the running sum's binade, so any
 // summation order serves), the node bbox (min/max: order-free), and the
 // integer-sum path.  With t the finest lsb of a sum's addends over the node,
@@ -341,13 +363,19 @@ __device__ __forceinline__ uint64_t units_add_sat(uint64_t s, uint64_t ub, int32
   return v >= SD_M53 ? SD_M53 : v;
 }
 
-__global__ void __launch_bounds__(256)
+constexpr int PFX_WAVES = 16;  // waves per workgroup of the per-node block passes
+constexpr int PFX_T = 64 * PFX_WAVES;
+
+__global__ void __launch_bounds__(PFX_T)
 k_cm_blk_prefix(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk, int32_t *cmflags,
                 sd_t *cms) {
-  __shared__ float wtot[4][4];
-  __shared__ float wmn[4][3], wmx[4][3];
-  __shared__ int32_t wlsb[4][4];
-  __shared__ uint64_t wun[4][4];
+  // a whole workgroup per big node: the top levels hold one to a few nodes
+  // of up to ~18k blocks each, so the chunk loop below is the critical path
+  constexpr int NW = PFX_WAVES;
+  __shared__ float wtot[NW][4];
+  __shared__ float wmn[NW][3], wmx[NW][3];
+  __shared__ int32_t wlsb[NW][4];
+  __shared__ uint64_t wun[NW][4];
   const int l = lane_id(), w = threadIdx.x >> 6;
   for (int64_t t = lb + blockIdx.x; t < le; t += gridDim.x) {
     if (nodes[t].count <= CM_SMALL) continue;
@@ -356,7 +384,7 @@ k_cm_blk_prefix(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk, int32_t *cmf
     float mn[3] = {INFINITY, INFINITY, INFINITY};
     float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     int32_t lsb[4] = {1 << 20, 1 << 20, 1 << 20, 1 << 20};
-    for (int64_t c0 = 0; c0 < nb; c0 += 256) {
+    for (int64_t c0 = 0; c0 < nb; c0 += PFX_T) {
       const int64_t b = b0 + c0 + threadIdx.x;
       const bool v = c0 + threadIdx.x < nb;
       float a[4], inc[4];
@@ -369,20 +397,6 @@ k_cm_blk_prefix(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk, int32_t *cmf
           if (l >= o) inc[q] += y;
         }
       }
-      if (l == 63)
-        for (int q = 0; q < 4; ++q) wtot[w][q] = inc[q];
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        float before = 0.f, all = 0.f;
-        for (int k = 0; k < 4; ++k) {
-          if (k < w) before += wtot[k][q];
-          all += wtot[k][q];
-        }
-        if (v) blk[b].ein[q] = base[q] + before + (inc[q] - a[q]);
-        base[q] += all;
-      }
-      __syncthreads();
       if (v) {
         for (int d = 0; d < 3; ++d) {
           mn[d] = fminf(mn[d], blk[b].bmin[d]);
@@ -390,6 +404,20 @@ k_cm_blk_prefix(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk, int32_t *cmf
         }
         for (int q = 0; q < 4; ++q) lsb[q] = min(lsb[q], blk[b].lsb[q]);
       }
+      if (l == 63)
+        for (int q = 0; q < 4; ++q) wtot[w][q] = inc[q];
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        float before = 0.f, all = 0.f;
+        for (int k = 0; k < NW; ++k) {
+          if (k < w) before += wtot[k][q];
+          all += wtot[k][q];
+        }
+        if (v) blk[b].ein[q] = base[q] + before + (inc[q] - a[q]);
+        base[q] += all;
+      }
+      __syncthreads();
     }
     for (int d = 0; d < 3; ++d) {
       mn[d] = wave_min_f32(mn[d]);
@@ -407,11 +435,11 @@ k_cm_blk_prefix(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk, int32_t *cmf
     int32_t tq[4];
     for (int q = 0; q < 4; ++q) {
       tq[q] = wlsb[0][q];
-      for (int k = 1; k < 4; ++k) tq[q] = min(tq[q], wlsb[k][q]);
+      for (int k = 1; k < NW; ++k) tq[q] = min(tq[q], wlsb[k][q]);
     }
     // the node's integer sums: block sums rescaled to 2^tq, saturating at 2^53
     uint64_t su[4] = {0, 0, 0, 0};
-    for (int64_t c = threadIdx.x; c < nb; c += 256) {
+    for (int64_t c = threadIdx.x; c < nb; c += PFX_T) {
       const CmBlock &B = blk[b0 + c];
       for (int q = 0; q < 4; ++q)
         if (tq[q] != INT32_MIN) su[q] = units_add_sat(su[q], B.units[q], B.lsb[q], tq[q]);
@@ -423,7 +451,7 @@ k_cm_blk_prefix(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk, int32_t *cmf
     if (threadIdx.x == 0) {
       for (int d = 0; d < 3; ++d) {
         float a = wmn[0][d], b = wmx[0][d];
-        for (int k = 1; k < 4; ++k) {
+        for (int k = 1; k < NW; ++k) {
           a = fminf(a, wmn[k][d]);
           b = fmaxf(b, wmx[k][d]);
         }
@@ -432,7 +460,9 @@ k_cm_blk_prefix(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk, int32_t *cmf
       }
       int32_t f = 0;
       for (int q = 0; q < 4; ++q) {
-        const uint64_t tot = wun[0][q] + wun[1][q] + wun[2][q] + wun[3][q];
+        // NW wave sums of 64 saturated values each, below 2^59: no wrap
+        uint64_t tot = 0;
+        for (int k = 0; k < NW; ++k) tot += wun[k][q];
         // a negative addend, all addends zero, or a total that rounds
         if (tq[q] == INT32_MIN || tq[q] == (1 << 20) || tot >= SD_M53) continue;
         cms[4 * (t - lb) + q] = sd_from_units(tot, tq[q]);
@@ -440,6 +470,7 @@ k_cm_blk_prefix(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk, int32_t *cmf
       }
       cmflags[t - lb] = f;
     }
+    __syncthreads();  // the shared totals are read before the next node
   }
 }
 
@@ -455,7 +486,7 @@ __device__ __forceinline__ int32_t grid_exp_guess(float ein) {
 // in LDS, each lane composes CM_PER consecutive addends in order, then the 64
 // lane maps are composed in lane order.
 __global__ void __launch_bounds__(64)
-k_cm_blk_maps(CmBlock *blk, const Counters *ctr, const float *X,
+k_cm_blk_maps(CmBlock *blk, const Counters *ctr, bool unit, const float *X,
               const float *Y, const float *Z, const float *M, int64_t lb,
               const int32_t *cmflags) {
   __shared__ float buf[64 * (CM_PER + 1)];
@@ -473,7 +504,7 @@ k_cm_blk_maps(CmBlock *blk, const Counters *ctr, const float *X,
       __syncthreads();  // the previous block's maps have read buf
 #pragma unroll
       for (int k = l; k < CM_BLK; k += 64) {
-        float v = k < len ? addend(q, X, Y, Z, M, start + k) : 0.f;
+        float v = k < len ? addend(q, unit, X, Y, Z, M, start + k) : 0.f;
         buf[(k / CM_PER) * (CM_PER + 1) + (k % CM_PER)] = v;
       }
       __syncthreads();
@@ -531,7 +562,7 @@ k_cm_win_maps(const BNode *nodes, CmBlock *blk, const Counters *ctr) {
 // apply (binade guess wrong or binade crossed) is staged in LDS and
 // re-accumulated chunk by chunk.
 __global__ void __launch_bounds__(64)
-k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, const CmBlock *blk,
+k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, bool unit, const CmBlock *blk,
                const float *X, const float *Y, const float *Z,
                const float *M, Counters *ctr, sd_t *cms, const int32_t *cmflags) {
   __shared__ float buf[CM_BLK];
@@ -542,15 +573,30 @@ k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, const CmBlock *blk,
     if (cmflags[t - lb] & (1 << q)) continue;  // summed exactly
     const int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
     sd_t Sq = sd_zero();
+    // the window maps of 64 windows at once, one per lane (the loop over the
+    // windows is serial; it must not wait on a load per window)
+    uint64_t wd0 = 0, wd1 = 0;
+    int32_t wok = 0, wE = 0;
     for (int64_t bb = 0; bb < nb; bb += 64) {
+      const int wi = (int)((bb >> 6) & 63);
+      if (wi == 0) {
+        const int64_t lb0 = bb + 64 * (int64_t)l;  // leader block of lane l's window
+        wok = 0;
+        if (lb0 < nb) {
+          const CmBlock &W = blk[b0 + lb0];
+          wd0 = W.wd0[q];
+          wd1 = W.wd1[q];
+          wok = W.wok[q];
+          wE = W.wE[q];
+        }
+      }
       // the whole window at once when its blocks share a grid and S stays in it
       {
-        const CmBlock &W = blk[b0 + bb];
         sd_map g;
-        g.d0 = W.wd0[q];
-        g.d1 = W.wd1[q];
-        g.bad = !W.wok[q];
-        if (sd_map_apply(&Sq, g, W.wE[q])) continue;
+        g.d0 = readlane_u64(wd0, wi);
+        g.d1 = readlane_u64(wd1, wi);
+        g.bad = !readlane_i32(wok, wi);
+        if (sd_map_apply(&Sq, g, readlane_i32(wE, wi))) continue;
       }
       uint64_t md0 = 0, md1 = 0;
       int mbad = 1, mE = 0;
@@ -603,7 +649,7 @@ k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, const CmBlock *blk,
           const CmBlock &B = blk[b0 + bb + i];
           const int64_t start = B.start, len = B.len;
           __syncthreads();
-          stage_addends<8>(buf, q, X, Y, Z, M, start, len);
+          stage_addends<8>(buf, q, unit, X, Y, Z, M, start, len);
           __syncthreads();
           sd_block(Sq, buf, (int)len);
           ++i;
@@ -620,7 +666,7 @@ k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, const CmBlock *blk,
 // the split dimension (longest edge, ties x > y > z as coded at
 // RCBForceTree.cxx:843-852).
 __global__ void k_classify(BNode *nodes, int64_t lb, int64_t le, int64_t nd,
-                           float ppc, const float *M) {
+                           float ppc, bool unit_ppm) {
   int64_t t = lb + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= le) return;
   BNode &n = nodes[t];
@@ -631,7 +677,10 @@ __global__ void k_classify(BNode *nodes, int64_t lb, int64_t le, int64_t nd,
       n.xc[0] = n.xc[1] = n.xc[2] = __int_as_float(0x7fc00000);
     }
     n.tdr = ppc * pptdr_dev(n.xmin, n.xmax, n.xc);
-    n.ppm = 0.f;  // leaves with more than one member: k_leaf_ppm
+    // leaves with more than one member: k_leaf_ppm, or with unit masses
+    // (unit_ppm) the in-order float sum of count ones: the count, exact up
+    // to 2^24, where adding 1 rounds back to 2^24 (ties to even)
+    n.ppm = (unit_ppm && n.count > 1) ? (n.count < (1 << 24) ? (float)n.count : 16777216.f) : 0.f;
     return;
   }
   n.kind = NODE_SPLIT;
@@ -781,28 +830,38 @@ k_part_count(const BNode *nodes, PartBlock *pb, const Counters *ctr,
   }
 }
 
-// In-order exclusive scan of block counts per node; degenerate splits
-// (is == 0 or is == count, RCBForceTree.cxx:727-729) are marked.
-__global__ void __launch_bounds__(64)
+// In-order exclusive scan of block counts per node, a workgroup per node;
+// degenerate splits (is == 0 or is == count, RCBForceTree.cxx:727-729) are
+// marked.
+__global__ void __launch_bounds__(PFX_T)
 k_part_scan(BNode *nodes, int64_t lb, int64_t le, PartBlock *pb) {
+  constexpr int NW = PFX_WAVES;
+  __shared__ int64_t wsum[NW];
+  const int l = lane_id(), w = threadIdx.x >> 6;
   for (int64_t t = lb + blockIdx.x; t < le; t += gridDim.x) {
-    if (nodes[t].kind != NODE_SPLIT) continue;
-    int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
+    if (nodes[t].kind != NODE_SPLIT) continue;  // uniform over the workgroup
+    const int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
     int64_t run = 0;
-    int l = lane_id();
-    for (int64_t c0 = 0; c0 < nb; c0 += 64) {
-      int64_t b = c0 + l;
-      int64_t v = b < nb ? pb[b0 + b].cnt : 0;
-      // inclusive scan
-      int64_t s = v;
+    for (int64_t c0 = 0; c0 < nb; c0 += PFX_T) {
+      const int64_t b = c0 + threadIdx.x;
+      const int64_t v = b < nb ? pb[b0 + b].cnt : 0;
+      int64_t s = v;  // inclusive scan over the wave
       for (int o = 1; o < 64; o <<= 1) {
-        int64_t u = (int64_t)__shfl_up((long long)s, o);
+        const int64_t u = (int64_t)__shfl_up((long long)s, o);
         if (l >= o) s += u;
       }
-      if (b < nb) pb[b0 + b].base = run + s - v;
-      run += (int64_t)__shfl((long long)s, 63);
+      if (l == 63) wsum[w] = s;
+      __syncthreads();
+      int64_t before = 0, all = 0;
+      for (int k = 0; k < NW; ++k) {
+        if (k < w) before += wsum[k];
+        all += wsum[k];
+      }
+      if (b < nb) pb[b0 + b].base = run + before + s - v;
+      run += all;
+      __syncthreads();  // wsum is rewritten by the next chunk
     }
-    if (l == 0) {
+    if (threadIdx.x == 0) {
       nodes[t].is = run;
       if (run == 0 || run == nodes[t].count) nodes[t].kind = NODE_DEGEN;
     }
@@ -869,28 +928,35 @@ k_part_chain(const BNode *nodes, const PartBlock *pb, const Counters *ctrc,
   for (int64_t b = blockIdx.x; b < ctrc->part_blocks; b += gridDim.x) {
     const PartBlock &B = pb[b];
     const BNode &n = nodes[B.node];
-    const int64_t off = n.offset, is = n.is, cnt = n.count, nr = cnt - is;
+    // positions within a node are below 2^31 (int32 permutation), so the
+    // stretch arithmetic runs in 32 bits (a 64-bit division is a long
+    // software sequence)
+    const int64_t off = n.offset;
+    const uint32_t is = (uint32_t)n.is, cnt = (uint32_t)n.count, nr = cnt - is;
+    const int32_t *idx_n = idx + off;
+    const int32_t *rights_n = rights + off;
     for (int k = threadIdx.x; k < B.len; k += 256) {
       int64_t e = B.start + k;
       if (dest[e] != -1) continue;
-      int64_t j = e - off;
+      uint32_t j = (uint32_t)(e - off);
       bool done = false;
       for (int it = 0; it < MAXHOP; ++it) {
-        const int64_t f = idx[off + j];  // f > j
+        const uint32_t f = (uint32_t)idx_n[j];  // f > j
         if (f >= is) {
           dest[e] = (int32_t)f;
           done = true;
           break;
         }
-        const int64_t step = f - j;  // keys >= pivot below f
-        const int64_t lim = step < nr ? rights[off + step] : cnt;
-        const int64_t v = f + ((is - f + step - 1) / step) * step;
+        const uint32_t step = f - j;  // keys >= pivot below f
+        const uint32_t lim = step < nr ? (uint32_t)rights_n[step] : cnt;
+        // f < is < 2^31 and step < 2^31: v < 2^32
+        const uint32_t v = f + ((is - f + step - 1u) / step) * step;
         if (v < lim) {
           dest[e] = (int32_t)v;
           done = true;
           break;
         }
-        j = f + ((lim - 1 - f) / step) * step;
+        j = f + ((lim - 1u - f) / step) * step;
       }
       if (!done) {
         dest[e] = -2 - (int32_t)j;  // unresolved; continue from j
@@ -946,7 +1012,7 @@ k_part_jump_final(const BNode *nodes, const PartBlock *pb,
 }
 
 __global__ void __launch_bounds__(256)
-k_part_scatter(const BNode *nodes, const PartBlock *pb, const Counters *ctr,
+k_part_scatter(const BNode *nodes, const PartBlock *pb, const Counters *ctr, bool unit,
                const int32_t *dest, const float *X, const float *Y,
                const float *Z, const float *M, const int32_t *P, float *X2,
                float *Y2, float *Z2, float *M2, int32_t *P2) {
@@ -959,7 +1025,7 @@ k_part_scatter(const BNode *nodes, const PartBlock *pb, const Counters *ctr,
       X2[o] = X[e];
       Y2[o] = Y[e];
       Z2[o] = Z[e];
-      M2[o] = M[e];
+      if (!unit) M2[o] = M[e];  // unit masses are not moved (never read again)
       P2[o] = P[e];
     }
   }
@@ -1094,14 +1160,14 @@ __global__ void k_box_of(const BNode *nodes, int64_t nb, int32_t *box_of,
   box_of[t] = root_box[u];
 }
 
-__global__ void k_gather_sorted(int64_t n, const int32_t *Pfin,
+__global__ void k_gather_sorted(int64_t n, const int32_t *Pfin, bool unit,
                                 const float *x, const float *y, const float *z,
                                 const float *m, float *xs, float *ys,
                                 float *zs, float *ms, PJ *pj) {
   int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (k >= n) return;
   int32_t s = Pfin[k];
-  float a = x[s], b = y[s], c = z[s], d = m[s];
+  float a = x[s], b = y[s], c = z[s], d = unit ? 1.f : m[s];
   xs[k] = a;
   ys[k] = b;
   zs[k] = c;
@@ -1260,6 +1326,19 @@ __global__ void k_init_roots(BNode *nodes, int64_t nboxes,
   }
 }
 
+// flag[0] = 0 unless every mass is exactly 1.0f (flag[0] preset to 1 by
+// k_set_flag); every writer stores the same value.
+__global__ void k_set_flag(int32_t *flag) { flag[0] = 1; }
+
+__global__ void __launch_bounds__(256)
+k_unit_check(int64_t n, const float *m, int32_t *flag) {
+  bool ok = true;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    ok = ok && m[i] == 1.0f;
+  if (!__all(ok) && lane_id() == 0) flag[0] = 0;
+}
+
 __global__ void k_zero_counters(Counters *ctr) {
   int64_t *p = reinterpret_cast<int64_t *>(ctr);
   if (threadIdx.x < sizeof(Counters) / sizeof(int64_t)) p[threadIdx.x] = 0;
@@ -1304,6 +1383,7 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
   HSR_HIP(dbuf_reserve(S.rights, sizeof(int32_t) * nn));
   HSR_HIP(dbuf_reserve(S.counters, sizeof(Counters) + 64));
   HSR_HIP(dbuf_reserve(S.d, sizeof(int64_t) * (nboxes + 1) + sizeof(float) * 6 * nboxes + 64));
+  // (the 64 spare bytes after the box table hold the unit-mass flag)
 
   float *X[2] = {dptr<float>(S.x0), dptr<float>(S.x1)};
   float *Y[2] = {dptr<float>(S.y0), dptr<float>(S.y1)};
@@ -1317,22 +1397,36 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
   int32_t *rights = dptr<int32_t>(S.rights);
   Counters *ctr = dptr<Counters>(S.counters);
 
-  if (n > 0) {
-    HSR_HIP(hipMemcpyAsync(X[0], x, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
-    HSR_HIP(hipMemcpyAsync(Y[0], y, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
-    HSR_HIP(hipMemcpyAsync(Z[0], z, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
-    HSR_HIP(hipMemcpyAsync(Mm[0], m, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
-    k_iota<<<nblk_for(n, 256), 256, 0, st>>>(n, P[0]);
-  }
   // box table
   int64_t *d_box_off = dptr<int64_t>(S.d);
   float *d_min = reinterpret_cast<float *>(d_box_off + nboxes + 1);
   float *d_max = d_min + 3 * nboxes;
+  // Unit masses (map2 sets every mass to 1 before the tree build): the mass
+  // sums are the counts, the addends w x are x, and the partition need not
+  // move the masses.  Every result is the same as with the general path.
+  int32_t *d_unit = reinterpret_cast<int32_t *>(d_max + 3 * nboxes);
+  k_set_flag<<<1, 1, 0, st>>>(d_unit);
+  if (n > 0)
+    k_unit_check<<<(unsigned)std::min<int64_t>(nblk_for(n, 256), 4096), 256, 0, st>>>(n, m, d_unit);
+  int32_t unit_h = 0;
+  HSR_HIP(hipMemcpyAsync(&unit_h, d_unit, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HSR_HIP(stream_wait(st));
+  const bool unit = unit_h != 0;
+  T.unit = unit;
+  const bool unit_ppm = unit;  // leaf pseudo masses set by k_classify
+  if (n > 0) {
+    HSR_HIP(hipMemcpyAsync(X[0], x, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+    HSR_HIP(hipMemcpyAsync(Y[0], y, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+    HSR_HIP(hipMemcpyAsync(Z[0], z, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+    if (!unit) HSR_HIP(hipMemcpyAsync(Mm[0], m, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+    k_iota<<<nblk_for(n, 256), 256, 0, st>>>(n, P[0]);
+  }
   HSR_HIP(hipMemcpyAsync(d_box_off, box_off_h, sizeof(int64_t) * (nboxes + 1), hipMemcpyHostToDevice, st));
   HSR_HIP(hipMemcpyAsync(d_min, min_loc_h, sizeof(float) * 3 * nboxes, hipMemcpyHostToDevice, st));
   HSR_HIP(hipMemcpyAsync(d_max, max_loc_h, sizeof(float) * 3 * nboxes, hipMemcpyHostToDevice, st));
   BNode *nodes = dptr<BNode>(T.bnodes);
   k_init_roots<<<nblk_for(nboxes, 64), 64, 0, st>>>(nodes, nboxes, d_box_off, d_min, d_max, P[0], n);
+  const unsigned nq = unit ? 3u : 4u;  // sums formed from addends
 
   T.level_start.assign(1, 0);
   T.level_start.push_back(nboxes);
@@ -1370,25 +1464,28 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     const unsigned gnl = (unsigned)std::min<int64_t>(nl, gbig);
     HSR_HIP(dbuf_reserve(S.cmfail, sizeof(int32_t) * 4 * (size_t)nl));
     int32_t *cm_list = dptr<int32_t>(S.cmfail);
-    k_cm_small<<<dim3((unsigned)nl, 4), 64, 0, st>>>(nodes, lb, le, X[cur], Y[cur], Z[cur], Mm[cur],
-                                                    cms, cm_list, ctr);
+    k_cm_small<<<dim3((unsigned)nl, nq), 64, 0, st>>>(nodes, lb, le, unit, X[cur], Y[cur], Z[cur], Mm[cur],
+                                                     cms, cm_list, ctr);
     k_cm_small_ordered<<<std::min<unsigned>((unsigned)std::max<int64_t>(1, nl), 4 * gbig), 64, 0, st>>>(
-        nodes, lb, X[cur], Y[cur], Z[cur], Mm[cur], cms, cm_list, ctr);
+        nodes, lb, unit, X[cur], Y[cur], Z[cur], Mm[cur], cms, cm_list, ctr);
     k_cm_plan<<<gnl, 64, 0, st>>>(nodes, lb, le, cmb, ctr);
     unsigned cmg = (unsigned)std::max<int64_t>(1, active / CM_BLK + active / CM_SMALL + 1);
     cmg = std::min(cmg, gbig);
-    k_cm_blk_stats<<<cmg, 64, 0, st>>>(cmb, ctr, X[cur], Y[cur], Z[cur], Mm[cur]);
+    if (unit)
+      k_cm_blk_stats<true><<<cmg, 64, 0, st>>>(cmb, ctr, X[cur], Y[cur], Z[cur], Mm[cur]);
+    else
+      k_cm_blk_stats<false><<<cmg, 64, 0, st>>>(cmb, ctr, X[cur], Y[cur], Z[cur], Mm[cur]);
     HSR_HIP(dbuf_reserve(S.cmx, sizeof(int32_t) * (size_t)nl));
     int32_t *cmflags = dptr<int32_t>(S.cmx);
-    k_cm_blk_prefix<<<gnl, 256, 0, st>>>(nodes, lb, le, cmb, cmflags, cms);
-    k_cm_blk_maps<<<dim3(cmg, 4), 64, 0, st>>>(cmb, ctr, X[cur], Y[cur], Z[cur], Mm[cur], lb, cmflags);
+    k_cm_blk_prefix<<<gnl, PFX_T, 0, st>>>(nodes, lb, le, cmb, cmflags, cms);
+    k_cm_blk_maps<<<dim3(cmg, nq), 64, 0, st>>>(cmb, ctr, unit, X[cur], Y[cur], Z[cur], Mm[cur], lb, cmflags);
     k_cm_win_maps<<<cmg, 64, 0, st>>>(nodes, cmb, ctr);
-    k_cm_blk_chain<<<dim3(gnl, 4), 64, 0, st>>>(nodes, lb, le, cmb, X[cur], Y[cur], Z[cur],
-                                                        Mm[cur], ctr, cms, cmflags);
-    k_cm_finish<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, cms);
+    k_cm_blk_chain<<<dim3(gnl, nq), 64, 0, st>>>(nodes, lb, le, unit, cmb, X[cur], Y[cur], Z[cur],
+                                                 Mm[cur], ctr, cms, cmflags);
+    k_cm_finish<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, unit, cms);
     // ---- classify, finalize plain leaves
-    k_classify<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, nd, c->p.ppc, Mm[cur]);
-    k_leaf_ppm<<<gnl, 64, 0, st>>>(nodes, lb, le, nd, Mm[cur]);
+    k_classify<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, nd, c->p.ppc, unit_ppm);
+    if (!unit_ppm) k_leaf_ppm<<<gnl, 64, 0, st>>>(nodes, lb, le, nd, Mm[cur]);
     k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_LEAF, P[cur], Pfin);
     // ---- partition of splitting nodes
     int64_t pcap = active / PB + nl + 2;
@@ -1397,7 +1494,8 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     k_part_plan<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, pb, ctr);
     const unsigned pg = (unsigned)std::max<int64_t>(1, pcap);
     k_part_count<<<pg, 256, 0, st>>>(nodes, pb, ctr, X[cur], Y[cur], Z[cur]);
-    k_part_scan<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, pb);
+    k_part_scan<<<std::min<unsigned>((unsigned)nl, 4u * (unsigned)std::max(1, c->num_cus)), PFX_T, 0, st>>>(
+        nodes, lb, le, pb);
     // children depend only on the partition counts: create them now so a
     // single host synchronisation per level reads every counter
     k_make_children<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, le, ctr);
@@ -1411,7 +1509,7 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
         k_part_jump_round<<<pg, 256, 0, st>>>(nodes, pb, ctr, jmp);
       k_part_jump_final<<<pg, 256, 0, st>>>(nodes, pb, ctr, jmp, dest);
     }
-    k_part_scatter<<<pg, 256, 0, st>>>(nodes, pb, ctr, dest, X[cur], Y[cur], Z[cur], Mm[cur], P[cur],
+    k_part_scatter<<<pg, 256, 0, st>>>(nodes, pb, ctr, unit, dest, X[cur], Y[cur], Z[cur], Mm[cur], P[cur],
                                         X[cur ^ 1], Y[cur ^ 1], Z[cur ^ 1], Mm[cur ^ 1], P[cur ^ 1]);
     k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_DEGEN, P[cur ^ 1], Pfin);
     HSR_HIP(hipGetLastError());
@@ -1472,7 +1570,7 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
   const int tdpts = c->p.tdpts;
   HSR_HIP(dbuf_reserve(T.pj, sizeof(PJ) * (n + tdpts * T.nnodes + 1)));
   if (n > 0)
-    k_gather_sorted<<<nblk_for(n, 256), 256, 0, st>>>(n, Pfin, x, y, z, m, dptr<float>(T.xs), dptr<float>(T.ys),
+    k_gather_sorted<<<nblk_for(n, 256), 256, 0, st>>>(n, Pfin, unit, x, y, z, m, dptr<float>(T.xs), dptr<float>(T.ys),
                                                      dptr<float>(T.zs), dptr<float>(T.ms), dptr<PJ>(T.pj));
   if (tdpts == 12) {
     // quadrupole pseudo-particles, deepest BFS level first; the orphan
