@@ -52,6 +52,13 @@ struct CmBlock {
   int32_t wok[4];
   int32_t lsb[4];          // finest lsb exponent of the addends (INT32_MIN: a negative one)
   uint64_t units[4];       // their sum in units of 2^lsb (~0: not an integer there)
+  // the leader block also holds its window's reductions (a chunk of up to
+  // 64 blocks) and the exclusive prefix of the approximate sums over the
+  // node's chunks
+  float capprox[4], cbase[4];
+  float cbmin[3], cbmax[3];
+  int32_t clsb[4];         // finest lsb over the chunk (INT32_MIN: a negative addend)
+  uint64_t cunits[4];      // its sum in units of 2^clsb, saturated at 2^53
 };
 
 struct PartBlock {

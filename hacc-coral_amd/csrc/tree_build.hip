@@ -224,27 +224,51 @@ __global__ void k_cm_finish(BNode *nodes, int64_t lb, int64_t le, bool unit, con
 // ------------------------------------------------------------ cm (big)
 // Block table for nodes with count > CM_SMALL.
 __global__ void __launch_bounds__(64)
-k_cm_plan(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk,
-          Counters *ctr) {
-  for (int64_t t = lb + blockIdx.x; t < le; t += gridDim.x) {
-    int64_t cnt = nodes[t].count;
-    if (cnt <= CM_SMALL) {
-      if (lane_id() == 0) nodes[t].nblk = 0;
-      continue;
-    }
-    int64_t nb = (cnt + CM_BLK - 1) / CM_BLK;
-    int64_t b0 = 0;
-    if (lane_id() == 0) {
-      b0 = (int64_t)atomicAdd((unsigned long long *)&ctr->cm_blocks,
-                              (unsigned long long)nb);
-      nodes[t].blk0 = b0;
-      nodes[t].nblk = nb;
-    }
-    b0 = (int64_t)__shfl((long long)b0, 0);
-    int64_t off = nodes[t].offset;
-    for (int64_t b = lane_id(); b < nb; b += 64) {
+k_cm_plan(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk, Counters *ctr) {
+  // one lane per node: a wave scan of the block counts and one atomic per
+  // wave reserve the blocks; then the wave fills them node by node
+  const int l = lane_id();
+  const int64_t t0 = lb + (int64_t)blockIdx.x * 64;
+  const int64_t t = t0 + l;
+  int64_t nb = 0;
+  if (t < le) {
+    const int64_t cnt = nodes[t].count;
+    if (cnt > CM_SMALL) nb = (cnt + CM_BLK - 1) / CM_BLK;
+    else nodes[t].nblk = 0;
+  }
+  int64_t inc = nb;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int64_t y = (int64_t)__shfl_up((long long)inc, o);
+    if (l >= o) inc += y;
+  }
+  int64_t base = 0;
+  if (l == 63 && inc > 0)
+    base = (int64_t)atomicAdd((unsigned long long *)&ctr->cm_blocks, (unsigned long long)inc);
+  base = (int64_t)__shfl((long long)base, 63);
+  const int64_t b0 = base + inc - nb;
+  if (nb > 0) {
+    nodes[t].blk0 = b0;
+    nodes[t].nblk = nb;
+  }
+  // a few blocks by the node's own lane, longer tables by the whole wave
+  if (nb > 0 && nb <= 4) {
+    const int64_t off = nodes[t].offset, cnt = nodes[t].count;
+    for (int64_t b = 0; b < nb; ++b) {
       CmBlock &B = blk[b0 + b];
       B.node = t;
+      B.start = off + b * CM_BLK;
+      B.len = min<int64_t>(CM_BLK, cnt - b * CM_BLK);
+    }
+  }
+  for (uint64_t big = __ballot(nb > 4); big; big &= big - 1) {
+    const int j = __builtin_ctzll(big);
+    const int64_t nbj = (int64_t)__shfl((long long)nb, j);
+    const int64_t b0j = (int64_t)__shfl((long long)b0, j);
+    const int64_t tj = t0 + j;
+    const int64_t off = nodes[tj].offset, cnt = nodes[tj].count;
+    for (int64_t b = l; b < nbj; b += 64) {
+      CmBlock &B = blk[b0j + b];
+      B.node = tj;
       B.start = off + b * CM_BLK;
       B.len = min<int64_t>(CM_BLK, cnt - b * CM_BLK);
     }
@@ -346,14 +370,17 @@ k_cm_blk_stats(CmBlock *blk, const Counters *ctr, const float *X,
   }
 }
 
-// Per big node (one workgroup of 16 waves): exclusive prefix of the
-// approximate block sums (only a guess of the running sum's binade, so any
-// summation order serves), the node bbox (min/max: order-free), and the
-// integer-sum path.  With t the finest lsb of a sum's addends over the node,
-// every addend a nonnegative multiple of 2^t and their total below 2^(t+53),
-// no partial sum of the sequential double additions rounds (each is such a
-// multiple below 2^(t+53)), so the sum is that integer times 2^t: written to
-// cms and flagged done (bit q of cmflags) for the ordered path to skip.
+// Big nodes' reductions of their block records in two passes: a wave per
+// 64-block chunk, then a wave per node over its chunks (the top levels hold
+// one to a few nodes of up to ~18k blocks, so no pass runs a whole node on
+// one compute unit).  They give the exclusive prefix of the approximate
+// block sums (only a guess of the running sum's binade, so any summation
+// order serves), the node bbox (min/max: order-free), and the integer-sum
+// path: with t the finest lsb of a sum's addends over the node, every addend
+// a nonnegative multiple of 2^t and their total below 2^(t+53), no partial
+// sum of the sequential double additions rounds (each is such a multiple
+// below 2^(t+53)), so the sum is that integer times 2^t: written to cms and
+// flagged done (bit q of cmflags) for the ordered path to skip.
 __device__ __forceinline__ uint64_t units_add_sat(uint64_t s, uint64_t ub, int32_t tb, int32_t tq) {
   if (s >= SD_M53 || ub == ~0ull) return SD_M53;
   if (ub == 0) return s;
@@ -363,114 +390,202 @@ __device__ __forceinline__ uint64_t units_add_sat(uint64_t s, uint64_t ub, int32
   return v >= SD_M53 ? SD_M53 : v;
 }
 
-constexpr int PFX_WAVES = 16;  // waves per workgroup of the per-node block passes
+constexpr int PFX_WAVES = 16;  // waves per workgroup of the top levels' per-node scans
 constexpr int PFX_T = 64 * PFX_WAVES;
 
-__global__ void __launch_bounds__(PFX_T)
-k_cm_blk_prefix(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk, int32_t *cmflags,
-                sd_t *cms) {
-  // a whole workgroup per big node: the top levels hold one to a few nodes
-  // of up to ~18k blocks each, so the chunk loop below is the critical path
-  constexpr int NW = PFX_WAVES;
-  __shared__ float wtot[NW][4];
-  __shared__ float wmn[NW][3], wmx[NW][3];
-  __shared__ int32_t wlsb[NW][4];
-  __shared__ uint64_t wun[NW][4];
-  const int l = lane_id(), w = threadIdx.x >> 6;
+// One wave per chunk (the up to 64 blocks from a leader block, the first of
+// every 64 of a node with more than 64 blocks), a lane per block: the local exclusive prefix of the
+// approximate sums (into the block's ein; k_cm_blk_maps adds the chunk's
+// base), and the chunk's totals, bbox, finest lsb and integer sum in units
+// of that lsb (saturating at 2^53: a chunk at the bound leaves its node
+// above it in any finer unit), kept in the leader.
+__global__ void __launch_bounds__(64)
+k_cm_chunk(const BNode *nodes, CmBlock *blk, const Counters *ctr) {
+  const int l = lane_id();
+  const int64_t nblocks = ctr->cm_blocks;
+  // a wave per 64-block range of the table, lane l testing block r + l; the
+  // range's leaders are taken in turn (one at the top levels, where leaders
+  // sit at multiples of 64)
+  for (int64_t r = 64 * (int64_t)blockIdx.x; r < nblocks; r += 64 * (int64_t)gridDim.x) {
+    uint64_t lead;
+    {
+      const int64_t c = r + l;
+      bool isl = false;
+      if (c < nblocks) {
+        const BNode &nd = nodes[blk[c].node];
+        isl = ((c - nd.blk0) & 63) == 0 && nd.nblk > 64;  // k_cm_node takes single chunks
+      }
+      lead = __ballot(isl);
+    }
+    while (lead) {
+      const int64_t c = r + __builtin_ctzll(lead);
+      lead &= lead - 1;
+      const BNode &nd = nodes[blk[c].node];
+      const int64_t rel = c - nd.blk0;
+      const int nbc = (int)min<int64_t>(64, nd.nblk - rel);
+      const bool v = l < nbc;
+      CmBlock &B = blk[c + (v ? l : 0)];
+      float a[4], mn[3], mx[3];
+      int32_t lsb[4];
+      uint64_t un[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        a[q] = v ? B.approx[q] : 0.f;
+        lsb[q] = v ? B.lsb[q] : (1 << 20);
+        un[q] = v ? B.units[q] : 0ull;
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        mn[d] = v ? B.bmin[d] : INFINITY;
+        mx[d] = v ? B.bmax[d] : -INFINITY;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        float inc = a[q];  // inclusive scan over the lanes
+        for (int o = 1; o < 64; o <<= 1) {
+          const float y = __shfl_up(inc, o);
+          if (l >= o) inc += y;
+        }
+        if (v) B.ein[q] = inc - a[q];
+        a[q] = __shfl(inc, 63);  // the chunk's total
+        const int32_t tc = wave_min_i32(lsb[q]);
+        uint64_t su = 0;
+        if (tc != INT32_MIN) su = units_add_sat(0ull, un[q], lsb[q], tc);
+        su = wave_sum_u64(su);  // 64 values of at most 2^53: no wrap
+        lsb[q] = tc;
+        un[q] = su >= SD_M53 ? SD_M53 : su;
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        mn[d] = wave_min_f32(mn[d]);
+        mx[d] = wave_max_f32(mx[d]);
+      }
+      if (l == 0) {
+        CmBlock &C = blk[c];
+        for (int q = 0; q < 4; ++q) {
+          C.capprox[q] = a[q];
+          C.clsb[q] = lsb[q];
+          C.cunits[q] = lsb[q] == INT32_MIN ? SD_M53 : un[q];
+        }
+        for (int d = 0; d < 3; ++d) {
+          C.cbmin[d] = mn[d];
+          C.cbmax[d] = mx[d];
+        }
+      }
+    }
+  }
+}
+
+// One wave per big node over its chunks: the chunks' exclusive prefix
+// (chunk base), the node bbox, the node's finest lsb tq and the integer sum
+// in units of 2^tq (saturating).
+__global__ void __launch_bounds__(64)
+k_cm_node(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk, int32_t *cmflags, sd_t *cms) {
+  const int l = lane_id();
   for (int64_t t = lb + blockIdx.x; t < le; t += gridDim.x) {
     if (nodes[t].count <= CM_SMALL) continue;
-    const int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
+    const int64_t b0 = nodes[t].blk0, nc = (nodes[t].nblk + 63) / 64;
     float base[4] = {0.f, 0.f, 0.f, 0.f};
     float mn[3] = {INFINITY, INFINITY, INFINITY};
     float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     int32_t lsb[4] = {1 << 20, 1 << 20, 1 << 20, 1 << 20};
-    for (int64_t c0 = 0; c0 < nb; c0 += PFX_T) {
-      const int64_t b = b0 + c0 + threadIdx.x;
-      const bool v = c0 + threadIdx.x < nb;
-      float a[4], inc[4];
+    if (nc == 1) {
+      // at most 64 blocks: their records directly, a lane per block (the
+      // node is its only chunk, whose base is zero)
+      const int nb = (int)nodes[t].nblk;
+      const bool v = l < nb;
+      CmBlock &B = blk[b0 + (v ? l : 0)];
+      uint64_t su[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        a[q] = v ? blk[b].approx[q] : 0.f;
-        inc[q] = a[q];  // inclusive scan over the wave's lanes
+        const float a = v ? B.approx[q] : 0.f;
+        float inc = a;
         for (int o = 1; o < 64; o <<= 1) {
-          const float y = __shfl_up(inc[q], o);
-          if (l >= o) inc[q] += y;
+          const float y = __shfl_up(inc, o);
+          if (l >= o) inc += y;
         }
+        if (v) B.ein[q] = inc - a;
+        const int32_t lq = v ? B.lsb[q] : (1 << 20);
+        const int32_t tq = wave_min_i32(lq);
+        uint64_t u = 0;
+        if (tq != INT32_MIN && v) u = units_add_sat(0ull, B.units[q], lq, tq);
+        su[q] = wave_sum_u64(u);  // 64 values of at most 2^53
+        lsb[q] = tq;
       }
-      if (v) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        mn[d] = wave_min_f32(v ? B.bmin[d] : INFINITY);
+        mx[d] = wave_max_f32(v ? B.bmax[d] : -INFINITY);
+      }
+      if (l == 0) {
+        for (int q = 0; q < 4; ++q) blk[b0].cbase[q] = 0.f;
         for (int d = 0; d < 3; ++d) {
-          mn[d] = fminf(mn[d], blk[b].bmin[d]);
-          mx[d] = fmaxf(mx[d], blk[b].bmax[d]);
+          nodes[t].xmin[d] = mn[d];
+          nodes[t].xmax[d] = mx[d];
         }
-        for (int q = 0; q < 4; ++q) lsb[q] = min(lsb[q], blk[b].lsb[q]);
+        int32_t f = 0;
+        for (int q = 0; q < 4; ++q) {
+          // a negative addend, all addends zero, or a total that rounds
+          if (lsb[q] == INT32_MIN || lsb[q] == (1 << 20) || su[q] >= SD_M53) continue;
+          cms[4 * (t - lb) + q] = sd_from_units(su[q], lsb[q]);
+          f |= 1 << q;
+        }
+        cmflags[t - lb] = f;
       }
-      if (l == 63)
-        for (int q = 0; q < 4; ++q) wtot[w][q] = inc[q];
-      __syncthreads();
+      continue;
+    }
+    for (int64_t k0 = 0; k0 < nc; k0 += 64) {
+      const bool v = k0 + l < nc;
+      CmBlock &C = blk[b0 + 64 * (v ? k0 + l : 0)];  // the chunk's leader
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        float before = 0.f, all = 0.f;
-        for (int k = 0; k < NW; ++k) {
-          if (k < w) before += wtot[k][q];
-          all += wtot[k][q];
+        const float a = v ? C.capprox[q] : 0.f;
+        float inc = a;
+        for (int o = 1; o < 64; o <<= 1) {
+          const float y = __shfl_up(inc, o);
+          if (l >= o) inc += y;
         }
-        if (v) blk[b].ein[q] = base[q] + before + (inc[q] - a[q]);
-        base[q] += all;
+        if (v) C.cbase[q] = base[q] + (inc - a);
+        base[q] += __shfl(inc, 63);
+        if (v) lsb[q] = min(lsb[q], C.clsb[q]);
       }
-      __syncthreads();
+      if (v)
+        for (int d = 0; d < 3; ++d) {
+          mn[d] = fminf(mn[d], C.cbmin[d]);
+          mx[d] = fmaxf(mx[d], C.cbmax[d]);
+        }
     }
+    int32_t tq[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) tq[q] = wave_min_i32(lsb[q]);
+    uint64_t su[4] = {0, 0, 0, 0};
+    for (int64_t k = l; k < nc; k += 64) {
+      const CmBlock &C = blk[b0 + 64 * k];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (tq[q] != INT32_MIN) su[q] = units_add_sat(su[q], C.cunits[q], C.clsb[q], tq[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) su[q] = wave_sum_u64(su[q]);  // 64 values of at most 2^53
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
       mn[d] = wave_min_f32(mn[d]);
       mx[d] = wave_max_f32(mx[d]);
     }
-    for (int q = 0; q < 4; ++q) lsb[q] = wave_min_i32(lsb[q]);
     if (l == 0) {
       for (int d = 0; d < 3; ++d) {
-        wmn[w][d] = mn[d];
-        wmx[w][d] = mx[d];
-      }
-      for (int q = 0; q < 4; ++q) wlsb[w][q] = lsb[q];
-    }
-    __syncthreads();
-    int32_t tq[4];
-    for (int q = 0; q < 4; ++q) {
-      tq[q] = wlsb[0][q];
-      for (int k = 1; k < NW; ++k) tq[q] = min(tq[q], wlsb[k][q]);
-    }
-    // the node's integer sums: block sums rescaled to 2^tq, saturating at 2^53
-    uint64_t su[4] = {0, 0, 0, 0};
-    for (int64_t c = threadIdx.x; c < nb; c += PFX_T) {
-      const CmBlock &B = blk[b0 + c];
-      for (int q = 0; q < 4; ++q)
-        if (tq[q] != INT32_MIN) su[q] = units_add_sat(su[q], B.units[q], B.lsb[q], tq[q]);
-    }
-    for (int q = 0; q < 4; ++q) su[q] = wave_sum_u64(su[q]);  // 64 values below 2^54
-    if (l == 0)
-      for (int q = 0; q < 4; ++q) wun[w][q] = su[q];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      for (int d = 0; d < 3; ++d) {
-        float a = wmn[0][d], b = wmx[0][d];
-        for (int k = 1; k < NW; ++k) {
-          a = fminf(a, wmn[k][d]);
-          b = fmaxf(b, wmx[k][d]);
-        }
-        nodes[t].xmin[d] = a;
-        nodes[t].xmax[d] = b;
+        nodes[t].xmin[d] = mn[d];
+        nodes[t].xmax[d] = mx[d];
       }
       int32_t f = 0;
       for (int q = 0; q < 4; ++q) {
-        // NW wave sums of 64 saturated values each, below 2^59: no wrap
-        uint64_t tot = 0;
-        for (int k = 0; k < NW; ++k) tot += wun[k][q];
         // a negative addend, all addends zero, or a total that rounds
-        if (tq[q] == INT32_MIN || tq[q] == (1 << 20) || tot >= SD_M53) continue;
-        cms[4 * (t - lb) + q] = sd_from_units(tot, tq[q]);
+        if (tq[q] == INT32_MIN || tq[q] == (1 << 20) || su[q] >= SD_M53) continue;
+        cms[4 * (t - lb) + q] = sd_from_units(su[q], tq[q]);
         f |= 1 << q;
       }
       cmflags[t - lb] = f;
     }
-    __syncthreads();  // the shared totals are read before the next node
   }
 }
 
@@ -486,7 +601,8 @@ __device__ __forceinline__ int32_t grid_exp_guess(float ein) {
 // in LDS, each lane composes CM_PER consecutive addends in order, then the 64
 // lane maps are composed in lane order.
 __global__ void __launch_bounds__(64)
-k_cm_blk_maps(CmBlock *blk, const Counters *ctr, bool unit, const float *X,
+k_cm_blk_maps(const BNode *nodes, CmBlock *blk, const Counters *ctr, bool unit,
+              const float *X,
               const float *Y, const float *Z, const float *M, int64_t lb,
               const int32_t *cmflags) {
   __shared__ float buf[64 * (CM_PER + 1)];
@@ -496,7 +612,9 @@ k_cm_blk_maps(CmBlock *blk, const Counters *ctr, bool unit, const float *X,
     if (cmflags[B.node - lb] & (1 << q)) continue;  // summed exactly
     const int l = lane_id();
     const int64_t start = B.start, len = B.len;
-    const int32_t E = grid_exp_guess(B.ein[q]);
+    const int64_t b0 = nodes[B.node].blk0;
+    const float ein = B.ein[q] + blk[b0 + ((b - b0) & ~(int64_t)63)].cbase[q];
+    const int32_t E = grid_exp_guess(ein);
     sd_map acc = sd_map_identity();
     if (E == INT32_MIN) {
       acc.bad = 1;
@@ -762,11 +880,11 @@ k_leaf_ppm(BNode *nodes, int64_t lb, int64_t le, int64_t nd, const float *M) {
 // time read buffer `P`; degenerate splits read the partition's output).
 __global__ void __launch_bounds__(64)
 k_finalize_leaves(const BNode *nodes, int64_t lb, int64_t le, int kind,
-                  const int32_t *P, int32_t *Pfin) {
+                  bool ident, const int32_t *P, int32_t *Pfin) {
   for (int64_t t = lb + blockIdx.x; t < le; t += gridDim.x) {
     if (nodes[t].kind != kind) continue;
     int64_t off = nodes[t].offset, cnt = nodes[t].count;
-    for (int64_t k = lane_id(); k < cnt; k += 64) Pfin[off + k] = P[off + k];
+    for (int64_t k = lane_id(); k < cnt; k += 64) Pfin[off + k] = ident ? (int32_t)(off + k) : P[off + k];
   }
 }
 
@@ -795,9 +913,19 @@ k_part_plan(BNode *nodes, int64_t lb, int64_t le, PartBlock *pb,
     nodes[t].blk0 = b0;
     nodes[t].nblk = nb;
   }
-  for (int j = 0; j < 64; ++j) {
+  // a few blocks by the node's own lane, longer tables by the whole wave
+  if (nb > 0 && nb <= 4) {
+    const int64_t off = nodes[t].offset, cnt = nodes[t].count;
+    for (int64_t b = 0; b < nb; ++b) {
+      PartBlock &B = pb[b0 + b];
+      B.node = t;
+      B.start = off + b * PB;
+      B.len = min<int64_t>(PB, cnt - b * PB);
+    }
+  }
+  for (uint64_t big = __ballot(nb > 4); big; big &= big - 1) {
+    const int j = __builtin_ctzll(big);
     const int64_t nbj = (int64_t)__shfl((long long)nb, j);
-    if (nbj == 0) continue;
     const int64_t b0j = (int64_t)__shfl((long long)b0, j);
     const int64_t tj = t0 + j;
     const int64_t off = nodes[tj].offset, cnt = nodes[tj].count;
@@ -830,19 +958,21 @@ k_part_count(const BNode *nodes, PartBlock *pb, const Counters *ctr,
   }
 }
 
-// In-order exclusive scan of block counts per node, a workgroup per node;
-// degenerate splits (is == 0 or is == count, RCBForceTree.cxx:727-729) are
-// marked.
-__global__ void __launch_bounds__(PFX_T)
+// In-order exclusive scan of block counts per node, a workgroup of NW waves
+// per node (16 at the top levels, where a node holds thousands of blocks,
+// one below); degenerate splits (is == 0 or is == count,
+// RCBForceTree.cxx:727-729) are marked.
+template <int NW>
+__global__ void __launch_bounds__(64 * NW)
 k_part_scan(BNode *nodes, int64_t lb, int64_t le, PartBlock *pb) {
-  constexpr int NW = PFX_WAVES;
+  constexpr int T = 64 * NW;
   __shared__ int64_t wsum[NW];
   const int l = lane_id(), w = threadIdx.x >> 6;
   for (int64_t t = lb + blockIdx.x; t < le; t += gridDim.x) {
     if (nodes[t].kind != NODE_SPLIT) continue;  // uniform over the workgroup
     const int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
     int64_t run = 0;
-    for (int64_t c0 = 0; c0 < nb; c0 += PFX_T) {
+    for (int64_t c0 = 0; c0 < nb; c0 += T) {
       const int64_t b = c0 + threadIdx.x;
       const int64_t v = b < nb ? pb[b0 + b].cnt : 0;
       int64_t s = v;  // inclusive scan over the wave
@@ -935,10 +1065,19 @@ k_part_chain(const BNode *nodes, const PartBlock *pb, const Counters *ctrc,
     const uint32_t is = (uint32_t)n.is, cnt = (uint32_t)n.count, nr = cnt - is;
     const int32_t *idx_n = idx + off;
     const int32_t *rights_n = rights + off;
+    // The chains start at the keys >= pivot below is: the first entries of
+    // the node's sorted rights list.  Block B walks the rights with list
+    // indices in its own element range, so every active lane holds a chain
+    // (most elements start none) and neighbouring lanes start at nearby
+    // positions, with similar chain lengths.
+    const int64_t r0 = B.start - off;
     for (int k = threadIdx.x; k < B.len; k += 256) {
-      int64_t e = B.start + k;
-      if (dest[e] != -1) continue;
-      uint32_t j = (uint32_t)(e - off);
+      const int64_t r = r0 + k;
+      if (r >= (int64_t)nr) break;
+      const uint32_t p = (uint32_t)rights_n[r];
+      if (p >= is) break;  // sorted: this thread's later entries are too
+      const int64_t e = off + p;
+      uint32_t j = p;
       bool done = false;
       for (int it = 0; it < MAXHOP; ++it) {
         const uint32_t f = (uint32_t)idx_n[j];  // f > j
@@ -1012,7 +1151,7 @@ k_part_jump_final(const BNode *nodes, const PartBlock *pb,
 }
 
 __global__ void __launch_bounds__(256)
-k_part_scatter(const BNode *nodes, const PartBlock *pb, const Counters *ctr, bool unit,
+k_part_scatter(const BNode *nodes, const PartBlock *pb, const Counters *ctr, bool unit, bool ident,
                const int32_t *dest, const float *X, const float *Y,
                const float *Z, const float *M, const int32_t *P, float *X2,
                float *Y2, float *Z2, float *M2, int32_t *P2) {
@@ -1026,7 +1165,7 @@ k_part_scatter(const BNode *nodes, const PartBlock *pb, const Counters *ctr, boo
       Y2[o] = Y[e];
       Z2[o] = Z[e];
       if (!unit) M2[o] = M[e];  // unit masses are not moved (never read again)
-      P2[o] = P[e];
+      P2[o] = ident ? (int32_t)e : P[e];  // the first level's order is the input's
     }
   }
 }
@@ -1333,20 +1472,23 @@ __global__ void k_set_flag(int32_t *flag) { flag[0] = 1; }
 __global__ void __launch_bounds__(256)
 k_unit_check(int64_t n, const float *m, int32_t *flag) {
   bool ok = true;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    ok = ok && m[i] == 1.0f;
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+  // float4 loads from the first 16-byte boundary, scalar head and tail
+  const int64_t head = std::min<int64_t>(n, (int64_t)(((16 - ((uintptr_t)m & 15)) & 15) / 4));
+  const int64_t nv = (n - head) / 4;
+  const float4 *m4 = reinterpret_cast<const float4 *>(m + head);
+  for (int64_t i = g; i < nv; i += stride) {
+    const float4 v = m4[i];
+    ok = ok && v.x == 1.0f && v.y == 1.0f && v.z == 1.0f && v.w == 1.0f;
+  }
+  if (g < head) ok = ok && m[g] == 1.0f;
+  for (int64_t i = head + 4 * nv + g; i < n; i += stride) ok = ok && m[i] == 1.0f;
   if (!__all(ok) && lane_id() == 0) flag[0] = 0;
 }
 
 __global__ void k_zero_counters(Counters *ctr) {
   int64_t *p = reinterpret_cast<int64_t *>(ctr);
   if (threadIdx.x < sizeof(Counters) / sizeof(int64_t)) p[threadIdx.x] = 0;
-}
-
-__global__ void k_iota(int64_t n, int32_t *P) {
-  int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (k < n) P[k] = (int32_t)k;
 }
 
 inline unsigned nblk_for(int64_t n, int t) {
@@ -1414,13 +1556,6 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
   const bool unit = unit_h != 0;
   T.unit = unit;
   const bool unit_ppm = unit;  // leaf pseudo masses set by k_classify
-  if (n > 0) {
-    HSR_HIP(hipMemcpyAsync(X[0], x, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
-    HSR_HIP(hipMemcpyAsync(Y[0], y, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
-    HSR_HIP(hipMemcpyAsync(Z[0], z, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
-    if (!unit) HSR_HIP(hipMemcpyAsync(Mm[0], m, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
-    k_iota<<<nblk_for(n, 256), 256, 0, st>>>(n, P[0]);
-  }
   HSR_HIP(hipMemcpyAsync(d_box_off, box_off_h, sizeof(int64_t) * (nboxes + 1), hipMemcpyHostToDevice, st));
   HSR_HIP(hipMemcpyAsync(d_min, min_loc_h, sizeof(float) * 3 * nboxes, hipMemcpyHostToDevice, st));
   HSR_HIP(hipMemcpyAsync(d_max, max_loc_h, sizeof(float) * 3 * nboxes, hipMemcpyHostToDevice, st));
@@ -1451,6 +1586,10 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
       nodes = dptr<BNode>(T.bnodes);
     }
     k_zero_counters<<<1, 64, 0, st>>>(ctr);
+    // the first level reads the caller's arrays; its partition writes the
+    // first copy (buffer 1), and the levels after it alternate
+    const float *Xr = depth == 0 ? x : X[cur], *Yr = depth == 0 ? y : Y[cur];
+    const float *Zr = depth == 0 ? z : Z[cur], *Mr = depth == 0 ? m : Mm[cur];
     // ---- centre of mass
     int64_t cm_cap = active / CM_BLK + active / CM_SMALL + 2;
     HSR_HIP(dbuf_reserve(S.blocks, sizeof(CmBlock) * cm_cap));
@@ -1464,42 +1603,47 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     const unsigned gnl = (unsigned)std::min<int64_t>(nl, gbig);
     HSR_HIP(dbuf_reserve(S.cmfail, sizeof(int32_t) * 4 * (size_t)nl));
     int32_t *cm_list = dptr<int32_t>(S.cmfail);
-    k_cm_small<<<dim3((unsigned)nl, nq), 64, 0, st>>>(nodes, lb, le, unit, X[cur], Y[cur], Z[cur], Mm[cur],
+    k_cm_small<<<dim3((unsigned)nl, nq), 64, 0, st>>>(nodes, lb, le, unit, Xr, Yr, Zr, Mr,
                                                      cms, cm_list, ctr);
     k_cm_small_ordered<<<std::min<unsigned>((unsigned)std::max<int64_t>(1, nl), 4 * gbig), 64, 0, st>>>(
-        nodes, lb, unit, X[cur], Y[cur], Z[cur], Mm[cur], cms, cm_list, ctr);
-    k_cm_plan<<<gnl, 64, 0, st>>>(nodes, lb, le, cmb, ctr);
+        nodes, lb, unit, Xr, Yr, Zr, Mr, cms, cm_list, ctr);
+    k_cm_plan<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, cmb, ctr);
     unsigned cmg = (unsigned)std::max<int64_t>(1, active / CM_BLK + active / CM_SMALL + 1);
     cmg = std::min(cmg, gbig);
     if (unit)
-      k_cm_blk_stats<true><<<cmg, 64, 0, st>>>(cmb, ctr, X[cur], Y[cur], Z[cur], Mm[cur]);
+      k_cm_blk_stats<true><<<cmg, 64, 0, st>>>(cmb, ctr, Xr, Yr, Zr, Mr);
     else
-      k_cm_blk_stats<false><<<cmg, 64, 0, st>>>(cmb, ctr, X[cur], Y[cur], Z[cur], Mm[cur]);
+      k_cm_blk_stats<false><<<cmg, 64, 0, st>>>(cmb, ctr, Xr, Yr, Zr, Mr);
     HSR_HIP(dbuf_reserve(S.cmx, sizeof(int32_t) * (size_t)nl));
     int32_t *cmflags = dptr<int32_t>(S.cmx);
-    k_cm_blk_prefix<<<gnl, PFX_T, 0, st>>>(nodes, lb, le, cmb, cmflags, cms);
-    k_cm_blk_maps<<<dim3(cmg, nq), 64, 0, st>>>(cmb, ctr, unit, X[cur], Y[cur], Z[cur], Mm[cur], lb, cmflags);
+    k_cm_chunk<<<std::max(1u, std::min<unsigned>(gbig, (unsigned)(cm_cap / 64 + 1))), 64, 0, st>>>(nodes, cmb, ctr);
+    k_cm_node<<<gnl, 64, 0, st>>>(nodes, lb, le, cmb, cmflags, cms);
+    k_cm_blk_maps<<<dim3(cmg, nq), 64, 0, st>>>(nodes, cmb, ctr, unit, Xr, Yr, Zr, Mr, lb,
+                                                cmflags);
     k_cm_win_maps<<<cmg, 64, 0, st>>>(nodes, cmb, ctr);
-    k_cm_blk_chain<<<dim3(gnl, nq), 64, 0, st>>>(nodes, lb, le, unit, cmb, X[cur], Y[cur], Z[cur],
-                                                 Mm[cur], ctr, cms, cmflags);
+    k_cm_blk_chain<<<dim3(gnl, nq), 64, 0, st>>>(nodes, lb, le, unit, cmb, Xr, Yr, Zr, Mr, ctr, cms,
+                                                 cmflags);
     k_cm_finish<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, unit, cms);
     // ---- classify, finalize plain leaves
     k_classify<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, nd, c->p.ppc, unit_ppm);
-    if (!unit_ppm) k_leaf_ppm<<<gnl, 64, 0, st>>>(nodes, lb, le, nd, Mm[cur]);
-    k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_LEAF, P[cur], Pfin);
+    if (!unit_ppm) k_leaf_ppm<<<gnl, 64, 0, st>>>(nodes, lb, le, nd, Mr);
+    k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_LEAF, depth == 0, P[cur], Pfin);
     // ---- partition of splitting nodes
     int64_t pcap = active / PB + nl + 2;
     HSR_HIP(dbuf_reserve(S.pblocks, sizeof(PartBlock) * pcap));
     PartBlock *pb = dptr<PartBlock>(S.pblocks);
     k_part_plan<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, pb, ctr);
     const unsigned pg = (unsigned)std::max<int64_t>(1, pcap);
-    k_part_count<<<pg, 256, 0, st>>>(nodes, pb, ctr, X[cur], Y[cur], Z[cur]);
-    k_part_scan<<<std::min<unsigned>((unsigned)nl, 4u * (unsigned)std::max(1, c->num_cus)), PFX_T, 0, st>>>(
-        nodes, lb, le, pb);
+    k_part_count<<<pg, 256, 0, st>>>(nodes, pb, ctr, Xr, Yr, Zr);
+    if (active / PB > 64 * nl)  // over 64 blocks per node on average
+      k_part_scan<PFX_WAVES><<<std::min<unsigned>((unsigned)nl, 4u * (unsigned)std::max(1, c->num_cus)), PFX_T,
+                               0, st>>>(nodes, lb, le, pb);
+    else
+      k_part_scan<1><<<std::min<unsigned>((unsigned)nl, gbig), 64, 0, st>>>(nodes, lb, le, pb);
     // children depend only on the partition counts: create them now so a
     // single host synchronisation per level reads every counter
     k_make_children<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, le, ctr);
-    k_part_rank<<<pg, 256, 0, st>>>(nodes, pb, ctr, X[cur], Y[cur], Z[cur], idx, rights, dest);
+    k_part_rank<<<pg, 256, 0, st>>>(nodes, pb, ctr, Xr, Yr, Zr, idx, rights, dest);
     k_part_chain<<<pg, 256, 0, st>>>(nodes, pb, ctr, idx, rights, dest, ctr);
     HSR_HIP(hipMemcpyAsync(&hc, ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
     HSR_HIP(stream_wait(st));
@@ -1509,9 +1653,9 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
         k_part_jump_round<<<pg, 256, 0, st>>>(nodes, pb, ctr, jmp);
       k_part_jump_final<<<pg, 256, 0, st>>>(nodes, pb, ctr, jmp, dest);
     }
-    k_part_scatter<<<pg, 256, 0, st>>>(nodes, pb, ctr, unit, dest, X[cur], Y[cur], Z[cur], Mm[cur], P[cur],
+    k_part_scatter<<<pg, 256, 0, st>>>(nodes, pb, ctr, unit, depth == 0, dest, Xr, Yr, Zr, Mr, P[cur],
                                         X[cur ^ 1], Y[cur ^ 1], Z[cur ^ 1], Mm[cur ^ 1], P[cur ^ 1]);
-    k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_DEGEN, P[cur ^ 1], Pfin);
+    k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_DEGEN, false, P[cur ^ 1], Pfin);
     HSR_HIP(hipGetLastError());
 #ifdef HSR_CM_STATS
     fprintf(stderr, "[cm] level %d nodes %lld blocks %lld scans %lld fallback %lld small ordered %lld\n",
