@@ -58,6 +58,56 @@ __device__ __forceinline__ const float *coord(int d, const float *X,
   return d == 0 ? X : (d == 1 ? Y : Z);
 }
 
+// A finite float as odd * 2^lsb (odd = 0 and lsb = 2^20 for zero), without
+// branches: the mantissa with its hidden bit, its trailing zeros shifted out.
+__device__ __forceinline__ void f32_odd_lsb(float a, uint32_t &odd, int32_t &lsb) {
+  const uint32_t u = __float_as_uint(a);
+  const uint32_t e = (u >> 23) & 255u, fr = u & 0x7fffffu;
+  const uint32_t fm = e ? (fr | 0x800000u) : fr;
+  const int32_t fe = e ? (int32_t)e - 150 : -149;
+  const int tz = __builtin_ctz(fm | 0x80000000u);  // fm < 2^24: bit 31 stops a zero
+  odd = fm >> tz;
+  lsb = fm ? fe + tz : (1 << 20);
+}
+
+// A lane's exact integer sum of nonnegative floats in units of the finest
+// lsb t seen so far (one pass: a finer addend rescales the sum).  bad: an
+// addend is negative, or the sum reaches 2^53 units (in any finer unit it
+// stays there), so no exact integer sum below 2^53 exists.
+struct LaneSum {
+  uint64_t u = 0;
+  int32_t t = 1 << 20;
+  bool bad = false;
+};
+
+__device__ __forceinline__ void lane_sum_add(LaneSum &S, float a) {
+  uint32_t odd;
+  int32_t la;
+  f32_odd_lsb(a, odd, la);
+  const bool nz = odd != 0u;
+  const bool finer = nz && la < S.t;
+  const uint32_t d = finer ? (uint32_t)(S.t - la) : 0u;
+  const bool ovr = finer && S.u != 0 && (d >= 53u || (S.u >> (53u - d)) != 0);
+  S.u = finer ? (d >= 53u ? 0ull : S.u << d) : S.u;
+  S.t = finer ? la : S.t;
+  const uint32_t sh = nz ? (uint32_t)(la - S.t) : 0u;  // la >= t now
+  const bool ova = nz && (sh >= 53u || ((uint64_t)odd >> (53u - sh)) != 0);
+  S.u += (nz && !ova) ? ((uint64_t)odd << sh) : 0ull;
+  S.bad = S.bad || ovr || ova || S.u >= SD_M53 || (nz && a < 0.f);
+}
+
+// The wave's sum from its lanes': the finest lsb t over the wave and the
+// total in units of 2^t, or bad (no exact integer sum below 2^53 units).
+__device__ __forceinline__ uint64_t wave_lane_sum(const LaneSum &S, int32_t &t, bool &bad) {
+  t = wave_min_i32(S.t);
+  const uint32_t d = (uint32_t)(S.t - t);
+  const bool ov = S.u != 0 && (d >= 53u || (S.u >> (53u - d)) != 0);
+  const uint64_t u = ov ? 0ull : (d >= 53u ? 0ull : S.u << d);
+  const uint64_t tot = wave_sum_u64(u);  // 64 values below 2^53
+  bad = __any(S.bad || ov) || tot >= SD_M53;
+  return tot;
+}
+
 // ------------------------------------------------------------ cm (small)
 // With unit masses (every mass exactly 1.0f) the addend w * x is x exactly,
 // the mass sum is the count, and M is not read (the partition then stops
@@ -142,10 +192,10 @@ k_cm_small(BNode *nodes, int64_t lb, int64_t le, bool unit, const float *X,
       }
     }
   }
-  // two passes over the addends, 8 loads in flight per lane (the second
-  // pass reads them again from L2)
-  int32_t tl = 1 << 20;  // the lsb exponent of zero (sd_lsb_f32)
+  // one pass over the addends, 8 loads in flight per lane
+  LaneSum ls;
   for (int i0 = 0; i0 < SM_PER; i0 += 8) {
+    if ((int64_t)i0 * 64 >= cnt) break;  // uniform
     float v[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
@@ -153,31 +203,14 @@ k_cm_small(BNode *nodes, int64_t lb, int64_t le, bool unit, const float *X,
       v[i] = k < cnt ? addend(q, unit, X, Y, Z, M, off + k) : 0.f;
     }
 #pragma unroll
-    for (int i = 0; i < 8; ++i) tl = min(tl, sd_lsb_f32(v[i]));
+    for (int i = 0; i < 8; ++i) lane_sum_add(ls, v[i]);
   }
-  tl = wave_min_i32(tl);
-  uint64_t u = 0;  // < 2048 addends below 2^53 units each: no wrap
-  int bad = 0;
-  for (int i0 = 0; i0 < SM_PER; i0 += 8) {
-    float v[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int64_t k = (i0 + i) * 64 + l;
-      v[i] = k < cnt ? addend(q, unit, X, Y, Z, M, off + k) : 0.f;
-    }
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const uint64_t uu = sd_units_f32(v[i], tl);
-      if (uu == ~0ull) bad = 1;
-      else u += uu;
-    }
-  }
-  if (!__any(bad)) {
-    const uint64_t tot = wave_sum_u64(u);
-    if (tot < SD_M53) {
-      if (l == 0) cms[4 * (t - lb) + q] = sd_from_units(tot, tl);
-      return;
-    }
+  int32_t tl;
+  bool bad;
+  const uint64_t tot = wave_lane_sum(ls, tl, bad);
+  if (!bad) {
+    if (l == 0) cms[4 * (t - lb) + q] = sd_from_units(tot, tl);
+    return;
   }
   if (l == 0) {
     const int64_t k = (int64_t)atomicAdd((unsigned long long *)&ctr->cm_fail, 1ull);
@@ -277,80 +310,75 @@ k_cm_plan(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk, Counters *ctr) {
 
 // Approximate sums and bbox per block (one wave per block).  With unit
 // masses (UNIT) the mass sum's record is written directly: approximate sum
-// and integer sum len in units of 2^0.
+// and integer sum len in units of 2^0.  The block's integer sum of a
+// nonnegative sum is taken in units of its finest lsb (LaneSum); one that
+// reaches 2^53 units is marked as not an integer there, which leaves its
+// node off the integer path just as the node's total would.
 template <bool UNIT>
 __global__ void __launch_bounds__(64)
 k_cm_blk_stats(CmBlock *blk, const Counters *ctr, const float *X,
                const float *Y, const float *Z, const float *M) {
   constexpr int NQ = UNIT ? 3 : 4;  // sums formed from the addends
+  constexpr int RU = 4;             // elements per lane per load round
   const int l = lane_id();
   for (int64_t b = blockIdx.x; b < ctr->cm_blocks; b += gridDim.x) {
     CmBlock &B = blk[b];
     const int64_t start = B.start;
     const int len = (int)B.len;
-    // the block's elements in registers, CM_PER per lane, one load round
-    float xv[CM_PER], yv[CM_PER], zv[CM_PER], wv[CM_PER];
-#pragma unroll
-    for (int i = 0; i < CM_PER; ++i) {
-      const int k = i * 64 + l;
-      const int64_t e = start + (k < len ? k : 0);
-      wv[i] = UNIT ? 1.f : M[e];
-      xv[i] = X[e];
-      yv[i] = Y[e];
-      zv[i] = Z[e];
-    }
     float s[4] = {0.f, 0.f, 0.f, 0.f};
     float mn[3] = {INFINITY, INFINITY, INFINITY};
     float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    int32_t lsb[4] = {1 << 20, 1 << 20, 1 << 20, 1 << 20};
+    LaneSum ls[4];
+    bool neg[4] = {false, false, false, false};
+    // rounds of RU elements per lane: the loads of a round are in flight
+    // together and few values stay live
+#pragma unroll 1
+    for (int i0 = 0; i0 < CM_PER; i0 += RU) {
+      float xv[RU], yv[RU], zv[RU], wv[RU];
 #pragma unroll
-    for (int i = 0; i < CM_PER; ++i) {
-      if (i * 64 + l >= len) continue;
-      mn[0] = fminf(mn[0], xv[i]); mx[0] = fmaxf(mx[0], xv[i]);
-      mn[1] = fminf(mn[1], yv[i]); mx[1] = fmaxf(mx[1], yv[i]);
-      mn[2] = fminf(mn[2], zv[i]); mx[2] = fmaxf(mx[2], zv[i]);
-      const float w = wv[i];
-      // the addends of BGQCM.c (w * x is x exactly when w = 1)
-      const float a[4] = {UNIT ? xv[i] : w * xv[i], UNIT ? yv[i] : w * yv[i],
-                          UNIT ? zv[i] : w * zv[i], w};
+      for (int i = 0; i < RU; ++i) {
+        const int k = (i0 + i) * 64 + l;
+        const int64_t e = start + (k < len ? k : 0);
+        wv[i] = UNIT ? 1.f : M[e];
+        xv[i] = X[e];
+        yv[i] = Y[e];
+        zv[i] = Z[e];
+      }
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        s[q] += a[q];
-        lsb[q] = a[q] < 0.f ? INT32_MIN : min(lsb[q], sd_lsb_f32(a[q]));
+      for (int i = 0; i < RU; ++i) {
+        const bool v = (i0 + i) * 64 + l < len;
+        mn[0] = fminf(mn[0], v ? xv[i] : INFINITY); mx[0] = fmaxf(mx[0], v ? xv[i] : -INFINITY);
+        mn[1] = fminf(mn[1], v ? yv[i] : INFINITY); mx[1] = fmaxf(mx[1], v ? yv[i] : -INFINITY);
+        mn[2] = fminf(mn[2], v ? zv[i] : INFINITY); mx[2] = fmaxf(mx[2], v ? zv[i] : -INFINITY);
+        const float w = wv[i];
+        // the addends of BGQCM.c (w * x is x exactly when w = 1); zero outside
+        const float a[4] = {v ? (UNIT ? xv[i] : w * xv[i]) : 0.f, v ? (UNIT ? yv[i] : w * yv[i]) : 0.f,
+                            v ? (UNIT ? zv[i] : w * zv[i]) : 0.f, v ? w : 0.f};
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          s[q] += a[q];
+          neg[q] = neg[q] || a[q] < 0.f;
+          lane_sum_add(ls[q], a[q]);
+        }
       }
     }
+    int32_t lsb[4] = {1 << 20, 1 << 20, 1 << 20, 1 << 20};
+    uint64_t un[4] = {0, 0, 0, 0};
+    bool bad[4] = {false, false, false, false};
+#pragma unroll
     for (int q = 0; q < NQ; ++q) {
       s[q] = wave_sum_f32(s[q]);
-      lsb[q] = wave_min_i32(lsb[q]);
-    }
-    // the block's integer sums in units of its finest lsb; fewer than 1024
-    // addends below 2^53 units each
-    uint64_t un[4] = {0, 0, 0, 0};
-    int bad[4] = {0, 0, 0, 0};
-#pragma unroll
-    for (int i = 0; i < CM_PER; ++i) {
-      if (i * 64 + l >= len) continue;
-      const float w = wv[i];
-      const float a[4] = {UNIT ? xv[i] : w * xv[i], UNIT ? yv[i] : w * yv[i],
-                          UNIT ? zv[i] : w * zv[i], w};
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        if (lsb[q] == INT32_MIN) continue;
-        const uint64_t uu = sd_units_f32(a[q], lsb[q]);
-        if (uu == ~0ull) bad[q] = 1;
-        else un[q] += uu;
-      }
-    }
-    for (int q = 0; q < NQ; ++q) {
-      bad[q] = __any(bad[q]) || lsb[q] == INT32_MIN;
-      un[q] = wave_sum_u64(un[q]);
+      un[q] = wave_lane_sum(ls[q], lsb[q], bad[q]);
+      // a negative addend: no integer path for this sum (INT32_MIN)
+      if (__any(neg[q])) lsb[q] = INT32_MIN;
+      bad[q] = bad[q] || lsb[q] == INT32_MIN;
     }
     if (UNIT) {
       // len ones: the float sum, the lsb of 1.0 and the count are exact
       s[3] = (float)len;
       lsb[3] = 0;
       un[3] = (uint64_t)len;
-      bad[3] = 0;
+      bad[3] = false;
     }
     for (int d = 0; d < 3; ++d) {
       mn[d] = wave_min_f32(mn[d]);
