@@ -240,18 +240,15 @@ k_cm_small_ordered(const BNode *nodes, int64_t lb, bool unit, const float *X, co
   }
 }
 
-// xc = (float)(sum w x / sum w) per dimension for every non-empty node of
-// the level, from the four exact sums (BGQCM.c:208-210).
-__global__ void k_cm_finish(BNode *nodes, int64_t lb, int64_t le, bool unit, const sd_t *cms) {
-  const int64_t t = lb + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= le) return;
-  const int64_t cnt = nodes[t].count;
+// xc = (float)(sum w x / sum w) per dimension of a non-empty node of the
+// level, from the four exact sums (BGQCM.c:208-210).
+__device__ __forceinline__ void cm_finish_node(BNode &n, bool unit, const sd_t *S) {
+  const int64_t cnt = n.count;
   if (cnt == 0) return;
-  const sd_t *S = cms + 4 * (t - lb);
   // unit masses: the mass sum is the count, exactly (in units of 2^0, as the
   // integer-sum path forms it from addends 1.0)
   const sd_t Sm = unit ? sd_from_units((uint64_t)cnt, 0) : S[3];
-  for (int d = 0; d < 3; ++d) nodes[t].xc[d] = sd_div_to_f32(S[d], Sm);
+  for (int d = 0; d < 3; ++d) n.xc[d] = sd_div_to_f32(S[d], Sm);
 }
 
 // ------------------------------------------------------------ cm (big)
@@ -810,12 +807,8 @@ k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, bool unit, const CmBlock *b
 // Leaves (count <= nd) get tdr and, for count > 1, ppm = sequential float
 // sum of member masses (pp with TDPTS = 1: mass*(1+0+0)); other nodes get
 // the split dimension (longest edge, ties x > y > z as coded at
-// RCBForceTree.cxx:843-852).
-__global__ void k_classify(BNode *nodes, int64_t lb, int64_t le, int64_t nd,
-                           float ppc, bool unit_ppm) {
-  int64_t t = lb + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= le) return;
-  BNode &n = nodes[t];
+// RCBForceTree.cxx:843-852).  Returns whether the node splits.
+__device__ __forceinline__ bool classify_node(BNode &n, int64_t nd, float ppc, bool unit_ppm) {
   n.cl = n.cr = -1;
   if (n.count <= nd) {
     n.kind = NODE_LEAF;
@@ -827,7 +820,7 @@ __global__ void k_classify(BNode *nodes, int64_t lb, int64_t le, int64_t nd,
     // (unit_ppm) the in-order float sum of count ones: the count, exact up
     // to 2^24, where adding 1 rounds back to 2^24 (ties to even)
     n.ppm = (unit_ppm && n.count > 1) ? (n.count < (1 << 24) ? (float)n.count : 16777216.f) : 0.f;
-    return;
+    return false;
   }
   n.kind = NODE_SPLIT;
   float xl0 = n.xmax[0] - n.xmin[0];
@@ -838,6 +831,7 @@ __global__ void k_classify(BNode *nodes, int64_t lb, int64_t le, int64_t nd,
   else if (xl1 > xl2) d = 1;
   else d = 2;
   n.split_d = d;
+  return true;
 }
 
 // Lowest set bit of a finite nonzero float, as a power-of-two exponent.
@@ -918,15 +912,21 @@ k_finalize_leaves(const BNode *nodes, int64_t lb, int64_t le, int kind,
 
 // ------------------------------------------------------------ partition
 __global__ void __launch_bounds__(64)
-k_part_plan(BNode *nodes, int64_t lb, int64_t le, PartBlock *pb,
-            Counters *ctr) {
-  // one lane per node: a wave scan of the block counts and one atomic per
-  // wave reserve the blocks; then the wave fills them node by node
+k_part_plan(BNode *nodes, int64_t lb, int64_t le, bool unit, const sd_t *cms, int64_t nd,
+            float ppc, bool unit_ppm, PartBlock *pb, Counters *ctr) {
+  // one lane per node: its centre of mass and class (cm_finish_node,
+  // classify_node), then a wave scan of the block counts of the splitting
+  // nodes and one atomic per wave reserve their partition blocks; then the
+  // wave fills them node by node
   const int l = lane_id();
   const int64_t t0 = lb + (int64_t)blockIdx.x * 64;
   const int64_t t = t0 + l;
   int64_t nb = 0;
-  if (t < le && nodes[t].kind == NODE_SPLIT) nb = (nodes[t].count + PB - 1) / PB;
+  if (t < le) {
+    BNode &n = nodes[t];
+    cm_finish_node(n, unit, cms + 4 * (t - lb));
+    if (classify_node(n, nd, ppc, unit_ppm)) nb = (n.count + PB - 1) / PB;
+  }
   int64_t inc = nb;
   for (int o = 1; o < 64; o <<= 1) {
     const int64_t y = (int64_t)__shfl_up((long long)inc, o);
@@ -1201,8 +1201,12 @@ k_part_scatter(const BNode *nodes, const PartBlock *pb, const Counters *ctr, boo
 // Children of non-degenerate splits form the next level
 // (RCBForceTree.cxx:731-770: left = [offset, offset+is), right = rest).
 __global__ void k_make_children(BNode *nodes, int64_t lb, int64_t le,
-                                int64_t next_base, Counters *ctr) {
+                                int64_t next_base, Counters *ctr, Counters *ctr_next) {
   int64_t t = lb + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  // the next level's counters (the other of two, read by the host before
+  // this level ends) start from zero
+  if (blockIdx.x == 0 && threadIdx.x < sizeof(Counters) / sizeof(int64_t))
+    reinterpret_cast<int64_t *>(ctr_next)[threadIdx.x] = 0;
   if (t >= le) return;
   BNode &n = nodes[t];
   if (n.kind != NODE_SPLIT) return;
@@ -1551,7 +1555,7 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
   HSR_HIP(dbuf_reserve(S.b, sizeof(int32_t) * nn));  // dest
   HSR_HIP(dbuf_reserve(S.c, sizeof(int32_t) * nn));  // jmp
   HSR_HIP(dbuf_reserve(S.rights, sizeof(int32_t) * nn));
-  HSR_HIP(dbuf_reserve(S.counters, sizeof(Counters) + 64));
+  HSR_HIP(dbuf_reserve(S.counters, 2 * sizeof(Counters) + 64));
   HSR_HIP(dbuf_reserve(S.d, sizeof(int64_t) * (nboxes + 1) + sizeof(float) * 6 * nboxes + 64));
   // (the 64 spare bytes after the box table hold the unit-mass flag)
 
@@ -1565,7 +1569,7 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
   int32_t *dest = dptr<int32_t>(S.b);
   int32_t *jmp = dptr<int32_t>(S.c);
   int32_t *rights = dptr<int32_t>(S.rights);
-  Counters *ctr = dptr<Counters>(S.counters);
+  Counters *ctr2 = dptr<Counters>(S.counters);  // two sets, alternating by level
 
   // box table
   int64_t *d_box_off = dptr<int64_t>(S.d);
@@ -1613,7 +1617,8 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
       cap = ncap;
       nodes = dptr<BNode>(T.bnodes);
     }
-    k_zero_counters<<<1, 64, 0, st>>>(ctr);
+    Counters *ctr = ctr2 + (depth & 1);
+    if (depth == 0) k_zero_counters<<<1, 64, 0, st>>>(ctr);
     // the first level reads the caller's arrays; its partition writes the
     // first copy (buffer 1), and the levels after it alternate
     const float *Xr = depth == 0 ? x : X[cur], *Yr = depth == 0 ? y : Y[cur];
@@ -1651,16 +1656,15 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     k_cm_win_maps<<<cmg, 64, 0, st>>>(nodes, cmb, ctr);
     k_cm_blk_chain<<<dim3(gnl, nq), 64, 0, st>>>(nodes, lb, le, unit, cmb, Xr, Yr, Zr, Mr, ctr, cms,
                                                  cmflags);
-    k_cm_finish<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, unit, cms);
-    // ---- classify, finalize plain leaves
-    k_classify<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, nd, c->p.ppc, unit_ppm);
-    if (!unit_ppm) k_leaf_ppm<<<gnl, 64, 0, st>>>(nodes, lb, le, nd, Mr);
-    k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_LEAF, depth == 0, P[cur], Pfin);
-    // ---- partition of splitting nodes
+    // ---- centre of mass, class and partition blocks (a lane per node),
+    // finalize plain leaves
     int64_t pcap = active / PB + nl + 2;
     HSR_HIP(dbuf_reserve(S.pblocks, sizeof(PartBlock) * pcap));
     PartBlock *pb = dptr<PartBlock>(S.pblocks);
-    k_part_plan<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, pb, ctr);
+    k_part_plan<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, unit, cms, nd, c->p.ppc, unit_ppm, pb, ctr);
+    if (!unit_ppm) k_leaf_ppm<<<gnl, 64, 0, st>>>(nodes, lb, le, nd, Mr);
+    k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_LEAF, depth == 0, P[cur], Pfin);
+    // ---- partition of splitting nodes
     const unsigned pg = (unsigned)std::max<int64_t>(1, pcap);
     k_part_count<<<pg, 256, 0, st>>>(nodes, pb, ctr, Xr, Yr, Zr);
     if (active / PB > 64 * nl)  // over 64 blocks per node on average
@@ -1670,7 +1674,7 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
       k_part_scan<1><<<std::min<unsigned>((unsigned)nl, gbig), 64, 0, st>>>(nodes, lb, le, pb);
     // children depend only on the partition counts: create them now so a
     // single host synchronisation per level reads every counter
-    k_make_children<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, le, ctr);
+    k_make_children<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, le, ctr, ctr2 + ((depth + 1) & 1));
     k_part_rank<<<pg, 256, 0, st>>>(nodes, pb, ctr, Xr, Yr, Zr, idx, rights, dest);
     k_part_chain<<<pg, 256, 0, st>>>(nodes, pb, ctr, idx, rights, dest, ctr);
     HSR_HIP(hipMemcpyAsync(&hc, ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
