@@ -36,9 +36,9 @@ struct BNode {
 enum { NODE_LEAF = 0, NODE_SPLIT = 1, NODE_DEGEN = 2 };
 
 // Exact centre-of-mass accumulators of one block of a big node.
+// (a block's node is in a separate int32 table; its element range follows
+// from the node's offset, count and first block)
 struct CmBlock {
-  int64_t node;            // BFS id
-  int64_t start, len;      // absolute element range
   float approx[4];         // float sums of x, y, z, m addends
   float ein[4];            // approximate exclusive prefix within the node
   uint64_t d0[4], d1[4];   // fixed-grid maps
@@ -62,8 +62,6 @@ struct CmBlock {
 };
 
 struct PartBlock {
-  int64_t node;            // BFS id
-  int64_t start, len;      // absolute element range
   int64_t cnt;             // keys < pivot in the block
   int64_t base;            // exclusive prefix within the node
 };
@@ -173,6 +171,7 @@ struct Scratch {
   DBuf cmx;        // tree build: big nodes' sums done by the integer path (bit q)
   DBuf x1, y1, z1, m1, p1, x0, y0, z0, m0, p0;
   DBuf blocks, pblocks, rights;
+  DBuf cnode, pnode;  // node (BFS id) of each centre-of-mass / partition block
   DBuf counters, qctr;
   DBuf host_in[10];
 };

@@ -108,6 +108,15 @@ __device__ __forceinline__ uint64_t wave_lane_sum(const LaneSum &S, int32_t &t, 
   return tot;
 }
 
+// A block's element range from its node: blocks of bs elements from the
+// node's first element, blk0 the node's first block in the level's table.
+__device__ __forceinline__ void blk_range(const BNode &n, int64_t b, int64_t bs, int64_t &start,
+                                          int &len) {
+  const int64_t rel = (b - n.blk0) * bs;
+  start = n.offset + rel;
+  len = (int)min<int64_t>(bs, n.count - rel);
+}
+
 // ------------------------------------------------------------ cm (small)
 // With unit masses (every mass exactly 1.0f) the addend w * x is x exactly,
 // the mass sum is the count, and M is not read (the partition then stops
@@ -254,7 +263,7 @@ __device__ __forceinline__ void cm_finish_node(BNode &n, bool unit, const sd_t *
 // ------------------------------------------------------------ cm (big)
 // Block table for nodes with count > CM_SMALL.
 __global__ void __launch_bounds__(64)
-k_cm_plan(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk, Counters *ctr) {
+k_cm_plan(BNode *nodes, int64_t lb, int64_t le, int32_t *cnode, Counters *ctr) {
   // one lane per node: a wave scan of the block counts and one atomic per
   // wave reserve the blocks; then the wave fills them node by node
   const int l = lane_id();
@@ -281,27 +290,13 @@ k_cm_plan(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk, Counters *ctr) {
     nodes[t].nblk = nb;
   }
   // a few blocks by the node's own lane, longer tables by the whole wave
-  if (nb > 0 && nb <= 4) {
-    const int64_t off = nodes[t].offset, cnt = nodes[t].count;
-    for (int64_t b = 0; b < nb; ++b) {
-      CmBlock &B = blk[b0 + b];
-      B.node = t;
-      B.start = off + b * CM_BLK;
-      B.len = min<int64_t>(CM_BLK, cnt - b * CM_BLK);
-    }
-  }
+  if (nb > 0 && nb <= 4)
+    for (int64_t b = 0; b < nb; ++b) cnode[b0 + b] = (int32_t)t;
   for (uint64_t big = __ballot(nb > 4); big; big &= big - 1) {
     const int j = __builtin_ctzll(big);
     const int64_t nbj = (int64_t)__shfl((long long)nb, j);
     const int64_t b0j = (int64_t)__shfl((long long)b0, j);
-    const int64_t tj = t0 + j;
-    const int64_t off = nodes[tj].offset, cnt = nodes[tj].count;
-    for (int64_t b = l; b < nbj; b += 64) {
-      CmBlock &B = blk[b0j + b];
-      B.node = tj;
-      B.start = off + b * CM_BLK;
-      B.len = min<int64_t>(CM_BLK, cnt - b * CM_BLK);
-    }
+    for (int64_t b = l; b < nbj; b += 64) cnode[b0j + b] = (int32_t)(t0 + j);
   }
 }
 
@@ -313,15 +308,16 @@ k_cm_plan(BNode *nodes, int64_t lb, int64_t le, CmBlock *blk, Counters *ctr) {
 // node off the integer path just as the node's total would.
 template <bool UNIT>
 __global__ void __launch_bounds__(64)
-k_cm_blk_stats(CmBlock *blk, const Counters *ctr, const float *X,
-               const float *Y, const float *Z, const float *M) {
+k_cm_blk_stats(const BNode *nodes, const int32_t *cnode, CmBlock *blk, const Counters *ctr,
+               const float *X, const float *Y, const float *Z, const float *M) {
   constexpr int NQ = UNIT ? 3 : 4;  // sums formed from the addends
   constexpr int RU = 4;             // elements per lane per load round
   const int l = lane_id();
   for (int64_t b = blockIdx.x; b < ctr->cm_blocks; b += gridDim.x) {
     CmBlock &B = blk[b];
-    const int64_t start = B.start;
-    const int len = (int)B.len;
+    int64_t start;
+    int len;
+    blk_range(nodes[cnode[b]], b, CM_BLK, start, len);
     float s[4] = {0.f, 0.f, 0.f, 0.f};
     float mn[3] = {INFINITY, INFINITY, INFINITY};
     float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -425,7 +421,7 @@ constexpr int PFX_T = 64 * PFX_WAVES;
 // of that lsb (saturating at 2^53: a chunk at the bound leaves its node
 // above it in any finer unit), kept in the leader.
 __global__ void __launch_bounds__(64)
-k_cm_chunk(const BNode *nodes, CmBlock *blk, const Counters *ctr) {
+k_cm_chunk(const BNode *nodes, const int32_t *cnode, CmBlock *blk, const Counters *ctr) {
   const int l = lane_id();
   const int64_t nblocks = ctr->cm_blocks;
   // a wave per 64-block range of the table, lane l testing block r + l; the
@@ -437,7 +433,7 @@ k_cm_chunk(const BNode *nodes, CmBlock *blk, const Counters *ctr) {
       const int64_t c = r + l;
       bool isl = false;
       if (c < nblocks) {
-        const BNode &nd = nodes[blk[c].node];
+        const BNode &nd = nodes[cnode[c]];
         isl = ((c - nd.blk0) & 63) == 0 && nd.nblk > 64;  // k_cm_node takes single chunks
       }
       lead = __ballot(isl);
@@ -445,7 +441,7 @@ k_cm_chunk(const BNode *nodes, CmBlock *blk, const Counters *ctr) {
     while (lead) {
       const int64_t c = r + __builtin_ctzll(lead);
       lead &= lead - 1;
-      const BNode &nd = nodes[blk[c].node];
+      const BNode &nd = nodes[cnode[c]];
       const int64_t rel = c - nd.blk0;
       const int nbc = (int)min<int64_t>(64, nd.nblk - rel);
       const bool v = l < nbc;
@@ -626,18 +622,20 @@ __device__ __forceinline__ int32_t grid_exp_guess(float ein) {
 // in LDS, each lane composes CM_PER consecutive addends in order, then the 64
 // lane maps are composed in lane order.
 __global__ void __launch_bounds__(64)
-k_cm_blk_maps(const BNode *nodes, CmBlock *blk, const Counters *ctr, bool unit,
-              const float *X,
-              const float *Y, const float *Z, const float *M, int64_t lb,
-              const int32_t *cmflags) {
+k_cm_blk_maps(const BNode *nodes, const int32_t *cnode, CmBlock *blk, const Counters *ctr,
+              bool unit, const float *X, const float *Y, const float *Z, const float *M,
+              int64_t lb, const int32_t *cmflags) {
   __shared__ float buf[64 * (CM_PER + 1)];
   const int q = blockIdx.y;  // one wave per (block, sum)
   for (int64_t b = blockIdx.x; b < ctr->cm_blocks; b += gridDim.x) {
     CmBlock &B = blk[b];
-    if (cmflags[B.node - lb] & (1 << q)) continue;  // summed exactly
+    const int64_t t = cnode[b];
+    if (cmflags[t - lb] & (1 << q)) continue;  // summed exactly
     const int l = lane_id();
-    const int64_t start = B.start, len = B.len;
-    const int64_t b0 = nodes[B.node].blk0;
+    int64_t start;
+    int len;
+    blk_range(nodes[t], b, CM_BLK, start, len);
+    const int64_t b0 = nodes[t].blk0;
     const float ein = B.ein[q] + blk[b0 + ((b - b0) & ~(int64_t)63)].cbase[q];
     const int32_t E = grid_exp_guess(ein);
     sd_map acc = sd_map_identity();
@@ -668,9 +666,9 @@ k_cm_blk_maps(const BNode *nodes, CmBlock *blk, const Counters *ctr, bool unit,
 // the window's block maps in order when they share one grid (one wave per
 // block; waves of other blocks return at once).
 __global__ void __launch_bounds__(64)
-k_cm_win_maps(const BNode *nodes, CmBlock *blk, const Counters *ctr) {
+k_cm_win_maps(const BNode *nodes, const int32_t *cnode, CmBlock *blk, const Counters *ctr) {
   for (int64_t b = blockIdx.x; b < ctr->cm_blocks; b += gridDim.x) {
-    const int64_t t = blk[b].node;
+    const int64_t t = cnode[b];
     const int64_t b0 = nodes[t].blk0, nb = nodes[t].nblk;
     if ((b - b0) % 64 != 0) continue;
     const int l = lane_id();
@@ -789,8 +787,9 @@ k_cm_blk_chain(BNode *nodes, int64_t lb, int64_t le, bool unit, const CmBlock *b
 #ifdef HSR_CM_STATS
           if (l == 0) atomicAdd((unsigned long long *)&ctr->pad[1], 1ull);
 #endif
-          const CmBlock &B = blk[b0 + bb + i];
-          const int64_t start = B.start, len = B.len;
+          int64_t start;
+          int len;
+          blk_range(nodes[t], b0 + bb + i, CM_BLK, start, len);
           __syncthreads();
           stage_addends<8>(buf, q, unit, X, Y, Z, M, start, len);
           __syncthreads();
@@ -913,7 +912,7 @@ k_finalize_leaves(const BNode *nodes, int64_t lb, int64_t le, int kind,
 // ------------------------------------------------------------ partition
 __global__ void __launch_bounds__(64)
 k_part_plan(BNode *nodes, int64_t lb, int64_t le, bool unit, const sd_t *cms, int64_t nd,
-            float ppc, bool unit_ppm, PartBlock *pb, Counters *ctr) {
+            float ppc, bool unit_ppm, int32_t *pnode, Counters *ctr) {
   // one lane per node: its centre of mass and class (cm_finish_node,
   // classify_node), then a wave scan of the block counts of the splitting
   // nodes and one atomic per wave reserve their partition blocks; then the
@@ -942,41 +941,29 @@ k_part_plan(BNode *nodes, int64_t lb, int64_t le, bool unit, const sd_t *cms, in
     nodes[t].nblk = nb;
   }
   // a few blocks by the node's own lane, longer tables by the whole wave
-  if (nb > 0 && nb <= 4) {
-    const int64_t off = nodes[t].offset, cnt = nodes[t].count;
-    for (int64_t b = 0; b < nb; ++b) {
-      PartBlock &B = pb[b0 + b];
-      B.node = t;
-      B.start = off + b * PB;
-      B.len = min<int64_t>(PB, cnt - b * PB);
-    }
-  }
+  if (nb > 0 && nb <= 4)
+    for (int64_t b = 0; b < nb; ++b) pnode[b0 + b] = (int32_t)t;
   for (uint64_t big = __ballot(nb > 4); big; big &= big - 1) {
     const int j = __builtin_ctzll(big);
     const int64_t nbj = (int64_t)__shfl((long long)nb, j);
     const int64_t b0j = (int64_t)__shfl((long long)b0, j);
-    const int64_t tj = t0 + j;
-    const int64_t off = nodes[tj].offset, cnt = nodes[tj].count;
-    for (int64_t b = l; b < nbj; b += 64) {
-      PartBlock &B = pb[b0j + b];
-      B.node = tj;
-      B.start = off + b * PB;
-      B.len = min<int64_t>(PB, cnt - b * PB);
-    }
+    for (int64_t b = l; b < nbj; b += 64) pnode[b0j + b] = (int32_t)(t0 + j);
   }
 }
 
 // Keys < pivot per block (RCBForceTree.cxx:638-645), pivot = xc[d].
 __global__ void __launch_bounds__(256)
-k_part_count(const BNode *nodes, PartBlock *pb, const Counters *ctr,
+k_part_count(const BNode *nodes, const int32_t *pnode, PartBlock *pb, const Counters *ctr,
              const float *X, const float *Y, const float *Z) {
   for (int64_t b = blockIdx.x; b < ctr->part_blocks; b += gridDim.x) {
-    const PartBlock &B = pb[b];
-    const BNode &n = nodes[B.node];
+    const BNode &n = nodes[pnode[b]];
+    int64_t bstart;
+    int blen;
+    blk_range(n, b, PB, bstart, blen);
     const float *key = coord(n.split_d, X, Y, Z);
     float pv = n.xc[n.split_d];
     int c = 0;
-    for (int k = threadIdx.x; k < B.len; k += 256) c += key[B.start + k] < pv;
+    for (int k = threadIdx.x; k < blen; k += 256) c += key[bstart + k] < pv;
     __shared__ int red[4];
     __syncthreads();  // thread 0 has read the previous block's red
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
@@ -1030,12 +1017,15 @@ k_part_scan(BNode *nodes, int64_t lb, int64_t le, PartBlock *pb) {
 // recorded as idx[r] = p; others at p >= is stay; others below is are
 // resolved by the swap chain (dest = -1 here).
 __global__ void __launch_bounds__(256)
-k_part_rank(const BNode *nodes, const PartBlock *pb, const Counters *ctr,
+k_part_rank(const BNode *nodes, const int32_t *pnode, const PartBlock *pb, const Counters *ctr,
             const float *X, const float *Y, const float *Z, int32_t *idx,
             int32_t *rights, int32_t *dest) {
   for (int64_t b = blockIdx.x; b < ctr->part_blocks; b += gridDim.x) {
     const PartBlock &B = pb[b];
-    const BNode &n = nodes[B.node];
+    const BNode &n = nodes[pnode[b]];
+    int64_t bstart;
+    int blen;
+    blk_range(n, b, PB, bstart, blen);
     const float *key = coord(n.split_d, X, Y, Z);
     float pv = n.xc[n.split_d];
     int64_t off = n.offset, is = n.is;
@@ -1044,8 +1034,8 @@ k_part_rank(const BNode *nodes, const PartBlock *pb, const Counters *ctr,
     int w = threadIdx.x >> 6;
     for (int r = 0; r < PB / 256; ++r) {
       int k = r * 256 + threadIdx.x;
-      bool valid = k < B.len;
-      int64_t e = B.start + k;
+      bool valid = k < blen;
+      int64_t e = bstart + k;
       bool f = valid && key[e] < pv;
       uint64_t bal = __ballot(f);
       int pre = __popcll(bal & lanemask_lt());
@@ -1080,12 +1070,14 @@ k_part_rank(const BNode *nodes, const PartBlock *pb, const Counters *ctr,
 // that entered such a stretch with step k continues f, f+k, f+2k, ... up to
 // rights[k]; the walk therefore jumps whole stretches arithmetically.
 __global__ void __launch_bounds__(256)
-k_part_chain(const BNode *nodes, const PartBlock *pb, const Counters *ctrc,
+k_part_chain(const BNode *nodes, const int32_t *pnode, const Counters *ctrc,
              const int32_t *idx, const int32_t *rights, int32_t *dest,
              Counters *ctr) {
   for (int64_t b = blockIdx.x; b < ctrc->part_blocks; b += gridDim.x) {
-    const PartBlock &B = pb[b];
-    const BNode &n = nodes[B.node];
+    const BNode &n = nodes[pnode[b]];
+    int64_t bstart;
+    int blen;
+    blk_range(n, b, PB, bstart, blen);
     // positions within a node are below 2^31 (int32 permutation), so the
     // stretch arithmetic runs in 32 bits (a 64-bit division is a long
     // software sequence)
@@ -1098,8 +1090,8 @@ k_part_chain(const BNode *nodes, const PartBlock *pb, const Counters *ctrc,
     // indices in its own element range, so every active lane holds a chain
     // (most elements start none) and neighbouring lanes start at nearby
     // positions, with similar chain lengths.
-    const int64_t r0 = B.start - off;
-    for (int k = threadIdx.x; k < B.len; k += 256) {
+    const int64_t r0 = bstart - off;
+    for (int k = threadIdx.x; k < blen; k += 256) {
       const int64_t r = r0 + k;
       if (r >= (int64_t)nr) break;
       const uint32_t p = (uint32_t)rights_n[r];
@@ -1136,26 +1128,30 @@ k_part_chain(const BNode *nodes, const PartBlock *pb, const Counters *ctrc,
 // Pointer-jumping fallback for long chains: jmp[off+j] = idx-iterate of j
 // advanced until >= is; log2 rounds resolve every chain.
 __global__ void __launch_bounds__(256)
-k_part_jump_init(const BNode *nodes, const PartBlock *pb, const Counters *ctr,
+k_part_jump_init(const BNode *nodes, const int32_t *pnode, const Counters *ctr,
                  const int32_t *idx, int32_t *jmp) {
   for (int64_t b = blockIdx.x; b < ctr->part_blocks; b += gridDim.x) {
-    const PartBlock &B = pb[b];
-    const BNode &n = nodes[B.node];
-    for (int k = threadIdx.x; k < B.len; k += 256) {
-      int64_t p = B.start + k - n.offset;
+    const BNode &n = nodes[pnode[b]];
+    int64_t bstart;
+    int blen;
+    blk_range(n, b, PB, bstart, blen);
+    for (int k = threadIdx.x; k < blen; k += 256) {
+      int64_t p = bstart + k - n.offset;
       if (p < n.is) jmp[n.offset + p] = idx[n.offset + p];
     }
   }
 }
 
 __global__ void __launch_bounds__(256)
-k_part_jump_round(const BNode *nodes, const PartBlock *pb,
+k_part_jump_round(const BNode *nodes, const int32_t *pnode,
                   const Counters *ctr, int32_t *jmp) {
   for (int64_t b = blockIdx.x; b < ctr->part_blocks; b += gridDim.x) {
-    const PartBlock &B = pb[b];
-    const BNode &n = nodes[B.node];
-    for (int k = threadIdx.x; k < B.len; k += 256) {
-      int64_t p = B.start + k - n.offset;
+    const BNode &n = nodes[pnode[b]];
+    int64_t bstart;
+    int blen;
+    blk_range(n, b, PB, bstart, blen);
+    for (int k = threadIdx.x; k < blen; k += 256) {
+      int64_t p = bstart + k - n.offset;
       if (p < n.is) {
         int32_t j = jmp[n.offset + p];
         if (j < n.is) jmp[n.offset + p] = jmp[n.offset + j];
@@ -1165,13 +1161,15 @@ k_part_jump_round(const BNode *nodes, const PartBlock *pb,
 }
 
 __global__ void __launch_bounds__(256)
-k_part_jump_final(const BNode *nodes, const PartBlock *pb,
+k_part_jump_final(const BNode *nodes, const int32_t *pnode,
                   const Counters *ctr, const int32_t *jmp, int32_t *dest) {
   for (int64_t b = blockIdx.x; b < ctr->part_blocks; b += gridDim.x) {
-    const PartBlock &B = pb[b];
-    const BNode &n = nodes[B.node];
-    for (int k = threadIdx.x; k < B.len; k += 256) {
-      int64_t e = B.start + k;
+    const BNode &n = nodes[pnode[b]];
+    int64_t bstart;
+    int blen;
+    blk_range(n, b, PB, bstart, blen);
+    for (int k = threadIdx.x; k < blen; k += 256) {
+      int64_t e = bstart + k;
       int32_t d = dest[e];
       if (d <= -2) dest[e] = jmp[n.offset + (-2 - d)];
     }
@@ -1179,15 +1177,18 @@ k_part_jump_final(const BNode *nodes, const PartBlock *pb,
 }
 
 __global__ void __launch_bounds__(256)
-k_part_scatter(const BNode *nodes, const PartBlock *pb, const Counters *ctr, bool unit, bool ident,
+k_part_scatter(const BNode *nodes, const int32_t *pnode, const Counters *ctr, bool unit, bool ident,
                const int32_t *dest, const float *X, const float *Y,
                const float *Z, const float *M, const int32_t *P, float *X2,
                float *Y2, float *Z2, float *M2, int32_t *P2) {
   for (int64_t b = blockIdx.x; b < ctr->part_blocks; b += gridDim.x) {
-    const PartBlock &B = pb[b];
-    int64_t off = nodes[B.node].offset;
-    for (int k = threadIdx.x; k < B.len; k += 256) {
-      int64_t e = B.start + k;
+    const BNode &n = nodes[pnode[b]];
+    int64_t bstart;
+    int blen;
+    blk_range(n, b, PB, bstart, blen);
+    const int64_t off = n.offset;
+    for (int k = threadIdx.x; k < blen; k += 256) {
+      int64_t e = bstart + k;
       int64_t o = off + dest[e];
       X2[o] = X[e];
       Y2[o] = Y[e];
@@ -1640,20 +1641,23 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
                                                      cms, cm_list, ctr);
     k_cm_small_ordered<<<std::min<unsigned>((unsigned)std::max<int64_t>(1, nl), 4 * gbig), 64, 0, st>>>(
         nodes, lb, unit, Xr, Yr, Zr, Mr, cms, cm_list, ctr);
-    k_cm_plan<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, cmb, ctr);
+    HSR_HIP(dbuf_reserve(S.cnode, sizeof(int32_t) * cm_cap));
+    int32_t *cnode = dptr<int32_t>(S.cnode);
+    k_cm_plan<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, cnode, ctr);
     unsigned cmg = (unsigned)std::max<int64_t>(1, active / CM_BLK + active / CM_SMALL + 1);
     cmg = std::min(cmg, gbig);
     if (unit)
-      k_cm_blk_stats<true><<<cmg, 64, 0, st>>>(cmb, ctr, Xr, Yr, Zr, Mr);
+      k_cm_blk_stats<true><<<cmg, 64, 0, st>>>(nodes, cnode, cmb, ctr, Xr, Yr, Zr, Mr);
     else
-      k_cm_blk_stats<false><<<cmg, 64, 0, st>>>(cmb, ctr, Xr, Yr, Zr, Mr);
+      k_cm_blk_stats<false><<<cmg, 64, 0, st>>>(nodes, cnode, cmb, ctr, Xr, Yr, Zr, Mr);
     HSR_HIP(dbuf_reserve(S.cmx, sizeof(int32_t) * (size_t)nl));
     int32_t *cmflags = dptr<int32_t>(S.cmx);
-    k_cm_chunk<<<std::max(1u, std::min<unsigned>(gbig, (unsigned)(cm_cap / 64 + 1))), 64, 0, st>>>(nodes, cmb, ctr);
+    const unsigned cmr = std::max(1u, std::min<unsigned>(gbig, (unsigned)(cm_cap / 64 + 1)));
+    k_cm_chunk<<<cmr, 64, 0, st>>>(nodes, cnode, cmb, ctr);
     k_cm_node<<<gnl, 64, 0, st>>>(nodes, lb, le, cmb, cmflags, cms);
-    k_cm_blk_maps<<<dim3(cmg, nq), 64, 0, st>>>(nodes, cmb, ctr, unit, Xr, Yr, Zr, Mr, lb,
+    k_cm_blk_maps<<<dim3(cmg, nq), 64, 0, st>>>(nodes, cnode, cmb, ctr, unit, Xr, Yr, Zr, Mr, lb,
                                                 cmflags);
-    k_cm_win_maps<<<cmg, 64, 0, st>>>(nodes, cmb, ctr);
+    k_cm_win_maps<<<cmg, 64, 0, st>>>(nodes, cnode, cmb, ctr);
     k_cm_blk_chain<<<dim3(gnl, nq), 64, 0, st>>>(nodes, lb, le, unit, cmb, Xr, Yr, Zr, Mr, ctr, cms,
                                                  cmflags);
     // ---- centre of mass, class and partition blocks (a lane per node),
@@ -1661,12 +1665,14 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     int64_t pcap = active / PB + nl + 2;
     HSR_HIP(dbuf_reserve(S.pblocks, sizeof(PartBlock) * pcap));
     PartBlock *pb = dptr<PartBlock>(S.pblocks);
-    k_part_plan<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, unit, cms, nd, c->p.ppc, unit_ppm, pb, ctr);
+    HSR_HIP(dbuf_reserve(S.pnode, sizeof(int32_t) * pcap));
+    int32_t *pnode = dptr<int32_t>(S.pnode);
+    k_part_plan<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, unit, cms, nd, c->p.ppc, unit_ppm, pnode, ctr);
     if (!unit_ppm) k_leaf_ppm<<<gnl, 64, 0, st>>>(nodes, lb, le, nd, Mr);
     k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_LEAF, depth == 0, P[cur], Pfin);
     // ---- partition of splitting nodes
     const unsigned pg = (unsigned)std::max<int64_t>(1, pcap);
-    k_part_count<<<pg, 256, 0, st>>>(nodes, pb, ctr, Xr, Yr, Zr);
+    k_part_count<<<pg, 256, 0, st>>>(nodes, pnode, pb, ctr, Xr, Yr, Zr);
     if (active / PB > 64 * nl)  // over 64 blocks per node on average
       k_part_scan<PFX_WAVES><<<std::min<unsigned>((unsigned)nl, 4u * (unsigned)std::max(1, c->num_cus)), PFX_T,
                                0, st>>>(nodes, lb, le, pb);
@@ -1675,17 +1681,17 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     // children depend only on the partition counts: create them now so a
     // single host synchronisation per level reads every counter
     k_make_children<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, le, ctr, ctr2 + ((depth + 1) & 1));
-    k_part_rank<<<pg, 256, 0, st>>>(nodes, pb, ctr, Xr, Yr, Zr, idx, rights, dest);
-    k_part_chain<<<pg, 256, 0, st>>>(nodes, pb, ctr, idx, rights, dest, ctr);
+    k_part_rank<<<pg, 256, 0, st>>>(nodes, pnode, pb, ctr, Xr, Yr, Zr, idx, rights, dest);
+    k_part_chain<<<pg, 256, 0, st>>>(nodes, pnode, ctr, idx, rights, dest, ctr);
     HSR_HIP(hipMemcpyAsync(&hc, ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
     HSR_HIP(stream_wait(st));
     if (hc.unresolved > 0) {
-      k_part_jump_init<<<pg, 256, 0, st>>>(nodes, pb, ctr, idx, jmp);
+      k_part_jump_init<<<pg, 256, 0, st>>>(nodes, pnode, ctr, idx, jmp);
       for (int r = 0; r < 32; ++r)
-        k_part_jump_round<<<pg, 256, 0, st>>>(nodes, pb, ctr, jmp);
-      k_part_jump_final<<<pg, 256, 0, st>>>(nodes, pb, ctr, jmp, dest);
+        k_part_jump_round<<<pg, 256, 0, st>>>(nodes, pnode, ctr, jmp);
+      k_part_jump_final<<<pg, 256, 0, st>>>(nodes, pnode, ctr, jmp, dest);
     }
-    k_part_scatter<<<pg, 256, 0, st>>>(nodes, pb, ctr, unit, depth == 0, dest, Xr, Yr, Zr, Mr, P[cur],
+    k_part_scatter<<<pg, 256, 0, st>>>(nodes, pnode, ctr, unit, depth == 0, dest, Xr, Yr, Zr, Mr, P[cur],
                                         X[cur ^ 1], Y[cur ^ 1], Z[cur ^ 1], Mm[cur ^ 1], P[cur ^ 1]);
     k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_DEGEN, false, P[cur ^ 1], Pfin);
     HSR_HIP(hipGetLastError());
