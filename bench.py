@@ -530,6 +530,7 @@ def main():
     for _ in range(a.warmup):
         step(nd)
     torch.cuda.synchronize()
+    g0 = ctxs[nd].stats() if a.warmup > 0 else None
     stats = []
     t0 = time.perf_counter()
     for _ in range(a.steps):
@@ -573,7 +574,13 @@ def main():
                    "breakdown_ms": {"build": float(np.mean([s[1] for s in stats])),
                                     "lists": float(np.mean([s[2] for s in stats])),
                                     "kick": float(np.mean([s[3] for s in stats])),
-                                    "leaf_kernel": kern_ms}},
+                                    "leaf_kernel": kern_ms},
+                   # the force phase is one captured graph, replayed across
+                   # the steps' list builds
+                   "force_graph": {"captures_in_timed_steps":
+                                   int(st.graph_captures - (g0.graph_captures if g0 else 0)),
+                                   "replays_in_timed_steps":
+                                   int(st.graph_replays - (g0.graph_replays if g0 else 0))}},
         "roofline": {"bound": "valu", "pipe": "FP32 VALU (no MFMA: the pair force is not a contraction)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,

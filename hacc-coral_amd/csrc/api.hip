@@ -339,14 +339,23 @@ void hsr_set_error(const std::string &msg) { g_err = msg; }
 
 hipError_t dbuf_reserve(DBuf &b, size_t bytes) {
   if (bytes <= b.bytes && b.p) return hipSuccess;
+  // a buffer that grows again takes a quarter more than asked, so slowly
+  // growing sizes (list lengths from step to step) settle on one allocation
+  // and its address (which a captured force phase holds)
+  size_t want = std::max<size_t>(bytes, 256);
   if (b.p) {
+    want = std::max(want, b.bytes + b.bytes / 4);
     hipError_t e = hipFree(b.p);
     if (e != hipSuccess) return e;
     b.p = nullptr;
     b.bytes = 0;
   }
-  size_t want = std::max<size_t>(bytes, 256);
   hipError_t e = hipMalloc(&b.p, want);
+  if (e != hipSuccess && want > bytes) {  // no room for the slack: the size asked
+    (void)hipGetLastError();
+    want = std::max<size_t>(bytes, 256);
+    e = hipMalloc(&b.p, want);
+  }
   if (e != hipSuccess) {
     b.p = nullptr;
     return e;
@@ -864,6 +873,8 @@ hsr_status hsr_get_stats(hsr_ctx *c, hsr_stats *st) {
   st->pairs = c->lists.pairs;
   st->evaluated_pairs = c->evaluated_pairs;
   st->unmasked_pairs = c->unmasked_pairs;
+  st->graph_captures = c->graph_captures;
+  st->graph_replays = c->graph_launches - c->graph_captures;
   st->build_ms = c->build_ms;
   st->lists_ms = c->lists_ms;
   st->force_ms = c->force_ms;

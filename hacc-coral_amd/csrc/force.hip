@@ -419,25 +419,27 @@ k_reduce_chunks(const hsr_node *cn, const int32_t *leaves,
                 const int64_t *multi, const int64_t *nmulti_p,
                 const int64_t *leaf_pbase, const int32_t *leaf_nchunk,
                 const int32_t *iperm, const float *part, float *fx, float *fy, float *fz) {
-  const int64_t k = blockIdx.x;
-  if (k >= *nmulti_p) return;
-  const int64_t lf = multi[k];
-  const hsr_node &nd = cn[leaves[lf]];
-  const int64_t cnt = nd.count, off = nd.offset;
-  const int nc = leaf_nchunk[lf];
-  const float *p0 = part + leaf_pbase[lf];
-  for (int64_t i = threadIdx.x; i < cnt; i += 64) {
-    float sx = 0.f, sy = 0.f, sz = 0.f;
-    for (int c = 0; c < nc; ++c) {
-      const float *p = p0 + (int64_t)c * 3 * cnt + 3 * i;
-      sx += p[0];
-      sy += p[1];
-      sz += p[2];
+  // a fixed grid over the device-resident count, so a captured force phase
+  // serves every list build
+  for (int64_t k = blockIdx.x; k < *nmulti_p; k += gridDim.x) {
+    const int64_t lf = multi[k];
+    const hsr_node &nd = cn[leaves[lf]];
+    const int64_t cnt = nd.count, off = nd.offset;
+    const int nc = leaf_nchunk[lf];
+    const float *p0 = part + leaf_pbase[lf];
+    for (int64_t i = threadIdx.x; i < cnt; i += 64) {
+      float sx = 0.f, sy = 0.f, sz = 0.f;
+      for (int c = 0; c < nc; ++c) {
+        const float *p = p0 + (int64_t)c * 3 * cnt + 3 * i;
+        sx += p[0];
+        sy += p[1];
+        sz += p[2];
+      }
+      const int32_t q = iperm[off + i];  // partials are kept in target order
+      fx[q] = sx;
+      fy[q] = sy;
+      fz[q] = sz;
     }
-    const int32_t q = iperm[off + i];  // partials are kept in target order
-    fx[q] = sx;
-    fy[q] = sy;
-    fz[q] = sz;
   }
 }
 
@@ -677,17 +679,20 @@ hsr_status force_plan(hsr_ctx *c) {
     LS.nitems = 0;
     return HSR_OK;
   }
-  // scratch layout (int64 unless noted)
-  HSR_HIP(dbuf_reserve(LS.plan, sizeof(int64_t) * 8 * (nl + 1) + sizeof(int32_t) * (nl + 1) + 256));
+  // scratch layout (int64 unless noted), arrays at a stride that only grows,
+  // so their addresses (held by a captured force phase) survive list builds
+  if (nl + 1 > LS.plan_stride) LS.plan_stride = std::max<int64_t>(nl + 1, LS.plan_stride + LS.plan_stride / 4);
+  const int64_t ps = LS.plan_stride;
+  HSR_HIP(dbuf_reserve(LS.plan, sizeof(int64_t) * 8 * ps + sizeof(int32_t) * ps + 256));
   int64_t *nitem = dptr<int64_t>(LS.plan);
-  int64_t *item_base = nitem + (nl + 1);
-  int64_t *mflag = item_base + (nl + 1);
-  int64_t *mpos = mflag + (nl + 1);
-  int64_t *psize = mpos + (nl + 1);
-  int64_t *pbase = psize + (nl + 1);
-  int64_t *multi = pbase + (nl + 1);
-  int64_t *counts = multi + (nl + 1);  // [0] nitems, [1] nmulti, [2] partial floats
-  int32_t *nchunk = reinterpret_cast<int32_t *>(counts + (nl + 1));
+  int64_t *item_base = nitem + ps;
+  int64_t *mflag = item_base + ps;
+  int64_t *mpos = mflag + ps;
+  int64_t *psize = mpos + ps;
+  int64_t *pbase = psize + ps;
+  int64_t *multi = pbase + ps;
+  int64_t *counts = multi + ps;  // [0] nitems, [1] nmulti, [2] partial floats
+  int32_t *nchunk = reinterpret_cast<int32_t *>(counts + ps);
   const hsr_node *cn = dptr<hsr_node>(c->tree.canon_nodes);
   const int32_t *leaves = dptr<int32_t>(LS.leaves);
   const int64_t *ent_ptr = dptr<int64_t>(LS.ent_ptr);
@@ -764,8 +769,11 @@ static hsr_status force_enqueue(hsr_ctx *c, float *fx, float *fy, float *fz,
                                 bool timed);
 
 // The force phase: zero F, reset the work queue, persistent leaf kernel,
-// chunk reduction.  With tuning.use_graph the four launches are captured
-// once per list build (and output buffers) and replayed as one graph.
+// chunk reduction.  With tuning.use_graph the launches are captured once and
+// replayed as one graph across list builds: their grids are fixed (the
+// persistent kernel and the chunk reduction read their counts from device
+// memory), so a new capture is needed only when an address they use, the
+// particle count or the settings change.
 hsr_status force_run(hsr_ctx *c, float *fx, float *fy, float *fz) {
   hipStream_t st = c->stream;
   if (c->law.kind == LAWK_INTERP && !c->law.bins) {
@@ -773,9 +781,16 @@ hsr_status force_run(hsr_ctx *c, float *fx, float *fy, float *fz) {
     return HSR_ERR_STATE;
   }
   if (!c->tuning.use_graph) return force_enqueue(c, fx, fy, fz, true);
-  const bool stale = c->graph_exec == nullptr || c->graph_gen != c->lists_gen ||
-                     c->graph_out[0] != fx || c->graph_out[1] != fy ||
-                     c->graph_out[2] != fz;
+  const ListState &LS = c->lists;
+  const uintptr_t sig[20] = {(uintptr_t)fx, (uintptr_t)fy, (uintptr_t)fz, (uintptr_t)c->tree.n,
+                             (uintptr_t)(LS.nleaves == 0 || LS.nitems == 0), (uintptr_t)c->tree.canon_nodes.p,
+                             (uintptr_t)LS.leaves.p, (uintptr_t)LS.items.p, (uintptr_t)LS.counts,
+                             (uintptr_t)LS.jidx.p, (uintptr_t)c->tree.pj.p, (uintptr_t)LS.iperm.p,
+                             (uintptr_t)LS.part.p, (uintptr_t)LS.multi, (uintptr_t)LS.leaf_pbase,
+                             (uintptr_t)LS.leaf_nchunk, (uintptr_t)c->s.qctr.p, 0, 0, 0};
+  bool same = c->graph_exec != nullptr && c->graph_gen != -1;
+  for (int k = 0; k < 20 && same; ++k) same = c->graph_sig[k] == sig[k];
+  const bool stale = !same;
   if (stale) {
     if (c->graph_exec) {
       HSR_HIP(hipGraphExecDestroy(c->graph_exec));
@@ -796,10 +811,9 @@ hsr_status force_run(hsr_ctx *c, float *fx, float *fy, float *fz) {
     HSR_HIP(e);
     c->graph = g;
     HSR_HIP(hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
-    c->graph_gen = c->lists_gen;
-    c->graph_out[0] = fx;
-    c->graph_out[1] = fy;
-    c->graph_out[2] = fz;
+    c->graph_gen = 0;
+    for (int k = 0; k < 20; ++k) c->graph_sig[k] = sig[k];
+    c->graph_captures++;
   }
   HSR_HIP(hipEventRecord(c->ev[6], st));
   HSR_HIP(hipGraphLaunch(c->graph_exec, st));
@@ -828,7 +842,9 @@ static hsr_status force_enqueue(hsr_ctx *c, float *fx, float *fy, float *fz,
   force_shape(c->tuning, &R, &U, &T);
   A.pmax = c->tuning.force_pass == 32 * R ? 32 * R : 64 * R;
   const int wpc = c->tuning.waves_per_cu > 0 ? c->tuning.waves_per_cu : 32;
-  const int64_t grid = std::min<int64_t>(LS.nitems, (int64_t)std::max(1, c->num_cus) * wpc);
+  // the persistent grid does not depend on the item count (waves beyond the
+  // items leave at their first pop), so a captured launch fits any list
+  const int64_t grid = (int64_t)std::max(1, c->num_cus) * wpc;
   const hsr_node *cn = dptr<hsr_node>(c->tree.canon_nodes);
   const int32_t *leaves = dptr<int32_t>(LS.leaves);
   const Item *items = dptr<Item>(LS.items);
@@ -868,10 +884,10 @@ static hsr_status force_enqueue(hsr_ctx *c, float *fx, float *fy, float *fz,
     HSR_HIP(hipEventRecord(c->ev[7], st));
     c->kernel_timed = true;
   }
-  if (LS.nmulti > 0)
-    k_reduce_chunks<<<(unsigned)LS.nmulti, 64, 0, st>>>(cn, leaves, LS.multi, LS.counts + 1, LS.leaf_pbase,
-                                                       LS.leaf_nchunk, dptr<int32_t>(LS.iperm), part,
-                                                       fx, fy, fz);
+  // a fixed grid over the device-resident multi-chunk count (which may be 0)
+  k_reduce_chunks<<<(unsigned)std::max(1, c->num_cus) * 4, 64, 0, st>>>(
+      cn, leaves, LS.multi, LS.counts + 1, LS.leaf_pbase, LS.leaf_nchunk, dptr<int32_t>(LS.iperm), part,
+      fx, fy, fz);
   HSR_HIP(hipGetLastError());
   return HSR_OK;
 }

@@ -137,6 +137,7 @@ struct ListState {
   DBuf plan;                  // per-leaf chunk plan
   DBuf part;                  // chunk partial forces of long lists
   int64_t nitems = 0, nmulti = 0, chunk = 0;
+  int64_t plan_stride = 0;    // per-leaf plan arrays' stride (grows only)
   int64_t *multi = nullptr, *counts = nullptr, *leaf_pbase = nullptr;
   int32_t *leaf_nchunk = nullptr;
   DBuf stack;                 // DFS stacks
@@ -224,9 +225,9 @@ struct hsr_ctx {
   int64_t unmasked_pairs = 0;   // of which unit-mass entries inside the cutoff
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
-  int64_t graph_gen = -1;
-  const float *graph_out[3] = {nullptr, nullptr, nullptr};
-  int64_t graph_launches = 0;
+  int64_t graph_gen = -1;       // -1: the captured force phase is stale
+  uintptr_t graph_sig[20] = {}; // what the captured force phase was built on
+  int64_t graph_captures = 0, graph_launches = 0;
 };
 
 void hsr_set_error(const std::string &msg);

@@ -86,7 +86,8 @@ class Stats(C.Structure):
                 ("drift_factor", C.c_double), ("kick_coeff", C.c_double),
                 ("cic_ms", C.c_double), ("inverse_cic_ms", C.c_double),
                 ("refresh_ms", C.c_double), ("refresh_alive", C.c_int64),
-                ("refresh_dead", C.c_int64), ("unmasked_pairs", C.c_int64)]
+                ("refresh_dead", C.c_int64), ("unmasked_pairs", C.c_int64),
+                ("graph_captures", C.c_int64), ("graph_replays", C.c_int64)]
 
 
 class Domain(C.Structure):

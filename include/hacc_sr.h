@@ -185,6 +185,10 @@ typedef struct {
   int64_t unmasked_pairs; /* of evaluated_pairs, those of unit-mass entries
                              whose every pair lies inside the cutoff (the leaf
                              kernel skips their mask and mass) */
+  int64_t graph_captures; /* force-phase graphs captured (tuning.use_graph):
+                             a new one only when a buffer address, the
+                             particle count or the settings change */
+  int64_t graph_replays;  /* force phases run as a replay of such a graph */
 } hsr_stats;
 
 hsr_status hsr_get_stats(hsr_ctx *ctx, hsr_stats *st);

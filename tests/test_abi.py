@@ -62,6 +62,7 @@ int main(void) {
          offsetof(hsr_domain, grid2phys_vel), offsetof(hsr_stats, refresh_dead));
   printf("%zu %zu %zu\n", offsetof(hsr_stats, unmasked_pairs), offsetof(hsr_tuning, unroll),
          offsetof(hsr_params, n_interp));
+  printf("%zu %zu\n", offsetof(hsr_stats, graph_captures), offsetof(hsr_stats, graph_replays));
   return 0;
 }
 """)
@@ -78,7 +79,8 @@ int main(void) {
             C.sizeof(_lib.Domain), _lib.Domain.ng_local_total.offset,
             _lib.Domain.grid2phys_vel.offset, _lib.Stats.refresh_dead.offset,
             _lib.Stats.unmasked_pairs.offset, _lib.Tuning.unroll.offset,
-            _lib.Params.n_interp.offset]
+            _lib.Params.n_interp.offset, _lib.Stats.graph_captures.offset,
+            _lib.Stats.graph_replays.offset]
     assert got == want
     assert _lib.NODE_DTYPE.itemsize == C.sizeof(_lib.Node)
 

@@ -426,6 +426,48 @@ def test_graph_replay_matches_eager():
     say(f"  replays ok; after re-listing rel rms {e2:.3e}")
 
 
+def test_graph_replayed_across_list_builds():
+    """The captured force phase is replayed, not captured again, after a new
+    build and list pass of the same particle set (fixed grids, device-side
+    counts, buffer addresses unchanged), and every replay equals the eager
+    force phase bitwise; moved particles (new tree, new lists) replay too
+    when their buffers fit, and still equal the eager result and the
+    oracle."""
+    say("case force-phase graph across list builds")
+    x, y, z, m = uniform(40000, 16.0, 11)
+    lo, hi, flo, fhi = [0, 0, 0], [16.0] * 3, [1.0] * 3, [15.0] * 3
+    tp, tree, lists, ctx, arrs = run_both(x, y, z, m, lo, hi, flo, fhi, 96)
+    eager = np.stack(ctx.forces(), 1).view(np.uint32).copy()
+    ctx.set_tuning(use_graph=1)
+    first = np.stack(ctx.forces(), 1).view(np.uint32).copy()
+    assert np.array_equal(first, eager)
+    s0 = ctx.stats()
+    for rep in range(3):
+        X, Y, Z, M = x.copy(), y.copy(), z.copy(), m.copy()
+        ctx.build(X, Y, Z, M, min_loc=lo, max_loc=hi)
+        ctx.lists(flo, fhi)
+        got = np.stack(ctx.forces(), 1).view(np.uint32)
+        assert np.array_equal(got, eager), f"replay {rep} after a rebuild differs"
+    s1 = ctx.stats()
+    say(f"  captures {s0.graph_captures} -> {s1.graph_captures}, replays {s0.graph_replays} -> {s1.graph_replays}")
+    assert s1.graph_captures == s0.graph_captures, "a rebuild of the same particles captured again"
+    assert s1.graph_replays == s0.graph_replays + 3
+    # moved particles: new tree and lists, results as eager and as the oracle
+    rng = np.random.default_rng(5)
+    x2 = np.clip(x + rng.normal(0, 0.05, x.shape).astype(np.float32), 0, np.float32(15.999))
+    X2, Y2, Z2, M2 = x2.copy(), y.copy(), z.copy(), m.copy()
+    ctx.build(X2, Y2, Z2, M2, min_loc=lo, max_loc=hi)
+    ctx.lists(flo, fhi)
+    g2 = np.stack(ctx.forces(), 1).view(np.uint32).copy()
+    ctx.set_tuning(use_graph=0)
+    e2 = np.stack(ctx.forces(), 1).view(np.uint32)
+    assert np.array_equal(g2, e2)
+    tree2 = O.build_tree(tp, x2.copy(), y.copy(), z.copy(), m.copy())
+    lists2 = O.build_lists(tp, tree2)
+    err = check_forces(tp, tree2, lists2, ctx)
+    say(f"  moved particles: graph = eager bitwise, rel rms vs oracle {err:.3e}")
+
+
 def test_culling_and_target_order_change_rounding_only():
     """Skipping list entries beyond the cutoff of a pass's targets, and
     sorting a leaf's targets along its longest edge, leave every pair that
