@@ -768,6 +768,12 @@ hsr_status force_plan(hsr_ctx *c) {
 static hsr_status force_enqueue(hsr_ctx *c, float *fx, float *fy, float *fz,
                                 bool timed);
 
+static unsigned reduce_grid(const hsr_ctx *c) {
+  unsigned g = 256;
+  while ((int64_t)g < c->lists.nmulti) g <<= 1;
+  return g;
+}
+
 // The force phase: zero F, reset the work queue, persistent leaf kernel,
 // chunk reduction.  With tuning.use_graph the launches are captured once and
 // replayed as one graph across list builds: their grids are fixed (the
@@ -787,7 +793,7 @@ hsr_status force_run(hsr_ctx *c, float *fx, float *fy, float *fz) {
                              (uintptr_t)LS.leaves.p, (uintptr_t)LS.items.p, (uintptr_t)LS.counts,
                              (uintptr_t)LS.jidx.p, (uintptr_t)c->tree.pj.p, (uintptr_t)LS.iperm.p,
                              (uintptr_t)LS.part.p, (uintptr_t)LS.multi, (uintptr_t)LS.leaf_pbase,
-                             (uintptr_t)LS.leaf_nchunk, (uintptr_t)c->s.qctr.p, 0, 0, 0};
+                             (uintptr_t)LS.leaf_nchunk, (uintptr_t)c->s.qctr.p, (uintptr_t)reduce_grid(c), 0, 0};
   bool same = c->graph_exec != nullptr && c->graph_gen != -1;
   for (int k = 0; k < 20 && same; ++k) same = c->graph_sig[k] == sig[k];
   const bool stale = !same;
@@ -884,8 +890,10 @@ static hsr_status force_enqueue(hsr_ctx *c, float *fx, float *fy, float *fz,
     HSR_HIP(hipEventRecord(c->ev[7], st));
     c->kernel_timed = true;
   }
-  // a fixed grid over the device-resident multi-chunk count (which may be 0)
-  k_reduce_chunks<<<(unsigned)std::max(1, c->num_cus) * 4, 64, 0, st>>>(
+  // a grid over the device-resident multi-chunk count (which may be 0): a
+  // wave per multi-chunk leaf, the grid rounded up to a power of two so a
+  // captured force phase serves later lists unless their count outgrows it
+  k_reduce_chunks<<<reduce_grid(c), 64, 0, st>>>(
       cn, leaves, LS.multi, LS.counts + 1, LS.leaf_pbase, LS.leaf_nchunk, dptr<int32_t>(LS.iperm), part,
       fx, fy, fz);
   HSR_HIP(hipGetLastError());

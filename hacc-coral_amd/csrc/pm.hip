@@ -137,19 +137,37 @@ constexpr int RUN_SHORT = 64;
 // most RUN_SHORT are added in order here; longer runs are queued for a wave
 // each, except the last run of the sorted order, which is the outside slot
 // whenever any corner fell outside the grid -- typically the longest run by
-// far -- and goes to a workgroup of its own.
-__global__ void __launch_bounds__(256)
+// far -- and goes to a workgroup of its own.  Each wave owns 64 positions
+// and stages their keys and terms in LDS with coalesced loads; a head
+// thread adds its run from LDS, and only the one run that crosses the end
+// of the window reads its remaining terms from memory.
+constexpr int RUNS_WAVES = 4;
+__global__ void __launch_bounds__(64 * RUNS_WAVES)
 k_cic_runs(int64_t m, const uint32_t *__restrict__ key, const double *__restrict__ ts,
            float *__restrict__ rho, int64_t *__restrict__ long_runs,
            unsigned long long *__restrict__ nlong, int64_t *__restrict__ tail_start) {
-  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  __shared__ uint32_t sk[RUNS_WAVES][64];
+  __shared__ double sv[RUNS_WAVES][64];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int64_t w0 = ((int64_t)blockIdx.x * RUNS_WAVES + w) * 64;
+  const int64_t p = w0 + l;
+  // past the end: a key no cell has (cells and the outside slot are below
+  // 2^31)
+  const uint32_t cell = p < m ? key[p] : 0xffffffffu;
+  sk[w][l] = cell;
+  sv[w][l] = p < m ? ts[p] : 0.0;
+  __syncthreads();
+  // run ends in the window, by ballot: lane l's run ends at the first end at
+  // or after l
+  const bool endm = l == 63 || sk[w][l + 1 < 64 ? l + 1 : 63] != cell;
+  const uint64_t ends = __ballot(endm);
   if (p >= m) return;
-  const uint32_t cell = key[p];
-  if (p > 0 && key[p - 1] == cell) return;
-  const int64_t lim = min(m, p + RUN_SHORT + 1);
-  int64_t e = p;
-  while (e < lim && key[e] == cell) ++e;
-  if (e - p > RUN_SHORT) {
+  if (p > 0 && (l > 0 ? sk[w][l - 1] : key[p - 1]) == cell) return;
+  const int e = l + 1 + __builtin_ctzll(ends >> l);
+  int64_t eg = w0 + e;  // the run continues past the window: from memory
+  if (e == 64)
+    while (eg < m && eg < p + RUN_SHORT + 1 && key[eg] == cell) ++eg;
+  if (eg - p > RUN_SHORT) {
     if (key[m - 1] == cell) {
       // the last run (the outside slot, or the largest cell): a workgroup
       *tail_start = p;
@@ -159,8 +177,17 @@ k_cic_runs(int64_t m, const uint32_t *__restrict__ key, const double *__restrict
     long_runs[slot] = p;
     return;
   }
+  // in order, eight terms read from LDS at a time
   float r = 0.f;
-  for (int64_t q = p; q < e; ++q) r = cic_add(r, ts[q]);
+  for (int q = l; q < e; q += 8) {
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = sv[w][min(q + k, 63)];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (q + k < e) r = cic_add(r, v[k]);
+  }
+  for (int64_t q = w0 + 64; q < eg; ++q) r = cic_add(r, ts[q]);
   rho[cell] = r;
 }
 
@@ -209,15 +236,26 @@ __device__ __forceinline__ PMap pmap_term(double t, int E) {
   return m;
 }
 
-// One wave per long run: the lanes load 64 consecutive contributions at a
-// time.  With all terms nonnegative (the deposit's weights are) the wave
-// applies them by the map scan above; otherwise it adds them in order,
-// reading lane i's term with v_readlane.  Every lane carries the same sum.
-// Waves loop over the queue; each exits when the queue is drained.
+// One wave per long run: the lanes load LT consecutive contributions each,
+// 64 LT at a time.  With all terms nonnegative (the deposit's weights are)
+// the wave applies them by the map scan above -- each lane composes its LT
+// terms in order, then the lanes' maps are scanned -- and applies the
+// longest prefix that stays in the binade; in the lane where the sum leaves
+// it the terms are stepped one at a time to the crossing term, which is
+// added directly.  Otherwise the terms are added in order, reading them
+// with v_readlane.  Every lane carries the same sum.  Waves loop over the
+// queue; each exits when the queue is drained.
+constexpr int LT = 4;
+
+__device__ __forceinline__ double pick(const double (&t)[LT], int k) {
+  return k == 0 ? t[0] : k == 1 ? t[1] : k == 2 ? t[2] : t[3];
+}
+
 __global__ void __launch_bounds__(64)
 k_cic_long(int64_t m, const uint32_t *__restrict__ key, const double *__restrict__ ts,
            float *__restrict__ rho, const int64_t *__restrict__ long_runs,
            const unsigned long long *__restrict__ nlong) {
+  static_assert(LT == 4, "pick() reads four terms per lane");
   const int l = threadIdx.x;
   const int64_t nruns = (int64_t)*nlong;
   for (int64_t run = blockIdx.x; run < nruns; run += gridDim.x) {
@@ -225,29 +263,43 @@ k_cic_long(int64_t m, const uint32_t *__restrict__ key, const double *__restrict
     const uint32_t cell = key[p0];
     float r = 0.f;
     bool done = false;
-    for (int64_t base = p0; !done; base += 64) {
-      const int64_t q = base + l;
-      const bool in = q < m && key[q] == cell;
-      const double t = in ? ts[q] : 0.0;
-      // contributions of one cell are contiguous: the live lanes are a prefix
-      const int cnt = __popcll(__ballot(in));
-      done = cnt < 64;
-      if (__ballot(in && !(t >= 0.0)) != 0ull) {
-        for (int i = 0; i < cnt; ++i) r = cic_add(r, readlane_f64(t, i));
+    for (int64_t base = p0; !done; base += 64 * LT) {
+      double t[LT];
+      int mine = 0;
+      bool neg = false;
+#pragma unroll
+      for (int k = 0; k < LT; ++k) {
+        const int64_t q = base + LT * l + k;
+        const bool in = q < m && key[q] == cell;
+        t[k] = in ? ts[q] : 0.0;
+        mine += in;
+        neg = neg || (in && !(t[k] >= 0.0));
+      }
+      // contributions of one cell are contiguous: the live terms are a prefix
+      int cnt = mine;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+      done = cnt < 64 * LT;
+      if (__ballot(neg) != 0ull) {
+        for (int i = 0; i < cnt; ++i) r = cic_add(r, readlane_f64(pick(t, i % LT), i / LT));
         continue;
       }
       int s = 0;
       while (s < cnt) {
         if (!(r >= 0x1p-125f)) {  // zero or subnormal: the double sum's grid varies
-          r = cic_add(r, readlane_f64(t, s));
+          r = cic_add(r, readlane_f64(pick(t, s % LT), s / LT));
           ++s;
           continue;
         }
         const int E = ilogbf(r);
         const double u = ldexp(1.0, E - 23);
         const int R = (int)((double)r / u);
-        const bool live = l >= s && l < cnt;
-        PMap mp = live ? pmap_term(t, E) : PMap{0, 0};
+        PMap mp = PMap{0, 0};
+#pragma unroll
+        for (int k = 0; k < LT; ++k) {
+          const int i = LT * l + k;
+          if (i >= s && i < cnt) mp = pmap_compose(mp, pmap_term(t[k], E));
+        }
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
           PMap pv;
@@ -257,14 +309,28 @@ k_cic_long(int64_t m, const uint32_t *__restrict__ key, const double *__restrict
         }
         const int d = (R & 1) ? mp.d1 : mp.d0;
         const int Rl = d >= PMAP_SAT ? PMAP_SAT : R + d;
-        const unsigned long long cross = __ballot(live && Rl >= (1 << 24));
-        const int j = cross ? (__ffsll((long long)cross) - 1) : cnt;
-        if (j > s) r = (float)((double)__shfl(Rl, j - 1) * u);
-        if (j < cnt) {
-          r = cic_add(r, readlane_f64(t, j));
-          s = j + 1;
-        } else {
+        // R only grows: the lanes past the crossing one cross too
+        const unsigned long long cross = __ballot(Rl >= (1 << 24));
+        if (!cross) {
+          r = (float)((double)__shfl(Rl, 63) * u);
           s = cnt;
+          continue;
+        }
+        const int j = __ffsll((long long)cross) - 1;
+        // the lanes before j applied, then lane j's terms one at a time
+        int Rj = j > 0 ? __shfl(Rl, j - 1) : R;
+        for (int k = 0; k < LT; ++k) {  // uniform
+          const int i = LT * j + k;
+          if (i < s || i >= cnt) continue;
+          const double tk = readlane_f64(pick(t, k), j);
+          const PMap m1 = pmap_term(tk, E);
+          const int dk = (Rj & 1) ? m1.d1 : m1.d0;
+          if (dk >= PMAP_SAT || Rj + dk >= (1 << 24)) {  // the crossing term
+            r = cic_add((float)((double)Rj * u), tk);
+            s = i + 1;
+            break;
+          }
+          Rj += dk;
         }
       }
     }
@@ -274,27 +340,35 @@ k_cic_long(int64_t m, const uint32_t *__restrict__ key, const double *__restrict
 
 constexpr int TAIL_THREADS = 1024;
 
-// The last run [p0, m) by one workgroup: 1024 terms per step, maps scanned
-// within each wave and across the 16 waves through LDS, the longest prefix
-// that stays in the binade applied, the crossing term added by thread 0.
+// The last run [p0, m) by one workgroup: LT TAIL_THREADS terms per step, LT
+// per thread composed in order, maps scanned within each wave and across
+// the 16 waves through LDS, the longest prefix that stays in the binade
+// applied; the thread holding the crossing term steps its terms to it and
+// adds it directly.
 __global__ void __launch_bounds__(TAIL_THREADS)
 k_cic_tail(int64_t m, const uint32_t *__restrict__ key, const double *__restrict__ ts,
            float *__restrict__ rho, const int64_t *__restrict__ tail_start) {
   __shared__ PMap wtot[TAIL_THREADS / 64];
   __shared__ float r_sh;
-  __shared__ int j_sh, neg_sh;
+  __shared__ int j_sh, neg_sh, s_sh, rj_sh;
   const int64_t p0 = *tail_start;
   if (p0 < 0) return;
   const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
   if (tid == 0) r_sh = 0.f;
   __syncthreads();
-  for (int64_t base = p0; base < m; base += TAIL_THREADS) {
-    const int cnt = (int)min<int64_t>(TAIL_THREADS, m - base);
-    const bool in = tid < cnt;
-    const double t = in ? ts[base + tid] : 0.0;
+  for (int64_t base = p0; base < m; base += LT * TAIL_THREADS) {
+    const int cnt = (int)min<int64_t>(LT * TAIL_THREADS, m - base);
+    double t[LT];
+    bool neg = false;
+#pragma unroll
+    for (int k = 0; k < LT; ++k) {
+      const int i = LT * tid + k;
+      t[k] = i < cnt ? ts[base + i] : 0.0;
+      neg = neg || (i < cnt && !(t[k] >= 0.0));
+    }
     if (tid == 0) neg_sh = 0;
     __syncthreads();
-    if (in && !(t >= 0.0)) neg_sh = 1;
+    if (neg) neg_sh = 1;
     __syncthreads();
     if (neg_sh) {
       if (tid == 0) {
@@ -318,8 +392,12 @@ k_cic_tail(int64_t m, const uint32_t *__restrict__ key, const double *__restrict
       const int E = ilogbf(r);
       const double u = ldexp(1.0, E - 23);
       const int R = (int)((double)r / u);
-      const bool live = tid >= s && tid < cnt;
-      PMap mp = live ? pmap_term(t, E) : PMap{0, 0};
+      PMap mp = PMap{0, 0};
+#pragma unroll
+      for (int k = 0; k < LT; ++k) {
+        const int i = LT * tid + k;
+        if (i >= s && i < cnt) mp = pmap_compose(mp, pmap_term(t[k], E));
+      }
 #pragma unroll
       for (int off = 1; off < 64; off <<= 1) {
         PMap pv;
@@ -328,7 +406,7 @@ k_cic_tail(int64_t m, const uint32_t *__restrict__ key, const double *__restrict
         if (l >= off) mp = pmap_compose(pv, mp);
       }
       if (l == 63) wtot[w] = mp;
-      if (tid == 0) j_sh = cnt;
+      if (tid == 0) j_sh = TAIL_THREADS;
       __syncthreads();
       if (w == 0) {  // exclusive prefix of the wave totals, in wave order
         PMap tv = l < TAIL_THREADS / 64 ? wtot[l] : PMap{0, 0};
@@ -346,20 +424,42 @@ k_cic_tail(int64_t m, const uint32_t *__restrict__ key, const double *__restrict
         if (l < TAIL_THREADS / 64) wtot[l] = ex;
       }
       __syncthreads();
-      mp = pmap_compose(wtot[w], mp);
-      const int d = (R & 1) ? mp.d1 : mp.d0;
+      const PMap ex = wtot[w];  // the waves before this one
+      const PMap inc = pmap_compose(ex, mp);
+      const int d = (R & 1) ? inc.d1 : inc.d0;
       const int Rl = d >= PMAP_SAT ? PMAP_SAT : R + d;
-      if (live && Rl >= (1 << 24)) atomicMin(&j_sh, tid);
+      // R only grows: the threads past the crossing one cross too
+      if (Rl >= (1 << 24)) atomicMin(&j_sh, tid);
       __syncthreads();
       const int j = j_sh;
-      if (j > s && tid == j - 1) r_sh = (float)((double)Rl * u);
-      __syncthreads();
-      if (j < cnt) {
-        if (tid == 0) r_sh = cic_add(r_sh, ts[base + j]);
-        s = j + 1;
-      } else {
+      if (j == TAIL_THREADS) {  // no crossing: all applied
+        if (tid == TAIL_THREADS - 1) r_sh = (float)((double)Rl * u);
         s = cnt;
+        __syncthreads();
+        continue;
       }
+      // the threads before j applied: R after them is thread j - 1's
+      // inclusive value; then thread j steps its own terms to the crossing
+      if (tid == j - 1) rj_sh = Rl;
+      if (j == 0 && tid == 0) rj_sh = R;
+      __syncthreads();
+      if (tid == j) {
+        int Rj = rj_sh;
+        for (int k = 0; k < LT; ++k) {
+          const int i = LT * tid + k;
+          if (i < s || i >= cnt) continue;
+          const PMap m1 = pmap_term(t[k], E);
+          const int dk = (Rj & 1) ? m1.d1 : m1.d0;
+          if (dk >= PMAP_SAT || Rj + dk >= (1 << 24)) {  // the crossing term
+            r_sh = cic_add((float)((double)Rj * u), t[k]);
+            s_sh = i + 1;
+            break;
+          }
+          Rj += dk;
+        }
+      }
+      __syncthreads();
+      s = s_sh;
       __syncthreads();
     }
   }
@@ -443,7 +543,8 @@ hsr_status cic_run(hsr_ctx *c, int64_t n, const float *x, const float *y, const 
   HSR_HIP(dbuf_reserve(c->s.cub, tmp));
   HSR_HIP(hipcub::DeviceRadixSort::SortPairs(c->s.cub.p, tmp, key, key2, val, val2, (int)m, 0, bits, st));
   k_cic_gather<<<nblk(m, 256), 256, 0, st>>>(m, val2, term, ts);
-  k_cic_runs<<<nblk(m, 256), 256, 0, st>>>(m, key2, ts, rho, long_runs, nlong, tail_start);
+  k_cic_runs<<<nblk(m, 64 * RUNS_WAVES), 64 * RUNS_WAVES, 0, st>>>(m, key2, ts, rho, long_runs, nlong,
+                                                                  tail_start);
   const int waves = std::max(1, c->num_cus) * 32;  // latency-bound: many waves in flight
   k_cic_long<<<waves, 64, 0, st>>>(m, key2, ts, rho, long_runs, nlong);
   k_cic_tail<<<1, TAIL_THREADS, 0, st>>>(m, key2, ts, rho, tail_start);
