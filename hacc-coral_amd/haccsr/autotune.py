@@ -50,12 +50,17 @@ def tune_kernel_shape(ctx, run_force, shapes=SHAPES, repeats=2, log=None):
     return {"i_per_lane": r, "unroll": u, "tile": t}, times
 
 
+TIE = 0.005  # leaf sizes within half a percent of the fastest count as tied
+
+
 def tune_leaf_size(run_step, candidates=DEFAULT_CANDIDATES, repeats=2, log=None, refine=True):
     """run_step(nd) performs one synchronized step with leaf size nd.
     Returns (best_nd, {nd: seconds}).  Candidates are extended by factors of
     1.5 past an edge until the fastest has a slower neighbour on each side
     (or a bound is reached); with refine, the midpoints between the fastest
-    and its neighbours are timed too."""
+    and its neighbours are timed too.  The step time is flat near its
+    minimum, so the smallest leaf size within TIE of the fastest is chosen:
+    run-to-run noise then does not move the choice."""
     times = {}
 
     def timeit(nd):
@@ -93,5 +98,6 @@ def tune_leaf_size(run_step, candidates=DEFAULT_CANDIDATES, repeats=2, log=None,
                 mid = (order[j] + best) // 2
                 if mid not in times and mid != order[j]:
                     timeit(mid)
-    best_nd = min(times, key=times.get)
+    tmin = min(times.values())
+    best_nd = min(nd for nd, t in times.items() if t <= tmin * (1 + TIE))
     return best_nd, times
