@@ -36,7 +36,7 @@ PEAK_HBM_GBPS = 8000.0       # MI355X HBM3E
 # figure is reported only for the build it was measured on.
 MEASURED_TRAFFIC = {
     # profiles/k_force_256c_traffic.md
-    ("B", 256, 640, 1, (2, 2, 128), "6237f4f277288489"): 147129793408,
+    ("B", 256, 640, 1, (2, 2, 128), "ba11f51afafb0511"): 147090420416,
 }
 
 
