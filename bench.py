@@ -36,6 +36,7 @@ PEAK_HBM_GBPS = 8000.0       # MI355X HBM3E
 # figure is reported only for the build it was measured on.
 MEASURED_TRAFFIC = {
     # profiles/k_force_256c_traffic.md
+    ("B", 256, 512, 1, (2, 2, 128), "ba11f51afafb0511"): 144144650880,
     ("B", 256, 640, 1, (2, 2, 128), "ba11f51afafb0511"): 147090420416,
 }
 
