@@ -482,7 +482,8 @@ hsr_status hsr_destroy(hsr_ctx *c) {
   DBuf *bufs[] = {&c->tree.bnodes, &c->tree.canon_nodes, &c->tree.pfin, &c->tree.pj, &c->tree.ppm12,
                   &c->tree.xs, &c->tree.ys, &c->tree.zs, &c->tree.ms,
                   &c->lists.leaves, &c->lists.seg_ptr, &c->lists.seg, &c->lists.ent_ptr,
-                  &c->lists.jidx, &c->lists.items, &c->lists.stack, &c->lists.walk, &c->lists.item_keys,
+                  &c->lists.jidx, &c->lists.items, &c->lists.stack, &c->lists.walk, &c->lists.segcache,
+                  &c->lists.item_keys,
                   &c->lists.plan, &c->lists.part, &c->s.qctr,
                   &c->adj.fw, &c->adj.fw_items, &c->adj.tr, &c->adj.tr_items, &c->adj.seg,
                   &c->adj.tj, &c->adj.par, &c->adj.counts, &c->adj.qctr, &c->adj.part, &c->adj.H,

@@ -142,6 +142,7 @@ struct ListState {
   int32_t *leaf_nchunk = nullptr;
   DBuf stack;                 // DFS stacks
   DBuf walk;                  // per (leaf, sibling walk): counts and offsets
+  DBuf segcache;              // the first segments each walk finds (count pass)
   DBuf iperm;                 // int32 [n]: target order inside each listed leaf
 };
 

@@ -34,6 +34,11 @@ struct ListParams {
   int64_t stack_depth;
   int64_t nthreads;
   int64_t *overflow;
+  // the count pass keeps each walk's first `cache` segments (offset or node
+  // id, count or -1) at cache_seg[i * nthreads + walk]; the fill pass copies
+  // the walks that fit instead of walking them again
+  int cache;
+  int2 *cache_seg;
 };
 
 // Leaf of box b in the right-first order gets slot leaf_base[b] + leaf_rank.
@@ -138,6 +143,20 @@ k_walk(const hsr_node *__restrict__ t, const int32_t *__restrict__ leaves,
   const float tanOA = lp.tan_oa;
   const hsr_node &L = t[tl];
   const int64_t toff = L.offset, tend = L.offset + L.count;
+  if (FILL && nseg_out[gid] <= lp.cache) {
+    // the count pass kept this walk's segments: copy them
+    const int64_t ns = nseg_out[gid];
+    const int64_t so = seg_ptr[k] + slot_seg[gid];
+    int64_t nent = slot_ent[gid];
+    for (int64_t i = 0; i < ns; ++i) {
+      const int2 s = lp.cache_seg[i * lp.nthreads + gid];
+      seg_a[so + i] = s.x;
+      seg_b[so + i] = s.y;
+      seg_e[so + i] = nent;
+      nent += s.y < 0 ? lp.tdpts : s.y;
+    }
+    return;
+  }
   bool right;
   const int64_t sib = ancestor_sibling(t, root, tl, toff, tend, d, &right);
   if (!FILL) side_out[gid] = right ? 1 : 0;
@@ -171,15 +190,17 @@ k_walk(const hsr_node *__restrict__ t, const int32_t *__restrict__ leaves,
       ++sp;                                                 \
     }                                                       \
   } while (0)
-#define EMIT(A, B, LEN)                   \
-  do {                                    \
-    if (FILL) {                           \
-      seg_a[so + nseg] = (A);             \
-      seg_b[so + nseg] = (B);             \
-      seg_e[so + nseg] = nent;            \
-    }                                     \
-    ++nseg;                               \
-    nent += (LEN);                        \
+#define EMIT(A, B, LEN)                                                        \
+  do {                                                                         \
+    if (FILL) {                                                                \
+      seg_a[so + nseg] = (A);                                                  \
+      seg_b[so + nseg] = (B);                                                  \
+      seg_e[so + nseg] = nent;                                                 \
+    } else if (nseg < lp.cache) {                                              \
+      lp.cache_seg[nseg * lp.nthreads + gid] = make_int2((int)(A), (int)(B));  \
+    }                                                                          \
+    ++nseg;                                                                    \
+    nent += (LEN);                                                             \
   } while (0)
   while (sp > 0 && !dead) {
     --sp;
@@ -440,6 +461,10 @@ hsr_status lists_build_device(hsr_ctx *c, const float *min_force_h,
   lp.tdpts = c->p.tdpts;
   lp.stack_depth = depth;
   lp.nthreads = nwalk;
+  // segment cache: up to 128 per walk within 2 GiB (at 256^3, 1.1 GB)
+  lp.cache = (int)std::min<int64_t>(128, ((int64_t)1 << 31) / ((int64_t)sizeof(int2) * nwalk));
+  HSR_HIP(dbuf_reserve(LS.segcache, sizeof(int2) * (size_t)std::max(lp.cache, 1) * (size_t)nwalk));
+  lp.cache_seg = dptr<int2>(LS.segcache);
   HSR_HIP(dbuf_reserve(S.counters, 256));
   lp.overflow = dptr<int64_t>(S.counters) + 20;
   k_zero_i64<<<1, 64, 0, st>>>(lp.overflow, 1);
@@ -472,7 +497,7 @@ hsr_status lists_build_device(hsr_ctx *c, const float *min_force_h,
   HSR_HIP(dbuf_reserve(LS.jidx, sizeof(int32_t) * (LS.nentries + 1)));
   if (nl > 0) {
     k_walk<true><<<nblk(nl * ndepth, 64), 64, 0, st>>>(cn, leaves, leaf_box, d_cbase, nl, ndepth, lp,
-                                                         stk, nullptr, nullptr, nullptr,
+                                                         stk, cnt_seg, nullptr, nullptr,
                                                          dptr<int64_t>(LS.seg_ptr),
                                                          slot_seg, slot_ent, seg_a, seg_b, seg_e);
     k_walk_own<<<nblk(nl, 64), 64, 0, st>>>(cn, leaves, nl, dptr<int64_t>(LS.seg_ptr),
