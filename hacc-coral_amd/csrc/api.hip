@@ -491,7 +491,7 @@ hsr_status hsr_destroy(hsr_ctx *c) {
                   &c->s.a, &c->s.b, &c->s.c, &c->s.d, &c->s.e, &c->s.f, &c->s.g, &c->s.h,
                   &c->s.cub, &c->s.cms, &c->s.cmfail, &c->s.cmx, &c->s.x1, &c->s.y1, &c->s.z1, &c->s.m1, &c->s.p1,
                   &c->s.x0, &c->s.y0, &c->s.z0, &c->s.m0, &c->s.p0, &c->s.blocks, &c->s.rights,
-                  &c->s.pblocks, &c->s.cnode, &c->s.pnode, &c->s.counters, &c->dev_v, &c->force};
+                  &c->s.pblocks, &c->s.cnode, &c->s.pnode, &c->s.pmask, &c->s.counters, &c->dev_v, &c->force};
   if (c->adj.graph_exec) (void)hipGraphExecDestroy(c->adj.graph_exec);
   if (c->adj.graph) (void)hipGraphDestroy(c->adj.graph);
   for (DBuf *b : bufs) dbuf_free(*b);

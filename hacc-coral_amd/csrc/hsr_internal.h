@@ -174,6 +174,7 @@ struct Scratch {
   DBuf x1, y1, z1, m1, p1, x0, y0, z0, m0, p0;
   DBuf blocks, pblocks, rights;
   DBuf cnode, pnode;  // node (BFS id) of each centre-of-mass / partition block
+  DBuf pmask;         // partition flags (key < pivot), 64 per word, per block
   DBuf counters, qctr;
   DBuf host_in[10];
 };

@@ -954,7 +954,7 @@ k_part_plan(BNode *nodes, int64_t lb, int64_t le, bool unit, const sd_t *cms, in
 // Keys < pivot per block (RCBForceTree.cxx:638-645), pivot = xc[d].
 __global__ void __launch_bounds__(256)
 k_part_count(const BNode *nodes, const int32_t *pnode, PartBlock *pb, const Counters *ctr,
-             const float *X, const float *Y, const float *Z) {
+             const float *X, const float *Y, const float *Z, uint64_t *mask) {
   for (int64_t b = blockIdx.x; b < ctr->part_blocks; b += gridDim.x) {
     const BNode &n = nodes[pnode[b]];
     int64_t bstart;
@@ -963,7 +963,16 @@ k_part_count(const BNode *nodes, const int32_t *pnode, PartBlock *pb, const Coun
     const float *key = coord(n.split_d, X, Y, Z);
     float pv = n.xc[n.split_d];
     int c = 0;
-    for (int k = threadIdx.x; k < blen; k += 256) c += key[bstart + k] < pv;
+    // the flags also go to the block's mask words (64 elements each), which
+    // the rank kernel reads instead of the keys
+#pragma unroll
+    for (int r = 0; r < PB / 256; ++r) {
+      const int k = r * 256 + (int)threadIdx.x;
+      const bool f = k < blen && key[bstart + k] < pv;
+      const uint64_t bal = __ballot(f);
+      if (lane_id() == 0) mask[b * (PB / 64) + (k >> 6)] = bal;
+      c += f;
+    }
     __shared__ int red[4];
     __syncthreads();  // thread 0 has read the previous block's red
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
@@ -1018,16 +1027,13 @@ k_part_scan(BNode *nodes, int64_t lb, int64_t le, PartBlock *pb) {
 // resolved by the swap chain (dest = -1 here).
 __global__ void __launch_bounds__(256)
 k_part_rank(const BNode *nodes, const int32_t *pnode, const PartBlock *pb, const Counters *ctr,
-            const float *X, const float *Y, const float *Z, int32_t *idx,
-            int32_t *rights, int32_t *dest) {
+            const uint64_t *mask, int32_t *idx, int32_t *rights, int32_t *dest) {
   for (int64_t b = blockIdx.x; b < ctr->part_blocks; b += gridDim.x) {
     const PartBlock &B = pb[b];
     const BNode &n = nodes[pnode[b]];
     int64_t bstart;
     int blen;
     blk_range(n, b, PB, bstart, blen);
-    const float *key = coord(n.split_d, X, Y, Z);
-    float pv = n.xc[n.split_d];
     int64_t off = n.offset, is = n.is;
     __shared__ int wsum[4];
     int64_t run = B.base;
@@ -1036,8 +1042,9 @@ k_part_rank(const BNode *nodes, const int32_t *pnode, const PartBlock *pb, const
       int k = r * 256 + threadIdx.x;
       bool valid = k < blen;
       int64_t e = bstart + k;
-      bool f = valid && key[e] < pv;
-      uint64_t bal = __ballot(f);
+      // this wave's 64 flags, set by k_part_count (zero past the block)
+      const uint64_t bal = mask[b * (PB / 64) + (k >> 6)];
+      const bool f = (bal >> lane_id()) & 1ull;
       int pre = __popcll(bal & lanemask_lt());
       if (lane_id() == 0) wsum[w] = __popcll(bal);
       __syncthreads();
@@ -1672,7 +1679,9 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_LEAF, depth == 0, P[cur], Pfin);
     // ---- partition of splitting nodes
     const unsigned pg = (unsigned)std::max<int64_t>(1, pcap);
-    k_part_count<<<pg, 256, 0, st>>>(nodes, pnode, pb, ctr, Xr, Yr, Zr);
+    HSR_HIP(dbuf_reserve(S.pmask, sizeof(uint64_t) * (PB / 64) * (size_t)pcap));
+    uint64_t *pmask = dptr<uint64_t>(S.pmask);
+    k_part_count<<<pg, 256, 0, st>>>(nodes, pnode, pb, ctr, Xr, Yr, Zr, pmask);
     if (active / PB > 64 * nl)  // over 64 blocks per node on average
       k_part_scan<PFX_WAVES><<<std::min<unsigned>((unsigned)nl, 4u * (unsigned)std::max(1, c->num_cus)), PFX_T,
                                0, st>>>(nodes, lb, le, pb);
@@ -1681,7 +1690,7 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     // children depend only on the partition counts: create them now so a
     // single host synchronisation per level reads every counter
     k_make_children<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, le, ctr, ctr2 + ((depth + 1) & 1));
-    k_part_rank<<<pg, 256, 0, st>>>(nodes, pnode, pb, ctr, Xr, Yr, Zr, idx, rights, dest);
+    k_part_rank<<<pg, 256, 0, st>>>(nodes, pnode, pb, ctr, pmask, idx, rights, dest);
     k_part_chain<<<pg, 256, 0, st>>>(nodes, pnode, ctr, idx, rights, dest, ctr);
     HSR_HIP(hipMemcpyAsync(&hc, ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
     HSR_HIP(stream_wait(st));
