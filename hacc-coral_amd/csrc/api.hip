@@ -1020,6 +1020,10 @@ hsr_status hsr_set_tuning(hsr_ctx *c, const hsr_tuning *t) {
     g_err = "hsr_set_tuning: max_plan_bytes must be >= 0";
     return HSR_ERR_ARG;
   }
+  if (t->chain_hops < 0 || t->chain_hops > 4096) {
+    g_err = "hsr_set_tuning: chain_hops must lie in [0, 4096]";
+    return HSR_ERR_ARG;
+  }
   const bool replan = t->tree_order_targets != c->tuning.tree_order_targets ||
                       t->chunk != c->tuning.chunk;
   const hsr_tuning old = c->tuning;

@@ -27,7 +27,7 @@ constexpr int CM_BLK = 1024;        // elements per block of a big node
 constexpr int CM_PER = CM_BLK / 64; // consecutive addends each lane composes
 constexpr int64_t CM_SMALL = 2048;  // nodes up to this size: one wave each
 constexpr int PB = 1024;            // partition block (256 threads x 4)
-constexpr int MAXHOP = 256;         // swap-chain hops before pointer jumping
+constexpr int MAXHOP = 256;         // swap-chain hops before pointer jumping (default)
 
 struct Counters {
   int64_t cm_blocks;      // big-node cm blocks allocated this level
@@ -1079,7 +1079,7 @@ k_part_rank(const BNode *nodes, const int32_t *pnode, const PartBlock *pb, const
 __global__ void __launch_bounds__(256)
 k_part_chain(const BNode *nodes, const int32_t *pnode, const Counters *ctrc,
              const int32_t *idx, const int32_t *rights, int32_t *dest,
-             Counters *ctr) {
+             Counters *ctr, int maxhop) {
   for (int64_t b = blockIdx.x; b < ctrc->part_blocks; b += gridDim.x) {
     const BNode &n = nodes[pnode[b]];
     int64_t bstart;
@@ -1106,7 +1106,7 @@ k_part_chain(const BNode *nodes, const int32_t *pnode, const Counters *ctrc,
       const int64_t e = off + p;
       uint32_t j = p;
       bool done = false;
-      for (int it = 0; it < MAXHOP; ++it) {
+      for (int it = 0; it < maxhop; ++it) {
         const uint32_t f = (uint32_t)idx_n[j];  // f > j
         if (f >= is) {
           dest[e] = (int32_t)f;
@@ -1691,7 +1691,8 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     // single host synchronisation per level reads every counter
     k_make_children<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, le, ctr, ctr2 + ((depth + 1) & 1));
     k_part_rank<<<pg, 256, 0, st>>>(nodes, pnode, pb, ctr, pmask, idx, rights, dest);
-    k_part_chain<<<pg, 256, 0, st>>>(nodes, pnode, ctr, idx, rights, dest, ctr);
+    k_part_chain<<<pg, 256, 0, st>>>(nodes, pnode, ctr, idx, rights, dest, ctr,
+                                     c->tuning.chain_hops > 0 ? c->tuning.chain_hops : MAXHOP);
     HSR_HIP(hipMemcpyAsync(&hc, ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
     HSR_HIP(stream_wait(st));
     if (hc.unresolved > 0) {

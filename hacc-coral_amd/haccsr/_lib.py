@@ -115,7 +115,7 @@ class Tuning(C.Structure):
                 ("waves_per_cu", C.c_int), ("use_graph", C.c_int),
                 ("adjoint_mode", C.c_int), ("cull_off", C.c_int),
                 ("tree_order_targets", C.c_int), ("force_pass", C.c_int),
-                ("max_plan_bytes", C.c_int64), ("unroll", C.c_int), ("reserved0", C.c_int)]
+                ("max_plan_bytes", C.c_int64), ("unroll", C.c_int), ("chain_hops", C.c_int)]
 
 
 _lib = None

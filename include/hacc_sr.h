@@ -441,7 +441,9 @@ typedef struct {
                          A plan above it fails with HSR_ERR_NOMEM before any
                          launch (so does one of 2^31 work items or more) */
   int unroll;         /* leaf kernel shape: j entries per unrolled step (0 = 2) */
-  int reserved0;
+  int chain_hops;     /* tree build: stretch steps a partition swap chain takes
+                         before the pointer-jumping pass resolves it, 0..4096
+                         (0 = 256); the tree is the same for every value */
 } hsr_tuning;
 
 /* Out-of-range fields give HSR_ERR_ARG and leave the tuning unchanged. */

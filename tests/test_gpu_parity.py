@@ -426,6 +426,42 @@ def test_graph_replay_matches_eager():
     say(f"  replays ok; after re-listing rel rms {e2:.3e}")
 
 
+@pytest.mark.parametrize("hops", [1, 2, 7])
+def test_partition_chains_by_pointer_jumping(hops):
+    """The partition's swap chains that take more than chain_hops stretch
+    steps are resolved by the pointer-jumping pass: with 1, 2 and 7 steps
+    (the default is 256, which almost every chain needs fewer of) most
+    chains go through it, and the tree, permutation and lists stay bit-exact
+    with the oracle."""
+    say(f"case partition chains by pointer jumping, chain_hops {hops}")
+    for (x, y, z, m), box, nd in ((uniform(30000, 14.0, 21), 14.0, 64),
+                                  (uniform(5000, 6.0, 22), 6.0, 16)):
+        lo, hi, flo, fhi = [0, 0, 0], [box] * 3, [1.0] * 3, [box - 1.0] * 3
+        tp = O.TreeParams(np.asarray(lo, np.float32), np.asarray(hi, np.float32),
+                          np.asarray(flo, np.float32), np.asarray(fhi, np.float32),
+                          fsrrmax=3.12, rsm=0.1, oa=0.1, nd=nd)
+        tree = O.build_tree(tp, x, y, z, m)
+        lists = O.build_lists(tp, tree)
+        ctx = Context(nd=nd, fsrrmax=3.12, rsm=0.1, oa=0.1)
+        ctx.set_tuning(chain_hops=hops)
+        X, Y, Z, M = x.copy(), y.copy(), z.copy(), m.copy()
+        ids = np.arange(x.shape[0], dtype=np.int64)
+        ctx.build(X, Y, Z, M, ids=ids, min_loc=lo, max_loc=hi)
+        ctx.lists(flo, fhi)
+        check_tree(tree, ctx, (X, Y, Z, M, ids))
+        check_lists(lists, ctx)
+        ctx.close()
+
+
+def test_chain_hops_validation():
+    ctx = Context(nd=64, fsrrmax=3.12, rsm=0.1, oa=0.1)
+    for bad in (-1, 4097):
+        with pytest.raises(_lib.HsrError, match="chain_hops"):
+            ctx.set_tuning(chain_hops=bad)
+    ctx.set_tuning(chain_hops=4096)
+    ctx.close()
+
+
 def test_graph_replayed_across_list_builds():
     """The captured force phase is replayed, not captured again, after a new
     build and list pass of the same particle set (fixed grids, device-side
