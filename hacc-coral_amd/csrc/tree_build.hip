@@ -901,12 +901,31 @@ k_leaf_ppm(BNode *nodes, int64_t lb, int64_t le, int64_t nd, const float *M) {
 // time read buffer `P`; degenerate splits read the partition's output).
 __global__ void __launch_bounds__(64)
 k_finalize_leaves(const BNode *nodes, int64_t lb, int64_t le, int kind,
-                  bool ident, const int32_t *P, int32_t *Pfin) {
+                  bool ident, const int32_t *P, int32_t *Pfin, bool unit, const float *X,
+                  const float *Y, const float *Z, const float *M, float *xs, float *ys,
+                  float *zs, float *ms) {
+  // the node's elements are in their final positions: their input index
+  // and their sorted x, y, z, m (read in order here rather than gathered
+  // through the permutation afterwards)
   for (int64_t t = lb + blockIdx.x; t < le; t += gridDim.x) {
     if (nodes[t].kind != kind) continue;
     int64_t off = nodes[t].offset, cnt = nodes[t].count;
-    for (int64_t k = lane_id(); k < cnt; k += 64) Pfin[off + k] = ident ? (int32_t)(off + k) : P[off + k];
+    for (int64_t k = lane_id(); k < cnt; k += 64) {
+      const int64_t e = off + k;
+      Pfin[e] = ident ? (int32_t)e : P[e];
+      xs[e] = X[e];
+      ys[e] = Y[e];
+      zs[e] = Z[e];
+      ms[e] = unit ? 1.f : M[e];
+    }
   }
+}
+
+// The particles' j-records from the sorted arrays.
+__global__ void k_pj_particles(int64_t n, const float *xs, const float *ys, const float *zs,
+                               const float *ms, PJ *pj) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < n) pj[k] = PJ{xs[k], ys[k], zs[k], ms[k]};
 }
 
 // ------------------------------------------------------------ partition
@@ -1339,21 +1358,6 @@ __global__ void k_box_of(const BNode *nodes, int64_t nb, int32_t *box_of,
   box_of[t] = root_box[u];
 }
 
-__global__ void k_gather_sorted(int64_t n, const int32_t *Pfin, bool unit,
-                                const float *x, const float *y, const float *z,
-                                const float *m, float *xs, float *ys,
-                                float *zs, float *ms, PJ *pj) {
-  int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  int32_t s = Pfin[k];
-  float a = x[s], b = y[s], c = z[s], d = unit ? 1.f : m[s];
-  xs[k] = a;
-  ys[k] = b;
-  zs[k] = c;
-  ms[k] = d;
-  pj[k] = PJ{a, b, c, d};
-}
-
 // ------------------------------------------------- quadrupole (TDPTS = 12)
 // Hardin and Sloane's 12-point spherical 4-design (the icosahedron's
 // vertices) as the reference tabulates it (RCBForceTree.cxx:229-272), in
@@ -1578,6 +1582,11 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
   int32_t *jmp = dptr<int32_t>(S.c);
   int32_t *rights = dptr<int32_t>(S.rights);
   Counters *ctr2 = dptr<Counters>(S.counters);  // two sets, alternating by level
+  // sorted arrays, written as nodes stop splitting
+  for (DBuf *bb : {&T.xs, &T.ys, &T.zs, &T.ms})
+    HSR_HIP(dbuf_reserve(*bb, sizeof(float) * nn));
+  float *xs_ = dptr<float>(T.xs), *ys_ = dptr<float>(T.ys), *zs_ = dptr<float>(T.zs),
+        *ms_ = dptr<float>(T.ms);
 
   // box table
   int64_t *d_box_off = dptr<int64_t>(S.d);
@@ -1676,7 +1685,8 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     int32_t *pnode = dptr<int32_t>(S.pnode);
     k_part_plan<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, unit, cms, nd, c->p.ppc, unit_ppm, pnode, ctr);
     if (!unit_ppm) k_leaf_ppm<<<gnl, 64, 0, st>>>(nodes, lb, le, nd, Mr);
-    k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_LEAF, depth == 0, P[cur], Pfin);
+    k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_LEAF, depth == 0, P[cur], Pfin, unit, Xr,
+                                                   Yr, Zr, Mr, xs_, ys_, zs_, ms_);
     // ---- partition of splitting nodes
     const unsigned pg = (unsigned)std::max<int64_t>(1, pcap);
     HSR_HIP(dbuf_reserve(S.pmask, sizeof(uint64_t) * (PB / 64) * (size_t)pcap));
@@ -1703,7 +1713,9 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     }
     k_part_scatter<<<pg, 256, 0, st>>>(nodes, pnode, ctr, unit, depth == 0, dest, Xr, Yr, Zr, Mr, P[cur],
                                         X[cur ^ 1], Y[cur ^ 1], Z[cur ^ 1], Mm[cur ^ 1], P[cur ^ 1]);
-    k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_DEGEN, false, P[cur ^ 1], Pfin);
+    k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_DEGEN, false, P[cur ^ 1], Pfin, unit,
+                                                   X[cur ^ 1], Y[cur ^ 1], Z[cur ^ 1], Mm[cur ^ 1], xs_,
+                                                   ys_, zs_, ms_);
     HSR_HIP(hipGetLastError());
 #ifdef HSR_CM_STATS
     fprintf(stderr, "[cm] level %d nodes %lld blocks %lld scans %lld fallback %lld small ordered %lld\n",
@@ -1756,14 +1768,11 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
   hsr_node *cn = dptr<hsr_node>(T.canon_nodes);
   k_write_canon<<<nblk_for(nb, 256), 256, 0, st>>>(nodes, nb, d_cbase, box_of, cn);
 
-  // ---- sorted arrays and j-particle records
-  for (DBuf *bb : {&T.xs, &T.ys, &T.zs, &T.ms})
-    HSR_HIP(dbuf_reserve(*bb, sizeof(float) * nn));
+  // ---- j-particle records of the particles (the sorted arrays are done)
   const int tdpts = c->p.tdpts;
   HSR_HIP(dbuf_reserve(T.pj, sizeof(PJ) * (n + tdpts * T.nnodes + 1)));
   if (n > 0)
-    k_gather_sorted<<<nblk_for(n, 256), 256, 0, st>>>(n, Pfin, unit, x, y, z, m, dptr<float>(T.xs), dptr<float>(T.ys),
-                                                     dptr<float>(T.zs), dptr<float>(T.ms), dptr<PJ>(T.pj));
+    k_pj_particles<<<nblk_for(n, 256), 256, 0, st>>>(n, xs_, ys_, zs_, ms_, dptr<PJ>(T.pj));
   if (tdpts == 12) {
     // quadrupole pseudo-particles, deepest BFS level first; the orphan
     // slots of degenerate nodes keep the zeros
