@@ -13,12 +13,15 @@
 //
 // The deposit is a scatter whose result depends on the order of the float
 // additions into each cell.  To reproduce the reference's sequential order
-// in parallel it writes the 8 contributions of every particle (cell, term)
-// in (particle, corner) order, sorts them stably by cell, and adds each
-// cell's run in that order: one thread per short run, one wave per long run
-// (a dense clump), streaming the run 64 contributions at a time through
-// exact fixed-grid maps of the float accumulation.  The interpolation is a
-// gather, one thread per particle.
+// in parallel it sorts the particles stably by base cell; a grid cell's
+// contributions are then the merge, by particle index, of the particle
+// lists of its eight neighbouring base cells.  A cell with at most 64 is
+// merged and added by its own thread; longer cells (dense clumps) have
+// their merged order staged and added by a wave each, 256 contributions at
+// a time through exact fixed-grid maps of the float accumulation; the slot
+// for corners outside the grid gets its contributions in (particle, corner)
+// order and a workgroup.  The interpolation is a gather, one thread per
+// particle.
 #include "hsr_internal.h"
 
 #include <hipcub/hipcub.hpp>
@@ -88,33 +91,6 @@ __device__ __forceinline__ double corner_term(int q, float s, float ab, float de
   }
 }
 
-// Contributions of particle nn at 8 nn + q: its corner cell (sort key), its
-// position in the sequential order (sort value) and its term.
-__global__ void __launch_bounds__(256)
-k_cic_contrib(int64_t n, const float *__restrict__ x, const float *__restrict__ y,
-              const float *__restrict__ z, Grid g, float c, uint32_t *__restrict__ key,
-              int32_t *__restrict__ val, double *__restrict__ term) {
-  const int64_t nn = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (nn >= n) return;
-  const Corners cc = corners(x[nn], y[nn], z[nn], g);
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int64_t k = 8 * nn + q;
-    key[k] = cc.cell[q];
-    val[k] = (int32_t)k;
-    term[k] = corner_term(q, c, cc.ab, cc.de, cc.gh);
-  }
-}
-
-// Terms in sorted order.
-__global__ void __launch_bounds__(256)
-k_cic_gather(int64_t m, const int32_t *__restrict__ val, const double *__restrict__ term,
-             double *__restrict__ ts) {
-  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (p >= m) return;
-  ts[p] = term[val[p]];
-}
-
 // The reference's accumulation step for one contribution
 // (Particles.cxx:632-639): corner 0 adds a float term in float, the others
 // add a double term and round to float.  The float addition r + t equals
@@ -129,66 +105,6 @@ __device__ __forceinline__ double readlane_f64(double v, int i) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), i);
   const int hi = __builtin_amdgcn_readlane(__double2hiint(v), i);
   return __hiloint2double(hi, lo);
-}
-
-constexpr int RUN_SHORT = 64;
-
-// One thread per run of equal cells in the sorted contributions: runs of at
-// most RUN_SHORT are added in order here; longer runs are queued for a wave
-// each, except the last run of the sorted order, which is the outside slot
-// whenever any corner fell outside the grid -- typically the longest run by
-// far -- and goes to a workgroup of its own.  Each wave owns 64 positions
-// and stages their keys and terms in LDS with coalesced loads; a head
-// thread adds its run from LDS, and only the one run that crosses the end
-// of the window reads its remaining terms from memory.
-constexpr int RUNS_WAVES = 4;
-__global__ void __launch_bounds__(64 * RUNS_WAVES)
-k_cic_runs(int64_t m, const uint32_t *__restrict__ key, const double *__restrict__ ts,
-           float *__restrict__ rho, int64_t *__restrict__ long_runs,
-           unsigned long long *__restrict__ nlong, int64_t *__restrict__ tail_start) {
-  __shared__ uint32_t sk[RUNS_WAVES][64];
-  __shared__ double sv[RUNS_WAVES][64];
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int64_t w0 = ((int64_t)blockIdx.x * RUNS_WAVES + w) * 64;
-  const int64_t p = w0 + l;
-  // past the end: a key no cell has (cells and the outside slot are below
-  // 2^31)
-  const uint32_t cell = p < m ? key[p] : 0xffffffffu;
-  sk[w][l] = cell;
-  sv[w][l] = p < m ? ts[p] : 0.0;
-  __syncthreads();
-  // run ends in the window, by ballot: lane l's run ends at the first end at
-  // or after l
-  const bool endm = l == 63 || sk[w][l + 1 < 64 ? l + 1 : 63] != cell;
-  const uint64_t ends = __ballot(endm);
-  if (p >= m) return;
-  if (p > 0 && (l > 0 ? sk[w][l - 1] : key[p - 1]) == cell) return;
-  const int e = l + 1 + __builtin_ctzll(ends >> l);
-  int64_t eg = w0 + e;  // the run continues past the window: from memory
-  if (e == 64)
-    while (eg < m && eg < p + RUN_SHORT + 1 && key[eg] == cell) ++eg;
-  if (eg - p > RUN_SHORT) {
-    if (key[m - 1] == cell) {
-      // the last run (the outside slot, or the largest cell): a workgroup
-      *tail_start = p;
-      return;
-    }
-    const unsigned long long slot = atomicAdd(nlong, 1ull);
-    long_runs[slot] = p;
-    return;
-  }
-  // in order, eight terms read from LDS at a time
-  float r = 0.f;
-  for (int q = l; q < e; q += 8) {
-    double v[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = sv[w][min(q + k, 63)];
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (q + k < e) r = cic_add(r, v[k]);
-  }
-  for (int64_t q = w0 + 64; q < eg; ++q) r = cic_add(r, ts[q]);
-  rho[cell] = r;
 }
 
 // Fixed-grid maps of the accumulation r <- (float)((double)r + t), t >= 0.
@@ -466,6 +382,260 @@ k_cic_tail(int64_t m, const uint32_t *__restrict__ key, const double *__restrict
   if (tid == 0) rho[key[p0]] = r_sh;
 }
 
+// ------------------------------------------------- deposit by base cells
+// A particle with floor coordinates (ix, iy, iz) deposits into the cells
+// (ix, iy, iz) + d_q for its eight corners q (corners()), so a grid cell c
+// receives, from each q, the particles whose base cell is c - d_q.  With
+// the particles sorted stably by base cell, each such list is in particle
+// order, and c's contributions in the reference's order (particle by
+// particle; one particle reaches c through one corner at most) are the
+// merge of its eight lists by particle index.  Base cells run over
+// [-1, ng - 1] per axis (an extended grid, shifted by one); a particle
+// outside it has all its corners outside the grid.  Corners outside the
+// grid go to the outside slot, in (particle, corner) order.
+
+// corner offsets d_q in corners()' order
+__constant__ int kCornerDx[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+__constant__ int kCornerDy[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+__constant__ int kCornerDz[8] = {0, 0, 1, 1, 1, 1, 0, 0};
+
+struct Ext {
+  uint32_t e1, e2;  // extended sizes ng1 + 1, ng2 + 1 (ng0 + 1 along x)
+  uint32_t out;     // (ng0 + 1)(ng1 + 1)(ng2 + 1): the key of particles outside
+};
+
+// The weights of corners() alone.
+__device__ __forceinline__ void weights(float xx, float yy, float zz, float &ab, float &de,
+                                        float &gh) {
+  const int ix = x86_floor_int(xx), iy = x86_floor_int(yy), iz = x86_floor_int(zz);
+  ab = (float)(1.0 + (double)((float)ix - xx));
+  de = (float)(1.0 + (double)((float)iy - yy));
+  gh = (float)(1.0 + (double)((float)iz - zz));
+}
+
+// Per particle: its extended base-cell key, its index (the sort's value) and
+// the number of its corners outside the grid.
+__global__ void __launch_bounds__(256)
+k_cic_base(int64_t n, const float *__restrict__ x, const float *__restrict__ y,
+           const float *__restrict__ z, Grid g, Ext ex, uint32_t *__restrict__ key,
+           int32_t *__restrict__ idx, int32_t *__restrict__ nout) {
+  const int64_t nn = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (nn >= n) return;
+  const float xx = x[nn], yy = y[nn], zz = z[nn];
+  const Corners cc = corners(xx, yy, zz, g);
+  int no = 0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) no += cc.cell[q] == g.safe;
+  const int ix = x86_floor_int(xx), iy = x86_floor_int(yy), iz = x86_floor_int(zz);
+  const bool in = ix >= -1 && ix < g.ng0 && iy >= -1 && iy < g.ng1 && iz >= -1 && iz < g.ng2;
+  key[nn] = in ? (((uint32_t)(ix + 1) * ex.e1 + (uint32_t)(iy + 1)) * ex.e2 + (uint32_t)(iz + 1)) : ex.out;
+  idx[nn] = (int32_t)nn;
+  nout[nn] = no;
+}
+
+// [beg, end) of every base cell in the sorted order (left 0 when empty).
+__global__ void __launch_bounds__(256)
+k_cic_ranges(int64_t n, const uint32_t *__restrict__ key, Ext ex, int32_t *__restrict__ beg,
+             int32_t *__restrict__ end) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t k = key[p];
+  if (k == ex.out) return;
+  if (p == 0 || key[p - 1] != k) beg[k] = (int32_t)p;
+  if (p == n - 1 || key[p + 1] != k) end[k] = (int32_t)(p + 1);
+}
+
+constexpr int CELL_SHORT = 64;  // contributions a cell's own thread merges and adds
+
+// The eight source lists of grid cell c = (i, j, k).
+__device__ __forceinline__ void cell_lists(int i, int j, int k, const Ext &ex,
+                                           const int32_t *beg, const int32_t *end,
+                                           int32_t (&s)[8], int32_t (&e)[8]) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const uint32_t b = ((uint32_t)(i - kCornerDx[q] + 1) * ex.e1 + (uint32_t)(j - kCornerDy[q] + 1)) * ex.e2 +
+                       (uint32_t)(k - kCornerDz[q] + 1);
+    s[q] = beg[b];
+    e[q] = end[b];
+  }
+}
+
+// The next contribution of the merge: the list whose head particle comes
+// first (lists never share a particle).
+__device__ __forceinline__ int merge_pick(const int32_t (&hv)[8]) {
+  int qs = 0;
+  int32_t best = hv[0];
+#pragma unroll
+  for (int q = 1; q < 8; ++q)
+    if (hv[q] < best) {
+      best = hv[q];
+      qs = q;
+    }
+  return qs;
+}
+
+__device__ __forceinline__ void merge_advance(int qs, int32_t (&s)[8], const int32_t (&e)[8],
+                                              int32_t (&hv)[8], const int32_t *idx) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    if (q == qs) {
+      ++s[q];
+      hv[q] = s[q] < e[q] ? idx[s[q]] : INT_MAX;
+    }
+}
+
+// One thread per grid cell: at most CELL_SHORT contributions are merged and
+// added here; longer cells are queued (their count for the staging offsets).
+__global__ void __launch_bounds__(256)
+k_cic_cells(Grid g, Ext ex, const int32_t *__restrict__ beg, const int32_t *__restrict__ end,
+            const int32_t *__restrict__ idx, const float *__restrict__ x,
+            const float *__restrict__ y, const float *__restrict__ z, float c,
+            float *__restrict__ rho, uint32_t *__restrict__ long_cell,
+            int64_t *__restrict__ long_k, unsigned long long *__restrict__ nlong) {
+  const int64_t cid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (cid >= (int64_t)g.safe) return;
+  const int k = (int)(cid % g.ng2);
+  const int64_t ij = cid / g.ng2;
+  const int j = (int)(ij % g.ng1), i = (int)(ij / g.ng1);
+  int32_t s[8], e[8];
+  cell_lists(i, j, k, ex, beg, end, s, e);
+  int K = 0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) K += e[q] - s[q];
+  if (K == 0) return;
+  if (K > CELL_SHORT) {
+    const unsigned long long slot = atomicAdd(nlong, 1ull);
+    long_cell[slot] = (uint32_t)cid;
+    long_k[slot] = K;
+    return;
+  }
+  int32_t hv[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) hv[q] = s[q] < e[q] ? idx[s[q]] : INT_MAX;
+  float r = 0.f;
+  for (int t = 0; t < K; ++t) {
+    const int qs = merge_pick(hv);
+    const int32_t nn = hv[qs];
+    float ab, de, gh;
+    weights(x[nn], y[nn], z[nn], ab, de, gh);
+    r = cic_add(r, corner_term(qs, c, ab, de, gh));
+    merge_advance(qs, s, e, hv, idx);
+  }
+  rho[cid] = r;
+}
+
+// One workgroup per long cell: its contributions' terms in merged order
+// written at its staging offset, with the cell as the run key.  When the
+// eight lists hold disjoint index ranges -- the particles of a base cell
+// are contiguous when the input is in cell order, as the resort leaves it
+// -- the merge is the lists in the order of their first particles, copied
+// by all threads; otherwise thread 0 merges them one contribution at a time.
+__global__ void __launch_bounds__(256)
+k_cic_long_order(Grid g, Ext ex, const int32_t *__restrict__ beg, const int32_t *__restrict__ end,
+                 const int32_t *__restrict__ idx, const uint32_t *__restrict__ long_cell,
+                 const int64_t *__restrict__ long_base, int64_t nl, const float *__restrict__ x,
+                 const float *__restrict__ y, const float *__restrict__ z, float c,
+                 uint32_t *__restrict__ skey, double *__restrict__ ts) {
+  __shared__ int32_t ss[8], se[8], soff[8];
+  __shared__ int disjoint;
+  for (int64_t t = blockIdx.x; t < nl; t += gridDim.x) {
+    const uint32_t cid = long_cell[t];
+    const int64_t base = long_base[t];
+    if (threadIdx.x == 0) {
+      const int k = (int)(cid % (uint32_t)g.ng2);
+      const uint32_t ij = cid / (uint32_t)g.ng2;
+      const int j = (int)(ij % (uint32_t)g.ng1), i = (int)(ij / (uint32_t)g.ng1);
+      int32_t s[8], e[8];
+      cell_lists(i, j, k, ex, beg, end, s, e);
+      // each non-empty list's first and last particle; the lists ordered by
+      // their first (a selection sort of eight)
+      int32_t first[8], last[8], ord[8];
+      int nq = 0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        ss[q] = s[q];
+        se[q] = e[q];
+        soff[q] = 0;
+        first[q] = s[q] < e[q] ? idx[s[q]] : INT_MAX;
+        last[q] = s[q] < e[q] ? idx[e[q] - 1] : INT_MAX;
+      }
+      for (int q = 0; q < 8; ++q)
+        if (s[q] < e[q]) ord[nq++] = q;
+      for (int a2 = 0; a2 < nq; ++a2)
+        for (int b2 = a2 + 1; b2 < nq; ++b2)
+          if (first[ord[b2]] < first[ord[a2]]) {
+            const int tq = ord[a2];
+            ord[a2] = ord[b2];
+            ord[b2] = tq;
+          }
+      int dj = 1, off = 0;
+      for (int a2 = 0; a2 < nq; ++a2) {
+        const int q = ord[a2];
+        if (a2 + 1 < nq && !(last[q] < first[ord[a2 + 1]])) dj = 0;
+        soff[q] = off;
+        off += e[q] - s[q];
+      }
+      disjoint = dj;
+      if (!dj) {
+        int32_t hv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) hv[q] = s[q] < e[q] ? idx[s[q]] : INT_MAX;
+        for (int u = 0; u < off; ++u) {
+          const int qs = merge_pick(hv);
+          const int32_t nn = hv[qs];
+          float ab, de, gh;
+          weights(x[nn], y[nn], z[nn], ab, de, gh);
+          skey[base + u] = cid;
+          ts[base + u] = corner_term(qs, c, ab, de, gh);
+          merge_advance(qs, s, e, hv, idx);
+        }
+      }
+    }
+    __syncthreads();
+    if (disjoint) {
+      for (int q = 0; q < 8; ++q) {
+        const int32_t s0 = ss[q], len = se[q] - ss[q];
+        const int64_t o = base + soff[q];
+        for (int u = threadIdx.x; u < len; u += blockDim.x) {
+          const int32_t nn = idx[s0 + u];
+          float ab, de, gh;
+          weights(x[nn], y[nn], z[nn], ab, de, gh);
+          skey[o + u] = cid;
+          ts[o + u] = corner_term(q, c, ab, de, gh);
+        }
+      }
+    }
+    __syncthreads();  // the shared lists are rewritten for the next cell
+  }
+}
+
+// The outside slot's contributions in (particle, corner) order, from
+// position base (out_base: exclusive prefix of the particles' counts).
+__global__ void __launch_bounds__(256)
+k_cic_out_terms(int64_t n, const float *__restrict__ x, const float *__restrict__ y,
+                const float *__restrict__ z, Grid g, float c, const int32_t *__restrict__ nout,
+                const int32_t *__restrict__ out_base, int64_t base, uint32_t *__restrict__ skey,
+                double *__restrict__ ts) {
+  const int64_t nn = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (nn >= n || nout[nn] == 0) return;
+  const Corners cc = corners(x[nn], y[nn], z[nn], g);
+  int64_t p = base + out_base[nn];
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    if (cc.cell[q] == g.safe) {
+      skey[p] = g.safe;
+      ts[p] = corner_term(q, c, cc.ab, cc.de, cc.gh);
+      ++p;
+    }
+}
+
+__global__ void k_set_runs(int64_t nl, const int64_t *__restrict__ long_base,
+                           int64_t *__restrict__ long_runs, int64_t tail, int64_t *__restrict__ tail_start) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t < nl) long_runs[t] = long_base[t];
+  if (t == 0) *tail_start = tail;
+}
+
 // Particles.cxx:682-711, one component: f accumulated as the deposit's runs
 // are (grid[safe] read as 0), then v += f * fscal * tau in float.
 __global__ void __launch_bounds__(256)
@@ -509,45 +679,134 @@ Grid make_grid(const int32_t ng[3]) {
 
 }  // namespace
 
+namespace {
+
+__global__ void k_fill_i32(int64_t n, int32_t *a) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < n) a[k] = 0;
+}
+
+template <class T>
+hsr_status pm_excl_scan(hsr_ctx *c, const T *in, T *out, int64_t n) {
+  size_t tmp = 0;
+  HSR_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, c->stream));
+  HSR_HIP(dbuf_reserve(c->s.cub, tmp + 256));
+  HSR_HIP(hipcub::DeviceScan::ExclusiveSum(c->s.cub.p, tmp, in, out, (int)n, c->stream));
+  return HSR_OK;
+}
+
+}  // namespace
+
 hsr_status cic_run(hsr_ctx *c, int64_t n, const float *x, const float *y, const float *z,
                    const int32_t ng[3], float gpscal, float *rho) {
   hipStream_t st = c->stream;
   const Grid g = make_grid(ng);
   const int64_t ncell = (int64_t)g.safe + 1;
+  const uint64_t ext = (uint64_t)(ng[0] + 1) * (uint64_t)(ng[1] + 1) * (uint64_t)(ng[2] + 1);
+  if (ext >= 0xffffffffull) {
+    hsr_set_error("hsr_cic: (ng[0]+1)(ng[1]+1)(ng[2]+1) must be below 2^32 - 1");
+    return HSR_ERR_ARG;
+  }
   k_fill0<<<nblk(ncell, 256), 256, 0, st>>>(ncell, rho);
   HSR_HIP(hipGetLastError());
   if (n <= 0) return HSR_OK;
   const float cc = gpscal * gpscal * gpscal;  // Particles.cxx:606
-  const int64_t m = 8 * n;
+  Ext ex;
+  ex.e1 = (uint32_t)ng[1] + 1u;
+  ex.e2 = (uint32_t)ng[2] + 1u;
+  ex.out = (uint32_t)ext;
   int bits = 1;
-  while (bits < 32 && ((uint64_t)g.safe >> bits) != 0) ++bits;
-  // per contribution: key, key', val, val' (4 x 4 B), its term and the
-  // sorted terms (2 x 8 B) and the long-run queue (8 B; at most one run per
-  // contribution)
-  const size_t mm = (size_t)m;
-  HSR_HIP(dbuf_reserve(c->sub.keys, 40 * mm + 1024));
-  uint32_t *key = dptr<uint32_t>(c->sub.keys);
-  uint32_t *key2 = key + mm;
-  int32_t *val = (int32_t *)(key2 + mm);
-  int32_t *val2 = val + mm;
-  double *term = (double *)(val2 + mm);  // 16 m bytes in: 8-byte aligned
-  double *ts = term + mm;
-  int64_t *long_runs = (int64_t *)(ts + mm);
-  unsigned long long *nlong = (unsigned long long *)(long_runs + mm);
-  int64_t *tail_start = (int64_t *)(nlong + 1);
+  while (bits < 32 && ((uint64_t)ex.out >> bits) != 0) ++bits;
+  // layout (8-byte arrays first): per grid cell the long-cell count, offset
+  // and run start (3 x 8 B); per staged contribution its term (8 B; at most
+  // 8 per particle); then 4-byte arrays: per particle its
+  // key, sorted key, index, sorted index, outside count and its prefix; per
+  // base cell its range; per grid cell the long cell's id; per staged
+  // contribution its run key; the two counters
+  const size_t nn = (size_t)n, nc = (size_t)g.safe, ms = 8 * nn, ne = (size_t)ex.out;
+  // (each array starts on a 256-byte boundary: 16 arrays)
+  const size_t bytes = 8 * (3 * nc + ms) + 4 * (6 * nn + 2 * ne + nc + ms) + 16 * 256;
+  HSR_HIP(dbuf_reserve(c->sub.keys, bytes));
+  char *bp = (char *)c->sub.keys.p;
+  auto take = [&bp](size_t b) {
+    char *r = bp;
+    bp += (b + 255) & ~(size_t)255;
+    return r;
+  };
+  int64_t *long_k = (int64_t *)take(8 * nc);
+  int64_t *long_base = (int64_t *)take(8 * nc);
+  int64_t *long_runs = (int64_t *)take(8 * nc);
+  double *ts = (double *)take(8 * ms);
+  uint32_t *key = (uint32_t *)take(4 * nn);
+  uint32_t *key2 = (uint32_t *)take(4 * nn);
+  int32_t *idx = (int32_t *)take(4 * nn);
+  int32_t *idx2 = (int32_t *)take(4 * nn);
+  int32_t *nout = (int32_t *)take(4 * nn);
+  int32_t *out_base = (int32_t *)take(4 * nn);
+  int32_t *beg = (int32_t *)take(4 * ne);
+  int32_t *end = (int32_t *)take(4 * ne);
+  uint32_t *long_cell = (uint32_t *)take(4 * nc);
+  uint32_t *skey = (uint32_t *)take(4 * ms);
+  unsigned long long *nlong = (unsigned long long *)take(8);
+  int64_t *tail_start = (int64_t *)take(8);
+  if ((size_t)(bp - (char *)c->sub.keys.p) > c->sub.keys.bytes) {
+    hsr_set_error("hsr_cic: staging layout exceeds its buffer");
+    return HSR_ERR_STATE;
+  }
   k_zero_u64<<<1, 64, 0, st>>>(nlong, tail_start);
-  k_cic_contrib<<<nblk(n, 256), 256, 0, st>>>(n, x, y, z, g, cc, key, val, term);
-  HSR_HIP(hipGetLastError());
+  // particles by base cell (stable)
+  k_cic_base<<<nblk(n, 256), 256, 0, st>>>(n, x, y, z, g, ex, key, idx, nout);
   size_t tmp = 0;
-  HSR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, key, key2, val, val2, (int)m, 0, bits, st));
-  HSR_HIP(dbuf_reserve(c->s.cub, tmp));
-  HSR_HIP(hipcub::DeviceRadixSort::SortPairs(c->s.cub.p, tmp, key, key2, val, val2, (int)m, 0, bits, st));
-  k_cic_gather<<<nblk(m, 256), 256, 0, st>>>(m, val2, term, ts);
-  k_cic_runs<<<nblk(m, 64 * RUNS_WAVES), 64 * RUNS_WAVES, 0, st>>>(m, key2, ts, rho, long_runs, nlong,
+  HSR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, key, key2, idx, idx2, (int)n, 0, bits, st));
+  HSR_HIP(dbuf_reserve(c->s.cub, tmp + 256));
+  HSR_HIP(hipcub::DeviceRadixSort::SortPairs(c->s.cub.p, tmp, key, key2, idx, idx2, (int)n, 0, bits, st));
+  k_fill_i32<<<nblk((int64_t)ne, 256), 256, 0, st>>>((int64_t)ne, beg);
+  k_fill_i32<<<nblk((int64_t)ne, 256), 256, 0, st>>>((int64_t)ne, end);
+  k_cic_ranges<<<nblk(n, 256), 256, 0, st>>>(n, key2, ex, beg, end);
+  // short cells summed, long ones queued
+  k_cic_cells<<<nblk((int64_t)nc, 256), 256, 0, st>>>(g, ex, beg, end, idx2, x, y, z, cc, rho, long_cell,
+                                                      long_k, nlong);
+  hsr_status s;
+  if ((s = pm_excl_scan(c, nout, out_base, n)) != HSR_OK) return s;
+  unsigned long long nl_h = 0;
+  int32_t last[2] = {0, 0};
+  HSR_HIP(hipMemcpyAsync(&nl_h, nlong, sizeof(nl_h), hipMemcpyDeviceToHost, st));
+  HSR_HIP(hipMemcpyAsync(&last[0], out_base + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HSR_HIP(hipMemcpyAsync(&last[1], nout + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HSR_HIP(stream_wait(st));
+  const int64_t nl = (int64_t)nl_h, total_out = (int64_t)last[0] + last[1];
+  int64_t total_long = 0;
+  if (nl > 0) {
+    if ((s = pm_excl_scan(c, long_k, long_base, nl)) != HSR_OK) return s;
+    int64_t lb[2] = {0, 0};
+    HSR_HIP(hipMemcpyAsync(&lb[0], long_base + nl - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HSR_HIP(hipMemcpyAsync(&lb[1], long_k + nl - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HSR_HIP(stream_wait(st));
+    total_long = lb[0] + lb[1];
+    // every corner of every particle is either in one cell's lists or
+    // outside the grid: the staging holds at most 8 per particle
+    if ((size_t)(total_long + total_out) > ms) {
+      hsr_set_error("hsr_cic: staged contributions exceed 8 per particle");
+      return HSR_ERR_STATE;
+    }
+    k_cic_long_order<<<(unsigned)std::min<int64_t>(nl, 65536), 256, 0, st>>>(
+        g, ex, beg, end, idx2, long_cell, long_base, nl, x, y, z, cc, skey, ts);
+  }
+  const int64_t m = total_long + total_out;
+  if ((size_t)m > ms) {
+    hsr_set_error("hsr_cic: staged contributions exceed 8 per particle");
+    return HSR_ERR_STATE;
+  }
+  if (total_out > 0)
+    k_cic_out_terms<<<nblk(n, 256), 256, 0, st>>>(n, x, y, z, g, cc, nout, out_base, total_long, skey, ts);
+  k_set_runs<<<nblk(std::max<int64_t>(nl, 1), 256), 256, 0, st>>>(nl, long_base, long_runs,
+                                                                  total_out > 0 ? total_long : -1,
                                                                   tail_start);
-  const int waves = std::max(1, c->num_cus) * 32;  // latency-bound: many waves in flight
-  k_cic_long<<<waves, 64, 0, st>>>(m, key2, ts, rho, long_runs, nlong);
-  k_cic_tail<<<1, TAIL_THREADS, 0, st>>>(m, key2, ts, rho, tail_start);
+  if (m > 0) {
+    const int waves = std::max(1, c->num_cus) * 32;  // latency-bound: many waves in flight
+    k_cic_long<<<waves, 64, 0, st>>>(m, skey, ts, rho, long_runs, nlong);
+    k_cic_tail<<<1, TAIL_THREADS, 0, st>>>(m, skey, ts, rho, tail_start);
+  }
   HSR_HIP(hipGetLastError());
   return HSR_OK;
 }

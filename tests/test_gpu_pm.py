@@ -48,6 +48,34 @@ def test_cic_deposit_bitexact(n, ng, seed):
     assert bad.size == 0, f"cells differ: {bad[:5]} ({got[bad[:3]]} vs {want[bad[:3]]})"
 
 
+@pytest.mark.parametrize("order", ["cell", "mixed"])
+def test_cic_deposit_orders(order):
+    """The deposit merges each cell's eight neighbour lists by particle
+    index: particles in cell order (as the resort leaves them) make the lists
+    disjoint index ranges, concatenated in order; a cell order with every
+    8th particle swapped to a random place leaves some dense cells' lists
+    interleaved, which are merged one contribution at a time.  Both
+    bit-exact, dense clumps (long cells) and the outside slot included."""
+    say(f"case cic input order {order}")
+    ng = (40, 36, 44)
+    x, y, z = positions(600000, ng, 11, clump=0.5)
+    ix, iy, iz = (np.floor(a).astype(np.int64) for a in (x, y, z))
+    inb = (ix >= 0) & (ix < ng[0]) & (iy >= 0) & (iy < ng[1]) & (iz >= 0) & (iz < ng[2])
+    key = np.where(inb, (ix * ng[1] + iy) * ng[2] + iz, ng[0] * ng[1] * ng[2])
+    perm = np.argsort(key, kind="stable")
+    if order == "mixed":
+        rng = np.random.default_rng(12)
+        sel = np.arange(0, perm.shape[0], 8)
+        perm[sel] = perm[sel][rng.permutation(sel.shape[0])]
+    x, y, z = x[perm].copy(), y[perm].copy(), z[perm].copy()
+    want = O.cic(x, y, z, ng, gpscal=1.25)
+    ctx = Context()
+    got = ctx.cic(x, y, z, ng, gpscal=1.25)
+    ctx.close()
+    bad = np.nonzero(bits(got) != bits(want))[0]
+    assert bad.size == 0, f"cells differ: {bad[:5]} ({got[bad[:3]]} vs {want[bad[:3]]})"
+
+
 def test_cic_device_tensors_and_reuse():
     """Device arrays in place; a second call on the same grid starts from
     zero again."""
