@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <climits>
+
 #include "softdouble.h"
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
@@ -13,25 +15,68 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
   return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
 
+// Exact, order-independent reductions (integer min / max / sum, float min /
+// max) by DPP: an inclusive scan within each row of 16 lanes (row_shr 1, 2,
+// 4, 8), then row 0's total into row 1 and row 2's into row 3 (row_bcast
+// 15), then rows 0..1's into rows 2..3 (row_bcast 31); lane 63 holds the
+// wave's result, which readlane broadcasts.  Disabled source lanes give the
+// identity.
+template <int CTRL, int RMASK>
+__device__ __forceinline__ int dpp_i32(int v, int ident) {
+  return __builtin_amdgcn_update_dpp(ident, v, CTRL, RMASK, 0xf, false);
+}
+
+#define HSR_DPP_REDUCE(v, ident, OP)                        \
+  do {                                                      \
+    v = OP(v, dpp_i32<0x111, 0xf>(v, ident));               \
+    v = OP(v, dpp_i32<0x112, 0xf>(v, ident));               \
+    v = OP(v, dpp_i32<0x114, 0xf>(v, ident));               \
+    v = OP(v, dpp_i32<0x118, 0xf>(v, ident));               \
+    v = OP(v, dpp_i32<0x142, 0xa>(v, ident));               \
+    v = OP(v, dpp_i32<0x143, 0xc>(v, ident));               \
+  } while (0)
+
+__device__ __forceinline__ int dpp_op_min(int a, int b) { return min(a, b); }
+__device__ __forceinline__ int dpp_op_max(int a, int b) { return max(a, b); }
+__device__ __forceinline__ int dpp_op_fmin(int a, int b) {
+  return __float_as_int(fminf(__int_as_float(a), __int_as_float(b)));
+}
+__device__ __forceinline__ int dpp_op_fmax(int a, int b) {
+  return __float_as_int(fmaxf(__int_as_float(a), __int_as_float(b)));
+}
+
 __device__ __forceinline__ int wave_min_i32(int v) {
-  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
-  return v;
+  HSR_DPP_REDUCE(v, INT_MAX, dpp_op_min);
+  return __builtin_amdgcn_readlane(v, 63);
 }
 
 __device__ __forceinline__ int wave_max_i32(int v) {
-  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
-  return v;
+  HSR_DPP_REDUCE(v, INT_MIN, dpp_op_max);
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+template <int CTRL, int RMASK>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, RMASK, 0xf, false);
+  const uint32_t hi =
+      (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, RMASK, 0xf, false);
+  return ((uint64_t)hi << 32) | lo;
 }
 
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
-  for (int o = 32; o > 0; o >>= 1)
-    v += (uint64_t)__shfl_xor((unsigned long long)v, o);
-  return v;
+  v += dpp_u64<0x111, 0xf>(v);
+  v += dpp_u64<0x112, 0xf>(v);
+  v += dpp_u64<0x114, 0xf>(v);
+  v += dpp_u64<0x118, 0xf>(v);
+  v += dpp_u64<0x142, 0xa>(v);
+  v += dpp_u64<0x143, 0xc>(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63);
+  return ((uint64_t)hi << 32) | lo;
 }
 
 __device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
-  for (int o = 32; o > 0; o >>= 1) v += (int64_t)__shfl_xor((long long)v, o);
-  return v;
+  return (int64_t)wave_sum_u64((uint64_t)v);
 }
 
 __device__ __forceinline__ float wave_sum_f32(float v) {
@@ -40,13 +85,15 @@ __device__ __forceinline__ float wave_sum_f32(float v) {
 }
 
 __device__ __forceinline__ float wave_min_f32(float v) {
-  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
-  return v;
+  int b = __float_as_int(v);
+  HSR_DPP_REDUCE(b, __float_as_int(INFINITY), dpp_op_fmin);
+  return __int_as_float(__builtin_amdgcn_readlane(b, 63));
 }
 
 __device__ __forceinline__ float wave_max_f32(float v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
-  return v;
+  int b = __float_as_int(v);
+  HSR_DPP_REDUCE(b, __float_as_int(-INFINITY), dpp_op_fmax);
+  return __int_as_float(__builtin_amdgcn_readlane(b, 63));
 }
 
 // In-order composition of the lanes' maps (lane 0's element first); the
