@@ -350,8 +350,9 @@ def pm_leg(load, dev, reps=3):
             "inverse_cic": {"ms": i_ms, "algorithmic_bytes": ib,
                             "GB/s": ib / (i_ms * 1e-3) / 1e9,
                             "frac": ib / (i_ms * 1e-3) / 1e9 / PEAK_HBM_GBPS},
-            "note": "deposit bit-exact with the reference's sequential float order: "
-                    "8 contributions per particle sorted by cell, runs added in order"}
+            "note": "deposit bit-exact with the reference's sequential float order: particles "
+                    "sorted by base cell, each cell's eight neighbour lists merged by particle "
+                    "index and its terms added in that order"}
 
 
 def refresh_leg(load, dev, reps=3):
