@@ -126,7 +126,7 @@ def load(path: str | None = None):
     global _lib
     if _lib is not None:
         return _lib
-    p = path or os.environ.get("HACCSR_LIB") or LIB_PATH
+    p = path or os.environ.get("HSR_LIBRARY") or LIB_PATH
     if not os.path.exists(p):
         raise HsrError(f"{p} is missing: build it with `make -C hacc-coral_amd`")
     # PyTorch ships its own HIP runtime.  Loaded first, its symbols are the
