@@ -16,11 +16,11 @@
 // in parallel it sorts the particles stably by base cell; a grid cell's
 // contributions are then the merge, by particle index, of the particle
 // lists of its eight neighbouring base cells.  A cell with at most 64 is
-// merged and added by its own thread; longer cells (dense clumps) have
-// their merged order staged and added by a wave each, 256 contributions at
-// a time through exact fixed-grid maps of the float accumulation; the slot
-// for corners outside the grid gets its contributions in (particle, corner)
-// order and a workgroup.  The interpolation is a gather, one thread per
+// merged and added by its own thread; a longer cell (dense clumps) by a
+// workgroup, which forms its terms in merged order into LDS and adds them,
+// 256 at a time, through exact fixed-grid maps of the float accumulation;
+// the slot for corners outside the grid gets its contributions in
+// (particle, corner) order and a workgroup.  The interpolation is a gather, one thread per
 // particle.
 #include "hsr_internal.h"
 
@@ -152,123 +152,100 @@ __device__ __forceinline__ PMap pmap_term(double t, int E) {
   return m;
 }
 
-// One wave per long run: the lanes load LT consecutive contributions each,
-// 64 LT at a time.  With all terms nonnegative (the deposit's weights are)
-// the wave applies them by the map scan above -- each lane composes its LT
-// terms in order, then the lanes' maps are scanned -- and applies the
-// longest prefix that stays in the binade; in the lane where the sum leaves
-// it the terms are stepped one at a time to the crossing term, which is
-// added directly.  Otherwise the terms are added in order, reading them
-// with v_readlane.  Every lane carries the same sum.  Waves loop over the
-// queue; each exits when the queue is drained.
+// One wave adds 64 LT terms of a cell's contributions in order: lane l
+// holds terms LT l .. LT l + LT - 1, the first cnt of the step live.  With
+// all terms nonnegative (the deposit's weights are) it applies them by the
+// map scan above -- each lane composes its LT terms in order, then the
+// lanes' maps are scanned -- and applies the longest prefix that stays in
+// the binade; in the lane where the sum leaves it the terms are stepped one
+// at a time to the crossing term, which is added directly.  Otherwise the
+// terms are added in order, reading them with v_readlane.  r is the same in
+// every lane.
 constexpr int LT = 4;
 
 __device__ __forceinline__ double pick(const double (&t)[LT], int k) {
   return k == 0 ? t[0] : k == 1 ? t[1] : k == 2 ? t[2] : t[3];
 }
 
-__global__ void __launch_bounds__(64)
-k_cic_long(int64_t m, const uint32_t *__restrict__ key, const double *__restrict__ ts,
-           float *__restrict__ rho, const int64_t *__restrict__ long_runs,
-           const unsigned long long *__restrict__ nlong) {
+__device__ __forceinline__ float wave_add_step(float r, const double (&t)[LT], int cnt) {
   static_assert(LT == 4, "pick() reads four terms per lane");
-  const int l = threadIdx.x;
-  const int64_t nruns = (int64_t)*nlong;
-  for (int64_t run = blockIdx.x; run < nruns; run += gridDim.x) {
-    const int64_t p0 = long_runs[run];
-    const uint32_t cell = key[p0];
-    float r = 0.f;
-    bool done = false;
-    for (int64_t base = p0; !done; base += 64 * LT) {
-      double t[LT];
-      int mine = 0;
-      bool neg = false;
+  const int l = threadIdx.x & 63;
+  bool neg = false;
 #pragma unroll
-      for (int k = 0; k < LT; ++k) {
-        const int64_t q = base + LT * l + k;
-        const bool in = q < m && key[q] == cell;
-        t[k] = in ? ts[q] : 0.0;
-        mine += in;
-        neg = neg || (in && !(t[k] >= 0.0));
-      }
-      // contributions of one cell are contiguous: the live terms are a prefix
-      int cnt = mine;
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
-      done = cnt < 64 * LT;
-      if (__ballot(neg) != 0ull) {
-        for (int i = 0; i < cnt; ++i) r = cic_add(r, readlane_f64(pick(t, i % LT), i / LT));
-        continue;
-      }
-      int s = 0;
-      while (s < cnt) {
-        if (!(r >= 0x1p-125f)) {  // zero or subnormal: the double sum's grid varies
-          r = cic_add(r, readlane_f64(pick(t, s % LT), s / LT));
-          ++s;
-          continue;
-        }
-        const int E = ilogbf(r);
-        const double u = ldexp(1.0, E - 23);
-        const int R = (int)((double)r / u);
-        PMap mp = PMap{0, 0};
-#pragma unroll
-        for (int k = 0; k < LT; ++k) {
-          const int i = LT * l + k;
-          if (i >= s && i < cnt) mp = pmap_compose(mp, pmap_term(t[k], E));
-        }
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-          PMap pv;
-          pv.d0 = __shfl_up(mp.d0, off);
-          pv.d1 = __shfl_up(mp.d1, off);
-          if (l >= off) mp = pmap_compose(pv, mp);
-        }
-        const int d = (R & 1) ? mp.d1 : mp.d0;
-        const int Rl = d >= PMAP_SAT ? PMAP_SAT : R + d;
-        // R only grows: the lanes past the crossing one cross too
-        const unsigned long long cross = __ballot(Rl >= (1 << 24));
-        if (!cross) {
-          r = (float)((double)__shfl(Rl, 63) * u);
-          s = cnt;
-          continue;
-        }
-        const int j = __ffsll((long long)cross) - 1;
-        // the lanes before j applied, then lane j's terms one at a time
-        int Rj = j > 0 ? __shfl(Rl, j - 1) : R;
-        for (int k = 0; k < LT; ++k) {  // uniform
-          const int i = LT * j + k;
-          if (i < s || i >= cnt) continue;
-          const double tk = readlane_f64(pick(t, k), j);
-          const PMap m1 = pmap_term(tk, E);
-          const int dk = (Rj & 1) ? m1.d1 : m1.d0;
-          if (dk >= PMAP_SAT || Rj + dk >= (1 << 24)) {  // the crossing term
-            r = cic_add((float)((double)Rj * u), tk);
-            s = i + 1;
-            break;
-          }
-          Rj += dk;
-        }
-      }
-    }
-    if (l == 0) rho[cell] = r;
+  for (int k = 0; k < LT; ++k) {
+    const int i = LT * l + k;
+    neg = neg || (i < cnt && !(t[k] >= 0.0));
   }
+  if (__ballot(neg) != 0ull) {
+    for (int i = 0; i < cnt; ++i) r = cic_add(r, readlane_f64(pick(t, i % LT), i / LT));
+    return r;
+  }
+  int s = 0;
+  while (s < cnt) {
+    if (!(r >= 0x1p-125f)) {  // zero or subnormal: the double sum's grid varies
+      r = cic_add(r, readlane_f64(pick(t, s % LT), s / LT));
+      ++s;
+      continue;
+    }
+    const int E = ilogbf(r);
+    const double u = ldexp(1.0, E - 23);
+    const int R = (int)((double)r / u);
+    PMap mp = PMap{0, 0};
+#pragma unroll
+    for (int k = 0; k < LT; ++k) {
+      const int i = LT * l + k;
+      if (i >= s && i < cnt) mp = pmap_compose(mp, pmap_term(t[k], E));
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      PMap pv;
+      pv.d0 = __shfl_up(mp.d0, off);
+      pv.d1 = __shfl_up(mp.d1, off);
+      if (l >= off) mp = pmap_compose(pv, mp);
+    }
+    const int d = (R & 1) ? mp.d1 : mp.d0;
+    const int Rl = d >= PMAP_SAT ? PMAP_SAT : R + d;
+    // R only grows: the lanes past the crossing one cross too
+    const unsigned long long cross = __ballot(Rl >= (1 << 24));
+    if (!cross) {
+      r = (float)((double)__shfl(Rl, 63) * u);
+      s = cnt;
+      continue;
+    }
+    const int j = __ffsll((long long)cross) - 1;
+    // the lanes before j applied, then lane j's terms one at a time
+    int Rj = j > 0 ? __shfl(Rl, j - 1) : R;
+    for (int k = 0; k < LT; ++k) {  // uniform
+      const int i = LT * j + k;
+      if (i < s || i >= cnt) continue;
+      const double tk = readlane_f64(pick(t, k), j);
+      const PMap m1 = pmap_term(tk, E);
+      const int dk = (Rj & 1) ? m1.d1 : m1.d0;
+      if (dk >= PMAP_SAT || Rj + dk >= (1 << 24)) {  // the crossing term
+        r = cic_add((float)((double)Rj * u), tk);
+        s = i + 1;
+        break;
+      }
+      Rj += dk;
+    }
+  }
+  return r;
 }
 
 constexpr int TAIL_THREADS = 1024;
 
-// The last run [p0, m) by one workgroup: LT TAIL_THREADS terms per step, LT
+// The outside slot's terms [0, m) by one workgroup: LT TAIL_THREADS terms per step, LT
 // per thread composed in order, maps scanned within each wave and across
 // the 16 waves through LDS, the longest prefix that stays in the binade
 // applied; the thread holding the crossing term steps its terms to it and
 // adds it directly.
 __global__ void __launch_bounds__(TAIL_THREADS)
 k_cic_tail(int64_t m, const uint32_t *__restrict__ key, const double *__restrict__ ts,
-           float *__restrict__ rho, const int64_t *__restrict__ tail_start) {
+           float *__restrict__ rho) {
   __shared__ PMap wtot[TAIL_THREADS / 64];
   __shared__ float r_sh;
   __shared__ int j_sh, neg_sh, s_sh, rj_sh;
-  const int64_t p0 = *tail_start;
-  if (p0 < 0) return;
+  const int64_t p0 = 0;
   const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
   if (tid == 0) r_sh = 0.f;
   __syncthreads();
@@ -524,28 +501,35 @@ k_cic_cells(Grid g, Ext ex, const int32_t *__restrict__ beg, const int32_t *__re
   rho[cid] = r;
 }
 
-// One workgroup per long cell: its contributions' terms in merged order
-// written at its staging offset, with the cell as the run key.  When the
-// eight lists hold disjoint index ranges -- the particles of a base cell
-// are contiguous when the input is in cell order, as the resort leaves it
-// -- the merge is the lists in the order of their first particles, copied
-// by all threads; otherwise thread 0 merges them one contribution at a time.
-__global__ void __launch_bounds__(256)
-k_cic_long_order(Grid g, Ext ex, const int32_t *__restrict__ beg, const int32_t *__restrict__ end,
-                 const int32_t *__restrict__ idx, const uint32_t *__restrict__ long_cell,
-                 const int64_t *__restrict__ long_base, int64_t nl, const float *__restrict__ x,
-                 const float *__restrict__ y, const float *__restrict__ z, float c,
-                 uint32_t *__restrict__ skey, double *__restrict__ ts) {
+// One wave per long cell: its contributions' terms formed in merged order
+// into LDS, LC_CHUNK at a time, and added in order (wave_add_step).  When
+// the eight lists hold disjoint index ranges -- the particles of a base
+// cell are contiguous when the input is in cell order, as the resort leaves
+// it -- the merge is the lists in the order of their first particles, and
+// all lanes form the terms; otherwise lane 0 merges them one contribution
+// at a time, its merge state carried across chunks.  A wave per workgroup
+// keeps many cells' latency-bound sums in flight on each CU.
+constexpr int LC_CHUNK = 2 * 64 * LT;
+
+__global__ void __launch_bounds__(64)
+k_cic_long_sum(Grid g, Ext ex, const int32_t *__restrict__ beg, const int32_t *__restrict__ end,
+               const int32_t *__restrict__ idx, const uint32_t *__restrict__ long_cell,
+               const int64_t *__restrict__ long_k, const unsigned long long *__restrict__ nlong,
+               const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+               float c, float *__restrict__ rho) {
+  __shared__ double tb[LC_CHUNK];
   __shared__ int32_t ss[8], se[8], soff[8];
   __shared__ int disjoint;
+  const int tid = threadIdx.x;
+  const int64_t nl = (int64_t)*nlong;
   for (int64_t t = blockIdx.x; t < nl; t += gridDim.x) {
     const uint32_t cid = long_cell[t];
-    const int64_t base = long_base[t];
-    if (threadIdx.x == 0) {
+    const int64_t K = long_k[t];
+    int32_t s[8], e[8], hv[8];  // thread 0's merge state
+    if (tid == 0) {
       const int k = (int)(cid % (uint32_t)g.ng2);
       const uint32_t ij = cid / (uint32_t)g.ng2;
       const int j = (int)(ij % (uint32_t)g.ng1), i = (int)(ij / (uint32_t)g.ng1);
-      int32_t s[8], e[8];
       cell_lists(i, j, k, ex, beg, end, s, e);
       // each non-empty list's first and last particle; the lists ordered by
       // their first (a selection sort of eight)
@@ -558,6 +542,7 @@ k_cic_long_order(Grid g, Ext ex, const int32_t *__restrict__ beg, const int32_t 
         soff[q] = 0;
         first[q] = s[q] < e[q] ? idx[s[q]] : INT_MAX;
         last[q] = s[q] < e[q] ? idx[e[q] - 1] : INT_MAX;
+        hv[q] = first[q];
       }
       for (int q = 0; q < 8; ++q)
         if (s[q] < e[q]) ord[nq++] = q;
@@ -576,36 +561,54 @@ k_cic_long_order(Grid g, Ext ex, const int32_t *__restrict__ beg, const int32_t 
         off += e[q] - s[q];
       }
       disjoint = dj;
-      if (!dj) {
-        int32_t hv[8];
+    }
+    __syncthreads();
+    float r = 0.f;
+    for (int64_t c0 = 0; c0 < K; c0 += LC_CHUNK) {
+      const int cl = (int)min<int64_t>(LC_CHUNK, K - c0);
+      if (disjoint) {
+        for (int u = tid; u < cl; u += blockDim.x) {
+          const int pos = (int)c0 + u;
+          // the list whose merged range holds pos (the ranges tile [0, K))
+          int q = -1;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) hv[q] = s[q] < e[q] ? idx[s[q]] : INT_MAX;
-        for (int u = 0; u < off; ++u) {
+          for (int qq = 0; qq < 8; ++qq)
+            if (pos >= soff[qq] && pos < soff[qq] + (se[qq] - ss[qq])) q = qq;
+          double term = 0.0;
+          if (q >= 0) {
+            const int32_t nn = idx[ss[q] + (pos - soff[q])];
+            float ab, de, gh;
+            weights(x[nn], y[nn], z[nn], ab, de, gh);
+            term = corner_term(q, c, ab, de, gh);
+          }
+          tb[u] = term;
+        }
+      } else if (tid == 0) {
+        for (int u = 0; u < cl; ++u) {
           const int qs = merge_pick(hv);
           const int32_t nn = hv[qs];
           float ab, de, gh;
           weights(x[nn], y[nn], z[nn], ab, de, gh);
-          skey[base + u] = cid;
-          ts[base + u] = corner_term(qs, c, ab, de, gh);
+          tb[u] = corner_term(qs, c, ab, de, gh);
           merge_advance(qs, s, e, hv, idx);
         }
       }
-    }
-    __syncthreads();
-    if (disjoint) {
-      for (int q = 0; q < 8; ++q) {
-        const int32_t s0 = ss[q], len = se[q] - ss[q];
-        const int64_t o = base + soff[q];
-        for (int u = threadIdx.x; u < len; u += blockDim.x) {
-          const int32_t nn = idx[s0 + u];
-          float ab, de, gh;
-          weights(x[nn], y[nn], z[nn], ab, de, gh);
-          skey[o + u] = cid;
-          ts[o + u] = corner_term(q, c, ab, de, gh);
+      __syncthreads();
+      {
+        for (int b = 0; b < cl; b += 64 * LT) {
+          const int cnt = min(64 * LT, cl - b);
+          double tt[LT];
+#pragma unroll
+          for (int k = 0; k < LT; ++k) {
+            const int i = LT * tid + k;
+            tt[k] = i < cnt ? tb[b + i] : 0.0;
+          }
+          r = wave_add_step(r, tt, cnt);
         }
       }
+      __syncthreads();  // tb is rewritten by the next chunk
     }
-    __syncthreads();  // the shared lists are rewritten for the next cell
+    if (tid == 0) rho[cid] = r;
   }
 }
 
@@ -629,12 +632,6 @@ k_cic_out_terms(int64_t n, const float *__restrict__ x, const float *__restrict_
     }
 }
 
-__global__ void k_set_runs(int64_t nl, const int64_t *__restrict__ long_base,
-                           int64_t *__restrict__ long_runs, int64_t tail, int64_t *__restrict__ tail_start) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t < nl) long_runs[t] = long_base[t];
-  if (t == 0) *tail_start = tail;
-}
 
 // Particles.cxx:682-711, one component: f accumulated as the deposit's runs
 // are (grid[safe] read as 0), then v += f * fscal * tau in float.
@@ -656,11 +653,8 @@ k_inverse_cic(int64_t n, const float *__restrict__ x, const float *__restrict__ 
   v[nn] = v[nn] + f * fscal * tau;
 }
 
-__global__ void k_zero_u64(unsigned long long *p, int64_t *tail) {
-  if (threadIdx.x == 0) {
-    p[0] = 0ull;
-    tail[0] = -1;
-  }
+__global__ void k_zero_u64(unsigned long long *p) {
+  if (threadIdx.x == 0) p[0] = 0ull;
 }
 
 __global__ void k_fill0(int64_t n, float *a) {
@@ -717,15 +711,15 @@ hsr_status cic_run(hsr_ctx *c, int64_t n, const float *x, const float *y, const 
   ex.out = (uint32_t)ext;
   int bits = 1;
   while (bits < 32 && ((uint64_t)ex.out >> bits) != 0) ++bits;
-  // layout (8-byte arrays first): per grid cell the long-cell count, offset
-  // and run start (3 x 8 B); per staged contribution its term (8 B; at most
-  // 8 per particle); then 4-byte arrays: per particle its
-  // key, sorted key, index, sorted index, outside count and its prefix; per
-  // base cell its range; per grid cell the long cell's id; per staged
-  // contribution its run key; the two counters
+  // layout (8-byte arrays first): per grid cell the long-cell count (8 B);
+  // per staged contribution of the outside slot its term (8 B; at most 8
+  // per particle); then 4-byte arrays: per particle its key, sorted key,
+  // index, sorted index, outside count and its prefix; per base cell its
+  // range; per grid cell the long cell's id; per staged contribution its
+  // run key; the long-cell counter
   const size_t nn = (size_t)n, nc = (size_t)g.safe, ms = 8 * nn, ne = (size_t)ex.out;
-  // (each array starts on a 256-byte boundary: 16 arrays)
-  const size_t bytes = 8 * (3 * nc + ms) + 4 * (6 * nn + 2 * ne + nc + ms) + 16 * 256;
+  // (each array starts on a 256-byte boundary: 14 arrays)
+  const size_t bytes = 8 * (nc + ms) + 4 * (6 * nn + 2 * ne + nc + ms) + 8 + 14 * 256;
   HSR_HIP(dbuf_reserve(c->sub.keys, bytes));
   char *bp = (char *)c->sub.keys.p;
   auto take = [&bp](size_t b) {
@@ -734,8 +728,6 @@ hsr_status cic_run(hsr_ctx *c, int64_t n, const float *x, const float *y, const 
     return r;
   };
   int64_t *long_k = (int64_t *)take(8 * nc);
-  int64_t *long_base = (int64_t *)take(8 * nc);
-  int64_t *long_runs = (int64_t *)take(8 * nc);
   double *ts = (double *)take(8 * ms);
   uint32_t *key = (uint32_t *)take(4 * nn);
   uint32_t *key2 = (uint32_t *)take(4 * nn);
@@ -748,12 +740,11 @@ hsr_status cic_run(hsr_ctx *c, int64_t n, const float *x, const float *y, const 
   uint32_t *long_cell = (uint32_t *)take(4 * nc);
   uint32_t *skey = (uint32_t *)take(4 * ms);
   unsigned long long *nlong = (unsigned long long *)take(8);
-  int64_t *tail_start = (int64_t *)take(8);
   if ((size_t)(bp - (char *)c->sub.keys.p) > c->sub.keys.bytes) {
     hsr_set_error("hsr_cic: staging layout exceeds its buffer");
     return HSR_ERR_STATE;
   }
-  k_zero_u64<<<1, 64, 0, st>>>(nlong, tail_start);
+  k_zero_u64<<<1, 64, 0, st>>>(nlong);
   // particles by base cell (stable)
   k_cic_base<<<nblk(n, 256), 256, 0, st>>>(n, x, y, z, g, ex, key, idx, nout);
   size_t tmp = 0;
@@ -763,49 +754,27 @@ hsr_status cic_run(hsr_ctx *c, int64_t n, const float *x, const float *y, const 
   k_fill_i32<<<nblk((int64_t)ne, 256), 256, 0, st>>>((int64_t)ne, beg);
   k_fill_i32<<<nblk((int64_t)ne, 256), 256, 0, st>>>((int64_t)ne, end);
   k_cic_ranges<<<nblk(n, 256), 256, 0, st>>>(n, key2, ex, beg, end);
-  // short cells summed, long ones queued
+  // short cells summed, long ones queued and summed a workgroup each (the
+  // grid loops over the device-side count)
   k_cic_cells<<<nblk((int64_t)nc, 256), 256, 0, st>>>(g, ex, beg, end, idx2, x, y, z, cc, rho, long_cell,
                                                       long_k, nlong);
+  const unsigned lgrid = (unsigned)std::max(1, c->num_cus) * 32;
+  k_cic_long_sum<<<lgrid, 64, 0, st>>>(g, ex, beg, end, idx2, long_cell, long_k, nlong, x, y, z, cc, rho);
+  // the outside slot: its contributions in (particle, corner) order
   hsr_status s;
   if ((s = pm_excl_scan(c, nout, out_base, n)) != HSR_OK) return s;
-  unsigned long long nl_h = 0;
   int32_t last[2] = {0, 0};
-  HSR_HIP(hipMemcpyAsync(&nl_h, nlong, sizeof(nl_h), hipMemcpyDeviceToHost, st));
   HSR_HIP(hipMemcpyAsync(&last[0], out_base + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
   HSR_HIP(hipMemcpyAsync(&last[1], nout + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
   HSR_HIP(stream_wait(st));
-  const int64_t nl = (int64_t)nl_h, total_out = (int64_t)last[0] + last[1];
-  int64_t total_long = 0;
-  if (nl > 0) {
-    if ((s = pm_excl_scan(c, long_k, long_base, nl)) != HSR_OK) return s;
-    int64_t lb[2] = {0, 0};
-    HSR_HIP(hipMemcpyAsync(&lb[0], long_base + nl - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    HSR_HIP(hipMemcpyAsync(&lb[1], long_k + nl - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    HSR_HIP(stream_wait(st));
-    total_long = lb[0] + lb[1];
-    // every corner of every particle is either in one cell's lists or
-    // outside the grid: the staging holds at most 8 per particle
-    if ((size_t)(total_long + total_out) > ms) {
-      hsr_set_error("hsr_cic: staged contributions exceed 8 per particle");
-      return HSR_ERR_STATE;
-    }
-    k_cic_long_order<<<(unsigned)std::min<int64_t>(nl, 65536), 256, 0, st>>>(
-        g, ex, beg, end, idx2, long_cell, long_base, nl, x, y, z, cc, skey, ts);
-  }
-  const int64_t m = total_long + total_out;
+  const int64_t m = (int64_t)last[0] + last[1];
   if ((size_t)m > ms) {
     hsr_set_error("hsr_cic: staged contributions exceed 8 per particle");
     return HSR_ERR_STATE;
   }
-  if (total_out > 0)
-    k_cic_out_terms<<<nblk(n, 256), 256, 0, st>>>(n, x, y, z, g, cc, nout, out_base, total_long, skey, ts);
-  k_set_runs<<<nblk(std::max<int64_t>(nl, 1), 256), 256, 0, st>>>(nl, long_base, long_runs,
-                                                                  total_out > 0 ? total_long : -1,
-                                                                  tail_start);
   if (m > 0) {
-    const int waves = std::max(1, c->num_cus) * 32;  // latency-bound: many waves in flight
-    k_cic_long<<<waves, 64, 0, st>>>(m, skey, ts, rho, long_runs, nlong);
-    k_cic_tail<<<1, TAIL_THREADS, 0, st>>>(m, skey, ts, rho, tail_start);
+    k_cic_out_terms<<<nblk(n, 256), 256, 0, st>>>(n, x, y, z, g, cc, nout, out_base, 0, skey, ts);
+    k_cic_tail<<<1, TAIL_THREADS, 0, st>>>(m, skey, ts, rho);
   }
   HSR_HIP(hipGetLastError());
   return HSR_OK;
