@@ -1604,7 +1604,7 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
   HSR_HIP(stream_wait(st));
   const bool unit = unit_h != 0;
   T.unit = unit;
-  const bool unit_ppm = unit;  // leaf pseudo masses set by k_classify
+  const bool unit_ppm = unit;  // leaf pseudo masses set by classify_node (k_part_plan)
   HSR_HIP(hipMemcpyAsync(d_box_off, box_off_h, sizeof(int64_t) * (nboxes + 1), hipMemcpyHostToDevice, st));
   HSR_HIP(hipMemcpyAsync(d_min, min_loc_h, sizeof(float) * 3 * nboxes, hipMemcpyHostToDevice, st));
   HSR_HIP(hipMemcpyAsync(d_max, max_loc_h, sizeof(float) * 3 * nboxes, hipMemcpyHostToDevice, st));
