@@ -37,6 +37,8 @@ struct Counters {
   int64_t unresolved;     // swap chains longer than MAXHOP
   int64_t cm_fail;        // small-node sums left to the ordered path
   int64_t pad[2];
+  int64_t next_big;       // children of this level with count > CM_SMALL
+  int64_t next_small;     // and with count <= CM_SMALL
 };
 
 // ------------------------------------------------------------ helpers
@@ -1241,6 +1243,10 @@ __global__ void k_make_children(BNode *nodes, int64_t lb, int64_t le,
   atomicAdd((unsigned long long *)&ctr->active_next,
             (unsigned long long)n.count);
   int64_t cl = next_base + 2 * k, cr = cl + 1;
+  // which centre-of-mass paths the next level needs
+  const int nbig = (n.is > CM_SMALL) + (n.count - n.is > CM_SMALL);
+  if (nbig > 0) atomicAdd((unsigned long long *)&ctr->next_big, (unsigned long long)nbig);
+  if (nbig < 2) atomicAdd((unsigned long long *)&ctr->next_small, (unsigned long long)(2 - nbig));
   n.cl = (int32_t)cl;
   n.cr = (int32_t)cr;
   BNode &L = nodes[cl];
@@ -1618,6 +1624,7 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
   int64_t active = n;
   Counters hc;
   int depth = 0;
+  bool run_big = true, run_small = true;
   for (;;) {
     int64_t lb = T.level_start[depth], le = T.level_start[depth + 1];
     int64_t nl = le - lb;
@@ -1653,29 +1660,36 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     const unsigned gnl = (unsigned)std::min<int64_t>(nl, gbig);
     HSR_HIP(dbuf_reserve(S.cmfail, sizeof(int32_t) * 4 * (size_t)nl));
     int32_t *cm_list = dptr<int32_t>(S.cmfail);
-    k_cm_small<<<dim3((unsigned)nl, nq), 64, 0, st>>>(nodes, lb, le, unit, Xr, Yr, Zr, Mr,
-                                                     cms, cm_list, ctr);
-    k_cm_small_ordered<<<std::min<unsigned>((unsigned)std::max<int64_t>(1, nl), 4 * gbig), 64, 0, st>>>(
-        nodes, lb, unit, Xr, Yr, Zr, Mr, cms, cm_list, ctr);
+    // the small-node and big-node paths are launched only for levels that
+    // hold such nodes (the previous level's children counts; both at the
+    // first level); each path's kernels skip the other's nodes
+    if (run_small) {
+      k_cm_small<<<dim3((unsigned)nl, nq), 64, 0, st>>>(nodes, lb, le, unit, Xr, Yr, Zr, Mr,
+                                                       cms, cm_list, ctr);
+      k_cm_small_ordered<<<std::min<unsigned>((unsigned)std::max<int64_t>(1, nl), 4 * gbig), 64, 0, st>>>(
+          nodes, lb, unit, Xr, Yr, Zr, Mr, cms, cm_list, ctr);
+    }
     HSR_HIP(dbuf_reserve(S.cnode, sizeof(int32_t) * cm_cap));
     int32_t *cnode = dptr<int32_t>(S.cnode);
-    k_cm_plan<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, cnode, ctr);
-    unsigned cmg = (unsigned)std::max<int64_t>(1, active / CM_BLK + active / CM_SMALL + 1);
-    cmg = std::min(cmg, gbig);
-    if (unit)
-      k_cm_blk_stats<true><<<cmg, 64, 0, st>>>(nodes, cnode, cmb, ctr, Xr, Yr, Zr, Mr);
-    else
-      k_cm_blk_stats<false><<<cmg, 64, 0, st>>>(nodes, cnode, cmb, ctr, Xr, Yr, Zr, Mr);
     HSR_HIP(dbuf_reserve(S.cmx, sizeof(int32_t) * (size_t)nl));
     int32_t *cmflags = dptr<int32_t>(S.cmx);
-    const unsigned cmr = std::max(1u, std::min<unsigned>(gbig, (unsigned)(cm_cap / 64 + 1)));
-    k_cm_chunk<<<cmr, 64, 0, st>>>(nodes, cnode, cmb, ctr);
-    k_cm_node<<<gnl, 64, 0, st>>>(nodes, lb, le, cmb, cmflags, cms);
-    k_cm_blk_maps<<<dim3(cmg, nq), 64, 0, st>>>(nodes, cnode, cmb, ctr, unit, Xr, Yr, Zr, Mr, lb,
-                                                cmflags);
-    k_cm_win_maps<<<cmg, 64, 0, st>>>(nodes, cnode, cmb, ctr);
-    k_cm_blk_chain<<<dim3(gnl, nq), 64, 0, st>>>(nodes, lb, le, unit, cmb, Xr, Yr, Zr, Mr, ctr, cms,
-                                                 cmflags);
+    if (run_big) {
+      k_cm_plan<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, cnode, ctr);
+      unsigned cmg = (unsigned)std::max<int64_t>(1, active / CM_BLK + active / CM_SMALL + 1);
+      cmg = std::min(cmg, gbig);
+      if (unit)
+        k_cm_blk_stats<true><<<cmg, 64, 0, st>>>(nodes, cnode, cmb, ctr, Xr, Yr, Zr, Mr);
+      else
+        k_cm_blk_stats<false><<<cmg, 64, 0, st>>>(nodes, cnode, cmb, ctr, Xr, Yr, Zr, Mr);
+      const unsigned cmr = std::max(1u, std::min<unsigned>(gbig, (unsigned)(cm_cap / 64 + 1)));
+      k_cm_chunk<<<cmr, 64, 0, st>>>(nodes, cnode, cmb, ctr);
+      k_cm_node<<<gnl, 64, 0, st>>>(nodes, lb, le, cmb, cmflags, cms);
+      k_cm_blk_maps<<<dim3(cmg, nq), 64, 0, st>>>(nodes, cnode, cmb, ctr, unit, Xr, Yr, Zr, Mr, lb,
+                                                  cmflags);
+      k_cm_win_maps<<<cmg, 64, 0, st>>>(nodes, cnode, cmb, ctr);
+      k_cm_blk_chain<<<dim3(gnl, nq), 64, 0, st>>>(nodes, lb, le, unit, cmb, Xr, Yr, Zr, Mr, ctr, cms,
+                                                   cmflags);
+    }
     // ---- centre of mass, class and partition blocks (a lane per node),
     // finalize plain leaves
     int64_t pcap = active / PB + nl + 2;
@@ -1724,6 +1738,8 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
 #endif
     T.level_start.push_back(le + 2 * hc.nsplit);
     active = hc.active_next;
+    run_big = hc.next_big > 0;
+    run_small = hc.next_small > 0;
     cur ^= 1;
     ++depth;
   }
