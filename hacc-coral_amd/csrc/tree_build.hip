@@ -39,6 +39,8 @@ struct Counters {
   int64_t pad[2];
   int64_t next_big;       // children of this level with count > CM_SMALL
   int64_t next_small;     // and with count <= CM_SMALL
+  int64_t next_leaf;      // and with count <= nd (leaves of the next level)
+  int64_t ndegen;         // degenerate splits of this level
 };
 
 // ------------------------------------------------------------ helpers
@@ -1009,7 +1011,7 @@ k_part_count(const BNode *nodes, const int32_t *pnode, PartBlock *pb, const Coun
 // RCBForceTree.cxx:727-729) are marked.
 template <int NW>
 __global__ void __launch_bounds__(64 * NW)
-k_part_scan(BNode *nodes, int64_t lb, int64_t le, PartBlock *pb) {
+k_part_scan(BNode *nodes, int64_t lb, int64_t le, PartBlock *pb, Counters *ctr) {
   constexpr int T = 64 * NW;
   __shared__ int64_t wsum[NW];
   const int l = lane_id(), w = threadIdx.x >> 6;
@@ -1038,7 +1040,10 @@ k_part_scan(BNode *nodes, int64_t lb, int64_t le, PartBlock *pb) {
     }
     if (threadIdx.x == 0) {
       nodes[t].is = run;
-      if (run == 0 || run == nodes[t].count) nodes[t].kind = NODE_DEGEN;
+      if (run == 0 || run == nodes[t].count) {
+        nodes[t].kind = NODE_DEGEN;
+        atomicAdd((unsigned long long *)&ctr->ndegen, 1ull);
+      }
     }
   }
 }
@@ -1229,7 +1234,7 @@ k_part_scatter(const BNode *nodes, const int32_t *pnode, const Counters *ctr, bo
 
 // Children of non-degenerate splits form the next level
 // (RCBForceTree.cxx:731-770: left = [offset, offset+is), right = rest).
-__global__ void k_make_children(BNode *nodes, int64_t lb, int64_t le,
+__global__ void k_make_children(BNode *nodes, int64_t lb, int64_t le, int64_t nd,
                                 int64_t next_base, Counters *ctr, Counters *ctr_next) {
   int64_t t = lb + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   // the next level's counters (the other of two, read by the host before
@@ -1247,6 +1252,8 @@ __global__ void k_make_children(BNode *nodes, int64_t lb, int64_t le,
   const int nbig = (n.is > CM_SMALL) + (n.count - n.is > CM_SMALL);
   if (nbig > 0) atomicAdd((unsigned long long *)&ctr->next_big, (unsigned long long)nbig);
   if (nbig < 2) atomicAdd((unsigned long long *)&ctr->next_small, (unsigned long long)(2 - nbig));
+  const int nleaf = (n.is <= nd) + (n.count - n.is <= nd);
+  if (nleaf > 0) atomicAdd((unsigned long long *)&ctr->next_leaf, (unsigned long long)nleaf);
   n.cl = (int32_t)cl;
   n.cr = (int32_t)cr;
   BNode &L = nodes[cl];
@@ -1624,7 +1631,7 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
   int64_t active = n;
   Counters hc;
   int depth = 0;
-  bool run_big = true, run_small = true;
+  bool run_big = true, run_small = true, run_leaf = true;
   for (;;) {
     int64_t lb = T.level_start[depth], le = T.level_start[depth + 1];
     int64_t nl = le - lb;
@@ -1698,9 +1705,10 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     HSR_HIP(dbuf_reserve(S.pnode, sizeof(int32_t) * pcap));
     int32_t *pnode = dptr<int32_t>(S.pnode);
     k_part_plan<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, unit, cms, nd, c->p.ppc, unit_ppm, pnode, ctr);
-    if (!unit_ppm) k_leaf_ppm<<<gnl, 64, 0, st>>>(nodes, lb, le, nd, Mr);
-    k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_LEAF, depth == 0, P[cur], Pfin, unit, Xr,
-                                                   Yr, Zr, Mr, xs_, ys_, zs_, ms_);
+    if (!unit_ppm && run_leaf) k_leaf_ppm<<<gnl, 64, 0, st>>>(nodes, lb, le, nd, Mr);
+    if (run_leaf)
+      k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_LEAF, depth == 0, P[cur], Pfin, unit,
+                                                     Xr, Yr, Zr, Mr, xs_, ys_, zs_, ms_);
     // ---- partition of splitting nodes
     const unsigned pg = (unsigned)std::max<int64_t>(1, pcap);
     HSR_HIP(dbuf_reserve(S.pmask, sizeof(uint64_t) * (PB / 64) * (size_t)pcap));
@@ -1708,12 +1716,12 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     k_part_count<<<pg, 256, 0, st>>>(nodes, pnode, pb, ctr, Xr, Yr, Zr, pmask);
     if (active / PB > 64 * nl)  // over 64 blocks per node on average
       k_part_scan<PFX_WAVES><<<std::min<unsigned>((unsigned)nl, 4u * (unsigned)std::max(1, c->num_cus)), PFX_T,
-                               0, st>>>(nodes, lb, le, pb);
+                               0, st>>>(nodes, lb, le, pb, ctr);
     else
-      k_part_scan<1><<<std::min<unsigned>((unsigned)nl, gbig), 64, 0, st>>>(nodes, lb, le, pb);
+      k_part_scan<1><<<std::min<unsigned>((unsigned)nl, gbig), 64, 0, st>>>(nodes, lb, le, pb, ctr);
     // children depend only on the partition counts: create them now so a
     // single host synchronisation per level reads every counter
-    k_make_children<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, le, ctr, ctr2 + ((depth + 1) & 1));
+    k_make_children<<<nblk_for(nl, 64), 64, 0, st>>>(nodes, lb, le, nd, le, ctr, ctr2 + ((depth + 1) & 1));
     k_part_rank<<<pg, 256, 0, st>>>(nodes, pnode, pb, ctr, pmask, idx, rights, dest);
     k_part_chain<<<pg, 256, 0, st>>>(nodes, pnode, ctr, idx, rights, dest, ctr,
                                      c->tuning.chain_hops > 0 ? c->tuning.chain_hops : MAXHOP);
@@ -1727,9 +1735,10 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     }
     k_part_scatter<<<pg, 256, 0, st>>>(nodes, pnode, ctr, unit, depth == 0, dest, Xr, Yr, Zr, Mr, P[cur],
                                         X[cur ^ 1], Y[cur ^ 1], Z[cur ^ 1], Mm[cur ^ 1], P[cur ^ 1]);
-    k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_DEGEN, false, P[cur ^ 1], Pfin, unit,
-                                                   X[cur ^ 1], Y[cur ^ 1], Z[cur ^ 1], Mm[cur ^ 1], xs_,
-                                                   ys_, zs_, ms_);
+    if (hc.ndegen > 0)
+      k_finalize_leaves<<<(unsigned)nl, 64, 0, st>>>(nodes, lb, le, NODE_DEGEN, false, P[cur ^ 1], Pfin, unit,
+                                                     X[cur ^ 1], Y[cur ^ 1], Z[cur ^ 1], Mm[cur ^ 1], xs_,
+                                                     ys_, zs_, ms_);
     HSR_HIP(hipGetLastError());
 #ifdef HSR_CM_STATS
     fprintf(stderr, "[cm] level %d nodes %lld blocks %lld scans %lld fallback %lld small ordered %lld\n",
@@ -1740,6 +1749,7 @@ hsr_status tree_build_device(hsr_ctx *c, int64_t nboxes,
     active = hc.active_next;
     run_big = hc.next_big > 0;
     run_small = hc.next_small > 0;
+    run_leaf = hc.next_leaf > 0;
     cur ^= 1;
     ++depth;
   }
