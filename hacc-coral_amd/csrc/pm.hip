@@ -462,7 +462,7 @@ __device__ __forceinline__ void merge_advance(int qs, int32_t (&s)[8], const int
 }
 
 // One thread per grid cell: at most CELL_SHORT contributions are merged and
-// added here; longer cells are queued (their count for the staging offsets).
+// added here; longer cells are queued with their counts for k_cic_long_sum.
 __global__ void __launch_bounds__(256)
 k_cic_cells(Grid g, Ext ex, const int32_t *__restrict__ beg, const int32_t *__restrict__ end,
             const int32_t *__restrict__ idx, const float *__restrict__ x,
@@ -525,7 +525,7 @@ k_cic_long_sum(Grid g, Ext ex, const int32_t *__restrict__ beg, const int32_t *_
   for (int64_t t = blockIdx.x; t < nl; t += gridDim.x) {
     const uint32_t cid = long_cell[t];
     const int64_t K = long_k[t];
-    int32_t s[8], e[8], hv[8];  // thread 0's merge state
+    int32_t s[8], e[8], hv[8];  // lane 0's merge state
     if (tid == 0) {
       const int k = (int)(cid % (uint32_t)g.ng2);
       const uint32_t ij = cid / (uint32_t)g.ng2;
