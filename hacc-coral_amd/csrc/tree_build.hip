@@ -1210,24 +1210,44 @@ k_part_jump_final(const BNode *nodes, const int32_t *pnode,
 }
 
 __global__ void __launch_bounds__(256)
-k_part_scatter(const BNode *nodes, const int32_t *pnode, const Counters *ctr, bool unit, bool ident,
-               const int32_t *dest, const float *X, const float *Y,
-               const float *Z, const float *M, const int32_t *P, float *X2,
-               float *Y2, float *Z2, float *M2, int32_t *P2) {
+k_part_scatter(const BNode *__restrict__ nodes, const int32_t *__restrict__ pnode,
+               const Counters *__restrict__ ctr, bool unit, bool ident, const int32_t *__restrict__ dest,
+               const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z,
+               const float *__restrict__ M, const int32_t *__restrict__ P, float *__restrict__ X2,
+               float *__restrict__ Y2, float *__restrict__ Z2, float *__restrict__ M2,
+               int32_t *__restrict__ P2) {
+  // the source and destination arrays are distinct buffers: each thread's
+  // PB / 256 elements are all read before any is written, so their loads
+  // are in flight together
+  constexpr int PER = PB / 256;
   for (int64_t b = blockIdx.x; b < ctr->part_blocks; b += gridDim.x) {
     const BNode &n = nodes[pnode[b]];
     int64_t bstart;
     int blen;
     blk_range(n, b, PB, bstart, blen);
     const int64_t off = n.offset;
-    for (int k = threadIdx.x; k < blen; k += 256) {
-      int64_t e = bstart + k;
-      int64_t o = off + dest[e];
-      X2[o] = X[e];
-      Y2[o] = Y[e];
-      Z2[o] = Z[e];
-      if (!unit) M2[o] = M[e];  // unit masses are not moved (never read again)
-      P2[o] = ident ? (int32_t)e : P[e];  // the first level's order is the input's
+    int32_t d[PER], vp[PER];
+    float vx[PER], vy[PER], vz[PER], vm[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int k = u * 256 + (int)threadIdx.x;
+      const int64_t e = bstart + (k < blen ? k : 0);
+      d[u] = dest[e];
+      vx[u] = X[e];
+      vy[u] = Y[e];
+      vz[u] = Z[e];
+      vm[u] = unit ? 0.f : M[e];
+      vp[u] = ident ? (int32_t)e : P[e];  // the first level's order is the input's
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      if (u * 256 + (int)threadIdx.x >= blen) continue;
+      const int64_t o = off + d[u];
+      X2[o] = vx[u];
+      Y2[o] = vy[u];
+      Z2[o] = vz[u];
+      if (!unit) M2[o] = vm[u];  // unit masses are not moved (never read again)
+      P2[o] = vp[u];
     }
   }
 }
