@@ -904,10 +904,11 @@ k_leaf_ppm(BNode *nodes, int64_t lb, int64_t le, int64_t nd, const float *M) {
 // Copy the final permutation of nodes that stop here (leaves at classify
 // time read buffer `P`; degenerate splits read the partition's output).
 __global__ void __launch_bounds__(64)
-k_finalize_leaves(const BNode *nodes, int64_t lb, int64_t le, int kind,
-                  bool ident, const int32_t *P, int32_t *Pfin, bool unit, const float *X,
-                  const float *Y, const float *Z, const float *M, float *xs, float *ys,
-                  float *zs, float *ms) {
+k_finalize_leaves(const BNode *__restrict__ nodes, int64_t lb, int64_t le, int kind, bool ident,
+                  const int32_t *__restrict__ P, int32_t *__restrict__ Pfin, bool unit,
+                  const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z,
+                  const float *__restrict__ M, float *__restrict__ xs, float *__restrict__ ys,
+                  float *__restrict__ zs, float *__restrict__ ms) {
   // the node's elements are in their final positions: their input index
   // and their sorted x, y, z, m (read in order here rather than gathered
   // through the permutation afterwards)
@@ -976,8 +977,9 @@ k_part_plan(BNode *nodes, int64_t lb, int64_t le, bool unit, const sd_t *cms, in
 
 // Keys < pivot per block (RCBForceTree.cxx:638-645), pivot = xc[d].
 __global__ void __launch_bounds__(256)
-k_part_count(const BNode *nodes, const int32_t *pnode, PartBlock *pb, const Counters *ctr,
-             const float *X, const float *Y, const float *Z, uint64_t *mask) {
+k_part_count(const BNode *__restrict__ nodes, const int32_t *__restrict__ pnode,
+             PartBlock *__restrict__ pb, const Counters *__restrict__ ctr, const float *__restrict__ X,
+             const float *__restrict__ Y, const float *__restrict__ Z, uint64_t *__restrict__ mask) {
   for (int64_t b = blockIdx.x; b < ctr->part_blocks; b += gridDim.x) {
     const BNode &n = nodes[pnode[b]];
     int64_t bstart;
@@ -987,11 +989,18 @@ k_part_count(const BNode *nodes, const int32_t *pnode, PartBlock *pb, const Coun
     float pv = n.xc[n.split_d];
     int c = 0;
     // the flags also go to the block's mask words (64 elements each), which
-    // the rank kernel reads instead of the keys
+    // the rank kernel reads instead of the keys; the keys are all loaded
+    // first (restrict-qualified buffers), so their loads are in flight together
+    float kv[PB / 256];
 #pragma unroll
     for (int r = 0; r < PB / 256; ++r) {
       const int k = r * 256 + (int)threadIdx.x;
-      const bool f = k < blen && key[bstart + k] < pv;
+      kv[r] = key[bstart + (k < blen ? k : 0)];
+    }
+#pragma unroll
+    for (int r = 0; r < PB / 256; ++r) {
+      const int k = r * 256 + (int)threadIdx.x;
+      const bool f = k < blen && kv[r] < pv;
       const uint64_t bal = __ballot(f);
       if (lane_id() == 0) mask[b * (PB / 64) + (k >> 6)] = bal;
       c += f;
@@ -1052,37 +1061,42 @@ k_part_scan(BNode *nodes, int64_t lb, int64_t le, PartBlock *pb, Counters *ctr) 
 // recorded as idx[r] = p; others at p >= is stay; others below is are
 // resolved by the swap chain (dest = -1 here).
 __global__ void __launch_bounds__(256)
-k_part_rank(const BNode *nodes, const int32_t *pnode, const PartBlock *pb, const Counters *ctr,
-            const uint64_t *mask, int32_t *idx, int32_t *rights, int32_t *dest) {
+k_part_rank(const BNode *__restrict__ nodes, const int32_t *__restrict__ pnode,
+            const PartBlock *__restrict__ pb, const Counters *__restrict__ ctr,
+            const uint64_t *__restrict__ mask, int32_t *__restrict__ idx, int32_t *__restrict__ rights,
+            int32_t *__restrict__ dest) {
+  // the block's 16 mask words (set by k_part_count, zero past the block)
+  // are read at once and their counts shared through LDS with one barrier
+  __shared__ int wcnt[PB / 64];
+  const int w = threadIdx.x >> 6, l = lane_id();
   for (int64_t b = blockIdx.x; b < ctr->part_blocks; b += gridDim.x) {
-    const PartBlock &B = pb[b];
     const BNode &n = nodes[pnode[b]];
     int64_t bstart;
     int blen;
     blk_range(n, b, PB, bstart, blen);
-    int64_t off = n.offset, is = n.is;
-    __shared__ int wsum[4];
-    int64_t run = B.base;
-    int w = threadIdx.x >> 6;
+    const int64_t off = n.offset, is = n.is, base = pb[b].base;
+    uint64_t bal[PB / 256];
+#pragma unroll
+    for (int r = 0; r < PB / 256; ++r) bal[r] = mask[b * (PB / 64) + r * 4 + w];
+    if (l == 0)
+#pragma unroll
+      for (int r = 0; r < PB / 256; ++r) wcnt[r * 4 + w] = __popcll(bal[r]);
+    __syncthreads();
+    int64_t run = base;
+#pragma unroll
     for (int r = 0; r < PB / 256; ++r) {
-      int k = r * 256 + threadIdx.x;
-      bool valid = k < blen;
-      int64_t e = bstart + k;
-      // this wave's 64 flags, set by k_part_count (zero past the block)
-      const uint64_t bal = mask[b * (PB / 64) + (k >> 6)];
-      const bool f = (bal >> lane_id()) & 1ull;
-      int pre = __popcll(bal & lanemask_lt());
-      if (lane_id() == 0) wsum[w] = __popcll(bal);
-      __syncthreads();
+      const int k = r * 256 + (int)threadIdx.x;
       int wbase = 0, tot = 0;
+#pragma unroll
       for (int i = 0; i < 4; ++i) {
-        if (i < w) wbase += wsum[i];
-        tot += wsum[i];
+        const int v = wcnt[r * 4 + i];
+        wbase += i < w ? v : 0;
+        tot += v;
       }
-      if (valid) {
-        int64_t p = e - off;
-        int64_t lefts_before = run + wbase + pre;
-        if (f) {
+      if (k < blen) {
+        const int64_t e = bstart + k, p = e - off;
+        const int64_t lefts_before = run + wbase + __popcll(bal[r] & lanemask_lt());
+        if ((bal[r] >> l) & 1ull) {
           idx[off + lefts_before] = (int32_t)p;
           dest[e] = (int32_t)lefts_before;
         } else {
@@ -1091,8 +1105,8 @@ k_part_rank(const BNode *nodes, const int32_t *pnode, const PartBlock *pb, const
         }
       }
       run += tot;
-      __syncthreads();
     }
+    __syncthreads();  // wcnt is rewritten for the next block
   }
 }
 
