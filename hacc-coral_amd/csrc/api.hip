@@ -115,11 +115,18 @@ __global__ void k_gather_rest(int64_t n, const int32_t *__restrict__ P, Gather6 
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int64_t s = P[k];
+  // the staging buffers are distinct from the arrays read: every value is
+  // loaded before any is stored, so the gathers are in flight together
+  float v[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) v[a] = g.in[a] ? g.in[a][s] : 0.f;
+  const int64_t idv = g.id_in ? g.id_in[s] : 0;
+  const uint16_t mv = g.mask_in ? g.mask_in[s] : (uint16_t)0;
 #pragma unroll
   for (int a = 0; a < 4; ++a)
-    if (g.in[a]) g.out[a][k] = g.in[a][s];
-  if (g.id_in) g.id_out[k] = g.id_in[s];
-  if (g.mask_in) g.mask_out[k] = g.mask_in[s];
+    if (g.in[a]) g.out[a][k] = v[a];
+  if (g.id_in) g.id_out[k] = idv;
+  if (g.mask_in) g.mask_out[k] = mv;
 }
 
 // Permute the ten arrays (device memory) in place with the build's final

@@ -333,22 +333,36 @@ __global__ void k_walk_own(const hsr_node *__restrict__ t, const int32_t *__rest
 // ranges by all lanes, the T pseudo-particles of node a as indices
 // count + T a + j (RCBForceTree.cxx:1051-1060).
 __global__ void __launch_bounds__(64)
-k_expand(int64_t nleaves, int64_t n, int tdpts, const int64_t *seg_ptr,
-         const int64_t *seg_a, const int64_t *seg_b, const int64_t *ent_ptr,
-         int32_t *jidx) {
+k_expand(int64_t nleaves, int64_t n, int tdpts, const int64_t *__restrict__ seg_ptr,
+         const int64_t *__restrict__ seg_a, const int64_t *__restrict__ seg_b,
+         const int64_t *__restrict__ ent_ptr, int32_t *__restrict__ jidx) {
   int64_t k = blockIdx.x;
   if (k >= nleaves) return;
   int l = lane_id();
   int64_t e = ent_ptr[k];
-  for (int64_t s = seg_ptr[k]; s < seg_ptr[k + 1]; ++s) {
-    int64_t a = seg_a[s], b = seg_b[s];
-    if (b < 0) {
-      if (l < tdpts) jidx[e + l] = (int32_t)(n + tdpts * a + l);
-      e += tdpts;
-    } else {
-      for (int64_t i = l; i < b; i += 64) jidx[e + i] = (int32_t)(a + i);
-      e += b;
+  const int64_t s0 = seg_ptr[k], s1 = seg_ptr[k + 1];
+  // 64 segments at a time: lane i reads segment s + i, a wave prefix of
+  // their entry counts places them, then they are written one by one
+  for (int64_t s = s0; s < s1; s += 64) {
+    const bool v = s + l < s1;
+    const int64_t a = v ? seg_a[s + l] : 0, b = v ? seg_b[s + l] : 0;
+    const int64_t len = !v ? 0 : (b < 0 ? (int64_t)tdpts : b);
+    int64_t inc = len;  // inclusive prefix over the lanes
+    for (int o = 1; o < 64; o <<= 1) {
+      const int64_t y = (int64_t)__shfl_up((long long)inc, o);
+      if (l >= o) inc += y;
     }
+    const int m = (int)min<int64_t>(64, s1 - s);
+    for (int j = 0; j < m; ++j) {
+      const int64_t aj = (int64_t)__shfl((long long)a, j), bj = (int64_t)__shfl((long long)b, j);
+      const int64_t ej = e + (int64_t)__shfl((long long)(inc - len), j);
+      if (bj < 0) {
+        if (l < tdpts) jidx[ej + l] = (int32_t)(n + tdpts * aj + l);
+      } else {
+        for (int64_t i = l; i < bj; i += 64) jidx[ej + i] = (int32_t)(aj + i);
+      }
+    }
+    e += (int64_t)__shfl((long long)inc, 63);
   }
 }
 
