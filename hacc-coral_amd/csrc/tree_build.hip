@@ -1117,9 +1117,9 @@ k_part_rank(const BNode *__restrict__ nodes, const int32_t *__restrict__ pnode,
 // that entered such a stretch with step k continues f, f+k, f+2k, ... up to
 // rights[k]; the walk therefore jumps whole stretches arithmetically.
 __global__ void __launch_bounds__(256)
-k_part_chain(const BNode *nodes, const int32_t *pnode, const Counters *ctrc,
-             const int32_t *idx, const int32_t *rights, int32_t *dest,
-             Counters *ctr, int maxhop) {
+k_part_chain(const BNode *__restrict__ nodes, const int32_t *__restrict__ pnode,
+             const Counters *ctrc, const int32_t *__restrict__ idx,
+             const int32_t *__restrict__ rights, int32_t *__restrict__ dest, Counters *ctr, int maxhop) {
   for (int64_t b = blockIdx.x; b < ctrc->part_blocks; b += gridDim.x) {
     const BNode &n = nodes[pnode[b]];
     int64_t bstart;
